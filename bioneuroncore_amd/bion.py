@@ -1,0 +1,345 @@
+"""BION v14 records as numpy structured dtypes, file read/write, and the synthetic networks
+the hot path is measured on.
+
+The record layouts are the fixed BioNeuronCore ABI (reference include/types.h:446-603,
+sizes asserted at :703-710); the file layout is header, neurons, segments, connections,
+synapses, nuclei (reference src/bion_loader_host.cpp:26-107, bion_save :449-504).
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+MODIFIER_DT = np.dtype([
+    ("type", "u1"), ("_pad", "u1", 7), ("density", "<f8"), ("state", "<f8", 8),
+    ("active_vars", "u1"), ("flags", "u1"), ("_reserved", "u1", 6)])
+SEGMENT_DT = np.dtype([
+    ("id", "<u4"), ("neuron_id", "<u4"), ("position", "<f4", 3), ("direction", "<f4", 3),
+    ("length", "<f4"), ("radius", "<f4"), ("V_m", "<f8"), ("intra_ions", "<f8", 8),
+    ("extra_ions", "<f8", 8), ("intra_nt", "<f4", 15), ("neurotransmitter", "<f4", 15),
+    ("modifiers", MODIFIER_DT, 32), ("modifier_count", "u1"), ("segment_type", "u1"),
+    ("reserved1", "u1", 6), ("ATP_level", "<f4"), ("temperature", "<f4"), ("I_buffer", "<f8"),
+    ("transcription_signal", "<f4"), ("intra_connection_ids", "<u4", 16),
+    ("intra_connection_count", "u1"), ("reserved2", "u1", 7), ("ER_Ca2", "<f4")])
+NEURON_DT = np.dtype([
+    ("id", "<u4"), ("segment_start", "<u4"), ("segment_count", "<u4"), ("soma_index", "<u4"),
+    ("threshold", "<f4"), ("type", "u1"), ("reserved", "u1", 3)])
+CONN_DT = np.dtype([
+    ("segment_a", "<u4"), ("segment_b", "<u4"), ("is_intracellular", "u1"), ("reserved", "u1", 3),
+    ("_pad", "<u4")])
+SYNAPSE_DT = np.dtype([
+    ("id", "<u4"), ("pre_segment", "<u4"), ("post_segment", "<u4"), ("type", "u1"),
+    ("reserved", "u1", 3), ("weight", "<f4"), ("delay", "<f4"), ("pre_area", "<f4"),
+    ("post_area", "<f4"), ("pre_autoreceptors", MODIFIER_DT, 8), ("pre_autoreceptor_count", "u1"),
+    ("reserved2", "u1", 7), ("post_receptors", MODIFIER_DT, 8), ("post_receptor_count", "u1"),
+    ("reserved3", "u1", 3), ("release_probability", "<f4"), ("release_amount", "<f4"),
+    ("cleft_neurotransmitter", "<f4", 15), ("facilitation", "<f4"), ("depression", "<f4"),
+    ("tau_clearance", "<f4"), ("refractory", "u1"), ("reserved4", "u1", 7), ("RRP", "<f4"),
+    ("RRP_max", "<f4"), ("RRP_recovery", "<f4"), ("bouton_Ca", "<f4"), ("spine_Ca", "<f4"),
+    ("spine_camkii", "<f4"), ("spine_calcineurin", "<f4")])
+HEADER_DT = np.dtype([
+    ("magic", "<u4"), ("version", "<u4"), ("mode", "u1"), ("diffusion_type", "u1"),
+    ("reserved", "<u2"), ("num_neurons", "<u4"), ("num_segments", "<u4"),
+    ("num_intra_connections", "<u4"), ("num_synapses", "<u4"), ("grid_dim_x", "<u4"),
+    ("grid_dim_y", "<u4"), ("grid_dim_z", "<u4"), ("voxel_size", "<f4"), ("timestamp", "<u8"),
+    ("build_id", "S16"), ("num_nuclei", "<u4"), ("nucleus_version", "<u4"),
+    ("file_sz_segment", "<u4"), ("file_sz_synapse", "<u4"), ("file_sz_nucleus", "<u4"),
+    ("file_sz_intraconnection", "<u4"), ("file_sz_neuron", "<u4"), ("file_sz_modifier", "<u4")])
+NUCLEUS_SIZE = 500
+CURRENT_STIM_DT = np.dtype([("segment_id", "<u4"), ("amplitude_pA", "<f4"), ("duration_ms", "<f4"),
+                            ("start_tick", "<u8")])
+VCLAMP_DT = np.dtype([("segment_id", "<u4"), ("target_mV", "<f4"), ("start_tick", "<u8"),
+                      ("end_tick", "<u8")])
+RAMP_DT = np.dtype([("segment_id", "<u4"), ("start_pA", "<f4"), ("end_pA", "<f4"),
+                    ("start_tick", "<u8"), ("end_tick", "<u8")])
+ION_INJ_DT = np.dtype([("segment_id", "<u4"), ("ion_idx", "<i4"), ("amount_mM", "<f4"),
+                       ("extracellular", "u1"), ("_pad", "u1", 3)])
+LIG_INJ_DT = np.dtype([("segment_id", "<u4"), ("nt_idx", "<i4"), ("amount", "<f4"),
+                       ("intracellular", "u1"), ("_pad2", "u1", 3)])
+
+assert MODIFIER_DT.itemsize == 88 and SEGMENT_DT.itemsize == 3216
+assert SYNAPSE_DT.itemsize == 1568 and CONN_DT.itemsize == 16 and NEURON_DT.itemsize == 24
+assert HEADER_DT.itemsize == 100
+
+BION_MAGIC = 0x42494F4E
+BION_VERSION = 14
+
+# Modifier IDs (reference include/types.h:22-263) used by the hot-path workloads.
+PUMP, AMPA, NMDA, GABAA = 7, 8, 9, 11
+SK, BK, K2P, KV7, KVDR, HCN1, NALCN = 103, 104, 107, 112, 123, 129, 133
+CAV21, NAV16, NAV12, KV31, KV11 = 140, 141, 142, 150, 156
+SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
+                       NAV16, NAV12, KV31, KV11)
+
+# Physiological ion means (reference include/cpg_config.h:400-412).
+INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
+EXTRA_IONS = np.array([140.0, 3.0, 1.2, 130.0, 2.0, 0.0, 4.0e-5, 0.0], dtype=np.float32)
+
+# Canonical densities (reference include/cpg_config.h:16-76, tests/test_cpu_gpu_parity_cuda.cpp:246-247).
+CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, PUMP: 1500.0,
+                 CAV21: 100.0, HCN1: 120.0, SK: 6.0, BK: 0.15, KV11: 0.346, KV31: 2.0, KV7: 2.0}
+# Gate initial values near rest (-65 mV).
+GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
+             CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
+             KV31: {0: 0.0, 1: 1.0}, KV7: {0: 0.1, 6: 1.0}, SK: {0: 0.0}, BK: {0: 0.0}}
+
+
+@dataclasses.dataclass
+class Network:
+    """A BION network held as structured arrays (the file's AoS records)."""
+    neurons: np.ndarray
+    segments: np.ndarray
+    connections: np.ndarray
+    synapses: np.ndarray
+    mode: int = 0
+
+    @property
+    def num_segments(self) -> int:
+        return int(self.segments.shape[0])
+
+    def copy(self) -> "Network":
+        return Network(self.neurons.copy(), self.segments.copy(), self.connections.copy(),
+                       self.synapses.copy(), self.mode)
+
+
+def make_header(net: Network) -> np.ndarray:
+    h = np.zeros(1, HEADER_DT)
+    h["magic"] = BION_MAGIC
+    h["version"] = BION_VERSION
+    h["mode"] = net.mode
+    h["num_neurons"] = len(net.neurons)
+    h["num_segments"] = len(net.segments)
+    h["num_intra_connections"] = len(net.connections)
+    h["num_synapses"] = len(net.synapses)
+    h["voxel_size"] = 1.0
+    h["timestamp"] = int(time.time())
+    h["build_id"] = b"bchip"
+    h["nucleus_version"] = 1
+    h["file_sz_segment"] = SEGMENT_DT.itemsize
+    h["file_sz_synapse"] = SYNAPSE_DT.itemsize
+    h["file_sz_nucleus"] = NUCLEUS_SIZE
+    h["file_sz_intraconnection"] = CONN_DT.itemsize
+    h["file_sz_neuron"] = NEURON_DT.itemsize
+    h["file_sz_modifier"] = MODIFIER_DT.itemsize
+    return h
+
+
+def write_bion(path: str, net: Network) -> None:
+    with open(path, "wb") as f:
+        f.write(make_header(net).tobytes())
+        for arr in (net.neurons, net.segments, net.connections, net.synapses):
+            f.write(np.ascontiguousarray(arr).tobytes())
+
+
+def read_bion(path: str) -> Network:
+    with open(path, "rb") as f:
+        raw = f.read()
+    h = np.frombuffer(raw[:100], HEADER_DT)[0]
+    if int(h["magic"]) != BION_MAGIC:
+        raise ValueError("bad BION magic")
+    if int(h["version"]) != BION_VERSION:
+        raise ValueError(f"BION version {int(h['version'])} not supported (v14 only)")
+    off = 100
+    out = []
+    for dt, n in ((NEURON_DT, h["num_neurons"]), (SEGMENT_DT, h["num_segments"]),
+                  (CONN_DT, h["num_intra_connections"]), (SYNAPSE_DT, h["num_synapses"])):
+        n = int(n)
+        out.append(np.frombuffer(raw[off:off + n * dt.itemsize], dt).copy())
+        off += n * dt.itemsize
+    return Network(*out, mode=int(h["mode"]))
+
+
+# ------------------------------------------------------------------ synthetic networks
+def _set_modifiers(segs: np.ndarray, types: list[int], densities: np.ndarray) -> None:
+    """types: per-slot IDs; densities: [n_seg, n_slot]."""
+    k = len(types)
+    mods = segs["modifiers"]
+    for j, t in enumerate(types):
+        mods["type"][:, j] = t
+        mods["density"][:, j] = densities[:, j]
+        for idx, val in GATE_INIT.get(t, {}).items():
+            mods["state"][:, j, idx] = val
+    segs["modifier_count"] = k
+
+
+def _init_segment_state(segs: np.ndarray, rng: np.random.Generator) -> None:
+    n = len(segs)
+    jit = rng.uniform(0.95, 1.05, size=(n, 8)).astype(np.float32)
+    segs["intra_ions"] = (INTRA_IONS[None, :] * jit).astype(np.float64)
+    segs["extra_ions"] = EXTRA_IONS[None, :].astype(np.float64)
+    segs["V_m"] = -65.0
+    segs["ATP_level"] = 1.0
+    segs["temperature"] = 37.0
+    segs["ER_Ca2"] = 0.5
+    segs["direction"] = (0.0, 1.0, 0.0)
+
+
+def _tree_parents(n_nodes: int) -> np.ndarray:
+    """Parent of each dendrite node in a binary tree laid out breadth-first; node 0 hangs
+    off the soma (parent -1 means soma)."""
+    return np.array([-1] + [(i - 1) // 2 for i in range(1, n_nodes)], dtype=np.int64)
+
+
+def build_network(n_neurons: int, seg_per_neuron: int, morphology: str, seg_types: list[int],
+                  soma_extra: list[int], synapses_per_neuron: float, receptor_mix: str,
+                  max_delay_ticks: int, seed: int, dt: float = 0.025, lognormal_sigma: float = 0.0,
+                  density_scale: dict | None = None, pre_pool: str = "soma") -> Network:
+    """Soma + dendrite (chain or breadth-first binary tree) neurons with a fixed modifier set
+    per segment, random chemical synapses from presynaptic somas onto uniform post segments."""
+    rng = np.random.default_rng(seed)
+    ns = n_neurons * seg_per_neuron
+    segs = np.zeros(ns, SEGMENT_DT)
+    nid = np.repeat(np.arange(n_neurons, dtype=np.uint32), seg_per_neuron)
+    local = np.tile(np.arange(seg_per_neuron), n_neurons)
+    segs["id"] = np.arange(ns, dtype=np.uint32)
+    segs["neuron_id"] = nid
+    is_soma = local == 0
+    segs["segment_type"] = np.where(is_soma, 1, 2)
+    segs["radius"] = np.where(is_soma, 5.0, rng.uniform(0.5, 2.0, ns)).astype(np.float32)
+    segs["length"] = np.where(is_soma, 10.0, rng.uniform(10.0, 50.0, ns)).astype(np.float32)
+    segs["position"][:, 0] = nid.astype(np.float32) * 100.0
+    segs["position"][:, 1] = local.astype(np.float32) * 20.0
+    _init_segment_state(segs, rng)
+
+    types = list(seg_types)
+    dens = np.empty((ns, len(types) + len(soma_extra)), dtype=np.float64)
+    for j, t in enumerate(types + list(soma_extra)):
+        base = CANON_DENSITY[t] * (density_scale or {}).get(t, 1.0)
+        if lognormal_sigma > 0:
+            dens[:, j] = base * rng.lognormal(0.0, lognormal_sigma, ns)
+        else:
+            dens[:, j] = base
+    _set_modifiers(segs, types, dens[:, :len(types)])
+    if soma_extra:
+        soma_idx = np.nonzero(is_soma)[0]
+        k = len(types)
+        for j, t in enumerate(soma_extra):
+            segs["modifiers"]["type"][soma_idx, k + j] = t
+            segs["modifiers"]["density"][soma_idx, k + j] = dens[soma_idx, k + j]
+            for idx, val in GATE_INIT.get(t, {}).items():
+                segs["modifiers"]["state"][soma_idx, k + j, idx] = val
+        segs["modifier_count"][soma_idx] = k + len(soma_extra)
+
+    neurons = np.zeros(n_neurons, NEURON_DT)
+    neurons["id"] = np.arange(n_neurons)
+    neurons["segment_start"] = np.arange(n_neurons) * seg_per_neuron
+    neurons["segment_count"] = seg_per_neuron
+    neurons["soma_index"] = np.arange(n_neurons) * seg_per_neuron
+    neurons["threshold"] = -40.0
+
+    # intracellular connections: chain (i -> i+1) or binary tree (child -> parent)
+    m = seg_per_neuron - 1
+    if morphology == "chain":
+        par_local = np.arange(0, m)            # node k+1 hangs off node k
+    elif morphology == "tree":
+        par_local = _tree_parents(m) + 1       # node k+1's parent (0 = soma)
+    else:
+        raise ValueError(morphology)
+    base = (np.arange(n_neurons, dtype=np.int64) * seg_per_neuron)[:, None]
+    a = (base + par_local[None, :]).reshape(-1)
+    b = (base + np.arange(1, seg_per_neuron)[None, :]).reshape(-1)
+    conns = np.zeros(len(a), CONN_DT)
+    conns["segment_a"] = a
+    conns["segment_b"] = b
+    conns["is_intracellular"] = 1
+
+    nsyn = int(round(synapses_per_neuron * n_neurons))
+    syns = np.zeros(nsyn, SYNAPSE_DT)
+    if nsyn:
+        post_neuron = np.repeat(np.arange(n_neurons), int(np.ceil(synapses_per_neuron)))[:nsyn] \
+            if synapses_per_neuron >= 1 else rng.integers(0, n_neurons, nsyn)
+        if synapses_per_neuron >= 1:
+            post = post_neuron * seg_per_neuron + rng.integers(0, seg_per_neuron, nsyn)
+        else:
+            post = rng.integers(0, ns, nsyn)
+        pre = rng.integers(0, n_neurons, nsyn) * seg_per_neuron
+        syns["id"] = np.arange(1, nsyn + 1, dtype=np.uint32)
+        syns["pre_segment"] = pre
+        syns["post_segment"] = post
+        syns["weight"] = 1.0
+        syns["delay"] = (rng.integers(1, max_delay_ticks + 1, nsyn) * np.float32(dt)).astype(np.float32)
+        syns["pre_area"] = 0.1
+        syns["post_area"] = 0.1
+        syns["release_probability"] = 0.5
+        syns["release_amount"] = 1.0
+        syns["tau_clearance"] = 5.0
+        syns["RRP"] = 10.0
+        syns["RRP_max"] = 10.0
+        syns["RRP_recovery"] = 0.02
+        syns["bouton_Ca"] = 0.0001
+        syns["spine_Ca"] = 0.0001
+        syns["depression"] = 0.01
+        pr = syns["post_receptors"]
+        if receptor_mix == "ampa":
+            pr["type"][:, 0] = AMPA
+            pr["density"][:, 0] = 500.0
+            syns["post_receptor_count"] = 1
+        elif receptor_mix == "ampa_nmda_gaba":
+            exc = rng.random(nsyn) < 0.7
+            pr["type"][:, 0] = np.where(exc, AMPA, GABAA)
+            pr["density"][:, 0] = np.where(exc, 500.0, 300.0)
+            pr["type"][:, 1] = np.where(exc, NMDA, 0)
+            pr["density"][:, 1] = np.where(exc, 50.0, 0.0)
+            pr["state"][:, 1, 2] = np.where(exc, 1.0, 0.0)   # NMDA CDI gate starts open
+            syns["post_receptor_count"] = np.where(exc, 2, 1)
+        else:
+            raise ValueError(receptor_mix)
+    return Network(neurons, segs, conns, syns)
+
+
+def config1(seed: int = 0xB10C0001) -> Network:
+    """64 neurons x (soma + 7-segment chain); Nav1.2/Kv-DR/K2P/NALCN/pump everywhere,
+    Cav2.1 on somas; 256 AMPA synapses (soma -> uniform segment), delays 1-5 ticks."""
+    return build_network(64, 8, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21], 4.0, "ampa", 5, seed)
+
+
+CONFIG2_TYPES = [NAV12, NAV16, KV11, KV31, KV7, CAV21, HCN1, SK, BK, K2P, PUMP, NALCN]
+
+
+def config2(n_neurons: int = 100_000, seed: int = 0xB10C0002, syn_per_neuron: float = 100.0) -> Network:
+    """n_neurons x (soma + 31-node binary tree) = 32 segments each, the 12-modifier set with
+    log-normal (sigma 0.3) densities, fan-in syn_per_neuron from uniform presynaptic somas,
+    70% AMPA+NMDA / 30% GABA-A, delays 1-20 ticks."""
+    return build_network(n_neurons, 32, "tree", CONFIG2_TYPES, [], syn_per_neuron, "ampa_nmda_gaba", 20,
+                         seed, lognormal_sigma=0.3)
+
+
+def config3(n_cables: int = 10_000, seg_per_cable: int = 100, seed: int = 0xB10C0003) -> Network:
+    """Passive unbranched cables with linear +-20% concentration ramps (electrodiffusion stress)."""
+    net = build_network(n_cables, seg_per_cable, "chain", [], [], 0.0, "ampa", 1, seed)
+    segs = net.segments
+    segs["modifier_count"] = 0
+    frac = np.tile(np.linspace(-0.2, 0.2, seg_per_cable), n_cables)
+    segs["intra_ions"][:, :7] = INTRA_IONS[None, :7] * (1.0 + frac[:, None])
+    segs["extra_ions"][:, :7] = EXTRA_IONS[None, :7] * (1.0 + frac[:, None])
+    segs["ATP_level"] = (1.0 + frac).astype(np.float32)
+    segs["temperature"] = (37.0 * (1.0 + 0.01 * frac)).astype(np.float32)
+    return net
+
+
+def config5_cell(seed: int, n_seg: int = 200, gmax_scale: np.ndarray | None = None) -> Network:
+    """One deterministic cell: soma + branched dendrite (binary tree), Nav1.2/Kv-DR/K2P/NALCN/HCN1."""
+    net = build_network(1, n_seg, "tree", [NAV12, KVDR, K2P, NALCN, HCN1], [], 0.0, "ampa", 1, seed)
+    if gmax_scale is not None:
+        net.segments["modifiers"]["density"][:, :5] *= gmax_scale
+    return net
+
+
+def poisson_soma_pulses(net: Network, rate_hz: float, ticks: int, dt: float, amp_pA: float,
+                        seed: int) -> list[np.ndarray]:
+    """Per-tick lists of 1-tick current pulses (CURRENT_STIM_DT) at rate_hz per soma."""
+    rng = np.random.default_rng(seed)
+    somas = np.nonzero(net.segments["segment_type"] == 1)[0]
+    p = rate_hz * dt * 1e-3
+    out = []
+    for t in range(ticks):
+        hit = somas[rng.random(len(somas)) < p]
+        st = np.zeros(len(hit), CURRENT_STIM_DT)
+        st["segment_id"] = net.segments["id"][hit]
+        st["amplitude_pA"] = amp_pA
+        st["duration_ms"] = dt
+        st["start_tick"] = t
+        out.append(st)
+    return out

@@ -1,0 +1,1230 @@
+// cpu_ref.cpp — the oracle: a single-threaded CPU restatement of BioNeuronCore's per-tick step
+// (core_run_tick, src/simulation_host.cpp:174-657) for the channel/receptor set the hot path
+// covers. TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg use it as the checker and CPU baseline; the product (libbiocore_hip_backend.so) never
+// links, loads or calls it.
+//
+// It reproduces the reference's arithmetic operation by operation: the same float/double
+// types per expression, the same evaluation order, no FMA contraction (-ffp-contract=off),
+// and glibc's ::exp/::pow, so results are bit-identical to the reference library. That
+// identity is checked by tests/test_oracle_vs_reference.py against oracle/_ref (the reference
+// compiled from its own sources) and against the committed golden fixtures.
+//
+// Scope: modifier IDs {7,8,9,11,103,104,107,112,123,129,133,140,141,142,150,156}; chemical
+// synapses without presynaptic autoreceptors; no glia (GLAST), gap junctions, consolidation,
+// mechano stimuli or plugins. Anything outside that set is reported by
+// oracle_unsupported_count() and computed as the reference's "unknown ID" default (0 current).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include "../include/biocore_abi.h"
+
+namespace orc {
+
+// ---- constants (reference include/types.h, units.h, run_tick_common.h, modifier_helpers.h) ----
+constexpr double kFaraday = 96485.0;
+constexpr float kGas = 8.314f;
+constexpr float kTemp = 310.15f;
+constexpr float kPi = 3.14159265f;
+constexpr float kEvr = 0.2f / (1.0f - 0.2f);   // EXTRA_VOLUME_RATIO
+constexpr float kAtoPa = 1e12f;
+constexpr float kPsToS = 1e-12f;
+constexpr float kMmToUm = 1e3f;
+constexpr float kCbarNa = 55.964f, kCbarK = 40.514f, kCbarCa = 0.12775f, kCbarCl = 36.20f;
+constexpr float kSigmaE = 3.612f;
+constexpr float kFrtMv = 0.03742f;
+constexpr float kChargeToPa = 96.485f;
+constexpr float kZ[7] = {1.0f, 1.0f, 2.0f, -1.0f, 2.0f, -1.0f, 1.0f};
+constexpr float kDiff[7] = {1.33f, 1.96f, 0.22f, 2.03f, 0.10f, 0.0f, 9.31f};
+constexpr float kBath[7] = {140.0f, 3.0f, 1.2f, 130.0f, 1.0f, 0.0f, 4.0e-5f};
+constexpr float kNtMass[15] = {147.0f, 103.0f, 153.0f, 176.0f, 146.0f, 267.0f, 169.0f, 75.0f,
+                               111.0f, 348.0f, 507.0f, 232.0f, 24.0f, 162.0f, 0.0f};
+
+enum : uint8_t {
+    T_MYELIN = 4, T_PUMP = 7, T_AMPA = 8, T_NMDA = 9, T_GABAA = 11, T_GLYCINE = 12, T_GABAB = 17,
+    T_5HT3 = 14, T_NICOTINIC = 13, T_D1 = 44, T_D2 = 22, T_5HT7 = 43, T_CAMKII = 82,
+    T_CALCINEURIN = 83, T_ECB = 86, T_SK = 103, T_BK = 104, T_K2P = 107, T_KV7 = 112,
+    T_PP1 = 113, T_KVDR = 123, T_HCN1 = 129, T_NALCN = 133, T_CAMP_PKA = 135, T_PKC = 137,
+    T_CAV21 = 140, T_NAV16 = 141, T_NAV12 = 142, T_GLAST = 147, T_KV31 = 150, T_KV11 = 156,
+};
+
+static uint64_t g_unsupported = 0;
+
+static inline float nt_diffusion(int nt) {
+    float mass = kNtMass[nt];
+    if (mass < 1.0f) mass = 1.0f;
+    return 1.33f * powf(23.0f / mass, 1.0f / 3.0f);
+}
+
+// ---- geometry (run_tick_common.h:209-264) ----
+static inline bool is_sphere(const Segment* s) {
+    return s->segment_type == 1 || s->segment_type == 3 || s->segment_type == 4;
+}
+static inline float membrane_area(const Segment* s) {
+    return is_sphere(s) ? 4.0f * kPi * s->radius * s->radius : 2.0f * kPi * s->radius * s->length;
+}
+static inline double volume(const Segment* s) {
+    if (is_sphere(s))
+        return (4.0 / 3.0) * kPi * (double)s->radius * (double)s->radius * (double)s->radius;
+    return kPi * (double)s->radius * (double)s->radius * (double)s->length;
+}
+static inline float capacitance(const Segment* s) {
+    float c = membrane_area(s) * 0.01f;
+    if (c < 1e-4f) c = 1e-4f;
+    for (uint8_t j = 0; j < s->modifier_count && j < 16; j++) {
+        if (s->modifiers[j].type == T_MYELIN) {
+            float gr = s->modifiers[j].state[0];
+            if (gr <= 0.0f || gr >= 1.0f) gr = 0.6f;
+            float factor = 100.0f * (1.0f / gr - 1.0f) / (1.0f / 0.6f - 1.0f);
+            c /= factor;
+            break;
+        }
+    }
+    return c;
+}
+static inline float eff_area(const Modifier* m, const Segment* s) {
+    if (m->state[7] > 0.0f) return m->state[7];
+    return membrane_area(s);
+}
+static inline void floor_ions(double* a) {
+    for (int i = 0; i < 7; i++)
+        if (a[i] < 1e-9) a[i] = 1e-9;
+}
+static inline bool has_myelin(const Segment* s) {
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == T_MYELIN) return true;
+    return false;
+}
+static inline float first_state_of(const Segment* s, uint8_t type, int k, float dflt) {
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == type) return s->modifiers[j].state[k];
+    return dflt;
+}
+// Ionotropic + blocked-by-myelin predicates (kernel_dispatch.h:283-309), restricted to the IDs
+// the covered set can contain; the full list matters only for myelinated segments.
+static inline bool is_ionotropic(uint8_t t) {
+    return t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194;
+}
+
+// ---- GHK (run_tick_common.h:274-312, modifier_helpers.h:48-75) ----
+static inline double ghk_mono(double V, double ci, double co) {
+    double vn = kFaraday * V * 0.001 / (kGas * kTemp);
+    if (fabs(vn) < 1e-6) return kFaraday * (ci - co);
+    double k = kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
+    return k * kFaraday * V * 0.001 * (ci - co * exp(-vn)) / (1.0 - exp(-vn));
+}
+static inline double ghk_di(double V, double ci, double co) {
+    double vn = 2.0 * kFaraday * V * 0.001 / (kGas * kTemp);
+    if (fabs(vn) < 1e-6) return 2.0 * kFaraday * (ci - co);
+    double k = 4.0 * kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
+    return k * kFaraday * V * 0.001 * (ci - co * exp(-vn)) / (1.0 - exp(-vn));
+}
+static inline double ghk_an(double V, double ci, double co) {
+    double vn = kFaraday * V * 0.001 / (kGas * kTemp);
+    if (fabs(vn) < 1e-6) return kFaraday * (co - ci);
+    double k = kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * co;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * ci;
+    return k * kFaraday * V * 0.001 * (co - ci * exp(-vn)) / (1.0 - exp(-vn));
+}
+static inline double p_single(float gamma_ps, float z, float cbar) {
+    double g = gamma_ps * kPsToS;
+    double z2f2 = z * z * kFaraday * kFaraday;
+    double rt = kGas * kTemp;
+    return g * rt / (z2f2 * cbar);
+}
+static inline void p_partial(float gamma_ps, float fna, float fk, float fca, double* pna,
+                             double* pk, double* pca) {
+    double w = fna * kCbarNa + fk * kCbarK + fca * 4.0 * kCbarCa;
+    *pna = p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa);
+    *pk = p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK);
+    *pca = p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa);
+}
+static inline void p_nalcn(float gamma_ps, double* pna, double* pk, double* pca) {
+    const double fna = 1.3, fk = 1.2, fca = 0.5;
+    double w = fna * kCbarNa + fk * kCbarK + fca * 4.0 * kCbarCa;
+    *pna = p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa);
+    *pk = p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK);
+    *pca = p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa);
+}
+
+// ---- gate helpers (modifier_helpers.h:132-271, 909-1013) ----
+static inline float vtrap(float x, float y) {
+    if (fabsf(x / y) < 1e-6f) return y * (1.0f - x / (2.0f * y));
+    return x / (1.0f - exp(-x / y));
+}
+struct NavGates { float am_off, am_k, ah_off, bh_off, h_off; };
+static constexpr NavGates kNav16 = {43.0f, 6.0f, 50.0f, 75.0f, 72.0f};
+static constexpr NavGates kNav12 = {23.0f, 6.0f, 41.5f, 66.5f, 63.5f};
+static inline float nav_am(const NavGates& g, float V) { return 0.182f * vtrap(V + g.am_off, g.am_k); }
+static inline float nav_bm(const NavGates& g, float V) { return 0.124f * vtrap(-(V + g.am_off), g.am_k); }
+static inline float nav_ah(const NavGates& g, float V) { return 0.024f * vtrap(V + g.ah_off, 5.0f); }
+static inline float nav_bh(const NavGates& g, float V) { return 0.0091f * vtrap(-(V + g.bh_off), 5.0f); }
+static inline float nav_minf(const NavGates& g, float V) {
+    float a = nav_am(g, V), b = nav_bm(g, V);
+    return a / (a + b);
+}
+static inline float nav_mtau(const NavGates& g, float V) { return 1.0f / (nav_am(g, V) + nav_bm(g, V)); }
+static inline float nav_hinf(const NavGates& g, float V) { return 1.0f / (1.0f + exp((V + g.h_off) / 6.2f)); }
+static inline float nav_htau(const NavGates& g, float V) { return 1.0f / (nav_ah(g, V) + nav_bh(g, V)); }
+
+static inline float cav21_minf(float V) { return 1.0f / (1.0f + exp(-(V - -15.0f) / (6.0f + 1e-6f))); }
+static inline float cav21_hinf(float V, float ca_um) {
+    float h = 1.0f / (1.0f + exp((V - -50.0f) / 8.0f));
+    h *= 1.0f / (1.0f + ca_um / 3.0f);
+    return h;
+}
+static inline float kdr_tau(float V, float vh) {
+    float tau = 14.6f * exp(-(V - vh) / 30.1f);
+    if (tau < 0.3f) tau = 0.3f;
+    if (tau > 200.0f) tau = 200.0f;
+    return tau;
+}
+static inline float sk_pinf(float ca) {
+    if (ca < 1e-9f) ca = 1e-9f;
+    float r = 0.0004f / ca;
+    return 1.0f / (1.0f + pow(r, 4.0f));
+}
+static inline float bk_po(float V, float ca) {
+    float vn = kFaraday * V * 1e-3f / (kGas * kTemp);
+    float J = 0.104f * exp(0.55f * vn);
+    float L = 3.7e-8f * exp(0.30f * vn);
+    if (ca < 0.0f) ca = 0.0f;
+    float K = ca / 0.044f;
+    float o = 1.0f + K * 23.0f + J * 32.0f + J * K * 23.0f * 32.0f * 3.7f;
+    float c = 1.0f + K + J + J * K * 3.7f;
+    o *= o; o *= o;
+    c *= c; c *= c;
+    float num = L * o;
+    return num / (num + c);
+}
+static inline float pump_vnorm(float V, float na, float ko, float atp_level) {
+    const float kmna = 10.0f, kmk = 1.5f;
+    float na3 = na * na * na, k2 = ko * ko;
+    float kna3 = kmna * kmna * kmna, kk2 = kmk * kmk;
+    float fna = na3 / (na3 + kna3), fk = k2 / (k2 + kk2);
+    float atp = atp_level;
+    if (atp < 0.0f) atp = 0.0f;
+    float fatp = atp / (atp + 0.5f);
+    float eta = 0.8f * kFaraday * (V * 1e-3f) / (kGas * kTemp);
+    float a2 = 60.0f * exp(0.5f * eta);
+    float b2 = 6.0f * exp(-(1.0f - 0.5f) * eta);
+    float a1 = 200.0f * fna * fatp;
+    float a3 = 200.0f * fk;
+    float a4 = 60.0f;
+    float denom = a3 / a1 + (b2 + a3) / a2 + 1.0f + a3 / a4;
+    float turnover = a3 / denom;
+    float k3 = 200.0f;
+    float vmax = k3 / (k3 / 200.0f + 1.0f + k3 / 60.0f);
+    return turnover / vmax;
+}
+
+// ---- channel currents (return pA, negative = outward) ----
+static double chan_pump(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:57-97
+    float area = eff_area(m, s);
+    float vn = pump_vnorm(s->V_m, s->intra_ions[0], s->extra_ions[1], s->ATP_level);
+    double N = m->density * area;
+    float ip = N * vn * 75.0f * 1.602176634e-19f * kAtoPa;
+    float vol = volume(s);
+    double cpm = kFaraday * vol / 1000.0;
+    float tr = ip * dt / cpm;
+    float na_av = (s->intra_ions[0] - 1e-9f) / 3.0f;
+    float k_av = (s->extra_ions[1] - 1e-9f) * kEvr / 2.0f;
+    if (na_av < 0.0f) na_av = 0.0f;
+    if (k_av < 0.0f) k_av = 0.0f;
+    float mx = (na_av < k_av) ? na_av : k_av;
+    if (tr > mx) tr = mx;
+    s->intra_ions[0] -= 3.0f * tr;
+    s->extra_ions[0] += 3.0f * tr / kEvr;
+    s->intra_ions[1] += 2.0f * tr;
+    s->extra_ions[1] -= 2.0f * tr / kEvr;
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    s->ATP_level -= ip * dt * 1e-8f;
+    if (s->ATP_level < 0.0f) s->ATP_level = 0.0f;
+    return -ip;
+}
+
+static double chan_nav(Modifier* m, Segment* s, float dt, const NavGates& g, float gamma,
+                       float ttx_ic50, float pkx) {   // hh_biological.cpp:193-277
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    if (s->V_m < -200.0f) s->V_m = -200.0f;
+    if (s->V_m > 200.0f) s->V_m = 200.0f;
+    V = s->V_m;
+    double mv = nav_minf(g, V) + (m->state[0] - nav_minf(g, V)) * exp(-dt / nav_mtau(g, V));
+    double hv = nav_hinf(g, V) + (m->state[1] - nav_hinf(g, V)) * exp(-dt / nav_htau(g, V));
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m->state[0] = mv;
+    m->state[1] = hv;
+    float op = mv * mv * mv * hv;
+    double ttx = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + ttx / ttx_ic50);
+    float pkc = first_state_of(s, T_PKC, 0, 0.0f);
+    float pka = first_state_of(s, T_CAMP_PKA, 1, 0.0f);
+    op *= (1.0f - pkx * (pkc + pka) * 0.5f);
+    if (op < 0.0f) op = 0.0f;
+    double P = p_single(gamma, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double ia = P * flux * (m->density * area) * op;
+    double ina = ia * kAtoPa;
+    double vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ina;
+}
+
+static double chan_cav21(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:364-388
+    float V = s->V_m;
+    float mi = cav21_minf(V);
+    m->state[0] += (mi - m->state[0]) * dt / 1.0f;
+    float ca = s->intra_ions[2] * kMmToUm;
+    float hi = cav21_hinf(V, ca);
+    m->state[1] += (hi - m->state[1]) * dt / 300.0f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    return 0.0f;
+}
+
+// K+ efflux bookkeeping shared by the K channels: flux in double, extra floor only.
+static inline double k_out(Segment* s, double ik, float dt) {
+    double vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+static double chan_kv31(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:1043-1100
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + exp(-(V - 10.0f) / (10.0f + 1e-6f)));
+    float tau_m = 1.0f;
+    m->state[0] += (mi - m->state[0]) * dt / tau_m;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float hi = 1.0f / (1.0f + exp((V + 30.0f) / 6.0f));
+    float tau_h = 2000.0f;
+    m->state[1] += (hi - m->state[1]) * dt / tau_h;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(27.0f, 1.0f, kCbarK);
+    double op = m->state[0] * m->state[0] * m->state[0] * m->state[1];
+    double tea = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + tea / 0.2f);
+    float pkc = first_state_of(s, T_PKC, 0, 0.0f);
+    float kp = 5.6e-6f, kd = 1e-4f;
+    m->state[2] += (kp * pkc * (1.0f - m->state[2]) - kd * m->state[2]) * dt;
+    if (m->state[2] < 0.0f) m->state[2] = 0.0f;
+    if (m->state[2] > 1.0f) m->state[2] = 1.0f;
+    op *= (1.0f - 0.27f * m->state[2]);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * op;
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+static double chan_kv11(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:1189-1231
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float vh = -30.0f, kk = 6.0f, gamma = 10.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (kk + 1e-6f)));
+    float tau_m = kdr_tau(V, vh);
+    m->state[0] += (mi - m->state[0]) * dt / tau_m;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float hi = 1.0f / (1.0f + exp((V + 40.0f) / 6.0f));
+    float tau_h = 3000.0f;
+    m->state[1] += (hi - m->state[1]) * dt / tau_h;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    double op = m->state[0] * m->state[0] * m->state[0] * m->state[0] * m->state[1];
+    double dtx = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    double tea = (m->state[6] > 0.0f) ? m->state[6] : 0.0f;
+    op /= (1.0f + dtx / 2.0f) * (1.0f + tea / 170.0f);
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * op;
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+static double chan_kvdr(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:2763-2798
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double mv = m->state[0];
+    double vh = (m->state[1] != 0.0f) ? m->state[1] : -30.0f;
+    double kk = (m->state[2] > 0.0f) ? m->state[2] : 6.0f;
+    double tau = (m->state[4] > 0.0f) ? m->state[4] : kdr_tau(V, vh);
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (kk + 1e-6f)));
+    mv += (mi - mv) * dt / (tau + 1e-6f);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m->state[0] = mv;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(12.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * mv;
+    double ik = ia * kAtoPa;
+    double vol = volume(s);
+    float cpmv = kFaraday * vol / 1000.0;   // float in the reference
+    s->intra_ions[1] -= (ik * dt) / cpmv;
+    s->extra_ions[1] += (ik * dt) / (cpmv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+static double chan_sk(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:2859-2888
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double op = m->state[0];
+    double gamma = (m->state[3] > 0.0f) ? m->state[3] : 10.0f;
+    double t_open = (m->state[4] > 0.0f) ? m->state[4] : 5.0f;
+    double t_close = (m->state[5] > 0.0f) ? m->state[5] : 50.0f;
+    float ca = s->intra_ions[2];
+    float pi = sk_pinf(ca);
+    float tau = (pi > op) ? t_open : t_close;
+    op += (pi - op) * dt / tau;
+    if (op < 0.0f) op = 0.0f;
+    if (op > 1.0f) op = 1.0f;
+    m->state[0] = op;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * op;
+    float ik = (float)(ia * kAtoPa);
+    double vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0f;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+static double chan_bk(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:2922-2949
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double op = m->state[0];
+    double gamma = (m->state[1] > 0.0f) ? m->state[1] : 250.0f;
+    double tau = (m->state[2] > 0.0f) ? m->state[2] : 1.0f;
+    float ca = s->intra_ions[2];
+    float pi = bk_po(V, ca);
+    op += (pi - op) * dt / tau;
+    if (op < 0.0f) op = 0.0f;
+    if (op > 1.0f) op = 1.0f;
+    m->state[0] = op;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * op;
+    float ik = (float)(ia * kAtoPa);
+    double vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0f;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+static double chan_k2p(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:3063-3085
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[0] > 0.0f) ? m->state[0] : 100.0f;
+    double po = (m->state[1] > 0.0f) ? m->state[1] : 0.012f;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * po;
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+static double chan_kv7(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:3158-3220
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double mv = m->state[0];
+    double vh = (m->state[1] != 0.0f) ? m->state[1] : -38.0f;
+    double ks = (m->state[2] > 0.0f) ? m->state[2] : 10.0f;
+    double t_act = (m->state[4] > 0.0f) ? m->state[4] : 100.0f;
+    double t_deact = (m->state[5] > 0.0f) ? m->state[5] : 60.0f;
+    double pip2 = (m->state[6] >= 0.0f) ? m->state[6] : 1.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / ks));
+    float tau_m = t_deact + (t_act - t_deact) * mi;
+    if (tau_m < 1.0f) tau_m = 1.0f;
+    mv += (mi - mv) * dt / tau_m;
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m->state[0] = mv;
+    if (pip2 < 0.01f) return 0.0f;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(7.0f, 1.0f, kCbarK);
+    float pkc = first_state_of(s, T_PKC, 0, 0.0f);
+    float sup = 1.0f - 0.8f * pkc;
+    if (sup < 0.0f) sup = 0.0f;
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * (mv * pip2 * sup);
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+static double chan_hcn1(Modifier* m, Segment* s, float dt) {   // modifier_helpers.h:1442-1484
+    double V = s->V_m;
+    float area = eff_area(m, s);
+    float vh = (m->state[1] != 0) ? m->state[1] : -69.0f;
+    float k = (m->state[2] > 0) ? m->state[2] : 6.0f;
+    float gamma = (m->state[3] > 0) ? m->state[3] : 0.84f;
+    float t_open = (m->state[4] > 0) ? m->state[4] : 84.0f;
+    float t_close = (m->state[5] > 0) ? m->state[5] : 300.0f;
+    float camp = first_state_of(s, T_CAMP_PKA, 0, 0.1f);
+    float ve = vh + 15.0f * camp / (camp + 0.5f);
+    float mv = m->state[0];
+    float vf = V;
+    float mi = 1.0f / (1.0f + exp((vf - ve) / k));
+    float tau = (V < ve) ? (t_open * exp((V - ve) / 15.0f)) : t_close;
+    mv += (mi - mv) * dt / tau;
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m->state[0] = mv;
+    double pna, pk, pca;
+    p_partial(gamma, 0.2f, 0.8f, 0.0f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double fk = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    double in = pna * fna * N * mv * kAtoPa;
+    double ik = pk * fk * N * mv * kAtoPa;
+    double vol = volume(s);
+    double cpm = kFaraday * vol / 1000.0;
+    s->intra_ions[0] -= (in * dt) / cpm;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    s->extra_ions[0] += (in * dt) / (cpm * kEvr);
+    s->extra_ions[1] += (ik * dt) / (cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(in + ik);
+}
+
+static double chan_nalcn(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:3609-3641
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[0] > 0.0f) ? m->state[0] : 27.0f;
+    double pna, pk, pca;
+    p_nalcn(gamma, &pna, &pk, &pca);
+    double N = m->density * area;
+    float ina = (float)(pna * ghk_mono(V, s->intra_ions[0], s->extra_ions[0]) * N * kAtoPa);
+    float ik = (float)(pk * ghk_mono(V, s->intra_ions[1], s->extra_ions[1]) * N * kAtoPa);
+    float ica = (float)(pca * ghk_di(V, s->intra_ions[2], s->extra_ions[2]) * N * kAtoPa);
+    double vol = volume(s);
+    double cpm = kFaraday * vol / 1000.0;
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->extra_ions[0] += (ina * dt) / (cpm * kEvr);
+    s->extra_ions[1] += (ik * dt) / (cpm * kEvr);
+    s->extra_ions[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(ina - ik + ica);
+}
+
+// Ionotropic receptor (modifier_helpers.h:1213-1276).
+struct Iono { float gamma, ec50, t_rise, t_decay; int nt; float pna, pk, pca; bool anion; float t_desens; };
+static constexpr Iono kAmpa = {8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f};
+static constexpr Iono kNmda = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
+static constexpr Iono kGabaa = {28.0f, 1.0f, 1.0f, 5.0f, 1, 0.0f, 0.0f, 0.0f, true, 2000.0f};
+
+static double iono_current(Modifier* m, Segment* s, float dt, const Iono& p) {
+    float area = eff_area(m, s);
+    double V = s->V_m;
+    double vol = volume(s);
+    double cpm = kFaraday * vol / 1000.0;
+    float gamma = (m->active_vars > 0 && m->state[m->active_vars] > 0.0f) ? m->state[m->active_vars] : p.gamma;
+    float nt = s->neurotransmitter[p.nt];
+    float r_inf = nt / (nt + p.ec50);
+    float r = m->state[0];
+    float tau = (r_inf > r) ? p.t_rise : p.t_decay;
+    m->state[0] += (r_inf - r) * dt / tau;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float op = m->state[0];
+    if (p.t_desens > 0.0f) {
+        float kd = p.ec50;
+        float d = m->state[1];
+        float d_inf = nt / (nt + kd);
+        d += (d_inf - d) * dt / p.t_desens;
+        if (d < 0.0f) d = 0.0f;
+        if (d > 1.0f) d = 1.0f;
+        m->state[1] = d;
+        op *= (1.0f - d);
+    }
+    double N = m->density * area;
+    const float scale = 1.0f;
+    if (p.anion) {
+        double pcl = p_single(gamma, 1.0f, kCbarCl);
+        double fcl = ghk_an(V, s->intra_ions[3], s->extra_ions[3]);
+        double icl = pcl * fcl * N * op * scale * kAtoPa;
+        s->intra_ions[3] += (icl * dt) / cpm;
+        floor_ions(s->intra_ions);
+        return -icl;
+    }
+    double pna, pk, pca;
+    p_partial(gamma, p.pna, p.pk, p.pca, &pna, &pk, &pca);
+    double ina = pna * ghk_mono(V, s->intra_ions[0], s->extra_ions[0]) * N * op * scale * kAtoPa;
+    double ik = pk * ghk_mono(V, s->intra_ions[1], s->extra_ions[1]) * N * op * scale * kAtoPa;
+    double itot = ina + ik;
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    if (p.pca > 0.0f) {
+        double ica = pca * ghk_di(V, s->intra_ions[2], s->extra_ions[2]) * N * op * scale * kAtoPa;
+        itot += ica;
+        s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    }
+    floor_ions(s->intra_ions);
+    return -itot;
+}
+static double nmda_cdi(Modifier* m, Segment* s, float dt) {   // modifier_helpers.h:1281-1291
+    double ca_um = s->intra_ions[2] * kMmToUm;
+    double cdi_inf = 1.0 - 0.8f * ca_um / (ca_um + 1.0f);
+    float tau = (cdi_inf < m->state[2]) ? 500.0f : 9000.0f;
+    m->state[2] += (cdi_inf - m->state[2]) * dt / tau;
+    if (m->state[2] < 0.0f) m->state[2] = 0.0f;
+    if (m->state[2] > 1.0f) m->state[2] = 1.0f;
+    float coag = 10.0f / (10.0f + 1.0f);
+    return m->state[2] * coag;
+}
+static double rec_nmda(Modifier* m, Segment* s, float dt) {   // receptors_biological.cpp:48-55
+    double I = iono_current(m, s, dt, kNmda);
+    float mg = 1.0f / (1.0f + (1.0f / 3.57f) * exp(-0.062f * s->V_m));
+    double r = I * mg;
+    return r * nmda_cdi(m, s, dt);
+}
+
+static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
+    switch (type) {
+        case T_PUMP: return chan_pump(m, s, dt);
+        case T_AMPA: return iono_current(m, s, dt, kAmpa);
+        case T_NMDA: return rec_nmda(m, s, dt);
+        case T_GABAA: return iono_current(m, s, dt, kGabaa);
+        case T_SK: return chan_sk(m, s, dt);
+        case T_BK: return chan_bk(m, s, dt);
+        case T_K2P: return chan_k2p(m, s, dt);
+        case T_KV7: return chan_kv7(m, s, dt);
+        case T_KVDR: return chan_kvdr(m, s, dt);
+        case T_HCN1: return chan_hcn1(m, s, dt);
+        case T_NALCN: return chan_nalcn(m, s, dt);
+        case T_CAV21: return chan_cav21(m, s, dt);
+        case T_NAV16: return chan_nav(m, s, dt, kNav16, 20.7f, 5.0f, 0.2f);
+        case T_NAV12: return chan_nav(m, s, dt, kNav12, 20.8f, 10.0f, 0.7f);
+        case T_KV31: return chan_kv31(m, s, dt);
+        case T_KV11: return chan_kv11(m, s, dt);
+        default: g_unsupported++; return 0.0;
+    }
+}
+
+// ---- step 2.5: electrodiffusion (diffusion_kernel.h:41-364) ----
+static inline float bernoulli(float x) {
+    if (fabsf(x) < 1e-3f) return 1.0f - 0.5f * x;
+    if (x > 50.0f) return 0.0f;
+    if (x < -50.0f) return -x;
+    return x / (exp(x) - 1.0f);
+}
+
+struct Ctx {
+    std::vector<double> phi, sigma, diag, phi_new, goff_d;
+    std::vector<double> dacc, dext;
+    std::vector<float> gdiag, goff, inpd, vold, vnew, off;
+    std::vector<float> dnt_in, dnt, datp, dtemp, dtxn;
+    std::vector<uint8_t> delay;   // MAX_DELAY_SLOTS(4) x 16-byte DelayedRelease per synapse
+    uint32_t delay_syn = UINT32_MAX;
+    std::vector<uint32_t> spikes;
+    int substeps_override = 0;
+};
+
+static void solve_phi(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint32_t nc) {
+    c.phi.assign(n, 0.0);
+    c.sigma.assign(n, 0.0);
+    c.diag.assign(n, 0.0);
+    c.phi_new.assign(n, 0.0);
+    c.goff_d.assign(nc, 0.0);
+    for (uint32_t i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int ion = 0; ion < 7; ion++) s += kZ[ion] * kZ[ion] * kDiff[ion] * sg[i].extra_ions[ion];
+        c.sigma[i] = kSigmaE * s;
+        double gb = c.sigma[i] * membrane_area(&sg[i]) / 1.0f;
+        c.diag[i] = gb;
+    }
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+        if (a >= n || b >= n) continue;
+        float rmin = sg[a].radius < sg[b].radius ? sg[a].radius : sg[b].radius;
+        float ac = kPi * rmin * rmin;
+        if (ac < 1e-6f) ac = 1e-6f;
+        float dist = (sg[a].length + sg[b].length) * 0.5f;
+        if (dist < 0.1f) dist = 0.1f;
+        double g = 0.5 * (c.sigma[a] + c.sigma[b]) * ac / dist;
+        c.goff_d[i] = g;
+        c.diag[a] += g;
+        c.diag[b] += g;
+    }
+    for (int it = 0; it < 50; it++) {
+        c.phi_new.assign(n, 0.0);
+        for (uint32_t i = 0; i < nc; i++) {
+            uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+            if (a >= n || b >= n) continue;
+            double g = c.goff_d[i];
+            if (g <= 0.0) continue;
+            c.phi_new[a] += g * c.phi[b];
+            c.phi_new[b] += g * c.phi[a];
+        }
+        for (uint32_t i = 0; i < n; i++) {
+            c.phi[i] = (c.diag[i] > 1e-9) ? (c.phi_new[i] - sg[i].I_buffer) / c.diag[i] : 0.0;
+            if (c.phi[i] > 10.0) c.phi[i] = 10.0;
+            if (c.phi[i] < -10.0) c.phi[i] = -10.0;
+        }
+    }
+}
+
+struct ConnGeom { float ac, dist, va, vb; };
+static inline ConnGeom conn_geom(const Segment* a, const Segment* b) {
+    ConnGeom g;
+    float rmin = a->radius < b->radius ? a->radius : b->radius;
+    g.ac = kPi * rmin * rmin;
+    if (g.ac < 1e-6f) g.ac = 1e-6f;
+    g.dist = (a->length + b->length) * 0.5f;
+    if (g.dist < 0.1f) g.dist = 0.1f;
+    g.va = volume(a);
+    g.vb = volume(b);
+    if (g.va < 1e-6f) g.va = 1e-6f;
+    if (g.vb < 1e-6f) g.vb = 1e-6f;
+    return g;
+}
+
+static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint32_t nc, float dt) {
+    if (nc == 0) return;
+    c.dext.assign((size_t)n * 7, 0.0);
+    c.dacc.assign((size_t)n * 7, 0.0);
+    c.dnt_in.assign((size_t)n * 15, 0.0f);
+    c.dnt.assign((size_t)n * 15, 0.0f);
+    c.datp.assign(n, 0.0f);
+    c.dtemp.assign(n, 0.0f);
+    c.dtxn.assign(n, 0.0f);
+    c.gdiag.assign(n, 0.0f);
+    c.goff.assign(nc, 0.0f);
+    c.inpd.assign(n, 0.0f);
+    c.vold.assign(n, 0.0f);
+    c.vnew.assign(n, 0.0f);
+    c.off.assign(n, 0.0f);
+    solve_phi(c, sg, n, cn, nc);
+    const std::vector<double>& phi = c.phi;
+
+    const int kSub = 5;
+    const float dts = dt / kSub;
+    for (int sub = 0; sub < kSub; sub++) {
+        std::fill(c.dacc.begin(), c.dacc.end(), 0.0);
+        std::fill(c.gdiag.begin(), c.gdiag.end(), 0.0f);
+        std::fill(c.inpd.begin(), c.inpd.end(), 0.0f);
+        for (uint32_t i = 0; i < nc; i++) {
+            uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+            if (a >= n || b >= n) continue;
+            if (cn[i].is_intracellular == 0) continue;
+            ConnGeom g = conn_geom(&sg[a], &sg[b]);
+            float sig = 0.0f;
+            for (int ion = 0; ion < 7; ion++) {
+                float cbar = 0.5f * (sg[a].intra_ions[ion] + sg[b].intra_ions[ion]);
+                sig += kZ[ion] * kZ[ion] * kDiff[ion] * cbar;
+            }
+            float gax = kSigmaE * sig * g.ac / g.dist;
+            c.goff[i] = gax;
+            c.gdiag[a] += gax;
+            c.gdiag[b] += gax;
+            float dV = (sg[a].V_m + phi[a]) - (sg[b].V_m + phi[b]);
+            float inp = 0.0f;
+            for (int ion = 0; ion < 7; ion++) {
+                float ca = sg[a].intra_ions[ion], cb = sg[b].intra_ions[ion];
+                float x = kZ[ion] * kFrtMv * dV;
+                float fl = kDiff[ion] * g.ac * dts / g.dist * (ca * bernoulli(-x) - cb * bernoulli(x));
+                c.dacc[(size_t)a * 7 + ion] -= (double)fl / g.va;
+                c.dacc[(size_t)b * 7 + ion] += (double)fl / g.vb;
+                inp += kZ[ion] * kChargeToPa * fl / dts;
+            }
+            c.inpd[a] += inp - gax * dV;
+            c.inpd[b] -= inp - gax * dV;
+        }
+        for (uint32_t i = 0; i < n; i++) { c.vold[i] = sg[i].V_m; c.vnew[i] = sg[i].V_m; }
+        for (int it = 0; it < 50; it++) {
+            std::fill(c.off.begin(), c.off.end(), 0.0f);
+            for (uint32_t i = 0; i < nc; i++) {
+                uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+                if (a >= n || b >= n) continue;
+                if (cn[i].is_intracellular == 0) continue;
+                c.off[a] += c.goff[i] * c.vnew[b];
+                c.off[b] += c.goff[i] * c.vnew[a];
+            }
+            for (uint32_t i = 0; i < n; i++) {
+                float cdt = capacitance(&sg[i]) / dts;
+                if (cdt < 1e-9f) cdt = 1e-9f;
+                c.vnew[i] = (cdt * c.vold[i] + c.inpd[i] + c.off[i]) / (cdt + c.gdiag[i]);
+            }
+        }
+        for (uint32_t i = 0; i < n; i++) sg[i].V_m = c.vnew[i];
+        for (uint32_t i = 0; i < n; i++)
+            for (int ion = 0; ion < 7; ion++) sg[i].intra_ions[ion] += (float)c.dacc[(size_t)i * 7 + ion];
+    }
+
+    for (int sub = 0; sub < kSub; sub++) {
+        std::fill(c.dext.begin(), c.dext.end(), 0.0);
+        for (uint32_t i = 0; i < nc; i++) {
+            uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+            if (a >= n || b >= n) continue;
+            ConnGeom g = conn_geom(&sg[a], &sg[b]);
+            for (int ion = 0; ion < 7; ion++) {
+                double ca = sg[a].extra_ions[ion], cb = sg[b].extra_ions[ion];
+                double dphi = phi[a] - phi[b];
+                float x = kZ[ion] * kFrtMv * dphi;
+                double fl = kDiff[ion] * g.ac * dts / g.dist * (ca * bernoulli(-x) - cb * bernoulli(x));
+                c.dext[(size_t)a * 7 + ion] -= fl / (g.va * kEvr);
+                c.dext[(size_t)b * 7 + ion] += fl / (g.vb * kEvr);
+            }
+        }
+        for (uint32_t i = 0; i < n; i++)
+            for (int ion = 0; ion < 7; ion++) sg[i].extra_ions[ion] += c.dext[(size_t)i * 7 + ion];
+    }
+
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+        if (a >= n || b >= n) continue;
+        ConnGeom g = conn_geom(&sg[a], &sg[b]);
+        bool intra = cn[i].is_intracellular != 0;
+        if (intra) {
+            for (int nt = 0; nt < 15; nt++) {
+                float dC = sg[a].intra_nt[nt] - sg[b].intra_nt[nt];
+                if (dC < 1e-12f && dC > -1e-12f) continue;
+                float fl = nt_diffusion(nt) * g.ac * dC * dt / g.dist;
+                c.dnt_in[(size_t)a * 15 + nt] -= fl / g.va;
+                c.dnt_in[(size_t)b * 15 + nt] += fl / g.vb;
+            }
+        }
+        for (int nt = 0; nt < 15; nt++) {
+            float dC = sg[a].neurotransmitter[nt] - sg[b].neurotransmitter[nt];
+            if (dC < 1e-12f && dC > -1e-12f) continue;
+            float fl = nt_diffusion(nt) * g.ac * dC * dt / g.dist;
+            c.dnt[(size_t)a * 15 + nt] -= fl / g.va;
+            c.dnt[(size_t)b * 15 + nt] += fl / g.vb;
+        }
+        if (intra) {
+            float d1 = sg[a].ATP_level - sg[b].ATP_level;
+            if (d1 > 1e-12f || d1 < -1e-12f) {
+                float fl = 0.14f * g.ac * d1 * dt / g.dist;
+                c.datp[a] -= fl / g.va;
+                c.datp[b] += fl / g.vb;
+            }
+            float d2 = sg[a].temperature - sg[b].temperature;
+            if (d2 > 0.001f || d2 < -0.001f) {
+                float fl = 0.0014f * g.ac * d2 * dt / g.dist;
+                c.dtemp[a] -= fl / g.va;
+                c.dtemp[b] += fl / g.vb;
+            }
+            float d3 = sg[a].transcription_signal - sg[b].transcription_signal;
+            if (d3 > 1e-9f || d3 < -1e-9f) {
+                float fl = 0.01f * g.ac * d3 * dt / g.dist;
+                c.dtxn[a] -= fl / g.va;
+                c.dtxn[b] += fl / g.vb;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        for (int ion = 0; ion < 7; ion++) {
+            sg[i].extra_ions[ion] += (kBath[ion] - sg[i].extra_ions[ion]) * dt / 1.0f;
+            if (sg[i].extra_ions[ion] < 1e-9) sg[i].extra_ions[ion] = 1e-9;
+            if (sg[i].extra_ions[ion] > 1000.0) sg[i].extra_ions[ion] = 1000.0;
+        }
+        for (int nt = 0; nt < 15; nt++) {
+            sg[i].intra_nt[nt] += c.dnt_in[(size_t)i * 15 + nt];
+            sg[i].neurotransmitter[nt] += c.dnt[(size_t)i * 15 + nt];
+        }
+        sg[i].ATP_level += c.datp[i];
+        sg[i].temperature += c.dtemp[i];
+        sg[i].transcription_signal += c.dtxn[i];
+    }
+}
+
+// ---- step 3 helpers (delay_buffer.h, kernel_dispatch.h:612-663, modifier_helpers.h:764-836) ----
+#pragma pack(push, 1)
+struct Delayed { uint64_t arrival; float amount; uint8_t nt; uint8_t pad[3]; };
+#pragma pack(pop)
+static_assert(sizeof(Delayed) == 16, "delay slot");
+
+static inline float xorshift_draw(uint32_t seed) {
+    uint32_t x = seed;
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return (float)(x & 0x7FFFFFFFu) / (float)0x7FFFFFFFu;
+}
+static inline uint8_t nt_of_receptor(uint8_t rt) {
+    if (rt == T_GABAA || rt == T_GABAB) return 1;
+    if (rt == T_GLYCINE) return 7;
+    if (rt == T_5HT3) return 3;
+    if (rt == T_NICOTINIC) return 4;
+    if (rt == T_D1 || rt == T_D2) return 2;
+    return 0;
+}
+static void rrp_recover(Synapse* y, const Segment* pre, float dt) {
+    if (y->RRP_max <= 0.0f || y->RRP >= y->RRP_max) return;
+    float ca = pre->intra_ions[2];
+    float caf = ca / (0.0005f + ca);
+    float camkii = 0, calcn = 0;
+    for (int j = 0; j < pre->modifier_count; j++) {
+        if (pre->modifiers[j].type == T_CAMKII) camkii = pre->modifiers[j].state[0];
+        if (pre->modifiers[j].type == T_CALCINEURIN) calcn = pre->modifiers[j].state[0];
+    }
+    float prim = y->RRP_recovery + camkii * 0.02f + calcn * 0.01f;
+    prim *= caf;
+    float rec = (y->RRP_max - y->RRP) * prim * dt;
+    y->RRP += rec;
+    if (y->RRP > y->RRP_max) y->RRP = y->RRP_max;
+}
+static float rrp_release(Synapse* y, float amount, float p) {
+    if (y->RRP_max <= 0.0f) return amount * p;
+    float av = amount;
+    if (av > y->RRP) av = y->RRP;
+    float rel = av * p;
+    y->RRP -= rel;
+    if (y->RRP < 0) y->RRP = 0;
+    return rel;
+}
+static void bouton_step(const Segment* pre, Synapse* y, float dt) {
+    float vb = pre->V_m;
+    float dens = 0.0f, mg = 0.0f, hg = 0.0f;
+    uint8_t mc = pre->modifier_count;
+    if (mc > 32) mc = 0;
+    for (uint8_t j = 0; j < mc; j++)
+        if (pre->modifiers[j].type == T_CAV21) {
+            dens = pre->modifiers[j].density;
+            float mi = cav21_minf(vb);
+            float hi = cav21_hinf(vb, y->bouton_Ca * kMmToUm);
+            mg = mi;
+            hg = hi + (pre->modifiers[j].state[1] - hi) * exp(-dt / 300.0f);
+        }
+    float open = mg * mg * hg;
+    double P = p_single(15.0f, 2.0f, kCbarCa);
+    double flux = ghk_di(vb, y->bouton_Ca, pre->extra_ions[2]);
+    double ia = P * flux * (dens * y->pre_area) * open;
+    float ica = (float)(ia * kAtoPa);
+    y->bouton_Ca -= ica * dt * 1000.0f / (2.0f * kFaraday * 0.1f);
+    y->bouton_Ca += (0.0001f - y->bouton_Ca) * dt / 5.0f;
+    if (y->bouton_Ca < 0.0f) y->bouton_Ca = 0.0f;
+}
+static void spine_ca(Synapse* y, float isyn, float dt) {
+    float ia = isyn < 0.0f ? -isyn : isyn;
+    float ica = ia * 0.1f;
+    float influx = ica * dt * 1000.0f / (2.0f * kFaraday * 0.05f * 100.0f);
+    y->spine_Ca += influx;
+    y->spine_Ca += (0.0001f - y->spine_Ca) * dt / 50.0f;
+    if (y->spine_Ca < 0.0f) y->spine_Ca = 0.0f;
+}
+static void spine_plast(Synapse* y, float dt) {
+    float cu = y->spine_Ca * kMmToUm;
+    float c4 = cu * cu * cu * cu;
+    float ck_inf = c4 / (c4 + 1.0f + 1e-9f);
+    y->spine_camkii += (ck_inf - y->spine_camkii) * dt / 2000.0f;
+    if (y->spine_camkii < 0.0f) y->spine_camkii = 0.0f;
+    if (y->spine_camkii > 1.0f) y->spine_camkii = 1.0f;
+    float c2 = cu * cu;
+    float cn_inf = c2 / (c2 + 0.05f * 0.05f + 1e-9f);
+    y->spine_calcineurin += (cn_inf - y->spine_calcineurin) * dt / 300.0f;
+    if (y->spine_calcineurin < 0.0f) y->spine_calcineurin = 0.0f;
+    if (y->spine_calcineurin > 1.0f) y->spine_calcineurin = 1.0f;
+}
+
+static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t ns, uint64_t tick, float dt) {
+    if (ns != c.delay_syn) {
+        c.delay.assign((size_t)ns * 4 * 16, 0);
+        c.delay_syn = ns;
+    }
+    if (ns > 0)
+        for (uint32_t i = 0; i < n; i++)
+            for (int k = 0; k < 15; k++) sg[i].neurotransmitter[k] = 0.0f;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint8_t j = 0; j < sg[i].modifier_count; j++)
+            if (sg[i].modifiers[j].type == T_NMDA) sg[i].neurotransmitter[0] = 1.0f;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint8_t j = 0; j < sg[i].modifier_count; j++)
+            if (sg[i].modifiers[j].type == T_5HT7) sg[i].neurotransmitter[3] = 0.0f;
+    for (uint32_t g = 0; g < n; g++)
+        for (uint8_t j = 0; j < sg[g].modifier_count; j++)
+            if (sg[g].modifiers[j].type == T_GLAST) g_unsupported++;
+
+    for (uint32_t i = 0; i < ns; i++) {
+        Synapse* y = &sy[i];
+        if (y->pre_segment >= n || y->post_segment >= n) continue;
+        Segment* pre = &sg[y->pre_segment];
+        Segment* post = &sg[y->post_segment];
+        Delayed* slots = reinterpret_cast<Delayed*>(&c.delay[(size_t)i * 64]);
+        if (y->delay > 0.0f && !c.delay.empty()) {
+            for (int s = 0; s < 4; s++) {
+                if (slots[s].arrival == 0) continue;
+                if (slots[s].arrival <= tick) {
+                    y->cleft_neurotransmitter[slots[s].nt] += slots[s].amount;
+                    slots[s].arrival = 0;
+                }
+            }
+        }
+        if (y->type == 1 || y->type == 2) { g_unsupported++; continue; }
+        if (y->pre_autoreceptor_count > 0) g_unsupported++;
+        rrp_recover(y, pre, dt);
+        bouton_step(pre, y, dt);
+        if (y->refractory == 0) {
+            float cu = y->bouton_Ca * kMmToUm;
+            float act = pow(cu, 4.0f) / (pow(cu, 4.0f) + pow(10.0f, 4.0f));
+            float p = y->release_probability * act;
+            uint32_t seed = y->id ^ ((uint32_t)tick) ^ (uint32_t)(int64_t)(pre->V_m * 1000.0f);
+            if (xorshift_draw(seed) < p) {
+                uint8_t nt = (y->post_receptor_count > 0) ? nt_of_receptor(y->post_receptors[0].type) : 0;
+                float amt = rrp_release(y, y->release_amount, p);
+                uint64_t dly = (y->delay <= 0.0f || dt <= 0.0f) ? 0 : (uint64_t)((y->delay / dt) + 0.5f);
+                if (dly > 0) {
+                    for (int s = 0; s < 4; s++)
+                        if (slots[s].arrival == 0) {
+                            slots[s].arrival = tick + dly;
+                            slots[s].amount = amt;
+                            slots[s].nt = nt;
+                            break;
+                        }
+                } else {
+                    y->cleft_neurotransmitter[nt] += amt;
+                }
+                y->refractory = (uint32_t)(20.0f / dt + 0.5f);
+            }
+        }
+        for (int k = 0; k < 15; k++) post->neurotransmitter[k] += y->cleft_neurotransmitter[k];
+        bool my = has_myelin(post);
+        float isyn = 0.0f;
+        for (int j = 0; j < y->post_receptor_count; j++) {
+            uint8_t rt = y->post_receptors[j].type;
+            if (my && is_ionotropic(rt)) continue;
+            y->post_receptors[j].state[7] = y->post_area;
+            isyn += dispatch(rt, &y->post_receptors[j], post, dt);
+        }
+        post->I_buffer += isyn;
+        spine_ca(y, isyn, dt);
+        for (int k = 0; k < 15; k++) {
+            y->cleft_neurotransmitter[k] *= exp(-dt / y->tau_clearance);
+            if (y->cleft_neurotransmitter[k] < 0.0f) y->cleft_neurotransmitter[k] = 0.0f;
+        }
+        float cu = y->spine_Ca * 1000.0f;
+        float csq = cu * cu;
+        float tau_theta = 300000.0f;
+        y->facilitation += (csq - y->facilitation) * dt / tau_theta;
+        if (y->facilitation < 0.0f) y->facilitation = 0.0f;
+        y->depression = y->facilitation + 0.01f;
+        spine_plast(y, dt);
+        float ck = y->spine_camkii, cn = y->spine_calcineurin;
+        bool inhib = false;
+        for (int j = 0; j < y->post_receptor_count; j++) {
+            uint8_t rt = y->post_receptors[j].type;
+            if (rt == T_GABAA || rt == T_GABAB || rt == T_GLYCINE) { inhib = true; break; }
+        }
+        float dw = 0.0f;
+        if (ck > 0.0f) dw += 0.0003f * ck * dt;
+        float pp1 = 0.5f;
+        for (uint8_t j = 0; j < post->modifier_count; j++)
+            if (post->modifiers[j].type == T_PP1) pp1 = post->modifiers[j].state[3];
+        if (cn > 0.0f) dw -= 0.0001f * cn * pp1 * dt;
+        if (inhib) dw = -dw;
+        y->weight += dw;
+        if (std::isnan(y->weight)) y->weight = 0.5f;
+        if (std::isnan(y->facilitation)) y->facilitation = 0.0f;
+        if (std::isnan(y->depression)) y->depression = 0.01f;
+        if (y->weight < 0.01f) y->weight = 0.01f;
+        if (y->weight > 10.0f) y->weight = 10.0f;
+        for (uint8_t j = 0; j < post->modifier_count; j++) {
+            if (post->modifiers[j].type == T_ECB) {
+                float ecb = post->modifiers[j].state[0];
+                if (ecb > 0.0f) {
+                    y->cleft_neurotransmitter[9] += ecb * 0.1f * dt;
+                    post->modifiers[j].state[0] *= 0.9f;
+                }
+                break;
+            }
+        }
+        if (y->refractory > 0) y->refractory--;
+    }
+}
+
+struct Stim {
+    CurrentStimulus* cur; uint32_t* ncur;
+    VoltageClamp* vc; uint32_t* nvc;
+    RampStimulus* rp; uint32_t* nrp;
+    IonInjection* ii; uint32_t* nii;
+    LigandInjection* li; uint32_t* nli;
+};
+
+static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint32_t nc, Synapse* sy,
+                     uint32_t ns, uint64_t* tick_count, float dt, Stim st) {
+    uint64_t tick = (*tick_count)++;
+    if (st.cur && st.ncur) {
+        uint32_t w = 0;
+        for (uint32_t r = 0; r < *st.ncur; r++)
+            if (!(tick >= st.cur[r].start_tick + (uint64_t)(st.cur[r].duration_ms / dt))) st.cur[w++] = st.cur[r];
+        *st.ncur = w;
+    }
+    if (st.vc && st.nvc) {
+        uint32_t w = 0;
+        for (uint32_t r = 0; r < *st.nvc; r++)
+            if (!(tick >= st.vc[r].end_tick)) st.vc[w++] = st.vc[r];
+        *st.nvc = w;
+    }
+    if (st.rp && st.nrp) {
+        uint32_t w = 0;
+        for (uint32_t r = 0; r < *st.nrp; r++)
+            if (!(tick >= st.rp[r].end_tick)) st.rp[w++] = st.rp[r];
+        *st.nrp = w;
+    }
+    // step 1 (simulation_host.cpp:233-307)
+    for (uint32_t i = 0; i < n; i++) {
+        Segment* s = &sg[i];
+        double itot = 0.0;
+        bool my = has_myelin(s);
+        for (uint8_t j = 0; j < s->modifier_count; j++) {
+            uint8_t t = s->modifiers[j].type;
+            if (my && t != T_MYELIN && (is_ionotropic(t) || t == T_K2P || t == T_NALCN || t == T_KV31 ||
+                                         t == T_KV11 || t == T_KV7 || t == T_KVDR || t == T_CAV21 ||
+                                         t == T_HCN1 || t == T_SK || t == T_BK))
+                continue;
+            s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
+                                                    : 2.0f * kPi * s->radius * s->length;
+            itot += dispatch(t, &s->modifiers[j], s, dt);
+        }
+        if (st.cur && st.ncur)
+            for (uint32_t k = 0; k < *st.ncur; k++) {
+                const CurrentStimulus& q = st.cur[k];
+                if (q.segment_id == s->id) {
+                    uint64_t dur = (uint64_t)(q.duration_ms / dt);
+                    if (tick >= q.start_tick && tick < q.start_tick + dur) itot += q.amplitude_pA;
+                }
+            }
+        if (st.rp && st.nrp)
+            for (uint32_t k = 0; k < *st.nrp; k++) {
+                const RampStimulus& q = st.rp[k];
+                if (q.segment_id == s->id && tick >= q.start_tick && tick < q.end_tick) {
+                    float frac = (q.end_tick > q.start_tick)
+                                     ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick)
+                                     : 1.0f;
+                    float amp = q.start_pA + (q.end_pA - q.start_pA) * frac;
+                    itot += amp;
+                }
+            }
+        if (st.ii && st.nii)
+            for (uint32_t k = 0; k < *st.nii; k++) {
+                const IonInjection& q = st.ii[k];
+                if (q.segment_id == s->id && q.ion_idx >= 0 && q.ion_idx < 7)
+                    (q.extracellular ? s->extra_ions : s->intra_ions)[q.ion_idx] += q.amount_mM;
+            }
+        if (st.li && st.nli)
+            for (uint32_t k = 0; k < *st.nli; k++) {
+                const LigandInjection& q = st.li[k];
+                if (q.segment_id == s->id && q.nt_idx >= 0 && q.nt_idx < 15) {
+                    if (q.intracellular) s->intra_nt[q.nt_idx] += q.amount;
+                    else s->neurotransmitter[q.nt_idx] += q.amount;
+                }
+            }
+        s->I_buffer = itot;
+    }
+    if (st.ii && st.nii) *st.nii = 0;
+    if (st.li && st.nli) *st.nli = 0;
+
+    diffusion(c, sg, n, cn, nc, dt);
+    synapses_step(c, sg, n, sy, ns, tick, dt);
+
+    // step 4 (simulation_host.cpp:596-649)
+    if (c.spikes.size() < n) c.spikes.resize(n, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        Segment* s = &sg[i];
+        if (std::isnan(s->I_buffer)) s->I_buffer = 0.0f;
+        if (s->I_buffer > 10000.0f) s->I_buffer = 10000.0f;
+        if (s->I_buffer < -10000.0f) s->I_buffer = -10000.0f;
+        float cm = capacitance(s);
+        float prev = s->V_m;
+        float dV = s->I_buffer / cm * dt;
+        if (dV > 50.0f) dV = 50.0f;
+        if (dV < -50.0f) dV = -50.0f;
+        s->V_m += dV;
+        if (prev <= -40.0f && s->V_m > -40.0f) c.spikes[i]++;
+        s->I_buffer = 0.0f;
+        if (std::isnan(s->position.x)) s->position.x = 0.0f;
+        if (std::isnan(s->position.y)) s->position.y = 50.0f;
+        if (std::isnan(s->position.z)) s->position.z = 0.0f;
+        if (std::isnan(s->radius)) s->radius = 5.0f;
+        if (std::isnan(s->length)) s->length = 10.0f;
+        if (st.vc && st.nvc)
+            for (uint32_t v = 0; v < *st.nvc; v++) {
+                const VoltageClamp& q = st.vc[v];
+                if (q.segment_id == s->id && tick >= q.start_tick && tick < q.end_tick) s->V_m = q.target_mV;
+            }
+        if (s->V_m != s->V_m) s->V_m = -65.0f;
+        else {
+            if (s->V_m < -120.0f) s->V_m = -120.0f;
+            if (s->V_m > 150.0f) s->V_m = 150.0f;
+        }
+    }
+}
+
+}  // namespace orc
+
+extern "C" {
+
+void* oracle_create(void) { return new orc::Ctx(); }
+void oracle_destroy(void* c) { delete static_cast<orc::Ctx*>(c); }
+
+// Same argument meaning as core_run_tick (src/simulation_host.cpp:660); the delay ring,
+// spike counters and scratch live in ctx instead of function-local statics.
+void oracle_run_tick(void* ctx, Segment* segments, uint32_t num_segments, IntraConnection* connections,
+                     uint32_t num_connections, Synapse* synapses, uint32_t num_synapses, uint64_t* tick_count,
+                     float dt, CurrentStimulus* stimuli, uint32_t* stimulus_count, VoltageClamp* vclamps,
+                     uint32_t* vclamp_count, RampStimulus* ramps, uint32_t* ramp_count, IonInjection* ion_inj,
+                     uint32_t* ion_inj_count, LigandInjection* lig_inj, uint32_t* lig_inj_count) {
+    orc::Stim st{stimuli, stimulus_count, vclamps, vclamp_count, ramps, ramp_count,
+                 ion_inj, ion_inj_count, lig_inj, lig_inj_count};
+    orc::run_tick(*static_cast<orc::Ctx*>(ctx), segments, num_segments, connections, num_connections,
+                  synapses, num_synapses, tick_count, dt, st);
+}
+
+// Spike counter of step 4 (prev <= -40 mV < V_m), as core_enable_spike_count.
+void oracle_spike_counts(void* ctx, uint32_t* out, uint32_t n) {
+    orc::Ctx* c = static_cast<orc::Ctx*>(ctx);
+    for (uint32_t i = 0; i < n; i++) out[i] = i < c->spikes.size() ? c->spikes[i] : 0;
+}
+
+// The 4 x 16-byte delay slots of every synapse (layout of DelayedRelease).
+uint64_t oracle_delay_bytes(void* ctx, uint8_t* out, uint64_t cap) {
+    orc::Ctx* c = static_cast<orc::Ctx*>(ctx);
+    uint64_t n = c->delay.size() < cap ? c->delay.size() : cap;
+    if (out && n) memcpy(out, c->delay.data(), n);
+    return c->delay.size();
+}
+
+uint64_t oracle_unsupported_count(void) { return orc::g_unsupported; }
+void oracle_reset_unsupported(void) { orc::g_unsupported = 0; }
+
+// Single-call entry for per-modifier known-answer tests: dispatches one modifier.
+double oracle_modifier_current(uint8_t type, Modifier* m, Segment* s, float dt) {
+    return orc::dispatch(type, m, s, dt);
+}
+
+float oracle_xorshift(uint32_t seed) { return orc::xorshift_draw(seed); }
+float oracle_bernoulli(float x) { return orc::bernoulli(x); }
+
+}  // extern "C"
