@@ -1,0 +1,1016 @@
+// bc_backend.hip — host side of libbiocore_hip_backend.so: BION v14 loading, the device
+// layout (segment tiles, synapse slot order, CSR lists), host<->device coherence, and the
+// C ABI of BioNeuronCore's GPU backend (src/gpu_backend_cuda.cu:316-390) plus extensions.
+#include <hip/hip_runtime.h>
+#include <dlfcn.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+#include "../../include/biocore_abi.h"
+#include "../../include/biocore_hip_backend.h"
+#include "bc_kernels.hip"
+
+namespace bc {
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "biocore_hip: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                 \
+            throw std::runtime_error("hip");                                             \
+        }                                                                                \
+    } while (0)
+
+// ------------------------------------------------------------------ BION v14 file I/O
+static void free_bion(BionData* b) {
+    delete[] b->neurons;
+    delete[] b->segments;
+    delete[] b->connections;
+    delete[] b->synapses;
+    delete[] reinterpret_cast<uint8_t*>(b->nuclei);
+    memset(b, 0, sizeof(BionData));
+}
+
+// Follows the reference's host loader (src/bion_loader_host.cpp:26-107, 358-425): same error
+// codes, same validation, same intra_connection_ids fill. Only the current v14 record sizes
+// are accepted; older versions return BION_ERR_BAD_VERSION.
+static int read_bion(const char* path, BionData* b) {
+    memset(b, 0, sizeof(BionData));
+    FILE* f = fopen(path, "rb");
+    if (!f) return BION_ERR_FILE_OPEN;
+    NetworkHeader h;
+    memset(&h, 0, sizeof(h));
+    if (fread(&h, 68, 1, f) != 1) { fclose(f); return BION_ERR_FILE_READ; }
+    if (h.version >= 5) {
+        size_t ext = h.version >= 10 ? (sizeof(NetworkHeader) - 68) : 16;
+        if (fread(reinterpret_cast<uint8_t*>(&h) + 68, ext, 1, f) != 1) { fclose(f); return BION_ERR_FILE_READ; }
+    }
+    if (h.magic != BION_MAGIC_U32) { fclose(f); return BION_ERR_BAD_MAGIC; }
+    if (h.version != BION_VERSION_CURRENT ||
+        (h.file_sz_segment && h.file_sz_segment != sizeof(Segment)) ||
+        (h.file_sz_synapse && h.file_sz_synapse != sizeof(Synapse)) ||
+        (h.file_sz_intraconnection && h.file_sz_intraconnection != sizeof(IntraConnection)) ||
+        (h.file_sz_neuron && h.file_sz_neuron != sizeof(Neuron))) {
+        fclose(f);
+        return BION_ERR_BAD_VERSION;
+    }
+    if (h.num_segments == 0) { fclose(f); return BION_ERR_ZERO_SEGS; }
+    b->header = h;
+    b->num_neurons = h.num_neurons;
+    b->num_segments = h.num_segments;
+    b->num_connections = h.num_intra_connections;
+    b->num_synapses = h.num_synapses;
+    b->neurons = new (std::nothrow) Neuron[b->num_neurons ? b->num_neurons : 1];
+    b->segments = new (std::nothrow) Segment[b->num_segments];
+    b->connections = new (std::nothrow) IntraConnection[b->num_connections ? b->num_connections : 1];
+    b->synapses = new (std::nothrow) Synapse[b->num_synapses ? b->num_synapses : 1];
+    if (!b->neurons || !b->segments || !b->connections || !b->synapses) {
+        free_bion(b); fclose(f); return BION_ERR_ALLOC;
+    }
+    auto rd = [&](void* dst, size_t sz, size_t n) { return n == 0 || fread(dst, sz, n, f) == n; };
+    if (!rd(b->neurons, sizeof(Neuron), b->num_neurons) || !rd(b->segments, sizeof(Segment), b->num_segments) ||
+        !rd(b->connections, sizeof(IntraConnection), b->num_connections) ||
+        !rd(b->synapses, sizeof(Synapse), b->num_synapses)) {
+        free_bion(b); fclose(f); return BION_ERR_FILE_READ;
+    }
+    if (h.num_nuclei > 0) {
+        size_t nsz = h.file_sz_nucleus ? h.file_sz_nucleus : sizeof(Nucleus);
+        if (nsz == sizeof(Nucleus)) {
+            uint8_t* raw = new (std::nothrow) uint8_t[(size_t)h.num_nuclei * nsz];
+            if (raw && fread(raw, nsz, h.num_nuclei, f) == h.num_nuclei) {
+                b->nuclei = reinterpret_cast<Nucleus*>(raw);
+                b->num_nuclei = h.num_nuclei;
+            } else {
+                delete[] raw;
+            }
+        }
+    }
+    fclose(f);
+    for (uint32_t i = 0; i < b->num_connections; i++)
+        if (b->connections[i].segment_a >= b->num_segments || b->connections[i].segment_b >= b->num_segments) {
+            free_bion(b); return BION_ERR_BAD_CONN;
+        }
+    for (uint32_t i = 0; i < b->num_synapses; i++) {
+        const Synapse& y = b->synapses[i];
+        if (y.pre_segment >= b->num_segments || y.post_segment >= b->num_segments) { free_bion(b); return BION_ERR_BAD_SYNAPSE; }
+        if (y.pre_autoreceptor_count > BC_MAX_PRE_AUTORECEPTORS || y.post_receptor_count > BC_MAX_POST_RECEPTORS) {
+            free_bion(b); return BION_ERR_BAD_MODCOUNT;
+        }
+        if (y.type >= 3) { free_bion(b); return BION_ERR_BAD_SYNAPSE; }
+    }
+    for (uint32_t i = 0; i < b->num_neurons; i++)
+        if ((uint64_t)b->neurons[i].segment_start + b->neurons[i].segment_count > b->num_segments) {
+            free_bion(b); return BION_ERR_BAD_NEURON;
+        }
+    for (uint32_t i = 0; i < b->num_segments; i++)
+        if (b->segments[i].modifier_count > BC_MAX_MODIFIERS) { free_bion(b); return BION_ERR_BAD_MODCOUNT; }
+    for (uint32_t i = 0; i < b->num_segments; i++) b->segments[i].intra_connection_count = 0;
+    for (uint32_t i = 0; i < b->num_connections; i++) {
+        uint32_t a = b->connections[i].segment_a, c = b->connections[i].segment_b;
+        if (b->segments[a].intra_connection_count < BC_MAX_INTRACONN_PER_SEG)
+            b->segments[a].intra_connection_ids[b->segments[a].intra_connection_count++] = i;
+        if (b->segments[c].intra_connection_count < BC_MAX_INTRACONN_PER_SEG)
+            b->segments[c].intra_connection_ids[b->segments[c].intra_connection_count++] = i;
+    }
+    return 0;
+}
+
+static int write_bion(const char* path, const BionData* b) {   // bion_save, src/bion_loader_host.cpp:449-504
+    FILE* f = fopen(path, "wb");
+    if (!f) return BION_ERR_FILE_OPEN;
+    NetworkHeader h = b->header;
+    h.magic = BION_MAGIC_U32;
+    h.version = BION_VERSION_CURRENT;
+    h.num_neurons = b->num_neurons;
+    h.num_segments = b->num_segments;
+    h.num_intra_connections = b->num_connections;
+    h.num_synapses = b->num_synapses;
+    h.num_nuclei = b->nuclei ? b->num_nuclei : 0;
+    h.nucleus_version = 1;
+    h.file_sz_segment = sizeof(Segment);
+    h.file_sz_synapse = sizeof(Synapse);
+    h.file_sz_nucleus = sizeof(Nucleus);
+    h.file_sz_intraconnection = sizeof(IntraConnection);
+    h.file_sz_neuron = sizeof(Neuron);
+    h.file_sz_modifier = sizeof(Modifier);
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+    auto wr = [&](const void* p, size_t sz, size_t n) { if (ok && n) ok = fwrite(p, sz, n, f) == n; };
+    wr(b->neurons, sizeof(Neuron), b->num_neurons);
+    wr(b->segments, sizeof(Segment), b->num_segments);
+    wr(b->connections, sizeof(IntraConnection), b->num_connections);
+    wr(b->synapses, sizeof(Synapse), b->num_synapses);
+    if (b->nuclei) wr(b->nuclei, sizeof(Nucleus), b->num_nuclei);
+    fclose(f);
+    return ok ? 0 : BION_ERR_WRITE_FAIL;
+}
+
+// ------------------------------------------------------------------ host-side static precompute
+static inline bool seg_sphere(const Segment& s) { return s.segment_type == 1 || s.segment_type == 3 || s.segment_type == 4; }
+static inline float seg_area(const Segment& s) {
+    return seg_sphere(s) ? 4.0f * kPi * s.radius * s.radius : 2.0f * kPi * s.radius * s.length;
+}
+static inline double seg_vol(const Segment& s) {
+    if (seg_sphere(s)) return (4.0 / 3.0) * kPi * (double)s.radius * (double)s.radius * (double)s.radius;
+    return kPi * (double)s.radius * (double)s.radius * (double)s.length;
+}
+static inline float seg_cap(const Segment& s) {   // run_tick_common.h:216-231
+    float c = seg_area(s) * 0.01f;
+    if (c < 1e-4f) c = 1e-4f;
+    for (uint8_t j = 0; j < s.modifier_count && j < 16; j++)
+        if (s.modifiers[j].type == 4) {
+            float gr = s.modifiers[j].state[0];
+            if (gr <= 0.0f || gr >= 1.0f) gr = 0.6f;
+            float factor = 100.0f * (1.0f / gr - 1.0f) / (1.0f / 0.6f - 1.0f);
+            c /= factor;
+            break;
+        }
+    return c;
+}
+static inline uint8_t host_nt_of_receptor(uint8_t rt) {
+    if (rt == 11 || rt == 17) return 1;
+    if (rt == 12) return 7;
+    if (rt == 14) return 3;
+    if (rt == 13) return 4;
+    if (rt == 44 || rt == 22) return 2;
+    return 0;
+}
+
+struct Sim;
+static std::mutex g_reg_mu;
+
+struct Sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    BionData host{};
+    bool loaded = false;
+    Net d{};
+    std::vector<void*> dmem;
+    std::vector<uint32_t> dev2host, host2dev, slot2syn;
+    uint32_t ntiles = 0;
+    size_t lds_bytes = 0;
+    uint64_t tick = 0;
+    uint64_t ticks_run = 0;
+    GpuStimulusData* stim_host = nullptr;
+    GpuStimulusData* stim_dev = nullptr;
+    bool stim_present = false, stim_dirty = false;
+    int coherence = BCHIP_COHERENCE_COMPAT;
+    bool host_stale = false;
+    uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
+    std::vector<uint8_t> seg_dispatched;   // [n*32] slot ran at least once (state[7] written)
+    hipEvent_t ev[8] = {};
+    float last_ms[8] = {};
+    int timing = 0;
+
+    template <class T>
+    T* dalloc(size_t n) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(n * sizeof(T), 64);
+        HIPCHK(hipMalloc(&p, bytes));
+        HIPCHK(hipMemsetAsync(p, 0, bytes, stream));
+        dmem.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* dupload(const std::vector<T>& v) {
+        T* p = dalloc<T>(v.size());
+        if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return p;
+    }
+    void free_device() {
+        for (void* p : dmem) (void)hipFree(p);
+        dmem.clear();
+        memset(&d, 0, sizeof(d));
+    }
+    ~Sim() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        free_device();
+        if (stim_dev) (void)hipFree(stim_dev);
+        delete stim_host;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        free_bion(&host);
+    }
+};
+
+// Union-find over all connections: islands of the connection graph.
+static uint32_t uf_find(std::vector<uint32_t>& p, uint32_t x) {
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+}
+
+static int build_device(Sim& S) {
+    BionData& B = S.host;
+    const uint32_t n = B.num_segments, nc = B.num_connections, ns = B.num_synapses;
+    Segment* sg = B.segments;
+    IntraConnection* cn = B.connections;
+    Synapse* sy = B.synapses;
+
+    // ---- islands, device order, tiles ----
+    std::vector<uint32_t> par(n);
+    std::iota(par.begin(), par.end(), 0u);
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t a = uf_find(par, cn[i].segment_a), b = uf_find(par, cn[i].segment_b);
+        if (a != b) par[std::max(a, b)] = std::min(a, b);
+    }
+    std::vector<uint32_t> root(n), comp_size(n, 0), comp_conns(n, 0);
+    for (uint32_t i = 0; i < n; i++) { root[i] = uf_find(par, i); comp_size[root[i]]++; }
+    for (uint32_t i = 0; i < nc; i++) comp_conns[root[cn[i].segment_a]]++;
+    // components ordered by their smallest segment index (= root after min-union)
+    std::vector<uint32_t> comp_start(n, 0);
+    S.dev2host.assign(n, 0);
+    S.host2dev.assign(n, 0);
+    {
+        uint32_t pos = 0;
+        for (uint32_t i = 0; i < n; i++)
+            if (root[i] == i) { comp_start[i] = pos; pos += comp_size[i]; }
+        std::vector<uint32_t> fill(n, 0);
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t r = root[i];
+            uint32_t dpos = comp_start[r] + fill[r]++;
+            S.dev2host[dpos] = i;
+            S.host2dev[i] = dpos;
+        }
+    }
+    std::vector<uint32_t> tile_seg0, tile_nseg, tile_conn0, tile_nconn, seg_tile(n);
+    {
+        uint32_t cur_s = 0, cur_c = 0, start = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (root[i] != i) continue;
+            uint32_t cs = comp_size[i], cc = comp_conns[i];
+            if (cs > (uint32_t)kTileSeg || cc > (uint32_t)kTileConn) return BCHIP_ERR_ISLAND_TOO_LARGE;
+            if (cur_s + cs > (uint32_t)kTileSeg || cur_c + cc > (uint32_t)kTileConn) {
+                tile_seg0.push_back(start); tile_nseg.push_back(cur_s); tile_nconn.push_back(cur_c);
+                start += cur_s; cur_s = 0; cur_c = 0;
+            }
+            cur_s += cs; cur_c += cc;
+        }
+        if (cur_s) { tile_seg0.push_back(start); tile_nseg.push_back(cur_s); tile_nconn.push_back(cur_c); }
+    }
+    S.ntiles = (uint32_t)tile_seg0.size();
+    for (uint32_t t = 0; t < S.ntiles; t++)
+        for (uint32_t k = 0; k < tile_nseg[t]; k++) seg_tile[tile_seg0[t] + k] = t;
+    tile_conn0.resize(S.ntiles);
+    {
+        uint32_t acc = 0;
+        for (uint32_t t = 0; t < S.ntiles; t++) { tile_conn0[t] = acc; acc += tile_nconn[t]; }
+    }
+    // connections placed per tile in global index order
+    std::vector<uint32_t> conn_ab(nc), conn_local(nc);
+    std::vector<uint8_t> conn_intra(nc);
+    {
+        std::vector<uint32_t> fill(S.ntiles, 0);
+        for (uint32_t i = 0; i < nc; i++) {
+            uint32_t da = S.host2dev[cn[i].segment_a], db = S.host2dev[cn[i].segment_b];
+            uint32_t t = seg_tile[da];
+            uint32_t loc = fill[t]++;
+            conn_local[i] = loc;
+            uint32_t pos = tile_conn0[t] + loc;
+            conn_ab[pos] = (da - tile_seg0[t]) | ((db - tile_seg0[t]) << 16);
+            conn_intra[pos] = cn[i].is_intracellular ? 1 : 0;
+        }
+    }
+    // per-segment incidence lists in global connection order (role a before role b)
+    std::vector<uint32_t> csr_off(n + 1, 0), csr_ent;
+    {
+        std::vector<uint32_t> deg(n, 0);
+        for (uint32_t i = 0; i < nc; i++) { deg[S.host2dev[cn[i].segment_a]]++; deg[S.host2dev[cn[i].segment_b]]++; }
+        for (uint32_t i = 0; i < n; i++) csr_off[i + 1] = csr_off[i] + deg[i];
+        csr_ent.resize(csr_off[n]);
+        std::vector<uint32_t> fill(n, 0);
+        for (uint32_t i = 0; i < nc; i++) {
+            uint32_t da = S.host2dev[cn[i].segment_a], db = S.host2dev[cn[i].segment_b];
+            csr_ent[csr_off[da] + fill[da]++] = (conn_local[i] << 1) | 0u;
+            csr_ent[csr_off[db] + fill[db]++] = (conn_local[i] << 1) | 1u;
+        }
+    }
+
+    // ---- segment SoA ----
+    const uint32_t npad = (n + 63) & ~63u;
+    uint32_t nslot = 1;
+    for (uint32_t i = 0; i < n; i++) nslot = std::max<uint32_t>(nslot, sg[i].modifier_count);
+    Net& d = S.d;
+    d.n = n; d.npad = npad; d.nslot = nslot; d.ns = ns; d.ntiles = S.ntiles; d.has_conn = nc > 0;
+    std::vector<uint32_t> seg_id(npad, 0);
+    std::vector<float> radius(npad, 0), length(npad, 0), area(npad, 0), cap(npad, 1), pkc(npad, 0), pka(npad, 0),
+        camp(npad, 0.1f), pp1(npad, 0.5f), camkii(npad, 0), calcn(npad, 0);
+    std::vector<double> vol(npad, 0);
+    std::vector<uint8_t> stype(npad, 0), sflags(npad, 0), mcount(npad, 0);
+    std::vector<int8_t> cav21(npad, -1);
+    std::vector<uint8_t> mtype((size_t)nslot * npad, 0), mav((size_t)nslot * npad, 0);
+    std::vector<double> mdens((size_t)nslot * npad, 0.0);
+    S.unsupported_mod = 0;
+    for (uint32_t di = 0; di < n; di++) {
+        const Segment& s = sg[S.dev2host[di]];
+        seg_id[di] = s.id;
+        radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type;
+        area[di] = seg_area(s); vol[di] = seg_vol(s); cap[di] = seg_cap(s);
+        mcount[di] = s.modifier_count;
+        bool seen_pkc = false, seen_pka = false;
+        uint8_t fl = 0;
+        for (uint32_t j = 0; j < s.modifier_count; j++) {
+            const Modifier& m = s.modifiers[j];
+            mtype[(size_t)j * npad + di] = m.type;
+            mav[(size_t)j * npad + di] = m.active_vars;
+            mdens[(size_t)j * npad + di] = m.density;
+            if (m.type == 4) fl |= SF_MYELIN;
+            if (m.type == 9) fl |= SF_BATH_NMDA;
+            if (m.type == 43) fl |= SF_BATH_5HT;
+            if (m.type == 137 && !seen_pkc) { pkc[di] = m.state[0]; seen_pkc = true; }
+            if (m.type == 135 && !seen_pka) { pka[di] = m.state[1]; camp[di] = m.state[0]; seen_pka = true; }
+            if (m.type == 113) pp1[di] = m.state[3];
+            if (m.type == 82) camkii[di] = m.state[0];
+            if (m.type == 83) calcn[di] = m.state[0];
+            if (m.type == M_CAV21) cav21[di] = (int8_t)j;
+            if (!modifier_supported(m.type)) S.unsupported_mod++;
+        }
+        sflags[di] = fl;
+    }
+    // ---- synapses, slot order ----
+    uint32_t nrec = 1;
+    for (uint32_t i = 0; i < ns; i++) nrec = std::max<uint32_t>(nrec, sy[i].post_receptor_count);
+    d.nrec = nrec;
+    S.slot2syn.resize(ns);
+    std::iota(S.slot2syn.begin(), S.slot2syn.end(), 0u);
+    std::stable_sort(S.slot2syn.begin(), S.slot2syn.end(), [&](uint32_t a, uint32_t b) {
+        return S.host2dev[sy[a].post_segment] < S.host2dev[sy[b].post_segment];
+    });
+    std::vector<uint32_t> syn2slot(ns);
+    for (uint32_t k = 0; k < ns; k++) syn2slot[S.slot2syn[k]] = k;
+    std::vector<uint32_t> post_off(n + 1, 0);
+    for (uint32_t i = 0; i < ns; i++) post_off[S.host2dev[sy[i].post_segment] + 1]++;
+    for (uint32_t i = 0; i < n; i++) post_off[i + 1] += post_off[i];
+    std::vector<int32_t> src(ns, -1);
+    {
+        std::vector<int64_t> last_onto(n, -1);
+        for (uint32_t i = 0; i < ns; i++) {
+            int64_t j = last_onto[sy[i].pre_segment];
+            src[syn2slot[i]] = j >= 0 ? (int32_t)syn2slot[j] : -1;
+            last_onto[sy[i].post_segment] = i;
+        }
+    }
+    std::vector<uint32_t> y_pre(ns), y_id(ns);
+    std::vector<uint8_t> y_flags(ns, 0), y_rcount(ns, 0), y_refr(ns, 0);
+    std::vector<float> y_delay(ns), y_pa(ns), y_qa(ns), y_prel(ns), y_amt(ns), y_tau(ns), y_rmax(ns), y_rrec(ns);
+    std::vector<float> y_w(ns), y_fac(ns), y_dep(ns), y_rrp(ns), y_bca(ns), y_sca(ns), y_ck(ns), y_cn(ns);
+    std::vector<uint16_t> y_mask(ns, 0);
+    std::vector<uint8_t> y_rtype((size_t)nrec * ns, 0), y_rav((size_t)nrec * ns, 0);
+    std::vector<double> y_rdens((size_t)nrec * ns, 0.0), y_rstate((size_t)nrec * 7 * ns, 0.0);
+    std::vector<float> y_cleft((size_t)15 * ns, 0.0f);
+    S.unsupported_syn = 0;
+    S.hazard_syn = 0;
+    for (uint32_t k = 0; k < ns; k++) {
+        const Synapse& y = sy[S.slot2syn[k]];
+        y_pre[k] = S.host2dev[y.pre_segment];
+        sflags[y_pre[k]] |= SF_WATCHED;
+        y_id[k] = y.id;
+        uint8_t fl = 0;
+        if (y.type == 1 || y.type == 2) { fl |= YF_SKIP; S.unsupported_syn++; }
+        if (y.pre_autoreceptor_count > 0) S.unsupported_syn++;
+        for (uint32_t j = 0; j < y.post_receptor_count; j++) {
+            uint8_t rt = y.post_receptors[j].type;
+            if (rt == 11 || rt == 17 || rt == 12) fl |= YF_INHIBITORY;
+            y_rtype[(size_t)j * ns + k] = rt;
+            y_rav[(size_t)j * ns + k] = y.post_receptors[j].active_vars;
+            y_rdens[(size_t)j * ns + k] = y.post_receptors[j].density;
+            for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
+            if (!modifier_supported(rt)) S.unsupported_mod++;
+        }
+        y_flags[k] = fl;
+        y_rcount[k] = y.post_receptor_count;
+        y_delay[k] = y.delay; y_pa[k] = y.pre_area; y_qa[k] = y.post_area; y_prel[k] = y.release_probability;
+        y_amt[k] = y.release_amount; y_tau[k] = y.tau_clearance; y_rmax[k] = y.RRP_max; y_rrec[k] = y.RRP_recovery;
+        y_w[k] = y.weight; y_fac[k] = y.facilitation; y_dep[k] = y.depression; y_refr[k] = y.refractory;
+        y_rrp[k] = y.RRP; y_bca[k] = y.bouton_Ca; y_sca[k] = y.spine_Ca; y_ck[k] = y.spine_camkii;
+        y_cn[k] = y.spine_calcineurin;
+        uint16_t mask = 0;
+        for (int q = 0; q < 15; q++) {
+            y_cleft[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
+            uint32_t bits;
+            memcpy(&bits, &y.cleft_neurotransmitter[q], 4);
+            if (bits) mask |= (1u << q);
+        }
+        mask |= 1u << (y.post_receptor_count > 0 ? host_nt_of_receptor(y.post_receptors[0].type) : 0);
+        const Segment& post = sg[y.post_segment];
+        for (uint32_t j = 0; j < post.modifier_count; j++)
+            if (post.modifiers[j].type == 86) { mask |= 1u << 9; S.unsupported_syn++; break; }
+        y_mask[k] = mask;
+        // Immediate releases read the presynaptic Ca before the post-side pass; exact only
+        // when no earlier synapse targets that presynaptic segment.
+        if (src[k] >= 0 && y.delay <= 0.0f && y.RRP_max > 0.0f) S.hazard_syn++;
+    }
+    for (uint32_t i = 0; i < n; i++) {   // glial GLAST capture is not covered
+        const Segment& s = sg[i];
+        for (uint32_t j = 0; j < s.modifier_count; j++)
+            if (s.modifiers[j].type == 147) { S.unsupported_mod++; break; }
+    }
+
+    // transmitter diffusion coefficients (types.h:436-443), host powf as the reference
+    static const float kMass[15] = {147.0f, 103.0f, 153.0f, 176.0f, 146.0f, 267.0f, 169.0f, 75.0f,
+                                    111.0f, 348.0f, 507.0f, 232.0f, 24.0f, 162.0f, 0.0f};
+    for (int k = 0; k < 15; k++) {
+        float mass = kMass[k];
+        if (mass < 1.0f) mass = 1.0f;
+        d.nt_diff[k] = 1.33f * powf(23.0f / mass, 1.0f / 3.0f);
+    }
+
+    // ---- device allocation + upload ----
+    d.V = S.dalloc<double>(npad);
+    d.in = S.dalloc<double>((size_t)7 * npad);
+    d.ex = S.dalloc<double>((size_t)7 * npad);
+    d.atp = S.dalloc<float>(npad);
+    d.temp = S.dalloc<float>(npad);
+    d.txn = S.dalloc<float>(npad);
+    d.nt = S.dalloc<float>((size_t)15 * npad);
+    d.nt_in = S.dalloc<float>((size_t)15 * npad);
+    d.ibuf = S.dalloc<double>(npad);
+    d.ca_snap = S.dalloc<double>(npad);
+    d.spikes = S.dalloc<uint32_t>(npad);
+    d.seg_id = S.dupload(seg_id);
+    d.radius = S.dupload(radius);
+    d.length = S.dupload(length);
+    d.stype = S.dupload(stype);
+    d.area = S.dupload(area);
+    d.vol = S.dupload(vol);
+    d.cap = S.dupload(cap);
+    d.pkc = S.dupload(pkc);
+    d.pka = S.dupload(pka);
+    d.camp = S.dupload(camp);
+    d.pp1 = S.dupload(pp1);
+    d.camkii = S.dupload(camkii);
+    d.calcn = S.dupload(calcn);
+    d.sflags = S.dupload(sflags);
+    d.cav21_slot = S.dupload(cav21);
+    d.mcount = S.dupload(mcount);
+    d.mtype = S.dupload(mtype);
+    d.mav = S.dupload(mav);
+    d.mdens = S.dupload(mdens);
+    d.mstate = S.dalloc<double>((size_t)nslot * 7 * npad);
+    d.tile_seg0 = S.dupload(tile_seg0);
+    d.tile_nseg = S.dupload(tile_nseg);
+    d.tile_conn0 = S.dupload(tile_conn0);
+    d.tile_nconn = S.dupload(tile_nconn);
+    d.conn_ab = S.dupload(conn_ab);
+    d.conn_intra = S.dupload(conn_intra);
+    d.csr_off = S.dupload(csr_off);
+    d.csr_ent = S.dupload(csr_ent);
+    d.y_pre = S.dupload(y_pre);
+    d.y_id = S.dupload(y_id);
+    d.y_flags = S.dupload(y_flags);
+    d.y_delay = S.dupload(y_delay);
+    d.y_pre_area = S.dupload(y_pa);
+    d.y_post_area = S.dupload(y_qa);
+    d.y_prel = S.dupload(y_prel);
+    d.y_amount = S.dupload(y_amt);
+    d.y_tau = S.dupload(y_tau);
+    d.y_rrp_max = S.dupload(y_rmax);
+    d.y_rrp_rec = S.dupload(y_rrec);
+    d.y_cleft_mask = S.dupload(y_mask);
+    d.y_src = S.dupload(src);
+    d.post_off = S.dupload(post_off);
+    d.y_rcount = S.dupload(y_rcount);
+    d.y_rtype = S.dupload(y_rtype);
+    d.y_rav = S.dupload(y_rav);
+    d.y_rdens = S.dupload(y_rdens);
+    d.y_rstate = S.dupload(y_rstate);
+    d.y_cleft = S.dupload(y_cleft);
+    d.y_weight = S.dupload(y_w);
+    d.y_fac = S.dupload(y_fac);
+    d.y_dep = S.dupload(y_dep);
+    d.y_refr = S.dupload(y_refr);
+    d.y_rrp = S.dupload(y_rrp);
+    d.y_bca = S.dupload(y_bca);
+    d.y_sca = S.dupload(y_sca);
+    d.y_camkii = S.dupload(y_ck);
+    d.y_calcn = S.dupload(y_cn);
+    d.y_relp = S.dalloc<float>(ns);
+    d.y_ca_after = S.dalloc<double>(ns);
+    d.d_mask = S.dalloc<uint8_t>(ns);
+    d.d_arr = S.dalloc<uint64_t>((size_t)4 * ns);
+    d.d_amt = S.dalloc<float>((size_t)4 * ns);
+    d.d_nt = S.dalloc<uint8_t>((size_t)4 * ns);
+    S.lds_bytes = sizeof(DiffLds);
+    S.seg_dispatched.assign((size_t)n * 32, 0);
+    return 0;
+}
+
+// ------------------------------------------------------------------ coherence
+// Transfers the dynamic state between the host AoS and the device SoA (both directions use
+// the same field list, so what the device computes is exactly what callers can read/write).
+static void xfer(Sim& S, bool to_device) {
+    Net& d = S.d;
+    BionData& B = S.host;
+    const uint32_t n = d.n, npad = d.npad, ns = d.ns, nslot = d.nslot, nrec = d.nrec;
+    auto cp = [&](void* dev, void* hostbuf, size_t bytes) {
+        if (!bytes) return;
+        if (to_device) HIPCHK(hipMemcpyAsync(dev, hostbuf, bytes, hipMemcpyHostToDevice, S.stream));
+        else HIPCHK(hipMemcpyAsync(hostbuf, dev, bytes, hipMemcpyDeviceToHost, S.stream));
+    };
+    std::vector<double> V(npad), in((size_t)7 * npad), ex((size_t)7 * npad), ms((size_t)nslot * 7 * npad);
+    std::vector<float> atp(npad), temp(npad), txn(npad), nt((size_t)15 * npad), nti((size_t)15 * npad);
+    std::vector<float> yw(ns), yf(ns), yd(ns), yr(ns), yb(ns), ys(ns), yk(ns), yc(ns), ycl((size_t)15 * ns);
+    std::vector<uint8_t> yrf(ns);
+    std::vector<double> yrs((size_t)nrec * 7 * ns);
+    if (to_device) {
+        for (uint32_t di = 0; di < n; di++) {
+            const Segment& s = B.segments[S.dev2host[di]];
+            V[di] = s.V_m;
+            for (int k = 0; k < 7; k++) { in[(size_t)k * npad + di] = s.intra_ions[k]; ex[(size_t)k * npad + di] = s.extra_ions[k]; }
+            atp[di] = s.ATP_level; temp[di] = s.temperature; txn[di] = s.transcription_signal;
+            for (int k = 0; k < 15; k++) { nt[(size_t)k * npad + di] = s.neurotransmitter[k]; nti[(size_t)k * npad + di] = s.intra_nt[k]; }
+            for (uint32_t j = 0; j < s.modifier_count; j++)
+                for (int q = 0; q < 7; q++) ms[((size_t)j * 7 + q) * npad + di] = s.modifiers[j].state[q];
+        }
+        for (uint32_t k = 0; k < ns; k++) {
+            const Synapse& y = B.synapses[S.slot2syn[k]];
+            yw[k] = y.weight; yf[k] = y.facilitation; yd[k] = y.depression; yr[k] = y.RRP; yb[k] = y.bouton_Ca;
+            ys[k] = y.spine_Ca; yk[k] = y.spine_camkii; yc[k] = y.spine_calcineurin; yrf[k] = y.refractory;
+            for (int q = 0; q < 15; q++) ycl[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
+            for (uint32_t j = 0; j < y.post_receptor_count; j++)
+                for (int q = 0; q < 7; q++) yrs[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
+        }
+    }
+    cp(d.V, V.data(), (size_t)npad * 8);
+    cp(d.in, in.data(), in.size() * 8);
+    cp(d.ex, ex.data(), ex.size() * 8);
+    cp(d.mstate, ms.data(), ms.size() * 8);
+    cp(d.atp, atp.data(), npad * 4);
+    cp(d.temp, temp.data(), npad * 4);
+    cp(d.txn, txn.data(), npad * 4);
+    cp(d.nt, nt.data(), nt.size() * 4);
+    cp(d.nt_in, nti.data(), nti.size() * 4);
+    if (ns) {
+        cp(d.y_weight, yw.data(), ns * 4);
+        cp(d.y_fac, yf.data(), ns * 4);
+        cp(d.y_dep, yd.data(), ns * 4);
+        cp(d.y_rrp, yr.data(), ns * 4);
+        cp(d.y_bca, yb.data(), ns * 4);
+        cp(d.y_sca, ys.data(), ns * 4);
+        cp(d.y_camkii, yk.data(), ns * 4);
+        cp(d.y_calcn, yc.data(), ns * 4);
+        cp(d.y_refr, yrf.data(), ns);
+        cp(d.y_cleft, ycl.data(), ycl.size() * 4);
+        cp(d.y_rstate, yrs.data(), yrs.size() * 8);
+    }
+    HIPCHK(hipStreamSynchronize(S.stream));
+    if (to_device) return;
+    const bool ran = S.ticks_run > 0;
+    for (uint32_t di = 0; di < n; di++) {
+        Segment& s = B.segments[S.dev2host[di]];
+        s.V_m = V[di];
+        for (int k = 0; k < 7; k++) { s.intra_ions[k] = in[(size_t)k * npad + di]; s.extra_ions[k] = ex[(size_t)k * npad + di]; }
+        s.ATP_level = atp[di]; s.temperature = temp[di]; s.transcription_signal = txn[di];
+        for (int k = 0; k < 15; k++) { s.neurotransmitter[k] = nt[(size_t)k * npad + di]; s.intra_nt[k] = nti[(size_t)k * npad + di]; }
+        if (ran) s.I_buffer = 0.0;
+        bool my = false;
+        for (uint32_t j = 0; j < s.modifier_count; j++) my |= s.modifiers[j].type == 4;
+        for (uint32_t j = 0; j < s.modifier_count; j++) {
+            Modifier& m = s.modifiers[j];
+            for (int q = 0; q < 7; q++) m.state[q] = ms[((size_t)j * 7 + q) * npad + di];
+            uint8_t t = m.type;
+            bool blocked = my && t != 4 &&
+                           ((t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194) || t == M_K2P ||
+                            t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 || t == M_KVDR ||
+                            t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK);
+            if (ran && !blocked) m.state[7] = seg_area(s);
+        }
+    }
+    for (uint32_t k = 0; k < ns; k++) {
+        Synapse& y = B.synapses[S.slot2syn[k]];
+        y.weight = yw[k]; y.facilitation = yf[k]; y.depression = yd[k]; y.RRP = yr[k]; y.bouton_Ca = yb[k];
+        y.spine_Ca = ys[k]; y.spine_camkii = yk[k]; y.spine_calcineurin = yc[k]; y.refractory = yrf[k];
+        for (int q = 0; q < 15; q++) y.cleft_neurotransmitter[q] = ycl[(size_t)q * ns + k];
+        const Segment& post = B.segments[y.post_segment];
+        bool my = false;
+        for (uint32_t j = 0; j < post.modifier_count; j++) my |= post.modifiers[j].type == 4;
+        for (uint32_t j = 0; j < y.post_receptor_count; j++) {
+            Modifier& m = y.post_receptors[j];
+            for (int q = 0; q < 7; q++) m.state[q] = yrs[((size_t)j * 7 + q) * ns + k];
+            uint8_t t = m.type;
+            bool blocked = my && (t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194);
+            if (ran && !(y.type == 1 || y.type == 2) && !blocked) m.state[7] = y.post_area;
+        }
+    }
+}
+
+static void sync_host(Sim& S) {
+    if (S.host_stale) { xfer(S, false); S.host_stale = false; }
+}
+
+// ------------------------------------------------------------------ tick
+static void run_tick(Sim& S, float dt) {
+    Net& d = S.d;
+    if (S.coherence == BCHIP_COHERENCE_COMPAT && !S.host_stale) xfer(S, true);
+    if (S.coherence == BCHIP_COHERENCE_COMPAT && S.host_stale) { xfer(S, false); S.host_stale = false; xfer(S, true); }
+    const uint64_t tick = S.tick++;
+    if (S.stim_dirty) {
+        HIPCHK(hipMemcpyAsync(S.stim_dev, S.stim_host, sizeof(GpuStimulusData), hipMemcpyHostToDevice, S.stream));
+        S.stim_dirty = false;
+    }
+    const GpuStimulusData* st = S.stim_present ? S.stim_dev : nullptr;
+    const uint32_t gs = (d.n + 255) / 256, gy = (d.ns + 255) / 256;
+    const bool tm = S.timing != 0;
+    if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
+    k_modifiers<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
+    if (d.has_conn) k_diffusion<<<S.ntiles, 256, S.lds_bytes, S.stream>>>(d, dt);
+    if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
+    if (d.ns) {
+        HIPCHK(hipMemcpyAsync(d.ca_snap, d.in + 2 * (size_t)d.npad, (size_t)d.n * 8, hipMemcpyDeviceToDevice, S.stream));
+        k_syn_pre<<<gy, 256, 0, S.stream>>>(d, tick, dt);
+    }
+    if (tm) HIPCHK(hipEventRecord(S.ev[3], S.stream));
+    k_syn_post<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    if (tm) HIPCHK(hipEventRecord(S.ev[4], S.stream));
+    if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, tick, dt);
+    if (tm) HIPCHK(hipEventRecord(S.ev[5], S.stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(S.stream));
+    if (tm)
+        for (int k = 0; k < 5; k++) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, S.ev[k], S.ev[k + 1]));
+            S.last_ms[k] = ms;
+        }
+    S.ticks_run++;
+    if (S.stim_present && (S.stim_host->ion_inj_count || S.stim_host->lig_inj_count)) {
+        S.stim_host->ion_inj_count = 0;   // one-shot (simulation_host.cpp:304-306)
+        S.stim_host->lig_inj_count = 0;
+        S.stim_dirty = true;
+    }
+    S.host_stale = true;
+    if (S.coherence == BCHIP_COHERENCE_COMPAT) sync_host(S);
+}
+
+static int load_into(Sim& S, BionData* src_owned) {
+    HIPCHK(hipSetDevice(S.device));
+    HIPCHK(hipStreamSynchronize(S.stream));
+    S.free_device();
+    free_bion(&S.host);
+    S.host = *src_owned;
+    memset(src_owned, 0, sizeof(BionData));
+    S.tick = 0;
+    S.ticks_run = 0;
+    S.host_stale = false;
+    int rc = build_device(S);
+    if (rc != 0) { S.free_device(); free_bion(&S.host); S.loaded = false; return rc; }
+    xfer(S, true);
+    S.loaded = true;
+    return 0;
+}
+
+}  // namespace bc
+
+using bc::Sim;
+
+// ================================================================== C ABI
+static std::mutex g_plugin_mu;
+static ModifierFunc g_mod_table[128];
+static ReleaseModulatorFunc g_rel_table[128];
+
+extern "C" {
+
+int gpu_backend_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void* gpu_backend_create_on_device(int device_index) {
+    int ndev = gpu_backend_device_count();
+    if (ndev == 0) return nullptr;
+    if (device_index < 0 || device_index >= ndev) device_index = 0;
+    try {
+        Sim* s = new Sim();
+        s->device = device_index;
+        HIPCHK(hipSetDevice(device_index));
+        HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        s->stim_host = new GpuStimulusData();
+        memset(s->stim_host, 0, sizeof(GpuStimulusData));
+        HIPCHK(hipMalloc(&s->stim_dev, sizeof(GpuStimulusData)));
+        return s;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void* gpu_backend_create(void) { return gpu_backend_create_on_device(0); }
+
+void gpu_backend_destroy(void* sim) {
+    if (sim) delete static_cast<Sim*>(sim);
+}
+
+int gpu_backend_load_bion(void* sim, const char* path) {
+    if (!sim || !path) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    BionData b;
+    int rc = bc::read_bion(path, &b);
+    if (rc != 0) return rc;
+    try {
+        return bc::load_into(S, &b);
+    } catch (...) {
+        bc::free_bion(&b);
+        return BION_ERR_ALLOC;
+    }
+}
+
+void gpu_backend_run_tick(void* sim, float dt) {
+    if (!sim) return;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return;
+    try {
+        (void)hipSetDevice(S.device);
+        bc::run_tick(S, dt);
+    } catch (...) {
+    }
+}
+
+void* gpu_backend_bion_data(void* sim) {
+    if (!sim) return nullptr;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return nullptr;
+    try {
+        (void)hipSetDevice(S.device);
+        bc::sync_host(S);
+    } catch (...) {
+    }
+    return &S.host;
+}
+
+void gpu_backend_upload_stimuli(void* sim, const GpuStimulusData* src) {
+    if (!sim || !src) return;
+    Sim& S = *static_cast<Sim*>(sim);
+    memcpy(S.stim_host, src, sizeof(GpuStimulusData));
+    auto cap = [](uint32_t& c) { if (c > BC_GPU_MAX_STIMULI) c = BC_GPU_MAX_STIMULI; };
+    cap(S.stim_host->current_stim_count); cap(S.stim_host->vclamp_count); cap(S.stim_host->ramp_count);
+    cap(S.stim_host->ion_inj_count); cap(S.stim_host->lig_inj_count);
+    S.stim_present = true;
+    S.stim_dirty = true;
+}
+
+int gpu_backend_get_device_info(int index, GpuDeviceInfo* info) {
+    if (!info) return -1;
+    int ndev = gpu_backend_device_count();
+    if (index < 0 || index >= ndev) return -1;
+    memset(info, 0, sizeof(GpuDeviceInfo));
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, index) != hipSuccess) return -1;
+    strncpy(info->name, p.name, 127);
+    info->vendor_id = 0x1002;
+    info->is_cpu = 0;
+    info->global_mem_bytes = p.totalGlobalMem;
+    info->max_compute_units = p.multiProcessorCount;
+    info->device_index = index;
+    strncpy(info->backend_type, "hip", 15);
+    return 0;
+}
+
+int gpu_backend_ibuf_offset(void) { return (int)offsetof(Segment, I_buffer); }
+int gpu_backend_vm_offset(void) { return (int)offsetof(Segment, V_m); }
+int gpu_backend_seg_size(void) { return (int)sizeof(Segment); }
+
+// ---- plugin registration (src/plugin_manager.cpp:37-75) ----
+void register_modifier_current(uint8_t type, ModifierFunc func) {
+    std::lock_guard<std::mutex> lk(g_plugin_mu);
+    if (type < 128) g_mod_table[type] = func;
+}
+void register_modulator_release(uint8_t type, ReleaseModulatorFunc func) {
+    std::lock_guard<std::mutex> lk(g_plugin_mu);
+    if (type < 128) g_rel_table[type] = func;
+}
+
+// ---- extensions ----
+int bchip_load_bion_data(void* sim, const BionData* src) {
+    if (!sim || !src || !src->segments || src->num_segments == 0) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    BionData b;
+    memset(&b, 0, sizeof(b));
+    b.header = src->header;
+    b.num_neurons = src->num_neurons;
+    b.num_segments = src->num_segments;
+    b.num_connections = src->num_connections;
+    b.num_synapses = src->num_synapses;
+    try {
+        b.neurons = new Neuron[b.num_neurons ? b.num_neurons : 1];
+        b.segments = new Segment[b.num_segments];
+        b.connections = new IntraConnection[b.num_connections ? b.num_connections : 1];
+        b.synapses = new Synapse[b.num_synapses ? b.num_synapses : 1];
+    } catch (...) {
+        bc::free_bion(&b);
+        return BION_ERR_ALLOC;
+    }
+    if (b.num_neurons) memcpy(b.neurons, src->neurons, sizeof(Neuron) * b.num_neurons);
+    memcpy(b.segments, src->segments, sizeof(Segment) * b.num_segments);
+    if (b.num_connections) memcpy(b.connections, src->connections, sizeof(IntraConnection) * b.num_connections);
+    if (b.num_synapses) memcpy(b.synapses, src->synapses, sizeof(Synapse) * b.num_synapses);
+    for (uint32_t i = 0; i < b.num_connections; i++)
+        if (b.connections[i].segment_a >= b.num_segments || b.connections[i].segment_b >= b.num_segments) {
+            bc::free_bion(&b); return BION_ERR_BAD_CONN;
+        }
+    for (uint32_t i = 0; i < b.num_synapses; i++)
+        if (b.synapses[i].pre_segment >= b.num_segments || b.synapses[i].post_segment >= b.num_segments ||
+            b.synapses[i].type >= 3 || b.synapses[i].post_receptor_count > BC_MAX_POST_RECEPTORS ||
+            b.synapses[i].pre_autoreceptor_count > BC_MAX_PRE_AUTORECEPTORS) {
+            bc::free_bion(&b); return BION_ERR_BAD_SYNAPSE;
+        }
+    for (uint32_t i = 0; i < b.num_segments; i++)
+        if (b.segments[i].modifier_count > BC_MAX_MODIFIERS) { bc::free_bion(&b); return BION_ERR_BAD_MODCOUNT; }
+    try {
+        return bc::load_into(S, &b);
+    } catch (...) {
+        bc::free_bion(&b);
+        return BION_ERR_ALLOC;
+    }
+}
+
+int bchip_save_bion(void* sim, const char* path) {
+    if (!sim || !path) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return -1;
+    try { bc::sync_host(S); } catch (...) { return BION_ERR_WRITE_FAIL; }
+    return bc::write_bion(path, &S.host);
+}
+
+int bchip_set_coherence(void* sim, int mode) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (mode != BCHIP_COHERENCE_COMPAT && mode != BCHIP_COHERENCE_FAST) return -1;
+    try { bc::sync_host(S); } catch (...) { return -1; }
+    S.coherence = mode;
+    return 0;
+}
+
+int bchip_push_host_state(void* sim) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return -1;
+    try {
+        (void)hipSetDevice(S.device);
+        bc::sync_host(S);
+        bc::xfer(S, true);
+    } catch (...) { return -1; }
+    return 0;
+}
+
+int bchip_run_ticks(void* sim, float dt, uint32_t count) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return -1;
+    try {
+        (void)hipSetDevice(S.device);
+        for (uint32_t k = 0; k < count; k++) bc::run_tick(S, dt);
+    } catch (...) { return -1; }
+    return 0;
+}
+
+uint64_t bchip_tick_count(void* sim) { return sim ? static_cast<Sim*>(sim)->tick : 0; }
+
+int bchip_spike_counts(void* sim, uint32_t* out, uint32_t n) {
+    if (!sim || !out) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded || n > S.d.n) return -1;
+    std::vector<uint32_t> dev(S.d.n);
+    if (hipMemcpy(dev.data(), S.d.spikes, (size_t)S.d.n * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    for (uint32_t h = 0; h < n; h++) out[h] = dev[S.host2dev[h]];
+    return 0;
+}
+
+int bchip_delay_slots(void* sim, uint8_t* out, uint64_t cap) {
+    // DelayedRelease layout (delay_buffer.h:15-26): u64 arrival, f32 amount, u8 nt, pad[3];
+    // 4 slots per synapse in original synapse order.
+    if (!sim || !out) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    const uint32_t ns = S.d.ns;
+    if (cap < (uint64_t)ns * 64) return -1;
+    std::vector<uint64_t> arr((size_t)4 * ns);
+    std::vector<float> amt((size_t)4 * ns);
+    std::vector<uint8_t> nt((size_t)4 * ns), mask(ns);
+    if (ns) {
+        (void)hipMemcpy(arr.data(), S.d.d_arr, arr.size() * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(amt.data(), S.d.d_amt, amt.size() * 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(nt.data(), S.d.d_nt, nt.size(), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(mask.data(), S.d.d_mask, ns, hipMemcpyDeviceToHost);
+    }
+    memset(out, 0, (size_t)ns * 64);
+    for (uint32_t k = 0; k < ns; k++) {
+        uint8_t* rec = out + (size_t)S.slot2syn[k] * 64;
+        for (int s = 0; s < 4; s++) {
+            uint8_t* e = rec + s * 16;
+            uint64_t a = (mask[k] & (1u << s)) ? arr[(size_t)s * ns + k] : 0;
+            memcpy(e, &a, 8);
+            memcpy(e + 8, &amt[(size_t)s * ns + k], 4);
+            e[12] = nt[(size_t)s * ns + k];
+        }
+    }
+    return 0;
+}
+
+int bchip_stats(void* sim, BchipStats* out) {
+    if (!sim || !out) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    memset(out, 0, sizeof(*out));
+    out->num_segments = S.d.n;
+    out->num_connections = S.host.num_connections;
+    out->num_synapses = S.d.ns;
+    out->num_tiles = S.ntiles;
+    out->modifier_slots = S.d.nslot;
+    out->receptor_slots = S.d.nrec;
+    out->unsupported_modifiers = S.unsupported_mod;
+    out->unsupported_synapses = S.unsupported_syn;
+    out->order_hazard_synapses = S.hazard_syn;
+    out->lds_bytes_per_tile = (uint32_t)S.lds_bytes;
+    for (int k = 0; k < 5; k++) out->last_phase_ms[k] = S.last_ms[k];
+    return 0;
+}
+
+int bchip_set_phase_timing(void* sim, int on) {
+    if (!sim) return -1;
+    static_cast<Sim*>(sim)->timing = on;
+    return 0;
+}
+
+void* bchip_stream(void* sim) { return sim ? static_cast<Sim*>(sim)->stream : nullptr; }
+
+int bchip_plugin_load(const char* path) {
+    // Mirrors plugin_load (src/plugin_manager.cpp:208-243): RTLD_GLOBAL so the plugin binds to
+    // this library's register_* symbols, then biocore_plugin_init(). Registered CPU functions
+    // are recorded; device ticks keep the built-in device behaviour for those IDs.
+    if (!path) return -1;
+    void* h = dlopen(path, RTLD_NOW | RTLD_GLOBAL);
+    if (!h) return -2;
+    using InitFn = int (*)(void);
+    InitFn init = reinterpret_cast<InitFn>(dlsym(h, "biocore_plugin_init"));
+    if (!init) return -3;
+    return init();
+}
+
+int bchip_registered_modifier(uint8_t type) {
+    std::lock_guard<std::mutex> lk(g_plugin_mu);
+    return type < 128 && g_mod_table[type] != nullptr;
+}
+
+int bchip_modifier_supported(uint8_t type) { return bc::modifier_supported(type) ? 1 : 0; }
+
+}  // extern "C"
+
+// C++-mangled registration entry points plugins link against
+// (_Z22register_user_modifierhPFdP8ModifierP7SegmentfE, _Z23register_user_modulatorhPFfP8ModifierPffE).
+void register_user_modifier(uint8_t type, ModifierFunc func) {
+    std::lock_guard<std::mutex> lk(g_plugin_mu);
+    if (type <= 127) g_mod_table[type] = func;
+}
+void register_user_modulator(uint8_t type, ReleaseModulatorFunc func) {
+    std::lock_guard<std::mutex> lk(g_plugin_mu);
+    if (type <= 127) g_rel_table[type] = func;
+}

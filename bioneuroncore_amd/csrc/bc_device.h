@@ -1,0 +1,646 @@
+// bc_device.h — device-side biophysics of the per-tick step, operating on register-resident
+// segment state. Every expression keeps the reference's float/double type and evaluation
+// order (the kernels are built with -ffp-contract=off) and all exp calls go through
+// bc::exp, which returns the bits of the reference's glibc ::exp. The functions cite the
+// reference routine they follow.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "bc_libm.h"
+
+#define BC_DEV __device__ static inline
+
+namespace bc {
+
+constexpr double kFaraday = 96485.0;
+constexpr float kGas = 8.314f;
+constexpr float kTemp = 310.15f;
+constexpr float kPi = 3.14159265f;
+constexpr float kEvr = 0.2f / (1.0f - 0.2f);
+constexpr float kAtoPa = 1e12f;
+constexpr float kPsToS = 1e-12f;
+constexpr float kMmToUm = 1e3f;
+constexpr float kCbarNa = 55.964f, kCbarK = 40.514f, kCbarCa = 0.12775f, kCbarCl = 36.20f;
+constexpr float kSigmaE = 3.612f;
+constexpr float kFrtMv = 0.03742f;
+constexpr float kChargeToPa = 96.485f;
+
+__device__ __constant__ static const float kZ[7] = {1.0f, 1.0f, 2.0f, -1.0f, 2.0f, -1.0f, 1.0f};
+__device__ __constant__ static const float kDiff[7] = {1.33f, 1.96f, 0.22f, 2.03f, 0.10f, 0.0f, 9.31f};
+__device__ __constant__ static const float kBath[7] = {140.0f, 3.0f, 1.2f, 130.0f, 1.0f, 0.0f, 4.0e-5f};
+
+enum : uint8_t {
+    M_PUMP = 7, M_AMPA = 8, M_NMDA = 9, M_GABAA = 11, M_SK = 103, M_BK = 104, M_K2P = 107,
+    M_KV7 = 112, M_KVDR = 123, M_HCN1 = 129, M_NALCN = 133, M_CAV21 = 140, M_NAV16 = 141,
+    M_NAV12 = 142, M_KV31 = 150, M_KV11 = 156,
+};
+
+__host__ __device__ static inline bool modifier_supported(uint8_t t) {
+    switch (t) {
+        case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
+        case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
+        case M_NAV12: case M_KV31: case M_KV11:
+            return true;
+        default:
+            return false;
+    }
+}
+
+// Segment state held in registers while its modifiers run (step 1) or while the receptors of
+// its incoming synapses run (step 3).
+struct SegRegs {
+    double V;
+    double in[7];
+    double ex[7];
+    float atp;
+    float area;    // membrane area: sphere 4*pi*r^2 or cylinder 2*pi*r*L (float)
+    double vol;    // volume (double, run_tick_common.h:260-264)
+    float pkc, pka, camp;   // cascade activities read by Nav/Kv3.1/Kv7/HCN (constant on device)
+    float nt[15];  // neurotransmitter (used by receptors)
+};
+
+// One modifier slot: states 0..6 live in SoA memory; state[7] (effective area) is a register.
+struct ModSlot {
+    double* st;
+    size_t stride;
+    double dens;
+    float a7;
+    uint8_t av;
+    __device__ double get(int k) const { return st[(size_t)k * stride]; }
+    __device__ void set(int k, double v) const { st[(size_t)k * stride] = v; }
+};
+
+BC_DEV float eff_area(const ModSlot& m, const SegRegs& s) { return (m.a7 > 0.0f) ? m.a7 : s.area; }
+BC_DEV void floor7(double* a) {
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+        if (a[i] < 1e-9) a[i] = 1e-9;
+}
+
+// ---- GHK (run_tick_common.h:274-312, modifier_helpers.h:48-75) ----
+BC_DEV double ghk_mono(double V, double ci, double co) {
+    double vn = kFaraday * V * 0.001 / (kGas * kTemp);
+    if (fabs(vn) < 1e-6) return kFaraday * (ci - co);
+    double k = kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
+    double e = bc::exp(-vn);
+    return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
+}
+BC_DEV double ghk_di(double V, double ci, double co) {
+    double vn = 2.0 * kFaraday * V * 0.001 / (kGas * kTemp);
+    if (fabs(vn) < 1e-6) return 2.0 * kFaraday * (ci - co);
+    double k = 4.0 * kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
+    double e = bc::exp(-vn);
+    return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
+}
+BC_DEV double ghk_an(double V, double ci, double co) {
+    double vn = kFaraday * V * 0.001 / (kGas * kTemp);
+    if (fabs(vn) < 1e-6) return kFaraday * (co - ci);
+    double k = kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * co;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * ci;
+    double e = bc::exp(-vn);
+    return k * kFaraday * V * 0.001 * (co - ci * e) / (1.0 - e);
+}
+BC_DEV double p_single(float gamma_ps, float z, float cbar) {
+    double g = gamma_ps * kPsToS;
+    double z2f2 = z * z * kFaraday * kFaraday;
+    double rt = kGas * kTemp;
+    return g * rt / (z2f2 * cbar);
+}
+BC_DEV void p_partial(float gamma_ps, float fna, float fk, float fca, double* pna, double* pk, double* pca) {
+    double w = fna * kCbarNa + fk * kCbarK + fca * 4.0 * kCbarCa;
+    *pna = p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa);
+    *pk = p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK);
+    *pca = p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa);
+}
+BC_DEV void p_nalcn(float gamma_ps, double* pna, double* pk, double* pca) {
+    const double fna = 1.3, fk = 1.2, fca = 0.5;
+    double w = fna * kCbarNa + fk * kCbarK + fca * 4.0 * kCbarCa;
+    *pna = p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa);
+    *pk = p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK);
+    *pca = p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa);
+}
+
+// ---- gates (modifier_helpers.h:132-271, 909-1013) ----
+BC_DEV float vtrap(float x, float y) {
+    if (fabsf(x / y) < 1e-6f) return y * (1.0f - x / (2.0f * y));
+    return x / (1.0f - bc::exp(-x / y));
+}
+struct NavK { float am_off, am_k, ah_off, bh_off, h_off; };
+BC_DEV float nav_am(const NavK& g, float V) { return 0.182f * vtrap(V + g.am_off, g.am_k); }
+BC_DEV float nav_bm(const NavK& g, float V) { return 0.124f * vtrap(-(V + g.am_off), g.am_k); }
+BC_DEV float nav_ah(const NavK& g, float V) { return 0.024f * vtrap(V + g.ah_off, 5.0f); }
+BC_DEV float nav_bh(const NavK& g, float V) { return 0.0091f * vtrap(-(V + g.bh_off), 5.0f); }
+
+BC_DEV float cav21_minf(float V) { return 1.0f / (1.0f + bc::exp(-(V - -15.0f) / (6.0f + 1e-6f))); }
+BC_DEV float cav21_hinf(float V, float ca_um) {
+    float h = 1.0f / (1.0f + bc::exp((V - -50.0f) / 8.0f));
+    h *= 1.0f / (1.0f + ca_um / 3.0f);
+    return h;
+}
+BC_DEV float kdr_tau(float V, float vh) {
+    float tau = 14.6f * bc::exp(-(V - vh) / 30.1f);
+    if (tau < 0.3f) tau = 0.3f;
+    if (tau > 200.0f) tau = 200.0f;
+    return tau;
+}
+// x^4 for a float x: x*x is exact in double, so one rounding remains; this is the correctly
+// rounded value that glibc's pow(x, 4.0) returns except in rare last-bit cases, and every use
+// rounds the result to float right after (probability of a differing float ~1e-12).
+BC_DEV double pow4(float x) {
+    double a = (double)x * (double)x;
+    return a * a;
+}
+BC_DEV float sk_pinf(float ca) {
+    if (ca < 1e-9f) ca = 1e-9f;
+    float r = 0.0004f / ca;
+    return 1.0f / (1.0f + pow4(r));
+}
+BC_DEV float bk_po(float V, float ca) {
+    float vn = kFaraday * V * 1e-3f / (kGas * kTemp);
+    float J = 0.104f * bc::exp(0.55f * vn);
+    float L = 3.7e-8f * bc::exp(0.30f * vn);
+    if (ca < 0.0f) ca = 0.0f;
+    float K = ca / 0.044f;
+    float o = 1.0f + K * 23.0f + J * 32.0f + J * K * 23.0f * 32.0f * 3.7f;
+    float c = 1.0f + K + J + J * K * 3.7f;
+    o *= o; o *= o;
+    c *= c; c *= c;
+    float num = L * o;
+    return num / (num + c);
+}
+BC_DEV float pump_vnorm(float V, float na, float ko, float atp_level) {
+    const float kmna = 10.0f, kmk = 1.5f;
+    float na3 = na * na * na, k2 = ko * ko;
+    float kna3 = kmna * kmna * kmna, kk2 = kmk * kmk;
+    float fna = na3 / (na3 + kna3), fk = k2 / (k2 + kk2);
+    float atp = atp_level;
+    if (atp < 0.0f) atp = 0.0f;
+    float fatp = atp / (atp + 0.5f);
+    float eta = 0.8f * kFaraday * (V * 1e-3f) / (kGas * kTemp);
+    float a2 = 60.0f * bc::exp(0.5f * eta);
+    float b2 = 6.0f * bc::exp(-(1.0f - 0.5f) * eta);
+    float a1 = 200.0f * fna * fatp;
+    float a3 = 200.0f * fk;
+    float a4 = 60.0f;
+    float denom = a3 / a1 + (b2 + a3) / a2 + 1.0f + a3 / a4;
+    float turnover = a3 / denom;
+    float k3 = 200.0f;
+    float vmax = k3 / (k3 / 200.0f + 1.0f + k3 / 60.0f);
+    return turnover / vmax;
+}
+
+// ---- channel currents: return pA (reference sign convention) ----
+BC_DEV double ch_pump(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:57-97
+    float area = eff_area(m, s);
+    float vn = pump_vnorm(s.V, s.in[0], s.ex[1], s.atp);
+    double N = m.dens * area;
+    float ip = N * vn * 75.0f * 1.602176634e-19f * kAtoPa;
+    float vol = s.vol;
+    double cpm = kFaraday * vol / 1000.0;
+    float tr = ip * dt / cpm;
+    float na_av = (s.in[0] - 1e-9f) / 3.0f;
+    float k_av = (s.ex[1] - 1e-9f) * kEvr / 2.0f;
+    if (na_av < 0.0f) na_av = 0.0f;
+    if (k_av < 0.0f) k_av = 0.0f;
+    float mx = (na_av < k_av) ? na_av : k_av;
+    if (tr > mx) tr = mx;
+    s.in[0] -= 3.0f * tr;
+    s.ex[0] += 3.0f * tr / kEvr;
+    s.in[1] += 2.0f * tr;
+    s.ex[1] -= 2.0f * tr / kEvr;
+    floor7(s.in);
+    floor7(s.ex);
+    s.atp -= ip * dt * 1e-8f;
+    if (s.atp < 0.0f) s.atp = 0.0f;
+    return -ip;
+}
+
+BC_DEV double ch_nav(const ModSlot& m, SegRegs& s, float dt, const NavK& g, float gamma, float ttx_ic50,
+                     float pkx) {   // hh_biological.cpp:193-277
+    float area = eff_area(m, s);
+    if (s.V < -200.0f) s.V = -200.0f;
+    if (s.V > 200.0f) s.V = 200.0f;
+    float V = s.V;
+    float am = nav_am(g, V), bm = nav_bm(g, V);
+    float minf = am / (am + bm);
+    float mtau = 1.0f / (am + bm);
+    float hinf = 1.0f / (1.0f + bc::exp((V + g.h_off) / 6.2f));
+    float htau = 1.0f / (nav_ah(g, V) + nav_bh(g, V));
+    double mv = minf + (m.get(0) - minf) * bc::exp(-dt / mtau);
+    double hv = hinf + (m.get(1) - hinf) * bc::exp(-dt / htau);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m.set(0, mv);
+    m.set(1, hv);
+    float op = mv * mv * mv * hv;
+    double s5 = m.get(5);
+    double ttx = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + ttx / ttx_ic50);
+    op *= (1.0f - pkx * (s.pkc + s.pka) * 0.5f);
+    if (op < 0.0f) op = 0.0f;
+    double P = p_single(gamma, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, s.in[0], s.ex[0]);
+    double ia = P * flux * (m.dens * area) * op;
+    double ina = ia * kAtoPa;
+    double cpv = kFaraday * s.vol / 1000.0;
+    s.in[0] -= (ina * dt) / cpv;
+    s.ex[0] += (ina * dt) / (cpv * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ina;
+}
+
+BC_DEV double ch_cav21(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:364-388
+    float V = s.V;
+    float mi = cav21_minf(V);
+    double s0 = m.get(0), s1 = m.get(1);
+    s0 += (mi - s0) * dt / 1.0f;
+    float ca = s.in[2] * kMmToUm;
+    float hi = cav21_hinf(V, ca);
+    s1 += (hi - s1) * dt / 300.0f;
+    if (s0 < 0.0f) s0 = 0.0f;
+    if (s1 < 0.0f) s1 = 0.0f;
+    if (s0 > 1.0f) s0 = 1.0f;
+    if (s1 > 1.0f) s1 = 1.0f;
+    m.set(0, s0);
+    m.set(1, s1);
+    return 0.0f;
+}
+
+BC_DEV double k_out(SegRegs& s, double ik, float dt) {
+    double cpv = kFaraday * s.vol / 1000.0;
+    s.in[1] -= (ik * dt) / cpv;
+    s.ex[1] += (ik * dt) / (cpv * kEvr);
+    floor7(s.ex);
+    return -ik;
+}
+
+BC_DEV double ch_kv31(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:1043-1100
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s0 = m.get(0), s1 = m.get(1), s2 = m.get(2), s5 = m.get(5);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - 10.0f) / (10.0f + 1e-6f)));
+    float tau_m = 1.0f;
+    s0 += (mi - s0) * dt / tau_m;
+    if (s0 < 0.0f) s0 = 0.0f;
+    if (s0 > 1.0f) s0 = 1.0f;
+    float hi = 1.0f / (1.0f + bc::exp((V + 30.0f) / 6.0f));
+    float tau_h = 2000.0f;
+    s1 += (hi - s1) * dt / tau_h;
+    if (s1 < 0.0f) s1 = 0.0f;
+    if (s1 > 1.0f) s1 = 1.0f;
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(27.0f, 1.0f, kCbarK);
+    double op = s0 * s0 * s0 * s1;
+    double tea = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + tea / 0.2f);
+    float kp = 5.6e-6f, kd = 1e-4f;
+    s2 += (kp * s.pkc * (1.0f - s2) - kd * s2) * dt;
+    if (s2 < 0.0f) s2 = 0.0f;
+    if (s2 > 1.0f) s2 = 1.0f;
+    op *= (1.0f - 0.27f * s2);
+    m.set(0, s0);
+    m.set(1, s1);
+    m.set(2, s2);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * op;
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+BC_DEV double ch_kv11(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:1189-1231
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s0 = m.get(0), s1 = m.get(1), s5 = m.get(5), s6 = m.get(6);
+    float vh = -30.0f, kk = 6.0f, gamma = 10.0f;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (kk + 1e-6f)));
+    float tau_m = kdr_tau(V, vh);
+    s0 += (mi - s0) * dt / tau_m;
+    if (s0 < 0.0f) s0 = 0.0f;
+    if (s0 > 1.0f) s0 = 1.0f;
+    float hi = 1.0f / (1.0f + bc::exp((V + 40.0f) / 6.0f));
+    float tau_h = 3000.0f;
+    s1 += (hi - s1) * dt / tau_h;
+    if (s1 < 0.0f) s1 = 0.0f;
+    if (s1 > 1.0f) s1 = 1.0f;
+    m.set(0, s0);
+    m.set(1, s1);
+    double op = s0 * s0 * s0 * s0 * s1;
+    double dtx = (s5 > 0.0f) ? s5 : 0.0f;
+    double tea = (s6 > 0.0f) ? s6 : 0.0f;
+    op /= (1.0f + dtx / 2.0f) * (1.0f + tea / 170.0f);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * op;
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+BC_DEV double ch_kvdr(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:2763-2798
+    float V = s.V;
+    float area = eff_area(m, s);
+    double mv = m.get(0), s1 = m.get(1), s2 = m.get(2), s4 = m.get(4);
+    double vh = (s1 != 0.0f) ? s1 : -30.0f;
+    double kk = (s2 > 0.0f) ? s2 : 6.0f;
+    double tau = (s4 > 0.0f) ? s4 : kdr_tau(V, vh);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (kk + 1e-6f)));
+    mv += (mi - mv) * dt / (tau + 1e-6f);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m.set(0, mv);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(12.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * mv;
+    double ik = ia * kAtoPa;
+    float cpmv = kFaraday * s.vol / 1000.0;
+    s.in[1] -= (ik * dt) / cpmv;
+    s.ex[1] += (ik * dt) / (cpmv * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ik;
+}
+
+BC_DEV double ch_sk(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:2859-2888
+    float V = s.V;
+    float area = eff_area(m, s);
+    double op = m.get(0), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
+    double gamma = (s3 > 0.0f) ? s3 : 10.0f;
+    double t_open = (s4 > 0.0f) ? s4 : 5.0f;
+    double t_close = (s5 > 0.0f) ? s5 : 50.0f;
+    float ca = s.in[2];
+    float pi = sk_pinf(ca);
+    float tau = (pi > op) ? t_open : t_close;
+    op += (pi - op) * dt / tau;
+    if (op < 0.0f) op = 0.0f;
+    if (op > 1.0f) op = 1.0f;
+    m.set(0, op);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * op;
+    float ik = (float)(ia * kAtoPa);
+    double cpv = kFaraday * s.vol / 1000.0f;
+    s.in[1] -= (ik * dt) / cpv;
+    s.ex[1] += (ik * dt) / (cpv * kEvr);
+    floor7(s.ex);
+    return -ik;
+}
+
+BC_DEV double ch_bk(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:2922-2949
+    float V = s.V;
+    float area = eff_area(m, s);
+    double op = m.get(0), s1 = m.get(1), s2 = m.get(2);
+    double gamma = (s1 > 0.0f) ? s1 : 250.0f;
+    double tau = (s2 > 0.0f) ? s2 : 1.0f;
+    float ca = s.in[2];
+    float pi = bk_po(V, ca);
+    op += (pi - op) * dt / tau;
+    if (op < 0.0f) op = 0.0f;
+    if (op > 1.0f) op = 1.0f;
+    m.set(0, op);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * op;
+    float ik = (float)(ia * kAtoPa);
+    double cpv = kFaraday * s.vol / 1000.0f;
+    s.in[1] -= (ik * dt) / cpv;
+    s.ex[1] += (ik * dt) / (cpv * kEvr);
+    floor7(s.ex);
+    return -ik;
+}
+
+BC_DEV double ch_k2p(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:3063-3085
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s0 = m.get(0), s1 = m.get(1);
+    double gamma = (s0 > 0.0f) ? s0 : 100.0f;
+    double po = (s1 > 0.0f) ? s1 : 0.012f;
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * po;
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+BC_DEV double ch_kv7(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:3158-3220
+    float V = s.V;
+    float area = eff_area(m, s);
+    double mv = m.get(0), s1 = m.get(1), s2 = m.get(2), s4 = m.get(4), s5 = m.get(5), s6 = m.get(6);
+    double vh = (s1 != 0.0f) ? s1 : -38.0f;
+    double ks = (s2 > 0.0f) ? s2 : 10.0f;
+    double t_act = (s4 > 0.0f) ? s4 : 100.0f;
+    double t_deact = (s5 > 0.0f) ? s5 : 60.0f;
+    double pip2 = (s6 >= 0.0f) ? s6 : 1.0f;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / ks));
+    float tau_m = t_deact + (t_act - t_deact) * mi;
+    if (tau_m < 1.0f) tau_m = 1.0f;
+    mv += (mi - mv) * dt / tau_m;
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m.set(0, mv);
+    if (pip2 < 0.01f) return 0.0f;
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(7.0f, 1.0f, kCbarK);
+    float sup = 1.0f - 0.8f * s.pkc;
+    if (sup < 0.0f) sup = 0.0f;
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m.dens * area) * (mv * pip2 * sup);
+    return k_out(s, ia * kAtoPa, dt);
+}
+
+BC_DEV double ch_hcn1(const ModSlot& m, SegRegs& s, float dt) {   // modifier_helpers.h:1442-1484
+    double V = s.V;
+    float area = eff_area(m, s);
+    double s1 = m.get(1), s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
+    float vh = (s1 != 0) ? s1 : -69.0f;
+    float k = (s2 > 0) ? s2 : 6.0f;
+    float gamma = (s3 > 0) ? s3 : 0.84f;
+    float t_open = (s4 > 0) ? s4 : 84.0f;
+    float t_close = (s5 > 0) ? s5 : 300.0f;
+    float camp = s.camp;
+    float ve = vh + 15.0f * camp / (camp + 0.5f);
+    float mv = m.get(0);
+    float vf = V;
+    float mi = 1.0f / (1.0f + bc::exp((vf - ve) / k));
+    float tau = (V < ve) ? (t_open * bc::exp((V - ve) / 15.0f)) : t_close;
+    mv += (mi - mv) * dt / tau;
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m.set(0, mv);
+    double pna, pk, pca;
+    p_partial(gamma, 0.2f, 0.8f, 0.0f, &pna, &pk, &pca);
+    double N = m.dens * area;
+    double fna = ghk_mono(V, s.in[0], s.ex[0]);
+    double fk = ghk_mono(V, s.in[1], s.ex[1]);
+    double in = pna * fna * N * mv * kAtoPa;
+    double ik = pk * fk * N * mv * kAtoPa;
+    double cpm = kFaraday * s.vol / 1000.0;
+    s.in[0] -= (in * dt) / cpm;
+    s.in[1] -= (ik * dt) / cpm;
+    s.ex[0] += (in * dt) / (cpm * kEvr);
+    s.ex[1] += (ik * dt) / (cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(in + ik);
+}
+
+BC_DEV double ch_nalcn(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:3609-3641
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s0 = m.get(0);
+    double gamma = (s0 > 0.0f) ? s0 : 27.0f;
+    double pna, pk, pca;
+    p_nalcn(gamma, &pna, &pk, &pca);
+    double N = m.dens * area;
+    float ina = (float)(pna * ghk_mono(V, s.in[0], s.ex[0]) * N * kAtoPa);
+    float ik = (float)(pk * ghk_mono(V, s.in[1], s.ex[1]) * N * kAtoPa);
+    float ica = (float)(pca * ghk_di(V, s.in[2], s.ex[2]) * N * kAtoPa);
+    double cpm = kFaraday * s.vol / 1000.0;
+    s.in[0] -= (ina * dt) / cpm;
+    s.in[1] -= (ik * dt) / cpm;
+    s.in[2] -= (ica * dt) / (2.0f * cpm);
+    s.ex[0] += (ina * dt) / (cpm * kEvr);
+    s.ex[1] += (ik * dt) / (cpm * kEvr);
+    s.ex[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(ina - ik + ica);
+}
+
+// Ionotropic receptor (modifier_helpers.h:1213-1276).
+struct IonoK { float gamma, ec50, t_rise, t_decay; int nt; float pna, pk, pca; bool anion; float t_desens; };
+
+BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
+    float area = eff_area(m, s);
+    double V = s.V;
+    double cpm = kFaraday * s.vol / 1000.0;
+    float gamma = p.gamma;
+    if (m.av > 0 && m.av < 7) {
+        double g = m.get(m.av);
+        if (g > 0.0f) gamma = g;
+    }
+    float nt = s.nt[p.nt];
+    float r_inf = nt / (nt + p.ec50);
+    double s0 = m.get(0);
+    float r = s0;
+    float tau = (r_inf > r) ? p.t_rise : p.t_decay;
+    s0 += (r_inf - r) * dt / tau;
+    if (s0 < 0.0f) s0 = 0.0f;
+    if (s0 > 1.0f) s0 = 1.0f;
+    m.set(0, s0);
+    float op = s0;
+    if (p.t_desens > 0.0f) {
+        float kd = p.ec50;
+        float d = m.get(1);
+        float d_inf = nt / (nt + kd);
+        d += (d_inf - d) * dt / p.t_desens;
+        if (d < 0.0f) d = 0.0f;
+        if (d > 1.0f) d = 1.0f;
+        m.set(1, d);
+        op *= (1.0f - d);
+    }
+    double N = m.dens * area;
+    const float scale = 1.0f;
+    if (p.anion) {
+        double pcl = p_single(gamma, 1.0f, kCbarCl);
+        double fcl = ghk_an(V, s.in[3], s.ex[3]);
+        double icl = pcl * fcl * N * op * scale * kAtoPa;
+        s.in[3] += (icl * dt) / cpm;
+        floor7(s.in);
+        return -icl;
+    }
+    double pna, pk, pca;
+    p_partial(gamma, p.pna, p.pk, p.pca, &pna, &pk, &pca);
+    double ina = pna * ghk_mono(V, s.in[0], s.ex[0]) * N * op * scale * kAtoPa;
+    double ik = pk * ghk_mono(V, s.in[1], s.ex[1]) * N * op * scale * kAtoPa;
+    double itot = ina + ik;
+    s.in[0] -= (ina * dt) / cpm;
+    s.in[1] -= (ik * dt) / cpm;
+    if (p.pca > 0.0f) {
+        double ica = pca * ghk_di(V, s.in[2], s.ex[2]) * N * op * scale * kAtoPa;
+        itot += ica;
+        s.in[2] -= (ica * dt) / (2.0f * cpm);
+    }
+    floor7(s.in);
+    return -itot;
+}
+
+BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_biological.cpp:48-55
+    const IonoK k = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
+    double I = iono(m, s, dt, k);
+    float mg = 1.0f / (1.0f + (1.0f / 3.57f) * bc::exp(-0.062f * s.V));
+    double r = I * mg;
+    // nmda_cdi_coagonist (modifier_helpers.h:1281-1291)
+    double ca_um = s.in[2] * kMmToUm;
+    double cdi_inf = 1.0 - 0.8f * ca_um / (ca_um + 1.0f);
+    double s2 = m.get(2);
+    float tau = (cdi_inf < s2) ? 500.0f : 9000.0f;
+    s2 += (cdi_inf - s2) * dt / tau;
+    if (s2 < 0.0f) s2 = 0.0f;
+    if (s2 > 1.0f) s2 = 1.0f;
+    m.set(2, s2);
+    float coag = 10.0f / (10.0f + 1.0f);
+    return r * (s2 * coag);
+}
+
+// Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
+// current and leave their state untouched, as the reference's device switch does.
+BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
+    switch (type) {
+        case M_PUMP: return ch_pump(m, s, dt);
+        case M_AMPA: {
+            const IonoK k = {8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f};
+            return iono(m, s, dt, k);
+        }
+        case M_NMDA: return rec_nmda(m, s, dt);
+        case M_GABAA: {
+            const IonoK k = {28.0f, 1.0f, 1.0f, 5.0f, 1, 0.0f, 0.0f, 0.0f, true, 2000.0f};
+            return iono(m, s, dt, k);
+        }
+        case M_SK: return ch_sk(m, s, dt);
+        case M_BK: return ch_bk(m, s, dt);
+        case M_K2P: return ch_k2p(m, s, dt);
+        case M_KV7: return ch_kv7(m, s, dt);
+        case M_KVDR: return ch_kvdr(m, s, dt);
+        case M_HCN1: return ch_hcn1(m, s, dt);
+        case M_NALCN: return ch_nalcn(m, s, dt);
+        case M_CAV21: return ch_cav21(m, s, dt);
+        case M_NAV16: {
+            const NavK g = {43.0f, 6.0f, 50.0f, 75.0f, 72.0f};
+            return ch_nav(m, s, dt, g, 20.7f, 5.0f, 0.2f);
+        }
+        case M_NAV12: {
+            const NavK g = {23.0f, 6.0f, 41.5f, 66.5f, 63.5f};
+            return ch_nav(m, s, dt, g, 20.8f, 10.0f, 0.7f);
+        }
+        case M_KV31: return ch_kv31(m, s, dt);
+        case M_KV11: return ch_kv11(m, s, dt);
+        default: return 0.0;
+    }
+}
+
+// ---- electrodiffusion helpers (diffusion_kernel.h:108-113) ----
+BC_DEV float bernoulli(float x) {
+    if (fabsf(x) < 1e-3f) return 1.0f - 0.5f * x;
+    if (x > 50.0f) return 0.0f;
+    if (x < -50.0f) return -x;
+    return x / (bc::exp(x) - 1.0f);
+}
+
+BC_DEV float xorshift_draw(uint32_t seed) {   // simulation_host.cpp:150-156
+    uint32_t x = seed;
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return (float)(x & 0x7FFFFFFFu) / (float)0x7FFFFFFFu;
+}
+
+}  // namespace bc

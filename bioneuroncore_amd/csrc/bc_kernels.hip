@@ -1,0 +1,719 @@
+// bc_kernels.hip — the per-tick step as gfx950 kernels.
+//
+//   k_modifiers      step 1: one lane per segment, modifier slots in order, stimuli
+//   k_diffusion      step 2.5: one workgroup per tile of whole islands, all state in LDS
+//   k_syn_pre        step 3, presynaptic half: delay-ring arrivals, bouton Ca, release draw
+//   k_syn_post       step 3, post-synaptic half in synapse order per post segment + step 4
+//   k_syn_rrp        step 3, vesicle-pool recovery/depletion and delay-ring enqueue
+//
+// Ordering facts that make the split exact (reference src/simulation_host.cpp:322-593):
+// a synapse's pre-side reads only pre V_m, extra Ca and Cav2.1 state, none of which step 3
+// writes; the post-side of a segment depends only on its own incoming synapses in index order;
+// the one cross-chain read is rrp_do_recovery's pre intra Ca, which k_syn_rrp takes from the
+// recorded post-chain history (y_ca_after) of the last earlier synapse onto that segment.
+#include <hip/hip_runtime.h>
+#include "bc_device.h"
+#include "bc_state.h"
+#include "../../include/biocore_abi.h"
+
+namespace bc {
+
+__device__ static inline bool is_ionotropic(uint8_t t) {
+    return t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194;
+}
+// kernel_dispatch.h:301-309 restricted to the covered IDs (the rest run as 0-current anyway).
+__device__ static inline bool myelin_blocked(uint8_t t) {
+    return is_ionotropic(t) || t == M_K2P || t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 ||
+           t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK;
+}
+__device__ static inline uint8_t nt_of_receptor(uint8_t rt) {   // run_tick_common.h:241-248
+    if (rt == 11 || rt == 17) return 1;
+    if (rt == 12) return 7;
+    if (rt == 14) return 3;
+    if (rt == 13) return 4;
+    if (rt == 44 || rt == 22) return 2;
+    return 0;
+}
+__device__ static inline uint64_t delay_ticks(float delay, float dt) {   // delay_buffer.h:61-64
+    if (delay <= 0.0f || dt <= 0.0f) return 0;
+    return (uint64_t)((delay / dt) + 0.5f);
+}
+
+// ------------------------------------------------------------------ step 1
+__global__ __launch_bounds__(256) void k_modifiers(Net d, const GpuStimulusData* __restrict__ st,
+                                                   uint64_t tick, float dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const size_t P = d.npad;
+    SegRegs s;
+    s.V = d.V[i];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        s.in[k] = d.in[k * P + i];
+        s.ex[k] = d.ex[k * P + i];
+    }
+    s.atp = d.atp[i];
+    s.area = d.area[i];
+    s.vol = d.vol[i];
+    s.pkc = d.pkc[i];
+    s.pka = d.pka[i];
+    s.camp = d.camp[i];
+    const bool my = d.sflags[i] & SF_MYELIN;
+    const uint32_t cnt = d.mcount[i];
+    double itot = 0.0;
+    for (uint32_t j = 0; j < cnt; j++) {
+        const uint8_t t = d.mtype[j * P + i];
+        if (my && t != 4 && myelin_blocked(t)) continue;
+        ModSlot m;
+        m.st = d.mstate + (size_t)j * 7 * P + i;
+        m.stride = P;
+        m.dens = d.mdens[j * P + i];
+        m.a7 = s.area;
+        m.av = d.mav[j * P + i];
+        itot += dispatch(t, m, s, dt);
+    }
+    if (st) {
+        const uint32_t sid = d.seg_id[i];
+        for (uint32_t k = 0; k < st->current_stim_count; k++) {
+            const CurrentStimulus q = st->current_stim[k];
+            if (q.segment_id == sid) {
+                uint64_t dur = (uint64_t)(q.duration_ms / dt);
+                if (tick >= q.start_tick && tick < q.start_tick + dur) itot += q.amplitude_pA;
+            }
+        }
+        for (uint32_t k = 0; k < st->ramp_count; k++) {
+            const RampStimulus q = st->ramps[k];
+            if (q.segment_id == sid && tick >= q.start_tick && tick < q.end_tick) {
+                float frac = (q.end_tick > q.start_tick)
+                                 ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick)
+                                 : 1.0f;
+                float amp = q.start_pA + (q.end_pA - q.start_pA) * frac;
+                itot += amp;
+            }
+        }
+        for (uint32_t k = 0; k < st->ion_inj_count; k++) {
+            const IonInjection q = st->ion_inj[k];
+            if (q.segment_id == sid && q.ion_idx >= 0 && q.ion_idx < 7) {
+                if (q.extracellular) s.ex[q.ion_idx] += q.amount_mM;
+                else s.in[q.ion_idx] += q.amount_mM;
+            }
+        }
+        for (uint32_t k = 0; k < st->lig_inj_count; k++) {
+            const LigandInjection q = st->lig_inj[k];
+            if (q.segment_id == sid && q.nt_idx >= 0 && q.nt_idx < 15) {
+                if (q.intracellular) d.nt_in[q.nt_idx * P + i] += q.amount;
+                else d.nt[q.nt_idx * P + i] += q.amount;
+            }
+        }
+    }
+    d.V[i] = s.V;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        d.in[k * P + i] = s.in[k];
+        d.ex[k * P + i] = s.ex[k];
+    }
+    d.atp[i] = s.atp;
+    d.ibuf[i] = itot;
+}
+
+// ------------------------------------------------------------------ step 2.5
+// LDS layout of one tile (kTileSeg segments, kTileConn connections).
+struct DiffLds {
+    double V[kTileSeg];
+    double phi[2][kTileSeg];
+    double sig[kTileSeg];
+    double in[7][kTileSeg];
+    double ex[7][kTileSeg];
+    float vf[2][kTileSeg];
+    float volf[kTileSeg];
+    double cflux[7][kTileConn];   // intra fluxes (as float) / extra fluxes / phi conductances
+    float cg[kTileConn];
+    float cterm[kTileConn];
+    float cac[kTileConn];
+    float cdist[kTileConn];
+    uint32_t cab[kTileConn];
+    uint8_t cintra[kTileConn];
+};
+
+__global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    DiffLds& L = *reinterpret_cast<DiffLds*>(lds_raw);
+    const uint32_t tile = blockIdx.x;
+    const uint32_t s0 = d.tile_seg0[tile];
+    const uint32_t nseg = d.tile_nseg[tile];
+    const uint32_t c0 = d.tile_conn0[tile];
+    const uint32_t nconn = d.tile_nconn[tile];
+    const uint32_t t = threadIdx.x;
+    const bool act = t < nseg;
+    const uint32_t g = s0 + t;
+    const size_t P = d.npad;
+
+    // per-segment registers
+    double ibuf = 0.0, diag = 0.0;
+    float area = 0.0f, cap = 0.0f, volf = 0.0f;
+    uint32_t e0 = 0, e1 = 0;
+    if (act) {
+        L.V[t] = d.V[g];
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            L.in[k][t] = d.in[k * P + g];
+            L.ex[k][t] = d.ex[k * P + g];
+        }
+        ibuf = d.ibuf[g];
+        area = d.area[g];
+        cap = d.cap[g];
+        volf = (float)d.vol[g];
+        if (volf < 1e-6f) volf = 1e-6f;
+        L.volf[t] = volf;
+        e0 = d.csr_off[g];
+        e1 = d.csr_off[g + 1];
+    }
+    for (uint32_t c = t; c < nconn; c += blockDim.x) {
+        uint32_t ab = d.conn_ab[c0 + c];
+        L.cab[c] = ab;
+        L.cintra[c] = d.conn_intra[c0 + c];
+    }
+    __syncthreads();
+    // connection geometry (diffusion_kernel.h:72-80,162-172)
+    for (uint32_t c = t; c < nconn; c += blockDim.x) {
+        uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
+        float ra = d.radius[s0 + a], rb = d.radius[s0 + b];
+        float rmin = ra < rb ? ra : rb;
+        float ac = kPi * rmin * rmin;
+        if (ac < 1e-6f) ac = 1e-6f;
+        float dist = (d.length[s0 + a] + d.length[s0 + b]) * 0.5f;
+        if (dist < 0.1f) dist = 0.1f;
+        L.cac[c] = ac;
+        L.cdist[c] = dist;
+    }
+    // ---- phi_e (diffusion_kernel.h:41-104) ----
+    if (act) {
+        double sg = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; k++) sg += kZ[k] * kZ[k] * kDiff[k] * L.ex[k][t];
+        sg = kSigmaE * sg;
+        L.sig[t] = sg;
+        diag = sg * area / 1.0f;
+        L.phi[0][t] = 0.0;
+    }
+    __syncthreads();
+    double* gphi = &L.cflux[0][0];
+    for (uint32_t c = t; c < nconn; c += blockDim.x) {
+        uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
+        gphi[c] = 0.5 * (L.sig[a] + L.sig[b]) * L.cac[c] / L.cdist[c];
+    }
+    __syncthreads();
+    if (act)
+        for (uint32_t e = e0; e < e1; e++) diag += gphi[d.csr_ent[e] >> 1];
+    int cur = 0;
+    for (int it = 0; it < 50; it++) {
+        if (act) {
+            double pn = 0.0;
+            for (uint32_t e = e0; e < e1; e++) {
+                uint32_t ent = d.csr_ent[e];
+                uint32_t c = ent >> 1;
+                double gg = gphi[c];
+                if (gg <= 0.0) continue;
+                uint32_t o = (ent & 1) ? (L.cab[c] & 0xFFFF) : (L.cab[c] >> 16);
+                pn += gg * L.phi[cur][o];
+            }
+            double ph = (diag > 1e-9) ? (pn - ibuf) / diag : 0.0;
+            if (ph > 10.0) ph = 10.0;
+            if (ph < -10.0) ph = -10.0;
+            L.phi[cur ^ 1][t] = ph;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    const double* phi = L.phi[cur];
+
+    // ---- intracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:151-238) ----
+    const int kSub = 5;
+    const float dts = dt / kSub;
+    float cdt = cap / dts;
+    if (cdt < 1e-9f) cdt = 1e-9f;
+    float* cfin = reinterpret_cast<float*>(&L.cflux[0][0]);   // [7][kTileConn] floats
+    for (int sub = 0; sub < kSub; sub++) {
+        for (uint32_t c = t; c < nconn; c += blockDim.x) {
+            if (!L.cintra[c]) continue;
+            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
+            float ac = L.cac[c], dist = L.cdist[c];
+            float sg = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                float cbar = 0.5f * (L.in[k][a] + L.in[k][b]);
+                sg += kZ[k] * kZ[k] * kDiff[k] * cbar;
+            }
+            float gax = kSigmaE * sg * ac / dist;
+            L.cg[c] = gax;
+            float dV = (L.V[a] + phi[a]) - (L.V[b] + phi[b]);
+            float inp = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                float ca = L.in[k][a], cb = L.in[k][b];
+                float x = kZ[k] * kFrtMv * dV;
+                float fl = kDiff[k] * ac * dts / dist * (ca * bernoulli(-x) - cb * bernoulli(x));
+                cfin[k * kTileConn + c] = fl;
+                inp += kZ[k] * kChargeToPa * fl / dts;
+            }
+            L.cterm[c] = inp - gax * dV;
+        }
+        __syncthreads();
+        float gdiag = 0.0f, inpd = 0.0f;
+        double dacc[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) dacc[k] = 0.0;
+        float vold = 0.0f;
+        if (act) {
+            for (uint32_t e = e0; e < e1; e++) {
+                uint32_t ent = d.csr_ent[e];
+                uint32_t c = ent >> 1;
+                if (!L.cintra[c]) continue;
+                gdiag += L.cg[c];
+                if (ent & 1) {
+                    inpd -= L.cterm[c];
+#pragma unroll
+                    for (int k = 0; k < 7; k++) dacc[k] += (double)cfin[k * kTileConn + c] / volf;
+                } else {
+                    inpd += L.cterm[c];
+#pragma unroll
+                    for (int k = 0; k < 7; k++) dacc[k] -= (double)cfin[k * kTileConn + c] / volf;
+                }
+            }
+            vold = L.V[t];
+            L.vf[0][t] = vold;
+        }
+        __syncthreads();
+        int vc = 0;
+        float vnew = vold;
+        for (int it = 0; it < 50; it++) {
+            if (act) {
+                float off = 0.0f;
+                for (uint32_t e = e0; e < e1; e++) {
+                    uint32_t ent = d.csr_ent[e];
+                    uint32_t c = ent >> 1;
+                    if (!L.cintra[c]) continue;
+                    uint32_t o = (ent & 1) ? (L.cab[c] & 0xFFFF) : (L.cab[c] >> 16);
+                    off += L.cg[c] * L.vf[vc][o];
+                }
+                vnew = (cdt * vold + inpd + off) / (cdt + gdiag);
+                L.vf[vc ^ 1][t] = vnew;
+            }
+            __syncthreads();
+            vc ^= 1;
+        }
+        if (act) {
+            L.V[t] = vnew;
+#pragma unroll
+            for (int k = 0; k < 7; k++) L.in[k][t] += (float)dacc[k];
+        }
+        __syncthreads();
+    }
+
+    // ---- extracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:245-281) ----
+    // phi is fixed across the substeps, so each connection's Bernoulli factors are too.
+    constexpr int kMaxConnPerLane = kTileConn / 256 > 0 ? kTileConn / 256 : 1;
+    float bm[kMaxConnPerLane][7], bp[kMaxConnPerLane][7];
+    {
+        int q = 0;
+        for (uint32_t c = t; c < nconn && q < kMaxConnPerLane; c += blockDim.x, q++) {
+            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
+            double dphi = phi[a] - phi[b];
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                float x = kZ[k] * kFrtMv * dphi;
+                bm[q][k] = bernoulli(-x);
+                bp[q][k] = bernoulli(x);
+            }
+        }
+    }
+    for (int sub = 0; sub < kSub; sub++) {
+        int q = 0;
+        for (uint32_t c = t; c < nconn && q < kMaxConnPerLane; c += blockDim.x, q++) {
+            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
+            float ac = L.cac[c], dist = L.cdist[c];
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                double ca = L.ex[k][a], cb = L.ex[k][b];
+                L.cflux[k][c] = kDiff[k] * ac * dts / dist * (ca * bm[q][k] - cb * bp[q][k]);
+            }
+        }
+        __syncthreads();
+        if (act) {
+            double dx[7];
+#pragma unroll
+            for (int k = 0; k < 7; k++) dx[k] = 0.0;
+            const float ve = volf * kEvr;
+            for (uint32_t e = e0; e < e1; e++) {
+                uint32_t ent = d.csr_ent[e];
+                uint32_t c = ent >> 1;
+                if (ent & 1) {
+#pragma unroll
+                    for (int k = 0; k < 7; k++) dx[k] += L.cflux[k][c] / ve;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 7; k++) dx[k] -= L.cflux[k][c] / ve;
+                }
+            }
+            // Only this lane reads its own ex during the gather; neighbours read cflux.
+#pragma unroll
+            for (int k = 0; k < 7; k++) L.ex[k][t] += dx[k];
+        }
+        __syncthreads();
+    }
+
+    // ---- transmitters, ATP, temperature, transcription (diffusion_kernel.h:284-345) ----
+    float dnt_in[15], dnt[15];
+#pragma unroll
+    for (int k = 0; k < 15; k++) { dnt_in[k] = 0.0f; dnt[k] = 0.0f; }
+    float datp = 0.0f, dtemp = 0.0f, dtxn = 0.0f;
+    for (uint32_t e = e0; e < e1; e++) {   // e0 == e1 on lanes past the tile
+        uint32_t ent = d.csr_ent[e];
+        uint32_t c = ent >> 1;
+        const bool role_b = ent & 1;
+        uint32_t a = s0 + (L.cab[c] & 0xFFFF), b = s0 + (L.cab[c] >> 16);
+        float ac = L.cac[c], dist = L.cdist[c];
+        const bool intra = L.cintra[c];
+        if (intra) {
+            for (int k = 0; k < 15; k++) {
+                float dC = d.nt_in[k * P + a] - d.nt_in[k * P + b];
+                if (dC < 1e-12f && dC > -1e-12f) continue;
+                float fl = d.nt_diff[k] * ac * dC * dt / dist;
+                if (role_b) dnt_in[k] += fl / volf;
+                else dnt_in[k] -= fl / volf;
+            }
+        }
+        for (int k = 0; k < 15; k++) {
+            float dC = d.nt[k * P + a] - d.nt[k * P + b];
+            if (dC < 1e-12f && dC > -1e-12f) continue;
+            float fl = d.nt_diff[k] * ac * dC * dt / dist;
+            if (role_b) dnt[k] += fl / volf;
+            else dnt[k] -= fl / volf;
+        }
+        if (intra) {
+            float d1 = d.atp[a] - d.atp[b];
+            if (d1 > 1e-12f || d1 < -1e-12f) {
+                float fl = 0.14f * ac * d1 * dt / dist;
+                if (role_b) datp += fl / volf; else datp -= fl / volf;
+            }
+            float d2 = d.temp[a] - d.temp[b];
+            if (d2 > 0.001f || d2 < -0.001f) {
+                float fl = 0.0014f * ac * d2 * dt / dist;
+                if (role_b) dtemp += fl / volf; else dtemp -= fl / volf;
+            }
+            float d3 = d.txn[a] - d.txn[b];
+            if (d3 > 1e-9f || d3 < -1e-9f) {
+                float fl = 0.01f * ac * d3 * dt / dist;
+                if (role_b) dtxn += fl / volf; else dtxn -= fl / volf;
+            }
+        }
+    }
+    // Every lane of the tile has read its neighbours' transmitter values before any write.
+    __syncthreads();
+    if (!act) return;
+    d.V[g] = L.V[t];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        d.in[k * P + g] = L.in[k][t];
+        double x = L.ex[k][t];
+        x += (kBath[k] - x) * dt / 1.0f;
+        if (x < 1e-9) x = 1e-9;
+        if (x > 1000.0) x = 1000.0;
+        d.ex[k * P + g] = x;
+    }
+    for (int k = 0; k < 15; k++) {
+        d.nt_in[k * P + g] += dnt_in[k];
+        d.nt[k * P + g] += dnt[k];
+    }
+    d.atp[g] += datp;
+    d.temp[g] += dtemp;
+    d.txn[g] += dtxn;
+}
+
+// Without any connection step 2.5 is skipped entirely (diffusion_kernel.h:118); nothing here.
+
+// ------------------------------------------------------------------ step 3, pre-side
+__global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= d.ns) return;
+    const size_t NS = d.ns;
+    d.y_relp[y] = 0.0f;
+    if (d.y_flags[y] & YF_SKIP) return;
+    const float delay = d.y_delay[y];
+    // delayed arrivals (delay_buffer.h:31-43)
+    uint8_t mask = d.d_mask[y];
+    if (delay > 0.0f && mask) {
+        for (int s = 0; s < 4; s++) {
+            if (!(mask & (1u << s))) continue;
+            uint64_t arr = d.d_arr[s * NS + y];
+            if (arr <= tick) {
+                uint8_t nt = d.d_nt[s * NS + y];
+                d.y_cleft[nt * NS + y] += d.d_amt[s * NS + y];
+                d.d_arr[s * NS + y] = 0;
+                mask &= ~(1u << s);
+            }
+        }
+        d.d_mask[y] = mask;
+    }
+    // bouton Ca microdomain (modifier_helpers.h:764-804), no presynaptic autoreceptors
+    const uint32_t pre = d.y_pre[y];
+    const size_t P = d.npad;
+    float vb = d.V[pre];
+    float bca = d.y_bca[y];
+    float dens = 0.0f, mg = 0.0f, hg = 0.0f;
+    int cs = d.cav21_slot[pre];
+    if (cs >= 0) {
+        dens = d.mdens[cs * P + pre];
+        float mi = cav21_minf(vb);
+        float hi = cav21_hinf(vb, bca * kMmToUm);
+        mg = mi;
+        hg = hi + (d.mstate[(size_t)cs * 7 * P + 1 * P + pre] - hi) * bc::exp(-dt / 300.0f);
+    }
+    float open = mg * mg * hg;
+    double Pc = p_single(15.0f, 2.0f, kCbarCa);
+    double flux = ghk_di(vb, bca, d.ex[2 * P + pre]);
+    double ia = Pc * flux * (dens * d.y_pre_area[y]) * open;
+    float ica = (float)(ia * kAtoPa);
+    bca -= ica * dt * 1000.0f / (2.0f * kFaraday * 0.1f);
+    bca += (0.0001f - bca) * dt / 5.0f;
+    if (bca < 0.0f) bca = 0.0f;
+    d.y_bca[y] = bca;
+    // Ca-triggered release (simulation_host.cpp:421-459)
+    uint8_t refr = d.y_refr[y];
+    if (refr == 0) {
+        float cu = bca * kMmToUm;
+        float act = pow4(cu) / (pow4(cu) + 10000.0);
+        float p = d.y_prel[y] * act;
+        uint32_t seed = d.y_id[y] ^ ((uint32_t)tick) ^ (uint32_t)(int64_t)(d.V[pre] * 1000.0f);
+        if (xorshift_draw(seed) < p) {
+            d.y_relp[y] = p;
+            refr = (uint8_t)(uint32_t)(20.0f / dt + 0.5f);
+            if (delay_ticks(delay, dt) == 0) {
+                // Immediate release: vesicle pool now (pre Ca = value at step-3 start).
+                float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
+                if (!(rmax <= 0.0f || rrp >= rmax)) {
+                    float ca = d.in[2 * P + pre];
+                    float caf = ca / (0.0005f + ca);
+                    float prim = d.y_rrp_rec[y] + d.camkii[pre] * 0.02f + d.calcn[pre] * 0.01f;
+                    prim *= caf;
+                    float rec = (rmax - rrp) * prim * dt;
+                    rrp += rec;
+                    if (rrp > rmax) rrp = rmax;
+                }
+                float amt;
+                float amount = d.y_amount[y];
+                if (rmax <= 0.0f) {
+                    amt = amount * p;
+                } else {
+                    float av = amount;
+                    if (av > rrp) av = rrp;
+                    amt = av * p;
+                    rrp -= amt;
+                    if (rrp < 0) rrp = 0;
+                }
+                d.y_rrp[y] = rrp;
+                uint8_t nt = d.y_rcount[y] > 0 ? nt_of_receptor(d.y_rtype[y]) : 0;
+                d.y_cleft[nt * NS + y] += amt;
+                d.y_relp[y] = -1.0f;   // finished here
+            }
+        }
+    }
+    if (refr > 0) refr--;
+    d.y_refr[y] = refr;
+}
+
+// ------------------------------------------------------------------ step 3 post-side + step 4
+__global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* __restrict__ st, uint64_t tick,
+                                                  float dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const size_t P = d.npad, NS = d.ns;
+    const uint8_t fl = d.sflags[i];
+    double ibuf = d.ibuf[i];
+    double V = d.V[i];
+    if (d.ns > 0) {
+        SegRegs s;
+        s.V = V;
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            s.in[k] = d.in[k * P + i];
+            s.ex[k] = d.ex[k * P + i];
+        }
+        s.area = d.area[i];
+        s.vol = d.vol[i];
+#pragma unroll
+        for (int k = 0; k < 15; k++) s.nt[k] = 0.0f;
+        if (fl & SF_BATH_NMDA) s.nt[0] = 1.0f;
+        if (fl & SF_BATH_5HT) s.nt[3] = 0.0f;
+        const bool my = fl & SF_MYELIN;
+        const uint32_t y0 = d.post_off[i], y1 = d.post_off[i + 1];
+        const float pp1 = d.pp1[i];
+        for (uint32_t y = y0; y < y1; y++) {
+            const uint8_t yf = d.y_flags[y];
+            if (yf & YF_SKIP) {
+                if (fl & SF_WATCHED) d.y_ca_after[y] = s.in[2];
+                continue;
+            }
+            const uint16_t cm = d.y_cleft_mask[y];
+#pragma unroll
+            for (int k = 0; k < 15; k++)
+                if (cm & (1u << k)) s.nt[k] += d.y_cleft[k * NS + y];
+            float isyn = 0.0f;
+            const uint32_t rc = d.y_rcount[y];
+            const float parea = d.y_post_area[y];
+            for (uint32_t j = 0; j < rc; j++) {
+                uint8_t rt = d.y_rtype[j * NS + y];
+                if (my && is_ionotropic(rt)) continue;
+                ModSlot m;
+                m.st = d.y_rstate + (size_t)j * 7 * NS + y;
+                m.stride = NS;
+                m.dens = d.y_rdens[j * NS + y];
+                m.a7 = parea;
+                m.av = d.y_rav[j * NS + y];
+                isyn += dispatch(rt, m, s, dt);
+            }
+            ibuf += isyn;
+            // spine Ca (modifier_helpers.h:808-816)
+            float sca = d.y_sca[y];
+            {
+                float ia = isyn < 0.0f ? -isyn : isyn;
+                float icaf = ia * 0.1f;
+                float influx = icaf * dt * 1000.0f / (2.0f * kFaraday * 0.05f * 100.0f);
+                sca += influx;
+                sca += (0.0001f - sca) * dt / 50.0f;
+                if (sca < 0.0f) sca = 0.0f;
+            }
+            // cleft clearance (simulation_host.cpp:505-510)
+            {
+                double E = bc::exp(-dt / d.y_tau[y]);
+#pragma unroll
+                for (int k = 0; k < 15; k++)
+                    if (cm & (1u << k)) {
+                        float v = d.y_cleft[k * NS + y];
+                        v *= E;
+                        if (v < 0.0f) v = 0.0f;
+                        d.y_cleft[k * NS + y] = v;
+                    }
+            }
+            // BCM threshold + spine CaMKII/calcineurin + weight (simulation_host.cpp:527-576)
+            float facil = d.y_fac[y];
+            float cu = sca * 1000.0f;
+            float csq = cu * cu;
+            float tau_theta = 300000.0f;
+            facil += (csq - facil) * dt / tau_theta;
+            if (facil < 0.0f) facil = 0.0f;
+            float dep = facil + 0.01f;
+            float ck = d.y_camkii[y], cn = d.y_calcn[y];
+            {
+                float c1 = sca * kMmToUm;
+                float c4 = c1 * c1 * c1 * c1;
+                float ck_inf = c4 / (c4 + 1.0f + 1e-9f);
+                ck += (ck_inf - ck) * dt / 2000.0f;
+                if (ck < 0.0f) ck = 0.0f;
+                if (ck > 1.0f) ck = 1.0f;
+                float c2 = c1 * c1;
+                float cn_inf = c2 / (c2 + 0.05f * 0.05f + 1e-9f);
+                cn += (cn_inf - cn) * dt / 300.0f;
+                if (cn < 0.0f) cn = 0.0f;
+                if (cn > 1.0f) cn = 1.0f;
+            }
+            float dw = 0.0f;
+            if (ck > 0.0f) dw += 0.0003f * ck * dt;
+            if (cn > 0.0f) dw -= 0.0001f * cn * pp1 * dt;
+            if (yf & YF_INHIBITORY) dw = -dw;
+            float w = d.y_weight[y] + dw;
+            if (w != w) w = 0.5f;
+            if (facil != facil) facil = 0.0f;
+            if (dep != dep) dep = 0.01f;
+            if (w < 0.01f) w = 0.01f;
+            if (w > 10.0f) w = 10.0f;
+            d.y_sca[y] = sca;
+            d.y_fac[y] = facil;
+            d.y_dep[y] = dep;
+            d.y_camkii[y] = ck;
+            d.y_calcn[y] = cn;
+            d.y_weight[y] = w;
+            if (fl & SF_WATCHED) d.y_ca_after[y] = s.in[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 7; k++) d.in[k * P + i] = s.in[k];
+#pragma unroll
+        for (int k = 0; k < 15; k++) d.nt[k * P + i] = s.nt[k];
+    } else {
+        if (fl & SF_BATH_NMDA) d.nt[0 * P + i] = 1.0f;
+        if (fl & SF_BATH_5HT) d.nt[3 * P + i] = 0.0f;
+    }
+    // ---- step 4 (simulation_host.cpp:596-649) ----
+    if (ibuf != ibuf) ibuf = 0.0f;
+    if (ibuf > 10000.0f) ibuf = 10000.0f;
+    if (ibuf < -10000.0f) ibuf = -10000.0f;
+    float cm = d.cap[i];
+    float prev = V;
+    float dV = ibuf / cm * dt;
+    if (dV > 50.0f) dV = 50.0f;
+    if (dV < -50.0f) dV = -50.0f;
+    V += dV;
+    if (prev <= -40.0f && V > -40.0f) d.spikes[i]++;
+    if (st) {
+        const uint32_t sid = d.seg_id[i];
+        for (uint32_t k = 0; k < st->vclamp_count; k++) {
+            const VoltageClamp q = st->vclamps[k];
+            if (q.segment_id == sid && tick >= q.start_tick && tick < q.end_tick) V = q.target_mV;
+        }
+    }
+    if (V != V) V = -65.0f;
+    else {
+        if (V < -120.0f) V = -120.0f;
+        if (V > 150.0f) V = 150.0f;
+    }
+    d.V[i] = V;
+}
+
+// ------------------------------------------------------------------ step 3, vesicle pool
+__global__ __launch_bounds__(256) void k_syn_rrp(Net d, uint64_t tick, float dt) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= d.ns) return;
+    if (d.y_flags[y] & YF_SKIP) return;
+    const float p = d.y_relp[y];
+    if (p < 0.0f) return;   // immediate release, finished in k_syn_pre
+    const size_t NS = d.ns;
+    float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
+    const uint32_t pre = d.y_pre[y];
+    if (!(rmax <= 0.0f || rrp >= rmax)) {   // rrp_do_recovery (kernel_dispatch.h:612-631)
+        int32_t src = d.y_src[y];
+        float ca = src >= 0 ? (float)d.y_ca_after[src] : (float)d.ca_snap[pre];
+        float caf = ca / (0.0005f + ca);
+        float prim = d.y_rrp_rec[y] + d.camkii[pre] * 0.02f + d.calcn[pre] * 0.01f;
+        prim *= caf;
+        float rec = (rmax - rrp) * prim * dt;
+        rrp += rec;
+        if (rrp > rmax) rrp = rmax;
+    }
+    if (p > 0.0f) {   // deferred release: rrp_do_release + enqueue (delay_buffer.h:47-58)
+        float amt;
+        float amount = d.y_amount[y];
+        if (rmax <= 0.0f) {
+            amt = amount * p;
+        } else {
+            float av = amount;
+            if (av > rrp) av = rrp;
+            amt = av * p;
+            rrp -= amt;
+            if (rrp < 0) rrp = 0;
+        }
+        uint8_t nt = d.y_rcount[y] > 0 ? nt_of_receptor(d.y_rtype[y]) : 0;
+        uint64_t arrival = tick + delay_ticks(d.y_delay[y], dt);
+        uint8_t mask = d.d_mask[y];
+        for (int s = 0; s < 4; s++)
+            if (!(mask & (1u << s))) {
+                d.d_arr[s * NS + y] = arrival;
+                d.d_amt[s * NS + y] = amt;
+                d.d_nt[s * NS + y] = nt;
+                d.d_mask[y] = mask | (1u << s);
+                break;
+            }
+    }
+    d.y_rrp[y] = rrp;
+}
+
+}  // namespace bc

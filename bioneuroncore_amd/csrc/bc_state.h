@@ -1,0 +1,130 @@
+// bc_state.h — device-resident network state (structure of arrays) and the per-launch views
+// the tick kernels receive by value.
+//
+// Segments are stored in "device order": connected islands (components of the
+// IntraConnection graph) are contiguous and packed into tiles of at most kTileSeg segments /
+// kTileConn connections; one workgroup owns one tile in the electrodiffusion kernel.
+// Synapses are stored in "slot order": sorted by (post segment in device order, synapse
+// index), so the ordered post-side walk of one segment reads a contiguous range.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+
+namespace bc {
+
+constexpr int kTileSeg = 256;
+constexpr int kTileConn = 256;
+constexpr int kNtCount = 15;
+
+// Segment flag bits.
+enum : uint8_t {
+    SF_MYELIN = 1,        // carries MOD_MYELIN: ionotropic + listed channels blocked
+    SF_BATH_NMDA = 2,     // carries an NMDA segment modifier: glutamate bath 1.0
+    SF_BATH_5HT = 4,      // carries a 5-HT7 segment modifier: serotonin bath 0.0
+    SF_WATCHED = 8,       // is the presynaptic segment of some synapse
+};
+
+// Synapse flag bits (static).
+enum : uint8_t {
+    YF_INHIBITORY = 1,    // a post receptor is GABA-A / GABA-B / glycine
+    YF_SKIP = 2,          // pre/post out of range or electrical: nothing runs
+};
+
+struct Net {
+    // sizes
+    uint32_t n;          // segments
+    uint32_t npad;       // stride of segment SoA arrays
+    uint32_t nslot;      // modifier slots per segment (max modifier_count)
+    uint32_t ns;         // synapses
+    uint32_t nrec;       // receptor slots per synapse (max post_receptor_count)
+    uint32_t ntiles;
+    uint32_t has_conn;   // any IntraConnection at all (step 2.5 runs)
+
+    // segment dynamic state
+    double* V;
+    double* in;          // [7][npad]
+    double* ex;          // [7][npad]
+    float* atp;
+    float* temp;
+    float* txn;
+    float* nt;           // neurotransmitter [15][npad]
+    float* nt_in;        // intra_nt [15][npad]
+    double* ibuf;
+    double* ca_snap;     // intra Ca at the start of step 3
+    uint32_t* spikes;
+
+    // segment static data
+    const uint32_t* seg_id;
+    const float* radius;
+    const float* length;
+    const uint8_t* stype;
+    const float* area;
+    const double* vol;
+    const float* cap;
+    const float* pkc;
+    const float* pka;
+    const float* camp;
+    const float* pp1;
+    const float* camkii;
+    const float* calcn;
+    const uint8_t* sflags;
+    const int8_t* cav21_slot;   // last Cav2.1 modifier slot, -1 if none
+
+    // modifiers [slot][npad] and states [slot][7][npad]
+    const uint8_t* mcount;
+    const uint8_t* mtype;
+    const uint8_t* mav;
+    const double* mdens;
+    double* mstate;
+
+    // electrodiffusion tiles
+    const uint32_t* tile_seg0;
+    const uint32_t* tile_nseg;
+    const uint32_t* tile_conn0;
+    const uint32_t* tile_nconn;
+    const uint32_t* conn_ab;     // (a_local | b_local << 16), tile-local, global connection order
+    const uint8_t* conn_intra;
+    const uint32_t* csr_off;     // [n+1] per device segment
+    const uint32_t* csr_ent;     // (tile-local connection << 1) | role (0 = a, 1 = b)
+
+    // synapses, slot order
+    const uint32_t* y_pre;       // device segment index
+    const uint32_t* y_id;
+    const uint8_t* y_flags;
+    const float* y_delay;
+    const float* y_pre_area;
+    const float* y_post_area;
+    const float* y_prel;
+    const float* y_amount;
+    const float* y_tau;
+    const float* y_rrp_max;
+    const float* y_rrp_rec;
+    const uint16_t* y_cleft_mask;
+    const int32_t* y_src;        // slot of the last earlier synapse whose post is my pre, -1
+    const uint32_t* post_off;    // [n+1] per device segment
+    uint8_t* y_rcount;
+    const uint8_t* y_rtype;      // [nrec][ns]
+    const uint8_t* y_rav;
+    const double* y_rdens;
+    double* y_rstate;            // [nrec][7][ns]
+    float* y_cleft;              // [15][ns]
+    float* y_weight;
+    float* y_fac;
+    float* y_dep;
+    uint8_t* y_refr;
+    float* y_rrp;
+    float* y_bca;
+    float* y_sca;
+    float* y_camkii;
+    float* y_calcn;
+    float* y_relp;               // release probability of a deferred release this tick, 0 if none
+    double* y_ca_after;          // post intra Ca after this synapse's receptors
+    uint8_t* d_mask;             // occupied delay slots (bit s)
+    uint64_t* d_arr;             // [4][ns]
+    float* d_amt;                // [4][ns]
+    uint8_t* d_nt;               // [4][ns]
+
+    float nt_diff[kNtCount];     // transmitter diffusion coefficients
+};
+
+}  // namespace bc

@@ -1,0 +1,88 @@
+// biocore_hip_backend.h — C ABI of libbiocore_hip_backend.so, the MI355X (gfx950) backend for
+// BioNeuronCore's per-tick simulation step.
+//
+// The first block is the drop-in contract: the same eleven symbols, argument meanings and error
+// behaviour as BioNeuronCore's GPU backends (src/gpu_backend_cuda.cu:316-390,
+// src/gpu_backend.cpp:40-144), resolved by callers with dlopen/dlsym
+// (src/daemon_server.cpp:198-206, tests/test_cpu_gpu_parity_cuda.cpp:45-70).
+// The second block is the plugin-registration surface plugins bind to
+// (src/plugin_manager.cpp:37-75, include/modifier_interface.h:38-39).
+// The third block ("bchip_*") is new: it has no counterpart in the reference.
+#pragma once
+#include <stdint.h>
+#include "biocore_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double (*ModifierFunc)(Modifier* mod, Segment* seg, float dt);
+typedef float (*ReleaseModulatorFunc)(Modifier* mod, float* cleft_nt, float dt);
+
+// ---------------------------------------------------------------- drop-in (reference ABI)
+// gpu_backend_create -> create_on_device(0)                     (gpu_backend_cuda.cu:330-332)
+void* gpu_backend_create(void);
+// Out-of-range index selects device 0; NULL when no device      (gpu_backend_cuda.cu:318-328)
+void* gpu_backend_create_on_device(int device_index);
+// NULL-safe                                                      (gpu_backend_cuda.cu:334-336)
+void gpu_backend_destroy(void* sim);
+// 0 ok; -1 NULL args; BION_ERR_* loader codes (bion_loader_host.cpp:14-24); v14 files only;
+// BCHIP_ERR_ISLAND_TOO_LARGE when a connected island exceeds one workgroup tile
+//                                                                (gpu_backend_cuda.cu:338-341)
+int gpu_backend_load_bion(void* sim, const char* path);
+// One tick (steps 1, 2.5, 3, 4); synchronous on return; NULL-safe (gpu_backend_cuda.cu:343-346)
+void gpu_backend_run_tick(void* sim, float dt);
+// Host-readable BionData (AoS), owned by sim; current as of the last tick (gpu_backend_cuda.cu:348-351)
+void* gpu_backend_bion_data(void* sim);
+// Deep copy; persists until the next upload; ion/ligand injections are one-shot
+//                                                                (gpu_backend_cuda.cu:353-356)
+void gpu_backend_upload_stimuli(void* sim, const GpuStimulusData* src);
+int gpu_backend_device_count(void);                             // (gpu_backend_cuda.cu:358-362)
+// 0 ok / -1 bad index or NULL info                               (gpu_backend_cuda.cu:364-382)
+int gpu_backend_get_device_info(int index, GpuDeviceInfo* info);
+int gpu_backend_ibuf_offset(void);   // 3128                      (gpu_backend_cuda.cu:386)
+int gpu_backend_vm_offset(void);     // 40                        (gpu_backend_cuda.cu:387)
+int gpu_backend_seg_size(void);      // 3216                      (gpu_backend_cuda.cu:388)
+
+// ---------------------------------------------------------------- plugin registration
+// IDs < 128; recorded for host-side lookups. Device ticks never call host function pointers
+// (as in the reference's device dispatch, kernel_dispatch.h:340-344).
+void register_modifier_current(uint8_t type, ModifierFunc func);     // plugin_manager.cpp:37-41
+void register_modulator_release(uint8_t type, ReleaseModulatorFunc func);   // plugin_manager.cpp:43-47
+
+// ---------------------------------------------------------------- extensions (no reference counterpart)
+#define BCHIP_COHERENCE_COMPAT 0   // AoS uploaded before and refreshed after every tick
+#define BCHIP_COHERENCE_FAST 1     // AoS refreshed only by gpu_backend_bion_data / bchip_save_bion;
+                                   // host edits reach the device only via bchip_push_host_state
+#define BCHIP_ERR_ISLAND_TOO_LARGE -20
+
+typedef struct BchipStats {
+    uint32_t num_segments, num_connections, num_synapses, num_tiles;
+    uint32_t modifier_slots, receptor_slots, lds_bytes_per_tile, _pad;
+    uint64_t unsupported_modifiers;   // modifier/receptor instances outside the device set (run as 0 current)
+    uint64_t unsupported_synapses;    // electrical synapses, autoreceptors, eCB (not run on device)
+    uint64_t order_hazard_synapses;   // immediate-release synapses whose pre segment is an earlier post
+    float last_phase_ms[5];           // step 1, 2.5, 3-pre, 3-post+4, 3-rrp of the last timed tick
+} BchipStats;
+
+int bchip_load_bion_data(void* sim, const BionData* src);   // load from in-memory AoS (copied)
+int bchip_save_bion(void* sim, const char* path);            // BION v14 save (bion_save layout)
+int bchip_set_coherence(void* sim, int mode);
+int bchip_push_host_state(void* sim);                        // upload AoS edits (FAST mode)
+int bchip_run_ticks(void* sim, float dt, uint32_t count);
+uint64_t bchip_tick_count(void* sim);
+int bchip_spike_counts(void* sim, uint32_t* out, uint32_t n);   // step-4 upward crossings of -40 mV
+int bchip_delay_slots(void* sim, uint8_t* out, uint64_t cap);   // 4 x DelayedRelease per synapse
+int bchip_stats(void* sim, BchipStats* out);
+int bchip_set_phase_timing(void* sim, int on);
+void* bchip_stream(void* sim);                                 // the sim's hipStream_t
+int bchip_plugin_load(const char* path);                       // dlopen + biocore_plugin_init
+int bchip_registered_modifier(uint8_t type);
+int bchip_modifier_supported(uint8_t type);
+
+#ifdef __cplusplus
+}
+// C++ registration entry points (plugin_manager.cpp:49-75).
+void register_user_modifier(uint8_t type, ModifierFunc func);
+void register_user_modulator(uint8_t type, ReleaseModulatorFunc func);
+#endif
