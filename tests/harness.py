@@ -175,6 +175,10 @@ def soma_clamp(net: bion.Network, amp_pA: float):
 
 def first_mismatch(a: np.ndarray, b: np.ndarray):
     """Byte-level comparison of two structured arrays; returns None or (index, field)."""
+    if len(a) != len(b):
+        return -1, "length", abs(len(a) - len(b))
+    if len(a) == 0:
+        return None
     ab = a.view(np.uint8).reshape(len(a), -1)
     bb = b.view(np.uint8).reshape(len(b), -1)
     diff = np.nonzero((ab != bb).any(axis=1))[0]
