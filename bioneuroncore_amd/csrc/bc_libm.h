@@ -298,7 +298,8 @@ BC_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki) {
     }
     sbits += 1022ull << 52;
     double scale = as_double(sbits);
-    double y = __builtin_fma(scale, tmp, scale);
+    double st = scale * tmp;
+    double y = scale + st;
     if (y < 1.0) {
         double lo = scale - y + scale * tmp;
         double hi = 1.0 + y;
