@@ -195,6 +195,8 @@ struct Sim {
     std::vector<void*> dmem;
     std::vector<uint32_t> dev2host, host2dev, slot2syn;
     uint32_t ntiles = 0;
+    uint32_t class_tile0[kTileClasses] = {}, class_ntiles[kTileClasses] = {};
+    TileSet tiles[kTileClasses] = {};
     size_t lds_bytes = 0;
     uint64_t tick = 0;
     uint64_t ticks_run = 0;
@@ -266,14 +268,26 @@ static int build_device(Sim& S) {
     std::vector<uint32_t> root(n), comp_size(n, 0), comp_conns(n, 0);
     for (uint32_t i = 0; i < n; i++) { root[i] = uf_find(par, i); comp_size[root[i]]++; }
     for (uint32_t i = 0; i < nc; i++) comp_conns[root[cn[i].segment_a]]++;
-    // components ordered by their smallest segment index (= root after min-union)
+    // size class of each island: the smallest tile that holds its segments and connections
+    std::vector<int> comp_class(n, -1);
+    std::vector<uint32_t> comps;
+    for (uint32_t i = 0; i < n; i++) {
+        if (root[i] != i) continue;
+        int cls = -1;
+        for (int k = 0; k < kTileClasses && cls < 0; k++)
+            if (comp_size[i] <= (uint32_t)kTileSize[k] && comp_conns[i] <= (uint32_t)kTileSize[k]) cls = k;
+        if (cls < 0) return BCHIP_ERR_ISLAND_TOO_LARGE;
+        comp_class[i] = cls;
+        comps.push_back(i);
+    }
+    // device order: islands by (size class, smallest segment index), segments in index order
+    std::stable_sort(comps.begin(), comps.end(), [&](uint32_t a, uint32_t b) { return comp_class[a] < comp_class[b]; });
     std::vector<uint32_t> comp_start(n, 0);
     S.dev2host.assign(n, 0);
     S.host2dev.assign(n, 0);
     {
         uint32_t pos = 0;
-        for (uint32_t i = 0; i < n; i++)
-            if (root[i] == i) { comp_start[i] = pos; pos += comp_size[i]; }
+        for (uint32_t r : comps) { comp_start[r] = pos; pos += comp_size[r]; }
         std::vector<uint32_t> fill(n, 0);
         for (uint32_t i = 0; i < n; i++) {
             uint32_t r = root[i];
@@ -283,19 +297,28 @@ static int build_device(Sim& S) {
         }
     }
     std::vector<uint32_t> tile_seg0, tile_nseg, tile_conn0, tile_nconn, seg_tile(n);
+    for (int k = 0; k < kTileClasses; k++) S.class_tile0[k] = S.class_ntiles[k] = 0;
     {
         uint32_t cur_s = 0, cur_c = 0, start = 0;
-        for (uint32_t i = 0; i < n; i++) {
-            if (root[i] != i) continue;
-            uint32_t cs = comp_size[i], cc = comp_conns[i];
-            if (cs > (uint32_t)kTileSeg || cc > (uint32_t)kTileConn) return BCHIP_ERR_ISLAND_TOO_LARGE;
-            if (cur_s + cs > (uint32_t)kTileSeg || cur_c + cc > (uint32_t)kTileConn) {
-                tile_seg0.push_back(start); tile_nseg.push_back(cur_s); tile_nconn.push_back(cur_c);
-                start += cur_s; cur_s = 0; cur_c = 0;
+        int cur_cls = -1;
+        auto flush = [&]() {
+            if (!cur_s) return;
+            tile_seg0.push_back(start); tile_nseg.push_back(cur_s); tile_nconn.push_back(cur_c);
+            S.class_ntiles[cur_cls]++;
+            start += cur_s; cur_s = 0; cur_c = 0;
+        };
+        for (uint32_t r : comps) {
+            int cls = comp_class[r];
+            uint32_t cap = kTileSize[cls];
+            if (cls != cur_cls) {
+                flush();
+                cur_cls = cls;
+                S.class_tile0[cls] = (uint32_t)tile_seg0.size();
             }
-            cur_s += cs; cur_c += cc;
+            if (cur_s + comp_size[r] > cap || cur_c + comp_conns[r] > cap) flush();
+            cur_s += comp_size[r]; cur_c += comp_conns[r];
         }
-        if (cur_s) { tile_seg0.push_back(start); tile_nseg.push_back(cur_s); tile_nconn.push_back(cur_c); }
+        flush();
     }
     S.ntiles = (uint32_t)tile_seg0.size();
     for (uint32_t t = 0; t < S.ntiles; t++)
@@ -390,6 +413,14 @@ static int build_device(Sim& S) {
     std::vector<uint32_t> post_off(n + 1, 0);
     for (uint32_t i = 0; i < ns; i++) post_off[S.host2dev[sy[i].post_segment] + 1]++;
     for (uint32_t i = 0; i < n; i++) post_off[i + 1] += post_off[i];
+    std::vector<uint32_t> walk(n);
+    std::iota(walk.begin(), walk.end(), 0u);
+    for (uint32_t w0 = 0; w0 < n; w0 += 256) {
+        uint32_t w1 = std::min<uint32_t>(n, w0 + 256);
+        std::stable_sort(walk.begin() + w0, walk.begin() + w1, [&](uint32_t a, uint32_t b) {
+            return post_off[a + 1] - post_off[a] > post_off[b + 1] - post_off[b];
+        });
+    }
     std::vector<int32_t> src(ns, -1);
     {
         std::vector<int64_t> last_onto(n, -1);
@@ -496,10 +527,16 @@ static int build_device(Sim& S) {
     d.mav = S.dupload(mav);
     d.mdens = S.dupload(mdens);
     d.mstate = S.dalloc<double>((size_t)nslot * 7 * npad);
-    d.tile_seg0 = S.dupload(tile_seg0);
-    d.tile_nseg = S.dupload(tile_nseg);
-    d.tile_conn0 = S.dupload(tile_conn0);
-    d.tile_nconn = S.dupload(tile_nconn);
+    {
+        const uint32_t* ts0 = S.dupload(tile_seg0);
+        const uint32_t* tns = S.dupload(tile_nseg);
+        const uint32_t* tc0 = S.dupload(tile_conn0);
+        const uint32_t* tnc = S.dupload(tile_nconn);
+        for (int k = 0; k < kTileClasses; k++) {
+            uint32_t o = S.class_tile0[k];
+            S.tiles[k] = TileSet{ts0 + o, tns + o, tc0 + o, tnc + o};
+        }
+    }
     d.conn_ab = S.dupload(conn_ab);
     d.conn_intra = S.dupload(conn_intra);
     d.csr_off = S.dupload(csr_off);
@@ -518,6 +555,7 @@ static int build_device(Sim& S) {
     d.y_cleft_mask = S.dupload(y_mask);
     d.y_src = S.dupload(src);
     d.post_off = S.dupload(post_off);
+    d.walk = S.dupload(walk);
     d.y_rcount = S.dupload(y_rcount);
     d.y_rtype = S.dupload(y_rtype);
     d.y_rav = S.dupload(y_rav);
@@ -539,7 +577,7 @@ static int build_device(Sim& S) {
     d.d_arr = S.dalloc<uint64_t>((size_t)4 * ns);
     d.d_amt = S.dalloc<float>((size_t)4 * ns);
     d.d_nt = S.dalloc<uint8_t>((size_t)4 * ns);
-    S.lds_bytes = sizeof(DiffLds);
+    S.lds_bytes = sizeof(DiffLds<256>);
     S.seg_dispatched.assign((size_t)n * 32, 0);
     return 0;
 }
@@ -663,7 +701,11 @@ static void run_tick(Sim& S, float dt) {
     if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
     k_modifiers<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
-    if (d.has_conn) k_diffusion<<<S.ntiles, 256, S.lds_bytes, S.stream>>>(d, dt);
+    if (d.has_conn) {
+        if (S.class_ntiles[0]) k_diffusion<64><<<S.class_ntiles[0], 64, sizeof(DiffLds<64>), S.stream>>>(d, S.tiles[0], dt);
+        if (S.class_ntiles[1]) k_diffusion<128><<<S.class_ntiles[1], 128, sizeof(DiffLds<128>), S.stream>>>(d, S.tiles[1], dt);
+        if (S.class_ntiles[2]) k_diffusion<256><<<S.class_ntiles[2], 256, sizeof(DiffLds<256>), S.stream>>>(d, S.tiles[2], dt);
+    }
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
         HIPCHK(hipMemcpyAsync(d.ca_snap, d.in + 2 * (size_t)d.npad, (size_t)d.n * 8, hipMemcpyDeviceToDevice, S.stream));

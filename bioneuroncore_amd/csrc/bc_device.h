@@ -8,7 +8,7 @@
 #include <stdint.h>
 #include "bc_libm.h"
 
-#define BC_DEV __device__ static inline
+#define BC_DEV __device__ static inline __attribute__((always_inline))
 
 namespace bc {
 
@@ -55,9 +55,44 @@ struct SegRegs {
     float atp;
     float area;    // membrane area: sphere 4*pi*r^2 or cylinder 2*pi*r*L (float)
     double vol;    // volume (double, run_tick_common.h:260-264)
+    double cpm;    // F * vol / 1000, the charge-per-mole factor every current divides by
     float pkc, pka, camp;   // cascade activities read by Nav/Kv3.1/Kv7/HCN (constant on device)
     float nt[15];  // neurotransmitter (used by receptors)
+    // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
+    // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
+    // every GHK call reuses these instead of recomputing the same exp.
+    double vkf, vkd;        // keys
+    double emf, edf, emd, edd;
+    double vkmg, emg;       // exp(-0.062 V) of the NMDA Mg2+ block, keyed on V
 };
+
+BC_DEV double vn_mono(double V) { return kFaraday * V * 0.001 / (kGas * kTemp); }
+BC_DEV double vn_di(double V) { return 2.0 * kFaraday * V * 0.001 / (kGas * kTemp); }
+BC_DEV void seg_consts_init(SegRegs& s) { s.cpm = kFaraday * s.vol / 1000.0; }
+BC_DEV void ghk_cache_init(SegRegs& s) {
+    double vf = (float)s.V;
+    s.vkf = vf;
+    s.vkd = s.V;
+    s.emf = bc::exp(-vn_mono(vf));
+    s.edf = bc::exp(-vn_di(vf));
+    s.emd = bc::exp(-vn_mono(s.V));
+    s.edd = bc::exp(-vn_di(s.V));
+    s.vkmg = __builtin_nan("");
+}
+BC_DEV void mg_cache_init(SegRegs& s) {
+    s.vkmg = s.V;
+    s.emg = bc::exp(-0.062f * s.V);
+}
+BC_DEV double exp_mono(const SegRegs& s, double V, double vn) {
+    if (V == s.vkf) return s.emf;
+    if (V == s.vkd) return s.emd;
+    return bc::exp(-vn);
+}
+BC_DEV double exp_di(const SegRegs& s, double V, double vn) {
+    if (V == s.vkf) return s.edf;
+    if (V == s.vkd) return s.edd;
+    return bc::exp(-vn);
+}
 
 // One modifier slot: states 0..6 live in SoA memory; state[7] (effective area) is a register.
 struct ModSlot {
@@ -78,17 +113,26 @@ BC_DEV void floor7(double* a) {
 }
 
 // ---- GHK (run_tick_common.h:274-312, modifier_helpers.h:48-75) ----
-BC_DEV double ghk_mono(double V, double ci, double co) {
-    double vn = kFaraday * V * 0.001 / (kGas * kTemp);
+BC_DEV double ghk_mono(const SegRegs& s, double V, double ci, double co) {
+    double vn = vn_mono(V);
     if (fabs(vn) < 1e-6) return kFaraday * (ci - co);
     double k = kFaraday / (kGas * kTemp);
     if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
     if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
-    double e = bc::exp(-vn);
+    double e = exp_mono(s, V, vn);
     return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
 }
-BC_DEV double ghk_di(double V, double ci, double co) {
-    double vn = 2.0 * kFaraday * V * 0.001 / (kGas * kTemp);
+BC_DEV double ghk_di(const SegRegs& s, double V, double ci, double co) {
+    double vn = vn_di(V);
+    if (fabs(vn) < 1e-6) return 2.0 * kFaraday * (ci - co);
+    double k = 4.0 * kFaraday / (kGas * kTemp);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
+    double e = exp_di(s, V, vn);
+    return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
+}
+BC_DEV double ghk_di_nocache(double V, double ci, double co) {
+    double vn = vn_di(V);
     if (fabs(vn) < 1e-6) return 2.0 * kFaraday * (ci - co);
     double k = 4.0 * kFaraday / (kGas * kTemp);
     if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
@@ -96,33 +140,32 @@ BC_DEV double ghk_di(double V, double ci, double co) {
     double e = bc::exp(-vn);
     return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
 }
-BC_DEV double ghk_an(double V, double ci, double co) {
-    double vn = kFaraday * V * 0.001 / (kGas * kTemp);
+BC_DEV double ghk_an(const SegRegs& s, double V, double ci, double co) {
+    double vn = vn_mono(V);
     if (fabs(vn) < 1e-6) return kFaraday * (co - ci);
     double k = kFaraday / (kGas * kTemp);
     if (vn > 20.0) return k * kFaraday * V * 0.001 * co;
     if (vn < -20.0) return k * kFaraday * V * 0.001 * ci;
-    double e = bc::exp(-vn);
+    double e = exp_mono(s, V, vn);
     return k * kFaraday * V * 0.001 * (co - ci * e) / (1.0 - e);
 }
-BC_DEV double p_single(float gamma_ps, float z, float cbar) {
-    double g = gamma_ps * kPsToS;
-    double z2f2 = z * z * kFaraday * kFaraday;
-    double rt = kGas * kTemp;
-    return g * rt / (z2f2 * cbar);
+__host__ __device__ constexpr double p_single(float gamma_ps, float z, float cbar) {
+    // evaluated at compile time for constant gamma: clang folds each IEEE operation exactly
+    return (double)(gamma_ps * kPsToS) * (double)(kGas * kTemp) / ((double)(z * z) * kFaraday * kFaraday * cbar);
 }
-BC_DEV void p_partial(float gamma_ps, float fna, float fk, float fca, double* pna, double* pk, double* pca) {
+struct P3 { double na, k, ca; };
+__host__ __device__ constexpr P3 p_partial3(float gamma_ps, float fna, float fk, float fca) {
     double w = fna * kCbarNa + fk * kCbarK + fca * 4.0 * kCbarCa;
-    *pna = p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa);
-    *pk = p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK);
-    *pca = p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa);
+    return P3{p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa),
+              p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK),
+              p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa)};
 }
-BC_DEV void p_nalcn(float gamma_ps, double* pna, double* pk, double* pca) {
+__host__ __device__ constexpr P3 p_nalcn3(float gamma_ps) {
     const double fna = 1.3, fk = 1.2, fca = 0.5;
     double w = fna * kCbarNa + fk * kCbarK + fca * 4.0 * kCbarCa;
-    *pna = p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa);
-    *pk = p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK);
-    *pca = p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa);
+    return P3{p_single((float)(gamma_ps * fna * kCbarNa / w), 1.0f, kCbarNa),
+              p_single((float)(gamma_ps * fk * kCbarK / w), 1.0f, kCbarK),
+              p_single((float)(gamma_ps * fca * 4.0 * kCbarCa / w), 2.0f, kCbarCa)};
 }
 
 // ---- gates (modifier_helpers.h:132-271, 909-1013) ----
@@ -246,10 +289,10 @@ BC_DEV double ch_nav(const ModSlot& m, SegRegs& s, float dt, const NavK& g, floa
     op *= (1.0f - pkx * (s.pkc + s.pka) * 0.5f);
     if (op < 0.0f) op = 0.0f;
     double P = p_single(gamma, 1.0f, kCbarNa);
-    double flux = ghk_mono(V, s.in[0], s.ex[0]);
+    double flux = ghk_mono(s, V, s.in[0], s.ex[0]);
     double ia = P * flux * (m.dens * area) * op;
     double ina = ia * kAtoPa;
-    double cpv = kFaraday * s.vol / 1000.0;
+    double cpv = s.cpm;
     s.in[0] -= (ina * dt) / cpv;
     s.ex[0] += (ina * dt) / (cpv * kEvr);
     floor7(s.in);
@@ -275,7 +318,7 @@ BC_DEV double ch_cav21(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologi
 }
 
 BC_DEV double k_out(SegRegs& s, double ik, float dt) {
-    double cpv = kFaraday * s.vol / 1000.0;
+    double cpv = s.cpm;
     s.in[1] -= (ik * dt) / cpv;
     s.ex[1] += (ik * dt) / (cpv * kEvr);
     floor7(s.ex);
@@ -309,7 +352,7 @@ BC_DEV double ch_kv31(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologic
     m.set(0, s0);
     m.set(1, s1);
     m.set(2, s2);
-    double flux = ghk_mono(V, ki, ko);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * op;
     return k_out(s, ia * kAtoPa, dt);
 }
@@ -337,7 +380,7 @@ BC_DEV double ch_kv11(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologic
     op /= (1.0f + dtx / 2.0f) * (1.0f + tea / 170.0f);
     float ki = s.in[1], ko = s.ex[1];
     double P = p_single(gamma, 1.0f, kCbarK);
-    double flux = ghk_mono(V, ki, ko);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * op;
     return k_out(s, ia * kAtoPa, dt);
 }
@@ -356,10 +399,10 @@ BC_DEV double ch_kvdr(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologic
     m.set(0, mv);
     float ki = s.in[1], ko = s.ex[1];
     double P = p_single(12.0f, 1.0f, kCbarK);
-    double flux = ghk_mono(V, ki, ko);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * mv;
     double ik = ia * kAtoPa;
-    float cpmv = kFaraday * s.vol / 1000.0;
+    float cpmv = s.cpm;
     s.in[1] -= (ik * dt) / cpmv;
     s.ex[1] += (ik * dt) / (cpmv * kEvr);
     floor7(s.in);
@@ -382,11 +425,11 @@ BC_DEV double ch_sk(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological
     if (op > 1.0f) op = 1.0f;
     m.set(0, op);
     float ki = s.in[1], ko = s.ex[1];
-    double P = p_single(gamma, 1.0f, kCbarK);
-    double flux = ghk_mono(V, ki, ko);
+    double P = (s3 > 0.0f) ? p_single(gamma, 1.0f, kCbarK) : p_single(10.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * op;
     float ik = (float)(ia * kAtoPa);
-    double cpv = kFaraday * s.vol / 1000.0f;
+    double cpv = s.cpm;
     s.in[1] -= (ik * dt) / cpv;
     s.ex[1] += (ik * dt) / (cpv * kEvr);
     floor7(s.ex);
@@ -406,11 +449,11 @@ BC_DEV double ch_bk(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological
     if (op > 1.0f) op = 1.0f;
     m.set(0, op);
     float ki = s.in[1], ko = s.ex[1];
-    double P = p_single(gamma, 1.0f, kCbarK);
-    double flux = ghk_mono(V, ki, ko);
+    double P = (s1 > 0.0f) ? p_single(gamma, 1.0f, kCbarK) : p_single(250.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * op;
     float ik = (float)(ia * kAtoPa);
-    double cpv = kFaraday * s.vol / 1000.0f;
+    double cpv = s.cpm;
     s.in[1] -= (ik * dt) / cpv;
     s.ex[1] += (ik * dt) / (cpv * kEvr);
     floor7(s.ex);
@@ -424,8 +467,8 @@ BC_DEV double ch_k2p(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologica
     double gamma = (s0 > 0.0f) ? s0 : 100.0f;
     double po = (s1 > 0.0f) ? s1 : 0.012f;
     float ki = s.in[1], ko = s.ex[1];
-    double P = p_single(gamma, 1.0f, kCbarK);
-    double flux = ghk_mono(V, ki, ko);
+    double P = (s0 > 0.0f) ? p_single(gamma, 1.0f, kCbarK) : p_single(100.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * po;
     return k_out(s, ia * kAtoPa, dt);
 }
@@ -451,7 +494,7 @@ BC_DEV double ch_kv7(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologica
     double P = p_single(7.0f, 1.0f, kCbarK);
     float sup = 1.0f - 0.8f * s.pkc;
     if (sup < 0.0f) sup = 0.0f;
-    double flux = ghk_mono(V, ki, ko);
+    double flux = ghk_mono(s, V, ki, ko);
     double ia = P * flux * (m.dens * area) * (mv * pip2 * sup);
     return k_out(s, ia * kAtoPa, dt);
 }
@@ -475,14 +518,14 @@ BC_DEV double ch_hcn1(const ModSlot& m, SegRegs& s, float dt) {   // modifier_he
     if (mv < 0.0f) mv = 0.0f;
     if (mv > 1.0f) mv = 1.0f;
     m.set(0, mv);
-    double pna, pk, pca;
-    p_partial(gamma, 0.2f, 0.8f, 0.0f, &pna, &pk, &pca);
+    const P3 pp = (s3 > 0) ? p_partial3(gamma, 0.2f, 0.8f, 0.0f) : p_partial3(0.84f, 0.2f, 0.8f, 0.0f);
+    double pna = pp.na, pk = pp.k;
     double N = m.dens * area;
-    double fna = ghk_mono(V, s.in[0], s.ex[0]);
-    double fk = ghk_mono(V, s.in[1], s.ex[1]);
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double fk = ghk_mono(s, V, s.in[1], s.ex[1]);
     double in = pna * fna * N * mv * kAtoPa;
     double ik = pk * fk * N * mv * kAtoPa;
-    double cpm = kFaraday * s.vol / 1000.0;
+    double cpm = s.cpm;
     s.in[0] -= (in * dt) / cpm;
     s.in[1] -= (ik * dt) / cpm;
     s.ex[0] += (in * dt) / (cpm * kEvr);
@@ -497,13 +540,13 @@ BC_DEV double ch_nalcn(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologi
     float area = eff_area(m, s);
     double s0 = m.get(0);
     double gamma = (s0 > 0.0f) ? s0 : 27.0f;
-    double pna, pk, pca;
-    p_nalcn(gamma, &pna, &pk, &pca);
+    const P3 pp = (s0 > 0.0f) ? p_nalcn3(gamma) : p_nalcn3(27.0f);
+    double pna = pp.na, pk = pp.k, pca = pp.ca;
     double N = m.dens * area;
-    float ina = (float)(pna * ghk_mono(V, s.in[0], s.ex[0]) * N * kAtoPa);
-    float ik = (float)(pk * ghk_mono(V, s.in[1], s.ex[1]) * N * kAtoPa);
-    float ica = (float)(pca * ghk_di(V, s.in[2], s.ex[2]) * N * kAtoPa);
-    double cpm = kFaraday * s.vol / 1000.0;
+    float ina = (float)(pna * ghk_mono(s, V, s.in[0], s.ex[0]) * N * kAtoPa);
+    float ik = (float)(pk * ghk_mono(s, V, s.in[1], s.ex[1]) * N * kAtoPa);
+    float ica = (float)(pca * ghk_di(s, V, s.in[2], s.ex[2]) * N * kAtoPa);
+    double cpm = s.cpm;
     s.in[0] -= (ina * dt) / cpm;
     s.in[1] -= (ik * dt) / cpm;
     s.in[2] -= (ica * dt) / (2.0f * cpm);
@@ -521,7 +564,7 @@ struct IonoK { float gamma, ec50, t_rise, t_decay; int nt; float pna, pk, pca; b
 BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     float area = eff_area(m, s);
     double V = s.V;
-    double cpm = kFaraday * s.vol / 1000.0;
+    double cpm = s.cpm;
     float gamma = p.gamma;
     if (m.av > 0 && m.av < 7) {
         double g = m.get(m.av);
@@ -550,22 +593,23 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     double N = m.dens * area;
     const float scale = 1.0f;
     if (p.anion) {
-        double pcl = p_single(gamma, 1.0f, kCbarCl);
-        double fcl = ghk_an(V, s.in[3], s.ex[3]);
+        double pcl = (gamma == p.gamma) ? p_single(p.gamma, 1.0f, kCbarCl) : p_single(gamma, 1.0f, kCbarCl);
+        double fcl = ghk_an(s, V, s.in[3], s.ex[3]);
         double icl = pcl * fcl * N * op * scale * kAtoPa;
         s.in[3] += (icl * dt) / cpm;
         floor7(s.in);
         return -icl;
     }
-    double pna, pk, pca;
-    p_partial(gamma, p.pna, p.pk, p.pca, &pna, &pk, &pca);
-    double ina = pna * ghk_mono(V, s.in[0], s.ex[0]) * N * op * scale * kAtoPa;
-    double ik = pk * ghk_mono(V, s.in[1], s.ex[1]) * N * op * scale * kAtoPa;
+    const P3 pp = (gamma == p.gamma) ? p_partial3(p.gamma, p.pna, p.pk, p.pca)
+                                     : p_partial3(gamma, p.pna, p.pk, p.pca);
+    double pna = pp.na, pk = pp.k, pca = pp.ca;
+    double ina = pna * ghk_mono(s, V, s.in[0], s.ex[0]) * N * op * scale * kAtoPa;
+    double ik = pk * ghk_mono(s, V, s.in[1], s.ex[1]) * N * op * scale * kAtoPa;
     double itot = ina + ik;
     s.in[0] -= (ina * dt) / cpm;
     s.in[1] -= (ik * dt) / cpm;
     if (p.pca > 0.0f) {
-        double ica = pca * ghk_di(V, s.in[2], s.ex[2]) * N * op * scale * kAtoPa;
+        double ica = pca * ghk_di(s, V, s.in[2], s.ex[2]) * N * op * scale * kAtoPa;
         itot += ica;
         s.in[2] -= (ica * dt) / (2.0f * cpm);
     }
@@ -576,7 +620,8 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
 BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_biological.cpp:48-55
     const IonoK k = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
     double I = iono(m, s, dt, k);
-    float mg = 1.0f / (1.0f + (1.0f / 3.57f) * bc::exp(-0.062f * s.V));
+    double emg = (s.V == s.vkmg) ? s.emg : bc::exp(-0.062f * s.V);
+    float mg = 1.0f / (1.0f + (1.0f / 3.57f) * emg);
     double r = I * mg;
     // nmda_cdi_coagonist (modifier_helpers.h:1281-1291)
     double ca_um = s.in[2] * kMmToUm;

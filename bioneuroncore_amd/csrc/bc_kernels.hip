@@ -42,6 +42,7 @@ __device__ static inline uint64_t delay_ticks(float delay, float dt) {   // dela
 // ------------------------------------------------------------------ step 1
 __global__ __launch_bounds__(256) void k_modifiers(Net d, const GpuStimulusData* __restrict__ st,
                                                    uint64_t tick, float dt) {
+    exp_table_stage();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n) return;
     const size_t P = d.npad;
@@ -58,6 +59,8 @@ __global__ __launch_bounds__(256) void k_modifiers(Net d, const GpuStimulusData*
     s.pkc = d.pkc[i];
     s.pka = d.pka[i];
     s.camp = d.camp[i];
+    seg_consts_init(s);
+    ghk_cache_init(s);
     const bool my = d.sflags[i] & SF_MYELIN;
     const uint32_t cnt = d.mcount[i];
     double itot = 0.0;
@@ -94,8 +97,13 @@ __global__ __launch_bounds__(256) void k_modifiers(Net d, const GpuStimulusData*
         for (uint32_t k = 0; k < st->ion_inj_count; k++) {
             const IonInjection q = st->ion_inj[k];
             if (q.segment_id == sid && q.ion_idx >= 0 && q.ion_idx < 7) {
-                if (q.extracellular) s.ex[q.ion_idx] += q.amount_mM;
-                else s.in[q.ion_idx] += q.amount_mM;
+                // register arrays: index with compile-time constants only
+#pragma unroll
+                for (int k = 0; k < 7; k++)
+                    if (k == q.ion_idx) {
+                        if (q.extracellular) s.ex[k] += q.amount_mM;
+                        else s.in[k] += q.amount_mM;
+                    }
             }
         }
         for (uint32_t k = 0; k < st->lig_inj_count; k++) {
@@ -117,105 +125,129 @@ __global__ __launch_bounds__(256) void k_modifiers(Net d, const GpuStimulusData*
 }
 
 // ------------------------------------------------------------------ step 2.5
-// LDS layout of one tile (kTileSeg segments, kTileConn connections).
+// One workgroup of TS lanes owns one tile: whole islands, at most TS segments and TS
+// connections, every per-segment and per-connection quantity of the step held in LDS.
+// Lane t owns segment t and connection t.
+//
+// Valences are +1 (Na, K, H), -1 (Cl, A) and +2 (Ca, Mg): x = z*F/RT*dV takes three values
+// per connection and B(-x) for z = -1 is B(x) for z = +1 bit for bit, so four Bernoulli
+// evaluations serve all seven ions.
+template <int TS>
 struct DiffLds {
-    double V[kTileSeg];
-    double phi[2][kTileSeg];
-    double sig[kTileSeg];
-    double in[7][kTileSeg];
-    double ex[7][kTileSeg];
-    float vf[2][kTileSeg];
-    float volf[kTileSeg];
-    double cflux[7][kTileConn];   // intra fluxes (as float) / extra fluxes / phi conductances
-    float cg[kTileConn];
-    float cterm[kTileConn];
-    float cac[kTileConn];
-    float cdist[kTileConn];
-    uint32_t cab[kTileConn];
-    uint8_t cintra[kTileConn];
+    double V[TS];
+    double phi[2][TS];
+    double ion[7][TS];     // intracellular ions (intra phase), then extracellular (extra phase)
+    double cbuf[7][TS];    // phi conductances / intra fluxes (float) / extra fluxes
+    float vf[2][TS];
+    float cg[TS];
+    float cterm[TS];
+    float cac[TS];
+    float cdist[TS];
+    uint32_t cab[TS];      // a_local | b_local << 16
+    uint32_t ent[2 * TS];  // other_local << 16 | conn << 1 | role, per segment in connection order
+    uint8_t cintra[TS];
 };
 
-__global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
+struct BPair { float m, p; };   // B(-x), B(x)
+
+__device__ static inline void bern4(float dV_or_phi_x1, float x2, BPair& b1, BPair& b2) {
+    b1.m = bernoulli(-dV_or_phi_x1);
+    b1.p = bernoulli(dV_or_phi_x1);
+    b2.m = bernoulli(-x2);
+    b2.p = bernoulli(x2);
+}
+// (B(-x), B(x)) of ion k given the +1 and +2 pairs.
+__device__ static inline BPair bsel(int k, const BPair& b1, const BPair& b2) {
+    if (k == 2 || k == 4) return b2;
+    if (k == 3 || k == 5) return BPair{b1.p, b1.m};
+    return b1;
+}
+
+template <int TS>
+__global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    DiffLds& L = *reinterpret_cast<DiffLds*>(lds_raw);
+    DiffLds<TS>& L = *reinterpret_cast<DiffLds<TS>*>(lds_raw);
+    exp_table_stage();
     const uint32_t tile = blockIdx.x;
-    const uint32_t s0 = d.tile_seg0[tile];
-    const uint32_t nseg = d.tile_nseg[tile];
-    const uint32_t c0 = d.tile_conn0[tile];
-    const uint32_t nconn = d.tile_nconn[tile];
+    const uint32_t s0 = ts.seg0[tile];
+    const uint32_t nseg = ts.nseg[tile];
+    const uint32_t c0 = ts.conn0[tile];
+    const uint32_t nconn = ts.nconn[tile];
     const uint32_t t = threadIdx.x;
     const bool act = t < nseg;
+    const bool cact = t < nconn;
     const uint32_t g = s0 + t;
     const size_t P = d.npad;
+    const uint32_t E0 = d.csr_off[s0];
+    const uint32_t nent = d.csr_off[s0 + nseg] - E0;
 
-    // per-segment registers
-    double ibuf = 0.0, diag = 0.0;
+    double ibuf = 0.0, diag = 0.0, sg = 0.0;
     float area = 0.0f, cap = 0.0f, volf = 0.0f;
     uint32_t e0 = 0, e1 = 0;
     if (act) {
         L.V[t] = d.V[g];
 #pragma unroll
         for (int k = 0; k < 7; k++) {
-            L.in[k][t] = d.in[k * P + g];
-            L.ex[k][t] = d.ex[k * P + g];
+            L.ion[k][t] = d.in[k * P + g];
+            sg += kZ[k] * kZ[k] * kDiff[k] * d.ex[k * P + g];
         }
         ibuf = d.ibuf[g];
         area = d.area[g];
         cap = d.cap[g];
         volf = (float)d.vol[g];
         if (volf < 1e-6f) volf = 1e-6f;
-        L.volf[t] = volf;
-        e0 = d.csr_off[g];
-        e1 = d.csr_off[g + 1];
+        e0 = d.csr_off[g] - E0;
+        e1 = d.csr_off[g + 1] - E0;
     }
-    for (uint32_t c = t; c < nconn; c += blockDim.x) {
-        uint32_t ab = d.conn_ab[c0 + c];
-        L.cab[c] = ab;
-        L.cintra[c] = d.conn_intra[c0 + c];
-    }
-    __syncthreads();
-    // connection geometry (diffusion_kernel.h:72-80,162-172)
-    for (uint32_t c = t; c < nconn; c += blockDim.x) {
-        uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
-        float ra = d.radius[s0 + a], rb = d.radius[s0 + b];
+    uint32_t ca_ = 0, cb_ = 0;
+    float cac = 0.0f, cdist = 0.0f;
+    bool cintra = false;
+    if (cact) {
+        uint32_t ab = d.conn_ab[c0 + t];
+        L.cab[t] = ab;
+        cintra = d.conn_intra[c0 + t] != 0;
+        L.cintra[t] = cintra;
+        // connection geometry (diffusion_kernel.h:72-80,162-172)
+        ca_ = ab & 0xFFFF;
+        cb_ = ab >> 16;
+        float ra = d.radius[s0 + ca_], rb = d.radius[s0 + cb_];
         float rmin = ra < rb ? ra : rb;
-        float ac = kPi * rmin * rmin;
-        if (ac < 1e-6f) ac = 1e-6f;
-        float dist = (d.length[s0 + a] + d.length[s0 + b]) * 0.5f;
-        if (dist < 0.1f) dist = 0.1f;
-        L.cac[c] = ac;
-        L.cdist[c] = dist;
+        cac = kPi * rmin * rmin;
+        if (cac < 1e-6f) cac = 1e-6f;
+        cdist = (d.length[s0 + ca_] + d.length[s0 + cb_]) * 0.5f;
+        if (cdist < 0.1f) cdist = 0.1f;
+        L.cac[t] = cac;
+        L.cdist[t] = cdist;
     }
     // ---- phi_e (diffusion_kernel.h:41-104) ----
+    double* sig = &L.cbuf[6][0];
     if (act) {
-        double sg = 0.0;
-#pragma unroll
-        for (int k = 0; k < 7; k++) sg += kZ[k] * kZ[k] * kDiff[k] * L.ex[k][t];
         sg = kSigmaE * sg;
-        L.sig[t] = sg;
+        sig[t] = sg;
         diag = sg * area / 1.0f;
         L.phi[0][t] = 0.0;
     }
     __syncthreads();
-    double* gphi = &L.cflux[0][0];
-    for (uint32_t c = t; c < nconn; c += blockDim.x) {
-        uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
-        gphi[c] = 0.5 * (L.sig[a] + L.sig[b]) * L.cac[c] / L.cdist[c];
+    for (uint32_t e = t; e < nent; e += TS) {
+        uint32_t en = d.csr_ent[E0 + e];
+        uint32_t c = en >> 1;
+        uint32_t o = (en & 1) ? (L.cab[c] & 0xFFFF) : (L.cab[c] >> 16);
+        L.ent[e] = (o << 16) | en;
     }
+    double* gphi = &L.cbuf[0][0];
+    if (cact) gphi[t] = 0.5 * (sig[ca_] + sig[cb_]) * cac / cdist;
     __syncthreads();
     if (act)
-        for (uint32_t e = e0; e < e1; e++) diag += gphi[d.csr_ent[e] >> 1];
+        for (uint32_t e = e0; e < e1; e++) diag += gphi[(L.ent[e] & 0xFFFF) >> 1];
     int cur = 0;
     for (int it = 0; it < 50; it++) {
         if (act) {
             double pn = 0.0;
             for (uint32_t e = e0; e < e1; e++) {
-                uint32_t ent = d.csr_ent[e];
-                uint32_t c = ent >> 1;
-                double gg = gphi[c];
+                uint32_t en = L.ent[e];
+                double gg = gphi[(en & 0xFFFF) >> 1];
                 if (gg <= 0.0) continue;
-                uint32_t o = (ent & 1) ? (L.cab[c] & 0xFFFF) : (L.cab[c] >> 16);
-                pn += gg * L.phi[cur][o];
+                pn += gg * L.phi[cur][en >> 16];
             }
             double ph = (diag > 1e-9) ? (pn - ibuf) / diag : 0.0;
             if (ph > 10.0) ph = 10.0;
@@ -232,31 +264,34 @@ __global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
     const float dts = dt / kSub;
     float cdt = cap / dts;
     if (cdt < 1e-9f) cdt = 1e-9f;
-    float* cfin = reinterpret_cast<float*>(&L.cflux[0][0]);   // [7][kTileConn] floats
+    float* cfin = reinterpret_cast<float*>(&L.cbuf[0][0]);   // [7][TS] floats
+    float kin[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) kin[k] = kDiff[k] * cac * dts / cdist;
     for (int sub = 0; sub < kSub; sub++) {
-        for (uint32_t c = t; c < nconn; c += blockDim.x) {
-            if (!L.cintra[c]) continue;
-            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
-            float ac = L.cac[c], dist = L.cdist[c];
-            float sg = 0.0f;
+        if (cact && cintra) {
+            const uint32_t a = ca_, b = cb_;
+            float sgi = 0.0f;
 #pragma unroll
             for (int k = 0; k < 7; k++) {
-                float cbar = 0.5f * (L.in[k][a] + L.in[k][b]);
-                sg += kZ[k] * kZ[k] * kDiff[k] * cbar;
+                float cbar = 0.5f * (L.ion[k][a] + L.ion[k][b]);
+                sgi += kZ[k] * kZ[k] * kDiff[k] * cbar;
             }
-            float gax = kSigmaE * sg * ac / dist;
-            L.cg[c] = gax;
+            float gax = kSigmaE * sgi * cac / cdist;
+            L.cg[t] = gax;
             float dV = (L.V[a] + phi[a]) - (L.V[b] + phi[b]);
+            BPair b1, b2;
+            bern4(1.0f * kFrtMv * dV, 2.0f * kFrtMv * dV, b1, b2);
             float inp = 0.0f;
 #pragma unroll
             for (int k = 0; k < 7; k++) {
-                float ca = L.in[k][a], cb = L.in[k][b];
-                float x = kZ[k] * kFrtMv * dV;
-                float fl = kDiff[k] * ac * dts / dist * (ca * bernoulli(-x) - cb * bernoulli(x));
-                cfin[k * kTileConn + c] = fl;
+                float ca = L.ion[k][a], cb = L.ion[k][b];
+                BPair bb = bsel(k, b1, b2);
+                float fl = kin[k] * (ca * bb.m - cb * bb.p);
+                cfin[k * TS + t] = fl;
                 inp += kZ[k] * kChargeToPa * fl / dts;
             }
-            L.cterm[c] = inp - gax * dV;
+            L.cterm[t] = inp - gax * dV;
         }
         __syncthreads();
         float gdiag = 0.0f, inpd = 0.0f;
@@ -266,18 +301,18 @@ __global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
         float vold = 0.0f;
         if (act) {
             for (uint32_t e = e0; e < e1; e++) {
-                uint32_t ent = d.csr_ent[e];
-                uint32_t c = ent >> 1;
+                uint32_t en = L.ent[e];
+                uint32_t c = (en & 0xFFFF) >> 1;
                 if (!L.cintra[c]) continue;
                 gdiag += L.cg[c];
-                if (ent & 1) {
+                if (en & 1) {
                     inpd -= L.cterm[c];
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dacc[k] += (double)cfin[k * kTileConn + c] / volf;
+                    for (int k = 0; k < 7; k++) dacc[k] += (double)cfin[k * TS + c] / volf;
                 } else {
                     inpd += L.cterm[c];
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dacc[k] -= (double)cfin[k * kTileConn + c] / volf;
+                    for (int k = 0; k < 7; k++) dacc[k] -= (double)cfin[k * TS + c] / volf;
                 }
             }
             vold = L.V[t];
@@ -286,17 +321,18 @@ __global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
         __syncthreads();
         int vc = 0;
         float vnew = vold;
+        const float den = cdt + gdiag;
+        const float base = cdt * vold + inpd;
         for (int it = 0; it < 50; it++) {
             if (act) {
                 float off = 0.0f;
                 for (uint32_t e = e0; e < e1; e++) {
-                    uint32_t ent = d.csr_ent[e];
-                    uint32_t c = ent >> 1;
+                    uint32_t en = L.ent[e];
+                    uint32_t c = (en & 0xFFFF) >> 1;
                     if (!L.cintra[c]) continue;
-                    uint32_t o = (ent & 1) ? (L.cab[c] & 0xFFFF) : (L.cab[c] >> 16);
-                    off += L.cg[c] * L.vf[vc][o];
+                    off += L.cg[c] * L.vf[vc][en >> 16];
                 }
-                vnew = (cdt * vold + inpd + off) / (cdt + gdiag);
+                vnew = (base + off) / den;
                 L.vf[vc ^ 1][t] = vnew;
             }
             __syncthreads();
@@ -305,37 +341,35 @@ __global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
         if (act) {
             L.V[t] = vnew;
 #pragma unroll
-            for (int k = 0; k < 7; k++) L.in[k][t] += (float)dacc[k];
+            for (int k = 0; k < 7; k++) L.ion[k][t] += (float)dacc[k];
         }
         __syncthreads();
     }
+    // intracellular ions are final: store them, then reuse the plane for extracellular ions
+    if (act) {
+        d.V[g] = L.V[t];
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            d.in[k * P + g] = L.ion[k][t];
+            L.ion[k][t] = d.ex[k * P + g];
+        }
+    }
+    __syncthreads();
 
     // ---- extracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:245-281) ----
     // phi is fixed across the substeps, so each connection's Bernoulli factors are too.
-    constexpr int kMaxConnPerLane = kTileConn / 256 > 0 ? kTileConn / 256 : 1;
-    float bm[kMaxConnPerLane][7], bp[kMaxConnPerLane][7];
-    {
-        int q = 0;
-        for (uint32_t c = t; c < nconn && q < kMaxConnPerLane; c += blockDim.x, q++) {
-            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
-            double dphi = phi[a] - phi[b];
-#pragma unroll
-            for (int k = 0; k < 7; k++) {
-                float x = kZ[k] * kFrtMv * dphi;
-                bm[q][k] = bernoulli(-x);
-                bp[q][k] = bernoulli(x);
-            }
-        }
+    BPair e1p, e2p;
+    if (cact) {
+        double dphi = phi[ca_] - phi[cb_];
+        bern4(1.0f * kFrtMv * dphi, 2.0f * kFrtMv * dphi, e1p, e2p);
     }
     for (int sub = 0; sub < kSub; sub++) {
-        int q = 0;
-        for (uint32_t c = t; c < nconn && q < kMaxConnPerLane; c += blockDim.x, q++) {
-            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
-            float ac = L.cac[c], dist = L.cdist[c];
+        if (cact) {
 #pragma unroll
             for (int k = 0; k < 7; k++) {
-                double ca = L.ex[k][a], cb = L.ex[k][b];
-                L.cflux[k][c] = kDiff[k] * ac * dts / dist * (ca * bm[q][k] - cb * bp[q][k]);
+                double ca = L.ion[k][ca_], cb = L.ion[k][cb_];
+                BPair bb = bsel(k, e1p, e2p);
+                L.cbuf[k][t] = kin[k] * (ca * bb.m - cb * bb.p);
             }
         }
         __syncthreads();
@@ -345,95 +379,107 @@ __global__ __launch_bounds__(256) void k_diffusion(Net d, float dt) {
             for (int k = 0; k < 7; k++) dx[k] = 0.0;
             const float ve = volf * kEvr;
             for (uint32_t e = e0; e < e1; e++) {
-                uint32_t ent = d.csr_ent[e];
-                uint32_t c = ent >> 1;
-                if (ent & 1) {
+                uint32_t en = L.ent[e];
+                uint32_t c = (en & 0xFFFF) >> 1;
+                if (en & 1) {
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dx[k] += L.cflux[k][c] / ve;
+                    for (int k = 0; k < 7; k++) dx[k] += L.cbuf[k][c] / ve;
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dx[k] -= L.cflux[k][c] / ve;
+                    for (int k = 0; k < 7; k++) dx[k] -= L.cbuf[k][c] / ve;
                 }
             }
-            // Only this lane reads its own ex during the gather; neighbours read cflux.
+            // Only this lane reads its own ex during the gather; neighbours read cbuf.
 #pragma unroll
-            for (int k = 0; k < 7; k++) L.ex[k][t] += dx[k];
+            for (int k = 0; k < 7; k++) L.ion[k][t] += dx[k];
         }
         __syncthreads();
     }
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            double x = L.ion[k][t];
+            x += (kBath[k] - x) * dt / 1.0f;
+            if (x < 1e-9) x = 1e-9;
+            if (x > 1000.0) x = 1000.0;
+            d.ex[k * P + g] = x;
+        }
+    }
+    __syncthreads();
 
     // ---- transmitters, ATP, temperature, transcription (diffusion_kernel.h:284-345) ----
-    float dnt_in[15], dnt[15];
+    // Staged in LDS (ion + cbuf planes are free now): 15 + 3 floats per segment, then 15.
+    float* st = reinterpret_cast<float*>(&L.ion[0][0]);   // >= 28 floats per segment
+    auto fick = [&](int nsp, int base_sp, bool do_scal, float* dsp, float& datp, float& dtemp, float& dtxn,
+                    const float* dcoef, bool intra_only) {
+        for (uint32_t e = e0; e < e1; e++) {
+            uint32_t en = L.ent[e];
+            uint32_t c = (en & 0xFFFF) >> 1;
+            const bool role_b = en & 1;
+            uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
+            float ac = L.cac[c], dist = L.cdist[c];
+            const bool intra = L.cintra[c];
+            if (!intra_only || intra) {
+                for (int k = 0; k < nsp; k++) {
+                    float dC = st[k * TS + a] - st[k * TS + b];
+                    if (dC < 1e-12f && dC > -1e-12f) continue;
+                    float fl = dcoef[base_sp + k] * ac * dC * dt / dist;
+                    if (role_b) dsp[k] += fl / volf;
+                    else dsp[k] -= fl / volf;
+                }
+            }
+            if (do_scal && intra) {
+                float d1 = st[15 * TS + a] - st[15 * TS + b];
+                if (d1 > 1e-12f || d1 < -1e-12f) {
+                    float fl = 0.14f * ac * d1 * dt / dist;
+                    if (role_b) datp += fl / volf; else datp -= fl / volf;
+                }
+                float d2 = st[16 * TS + a] - st[16 * TS + b];
+                if (d2 > 0.001f || d2 < -0.001f) {
+                    float fl = 0.0014f * ac * d2 * dt / dist;
+                    if (role_b) dtemp += fl / volf; else dtemp -= fl / volf;
+                }
+                float d3 = st[17 * TS + a] - st[17 * TS + b];
+                if (d3 > 1e-9f || d3 < -1e-9f) {
+                    float fl = 0.01f * ac * d3 * dt / dist;
+                    if (role_b) dtxn += fl / volf; else dtxn -= fl / volf;
+                }
+            }
+        }
+    };
+    float dnt[15], dnt_in[15];
 #pragma unroll
-    for (int k = 0; k < 15; k++) { dnt_in[k] = 0.0f; dnt[k] = 0.0f; }
-    float datp = 0.0f, dtemp = 0.0f, dtxn = 0.0f;
-    for (uint32_t e = e0; e < e1; e++) {   // e0 == e1 on lanes past the tile
-        uint32_t ent = d.csr_ent[e];
-        uint32_t c = ent >> 1;
-        const bool role_b = ent & 1;
-        uint32_t a = s0 + (L.cab[c] & 0xFFFF), b = s0 + (L.cab[c] >> 16);
-        float ac = L.cac[c], dist = L.cdist[c];
-        const bool intra = L.cintra[c];
-        if (intra) {
-            for (int k = 0; k < 15; k++) {
-                float dC = d.nt_in[k * P + a] - d.nt_in[k * P + b];
-                if (dC < 1e-12f && dC > -1e-12f) continue;
-                float fl = d.nt_diff[k] * ac * dC * dt / dist;
-                if (role_b) dnt_in[k] += fl / volf;
-                else dnt_in[k] -= fl / volf;
-            }
-        }
-        for (int k = 0; k < 15; k++) {
-            float dC = d.nt[k * P + a] - d.nt[k * P + b];
-            if (dC < 1e-12f && dC > -1e-12f) continue;
-            float fl = d.nt_diff[k] * ac * dC * dt / dist;
-            if (role_b) dnt[k] += fl / volf;
-            else dnt[k] -= fl / volf;
-        }
-        if (intra) {
-            float d1 = d.atp[a] - d.atp[b];
-            if (d1 > 1e-12f || d1 < -1e-12f) {
-                float fl = 0.14f * ac * d1 * dt / dist;
-                if (role_b) datp += fl / volf; else datp -= fl / volf;
-            }
-            float d2 = d.temp[a] - d.temp[b];
-            if (d2 > 0.001f || d2 < -0.001f) {
-                float fl = 0.0014f * ac * d2 * dt / dist;
-                if (role_b) dtemp += fl / volf; else dtemp -= fl / volf;
-            }
-            float d3 = d.txn[a] - d.txn[b];
-            if (d3 > 1e-9f || d3 < -1e-9f) {
-                float fl = 0.01f * ac * d3 * dt / dist;
-                if (role_b) dtxn += fl / volf; else dtxn -= fl / volf;
-            }
-        }
+    for (int k = 0; k < 15; k++) { dnt[k] = 0.0f; dnt_in[k] = 0.0f; }
+    float datp = 0.0f, dtemp = 0.0f, dtxn = 0.0f, z0 = 0.0f, z1 = 0.0f, z2 = 0.0f;
+    float own_nt[15], own_scal[3];
+    if (act) {
+        for (int k = 0; k < 15; k++) st[k * TS + t] = own_nt[k] = d.nt[k * P + g];
+        st[15 * TS + t] = own_scal[0] = d.atp[g];
+        st[16 * TS + t] = own_scal[1] = d.temp[g];
+        st[17 * TS + t] = own_scal[2] = d.txn[g];
     }
-    // Every lane of the tile has read its neighbours' transmitter values before any write.
     __syncthreads();
+    fick(15, 0, true, dnt, datp, dtemp, dtxn, d.nt_diff, false);
+    __syncthreads();
+    if (act)
+        for (int k = 0; k < 15; k++) st[k * TS + t] = d.nt_in[k * P + g];
+    __syncthreads();
+    fick(15, 0, false, dnt_in, z0, z1, z2, d.nt_diff, true);
     if (!act) return;
-    d.V[g] = L.V[t];
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-        d.in[k * P + g] = L.in[k][t];
-        double x = L.ex[k][t];
-        x += (kBath[k] - x) * dt / 1.0f;
-        if (x < 1e-9) x = 1e-9;
-        if (x > 1000.0) x = 1000.0;
-        d.ex[k * P + g] = x;
-    }
     for (int k = 0; k < 15; k++) {
-        d.nt_in[k * P + g] += dnt_in[k];
-        d.nt[k * P + g] += dnt[k];
+        d.nt_in[k * P + g] = st[k * TS + t] + dnt_in[k];
+        d.nt[k * P + g] = own_nt[k] + dnt[k];
     }
-    d.atp[g] += datp;
-    d.temp[g] += dtemp;
-    d.txn[g] += dtxn;
+    d.atp[g] = own_scal[0] + datp;
+    d.temp[g] = own_scal[1] + dtemp;
+    d.txn[g] = own_scal[2] + dtxn;
 }
 
 // Without any connection step 2.5 is skipped entirely (diffusion_kernel.h:118); nothing here.
 
 // ------------------------------------------------------------------ step 3, pre-side
 __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt) {
+    exp_table_stage();
     const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= d.ns) return;
     const size_t NS = d.ns;
@@ -471,7 +517,7 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt)
     }
     float open = mg * mg * hg;
     double Pc = p_single(15.0f, 2.0f, kCbarCa);
-    double flux = ghk_di(vb, bca, d.ex[2 * P + pre]);
+    double flux = ghk_di_nocache(vb, bca, d.ex[2 * P + pre]);
     double ia = Pc * flux * (dens * d.y_pre_area[y]) * open;
     float ica = (float)(ia * kAtoPa);
     bca -= ica * dt * 1000.0f / (2.0f * kFaraday * 0.1f);
@@ -525,8 +571,10 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt)
 // ------------------------------------------------------------------ step 3 post-side + step 4
 __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* __restrict__ st, uint64_t tick,
                                                   float dt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
+    exp_table_stage();
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= d.n) return;
+    const uint32_t i = d.walk[lane];
     const size_t P = d.npad, NS = d.ns;
     const uint8_t fl = d.sflags[i];
     double ibuf = d.ibuf[i];
@@ -541,6 +589,9 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
         }
         s.area = d.area[i];
         s.vol = d.vol[i];
+        seg_consts_init(s);
+        ghk_cache_init(s);
+        mg_cache_init(s);
 #pragma unroll
         for (int k = 0; k < 15; k++) s.nt[k] = 0.0f;
         if (fl & SF_BATH_NMDA) s.nt[0] = 1.0f;

@@ -280,9 +280,21 @@ static const uint64_t kExpTabHost[256] = {
 BC_HD double as_double(uint64_t u) { return __builtin_bit_cast(double, u); }
 BC_HD uint64_t as_u64(double d) { return __builtin_bit_cast(uint64_t, d); }
 
+// On the device the table is read from LDS: every kernel that evaluates exp stages it once
+// per workgroup with exp_table_stage() before its first exp.
+__device__ static inline uint64_t* exp_table_lds() {
+    __shared__ uint64_t t[256];
+    return t;
+}
+__device__ static inline void exp_table_stage() {
+    uint64_t* t = exp_table_lds();
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) t[i] = kExpTabDev[i];
+    __syncthreads();
+}
+
 BC_HD uint64_t exp_tab(uint32_t i) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return kExpTabDev[i];
+    return exp_table_lds()[i];
 #else
     return kExpTabHost[i];
 #endif

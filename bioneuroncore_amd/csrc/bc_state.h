@@ -12,8 +12,10 @@
 
 namespace bc {
 
-constexpr int kTileSeg = 256;
-constexpr int kTileConn = 256;
+// Tile size classes of the electrodiffusion kernel (segments = connections = lanes).
+constexpr int kTileClasses = 3;
+constexpr int kTileSize[kTileClasses] = {64, 128, 256};
+constexpr int kTileSeg = 256;   // largest island one tile holds
 constexpr int kNtCount = 15;
 
 // Segment flag bits.
@@ -28,6 +30,14 @@ enum : uint8_t {
 enum : uint8_t {
     YF_INHIBITORY = 1,    // a post receptor is GABA-A / GABA-B / glycine
     YF_SKIP = 2,          // pre/post out of range or electrical: nothing runs
+};
+
+// Tiles of one size class, launched together.
+struct TileSet {
+    const uint32_t* seg0;
+    const uint32_t* nseg;
+    const uint32_t* conn0;
+    const uint32_t* nconn;
 };
 
 struct Net {
@@ -77,11 +87,7 @@ struct Net {
     const double* mdens;
     double* mstate;
 
-    // electrodiffusion tiles
-    const uint32_t* tile_seg0;
-    const uint32_t* tile_nseg;
-    const uint32_t* tile_conn0;
-    const uint32_t* tile_nconn;
+    // electrodiffusion tiles (see TileSet); connections stored tile by tile
     const uint32_t* conn_ab;     // (a_local | b_local << 16), tile-local, global connection order
     const uint8_t* conn_intra;
     const uint32_t* csr_off;     // [n+1] per device segment
@@ -102,6 +108,8 @@ struct Net {
     const uint16_t* y_cleft_mask;
     const int32_t* y_src;        // slot of the last earlier synapse whose post is my pre, -1
     const uint32_t* post_off;    // [n+1] per device segment
+    const uint32_t* walk;        // lane -> segment for the post-side pass: each 256-segment
+                                 // window ordered by incoming-synapse count, longest first
     uint8_t* y_rcount;
     const uint8_t* y_rtype;      // [nrec][ns]
     const uint8_t* y_rav;

@@ -61,3 +61,95 @@ def test_config2_small_bitexact():
     gst = [backend.make_stimuli(current=c) for c in cur]
     o = _compare(net, 300, lambda t: H.Stimuli(current=cur[t]), lambda t: gst[t])
     assert o.spikes().sum() > 0
+
+
+def test_all_tile_classes_and_stimulus_kinds():
+    """64-, 128- and 256-lane tiles in one network (8-, 100- and 200-segment islands), with
+    current pulses, ramps, a voltage clamp and one-shot ion/ligand injections."""
+    a = bion.build_network(6, 8, "chain", [bion.NAV12, bion.KVDR, bion.K2P, bion.NALCN, bion.PUMP],
+                           [bion.CAV21], 4.0, "ampa", 5, 21)
+    b = bion.config3(n_cables=2, seg_per_cable=100, seed=22)
+    c = bion.config5_cell(23, n_seg=200)
+    net = _concat([a, b, c])
+    cur = H.soma_clamp(net, 220.0)
+    ramp = np.zeros(1, bion.RAMP_DT)
+    ramp["segment_id"], ramp["start_pA"], ramp["end_pA"], ramp["start_tick"], ramp["end_tick"] = 60, 0.0, 90.0, 5, 120
+    vc = np.zeros(1, bion.VCLAMP_DT)
+    vc["segment_id"], vc["target_mV"], vc["start_tick"], vc["end_tick"] = 150, -20.0, 30, 60
+    ion = np.zeros(1, bion.ION_INJ_DT)
+    ion["segment_id"], ion["ion_idx"], ion["amount_mM"] = 70, 2, 0.004
+    lig = np.zeros(1, bion.LIG_INJ_DT)
+    lig["segment_id"], lig["nt_idx"], lig["amount"] = 12, 0, 0.5
+
+    def kinds(t):
+        return dict(current=cur, ramp=ramp, vclamp=vc, ion=ion if t == 20 else None, lig=lig if t == 40 else None)
+    gst = [backend.make_stimuli(**kinds(t)) for t in range(150)]
+    _compare(net, 150, lambda t: H.Stimuli(**kinds(t)), lambda t: gst[t])
+
+
+def _concat(nets):
+    """Disjoint union of networks (segment, connection and synapse indices shifted)."""
+    segs, conns, syns, neurons = [], [], [], []
+    off = 0
+    sid = 0
+    for n in nets:
+        s = n.segments.copy()
+        s["id"] = np.arange(off, off + len(s))
+        c = n.connections.copy()
+        c["segment_a"] += off
+        c["segment_b"] += off
+        y = n.synapses.copy()
+        y["pre_segment"] += off
+        y["post_segment"] += off
+        y["id"] = np.arange(sid + 1, sid + 1 + len(y))
+        sid += len(y)
+        nr = n.neurons.copy()
+        nr["segment_start"] += off
+        nr["soma_index"] += off
+        segs.append(s); conns.append(c); syns.append(y); neurons.append(nr)
+        off += len(s)
+    return bion.Network(np.concatenate(neurons), np.concatenate(segs), np.concatenate(conns), np.concatenate(syns))
+
+
+def test_compat_coherence_and_host_edits():
+    """COMPAT mode: the AoS is current after every tick and host edits between ticks reach the
+    device (as the daemon writes V_m into the AoS, src/daemon_server.cpp:1634-1657)."""
+    net = bion.config1(seed=9)
+    clamp = H.soma_clamp(net, 200.0)
+    o = H.OracleSim(net.copy())
+    sim = backend.Simulation(coherence=backend.COHERENCE_COMPAT)
+    assert sim.load_network(net.copy()) == 0
+    sim.upload_stimuli(backend.make_stimuli(current=clamp))
+    view = sim.bion_data()
+    for t in range(60):
+        if t == 30:   # host edit between ticks on both sides
+            o.net.segments["V_m"][5] = -20.0
+            view.segments["V_m"][5] = -20.0
+        o.step(DT, H.Stimuli(current=clamp))
+        sim.run_tick(DT)
+        assert H.first_mismatch(o.net.segments, view.segments) is None, (t, H.first_mismatch(o.net.segments, view.segments))
+    assert H.first_mismatch(o.net.synapses, view.synapses) is None
+    sim.close()
+
+
+def test_load_bion_file_and_error_codes(tmp_path):
+    net = bion.config1(seed=4)
+    p = str(tmp_path / "n.bion")
+    bion.write_bion(p, net)
+    sim = backend.Simulation()
+    assert sim.load_bion(p) == 0
+    assert sim.load_bion(str(tmp_path / "missing.bion")) == -1
+    bad = tmp_path / "bad.bion"
+    bad.write_bytes(b"\0" * 200)
+    assert sim.load_bion(str(bad)) == -2
+    raw = bytearray(open(p, "rb").read())
+    raw[4:8] = (13).to_bytes(4, "little")
+    (tmp_path / "v13.bion").write_bytes(bytes(raw))
+    assert sim.load_bion(str(tmp_path / "v13.bion")) == -3
+    out = str(tmp_path / "saved.bion")
+    assert sim.load_bion(p) == 0
+    sim.run_ticks(DT, 5)
+    assert sim.save_bion(out) == 0
+    back = bion.read_bion(out)
+    np.testing.assert_array_equal(back.segments["V_m"], sim.bion_data().segments["V_m"])
+    sim.close()
