@@ -206,6 +206,7 @@ struct Sim {
     int coherence = BCHIP_COHERENCE_COMPAT;
     bool host_stale = false;
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
+    bool receptors_only = true;   // every receptor slot is AMPA, NMDA or GABA-A
     std::vector<uint8_t> seg_dispatched;   // [n*32] slot ran at least once (state[7] written)
     hipEvent_t ev[8] = {};
     float last_ms[8] = {};
@@ -430,7 +431,8 @@ static int build_device(Sim& S) {
             last_onto[sy[i].post_segment] = i;
         }
     }
-    std::vector<uint32_t> y_pre(ns), y_id(ns);
+    std::vector<uint32_t> y_pre(ns), y_id(ns), y_prew(ns), watched;
+    std::vector<int64_t> watch_idx(n, -1);
     std::vector<uint8_t> y_flags(ns, 0), y_rcount(ns, 0), y_refr(ns, 0);
     std::vector<float> y_delay(ns), y_pa(ns), y_qa(ns), y_prel(ns), y_amt(ns), y_tau(ns), y_rmax(ns), y_rrec(ns);
     std::vector<float> y_w(ns), y_fac(ns), y_dep(ns), y_rrp(ns), y_bca(ns), y_sca(ns), y_ck(ns), y_cn(ns);
@@ -440,10 +442,13 @@ static int build_device(Sim& S) {
     std::vector<float> y_cleft((size_t)15 * ns, 0.0f);
     S.unsupported_syn = 0;
     S.hazard_syn = 0;
+    S.receptors_only = true;
     for (uint32_t k = 0; k < ns; k++) {
         const Synapse& y = sy[S.slot2syn[k]];
         y_pre[k] = S.host2dev[y.pre_segment];
         sflags[y_pre[k]] |= SF_WATCHED;
+        if (watch_idx[y_pre[k]] < 0) { watch_idx[y_pre[k]] = (int64_t)watched.size(); watched.push_back(y_pre[k]); }
+        y_prew[k] = (uint32_t)watch_idx[y_pre[k]];
         y_id[k] = y.id;
         uint8_t fl = 0;
         if (y.type == 1 || y.type == 2) { fl |= YF_SKIP; S.unsupported_syn++; }
@@ -456,6 +461,7 @@ static int build_device(Sim& S) {
             y_rdens[(size_t)j * ns + k] = y.post_receptors[j].density;
             for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
             if (!modifier_supported(rt)) S.unsupported_mod++;
+            if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }
         y_flags[k] = fl;
         y_rcount[k] = y.post_receptor_count;
@@ -542,6 +548,10 @@ static int build_device(Sim& S) {
     d.csr_off = S.dupload(csr_off);
     d.csr_ent = S.dupload(csr_ent);
     d.y_pre = S.dupload(y_pre);
+    d.y_prew = S.dupload(y_prew);
+    d.watched = S.dupload(watched);
+    d.nwatched = (uint32_t)watched.size();
+    d.presum = S.dalloc<PreSum>(watched.size());
     d.y_id = S.dupload(y_id);
     d.y_flags = S.dupload(y_flags);
     d.y_delay = S.dupload(y_delay);
@@ -709,10 +719,12 @@ static void run_tick(Sim& S, float dt) {
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
         HIPCHK(hipMemcpyAsync(d.ca_snap, d.in + 2 * (size_t)d.npad, (size_t)d.n * 8, hipMemcpyDeviceToDevice, S.stream));
-        k_syn_pre<<<gy, 256, 0, S.stream>>>(d, tick, dt);
+        k_pre_summary<<<(d.nwatched + 255) / 256, 256, 0, S.stream>>>(d);
+        k_syn_pre<<<std::min<uint32_t>(gy, 8192), 256, 0, S.stream>>>(d, tick, dt);
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[3], S.stream));
-    k_syn_post<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    if (S.receptors_only) k_syn_post<true><<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    else k_syn_post<false><<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[4], S.stream));
     if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, tick, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[5], S.stream));

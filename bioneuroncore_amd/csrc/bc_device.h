@@ -636,6 +636,17 @@ BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_
     return r * (s2 * coag);
 }
 
+// Receptor-slot dispatch when every receptor in the network is AMPA, NMDA or GABA-A.
+BC_DEV double dispatch_receptor(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
+    if (type == M_NMDA) return rec_nmda(m, s, dt);
+    if (type == M_AMPA) {
+        const IonoK k = {8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f};
+        return iono(m, s, dt, k);
+    }
+    const IonoK k = {28.0f, 1.0f, 1.0f, 5.0f, 1, 0.0f, 0.0f, 0.0f, true, 2000.0f};
+    return iono(m, s, dt, k);
+}
+
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
 // current and leave their state untouched, as the reference's device switch does.
 BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {

@@ -40,7 +40,7 @@ __device__ static inline uint64_t delay_ticks(float delay, float dt) {   // dela
 }
 
 // ------------------------------------------------------------------ step 1
-__global__ __launch_bounds__(256) void k_modifiers(Net d, const GpuStimulusData* __restrict__ st,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_modifiers(Net d, const GpuStimulusData* __restrict__ st,
                                                    uint64_t tick, float dt) {
     exp_table_stage();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -237,13 +237,34 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     double* gphi = &L.cbuf[0][0];
     if (cact) gphi[t] = 0.5 * (sig[ca_] + sig[cb_]) * cac / cdist;
     __syncthreads();
+    // The first kRegDeg incidences of a lane stay in registers across the sweeps (trees have
+    // at most three); the rest are read from LDS in order after them.
+    constexpr int kRegDeg = 4;
+    double pg[kRegDeg];
+    uint32_t po[kRegDeg];
+    int pdeg = 0;
+    uint32_t pe_rest = e1;
     if (act)
-        for (uint32_t e = e0; e < e1; e++) diag += gphi[(L.ent[e] & 0xFFFF) >> 1];
+        for (uint32_t e = e0; e < e1; e++) {
+            uint32_t en = L.ent[e];
+            double gg = gphi[(en & 0xFFFF) >> 1];
+            diag += gg;
+            if (gg <= 0.0) continue;
+            if (pdeg < kRegDeg) {
+                pg[pdeg] = gg;
+                po[pdeg] = en >> 16;
+                pdeg++;
+                if (pdeg == kRegDeg) pe_rest = e + 1;
+            }
+        }
     int cur = 0;
     for (int it = 0; it < 50; it++) {
         if (act) {
             double pn = 0.0;
-            for (uint32_t e = e0; e < e1; e++) {
+#pragma unroll
+            for (int q = 0; q < kRegDeg; q++)
+                if (q < pdeg) pn += pg[q] * L.phi[cur][po[q]];
+            for (uint32_t e = pe_rest; e < e1; e++) {
                 uint32_t en = L.ent[e];
                 double gg = gphi[(en & 0xFFFF) >> 1];
                 if (gg <= 0.0) continue;
@@ -323,10 +344,27 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         float vnew = vold;
         const float den = cdt + gdiag;
         const float base = cdt * vold + inpd;
+        float jg[kRegDeg];
+        uint32_t jo[kRegDeg];
+        int jdeg = 0;
+        uint32_t je_rest = e1;
+        if (act)
+            for (uint32_t e = e0; e < e1 && jdeg < kRegDeg; e++) {
+                uint32_t en = L.ent[e];
+                uint32_t c = (en & 0xFFFF) >> 1;
+                if (!L.cintra[c]) continue;
+                jg[jdeg] = L.cg[c];
+                jo[jdeg] = en >> 16;
+                jdeg++;
+                if (jdeg == kRegDeg) je_rest = e + 1;
+            }
         for (int it = 0; it < 50; it++) {
             if (act) {
                 float off = 0.0f;
-                for (uint32_t e = e0; e < e1; e++) {
+#pragma unroll
+                for (int q = 0; q < kRegDeg; q++)
+                    if (q < jdeg) off += jg[q] * L.vf[vc][jo[q]];
+                for (uint32_t e = je_rest; e < e1; e++) {
                     uint32_t en = L.ent[e];
                     uint32_t c = (en & 0xFFFF) >> 1;
                     if (!L.cintra[c]) continue;
@@ -478,97 +516,137 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
 // Without any connection step 2.5 is skipped entirely (diffusion_kernel.h:118); nothing here.
 
 // ------------------------------------------------------------------ step 3, pre-side
+// Presynaptic summaries: one lane per segment that is the presynaptic segment of a synapse.
+__global__ __launch_bounds__(256) void k_pre_summary(Net d) {
+    exp_table_stage();
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= d.nwatched) return;
+    const uint32_t pre = d.watched[w];
+    const size_t P = d.npad;
+    PreSum ps;
+    ps.V = d.V[pre];
+    ps.exca = d.ex[2 * P + pre];
+    float vb = ps.V;
+    ps.vn = vn_di(vb);
+    ps.edi = bc::exp(-ps.vn);
+    int cs = d.cav21_slot[pre];
+    ps.has_cav = cs >= 0;
+    ps.dens = 0.0f;
+    ps.mi = 0.0f;
+    ps.h0 = 0.0f;
+    ps.h1 = 0.0;
+    if (cs >= 0) {
+        ps.dens = d.mdens[cs * P + pre];
+        ps.h1 = d.mstate[(size_t)cs * 7 * P + 1 * P + pre];
+        ps.mi = cav21_minf(vb);
+        ps.h0 = 1.0f / (1.0f + bc::exp((vb - -50.0f) / 8.0f));
+    }
+    ps._pad[0] = ps._pad[1] = 0;
+    d.presum[w] = ps;
+}
+
 __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt) {
     exp_table_stage();
-    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= d.ns) return;
+    const double e300 = bc::exp(-dt / 300.0f);   // Cav2.1 inactivation relaxation, tau 300 ms
     const size_t NS = d.ns;
-    d.y_relp[y] = 0.0f;
-    if (d.y_flags[y] & YF_SKIP) return;
-    const float delay = d.y_delay[y];
-    // delayed arrivals (delay_buffer.h:31-43)
-    uint8_t mask = d.d_mask[y];
-    if (delay > 0.0f && mask) {
-        for (int s = 0; s < 4; s++) {
-            if (!(mask & (1u << s))) continue;
-            uint64_t arr = d.d_arr[s * NS + y];
-            if (arr <= tick) {
-                uint8_t nt = d.d_nt[s * NS + y];
-                d.y_cleft[nt * NS + y] += d.d_amt[s * NS + y];
-                d.d_arr[s * NS + y] = 0;
-                mask &= ~(1u << s);
+    for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < d.ns; y += gridDim.x * blockDim.x) {
+        d.y_relp[y] = 0.0f;
+        if (d.y_flags[y] & YF_SKIP) continue;
+        const float delay = d.y_delay[y];
+        // delayed arrivals (delay_buffer.h:31-43)
+        uint8_t mask = d.d_mask[y];
+        if (delay > 0.0f && mask) {
+            for (int s = 0; s < 4; s++) {
+                if (!(mask & (1u << s))) continue;
+                uint64_t arr = d.d_arr[s * NS + y];
+                if (arr <= tick) {
+                    uint8_t nt = d.d_nt[s * NS + y];
+                    d.y_cleft[nt * NS + y] += d.d_amt[s * NS + y];
+                    d.d_arr[s * NS + y] = 0;
+                    mask &= ~(1u << s);
+                }
+            }
+            d.d_mask[y] = mask;
+        }
+        // bouton Ca microdomain (modifier_helpers.h:764-804), no presynaptic autoreceptors
+        const PreSum ps = d.presum[d.y_prew[y]];
+        float vb = ps.V;
+        float bca = d.y_bca[y];
+        float dens = 0.0f, mg = 0.0f, hg = 0.0f;
+        if (ps.has_cav) {
+            dens = ps.dens;
+            float hi = ps.h0;
+            hi *= 1.0f / (1.0f + (bca * kMmToUm) / 3.0f);
+            mg = ps.mi;
+            hg = hi + (ps.h1 - hi) * e300;
+        }
+        float open = mg * mg * hg;
+        double Pc = p_single(15.0f, 2.0f, kCbarCa);
+        double flux;
+        {   // ghk_divalent(V_bouton, bouton_Ca, extra Ca) with the per-segment exp
+            const double V = vb, ci = bca, co = ps.exca, vn = ps.vn;
+            const double kk = 4.0 * kFaraday / (kGas * kTemp);
+            if (fabs(vn) < 1e-6) flux = 2.0 * kFaraday * (ci - co);
+            else if (vn > 20.0) flux = kk * kFaraday * V * 0.001 * ci;
+            else if (vn < -20.0) flux = kk * kFaraday * V * 0.001 * co;
+            else flux = kk * kFaraday * V * 0.001 * (ci - co * ps.edi) / (1.0 - ps.edi);
+        }
+        double ia = Pc * flux * (dens * d.y_pre_area[y]) * open;
+        float ica = (float)(ia * kAtoPa);
+        bca -= ica * dt * 1000.0f / (2.0f * kFaraday * 0.1f);
+        bca += (0.0001f - bca) * dt / 5.0f;
+        if (bca < 0.0f) bca = 0.0f;
+        d.y_bca[y] = bca;
+        // Ca-triggered release (simulation_host.cpp:421-459)
+        uint8_t refr = d.y_refr[y];
+        if (refr == 0) {
+            float cu = bca * kMmToUm;
+            float act = pow4(cu) / (pow4(cu) + 10000.0);
+            float p = d.y_prel[y] * act;
+            uint32_t seed = d.y_id[y] ^ ((uint32_t)tick) ^ (uint32_t)(int64_t)(ps.V * 1000.0f);
+            if (xorshift_draw(seed) < p) {
+                d.y_relp[y] = p;
+                refr = (uint8_t)(uint32_t)(20.0f / dt + 0.5f);
+                if (delay_ticks(delay, dt) == 0) {
+                    // Immediate release: vesicle pool now (pre Ca = value at step-3 start).
+                    const uint32_t pre = d.y_pre[y];
+                    const size_t P = d.npad;
+                    float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
+                    if (!(rmax <= 0.0f || rrp >= rmax)) {
+                        float ca = d.in[2 * P + pre];
+                        float caf = ca / (0.0005f + ca);
+                        float prim = d.y_rrp_rec[y] + d.camkii[pre] * 0.02f + d.calcn[pre] * 0.01f;
+                        prim *= caf;
+                        float rec = (rmax - rrp) * prim * dt;
+                        rrp += rec;
+                        if (rrp > rmax) rrp = rmax;
+                    }
+                    float amt;
+                    float amount = d.y_amount[y];
+                    if (rmax <= 0.0f) {
+                        amt = amount * p;
+                    } else {
+                        float av = amount;
+                        if (av > rrp) av = rrp;
+                        amt = av * p;
+                        rrp -= amt;
+                        if (rrp < 0) rrp = 0;
+                    }
+                    d.y_rrp[y] = rrp;
+                    uint8_t nt = d.y_rcount[y] > 0 ? nt_of_receptor(d.y_rtype[y]) : 0;
+                    d.y_cleft[nt * NS + y] += amt;
+                    d.y_relp[y] = -1.0f;   // finished here
+                }
             }
         }
-        d.d_mask[y] = mask;
+        if (refr > 0) refr--;
+        d.y_refr[y] = refr;
     }
-    // bouton Ca microdomain (modifier_helpers.h:764-804), no presynaptic autoreceptors
-    const uint32_t pre = d.y_pre[y];
-    const size_t P = d.npad;
-    float vb = d.V[pre];
-    float bca = d.y_bca[y];
-    float dens = 0.0f, mg = 0.0f, hg = 0.0f;
-    int cs = d.cav21_slot[pre];
-    if (cs >= 0) {
-        dens = d.mdens[cs * P + pre];
-        float mi = cav21_minf(vb);
-        float hi = cav21_hinf(vb, bca * kMmToUm);
-        mg = mi;
-        hg = hi + (d.mstate[(size_t)cs * 7 * P + 1 * P + pre] - hi) * bc::exp(-dt / 300.0f);
-    }
-    float open = mg * mg * hg;
-    double Pc = p_single(15.0f, 2.0f, kCbarCa);
-    double flux = ghk_di_nocache(vb, bca, d.ex[2 * P + pre]);
-    double ia = Pc * flux * (dens * d.y_pre_area[y]) * open;
-    float ica = (float)(ia * kAtoPa);
-    bca -= ica * dt * 1000.0f / (2.0f * kFaraday * 0.1f);
-    bca += (0.0001f - bca) * dt / 5.0f;
-    if (bca < 0.0f) bca = 0.0f;
-    d.y_bca[y] = bca;
-    // Ca-triggered release (simulation_host.cpp:421-459)
-    uint8_t refr = d.y_refr[y];
-    if (refr == 0) {
-        float cu = bca * kMmToUm;
-        float act = pow4(cu) / (pow4(cu) + 10000.0);
-        float p = d.y_prel[y] * act;
-        uint32_t seed = d.y_id[y] ^ ((uint32_t)tick) ^ (uint32_t)(int64_t)(d.V[pre] * 1000.0f);
-        if (xorshift_draw(seed) < p) {
-            d.y_relp[y] = p;
-            refr = (uint8_t)(uint32_t)(20.0f / dt + 0.5f);
-            if (delay_ticks(delay, dt) == 0) {
-                // Immediate release: vesicle pool now (pre Ca = value at step-3 start).
-                float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
-                if (!(rmax <= 0.0f || rrp >= rmax)) {
-                    float ca = d.in[2 * P + pre];
-                    float caf = ca / (0.0005f + ca);
-                    float prim = d.y_rrp_rec[y] + d.camkii[pre] * 0.02f + d.calcn[pre] * 0.01f;
-                    prim *= caf;
-                    float rec = (rmax - rrp) * prim * dt;
-                    rrp += rec;
-                    if (rrp > rmax) rrp = rmax;
-                }
-                float amt;
-                float amount = d.y_amount[y];
-                if (rmax <= 0.0f) {
-                    amt = amount * p;
-                } else {
-                    float av = amount;
-                    if (av > rrp) av = rrp;
-                    amt = av * p;
-                    rrp -= amt;
-                    if (rrp < 0) rrp = 0;
-                }
-                d.y_rrp[y] = rrp;
-                uint8_t nt = d.y_rcount[y] > 0 ? nt_of_receptor(d.y_rtype[y]) : 0;
-                d.y_cleft[nt * NS + y] += amt;
-                d.y_relp[y] = -1.0f;   // finished here
-            }
-        }
-    }
-    if (refr > 0) refr--;
-    d.y_refr[y] = refr;
 }
 
 // ------------------------------------------------------------------ step 3 post-side + step 4
+// kReceptorsOnly: every receptor slot holds AMPA, NMDA or GABA-A (a leaner dispatch).
+template <bool kReceptorsOnly>
 __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* __restrict__ st, uint64_t tick,
                                                   float dt) {
     exp_table_stage();
@@ -599,6 +677,8 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
         const bool my = fl & SF_MYELIN;
         const uint32_t y0 = d.post_off[i], y1 = d.post_off[i + 1];
         const float pp1 = d.pp1[i];
+        float last_tau = __builtin_nanf("");   // clearance exp reused while tau repeats
+        double last_E = 0.0;
         for (uint32_t y = y0; y < y1; y++) {
             const uint8_t yf = d.y_flags[y];
             if (yf & YF_SKIP) {
@@ -621,7 +701,7 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
                 m.dens = d.y_rdens[j * NS + y];
                 m.a7 = parea;
                 m.av = d.y_rav[j * NS + y];
-                isyn += dispatch(rt, m, s, dt);
+                isyn += kReceptorsOnly ? dispatch_receptor(rt, m, s, dt) : dispatch(rt, m, s, dt);
             }
             ibuf += isyn;
             // spine Ca (modifier_helpers.h:808-816)
@@ -636,7 +716,9 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
             }
             // cleft clearance (simulation_host.cpp:505-510)
             {
-                double E = bc::exp(-dt / d.y_tau[y]);
+                const float tau = d.y_tau[y];
+                if (!(tau == last_tau)) { last_tau = tau; last_E = bc::exp(-dt / tau); }
+                const double E = last_E;
 #pragma unroll
                 for (int k = 0; k < 15; k++)
                     if (cm & (1u << k)) {

@@ -356,4 +356,40 @@ BC_HD double exp(double x) {
     return __builtin_fma(scale, tmp, scale);
 }
 
+// a / b, correctly rounded, given y = rcp_prep(b) (an IEEE division computed once for a divisor
+// reused many times). Two FMA Newton corrections make the quotient faithful; with y correctly
+// rounded, Markstein's theorem makes the last correction return RN(a / b). Zero, subnormal,
+// infinite and NaN results, subnormal or zero numerators and non-normal divisors (y = NaN) take
+// the IEEE division, so no underflow/overflow assumption is made.
+// tests/test_div_rcp.py checks bit identity with IEEE division on 10^8 random operands.
+BC_HD float rcp_prep(float b) {   // RN(1/b), or NaN when b is not a normal number
+    float ab = __builtin_fabsf(b);
+    return (ab >= 1.17549435e-38f && ab <= 3.40282347e+38f) ? 1.0f / b : __builtin_nanf("");
+}
+BC_HD float div_rcp(float a, float b, float y) {
+    float q0 = a * y;
+    float r0 = __builtin_fmaf(-b, q0, a);
+    float q1 = __builtin_fmaf(r0, y, q0);
+    float r1 = __builtin_fmaf(-b, q1, a);
+    float q2 = __builtin_fmaf(r1, y, q1);
+    float aq = __builtin_fabsf(q2), aa = __builtin_fabsf(a);
+    if (!(aq >= 1.17549435e-38f && aq <= 3.40282347e+38f && aa >= 1.17549435e-38f)) return a / b;
+    return q2;
+}
+BC_HD double rcp_prep(double b) {   // RN(1/b), or NaN when b is not a normal number
+    double ab = __builtin_fabs(b);
+    return (ab >= 2.2250738585072014e-308 && ab <= 1.7976931348623157e308) ? 1.0 / b : __builtin_nan("");
+}
+BC_HD double div_rcp(double a, double b, double y) {
+    double q0 = a * y;
+    double r0 = __builtin_fma(-b, q0, a);
+    double q1 = __builtin_fma(r0, y, q0);
+    double r1 = __builtin_fma(-b, q1, a);
+    double q2 = __builtin_fma(r1, y, q1);
+    double aq = __builtin_fabs(q2), aa = __builtin_fabs(a);
+    if (!(aq >= 2.2250738585072014e-308 && aq <= 1.7976931348623157e308 && aa >= 2.2250738585072014e-308))
+        return a / b;
+    return q2;
+}
+
 }  // namespace bc

@@ -40,6 +40,22 @@ struct TileSet {
     const uint32_t* nconn;
 };
 
+// Per-tick summary of one presynaptic segment: everything the bouton/release computation of
+// its outgoing synapses reads, including the V-only exponentials (64 bytes, one cache line).
+struct PreSum {
+    double V;      // V_m after step 2.5 (seed, bouton voltage)
+    double exca;   // extracellular Ca
+    double h1;     // Cav2.1 inactivation state[1] (last Cav2.1 modifier)
+    double vn;     // 2F*V/RT of the divalent GHK for V rounded to float
+    double edi;    // exp(-vn)
+    float dens;    // Cav2.1 density (0 without Cav2.1)
+    float mi;      // Cav2.1 m_inf(V)
+    float h0;      // 1 / (1 + exp((V + 50) / 8)), the voltage factor of h_inf
+    uint32_t has_cav;
+    uint32_t _pad[2];
+};
+static_assert(sizeof(PreSum) == 64, "PreSum layout");
+
 struct Net {
     // sizes
     uint32_t n;          // segments
@@ -95,6 +111,10 @@ struct Net {
 
     // synapses, slot order
     const uint32_t* y_pre;       // device segment index
+    const uint32_t* y_prew;      // index into the presynaptic summaries
+    const uint32_t* watched;     // device segment of each presynaptic summary
+    uint32_t nwatched;
+    PreSum* presum;
     const uint32_t* y_id;
     const uint8_t* y_flags;
     const float* y_delay;
