@@ -684,11 +684,12 @@ BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
 }
 
 // ---- electrodiffusion helpers (diffusion_kernel.h:108-113) ----
-BC_DEV float bernoulli(float x) {
+template <class Tab = ExpTabLds>
+BC_DEV float bernoulli(float x, Tab tab = Tab{}) {
     if (fabsf(x) < 1e-3f) return 1.0f - 0.5f * x;
     if (x > 50.0f) return 0.0f;
     if (x < -50.0f) return -x;
-    return x / (bc::exp(x) - 1.0f);
+    return x / (bc::exp_t(x, tab) - 1.0f);
 }
 
 BC_DEV float xorshift_draw(uint32_t seed) {   // simulation_host.cpp:150-156

@@ -134,27 +134,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // evaluations serve all seven ions.
 template <int TS>
 struct DiffLds {
-    double V[TS];
-    double phi[2][TS];
+    // Each phase overlays its own planes on u; ion and u together stage the transmitter phase.
     double ion[7][TS];     // intracellular ions (intra phase), then extracellular (extra phase)
-    double cbuf[7][TS];    // phi conductances / intra fluxes (float) / extra fluxes
-    float vf[2][TS];
-    float cg[TS];
-    float cterm[TS];
+    union {
+        struct {
+            double sig[TS];        // sigma_e per segment
+            double gphi[TS];       // phi conductance per connection
+            double phi[2][TS];
+        } p;                       // phi_e
+        struct {
+            double V[TS];
+            float flux[7][TS];     // per connection
+            float vf[2][TS];       // Jacobi iterates
+            float cg[TS];
+            float cterm[TS];
+        } i;                       // intracellular
+        double flux[7][TS];        // extracellular, per connection
+    } u;
     float cac[TS];
     float cdist[TS];
     uint32_t cab[TS];      // a_local | b_local << 16
-    uint32_t ent[2 * TS];  // other_local << 16 | conn << 1 | role, per segment in connection order
-    uint8_t cintra[TS];
+    uint16_t ent[2 * TS];  // incidences of each segment in connection order (ent_* below)
 };
+
+static_assert(sizeof(DiffLds<64>::ion) + sizeof(((DiffLds<64>*)nullptr)->u) >= 18 * 4 * 64,
+              "transmitter staging needs 18 floats per segment in ion + u");
+
+// Incidence entry: tile-local connection << 2 | intracellular << 1 | role (0 = end a, 1 = end b).
+BC_DEV uint32_t ent_conn(uint32_t en) { return en >> 2; }
+BC_DEV bool ent_intra(uint32_t en) { return (en >> 1) & 1; }
+template <int TS>
+BC_DEV uint32_t ent_other(const DiffLds<TS>& L, uint32_t en) {   // the segment at the other end
+    uint32_t ab = L.cab[en >> 2];
+    return (en & 1) ? (ab & 0xFFFF) : (ab >> 16);
+}
 
 struct BPair { float m, p; };   // B(-x), B(x)
 
 __device__ static inline void bern4(float dV_or_phi_x1, float x2, BPair& b1, BPair& b2) {
-    b1.m = bernoulli(-dV_or_phi_x1);
-    b1.p = bernoulli(dV_or_phi_x1);
-    b2.m = bernoulli(-x2);
-    b2.p = bernoulli(x2);
+    // the electrodiffusion kernel keeps its LDS for state: exp reads the global table
+    b1.m = bernoulli(-dV_or_phi_x1, ExpTabGlobal{});
+    b1.p = bernoulli(dV_or_phi_x1, ExpTabGlobal{});
+    b2.m = bernoulli(-x2, ExpTabGlobal{});
+    b2.p = bernoulli(x2, ExpTabGlobal{});
 }
 // (B(-x), B(x)) of ion k given the +1 and +2 pairs.
 __device__ static inline BPair bsel(int k, const BPair& b1, const BPair& b2) {
@@ -167,7 +189,6 @@ template <int TS>
 __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     DiffLds<TS>& L = *reinterpret_cast<DiffLds<TS>*>(lds_raw);
-    exp_table_stage();
     const uint32_t tile = blockIdx.x;
     const uint32_t s0 = ts.seg0[tile];
     const uint32_t nseg = ts.nseg[tile];
@@ -181,11 +202,11 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     const uint32_t E0 = d.csr_off[s0];
     const uint32_t nent = d.csr_off[s0 + nseg] - E0;
 
-    double ibuf = 0.0, diag = 0.0, sg = 0.0;
+    double ibuf = 0.0, diag = 0.0, sg = 0.0, vcur = 0.0;
     float area = 0.0f, cap = 0.0f, volf = 0.0f;
     uint32_t e0 = 0, e1 = 0;
     if (act) {
-        L.V[t] = d.V[g];
+        vcur = d.V[g];
 #pragma unroll
         for (int k = 0; k < 7; k++) {
             L.ion[k][t] = d.in[k * P + g];
@@ -206,7 +227,6 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         uint32_t ab = d.conn_ab[c0 + t];
         L.cab[t] = ab;
         cintra = d.conn_intra[c0 + t] != 0;
-        L.cintra[t] = cintra;
         // connection geometry (diffusion_kernel.h:72-80,162-172)
         ca_ = ab & 0xFFFF;
         cb_ = ab >> 16;
@@ -220,39 +240,42 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         L.cdist[t] = cdist;
     }
     // ---- phi_e (diffusion_kernel.h:41-104) ----
-    double* sig = &L.cbuf[6][0];
+    double* sig = L.u.p.sig;
     if (act) {
         sg = kSigmaE * sg;
         sig[t] = sg;
         diag = sg * area / 1.0f;
-        L.phi[0][t] = 0.0;
+        L.u.p.phi[0][t] = 0.0;
     }
     __syncthreads();
     for (uint32_t e = t; e < nent; e += TS) {
         uint32_t en = d.csr_ent[E0 + e];
         uint32_t c = en >> 1;
-        uint32_t o = (en & 1) ? (L.cab[c] & 0xFFFF) : (L.cab[c] >> 16);
-        L.ent[e] = (o << 16) | en;
+        L.ent[e] = (uint16_t)((c << 2) | ((d.conn_intra[c0 + c] != 0) << 1) | (en & 1));
     }
-    double* gphi = &L.cbuf[0][0];
+    double* gphi = L.u.p.gphi;
     if (cact) gphi[t] = 0.5 * (sig[ca_] + sig[cb_]) * cac / cdist;
     __syncthreads();
     // The first kRegDeg incidences of a lane stay in registers across the sweeps (trees have
     // at most three); the rest are read from LDS in order after them.
+    // Unused register slots point at the lane's own entry and are never added, so every read
+    // of a sweep issues at once and the sum keeps the reference's order.
     constexpr int kRegDeg = 4;
     double pg[kRegDeg];
     uint32_t po[kRegDeg];
+#pragma unroll
+    for (int q = 0; q < kRegDeg; q++) { pg[q] = 0.0; po[q] = t; }
     int pdeg = 0;
     uint32_t pe_rest = e1;
     if (act)
         for (uint32_t e = e0; e < e1; e++) {
             uint32_t en = L.ent[e];
-            double gg = gphi[(en & 0xFFFF) >> 1];
+            double gg = gphi[ent_conn(en)];
             diag += gg;
             if (gg <= 0.0) continue;
             if (pdeg < kRegDeg) {
                 pg[pdeg] = gg;
-                po[pdeg] = en >> 16;
+                po[pdeg] = ent_other(L, en);
                 pdeg++;
                 if (pdeg == kRegDeg) pe_rest = e + 1;
             }
@@ -260,32 +283,45 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     int cur = 0;
     for (int it = 0; it < 50; it++) {
         if (act) {
+            double pv[kRegDeg];
+#pragma unroll
+            for (int q = 0; q < kRegDeg; q++) pv[q] = L.u.p.phi[cur][po[q]];
             double pn = 0.0;
 #pragma unroll
-            for (int q = 0; q < kRegDeg; q++)
-                if (q < pdeg) pn += pg[q] * L.phi[cur][po[q]];
+            for (int q = 0; q < kRegDeg; q++) {
+                double s = pn + pg[q] * pv[q];
+                pn = q < pdeg ? s : pn;
+            }
             for (uint32_t e = pe_rest; e < e1; e++) {
                 uint32_t en = L.ent[e];
-                double gg = gphi[(en & 0xFFFF) >> 1];
+                double gg = gphi[ent_conn(en)];
                 if (gg <= 0.0) continue;
-                pn += gg * L.phi[cur][en >> 16];
+                pn += gg * L.u.p.phi[cur][ent_other(L, en)];
             }
             double ph = (diag > 1e-9) ? (pn - ibuf) / diag : 0.0;
             if (ph > 10.0) ph = 10.0;
             if (ph < -10.0) ph = -10.0;
-            L.phi[cur ^ 1][t] = ph;
+            L.u.p.phi[cur ^ 1][t] = ph;
         }
         __syncthreads();
         cur ^= 1;
     }
-    const double* phi = L.phi[cur];
+    // phi of both ends of each connection stays in registers; the phi planes are dead after.
+    double pha = 0.0, phb = 0.0;
+    if (cact) {
+        pha = L.u.p.phi[cur][ca_];
+        phb = L.u.p.phi[cur][cb_];
+    }
+    __syncthreads();
+    if (act) L.u.i.V[t] = vcur;
+    __syncthreads();
 
     // ---- intracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:151-238) ----
     const int kSub = 5;
     const float dts = dt / kSub;
     float cdt = cap / dts;
     if (cdt < 1e-9f) cdt = 1e-9f;
-    float* cfin = reinterpret_cast<float*>(&L.cbuf[0][0]);   // [7][TS] floats
+    float* cfin = &L.u.i.flux[0][0];   // [7][TS]
     float kin[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) kin[k] = kDiff[k] * cac * dts / cdist;
@@ -299,8 +335,8 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                 sgi += kZ[k] * kZ[k] * kDiff[k] * cbar;
             }
             float gax = kSigmaE * sgi * cac / cdist;
-            L.cg[t] = gax;
-            float dV = (L.V[a] + phi[a]) - (L.V[b] + phi[b]);
+            L.u.i.cg[t] = gax;
+            float dV = (L.u.i.V[a] + pha) - (L.u.i.V[b] + phb);
             BPair b1, b2;
             bern4(1.0f * kFrtMv * dV, 2.0f * kFrtMv * dV, b1, b2);
             float inp = 0.0f;
@@ -312,7 +348,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                 cfin[k * TS + t] = fl;
                 inp += kZ[k] * kChargeToPa * fl / dts;
             }
-            L.cterm[t] = inp - gax * dV;
+            L.u.i.cterm[t] = inp - gax * dV;
         }
         __syncthreads();
         float gdiag = 0.0f, inpd = 0.0f;
@@ -323,61 +359,70 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         if (act) {
             for (uint32_t e = e0; e < e1; e++) {
                 uint32_t en = L.ent[e];
-                uint32_t c = (en & 0xFFFF) >> 1;
-                if (!L.cintra[c]) continue;
-                gdiag += L.cg[c];
+                uint32_t c = ent_conn(en);
+                if (!ent_intra(en)) continue;
+                gdiag += L.u.i.cg[c];
                 if (en & 1) {
-                    inpd -= L.cterm[c];
+                    inpd -= L.u.i.cterm[c];
 #pragma unroll
                     for (int k = 0; k < 7; k++) dacc[k] += (double)cfin[k * TS + c] / volf;
                 } else {
-                    inpd += L.cterm[c];
+                    inpd += L.u.i.cterm[c];
 #pragma unroll
                     for (int k = 0; k < 7; k++) dacc[k] -= (double)cfin[k * TS + c] / volf;
                 }
             }
-            vold = L.V[t];
-            L.vf[0][t] = vold;
+            vold = (float)vcur;
+            L.u.i.vf[0][t] = vold;
         }
         __syncthreads();
         int vc = 0;
         float vnew = vold;
         const float den = cdt + gdiag;
         const float base = cdt * vold + inpd;
+        const double rden = 1.0 / (double)den;
         float jg[kRegDeg];
         uint32_t jo[kRegDeg];
+#pragma unroll
+        for (int q = 0; q < kRegDeg; q++) { jg[q] = 0.0f; jo[q] = t; }
         int jdeg = 0;
         uint32_t je_rest = e1;
         if (act)
             for (uint32_t e = e0; e < e1 && jdeg < kRegDeg; e++) {
                 uint32_t en = L.ent[e];
-                uint32_t c = (en & 0xFFFF) >> 1;
-                if (!L.cintra[c]) continue;
-                jg[jdeg] = L.cg[c];
-                jo[jdeg] = en >> 16;
+                uint32_t c = ent_conn(en);
+                if (!ent_intra(en)) continue;
+                jg[jdeg] = L.u.i.cg[c];
+                jo[jdeg] = ent_other(L, en);
                 jdeg++;
                 if (jdeg == kRegDeg) je_rest = e + 1;
             }
         for (int it = 0; it < 50; it++) {
             if (act) {
+                float jv[kRegDeg];
+#pragma unroll
+                for (int q = 0; q < kRegDeg; q++) jv[q] = L.u.i.vf[vc][jo[q]];
                 float off = 0.0f;
 #pragma unroll
-                for (int q = 0; q < kRegDeg; q++)
-                    if (q < jdeg) off += jg[q] * L.vf[vc][jo[q]];
+                for (int q = 0; q < kRegDeg; q++) {
+                    float s = off + jg[q] * jv[q];
+                    off = q < jdeg ? s : off;
+                }
                 for (uint32_t e = je_rest; e < e1; e++) {
                     uint32_t en = L.ent[e];
-                    uint32_t c = (en & 0xFFFF) >> 1;
-                    if (!L.cintra[c]) continue;
-                    off += L.cg[c] * L.vf[vc][en >> 16];
+                    uint32_t c = ent_conn(en);
+                    if (!ent_intra(en)) continue;
+                    off += L.u.i.cg[c] * L.u.i.vf[vc][ent_other(L, en)];
                 }
-                vnew = (base + off) / den;
-                L.vf[vc ^ 1][t] = vnew;
+                vnew = div_wide(base + off, den, rden);
+                L.u.i.vf[vc ^ 1][t] = vnew;
             }
             __syncthreads();
             vc ^= 1;
         }
         if (act) {
-            L.V[t] = vnew;
+            vcur = vnew;
+            L.u.i.V[t] = vcur;
 #pragma unroll
             for (int k = 0; k < 7; k++) L.ion[k][t] += (float)dacc[k];
         }
@@ -385,7 +430,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     }
     // intracellular ions are final: store them, then reuse the plane for extracellular ions
     if (act) {
-        d.V[g] = L.V[t];
+        d.V[g] = vcur;
 #pragma unroll
         for (int k = 0; k < 7; k++) {
             d.in[k * P + g] = L.ion[k][t];
@@ -398,7 +443,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     // phi is fixed across the substeps, so each connection's Bernoulli factors are too.
     BPair e1p, e2p;
     if (cact) {
-        double dphi = phi[ca_] - phi[cb_];
+        double dphi = pha - phb;
         bern4(1.0f * kFrtMv * dphi, 2.0f * kFrtMv * dphi, e1p, e2p);
     }
     for (int sub = 0; sub < kSub; sub++) {
@@ -407,7 +452,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             for (int k = 0; k < 7; k++) {
                 double ca = L.ion[k][ca_], cb = L.ion[k][cb_];
                 BPair bb = bsel(k, e1p, e2p);
-                L.cbuf[k][t] = kin[k] * (ca * bb.m - cb * bb.p);
+                L.u.flux[k][t] = kin[k] * (ca * bb.m - cb * bb.p);
             }
         }
         __syncthreads();
@@ -418,13 +463,13 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             const float ve = volf * kEvr;
             for (uint32_t e = e0; e < e1; e++) {
                 uint32_t en = L.ent[e];
-                uint32_t c = (en & 0xFFFF) >> 1;
+                uint32_t c = ent_conn(en);
                 if (en & 1) {
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dx[k] += L.cbuf[k][c] / ve;
+                    for (int k = 0; k < 7; k++) dx[k] += L.u.flux[k][c] / ve;
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dx[k] -= L.cbuf[k][c] / ve;
+                    for (int k = 0; k < 7; k++) dx[k] -= L.u.flux[k][c] / ve;
                 }
             }
             // Only this lane reads its own ex during the gather; neighbours read cbuf.
@@ -447,16 +492,16 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
 
     // ---- transmitters, ATP, temperature, transcription (diffusion_kernel.h:284-345) ----
     // Staged in LDS (ion + cbuf planes are free now): 15 + 3 floats per segment, then 15.
-    float* st = reinterpret_cast<float*>(&L.ion[0][0]);   // >= 28 floats per segment
+    float* st = reinterpret_cast<float*>(&L.ion[0][0]);   // ion + u: 28 floats per segment
     auto fick = [&](int nsp, int base_sp, bool do_scal, float* dsp, float& datp, float& dtemp, float& dtxn,
                     const float* dcoef, bool intra_only) {
         for (uint32_t e = e0; e < e1; e++) {
             uint32_t en = L.ent[e];
-            uint32_t c = (en & 0xFFFF) >> 1;
+            uint32_t c = ent_conn(en);
             const bool role_b = en & 1;
             uint32_t a = L.cab[c] & 0xFFFF, b = L.cab[c] >> 16;
             float ac = L.cac[c], dist = L.cdist[c];
-            const bool intra = L.cintra[c];
+            const bool intra = ent_intra(en);
             if (!intra_only || intra) {
                 for (int k = 0; k < nsp; k++) {
                     float dC = st[k * TS + a] - st[k * TS + b];

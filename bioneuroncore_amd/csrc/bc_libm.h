@@ -300,6 +300,19 @@ BC_HD uint64_t exp_tab(uint32_t i) {
 #endif
 }
 
+// Table sources for exp_t: the staged LDS copy (host: the host table), or the global copy for
+// kernels that keep their LDS for other data.
+struct ExpTabLds {
+    __host__ __device__ uint64_t operator()(uint32_t i) const { return exp_tab(i); }
+};
+struct ExpTabGlobal {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __device__ uint64_t operator()(uint32_t i) const { return kExpTabDev[i]; }
+#else
+    uint64_t operator()(uint32_t i) const { return kExpTabHost[i]; }
+#endif
+};
+
 // Large-|x| tail of exp (|x| >= 512): rescale so the exponent cannot overflow or lose
 // subnormal precision before the final multiply.
 BC_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki) {
@@ -322,7 +335,8 @@ BC_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki) {
     return 0x1p-1022 * y;
 }
 
-BC_HD double exp(double x) {
+template <class Tab>
+BC_HD double exp_t(double x, Tab tab) {
     const double kInvLn2N = 0x1.71547652b82fep0 * 128.0;
     const double kNegLn2HiN = -0x1.62e42fefa0000p-8;
     const double kNegLn2LoN = -0x1.cf79abc9e3b3ap-47;
@@ -346,8 +360,8 @@ BC_HD double exp(double x) {
     double r = __builtin_fma(kd, kNegLn2LoN, __builtin_fma(kd, kNegLn2HiN, x));
     uint32_t idx = 2u * (uint32_t)(ki % 128u);
     uint64_t top = ki << 45;
-    double tail = as_double(exp_tab(idx));
-    uint64_t sbits = exp_tab(idx + 1) + top;
+    double tail = as_double(tab(idx));
+    uint64_t sbits = tab(idx + 1) + top;
     double r2 = r * r;
     double tmp = __builtin_fma(r2 * r2, __builtin_fma(r, C5, C4),
                                __builtin_fma(r2, __builtin_fma(r, C3, C2), tail + r));
@@ -355,6 +369,7 @@ BC_HD double exp(double x) {
     double scale = as_double(sbits);
     return __builtin_fma(scale, tmp, scale);
 }
+BC_HD double exp(double x) { return exp_t(x, ExpTabLds{}); }
 
 // a / b, correctly rounded, given y = rcp_prep(b) (an IEEE division computed once for a divisor
 // reused many times). Two FMA Newton corrections make the quotient faithful; with y correctly
@@ -390,6 +405,18 @@ BC_HD double div_rcp(double a, double b, double y) {
     if (!(aq >= 2.2250738585072014e-308 && aq <= 1.7976931348623157e308 && aa >= 2.2250738585072014e-308))
         return a / b;
     return q2;
+}
+
+// a / b for floats, correctly rounded, given rb = RN(1 / (double)b). The product a * rb in
+// double is within 2^-52 relative of a / b, while a quotient of two 24-bit significands that is
+// a normal float lies at least 2^-49 (relative) away from every rounding midpoint and is never
+// on one, so rounding the product to float gives RN(a / b). Results outside the normal float
+// range, and every zero/infinite/NaN operand or divisor, take the IEEE division.
+BC_HD float div_wide(float a, float b, double rb) {
+    double p = (double)a * rb;
+    double ap = __builtin_fabs(p);
+    if (!(ap >= 0x1p-126 && ap < 0x1p128)) return a / b;
+    return (float)p;
 }
 
 }  // namespace bc

@@ -1,5 +1,6 @@
-"""bc::div_rcp (bioneuroncore_amd/csrc/bc_libm.h) returns the IEEE quotient bit for bit:
-random float and double operands over wide exponent ranges, divisors with all-ones and
+"""bc::div_rcp and bc::div_wide (bioneuroncore_amd/csrc/bc_libm.h) return the IEEE quotient bit
+for bit: random float and double operands over wide exponent ranges (subnormal operands and
+quotients near the underflow and overflow thresholds included), divisors with all-ones and
 near-power-of-two significands, and zero/subnormal/infinite edge cases."""
 import os
 import shutil
@@ -31,14 +32,26 @@ int main() {
         double R = A / B, Q = bc::div_rcp(A, B, bc::rcp_prep(B));
         if (memcmp(&R, &Q, 8)) badd++;
     }
+    long badw = 0;
+    for (long i = 0; i < 40000000L; i++) {
+        // exponents 0..254 (subnormal through huge), so quotients span underflow and overflow
+        uint32_t ea = (uint32_t)(g() % 255), eb = (uint32_t)(i % 4 == 0 ? g() % 255 : g() % 60 + 97);
+        uint32_t ma = g() & 0x7FFFFF, mb = (i % 3 == 0) ? 0x7FFFFF : (i % 3 == 1 ? (g() & 0xF) : (g() & 0x7FFFFF));
+        uint32_t ba = (ea << 23) | ma | ((g() & 1u) << 31), bb = (eb << 23) | mb | ((g() & 1u) << 31);
+        float a, b; memcpy(&a, &ba, 4); memcpy(&b, &bb, 4);
+        float r = a / b, q = bc::div_wide(a, b, 1.0 / (double)b);
+        if (memcmp(&r, &q, 4) && !(std::isnan(r) && std::isnan(q))) badw++;
+    }
     long bade = 0;
     const float ea[] = {0.0f, -0.0f, 1e-40f, -1e-42f, INFINITY, -INFINITY, 3e38f, 1e-37f, 5.0f};
-    const float eb[] = {1.0f, -3.0f, 1e-30f, 7e30f, 1e-38f, 3e-39f};
+    const float eb[] = {1.0f, -3.0f, 1e-30f, 7e30f, 1e-38f, 3e-39f, 0.0f, -0.0f, INFINITY, 1e-45f};
     for (float a : ea) for (float b : eb) {
         float r = a / b, q = bc::div_rcp(a, b, bc::rcp_prep(b));
         if (memcmp(&r, &q, 4) && !(std::isnan(r) && std::isnan(q))) bade++;
+        float w = bc::div_wide(a, b, 1.0 / (double)b);
+        if (memcmp(&r, &w, 4) && !(std::isnan(r) && std::isnan(w))) bade++;
     }
-    printf("%ld %ld %ld\n", badf, badd, bade);
+    printf("%ld %ld %ld %ld\n", badf, badd, bade, badw);
 }
 '''
 
@@ -51,5 +64,6 @@ def test_div_rcp_is_ieee_division(tmp_path):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     subprocess.run([hipcc, "-O2", "-ffp-contract=off", "-I", os.path.join(ROOT, "bioneuroncore_amd", "csrc"),
                     "-o", str(exe), str(src)], check=True, capture_output=True)
-    badf, badd, bade = map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split())
-    assert (badf, badd, bade) == (0, 0, 0)
+    badf, badd, bade, badw = map(int, subprocess.run([str(exe)], capture_output=True, text=True,
+                                                     check=True).stdout.split())
+    assert (badf, badd, bade, badw) == (0, 0, 0, 0)
