@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 from bioneuroncore_amd import backend, bion  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-PHASES = ("k_modifiers", "k_diffusion", "k_syn_pre", "k_syn_post", "k_syn_rrp")
+PHASES = ("k_modifiers", "k_diffusion", "k_syn_pre", "k_syn_post", "k_syn_plast", "k_syn_rrp")
 
 
 def algorithmic_bytes(net, stats):
@@ -51,7 +51,12 @@ def algorithmic_bytes(net, stats):
         "k_modifiers": n * (2 * seg_dyn + 8 + 4 + 8) + nmod * (1 + 8) + gates * 16 + params * 8,
         "k_diffusion": n * (2 * (8 + 7 * 8 + 7 * 8) + 8 + 4 + 4 + 4 + 8 + 2 * 4 * 3) + nc * 12,
         "k_syn_pre": ns * (4 + 4 + 4 + 4 + 4 + 2 * 4 + 1 + 2 * 1 + 4) + 64 * 0,
-        "k_syn_post": n * (2 * (8 + 7 * 8) + 7 * 8 + 8 + 4 + 15 * 4) + ns * (2 * 6 * 4 + 4 + 4 + 2 * 4),
+        # ordered pass: segment V, ions in/out, ibuf, cap, transmitter out; per synapse flags, mask,
+        # one cleft transmitter, receptor count, post area, receptor current out
+        "k_syn_post": n * (2 * (8 + 7 * 8) + 7 * 8 + 8 + 4 + 15 * 4) + ns * (1 + 2 + 4 + 1 + 4 + 4),
+        # flags, current, mask, tau, pp1 read; spine Ca, one cleft transmitter, BCM, CaMKII,
+        # calcineurin, weight read+written; depression written
+        "k_syn_plast": ns * (1 + 4 + 2 + 4 + 4 + 6 * 2 * 4 + 4),
         "k_syn_rrp": ns * (4 + 4 + 4 + 2 * 4 + 4),
     }
     rec = net.synapses["post_receptor_count"].astype(np.int64).sum()
@@ -110,7 +115,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--neurons", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 1 GiB device-to-device copies (the reference kernel for "
+                         "scaling FETCH_SIZE/WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one GPU; multi-GPU sharding is described in DESIGN.md, "
                                    "not launched", "n_gpus": args.gpus}))
@@ -132,7 +149,7 @@ def main():
         sim.upload_stimuli(stims[t])
         sim.run_tick(0.025)
     sim.set_phase_timing(True)
-    phase = np.zeros(5)
+    phase = np.zeros(len(PHASES))
     t0 = time.perf_counter()
     for t in range(args.warmup, args.warmup + args.steps):
         sim.upload_stimuli(stims[t])
@@ -156,7 +173,7 @@ def main():
                    "neurons": args.neurons, "segments": n_seg},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "phase_ms": {PHASES[k]: float(mean[k]) for k in range(5)},
+        "phase_ms": {PHASES[k]: float(mean[k]) for k in range(len(PHASES))},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

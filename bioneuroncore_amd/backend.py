@@ -56,7 +56,7 @@ class BchipStats(C.Structure):
                 ("num_tiles", C.c_uint32), ("modifier_slots", C.c_uint32), ("receptor_slots", C.c_uint32),
                 ("lds_bytes_per_tile", C.c_uint32), ("_pad", C.c_uint32),
                 ("unsupported_modifiers", C.c_uint64), ("unsupported_synapses", C.c_uint64),
-                ("order_hazard_synapses", C.c_uint64), ("last_phase_ms", C.c_float * 5)]
+                ("order_hazard_synapses", C.c_uint64), ("last_phase_ms", C.c_float * 6)]
 
 
 assert C.sizeof(BionDataC) == 168

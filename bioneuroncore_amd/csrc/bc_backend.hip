@@ -404,17 +404,17 @@ static int build_device(Sim& S) {
     uint32_t nrec = 1;
     for (uint32_t i = 0; i < ns; i++) nrec = std::max<uint32_t>(nrec, sy[i].post_receptor_count);
     d.nrec = nrec;
-    S.slot2syn.resize(ns);
-    std::iota(S.slot2syn.begin(), S.slot2syn.end(), 0u);
-    std::stable_sort(S.slot2syn.begin(), S.slot2syn.end(), [&](uint32_t a, uint32_t b) {
-        return S.host2dev[sy[a].post_segment] < S.host2dev[sy[b].post_segment];
-    });
-    std::vector<uint32_t> syn2slot(ns);
-    for (uint32_t k = 0; k < ns; k++) syn2slot[S.slot2syn[k]] = k;
-    std::vector<uint32_t> post_off(n + 1, 0);
+    // Incoming synapses of each device segment, in synapse index order (CSR).
+    std::vector<uint32_t> post_off(n + 1, 0), by_post(ns);
     for (uint32_t i = 0; i < ns; i++) post_off[S.host2dev[sy[i].post_segment] + 1]++;
     for (uint32_t i = 0; i < n; i++) post_off[i + 1] += post_off[i];
-    std::vector<uint32_t> walk(n);
+    {
+        std::vector<uint32_t> fill(post_off.begin(), post_off.end() - 1);
+        for (uint32_t i = 0; i < ns; i++) by_post[fill[S.host2dev[sy[i].post_segment]]++] = i;
+    }
+    // Lanes of the post-side pass: each 256-segment window ordered by chain length, longest
+    // first, so in every 64-lane wavefront the lanes still walking at step q are a prefix.
+    std::vector<uint32_t> walk(n), post_len(n);
     std::iota(walk.begin(), walk.end(), 0u);
     for (uint32_t w0 = 0; w0 < n; w0 += 256) {
         uint32_t w1 = std::min<uint32_t>(n, w0 + 256);
@@ -422,6 +422,31 @@ static int build_device(Sim& S) {
             return post_off[a + 1] - post_off[a] > post_off[b + 1] - post_off[b];
         });
     }
+    for (uint32_t l = 0; l < n; l++) post_len[l] = post_off[walk[l] + 1] - post_off[walk[l]];
+    // Slot order: wavefront by wavefront; inside one, step q of every lane still walking, lane by
+    // lane. Lane j of wavefront g reads step q at wave_base[g] + (steps before q) + j, so each
+    // step of the walk touches one contiguous run of slots.
+    const uint32_t nwave = (n + 63) / 64;
+    std::vector<uint32_t> wave_base(nwave + 1, 0);
+    S.slot2syn.assign(ns, 0);
+    {
+        uint32_t off = 0;
+        for (uint32_t g = 0; g < nwave; g++) {
+            wave_base[g] = off;
+            const uint32_t l0 = g * 64, l1 = std::min<uint32_t>(n, l0 + 64);
+            for (uint32_t q = 0;; q++) {
+                uint32_t cnt = 0;
+                for (uint32_t l = l0; l < l1 && post_len[l] > q; l++, cnt++)
+                    S.slot2syn[off + cnt] = by_post[post_off[walk[l]] + q];
+                if (cnt == 0) break;
+                off += cnt;
+            }
+        }
+        wave_base[nwave] = off;
+        if (off != ns) return BION_ERR_ALLOC;   // unreachable: every synapse has one slot
+    }
+    std::vector<uint32_t> syn2slot(ns);
+    for (uint32_t k = 0; k < ns; k++) syn2slot[S.slot2syn[k]] = k;
     std::vector<int32_t> src(ns, -1);
     {
         std::vector<int64_t> last_onto(n, -1);
@@ -436,6 +461,7 @@ static int build_device(Sim& S) {
     std::vector<uint8_t> y_flags(ns, 0), y_rcount(ns, 0), y_refr(ns, 0);
     std::vector<float> y_delay(ns), y_pa(ns), y_qa(ns), y_prel(ns), y_amt(ns), y_tau(ns), y_rmax(ns), y_rrec(ns);
     std::vector<float> y_w(ns), y_fac(ns), y_dep(ns), y_rrp(ns), y_bca(ns), y_sca(ns), y_ck(ns), y_cn(ns);
+    std::vector<float> y_pp1(ns);
     std::vector<uint16_t> y_mask(ns, 0);
     std::vector<uint8_t> y_rtype((size_t)nrec * ns, 0), y_rav((size_t)nrec * ns, 0);
     std::vector<double> y_rdens((size_t)nrec * ns, 0.0), y_rstate((size_t)nrec * 7 * ns, 0.0);
@@ -470,6 +496,7 @@ static int build_device(Sim& S) {
         y_w[k] = y.weight; y_fac[k] = y.facilitation; y_dep[k] = y.depression; y_refr[k] = y.refractory;
         y_rrp[k] = y.RRP; y_bca[k] = y.bouton_Ca; y_sca[k] = y.spine_Ca; y_ck[k] = y.spine_camkii;
         y_cn[k] = y.spine_calcineurin;
+        y_pp1[k] = pp1[S.host2dev[y.post_segment]];
         uint16_t mask = 0;
         for (int q = 0; q < 15; q++) {
             y_cleft[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
@@ -564,7 +591,8 @@ static int build_device(Sim& S) {
     d.y_rrp_rec = S.dupload(y_rrec);
     d.y_cleft_mask = S.dupload(y_mask);
     d.y_src = S.dupload(src);
-    d.post_off = S.dupload(post_off);
+    d.post_len = S.dupload(post_len);
+    d.wave_base = S.dupload(wave_base);
     d.walk = S.dupload(walk);
     d.y_rcount = S.dupload(y_rcount);
     d.y_rtype = S.dupload(y_rtype);
@@ -582,6 +610,8 @@ static int build_device(Sim& S) {
     d.y_camkii = S.dupload(y_ck);
     d.y_calcn = S.dupload(y_cn);
     d.y_relp = S.dalloc<float>(ns);
+    d.y_isyn = S.dalloc<float>(ns);
+    d.y_pp1 = S.dupload(y_pp1);
     d.y_ca_after = S.dalloc<double>(ns);
     d.d_mask = S.dalloc<uint8_t>(ns);
     d.d_arr = S.dalloc<uint64_t>((size_t)4 * ns);
@@ -723,15 +753,17 @@ static void run_tick(Sim& S, float dt) {
         k_syn_pre<<<std::min<uint32_t>(gy, 8192), 256, 0, S.stream>>>(d, tick, dt);
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[3], S.stream));
-    if (S.receptors_only) k_syn_post<true><<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
-    else k_syn_post<false><<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    if (S.receptors_only) k_syn_post_rec<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    else k_syn_post<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[4], S.stream));
-    if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, tick, dt);
+    if (d.ns) k_syn_plast<<<gy, 256, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[5], S.stream));
+    if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, tick, dt);
+    if (tm) HIPCHK(hipEventRecord(S.ev[6], S.stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(S.stream));
     if (tm)
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < 6; k++) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, S.ev[k], S.ev[k + 1]));
             S.last_ms[k] = ms;
@@ -1024,7 +1056,7 @@ int bchip_stats(void* sim, BchipStats* out) {
     out->unsupported_synapses = S.unsupported_syn;
     out->order_hazard_synapses = S.hazard_syn;
     out->lds_bytes_per_tile = (uint32_t)S.lds_bytes;
-    for (int k = 0; k < 5; k++) out->last_phase_ms[k] = S.last_ms[k];
+    for (int k = 0; k < 6; k++) out->last_phase_ms[k] = S.last_ms[k];
     return 0;
 }
 

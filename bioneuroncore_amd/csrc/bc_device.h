@@ -83,12 +83,18 @@ BC_DEV void mg_cache_init(SegRegs& s) {
     s.vkmg = s.V;
     s.emg = bc::exp(-0.062f * s.V);
 }
+// kSegV: the caller passes V == s.V (the double key) and both caches are initialised, so the
+// cached value is the exp itself, NaN V included (the cache was computed by the same expression).
+template <bool kSegV = false>
 BC_DEV double exp_mono(const SegRegs& s, double V, double vn) {
+    if (kSegV) return s.emd;
     if (V == s.vkf) return s.emf;
     if (V == s.vkd) return s.emd;
     return bc::exp(-vn);
 }
+template <bool kSegV = false>
 BC_DEV double exp_di(const SegRegs& s, double V, double vn) {
+    if (kSegV) return s.edd;
     if (V == s.vkf) return s.edf;
     if (V == s.vkd) return s.edd;
     return bc::exp(-vn);
@@ -106,6 +112,12 @@ struct ModSlot {
 };
 
 BC_DEV float eff_area(const ModSlot& m, const SegRegs& s) { return (m.a7 > 0.0f) ? m.a7 : s.area; }
+template <int N>
+BC_DEV void floor_n(double* a) {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (a[i] < 1e-9) a[i] = 1e-9;
+}
 BC_DEV void floor7(double* a) {
 #pragma unroll
     for (int i = 0; i < 7; i++)
@@ -113,22 +125,24 @@ BC_DEV void floor7(double* a) {
 }
 
 // ---- GHK (run_tick_common.h:274-312, modifier_helpers.h:48-75) ----
+template <bool kSegV = false>
 BC_DEV double ghk_mono(const SegRegs& s, double V, double ci, double co) {
     double vn = vn_mono(V);
     if (fabs(vn) < 1e-6) return kFaraday * (ci - co);
     double k = kFaraday / (kGas * kTemp);
     if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
     if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
-    double e = exp_mono(s, V, vn);
+    double e = exp_mono<kSegV>(s, V, vn);
     return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
 }
+template <bool kSegV = false>
 BC_DEV double ghk_di(const SegRegs& s, double V, double ci, double co) {
     double vn = vn_di(V);
     if (fabs(vn) < 1e-6) return 2.0 * kFaraday * (ci - co);
     double k = 4.0 * kFaraday / (kGas * kTemp);
     if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
     if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
-    double e = exp_di(s, V, vn);
+    double e = exp_di<kSegV>(s, V, vn);
     return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
 }
 BC_DEV double ghk_di_nocache(double V, double ci, double co) {
@@ -140,13 +154,14 @@ BC_DEV double ghk_di_nocache(double V, double ci, double co) {
     double e = bc::exp(-vn);
     return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
 }
+template <bool kSegV = false>
 BC_DEV double ghk_an(const SegRegs& s, double V, double ci, double co) {
     double vn = vn_mono(V);
     if (fabs(vn) < 1e-6) return kFaraday * (co - ci);
     double k = kFaraday / (kGas * kTemp);
     if (vn > 20.0) return k * kFaraday * V * 0.001 * co;
     if (vn < -20.0) return k * kFaraday * V * 0.001 * ci;
-    double e = exp_mono(s, V, vn);
+    double e = exp_mono<kSegV>(s, V, vn);
     return k * kFaraday * V * 0.001 * (co - ci * e) / (1.0 - e);
 }
 __host__ __device__ constexpr double p_single(float gamma_ps, float z, float cbar) {
@@ -561,6 +576,7 @@ BC_DEV double ch_nalcn(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologi
 // Ionotropic receptor (modifier_helpers.h:1213-1276).
 struct IonoK { float gamma, ec50, t_rise, t_decay; int nt; float pna, pk, pca; bool anion; float t_desens; };
 
+template <bool kSegV = false>
 BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     float area = eff_area(m, s);
     double V = s.V;
@@ -594,7 +610,7 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     const float scale = 1.0f;
     if (p.anion) {
         double pcl = (gamma == p.gamma) ? p_single(p.gamma, 1.0f, kCbarCl) : p_single(gamma, 1.0f, kCbarCl);
-        double fcl = ghk_an(s, V, s.in[3], s.ex[3]);
+        double fcl = ghk_an<kSegV>(s, V, s.in[3], s.ex[3]);
         double icl = pcl * fcl * N * op * scale * kAtoPa;
         s.in[3] += (icl * dt) / cpm;
         floor7(s.in);
@@ -603,13 +619,13 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     const P3 pp = (gamma == p.gamma) ? p_partial3(p.gamma, p.pna, p.pk, p.pca)
                                      : p_partial3(gamma, p.pna, p.pk, p.pca);
     double pna = pp.na, pk = pp.k, pca = pp.ca;
-    double ina = pna * ghk_mono(s, V, s.in[0], s.ex[0]) * N * op * scale * kAtoPa;
-    double ik = pk * ghk_mono(s, V, s.in[1], s.ex[1]) * N * op * scale * kAtoPa;
+    double ina = pna * ghk_mono<kSegV>(s, V, s.in[0], s.ex[0]) * N * op * scale * kAtoPa;
+    double ik = pk * ghk_mono<kSegV>(s, V, s.in[1], s.ex[1]) * N * op * scale * kAtoPa;
     double itot = ina + ik;
     s.in[0] -= (ina * dt) / cpm;
     s.in[1] -= (ik * dt) / cpm;
     if (p.pca > 0.0f) {
-        double ica = pca * ghk_di(s, V, s.in[2], s.ex[2]) * N * op * scale * kAtoPa;
+        double ica = pca * ghk_di<kSegV>(s, V, s.in[2], s.ex[2]) * N * op * scale * kAtoPa;
         itot += ica;
         s.in[2] -= (ica * dt) / (2.0f * cpm);
     }
@@ -617,10 +633,11 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     return -itot;
 }
 
+template <bool kSegV = false>   // kSegV: also requires mg_cache_init
 BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_biological.cpp:48-55
     const IonoK k = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
-    double I = iono(m, s, dt, k);
-    double emg = (s.V == s.vkmg) ? s.emg : bc::exp(-0.062f * s.V);
+    double I = iono<kSegV>(m, s, dt, k);
+    double emg = (kSegV || s.V == s.vkmg) ? s.emg : bc::exp(-0.062f * s.V);
     float mg = 1.0f / (1.0f + (1.0f / 3.57f) * emg);
     double r = I * mg;
     // nmda_cdi_coagonist (modifier_helpers.h:1281-1291)
@@ -636,15 +653,88 @@ BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_
     return r * (s2 * coag);
 }
 
-// Receptor-slot dispatch when every receptor in the network is AMPA, NMDA or GABA-A.
+// Receptor-slot dispatch when every receptor in the network is AMPA, NMDA or GABA-A; V is the
+// segment's V and both exp caches are initialised. Floors ions 0-3 only: the caller floors
+// ions 4-6 (untouched here) once after its walk, which floor7 per call amounts to. One body with the three parameter sets
+// selected per lane (the same operations as iono<true> with the constants of
+// dispatch(): AMPA and NMDA lanes of a wavefront run it together.
 BC_DEV double dispatch_receptor(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
-    if (type == M_NMDA) return rec_nmda(m, s, dt);
-    if (type == M_AMPA) {
-        const IonoK k = {8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f};
-        return iono(m, s, dt, k);
+    const bool nmda = type == M_NMDA, gaba = type == M_GABAA;
+    const float gamma0 = nmda ? 50.0f : (gaba ? 28.0f : 8.0f);
+    const float ec50 = gaba || nmda ? 1.0f : 0.5f;
+    const float t_rise = nmda ? 5.0f : (gaba ? 1.0f : 0.5f);
+    const float t_decay = nmda ? 50.0f : (gaba ? 5.0f : 3.0f);
+    const float t_desens = nmda ? 0.0f : (gaba ? 2000.0f : 3.0f);
+    float area = eff_area(m, s);
+    double V = s.V;
+    double cpm = s.cpm;
+    float gamma = gamma0;
+    if (m.av > 0 && m.av < 7) {
+        double g = m.get(m.av);
+        if (g > 0.0f) gamma = g;
     }
-    const IonoK k = {28.0f, 1.0f, 1.0f, 5.0f, 1, 0.0f, 0.0f, 0.0f, true, 2000.0f};
-    return iono(m, s, dt, k);
+    float nt = gaba ? s.nt[1] : s.nt[0];
+    float r_inf = nt / (nt + ec50);
+    double s0 = m.get(0);
+    float r = s0;
+    float tau = (r_inf > r) ? t_rise : t_decay;
+    s0 += (r_inf - r) * dt / tau;
+    if (s0 < 0.0f) s0 = 0.0f;
+    if (s0 > 1.0f) s0 = 1.0f;
+    m.set(0, s0);
+    float op = s0;
+    if (t_desens > 0.0f) {
+        float d = m.get(1);
+        float d_inf = nt / (nt + ec50);
+        d += (d_inf - d) * dt / t_desens;
+        if (d < 0.0f) d = 0.0f;
+        if (d > 1.0f) d = 1.0f;
+        m.set(1, d);
+        op *= (1.0f - d);
+    }
+    double N = m.dens * area;
+    const float scale = 1.0f;
+    if (gaba) {
+        constexpr double kPcl = p_single(28.0f, 1.0f, kCbarCl);
+        double pcl = (gamma == 28.0f) ? kPcl : p_single(gamma, 1.0f, kCbarCl);
+        double fcl = ghk_an<true>(s, V, s.in[3], s.ex[3]);
+        double icl = pcl * fcl * N * op * scale * kAtoPa;
+        s.in[3] += (icl * dt) / cpm;
+        floor_n<4>(s.in);   // ions 4-6: floored by the caller once after its walk
+        return -icl;
+    }
+    constexpr P3 kPA = p_partial3(8.0f, 0.5f, 0.5f, 0.0f);
+    constexpr P3 kPN = p_partial3(50.0f, 0.08f, 0.08f, 0.84f);
+    const float fna = nmda ? 0.08f : 0.5f, fca = nmda ? 0.84f : 0.0f;
+    P3 pp;
+    if (gamma == gamma0) pp = nmda ? kPN : kPA;
+    else pp = p_partial3(gamma, fna, fna, fca);
+    double ina = pp.na * ghk_mono<true>(s, V, s.in[0], s.ex[0]) * N * op * scale * kAtoPa;
+    double ik = pp.k * ghk_mono<true>(s, V, s.in[1], s.ex[1]) * N * op * scale * kAtoPa;
+    double itot = ina + ik;
+    s.in[0] -= (ina * dt) / cpm;
+    s.in[1] -= (ik * dt) / cpm;
+    if (nmda) {   // pca > 0
+        double ica = pp.ca * ghk_di<true>(s, V, s.in[2], s.ex[2]) * N * op * scale * kAtoPa;
+        itot += ica;
+        s.in[2] -= (ica * dt) / (2.0f * cpm);
+    }
+    floor_n<4>(s.in);
+    double I = -itot;
+    if (!nmda) return I;
+    // Mg2+ block and nmda_cdi_coagonist, as rec_nmda
+    float mg = 1.0f / (1.0f + (1.0f / 3.57f) * s.emg);
+    double rr = I * mg;
+    double ca_um = s.in[2] * kMmToUm;
+    double cdi_inf = 1.0 - 0.8f * ca_um / (ca_um + 1.0f);
+    double s2 = m.get(2);
+    float tau2 = (cdi_inf < s2) ? 500.0f : 9000.0f;
+    s2 += (cdi_inf - s2) * dt / tau2;
+    if (s2 < 0.0f) s2 = 0.0f;
+    if (s2 > 1.0f) s2 = 1.0f;
+    m.set(2, s2);
+    float coag = 10.0f / (10.0f + 1.0f);
+    return rr * (s2 * coag);
 }
 
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0

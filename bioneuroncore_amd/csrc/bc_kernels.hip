@@ -2,13 +2,17 @@
 //
 //   k_modifiers      step 1: one lane per segment, modifier slots in order, stimuli
 //   k_diffusion      step 2.5: one workgroup per tile of whole islands, all state in LDS
+//   k_pre_summary    step 3: per presynaptic segment, the V-only values its synapses share
 //   k_syn_pre        step 3, presynaptic half: delay-ring arrivals, bouton Ca, release draw
-//   k_syn_post       step 3, post-synaptic half in synapse order per post segment + step 4
+//   k_syn_post(_rec) step 3, post-synaptic receptors in synapse order per post segment + step 4
+//   k_syn_plast      step 3, per-synapse spine Ca, cleft clearance, BCM/CaMKII/weight
 //   k_syn_rrp        step 3, vesicle-pool recovery/depletion and delay-ring enqueue
 //
 // Ordering facts that make the split exact (reference src/simulation_host.cpp:322-593):
 // a synapse's pre-side reads only pre V_m, extra Ca and Cav2.1 state, none of which step 3
-// writes; the post-side of a segment depends only on its own incoming synapses in index order;
+// writes; the post-side of a segment depends only on its own incoming synapses in index order,
+// and of each synapse's post-side only the receptors feed that order (the rest reads the
+// synapse's own state and receptor current);
 // the one cross-chain read is rrp_do_recovery's pre intra Ca, which k_syn_rrp takes from the
 // recorded post-chain history (y_ca_after) of the last earlier synapse onto that segment.
 #include <hip/hip_runtime.h>
@@ -692,9 +696,8 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt)
 // ------------------------------------------------------------------ step 3 post-side + step 4
 // kReceptorsOnly: every receptor slot holds AMPA, NMDA or GABA-A (a leaner dispatch).
 template <bool kReceptorsOnly>
-__global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* __restrict__ st, uint64_t tick,
-                                                  float dt) {
-    exp_table_stage();
+__device__ __attribute__((always_inline)) inline void syn_post(const Net& d, const GpuStimulusData* __restrict__ st,
+                                                              uint64_t tick, float dt) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= d.n) return;
     const uint32_t i = d.walk[lane];
@@ -703,15 +706,26 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
     double ibuf = d.ibuf[i];
     double V = d.V[i];
     if (d.ns > 0) {
+        // With AMPA/NMDA/GABA-A receptors only, the walk reads and writes ions 0-3 and reads
+        // transmitters 0-1; ions 4-6 are only floored and transmitters 2-14 only summed, both
+        // done after the walk so the walk keeps fewer registers.
+        constexpr int kIonWalk = kReceptorsOnly ? 4 : 7;
+        constexpr int kNtWalk = kReceptorsOnly ? 2 : 15;
         SegRegs s;
         s.V = V;
 #pragma unroll
         for (int k = 0; k < 7; k++) {
-            s.in[k] = d.in[k * P + i];
+            s.in[k] = k < kIonWalk ? d.in[k * P + i] : 1.0;
             s.ex[k] = d.ex[k * P + i];
         }
         s.area = d.area[i];
         s.vol = d.vol[i];
+        if (!kReceptorsOnly) {   // channels in receptor slots read these and write ex / ATP
+            s.atp = d.atp[i];
+            s.pkc = d.pkc[i];
+            s.pka = d.pka[i];
+            s.camp = d.camp[i];
+        }
         seg_consts_init(s);
         ghk_cache_init(s);
         mg_cache_init(s);
@@ -720,11 +734,15 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
         if (fl & SF_BATH_NMDA) s.nt[0] = 1.0f;
         if (fl & SF_BATH_5HT) s.nt[3] = 0.0f;
         const bool my = fl & SF_MYELIN;
-        const uint32_t y0 = d.post_off[i], y1 = d.post_off[i + 1];
-        const float pp1 = d.pp1[i];
-        float last_tau = __builtin_nanf("");   // clearance exp reused while tau repeats
-        double last_E = 0.0;
-        for (uint32_t y = y0; y < y1; y++) {
+        const uint32_t len = d.post_len[lane];
+        const uint32_t wl = threadIdx.x & 63;
+        uint32_t off = d.wave_base[lane >> 6];
+        bool ran = false;
+        for (uint32_t q = 0; q < len; q++) {
+            // the lanes still walking are lanes 0..cnt-1 of this wavefront (sorted by length)
+            const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
+            const uint32_t y = off + wl;
+            off += cnt;
             const uint8_t yf = d.y_flags[y];
             if (yf & YF_SKIP) {
                 if (fl & SF_WATCHED) d.y_ca_after[y] = s.in[2];
@@ -732,7 +750,7 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
             }
             const uint16_t cm = d.y_cleft_mask[y];
 #pragma unroll
-            for (int k = 0; k < 15; k++)
+            for (int k = 0; k < kNtWalk; k++)
                 if (cm & (1u << k)) s.nt[k] += d.y_cleft[k * NS + y];
             float isyn = 0.0f;
             const uint32_t rc = d.y_rcount[y];
@@ -747,74 +765,40 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
                 m.a7 = parea;
                 m.av = d.y_rav[j * NS + y];
                 isyn += kReceptorsOnly ? dispatch_receptor(rt, m, s, dt) : dispatch(rt, m, s, dt);
+                ran = true;
             }
             ibuf += isyn;
-            // spine Ca (modifier_helpers.h:808-816)
-            float sca = d.y_sca[y];
-            {
-                float ia = isyn < 0.0f ? -isyn : isyn;
-                float icaf = ia * 0.1f;
-                float influx = icaf * dt * 1000.0f / (2.0f * kFaraday * 0.05f * 100.0f);
-                sca += influx;
-                sca += (0.0001f - sca) * dt / 50.0f;
-                if (sca < 0.0f) sca = 0.0f;
-            }
-            // cleft clearance (simulation_host.cpp:505-510)
-            {
-                const float tau = d.y_tau[y];
-                if (!(tau == last_tau)) { last_tau = tau; last_E = bc::exp(-dt / tau); }
-                const double E = last_E;
-#pragma unroll
-                for (int k = 0; k < 15; k++)
-                    if (cm & (1u << k)) {
-                        float v = d.y_cleft[k * NS + y];
-                        v *= E;
-                        if (v < 0.0f) v = 0.0f;
-                        d.y_cleft[k * NS + y] = v;
-                    }
-            }
-            // BCM threshold + spine CaMKII/calcineurin + weight (simulation_host.cpp:527-576)
-            float facil = d.y_fac[y];
-            float cu = sca * 1000.0f;
-            float csq = cu * cu;
-            float tau_theta = 300000.0f;
-            facil += (csq - facil) * dt / tau_theta;
-            if (facil < 0.0f) facil = 0.0f;
-            float dep = facil + 0.01f;
-            float ck = d.y_camkii[y], cn = d.y_calcn[y];
-            {
-                float c1 = sca * kMmToUm;
-                float c4 = c1 * c1 * c1 * c1;
-                float ck_inf = c4 / (c4 + 1.0f + 1e-9f);
-                ck += (ck_inf - ck) * dt / 2000.0f;
-                if (ck < 0.0f) ck = 0.0f;
-                if (ck > 1.0f) ck = 1.0f;
-                float c2 = c1 * c1;
-                float cn_inf = c2 / (c2 + 0.05f * 0.05f + 1e-9f);
-                cn += (cn_inf - cn) * dt / 300.0f;
-                if (cn < 0.0f) cn = 0.0f;
-                if (cn > 1.0f) cn = 1.0f;
-            }
-            float dw = 0.0f;
-            if (ck > 0.0f) dw += 0.0003f * ck * dt;
-            if (cn > 0.0f) dw -= 0.0001f * cn * pp1 * dt;
-            if (yf & YF_INHIBITORY) dw = -dw;
-            float w = d.y_weight[y] + dw;
-            if (w != w) w = 0.5f;
-            if (facil != facil) facil = 0.0f;
-            if (dep != dep) dep = 0.01f;
-            if (w < 0.01f) w = 0.01f;
-            if (w > 10.0f) w = 10.0f;
-            d.y_sca[y] = sca;
-            d.y_fac[y] = facil;
-            d.y_dep[y] = dep;
-            d.y_camkii[y] = ck;
-            d.y_calcn[y] = cn;
-            d.y_weight[y] = w;
+            d.y_isyn[y] = isyn;
             if (fl & SF_WATCHED) d.y_ca_after[y] = s.in[2];
         }
+        if (kReceptorsOnly) {
+            // the same chain again, for the transmitters the receptors do not read
+            uint32_t off2 = d.wave_base[lane >> 6];
+            for (uint32_t q = 0; q < len; q++) {
+                const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
+                const uint32_t y = off2 + wl;
+                off2 += cnt;
+                if (d.y_flags[y] & YF_SKIP) continue;
+                const uint32_t cm = d.y_cleft_mask[y];
+                if (!(cm & ~3u)) continue;
 #pragma unroll
-        for (int k = 0; k < 7; k++) d.in[k * P + i] = s.in[k];
+                for (int k = 2; k < 15; k++)
+                    if (cm & (1u << k)) s.nt[k] += d.y_cleft[k * NS + y];
+            }
+            // floor7 of every receptor call left ions 4-6 floored once and otherwise untouched
+#pragma unroll
+            for (int k = 4; k < 7; k++) {
+                double v = d.in[k * P + i];
+                if (ran && v < 1e-9) d.in[k * P + i] = 1e-9;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kIonWalk; k++) d.in[k * P + i] = s.in[k];
+        if (!kReceptorsOnly) {
+#pragma unroll
+            for (int k = 0; k < 7; k++) d.ex[k * P + i] = s.ex[k];
+            d.atp[i] = s.atp;
+        }
 #pragma unroll
         for (int k = 0; k < 15; k++) d.nt[k * P + i] = s.nt[k];
     } else {
@@ -845,6 +829,94 @@ __global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* 
         if (V > 150.0f) V = 150.0f;
     }
     d.V[i] = V;
+}
+
+// AMPA/NMDA/GABA-A receptors only: four wavefronts per SIMD fit in the register budget.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_syn_post_rec(
+    Net d, const GpuStimulusData* __restrict__ st, uint64_t tick, float dt) {
+    exp_table_stage();
+    syn_post<true>(d, st, tick, dt);
+}
+// Any receptor type (full modifier dispatch).
+__global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* __restrict__ st, uint64_t tick,
+                                                  float dt) {
+    exp_table_stage();
+    syn_post<false>(d, st, tick, dt);
+}
+
+// ------------------------------------------------------------------ step 3, synapse plasticity
+// The part of the post-side step that depends only on the synapse's own state and the receptor
+// current the ordered pass stored: spine Ca, cleft clearance, BCM threshold, spine
+// CaMKII/calcineurin and the weight (simulation_host.cpp:505-576). One lane per synapse.
+__global__ __launch_bounds__(256) void k_syn_plast(Net d, float dt) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= d.ns) return;
+    const uint8_t yf = d.y_flags[y];
+    if (yf & YF_SKIP) return;
+    const float isyn = d.y_isyn[y];
+    const uint16_t cm = d.y_cleft_mask[y];
+    const size_t NS = d.ns;
+    // spine Ca (modifier_helpers.h:808-816)
+    float sca = d.y_sca[y];
+    {
+        float ia = isyn < 0.0f ? -isyn : isyn;
+        float icaf = ia * 0.1f;
+        float influx = icaf * dt * 1000.0f / (2.0f * kFaraday * 0.05f * 100.0f);
+        sca += influx;
+        sca += (0.0001f - sca) * dt / 50.0f;
+        if (sca < 0.0f) sca = 0.0f;
+    }
+    // cleft clearance (simulation_host.cpp:505-510)
+    {
+        const float tau = d.y_tau[y];
+        const double E = bc::exp_t(-dt / tau, ExpTabGlobal{});
+#pragma unroll
+        for (int k = 0; k < 15; k++)
+            if (cm & (1u << k)) {
+                float v = d.y_cleft[k * NS + y];
+                v *= E;
+                if (v < 0.0f) v = 0.0f;
+                d.y_cleft[k * NS + y] = v;
+            }
+    }
+    // BCM threshold + spine CaMKII/calcineurin + weight (simulation_host.cpp:527-576)
+    float facil = d.y_fac[y];
+    float cu = sca * 1000.0f;
+    float csq = cu * cu;
+    float tau_theta = 300000.0f;
+    facil += (csq - facil) * dt / tau_theta;
+    if (facil < 0.0f) facil = 0.0f;
+    float dep = facil + 0.01f;
+    float ck = d.y_camkii[y], cn = d.y_calcn[y];
+    {
+        float c1 = sca * kMmToUm;
+        float c4 = c1 * c1 * c1 * c1;
+        float ck_inf = c4 / (c4 + 1.0f + 1e-9f);
+        ck += (ck_inf - ck) * dt / 2000.0f;
+        if (ck < 0.0f) ck = 0.0f;
+        if (ck > 1.0f) ck = 1.0f;
+        float c2 = c1 * c1;
+        float cn_inf = c2 / (c2 + 0.05f * 0.05f + 1e-9f);
+        cn += (cn_inf - cn) * dt / 300.0f;
+        if (cn < 0.0f) cn = 0.0f;
+        if (cn > 1.0f) cn = 1.0f;
+    }
+    float dw = 0.0f;
+    if (ck > 0.0f) dw += 0.0003f * ck * dt;
+    if (cn > 0.0f) dw -= 0.0001f * cn * d.y_pp1[y] * dt;
+    if (yf & YF_INHIBITORY) dw = -dw;
+    float w = d.y_weight[y] + dw;
+    if (w != w) w = 0.5f;
+    if (facil != facil) facil = 0.0f;
+    if (dep != dep) dep = 0.01f;
+    if (w < 0.01f) w = 0.01f;
+    if (w > 10.0f) w = 10.0f;
+    d.y_sca[y] = sca;
+    d.y_fac[y] = facil;
+    d.y_dep[y] = dep;
+    d.y_camkii[y] = ck;
+    d.y_calcn[y] = cn;
+    d.y_weight[y] = w;
 }
 
 // ------------------------------------------------------------------ step 3, vesicle pool

@@ -4,8 +4,10 @@
 // Segments are stored in "device order": connected islands (components of the
 // IntraConnection graph) are contiguous and packed into tiles of at most kTileSeg segments /
 // kTileConn connections; one workgroup owns one tile in the electrodiffusion kernel.
-// Synapses are stored in "slot order": sorted by (post segment in device order, synapse
-// index), so the ordered post-side walk of one segment reads a contiguous range.
+// Synapses are stored in "slot order": wavefront-interleaved for the post-side pass. Lane l of
+// that pass walks the incoming synapses of segment walk[l] in synapse index order; lanes are
+// sorted by chain length inside each wavefront, and step q of a wavefront's lanes occupies one
+// contiguous run of slots (see k_syn_post).
 #pragma once
 #include <stdint.h>
 #include <stddef.h>
@@ -127,9 +129,10 @@ struct Net {
     const float* y_rrp_rec;
     const uint16_t* y_cleft_mask;
     const int32_t* y_src;        // slot of the last earlier synapse whose post is my pre, -1
-    const uint32_t* post_off;    // [n+1] per device segment
     const uint32_t* walk;        // lane -> segment for the post-side pass: each 256-segment
                                  // window ordered by incoming-synapse count, longest first
+    const uint32_t* post_len;    // per lane: incoming synapses of walk[lane]
+    const uint32_t* wave_base;   // per 64-lane wavefront: its first slot
     uint8_t* y_rcount;
     const uint8_t* y_rtype;      // [nrec][ns]
     const uint8_t* y_rav;
@@ -145,6 +148,8 @@ struct Net {
     float* y_sca;
     float* y_camkii;
     float* y_calcn;
+    float* y_isyn;               // receptor current of the synapse this tick (post pass -> plasticity)
+    const float* y_pp1;          // PP1 activity of the post segment
     float* y_relp;               // release probability of a deferred release this tick, 0 if none
     double* y_ca_after;          // post intra Ca after this synapse's receptors
     uint8_t* d_mask;             // occupied delay slots (bit s)

@@ -62,7 +62,8 @@ typedef struct BchipStats {
     uint64_t unsupported_modifiers;   // modifier/receptor instances outside the device set (run as 0 current)
     uint64_t unsupported_synapses;    // electrical synapses, autoreceptors, eCB (not run on device)
     uint64_t order_hazard_synapses;   // immediate-release synapses whose pre segment is an earlier post
-    float last_phase_ms[5];           // step 1, 2.5, 3-pre, 3-post+4, 3-rrp of the last timed tick
+    float last_phase_ms[6];           // step 1, 2.5, 3-pre, 3-post+4, 3-plasticity, 3-rrp of the last
+                                      // timed tick
 } BchipStats;
 
 int bchip_load_bion_data(void* sim, const BionData* src);   // load from in-memory AoS (copied)

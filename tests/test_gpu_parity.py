@@ -63,6 +63,41 @@ def test_config2_small_bitexact():
     assert o.spikes().sum() > 0
 
 
+@pytest.mark.parametrize("generic", [False, True])
+def test_post_walk_paths(generic):
+    """The post-side walk with wavefronts of uneven chain lengths: transmitters other than the
+    receptors' own in the clefts (summed after the walk), ions 4-6 below the floor, myelinated
+    post segments (receptors skipped), and - with `generic` - one post slot holding a channel
+    instead of a receptor, which selects the full-dispatch kernel."""
+    net = bion.config2(n_neurons=12, syn_per_neuron=70.0, seed=31)
+    rng = np.random.default_rng(31)
+    y = net.synapses
+    pick = rng.random(len(y)) < 0.3
+    for k in (2, 3, 5, 9, 14):
+        sel = pick & (rng.random(len(y)) < 0.5)
+        y["cleft_neurotransmitter"][sel, k] = rng.uniform(0.01, 0.5, sel.sum()).astype(np.float32)
+    s = net.segments
+    low = rng.random(len(s)) < 0.2
+    for k in (4, 5, 6):
+        s["intra_ions"][low, k] = 1e-12
+    posts = np.unique(y["post_segment"])
+    for i in rng.choice(posts, 6, replace=False):   # myelin modifier (type 4) on a free slot
+        c = s["modifier_count"][i]
+        if c < s["modifiers"].shape[1]:
+            s["modifiers"][i, c]["type"] = 4
+            s["modifier_count"][i] = c + 1
+    if generic:
+        j = int(np.flatnonzero(y["post_receptor_count"] == 1)[0])
+        y["post_receptors"][j, 1]["type"] = bion.K2P
+        y["post_receptors"][j, 1]["density"] = 4.17
+        y["post_receptor_count"][j] = 2
+    pulses = bion.poisson_soma_pulses(net, 200.0, 120, DT, 300.0, 32)
+    clamp = H.soma_clamp(net, 150.0)
+    cur = [np.concatenate([clamp, p]) for p in pulses]
+    gst = [backend.make_stimuli(current=c) for c in cur]
+    _compare(net, 120, lambda t: H.Stimuli(current=cur[t]), lambda t: gst[t])
+
+
 def test_all_tile_classes_and_stimulus_kinds():
     """64-, 128- and 256-lane tiles in one network (8-, 100- and 200-segment islands), with
     current pulses, ramps, a voltage clamp and one-shot ion/ligand injections."""
