@@ -358,6 +358,12 @@ static int build_device(Sim& S) {
             csr_ent[csr_off[db] + fill[db]++] = (conn_local[i] << 1) | 1u;
         }
     }
+    // per tile: the largest incidence count of its segments (selects the sweep variant)
+    std::vector<uint32_t> tile_maxinc(S.ntiles, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t& m = tile_maxinc[seg_tile[i]];
+        m = std::max<uint32_t>(m, csr_off[i + 1] - csr_off[i]);
+    }
 
     // ---- segment SoA ----
     const uint32_t npad = (n + 63) & ~63u;
@@ -565,9 +571,10 @@ static int build_device(Sim& S) {
         const uint32_t* tns = S.dupload(tile_nseg);
         const uint32_t* tc0 = S.dupload(tile_conn0);
         const uint32_t* tnc = S.dupload(tile_nconn);
+        const uint32_t* tmi = S.dupload(tile_maxinc);
         for (int k = 0; k < kTileClasses; k++) {
             uint32_t o = S.class_tile0[k];
-            S.tiles[k] = TileSet{ts0 + o, tns + o, tc0 + o, tnc + o};
+            S.tiles[k] = TileSet{ts0 + o, tns + o, tc0 + o, tnc + o, tmi + o};
         }
     }
     d.conn_ab = S.dupload(conn_ab);

@@ -16,6 +16,7 @@
 // the one cross-chain read is rrp_do_recovery's pre intra Ca, which k_syn_rrp takes from the
 // recorded post-chain history (y_ca_after) of the last earlier synapse onto that segment.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "bc_device.h"
 #include "bc_state.h"
 #include "../../include/biocore_abi.h"
@@ -144,12 +145,12 @@ struct DiffLds {
         struct {
             double sig[TS];        // sigma_e per segment
             double gphi[TS];       // phi conductance per connection
-            double phi[2][TS];
+            double phi[2][TS + 1]; // entry TS of each plane: 0, the padding target
         } p;                       // phi_e
         struct {
             double V[TS];
             float flux[7][TS];     // per connection
-            float vf[2][TS];       // Jacobi iterates
+            float vf[2][TS + 1];   // Jacobi iterates; entry TS of each plane: 0
             float cg[TS];
             float cterm[TS];
         } i;                       // intracellular
@@ -251,6 +252,10 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         diag = sg * area / 1.0f;
         L.u.p.phi[0][t] = 0.0;
     }
+    if (t == 0) {
+        L.u.p.phi[0][TS] = 0.0;
+        L.u.p.phi[1][TS] = 0.0;
+    }
     __syncthreads();
     for (uint32_t e = t; e < nent; e += TS) {
         uint32_t en = d.csr_ent[E0 + e];
@@ -260,56 +265,62 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     double* gphi = L.u.p.gphi;
     if (cact) gphi[t] = 0.5 * (sig[ca_] + sig[cb_]) * cac / cdist;
     __syncthreads();
-    // The first kRegDeg incidences of a lane stay in registers across the sweeps (trees have
-    // at most three); the rest are read from LDS in order after them.
-    // Unused register slots point at the lane's own entry and are never added, so every read
-    // of a sweep issues at once and the sum keeps the reference's order.
-    constexpr int kRegDeg = 4;
-    double pg[kRegDeg];
-    uint32_t po[kRegDeg];
-#pragma unroll
-    for (int q = 0; q < kRegDeg; q++) { pg[q] = 0.0; po[q] = t; }
-    int pdeg = 0;
-    uint32_t pe_rest = e1;
-    if (act)
-        for (uint32_t e = e0; e < e1; e++) {
-            uint32_t en = L.ent[e];
-            double gg = gphi[ent_conn(en)];
-            diag += gg;
-            if (gg <= 0.0) continue;
-            if (pdeg < kRegDeg) {
-                pg[pdeg] = gg;
-                po[pdeg] = ent_other(L, en);
-                pdeg++;
-                if (pdeg == kRegDeg) pe_rest = e + 1;
-            }
-        }
+    // The first D incidences of a lane stay in registers across the sweeps, D = 2, 3 or 4 from
+    // the tile's largest incidence count (chains 2, trees 3); the rest are read from LDS in order
+    // after them. Unused register slots read the zero entry with weight 0: a sum that starts at
+    // +0 is never -0, so adding +0 leaves it unchanged, and every read of a sweep issues at once.
+    const uint32_t maxinc = ts.maxinc[tile];
+    const int dsel = maxinc > 3 ? 4 : (maxinc > 2 ? 3 : 2);
     int cur = 0;
-    for (int it = 0; it < 50; it++) {
-        if (act) {
-            double pv[kRegDeg];
+    auto phi_sweeps = [&](auto dc) {
+        constexpr int D = decltype(dc)::value;
+        double pg[D];
+        uint32_t po[D];
 #pragma unroll
-            for (int q = 0; q < kRegDeg; q++) pv[q] = L.u.p.phi[cur][po[q]];
-            double pn = 0.0;
-#pragma unroll
-            for (int q = 0; q < kRegDeg; q++) {
-                double s = pn + pg[q] * pv[q];
-                pn = q < pdeg ? s : pn;
-            }
-            for (uint32_t e = pe_rest; e < e1; e++) {
+        for (int q = 0; q < D; q++) { pg[q] = 0.0; po[q] = TS; }
+        int pdeg = 0;
+        uint32_t pe_rest = e1;
+        if (act)
+            for (uint32_t e = e0; e < e1; e++) {
                 uint32_t en = L.ent[e];
                 double gg = gphi[ent_conn(en)];
+                diag += gg;
                 if (gg <= 0.0) continue;
-                pn += gg * L.u.p.phi[cur][ent_other(L, en)];
+                if (pdeg < D) {
+                    pg[pdeg] = gg;
+                    po[pdeg] = ent_other(L, en);
+                    pdeg++;
+                    if (pdeg == D) pe_rest = e + 1;
+                }
             }
-            double ph = (diag > 1e-9) ? (pn - ibuf) / diag : 0.0;
-            if (ph > 10.0) ph = 10.0;
-            if (ph < -10.0) ph = -10.0;
-            L.u.p.phi[cur ^ 1][t] = ph;
+        const double ydiag = rcp_prep(diag);   // diag is fixed across the sweeps
+        for (int it = 0; it < 50; it++) {
+            if (act) {
+                double pv[D];
+#pragma unroll
+                for (int q = 0; q < D; q++) pv[q] = L.u.p.phi[cur][po[q]];
+                double pn = 0.0;
+#pragma unroll
+                for (int q = 0; q < D; q++) pn += pg[q] * pv[q];
+                for (uint32_t e = pe_rest; e < e1; e++) {
+                    uint32_t en = L.ent[e];
+                    double gg = gphi[ent_conn(en)];
+                    if (gg <= 0.0) continue;
+                    pn += gg * L.u.p.phi[cur][ent_other(L, en)];
+                }
+                double ph = (diag > 1e-9) ? div_rcp(pn - ibuf, diag, ydiag) : 0.0;
+                if (ph > 10.0) ph = 10.0;
+                if (ph < -10.0) ph = -10.0;
+                L.u.p.phi[cur ^ 1][t] = ph;
+            }
+            __syncthreads();
+            cur ^= 1;
         }
-        __syncthreads();
-        cur ^= 1;
-    }
+    };
+    if (dsel == 2) phi_sweeps(std::integral_constant<int, 2>{});
+    else if (dsel == 3) phi_sweeps(std::integral_constant<int, 3>{});
+    else phi_sweeps(std::integral_constant<int, 4>{});
+
     // phi of both ends of each connection stays in registers; the phi planes are dead after.
     double pha = 0.0, phb = 0.0;
     if (cact) {
@@ -318,6 +329,10 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     }
     __syncthreads();
     if (act) L.u.i.V[t] = vcur;
+    if (t == 0) {
+        L.u.i.vf[0][TS] = 0.0f;
+        L.u.i.vf[1][TS] = 0.0f;
+    }
     __syncthreads();
 
     // ---- intracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:151-238) ----
@@ -380,50 +395,53 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             L.u.i.vf[0][t] = vold;
         }
         __syncthreads();
-        int vc = 0;
         float vnew = vold;
         const float den = cdt + gdiag;
         const float base = cdt * vold + inpd;
         const double rden = 1.0 / (double)den;
-        float jg[kRegDeg];
-        uint32_t jo[kRegDeg];
+        auto jacobi = [&](auto dc) {   // register incidences as in phi_sweeps
+            constexpr int D = decltype(dc)::value;
+            float jg[D];
+            uint32_t jo[D];
 #pragma unroll
-        for (int q = 0; q < kRegDeg; q++) { jg[q] = 0.0f; jo[q] = t; }
-        int jdeg = 0;
-        uint32_t je_rest = e1;
-        if (act)
-            for (uint32_t e = e0; e < e1 && jdeg < kRegDeg; e++) {
-                uint32_t en = L.ent[e];
-                uint32_t c = ent_conn(en);
-                if (!ent_intra(en)) continue;
-                jg[jdeg] = L.u.i.cg[c];
-                jo[jdeg] = ent_other(L, en);
-                jdeg++;
-                if (jdeg == kRegDeg) je_rest = e + 1;
-            }
-        for (int it = 0; it < 50; it++) {
-            if (act) {
-                float jv[kRegDeg];
-#pragma unroll
-                for (int q = 0; q < kRegDeg; q++) jv[q] = L.u.i.vf[vc][jo[q]];
-                float off = 0.0f;
-#pragma unroll
-                for (int q = 0; q < kRegDeg; q++) {
-                    float s = off + jg[q] * jv[q];
-                    off = q < jdeg ? s : off;
-                }
-                for (uint32_t e = je_rest; e < e1; e++) {
+            for (int q = 0; q < D; q++) { jg[q] = 0.0f; jo[q] = TS; }
+            int jdeg = 0;
+            uint32_t je_rest = e1;
+            if (act)
+                for (uint32_t e = e0; e < e1 && jdeg < D; e++) {
                     uint32_t en = L.ent[e];
                     uint32_t c = ent_conn(en);
                     if (!ent_intra(en)) continue;
-                    off += L.u.i.cg[c] * L.u.i.vf[vc][ent_other(L, en)];
+                    jg[jdeg] = L.u.i.cg[c];
+                    jo[jdeg] = ent_other(L, en);
+                    jdeg++;
+                    if (jdeg == D) je_rest = e + 1;
                 }
-                vnew = div_wide(base + off, den, rden);
-                L.u.i.vf[vc ^ 1][t] = vnew;
+            int vc = 0;
+            for (int it = 0; it < 50; it++) {
+                if (act) {
+                    float jv[D];
+#pragma unroll
+                    for (int q = 0; q < D; q++) jv[q] = L.u.i.vf[vc][jo[q]];
+                    float off = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < D; q++) off += jg[q] * jv[q];
+                    for (uint32_t e = je_rest; e < e1; e++) {
+                        uint32_t en = L.ent[e];
+                        uint32_t c = ent_conn(en);
+                        if (!ent_intra(en)) continue;
+                        off += L.u.i.cg[c] * L.u.i.vf[vc][ent_other(L, en)];
+                    }
+                    vnew = div_wide(base + off, den, rden);
+                    L.u.i.vf[vc ^ 1][t] = vnew;
+                }
+                __syncthreads();
+                vc ^= 1;
             }
-            __syncthreads();
-            vc ^= 1;
-        }
+        };
+        if (dsel == 2) jacobi(std::integral_constant<int, 2>{});
+        else if (dsel == 3) jacobi(std::integral_constant<int, 3>{});
+        else jacobi(std::integral_constant<int, 4>{});
         if (act) {
             vcur = vnew;
             L.u.i.V[t] = vcur;

@@ -377,6 +377,22 @@ BC_HD double exp(double x) { return exp_t(x, ExpTabLds{}); }
 // infinite and NaN results, subnormal or zero numerators and non-normal divisors (y = NaN) take
 // the IEEE division, so no underflow/overflow assumption is made.
 // tests/test_div_rcp.py checks bit identity with IEEE division on 10^8 random operands.
+// The IEEE division these helpers fall back to must stay on its (rare) branch: the compiler
+// would otherwise evaluate it speculatively on every call and select. An empty volatile asm on
+// the dividend cannot be hoisted out of the branch.
+BC_HD float keep_branch(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+v"(x));
+#endif
+    return x;
+}
+BC_HD double keep_branch(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+v"(x));
+#endif
+    return x;
+}
+
 BC_HD float rcp_prep(float b) {   // RN(1/b), or NaN when b is not a normal number
     float ab = __builtin_fabsf(b);
     return (ab >= 1.17549435e-38f && ab <= 3.40282347e+38f) ? 1.0f / b : __builtin_nanf("");
@@ -388,7 +404,7 @@ BC_HD float div_rcp(float a, float b, float y) {
     float r1 = __builtin_fmaf(-b, q1, a);
     float q2 = __builtin_fmaf(r1, y, q1);
     float aq = __builtin_fabsf(q2), aa = __builtin_fabsf(a);
-    if (!(aq >= 1.17549435e-38f && aq <= 3.40282347e+38f && aa >= 1.17549435e-38f)) return a / b;
+    if (!(aq >= 1.17549435e-38f && aq <= 3.40282347e+38f && aa >= 1.17549435e-38f)) return keep_branch(a) / b;
     return q2;
 }
 BC_HD double rcp_prep(double b) {   // RN(1/b), or NaN when b is not a normal number
@@ -403,20 +419,25 @@ BC_HD double div_rcp(double a, double b, double y) {
     double q2 = __builtin_fma(r1, y, q1);
     double aq = __builtin_fabs(q2), aa = __builtin_fabs(a);
     if (!(aq >= 2.2250738585072014e-308 && aq <= 1.7976931348623157e308 && aa >= 2.2250738585072014e-308))
-        return a / b;
+        return keep_branch(a) / b;
     return q2;
 }
 
 // a / b for floats, correctly rounded, given rb = RN(1 / (double)b). The product a * rb in
-// double is within 2^-52 relative of a / b, while a quotient of two 24-bit significands that is
-// a normal float lies at least 2^-49 (relative) away from every rounding midpoint and is never
-// on one, so rounding the product to float gives RN(a / b). Results outside the normal float
-// range, and every zero/infinite/NaN operand or divisor, take the IEEE division.
+// double is within 2^-52 relative of q = a / b. A quotient of two 24-bit significands is at
+// least 2^-49 (relative) away from every float rounding midpoint of the normal range (the
+// overflow threshold included) and never on one, and at least 2^-48 away from the midpoint
+// just below the smallest normal unless exactly on it; so whenever the product rounds to a
+// normal float, that float is RN(q). Every other rounded product (zero, subnormal, infinite,
+// NaN) takes the IEEE division.
 BC_HD float div_wide(float a, float b, double rb) {
-    double p = (double)a * rb;
-    double ap = __builtin_fabs(p);
-    if (!(ap >= 0x1p-126 && ap < 0x1p128)) return a / b;
-    return (float)p;
+    float r = (float)((double)a * rb);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (!__builtin_amdgcn_classf(r, 0x108)) r = keep_branch(a) / b;   // class mask: -normal | +normal
+#else
+    if (!__builtin_isnormal(r)) r = a / b;
+#endif
+    return r;
 }
 
 }  // namespace bc

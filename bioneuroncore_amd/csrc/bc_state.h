@@ -40,6 +40,7 @@ struct TileSet {
     const uint32_t* nseg;
     const uint32_t* conn0;
     const uint32_t* nconn;
+    const uint32_t* maxinc;   // largest incidence count of a segment in the tile
 };
 
 // Per-tick summary of one presynaptic segment: everything the bouton/release computation of
