@@ -574,7 +574,7 @@ static int build_device(Sim& S) {
         const uint32_t* tmi = S.dupload(tile_maxinc);
         for (int k = 0; k < kTileClasses; k++) {
             uint32_t o = S.class_tile0[k];
-            S.tiles[k] = TileSet{ts0 + o, tns + o, tc0 + o, tnc + o, tmi + o};
+            S.tiles[k] = TileSet{ts0 + o, tns + o, tc0 + o, tnc + o, tmi + o, S.class_ntiles[k]};
         }
     }
     d.conn_ab = S.dupload(conn_ab);
@@ -749,7 +749,11 @@ static void run_tick(Sim& S, float dt) {
     k_modifiers<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
-        if (S.class_ntiles[0]) k_diffusion<64><<<S.class_ntiles[0], 64, sizeof(DiffLds<64>), S.stream>>>(d, S.tiles[0], dt);
+        if (S.class_ntiles[0]) {
+            constexpr int W = diff_tiles_per_block(64);
+            k_diffusion<64><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, W * sizeof(DiffLds<64>), S.stream>>>(
+                d, S.tiles[0], dt);
+        }
         if (S.class_ntiles[1]) k_diffusion<128><<<S.class_ntiles[1], 128, sizeof(DiffLds<128>), S.stream>>>(d, S.tiles[1], dt);
         if (S.class_ntiles[2]) k_diffusion<256><<<S.class_ntiles[2], 256, sizeof(DiffLds<256>), S.stream>>>(d, S.tiles[2], dt);
     }

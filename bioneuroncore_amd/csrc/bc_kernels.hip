@@ -165,6 +165,24 @@ struct DiffLds {
 static_assert(sizeof(DiffLds<64>::ion) + sizeof(((DiffLds<64>*)nullptr)->u) >= 18 * 4 * 64,
               "transmitter staging needs 18 floats per segment in ion + u");
 
+// Tiles per workgroup: 64-lane tiles are single wavefronts that synchronise at wavefront level,
+// so several share a workgroup (fewer workgroups per CU for the same resident wavefronts).
+constexpr int kDiffTiles64 = 1;
+__host__ __device__ constexpr int diff_tiles_per_block(int ts) { return ts == 64 ? kDiffTiles64 : 1; }
+
+// Barrier of one tile. A 64-lane tile is one wavefront, whose LDS accesses complete in issue
+// order: a wavefront-scope fence pair around a wave barrier orders them for the compiler.
+template <int TS>
+__device__ __attribute__((always_inline)) inline void tile_sync() {
+    if (TS == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Incidence entry: tile-local connection << 2 | intracellular << 1 | role (0 = end a, 1 = end b).
 BC_DEV uint32_t ent_conn(uint32_t en) { return en >> 2; }
 BC_DEV bool ent_intra(uint32_t en) { return (en >> 1) & 1; }
@@ -191,15 +209,18 @@ __device__ static inline BPair bsel(int k, const BPair& b1, const BPair& b2) {
 }
 
 template <int TS>
-__global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
+__global__ __launch_bounds__(TS * diff_tiles_per_block(TS)) void k_diffusion(Net d, TileSet ts, float dt) {
+    // 64-lane tiles: each wavefront of the workgroup owns its own tile and LDS slice.
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    DiffLds<TS>& L = *reinterpret_cast<DiffLds<TS>*>(lds_raw);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t sub = TS == 64 ? threadIdx.x >> 6 : 0;
+    DiffLds<TS>& L = *reinterpret_cast<DiffLds<TS>*>(lds_raw + sub * sizeof(DiffLds<TS>));
+    const uint32_t tile = blockIdx.x * diff_tiles_per_block(TS) + sub;
+    if (tile >= ts.count) return;   // whole wavefront: no workgroup barrier follows for TS = 64
     const uint32_t s0 = ts.seg0[tile];
     const uint32_t nseg = ts.nseg[tile];
     const uint32_t c0 = ts.conn0[tile];
     const uint32_t nconn = ts.nconn[tile];
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x & (TS - 1);
     const bool act = t < nseg;
     const bool cact = t < nconn;
     const uint32_t g = s0 + t;
@@ -256,7 +277,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         L.u.p.phi[0][TS] = 0.0;
         L.u.p.phi[1][TS] = 0.0;
     }
-    __syncthreads();
+    tile_sync<TS>();
     for (uint32_t e = t; e < nent; e += TS) {
         uint32_t en = d.csr_ent[E0 + e];
         uint32_t c = en >> 1;
@@ -264,7 +285,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
     }
     double* gphi = L.u.p.gphi;
     if (cact) gphi[t] = 0.5 * (sig[ca_] + sig[cb_]) * cac / cdist;
-    __syncthreads();
+    tile_sync<TS>();
     // The first D incidences of a lane stay in registers across the sweeps, D = 2, 3 or 4 from
     // the tile's largest incidence count (chains 2, trees 3); the rest are read from LDS in order
     // after them. Unused register slots read the zero entry with weight 0: a sum that starts at
@@ -294,6 +315,39 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                 }
             }
         const double ydiag = rcp_prep(diag);   // diag is fixed across the sweeps
+        if constexpr (TS == 64) {
+            if (maxinc <= 4) {   // registers + ds_bpermute, as the Jacobi sweeps below
+                int pa[D];
+#pragma unroll
+                for (int q = 0; q < D; q++) pa[q] = (int)((q < pdeg ? po[q] : t) * 4);
+                double pi = 0.0;
+                for (int it = 0; it < 50; it++) {
+                    const int lo = (int)(uint32_t)__double_as_longlong(pi);
+                    const int hi = (int)(uint32_t)((uint64_t)__double_as_longlong(pi) >> 32);
+                    double pv[D];
+#pragma unroll
+                    for (int q = 0; q < D; q++) {
+                        uint32_t vlo = (uint32_t)__builtin_amdgcn_ds_bpermute(pa[q], lo);
+                        uint32_t vhi = (uint32_t)__builtin_amdgcn_ds_bpermute(pa[q], hi);
+                        pv[q] = __longlong_as_double((long long)(((uint64_t)vhi << 32) | vlo));
+                    }
+                    double pn = 0.0;
+#pragma unroll
+                    for (int q = 0; q < D; q++) {
+                        double s = pn + pg[q] * pv[q];
+                        pn = q < pdeg ? s : pn;
+                    }
+                    double ph = (diag > 1e-9) ? div_rcp(pn - ibuf, diag, ydiag) : 0.0;
+                    if (ph > 10.0) ph = 10.0;
+                    if (ph < -10.0) ph = -10.0;
+                    pi = act ? ph : pi;
+                }
+                if (act) L.u.p.phi[0][t] = pi;
+                cur = 0;
+                tile_sync<TS>();
+                return;
+            }
+        }
         for (int it = 0; it < 50; it++) {
             if (act) {
                 double pv[D];
@@ -313,7 +367,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                 if (ph < -10.0) ph = -10.0;
                 L.u.p.phi[cur ^ 1][t] = ph;
             }
-            __syncthreads();
+            tile_sync<TS>();
             cur ^= 1;
         }
     };
@@ -327,13 +381,13 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         pha = L.u.p.phi[cur][ca_];
         phb = L.u.p.phi[cur][cb_];
     }
-    __syncthreads();
+    tile_sync<TS>();
     if (act) L.u.i.V[t] = vcur;
     if (t == 0) {
         L.u.i.vf[0][TS] = 0.0f;
         L.u.i.vf[1][TS] = 0.0f;
     }
-    __syncthreads();
+    tile_sync<TS>();
 
     // ---- intracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:151-238) ----
     const int kSub = 5;
@@ -369,7 +423,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             }
             L.u.i.cterm[t] = inp - gax * dV;
         }
-        __syncthreads();
+        tile_sync<TS>();
         float gdiag = 0.0f, inpd = 0.0f;
         double dacc[7];
 #pragma unroll
@@ -394,7 +448,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             vold = (float)vcur;
             L.u.i.vf[0][t] = vold;
         }
-        __syncthreads();
+        tile_sync<TS>();
         float vnew = vold;
         const float den = cdt + gdiag;
         const float base = cdt * vold + inpd;
@@ -417,6 +471,33 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                     jdeg++;
                     if (jdeg == D) je_rest = e + 1;
                 }
+            if constexpr (TS == 64) {
+                if (maxinc <= 4) {
+                    // The tile is one wavefront and every incidence has a register slot: the
+                    // iterates stay in registers, neighbours' values come by ds_bpermute. Padded
+                    // slots read the lane itself and are not added.
+                    int ja[D];
+#pragma unroll
+                    for (int q = 0; q < D; q++) ja[q] = (int)((q < jdeg ? jo[q] : t) * 4);
+                    float vi = vold;
+                    for (int it = 0; it < 50; it++) {
+                        float jv[D];
+#pragma unroll
+                        for (int q = 0; q < D; q++)
+                            jv[q] = __int_as_float(__builtin_amdgcn_ds_bpermute(ja[q], __float_as_int(vi)));
+                        float off = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < D; q++) {
+                            float s = off + jg[q] * jv[q];
+                            off = q < jdeg ? s : off;
+                        }
+                        float nv = div_wide(base + off, den, rden);
+                        vi = act ? nv : vi;
+                    }
+                    vnew = vi;
+                    return;
+                }
+            }
             int vc = 0;
             for (int it = 0; it < 50; it++) {
                 if (act) {
@@ -435,7 +516,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                     vnew = div_wide(base + off, den, rden);
                     L.u.i.vf[vc ^ 1][t] = vnew;
                 }
-                __syncthreads();
+                tile_sync<TS>();
                 vc ^= 1;
             }
         };
@@ -448,7 +529,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
 #pragma unroll
             for (int k = 0; k < 7; k++) L.ion[k][t] += (float)dacc[k];
         }
-        __syncthreads();
+        tile_sync<TS>();
     }
     // intracellular ions are final: store them, then reuse the plane for extracellular ions
     if (act) {
@@ -459,7 +540,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             L.ion[k][t] = d.ex[k * P + g];
         }
     }
-    __syncthreads();
+    tile_sync<TS>();
 
     // ---- extracellular Nernst-Planck, 5 substeps (diffusion_kernel.h:245-281) ----
     // phi is fixed across the substeps, so each connection's Bernoulli factors are too.
@@ -477,7 +558,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
                 L.u.flux[k][t] = kin[k] * (ca * bb.m - cb * bb.p);
             }
         }
-        __syncthreads();
+        tile_sync<TS>();
         if (act) {
             double dx[7];
 #pragma unroll
@@ -498,7 +579,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
 #pragma unroll
             for (int k = 0; k < 7; k++) L.ion[k][t] += dx[k];
         }
-        __syncthreads();
+        tile_sync<TS>();
     }
     if (act) {
 #pragma unroll
@@ -510,7 +591,7 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
             d.ex[k * P + g] = x;
         }
     }
-    __syncthreads();
+    tile_sync<TS>();
 
     // ---- transmitters, ATP, temperature, transcription (diffusion_kernel.h:284-345) ----
     // Staged in LDS (ion + cbuf planes are free now): 15 + 3 floats per segment, then 15.
@@ -563,12 +644,12 @@ __global__ __launch_bounds__(TS) void k_diffusion(Net d, TileSet ts, float dt) {
         st[16 * TS + t] = own_scal[1] = d.temp[g];
         st[17 * TS + t] = own_scal[2] = d.txn[g];
     }
-    __syncthreads();
+    tile_sync<TS>();
     fick(15, 0, true, dnt, datp, dtemp, dtxn, d.nt_diff, false);
-    __syncthreads();
+    tile_sync<TS>();
     if (act)
         for (int k = 0; k < 15; k++) st[k * TS + t] = d.nt_in[k * P + g];
-    __syncthreads();
+    tile_sync<TS>();
     fick(15, 0, false, dnt_in, z0, z1, z2, d.nt_diff, true);
     if (!act) return;
     for (int k = 0; k < 15; k++) {

@@ -41,6 +41,7 @@ struct TileSet {
     const uint32_t* conn0;
     const uint32_t* nconn;
     const uint32_t* maxinc;   // largest incidence count of a segment in the tile
+    uint32_t count;           // tiles in the set
 };
 
 // Per-tick summary of one presynaptic segment: everything the bouton/release computation of
