@@ -371,6 +371,8 @@ static int build_device(Sim& S) {
     for (uint32_t i = 0; i < n; i++) nslot = std::max<uint32_t>(nslot, sg[i].modifier_count);
     Net& d = S.d;
     d.n = n; d.npad = npad; d.nslot = nslot; d.ns = ns; d.ntiles = S.ntiles; d.has_conn = nc > 0;
+    d.nt_ex_live = ns == 0;
+    d.nt_in_live = 1;   // until the first upload of the state decides
     std::vector<uint32_t> seg_id(npad, 0);
     std::vector<float> radius(npad, 0), length(npad, 0), area(npad, 0), cap(npad, 1), pkc(npad, 0), pka(npad, 0),
         camp(npad, 0.1f), pp1(npad, 0.5f), camkii(npad, 0), calcn(npad, 0);
@@ -674,6 +676,14 @@ static void xfer(Sim& S, bool to_device) {
     cp(d.txn, txn.data(), npad * 4);
     cp(d.nt, nt.data(), nt.size() * 4);
     cp(d.nt_in, nti.data(), nti.size() * 4);
+    if (to_device) {   // intra_nt is live once any value is not +0.0
+        d.nt_in_live = 0;
+        for (size_t q = 0; q < nti.size() && !d.nt_in_live; q++) {
+            uint32_t bits;
+            memcpy(&bits, &nti[q], 4);
+            d.nt_in_live = bits != 0;
+        }
+    }
     if (ns) {
         cp(d.y_weight, yw.data(), ns * 4);
         cp(d.y_fac, yf.data(), ns * 4);
@@ -892,6 +902,8 @@ void gpu_backend_upload_stimuli(void* sim, const GpuStimulusData* src) {
     auto cap = [](uint32_t& c) { if (c > BC_GPU_MAX_STIMULI) c = BC_GPU_MAX_STIMULI; };
     cap(S.stim_host->current_stim_count); cap(S.stim_host->vclamp_count); cap(S.stim_host->ramp_count);
     cap(S.stim_host->ion_inj_count); cap(S.stim_host->lig_inj_count);
+    for (uint32_t k = 0; k < S.stim_host->lig_inj_count; k++)   // sources intra_nt from step 1 on
+        if (S.stim_host->lig_inj[k].intracellular) S.d.nt_in_live = 1;
     S.stim_present = true;
     S.stim_dirty = true;
 }

@@ -638,24 +638,29 @@ __global__ __launch_bounds__(TS * diff_tiles_per_block(TS)) void k_diffusion(Net
     for (int k = 0; k < 15; k++) { dnt[k] = 0.0f; dnt_in[k] = 0.0f; }
     float datp = 0.0f, dtemp = 0.0f, dtxn = 0.0f, z0 = 0.0f, z1 = 0.0f, z2 = 0.0f;
     float own_nt[15], own_scal[3];
+    const bool nt_ex = d.nt_ex_live != 0, nt_in = d.nt_in_live != 0;
     if (act) {
-        for (int k = 0; k < 15; k++) st[k * TS + t] = own_nt[k] = d.nt[k * P + g];
+        if (nt_ex)
+            for (int k = 0; k < 15; k++) st[k * TS + t] = own_nt[k] = d.nt[k * P + g];
         st[15 * TS + t] = own_scal[0] = d.atp[g];
         st[16 * TS + t] = own_scal[1] = d.temp[g];
         st[17 * TS + t] = own_scal[2] = d.txn[g];
     }
     tile_sync<TS>();
-    fick(15, 0, true, dnt, datp, dtemp, dtxn, d.nt_diff, false);
-    tile_sync<TS>();
-    if (act)
-        for (int k = 0; k < 15; k++) st[k * TS + t] = d.nt_in[k * P + g];
-    tile_sync<TS>();
-    fick(15, 0, false, dnt_in, z0, z1, z2, d.nt_diff, true);
-    if (!act) return;
-    for (int k = 0; k < 15; k++) {
-        d.nt_in[k * P + g] = st[k * TS + t] + dnt_in[k];
-        d.nt[k * P + g] = own_nt[k] + dnt[k];
+    if (nt_ex) fick(15, 0, true, dnt, datp, dtemp, dtxn, d.nt_diff, false);
+    else fick(0, 0, true, dnt, datp, dtemp, dtxn, d.nt_diff, false);
+    if (nt_in) {
+        tile_sync<TS>();
+        if (act)
+            for (int k = 0; k < 15; k++) st[k * TS + t] = d.nt_in[k * P + g];
+        tile_sync<TS>();
+        fick(15, 0, false, dnt_in, z0, z1, z2, d.nt_diff, true);
     }
+    if (!act) return;
+    if (nt_in)
+        for (int k = 0; k < 15; k++) d.nt_in[k * P + g] = st[k * TS + t] + dnt_in[k];
+    if (nt_ex)
+        for (int k = 0; k < 15; k++) d.nt[k * P + g] = own_nt[k] + dnt[k];
     d.atp[g] = own_scal[0] + datp;
     d.temp[g] = own_scal[1] + dtemp;
     d.txn[g] = own_scal[2] + dtxn;

@@ -69,6 +69,11 @@ struct Net {
     uint32_t nrec;       // receptor slots per synapse (max post_receptor_count)
     uint32_t ntiles;
     uint32_t has_conn;   // any IntraConnection at all (step 2.5 runs)
+    // Transmitter fields step 2.5 must diffuse. With synapses, step 3 rebuilds every segment's
+    // extracellular transmitters from zero before anything reads them, so their diffusion is
+    // dead; intra_nt stays +0.0 (and its diffusion an exact no-op) until something sources it.
+    uint32_t nt_ex_live;
+    uint32_t nt_in_live;
 
     // segment dynamic state
     double* V;

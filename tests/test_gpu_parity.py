@@ -98,6 +98,23 @@ def test_post_walk_paths(generic):
     _compare(net, 120, lambda t: H.Stimuli(current=cur[t]), lambda t: gst[t])
 
 
+@pytest.mark.parametrize("with_syn", [False, True])
+def test_transmitter_liveness(with_syn):
+    """Transmitter diffusion where it is live: without synapses the extracellular levels diffuse
+    (a ligand injection seeds them); intra_nt becomes live with an intracellular injection
+    after some ticks (until then it is +0.0 everywhere and its diffusion is skipped)."""
+    net = bion.config1(seed=12) if with_syn else bion.config3(n_cables=4, seg_per_cable=24, seed=13)
+    ex = np.zeros(1, bion.LIG_INJ_DT)
+    ex["segment_id"], ex["nt_idx"], ex["amount"] = 3, 2, 0.7
+    inj = np.zeros(1, bion.LIG_INJ_DT)
+    inj["segment_id"], inj["nt_idx"], inj["amount"], inj["intracellular"] = 5, 4, 0.9, 1
+
+    def kinds(t):
+        return dict(lig=ex if t == 4 else (inj if t == 15 else None))
+    gst = [backend.make_stimuli(**kinds(t)) for t in range(50)]
+    _compare(net, 50, lambda t: H.Stimuli(**kinds(t)), lambda t: gst[t])
+
+
 def test_all_tile_classes_and_stimulus_kinds():
     """64-, 128- and 256-lane tiles in one network (8-, 100- and 200-segment islands), with
     current pulses, ramps, a voltage clamp and one-shot ion/ligand injections."""
