@@ -802,10 +802,18 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt)
 template <bool kReceptorsOnly>
 __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, const GpuStimulusData* __restrict__ st,
                                                               uint64_t tick, float dt) {
+    // kStage: the block's 256-segment window is written back through LDS once every lane is
+    // done, one coalesced row per field; lanes walk the window in chain-length order and finish
+    // at different times, so direct writes would leave lines partly written for long stretches.
+    constexpr bool kStage = kReceptorsOnly;
+    __shared__ double o_in[kStage ? 4 : 1][256];
+    __shared__ float o_nt[kStage ? 15 : 1][256];
+    __shared__ double o_V[256];
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= d.n) return;
-    const uint32_t i = d.walk[lane];
     const size_t P = d.npad, NS = d.ns;
+    if (lane < d.n) {
+    const uint32_t i = d.walk[lane];
+    const uint32_t pos = i - blockIdx.x * blockDim.x;   // the walk permutes inside the window
     const uint8_t fl = d.sflags[i];
     double ibuf = d.ibuf[i];
     double V = d.V[i];
@@ -896,15 +904,20 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
                 if (ran && v < 1e-9) d.in[k * P + i] = 1e-9;
             }
         }
+        if (kStage) {
 #pragma unroll
-        for (int k = 0; k < kIonWalk; k++) d.in[k * P + i] = s.in[k];
-        if (!kReceptorsOnly) {
+            for (int k = 0; k < 4; k++) o_in[k][pos] = s.in[k];
+#pragma unroll
+            for (int k = 0; k < 15; k++) o_nt[kStage ? k : 0][pos] = s.nt[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kIonWalk; k++) d.in[k * P + i] = s.in[k];
 #pragma unroll
             for (int k = 0; k < 7; k++) d.ex[k * P + i] = s.ex[k];
             d.atp[i] = s.atp;
-        }
 #pragma unroll
-        for (int k = 0; k < 15; k++) d.nt[k * P + i] = s.nt[k];
+            for (int k = 0; k < 15; k++) d.nt[k * P + i] = s.nt[k];
+        }
     } else {
         if (fl & SF_BATH_NMDA) d.nt[0 * P + i] = 1.0f;
         if (fl & SF_BATH_5HT) d.nt[3 * P + i] = 0.0f;
@@ -932,7 +945,22 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
         if (V < -120.0f) V = -120.0f;
         if (V > 150.0f) V = 150.0f;
     }
-    d.V[i] = V;
+    if (kStage) o_V[pos] = V;
+    else d.V[i] = V;
+    }   // lane < n
+    if (kStage) {
+        __syncthreads();
+        const uint32_t t = threadIdx.x, g = blockIdx.x * blockDim.x + t;
+        if (g < d.n) {
+            if (d.ns > 0) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) d.in[k * P + g] = o_in[k][t];
+#pragma unroll
+                for (int k = 0; k < 15; k++) d.nt[k * P + g] = o_nt[kStage ? k : 0][t];
+            }
+            d.V[g] = o_V[t];
+        }
+    }
 }
 
 // AMPA/NMDA/GABA-A receptors only: four wavefronts per SIMD fit in the register budget.
