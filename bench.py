@@ -29,39 +29,77 @@ HBM_PEAK_GBS = 8000.0
 PHASES = ("k_modifiers", "k_diffusion", "k_syn_pre", "k_syn_post", "k_syn_plast", "k_syn_rrp")
 
 
-def algorithmic_bytes(net, stats):
+def algorithmic_bytes(net, nt_ex_live, nt_in_live):
     """Compulsory HBM bytes per launch of each kernel: every dynamic field it owns read once and
-    written once, every static parameter it needs read once (P64 = the ABI's precision)."""
+    written once, every static parameter it needs read once, at the ABI's precision (f64 V_m,
+    ions and modifier states; f32 elsewhere). Transmitter fields count only where the kernel
+    must move them (the electrodiffusion kernel skips them when they are dead)."""
     n = len(net.segments)
     nc = len(net.connections)
     ns = len(net.synapses)
     types = net.segments["modifiers"]["type"]
     count = net.segments["modifier_count"]
-    # dynamic gate states per modifier type (written) and parameter states read
+    # modifier states per type: gates read and written, parameters only read
     gate_w = {142: 2, 141: 2, 156: 2, 150: 3, 112: 1, 140: 2, 129: 1, 103: 1, 104: 1, 107: 0, 7: 0, 133: 0, 123: 1}
     param_r = {142: 1, 141: 1, 156: 2, 150: 1, 112: 5, 140: 0, 129: 5, 103: 3, 104: 2, 107: 2, 7: 0, 133: 1, 123: 3}
     gates = params = 0
     for t in np.unique(types[types > 0]):
-        k = int((types == t).sum())
-        gates += k * gate_w.get(int(t), 0)
-        params += k * param_r.get(int(t), 0)
+        c = int((types == t).sum())
+        gates += c * gate_w.get(int(t), 0)
+        params += c * param_r.get(int(t), 0)
     nmod = int(count.sum())
-    seg_dyn = 8 + 7 * 8 + 7 * 8 + 4            # V, intra, extra, ATP
-    mod = {
-        "k_modifiers": n * (2 * seg_dyn + 8 + 4 + 8) + nmod * (1 + 8) + gates * 16 + params * 8,
-        "k_diffusion": n * (2 * (8 + 7 * 8 + 7 * 8) + 8 + 4 + 4 + 4 + 8 + 2 * 4 * 3) + nc * 12,
-        "k_syn_pre": ns * (4 + 4 + 4 + 4 + 4 + 2 * 4 + 1 + 2 * 1 + 4) + 64 * 0,
-        # ordered pass: segment V, ions in/out, ibuf, cap, transmitter out; per synapse flags, mask,
-        # one cleft transmitter, receptor count, post area, receptor current out
-        "k_syn_post": n * (2 * (8 + 7 * 8) + 7 * 8 + 8 + 4 + 15 * 4) + ns * (1 + 2 + 4 + 1 + 4 + 4),
+    rec = int(net.synapses["post_receptor_count"].astype(np.int64).sum())
+    ion = 8 + 7 * 8 + 7 * 8          # V_m + 7 intra + 7 extra (f64)
+    nt_planes = 15 * 4 * 2           # 15 f32 transmitters, read + written
+    return {
+        # R: V, ions, ATP, area, vol, PKC/PKA/cAMP, flags, count; W: V, ions, ATP, I_buffer;
+        # per modifier type, active-vars and density; gate states R+W, parameter states R
+        "k_modifiers": n * (ion + 4 + 4 + 8 + 12 + 2) + n * (ion + 4 + 8)
+        + nmod * (1 + 1 + 8) + gates * 16 + params * 8,
+        # R: V, ions, I_buffer, area, cap, vol, radius, length, CSR offset, ATP/temp/txn;
+        # W: V, ions, ATP/temp/txn; per connection its end pair, intracellular flag and two
+        # incidence entries
+        "k_diffusion": n * (ion + 8 + 4 + 4 + 8 + 4 + 4 + 4 + 12) + n * (ion + 12)
+        + (n * nt_planes if nt_ex_live else 0) + (n * nt_planes if nt_in_live else 0) + nc * (4 + 1 + 8),
+        # per synapse: flags, refractory R+W, delay mask, pre index, summary index, bouton Ca
+        # R+W, P_rel, pre area, id, RRP, RRP_max, delay, amount, receptor count/type, release
+        # probability out; per presynaptic segment its 64-byte summary
+        "k_syn_pre": ns * (1 + 2 + 1 + 4 + 4 + 8 + 4 + 4 + 4 + 4 + 4 + 4 + 4 + 2 + 4),
+        # per segment R: V, ions 0-6 intra, 0-3 extra, I_buffer, area, vol, flags, cap, lane map,
+        # chain length; W: ions 0-3, 15 transmitters, V. Per synapse: flags, mask, one live
+        # cleft transmitter, receptor count, post area, current out, second-walk flags + mask;
+        # per receptor: type, active-vars, density, two states R+W
+        "k_syn_post": n * (8 + 7 * 8 + 4 * 8 + 8 + 4 + 8 + 1 + 4 + 4 + 4) + n * (4 * 8 + 15 * 4 + 8)
+        + ns * (1 + 2 + 4 + 1 + 4 + 4 + 3) + rec * (1 + 1 + 8 + 2 * 2 * 8),
         # flags, current, mask, tau, pp1 read; spine Ca, one cleft transmitter, BCM, CaMKII,
-        # calcineurin, weight read+written; depression written
+        # calcineurin, weight read + written; depression written
         "k_syn_plast": ns * (1 + 4 + 2 + 4 + 4 + 6 * 2 * 4 + 4),
-        "k_syn_rrp": ns * (4 + 4 + 4 + 2 * 4 + 4),
+        # flags, release probability, RRP R+W, RRP_max, source index, recovery rate
+        "k_syn_rrp": ns * (1 + 4 + 2 * 4 + 4 + 4 + 4),
     }
-    rec = net.synapses["post_receptor_count"].astype(np.int64).sum()
-    mod["k_syn_post"] += int(rec) * (1 + 8 + 2 * 2 * 8)
-    return mod
+
+
+def tick_bytes_model(net):
+    """Whole-tick compulsory bytes at the ABI's precision: 621 B per segment (504 dynamic R+W,
+    117 static), 12 B per connection, 193 B per synapse (the configuration-2 unit figures)."""
+    return len(net.segments) * 621 + len(net.connections) * 12 + len(net.synapses) * 193
+
+
+def measured_traffic(kernel, neurons):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    of this workload (already scaled by the copy-kernel calibration), or None when the summary
+    does not cover this kernel or size."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_tick_config2_3p2M.md")
+    if neurons != 100_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == kernel:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
 
 
 def build_workload(neurons, ticks, seed=0xB10C0002):
@@ -142,7 +180,9 @@ def main():
     if rc != 0:
         raise SystemExit(f"load failed: {rc}")
     print(f"# loaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    algo = algorithmic_bytes(net, sim.stats())
+    nt_in_live = bool(np.any(net.segments["intra_nt"].view(np.uint32)))
+    algo = algorithmic_bytes(net, len(net.synapses) == 0, nt_in_live)
+    tick_bytes = tick_bytes_model(net)
     n_seg = len(net.segments)
     del net
     for t in range(args.warmup):
@@ -163,6 +203,8 @@ def main():
     dom = int(np.argmax(mean))
     name = PHASES[dom]
     achieved = algo[name] / (mean[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(name, args.neurons)
+    tick_gbs = tick_bytes / (ms_step * 1e-3) / 1e9
     out = {
         "metric": "segment-ticks/s @3.2M seg (HBM roofline %); batched vs plain; adjoint/fwd cost",
         "value": value, "unit": "segment-ticks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -172,7 +214,10 @@ def main():
                                "10M AMPA+NMDA/GABA-A synapses, Poisson 5 Hz somatic pulses, dt 0.025 ms",
                    "neurons": args.neurons, "segments": n_seg},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": algo[name], "launch_ms": float(mean[dom])},
+        "tick_roofline": {"model_bytes_per_tick": tick_bytes, "achieved": tick_gbs, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": tick_gbs / HBM_PEAK_GBS},
         "phase_ms": {PHASES[k]: float(mean[k]) for k in range(len(PHASES))},
     }
     if not args.no_cpu:

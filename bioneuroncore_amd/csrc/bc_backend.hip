@@ -455,21 +455,31 @@ static int build_device(Sim& S) {
     }
     std::vector<uint32_t> syn2slot(ns);
     for (uint32_t k = 0; k < ns; k++) syn2slot[S.slot2syn[k]] = k;
+    // Ca history: one entry per synapse onto a presynaptic segment, each such segment's entries
+    // in its chain order; src = the entry of the last earlier synapse onto my pre segment.
+    std::vector<uint8_t> is_pre(n, 0);
+    for (uint32_t i = 0; i < ns; i++) is_pre[S.host2dev[sy[i].pre_segment]] = 1;
+    std::vector<uint32_t> hist_base(n, 0), chain_rank(ns, 0);
+    uint32_t nhist = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        for (uint32_t q = post_off[i]; q < post_off[i + 1]; q++) chain_rank[by_post[q]] = q - post_off[i];
+        if (is_pre[i]) { hist_base[i] = nhist; nhist += post_off[i + 1] - post_off[i]; }
+    }
     std::vector<int32_t> src(ns, -1);
     {
         std::vector<int64_t> last_onto(n, -1);
         for (uint32_t i = 0; i < ns; i++) {
             int64_t j = last_onto[sy[i].pre_segment];
-            src[syn2slot[i]] = j >= 0 ? (int32_t)syn2slot[j] : -1;
+            src[syn2slot[i]] = j >= 0 ? (int32_t)(hist_base[S.host2dev[sy[j].post_segment]] + chain_rank[j]) : -1;
             last_onto[sy[i].post_segment] = i;
         }
     }
     std::vector<uint32_t> y_pre(ns), y_id(ns), y_prew(ns), watched;
     std::vector<int64_t> watch_idx(n, -1);
     std::vector<uint8_t> y_flags(ns, 0), y_rcount(ns, 0), y_refr(ns, 0);
-    std::vector<float> y_delay(ns), y_pa(ns), y_qa(ns), y_prel(ns), y_amt(ns), y_tau(ns), y_rmax(ns), y_rrec(ns);
+    std::vector<float> y_delay(ns), y_pa(ns), y_qa(ns), y_prel(ns), y_amt(ns), y_tau(ns), y_rmax(ns);
     std::vector<float> y_w(ns), y_fac(ns), y_dep(ns), y_rrp(ns), y_bca(ns), y_sca(ns), y_ck(ns), y_cn(ns);
-    std::vector<float> y_pp1(ns);
+    std::vector<float> y_pp1(ns), y_prim0(ns);
     std::vector<uint16_t> y_mask(ns, 0);
     std::vector<uint8_t> y_rtype((size_t)nrec * ns, 0), y_rav((size_t)nrec * ns, 0);
     std::vector<double> y_rdens((size_t)nrec * ns, 0.0), y_rstate((size_t)nrec * 7 * ns, 0.0);
@@ -500,11 +510,16 @@ static int build_device(Sim& S) {
         y_flags[k] = fl;
         y_rcount[k] = y.post_receptor_count;
         y_delay[k] = y.delay; y_pa[k] = y.pre_area; y_qa[k] = y.post_area; y_prel[k] = y.release_probability;
-        y_amt[k] = y.release_amount; y_tau[k] = y.tau_clearance; y_rmax[k] = y.RRP_max; y_rrec[k] = y.RRP_recovery;
+        y_amt[k] = y.release_amount; y_tau[k] = y.tau_clearance; y_rmax[k] = y.RRP_max;
         y_w[k] = y.weight; y_fac[k] = y.facilitation; y_dep[k] = y.depression; y_refr[k] = y.refractory;
         y_rrp[k] = y.RRP; y_bca[k] = y.bouton_Ca; y_sca[k] = y.spine_Ca; y_ck[k] = y.spine_camkii;
         y_cn[k] = y.spine_calcineurin;
         y_pp1[k] = pp1[S.host2dev[y.post_segment]];
+        {
+            const uint32_t p = S.host2dev[y.pre_segment];
+            float a = y.RRP_recovery + camkii[p] * 0.02f;
+            y_prim0[k] = a + calcn[p] * 0.01f;
+        }
         uint16_t mask = 0;
         for (int q = 0; q < 15; q++) {
             y_cleft[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
@@ -546,7 +561,6 @@ static int build_device(Sim& S) {
     d.nt = S.dalloc<float>((size_t)15 * npad);
     d.nt_in = S.dalloc<float>((size_t)15 * npad);
     d.ibuf = S.dalloc<double>(npad);
-    d.ca_snap = S.dalloc<double>(npad);
     d.spikes = S.dalloc<uint32_t>(npad);
     d.seg_id = S.dupload(seg_id);
     d.radius = S.dupload(radius);
@@ -559,8 +573,6 @@ static int build_device(Sim& S) {
     d.pka = S.dupload(pka);
     d.camp = S.dupload(camp);
     d.pp1 = S.dupload(pp1);
-    d.camkii = S.dupload(camkii);
-    d.calcn = S.dupload(calcn);
     d.sflags = S.dupload(sflags);
     d.cav21_slot = S.dupload(cav21);
     d.mcount = S.dupload(mcount);
@@ -597,7 +609,7 @@ static int build_device(Sim& S) {
     d.y_amount = S.dupload(y_amt);
     d.y_tau = S.dupload(y_tau);
     d.y_rrp_max = S.dupload(y_rmax);
-    d.y_rrp_rec = S.dupload(y_rrec);
+    d.y_prim0 = S.dupload(y_prim0);
     d.y_cleft_mask = S.dupload(y_mask);
     d.y_src = S.dupload(src);
     d.post_len = S.dupload(post_len);
@@ -621,7 +633,8 @@ static int build_device(Sim& S) {
     d.y_relp = S.dalloc<float>(ns);
     d.y_isyn = S.dalloc<float>(ns);
     d.y_pp1 = S.dupload(y_pp1);
-    d.y_ca_after = S.dalloc<double>(ns);
+    d.hist_base = S.dupload(hist_base);
+    d.ca_hist = S.dalloc<double>(std::max<uint32_t>(nhist, 1));
     d.d_mask = S.dalloc<uint8_t>(ns);
     d.d_arr = S.dalloc<uint64_t>((size_t)4 * ns);
     d.d_amt = S.dalloc<float>((size_t)4 * ns);
@@ -769,7 +782,6 @@ static void run_tick(Sim& S, float dt) {
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
-        HIPCHK(hipMemcpyAsync(d.ca_snap, d.in + 2 * (size_t)d.npad, (size_t)d.n * 8, hipMemcpyDeviceToDevice, S.stream));
         k_pre_summary<<<(d.nwatched + 255) / 256, 256, 0, S.stream>>>(d);
         k_syn_pre<<<std::min<uint32_t>(gy, 8192), 256, 0, S.stream>>>(d, tick, dt);
     }

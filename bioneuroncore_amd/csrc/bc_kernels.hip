@@ -14,7 +14,7 @@
 // and of each synapse's post-side only the receptors feed that order (the rest reads the
 // synapse's own state and receptor current);
 // the one cross-chain read is rrp_do_recovery's pre intra Ca, which k_syn_rrp takes from the
-// recorded post-chain history (y_ca_after) of the last earlier synapse onto that segment.
+// recorded post-chain history (ca_hist) of the last earlier synapse onto that segment.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "bc_device.h"
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(256) void k_pre_summary(Net d) {
         ps.mi = cav21_minf(vb);
         ps.h0 = 1.0f / (1.0f + bc::exp((vb - -50.0f) / 8.0f));
     }
-    ps._pad[0] = ps._pad[1] = 0;
+    ps.ca_snap = d.in[2 * P + pre];
     d.presum[w] = ps;
 }
 
@@ -768,7 +768,7 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt)
                     if (!(rmax <= 0.0f || rrp >= rmax)) {
                         float ca = d.in[2 * P + pre];
                         float caf = ca / (0.0005f + ca);
-                        float prim = d.y_rrp_rec[y] + d.camkii[pre] * 0.02f + d.calcn[pre] * 0.01f;
+                        float prim = d.y_prim0[y];
                         prim *= caf;
                         float rec = (rmax - rrp) * prim * dt;
                         rrp += rec;
@@ -847,6 +847,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
         if (fl & SF_BATH_5HT) s.nt[3] = 0.0f;
         const bool my = fl & SF_MYELIN;
         const uint32_t len = d.post_len[lane];
+        const uint32_t hb = (fl & SF_WATCHED) ? d.hist_base[i] : 0;
         const uint32_t wl = threadIdx.x & 63;
         uint32_t off = d.wave_base[lane >> 6];
         bool ran = false;
@@ -857,7 +858,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
             off += cnt;
             const uint8_t yf = d.y_flags[y];
             if (yf & YF_SKIP) {
-                if (fl & SF_WATCHED) d.y_ca_after[y] = s.in[2];
+                if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
                 continue;
             }
             const uint16_t cm = d.y_cleft_mask[y];
@@ -881,7 +882,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
             }
             ibuf += isyn;
             d.y_isyn[y] = isyn;
-            if (fl & SF_WATCHED) d.y_ca_after[y] = s.in[2];
+            if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
         }
         if (kReceptorsOnly) {
             // the same chain again, for the transmitters the receptors do not read
@@ -1063,9 +1064,9 @@ __global__ __launch_bounds__(256) void k_syn_rrp(Net d, uint64_t tick, float dt)
     const uint32_t pre = d.y_pre[y];
     if (!(rmax <= 0.0f || rrp >= rmax)) {   // rrp_do_recovery (kernel_dispatch.h:612-631)
         int32_t src = d.y_src[y];
-        float ca = src >= 0 ? (float)d.y_ca_after[src] : (float)d.ca_snap[pre];
+        float ca = src >= 0 ? (float)d.ca_hist[src] : (float)d.presum[d.y_prew[y]].ca_snap;
         float caf = ca / (0.0005f + ca);
-        float prim = d.y_rrp_rec[y] + d.camkii[pre] * 0.02f + d.calcn[pre] * 0.01f;
+        float prim = d.y_prim0[y];   // RRP_recovery + CaMKII(pre) * 0.02 + calcineurin(pre) * 0.01
         prim *= caf;
         float rec = (rmax - rrp) * prim * dt;
         rrp += rec;

@@ -56,7 +56,7 @@ struct PreSum {
     float mi;      // Cav2.1 m_inf(V)
     float h0;      // 1 / (1 + exp((V + 50) / 8)), the voltage factor of h_inf
     uint32_t has_cav;
-    uint32_t _pad[2];
+    double ca_snap;  // intra Ca at the start of step 3 (RRP recovery without an earlier synapse)
 };
 static_assert(sizeof(PreSum) == 64, "PreSum layout");
 
@@ -85,7 +85,6 @@ struct Net {
     float* nt;           // neurotransmitter [15][npad]
     float* nt_in;        // intra_nt [15][npad]
     double* ibuf;
-    double* ca_snap;     // intra Ca at the start of step 3
     uint32_t* spikes;
 
     // segment static data
@@ -100,8 +99,6 @@ struct Net {
     const float* pka;
     const float* camp;
     const float* pp1;
-    const float* camkii;
-    const float* calcn;
     const uint8_t* sflags;
     const int8_t* cav21_slot;   // last Cav2.1 modifier slot, -1 if none
 
@@ -133,9 +130,13 @@ struct Net {
     const float* y_amount;
     const float* y_tau;
     const float* y_rrp_max;
-    const float* y_rrp_rec;
+    const float* y_prim0;        // RRP_recovery + CaMKII(pre) * 0.02 + calcineurin(pre) * 0.01
     const uint16_t* y_cleft_mask;
-    const int32_t* y_src;        // slot of the last earlier synapse whose post is my pre, -1
+    const int32_t* y_src;        // Ca-history entry of the last earlier synapse whose post is my
+                                 // pre (see ca_hist), -1 if none
+    const uint32_t* hist_base;   // per segment: its first Ca-history entry (presynaptic segments)
+    double* ca_hist;             // post intra Ca after each synapse onto a presynaptic segment, in
+                                 // chain order per segment
     const uint32_t* walk;        // lane -> segment for the post-side pass: each 256-segment
                                  // window ordered by incoming-synapse count, longest first
     const uint32_t* post_len;    // per lane: incoming synapses of walk[lane]
@@ -158,7 +159,6 @@ struct Net {
     float* y_isyn;               // receptor current of the synapse this tick (post pass -> plasticity)
     const float* y_pp1;          // PP1 activity of the post segment
     float* y_relp;               // release probability of a deferred release this tick, 0 if none
-    double* y_ca_after;          // post intra Ca after this synapse's receptors
     uint8_t* d_mask;             // occupied delay slots (bit s)
     uint64_t* d_arr;             // [4][ns]
     float* d_amt;                // [4][ns]
