@@ -200,9 +200,24 @@ struct Sim {
     size_t lds_bytes = 0;
     uint64_t tick = 0;
     uint64_t ticks_run = 0;
-    GpuStimulusData* stim_host = nullptr;
-    GpuStimulusData* stim_dev = nullptr;
+    GpuStimulusData* stim_host = nullptr;   // last gpu_backend_upload_stimuli payload
     bool stim_present = false, stim_dirty = false;
+    uint64_t* tick_dev = nullptr;
+    // stimulus lists (see StimDev): payloads, list words and the segments whose head is set
+    std::vector<CurrentStimulus> st_cur;
+    std::vector<RampStimulus> st_ramp;
+    std::vector<VoltageClamp> st_vc;
+    std::vector<IonInjection> st_ion;
+    std::vector<LigandInjection> st_lig;
+    std::vector<uint32_t> st_ent, st_segs[2], st_heads;
+    int st_cur_set = 0;
+    struct DBuf {
+        void* p = nullptr;
+        size_t cap = 0;
+    };
+    DBuf sb_cur, sb_ramp, sb_vc, sb_ion, sb_lig, sb_ent, sb_segs[2], sb_heads;
+    uint32_t* stim_head = nullptr;
+    std::vector<std::pair<uint32_t, uint32_t>> id2dev;   // (Segment.id, device index), sorted
     int coherence = BCHIP_COHERENCE_COMPAT;
     bool host_stale = false;
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
@@ -233,11 +248,36 @@ struct Sim {
         dmem.clear();
         memset(&d, 0, sizeof(d));
     }
+    // Device copy of v in b (grown as needed); the copy reads v asynchronously, so v must stay
+    // unchanged until the stream has passed it.
+    template <class T>
+    T* dput(DBuf& b, const std::vector<T>& v) {
+        size_t bytes = std::max<size_t>(v.size() * sizeof(T), 64);
+        if (b.cap < bytes) {
+            if (b.p) HIPCHK(hipStreamSynchronize(stream));
+            if (b.p) (void)hipFree(b.p);
+            b.p = nullptr;
+            b.cap = 0;
+            HIPCHK(hipMalloc(&b.p, bytes + bytes / 2));
+            b.cap = bytes + bytes / 2;
+        }
+        if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        return static_cast<T*>(b.p);
+    }
+    void free_stim_buffers() {
+        for (DBuf* b : {&sb_cur, &sb_ramp, &sb_vc, &sb_ion, &sb_lig, &sb_ent, &sb_segs[0], &sb_segs[1], &sb_heads}) {
+            if (b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+        st_segs[0].clear();
+        st_segs[1].clear();
+    }
     ~Sim() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         free_device();
-        if (stim_dev) (void)hipFree(stim_dev);
+        free_stim_buffers();
         delete stim_host;
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -373,7 +413,6 @@ static int build_device(Sim& S) {
     d.n = n; d.npad = npad; d.nslot = nslot; d.ns = ns; d.ntiles = S.ntiles; d.has_conn = nc > 0;
     d.nt_ex_live = ns == 0;
     d.nt_in_live = 1;   // until the first upload of the state decides
-    std::vector<uint32_t> seg_id(npad, 0);
     std::vector<float> radius(npad, 0), length(npad, 0), area(npad, 0), cap(npad, 1), pkc(npad, 0), pka(npad, 0),
         camp(npad, 0.1f), pp1(npad, 0.5f), camkii(npad, 0), calcn(npad, 0);
     std::vector<double> vol(npad, 0);
@@ -384,7 +423,6 @@ static int build_device(Sim& S) {
     S.unsupported_mod = 0;
     for (uint32_t di = 0; di < n; di++) {
         const Segment& s = sg[S.dev2host[di]];
-        seg_id[di] = s.id;
         radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type;
         area[di] = seg_area(s); vol[di] = seg_vol(s); cap[di] = seg_cap(s);
         mcount[di] = s.modifier_count;
@@ -552,6 +590,16 @@ static int build_device(Sim& S) {
     }
 
     // ---- device allocation + upload ----
+    S.tick_dev = S.dalloc<uint64_t>(1);
+    d.tick = S.tick_dev;
+    S.stim_head = S.dalloc<uint32_t>(npad);
+    HIPCHK(hipMemsetAsync(S.stim_head, 0xFF, (size_t)npad * 4, S.stream));
+    memset(&d.stim, 0, sizeof(d.stim));
+    d.stim.head = S.stim_head;
+    d.stim.inj_tick = ~0ull;
+    S.id2dev.resize(n);
+    for (uint32_t di = 0; di < n; di++) S.id2dev[di] = {sg[S.dev2host[di]].id, di};
+    std::sort(S.id2dev.begin(), S.id2dev.end());
     d.V = S.dalloc<double>(npad);
     d.in = S.dalloc<double>((size_t)7 * npad);
     d.ex = S.dalloc<double>((size_t)7 * npad);
@@ -562,7 +610,6 @@ static int build_device(Sim& S) {
     d.nt_in = S.dalloc<float>((size_t)15 * npad);
     d.ibuf = S.dalloc<double>(npad);
     d.spikes = S.dalloc<uint32_t>(npad);
-    d.seg_id = S.dupload(seg_id);
     d.radius = S.dupload(radius);
     d.length = S.dupload(length);
     d.stype = S.dupload(stype);
@@ -755,21 +802,98 @@ static void sync_host(Sim& S) {
     if (S.host_stale) { xfer(S, false); S.host_stale = false; }
 }
 
+// ------------------------------------------------------------------ stimuli
+// One stimulus entry bound to one device segment: key = kind << 28 | index into that kind's
+// payload array. Sorting by (segment, key) gives each segment's list in the reference's order.
+struct StimTrip {
+    uint32_t dev, key;
+    bool operator<(const StimTrip& o) const { return dev != o.dev ? dev < o.dev : key < o.key; }
+};
+
+// Appends the entries of one kind, matched by Segment.id through a sorted (id, device) map.
+template <class T>
+static void stim_match(const std::vector<T>& v, uint32_t kind, uint32_t index0,
+                       const std::vector<std::pair<uint32_t, uint32_t>>& ids, std::vector<StimTrip>& out) {
+    for (size_t k = 0; k < v.size(); k++) {
+        auto lo = std::lower_bound(ids.begin(), ids.end(), std::make_pair(v[k].segment_id, 0u));
+        for (; lo != ids.end() && lo->first == v[k].segment_id; ++lo)
+            out.push_back({lo->second, (kind << 28) | (uint32_t)(index0 + k)});
+    }
+}
+
+// Builds the per-segment lists from S.st_* and trips, uploads them and repoints the heads
+// (segments of the previous lists are cleared first). Injections apply on tick inj_tick.
+static void stim_commit(Sim& S, std::vector<StimTrip>& trips, uint64_t inj_tick) {
+    Net& d = S.d;
+    std::sort(trips.begin(), trips.end());
+    const int nxt = S.st_cur_set ^ 1;
+    std::vector<uint32_t>& segs = S.st_segs[nxt];
+    segs.clear();
+    S.st_heads.clear();
+    S.st_ent.clear();
+    for (size_t a = 0; a < trips.size();) {
+        size_t b = a;
+        while (b < trips.size() && trips[b].dev == trips[a].dev) b++;
+        segs.push_back(trips[a].dev);
+        S.st_heads.push_back((uint32_t)S.st_ent.size());
+        S.st_ent.push_back((uint32_t)(b - a));
+        for (size_t q = a; q < b; q++) S.st_ent.push_back(trips[q].key);
+        a = b;
+    }
+    d.stim.cur = S.dput(S.sb_cur, S.st_cur);
+    d.stim.ramp = S.dput(S.sb_ramp, S.st_ramp);
+    d.stim.vc = S.dput(S.sb_vc, S.st_vc);
+    d.stim.ion = S.dput(S.sb_ion, S.st_ion);
+    d.stim.lig = S.dput(S.sb_lig, S.st_lig);
+    d.stim.ent = S.dput(S.sb_ent, S.st_ent);
+    const uint32_t* nsegs = S.dput(S.sb_segs[nxt], segs);
+    const uint32_t* nheads = S.dput(S.sb_heads, S.st_heads);
+    const std::vector<uint32_t>& prev = S.st_segs[S.st_cur_set];
+    if (!prev.empty())
+        k_stim_map<<<((uint32_t)prev.size() + 255) / 256, 256, 0, S.stream>>>(
+            static_cast<const uint32_t*>(S.sb_segs[S.st_cur_set].p), nullptr, (uint32_t)prev.size(), S.stim_head);
+    if (!segs.empty())
+        k_stim_map<<<((uint32_t)segs.size() + 255) / 256, 256, 0, S.stream>>>(nsegs, nheads, (uint32_t)segs.size(),
+                                                                               S.stim_head);
+    S.st_cur_set = nxt;
+    d.stim.head = S.stim_head;
+    d.stim.any = segs.empty() ? 0 : 1;
+    d.stim.inj_tick = inj_tick;
+    for (const LigandInjection& q : S.st_lig)   // sources intra_nt from that tick on
+        if (q.intracellular) d.nt_in_live = 1;
+}
+
+// gpu_backend_upload_stimuli payload -> lists (Segment.id matching of the CPU tick).
+static void stim_from_upload(Sim& S) {
+    const GpuStimulusData& g = *S.stim_host;
+    S.st_cur.assign(g.current_stim, g.current_stim + g.current_stim_count);
+    S.st_ramp.assign(g.ramps, g.ramps + g.ramp_count);
+    S.st_vc.assign(g.vclamps, g.vclamps + g.vclamp_count);
+    S.st_ion.assign(g.ion_inj, g.ion_inj + g.ion_inj_count);
+    S.st_lig.assign(g.lig_inj, g.lig_inj + g.lig_inj_count);
+    std::vector<StimTrip> trips;
+    stim_match(S.st_cur, SK_CUR, 0, S.id2dev, trips);
+    stim_match(S.st_ramp, SK_RAMP, 0, S.id2dev, trips);
+    stim_match(S.st_ion, SK_ION, 0, S.id2dev, trips);
+    stim_match(S.st_lig, SK_LIG, 0, S.id2dev, trips);
+    stim_match(S.st_vc, SK_VCLAMP, 0, S.id2dev, trips);
+    stim_commit(S, trips, (S.st_ion.empty() && S.st_lig.empty()) ? ~0ull : S.tick);
+}
+
 // ------------------------------------------------------------------ tick
 static void run_tick(Sim& S, float dt) {
     Net& d = S.d;
     if (S.coherence == BCHIP_COHERENCE_COMPAT && !S.host_stale) xfer(S, true);
     if (S.coherence == BCHIP_COHERENCE_COMPAT && S.host_stale) { xfer(S, false); S.host_stale = false; xfer(S, true); }
-    const uint64_t tick = S.tick++;
-    if (S.stim_dirty) {
-        HIPCHK(hipMemcpyAsync(S.stim_dev, S.stim_host, sizeof(GpuStimulusData), hipMemcpyHostToDevice, S.stream));
+    if (S.stim_dirty) {   // injections of this upload apply on the tick about to run
+        stim_from_upload(S);
         S.stim_dirty = false;
     }
-    const GpuStimulusData* st = S.stim_present ? S.stim_dev : nullptr;
+    S.tick++;
     const uint32_t gs = (d.n + 255) / 256, gy = (d.ns + 255) / 256;
     const bool tm = S.timing != 0;
     if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
-    k_modifiers<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    k_modifiers<<<gs, 256, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
         if (S.class_ntiles[0]) {
@@ -783,15 +907,16 @@ static void run_tick(Sim& S, float dt) {
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
         k_pre_summary<<<(d.nwatched + 255) / 256, 256, 0, S.stream>>>(d);
-        k_syn_pre<<<std::min<uint32_t>(gy, 8192), 256, 0, S.stream>>>(d, tick, dt);
+        k_syn_pre<<<std::min<uint32_t>(gy, 8192), 256, 0, S.stream>>>(d, dt);
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[3], S.stream));
-    if (S.receptors_only) k_syn_post_rec<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
-    else k_syn_post<<<gs, 256, 0, S.stream>>>(d, st, tick, dt);
+    if (S.receptors_only) k_syn_post_rec<<<gs, 256, 0, S.stream>>>(d, dt);
+    else k_syn_post<<<gs, 256, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[4], S.stream));
     if (d.ns) k_syn_plast<<<gy, 256, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[5], S.stream));
-    if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, tick, dt);
+    if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, dt);
+    k_tick_advance<<<1, 64, 0, S.stream>>>(S.tick_dev);
     if (tm) HIPCHK(hipEventRecord(S.ev[6], S.stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(S.stream));
@@ -802,11 +927,6 @@ static void run_tick(Sim& S, float dt) {
             S.last_ms[k] = ms;
         }
     S.ticks_run++;
-    if (S.stim_present && (S.stim_host->ion_inj_count || S.stim_host->lig_inj_count)) {
-        S.stim_host->ion_inj_count = 0;   // one-shot (simulation_host.cpp:304-306)
-        S.stim_host->lig_inj_count = 0;
-        S.stim_dirty = true;
-    }
     S.host_stale = true;
     if (S.coherence == BCHIP_COHERENCE_COMPAT) sync_host(S);
 }
@@ -824,6 +944,9 @@ static int load_into(Sim& S, BionData* src_owned) {
     int rc = build_device(S);
     if (rc != 0) { S.free_device(); free_bion(&S.host); S.loaded = false; return rc; }
     xfer(S, true);
+    S.st_segs[0].clear();   // the lists named device segments of the previous network
+    S.st_segs[1].clear();
+    if (S.stim_present) S.stim_dirty = true;
     S.loaded = true;
     return 0;
 }
@@ -857,7 +980,6 @@ void* gpu_backend_create_on_device(int device_index) {
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
         s->stim_host = new GpuStimulusData();
         memset(s->stim_host, 0, sizeof(GpuStimulusData));
-        HIPCHK(hipMalloc(&s->stim_dev, sizeof(GpuStimulusData)));
         return s;
     } catch (...) {
         return nullptr;

@@ -45,11 +45,11 @@ __device__ static inline uint64_t delay_ticks(float delay, float dt) {   // dela
 }
 
 // ------------------------------------------------------------------ step 1
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_modifiers(Net d, const GpuStimulusData* __restrict__ st,
-                                                   uint64_t tick, float dt) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_modifiers(Net d, float dt) {
     exp_table_stage();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n) return;
+    const uint64_t tick = *d.tick;
     const size_t P = d.npad;
     SegRegs s;
     s.V = d.V[i];
@@ -80,42 +80,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         m.av = d.mav[j * P + i];
         itot += dispatch(t, m, s, dt);
     }
-    if (st) {
-        const uint32_t sid = d.seg_id[i];
-        for (uint32_t k = 0; k < st->current_stim_count; k++) {
-            const CurrentStimulus q = st->current_stim[k];
-            if (q.segment_id == sid) {
+    // stimuli (simulation_host.cpp:247-300): this segment's list, in the reference's order
+    const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
+    if (sh != ~0u) {
+        const uint32_t cnt = d.stim.ent[sh];
+        for (uint32_t e = 1; e <= cnt; e++) {
+            const uint32_t en = d.stim.ent[sh + e];
+            const uint32_t kind = en >> 28, x = en & 0x0FFFFFFFu;
+            if (kind == SK_CUR) {
+                const CurrentStimulus q = d.stim.cur[x];
                 uint64_t dur = (uint64_t)(q.duration_ms / dt);
                 if (tick >= q.start_tick && tick < q.start_tick + dur) itot += q.amplitude_pA;
-            }
-        }
-        for (uint32_t k = 0; k < st->ramp_count; k++) {
-            const RampStimulus q = st->ramps[k];
-            if (q.segment_id == sid && tick >= q.start_tick && tick < q.end_tick) {
-                float frac = (q.end_tick > q.start_tick)
-                                 ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick)
-                                 : 1.0f;
-                float amp = q.start_pA + (q.end_pA - q.start_pA) * frac;
-                itot += amp;
-            }
-        }
-        for (uint32_t k = 0; k < st->ion_inj_count; k++) {
-            const IonInjection q = st->ion_inj[k];
-            if (q.segment_id == sid && q.ion_idx >= 0 && q.ion_idx < 7) {
-                // register arrays: index with compile-time constants only
+            } else if (kind == SK_RAMP) {
+                const RampStimulus q = d.stim.ramp[x];
+                if (tick >= q.start_tick && tick < q.end_tick) {
+                    float frac = (q.end_tick > q.start_tick)
+                                     ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick)
+                                     : 1.0f;
+                    float amp = q.start_pA + (q.end_pA - q.start_pA) * frac;
+                    itot += amp;
+                }
+            } else if (kind == SK_ION) {
+                const IonInjection q = d.stim.ion[x];
+                if (tick == d.stim.inj_tick && q.ion_idx >= 0 && q.ion_idx < 7) {
+                    // register arrays: index with compile-time constants only
 #pragma unroll
-                for (int k = 0; k < 7; k++)
-                    if (k == q.ion_idx) {
-                        if (q.extracellular) s.ex[k] += q.amount_mM;
-                        else s.in[k] += q.amount_mM;
-                    }
-            }
-        }
-        for (uint32_t k = 0; k < st->lig_inj_count; k++) {
-            const LigandInjection q = st->lig_inj[k];
-            if (q.segment_id == sid && q.nt_idx >= 0 && q.nt_idx < 15) {
-                if (q.intracellular) d.nt_in[q.nt_idx * P + i] += q.amount;
-                else d.nt[q.nt_idx * P + i] += q.amount;
+                    for (int k = 0; k < 7; k++)
+                        if (k == q.ion_idx) {
+                            if (q.extracellular) s.ex[k] += q.amount_mM;
+                            else s.in[k] += q.amount_mM;
+                        }
+                }
+            } else if (kind == SK_LIG) {
+                const LigandInjection q = d.stim.lig[x];
+                if (tick == d.stim.inj_tick && q.nt_idx >= 0 && q.nt_idx < 15) {
+                    if (q.intracellular) d.nt_in[q.nt_idx * P + i] += q.amount;
+                    else d.nt[q.nt_idx * P + i] += q.amount;
+                }
             }
         }
     }
@@ -698,8 +699,9 @@ __global__ __launch_bounds__(256) void k_pre_summary(Net d) {
     d.presum[w] = ps;
 }
 
-__global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt) {
+__global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     exp_table_stage();
+    const uint64_t tick = *d.tick;
     const double e300 = bc::exp(-dt / 300.0f);   // Cav2.1 inactivation relaxation, tau 300 ms
     const size_t NS = d.ns;
     for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < d.ns; y += gridDim.x * blockDim.x) {
@@ -800,8 +802,7 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, uint64_t tick, float dt)
 // ------------------------------------------------------------------ step 3 post-side + step 4
 // kReceptorsOnly: every receptor slot holds AMPA, NMDA or GABA-A (a leaner dispatch).
 template <bool kReceptorsOnly>
-__device__ __attribute__((always_inline)) inline void syn_post(const Net& d, const GpuStimulusData* __restrict__ st,
-                                                              uint64_t tick, float dt) {
+__device__ __attribute__((always_inline)) inline void syn_post(const Net& d, float dt) {
     // kStage: the block's 256-segment window is written back through LDS once every lane is
     // done, one coalesced row per field; lanes walk the window in chain-length order and finish
     // at different times, so direct writes would leave lines partly written for long stretches.
@@ -934,11 +935,15 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
     if (dV < -50.0f) dV = -50.0f;
     V += dV;
     if (prev <= -40.0f && V > -40.0f) d.spikes[i]++;
-    if (st) {
-        const uint32_t sid = d.seg_id[i];
-        for (uint32_t k = 0; k < st->vclamp_count; k++) {
-            const VoltageClamp q = st->vclamps[k];
-            if (q.segment_id == sid && tick >= q.start_tick && tick < q.end_tick) V = q.target_mV;
+    const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
+    if (sh != ~0u) {   // voltage clamps in index order: the last active one wins
+        const uint64_t tick = *d.tick;
+        const uint32_t cnt = d.stim.ent[sh];
+        for (uint32_t e = 1; e <= cnt; e++) {
+            const uint32_t en = d.stim.ent[sh + e];
+            if ((en >> 28) != SK_VCLAMP) continue;
+            const VoltageClamp q = d.stim.vc[en & 0x0FFFFFFFu];
+            if (tick >= q.start_tick && tick < q.end_tick) V = q.target_mV;
         }
     }
     if (V != V) V = -65.0f;
@@ -966,15 +971,14 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, con
 
 // AMPA/NMDA/GABA-A receptors only: four wavefronts per SIMD fit in the register budget.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_syn_post_rec(
-    Net d, const GpuStimulusData* __restrict__ st, uint64_t tick, float dt) {
+    Net d, float dt) {
     exp_table_stage();
-    syn_post<true>(d, st, tick, dt);
+    syn_post<true>(d, dt);
 }
 // Any receptor type (full modifier dispatch).
-__global__ __launch_bounds__(256) void k_syn_post(Net d, const GpuStimulusData* __restrict__ st, uint64_t tick,
-                                                  float dt) {
+__global__ __launch_bounds__(256) void k_syn_post(Net d, float dt) {
     exp_table_stage();
-    syn_post<false>(d, st, tick, dt);
+    syn_post<false>(d, dt);
 }
 
 // ------------------------------------------------------------------ step 3, synapse plasticity
@@ -1053,7 +1057,8 @@ __global__ __launch_bounds__(256) void k_syn_plast(Net d, float dt) {
 }
 
 // ------------------------------------------------------------------ step 3, vesicle pool
-__global__ __launch_bounds__(256) void k_syn_rrp(Net d, uint64_t tick, float dt) {
+__global__ __launch_bounds__(256) void k_syn_rrp(Net d, float dt) {
+    const uint64_t tick = *d.tick;
     const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= d.ns) return;
     if (d.y_flags[y] & YF_SKIP) return;
@@ -1097,6 +1102,20 @@ __global__ __launch_bounds__(256) void k_syn_rrp(Net d, uint64_t tick, float dt)
             }
     }
     d.y_rrp[y] = rrp;
+}
+
+// ------------------------------------------------------------------ tick counter, stimulus lists
+// The tick the kernels read lives in device memory so a sequence of ticks can be captured once
+// and replayed; this one-lane kernel ends every tick.
+__global__ void k_tick_advance(uint64_t* tick) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *tick += 1;
+}
+
+// Points the listed segments at their stimulus lists (heads == nullptr: clears them).
+__global__ __launch_bounds__(256) void k_stim_map(const uint32_t* __restrict__ segs, const uint32_t* __restrict__ heads,
+                                                  uint32_t count, uint32_t* __restrict__ head) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) head[segs[k]] = heads ? heads[k] : ~0u;
 }
 
 }  // namespace bc

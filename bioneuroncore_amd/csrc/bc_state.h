@@ -11,6 +11,7 @@
 #pragma once
 #include <stdint.h>
 #include <stddef.h>
+#include "../../include/biocore_abi.h"
 
 namespace bc {
 
@@ -60,6 +61,22 @@ struct PreSum {
 };
 static_assert(sizeof(PreSum) == 64, "PreSum layout");
 
+// Stimuli as per-segment ordered lists. head[s] is ~0u, or the index in ent of a count word
+// followed by that many entries (kind << 28 | payload index), sorted by kind in the order the
+// tick applies them and, within a kind, by stimulus index - the reference's matching order.
+enum : uint32_t { SK_CUR = 0, SK_RAMP = 1, SK_ION = 2, SK_LIG = 3, SK_VCLAMP = 4 };
+struct StimDev {
+    const uint32_t* head;   // [n]
+    const uint32_t* ent;
+    const CurrentStimulus* cur;
+    const RampStimulus* ramp;
+    const VoltageClamp* vc;
+    const IonInjection* ion;
+    const LigandInjection* lig;
+    uint64_t inj_tick;      // one-shot injections apply on this tick only
+    uint32_t any;           // some segment has a list
+};
+
 struct Net {
     // sizes
     uint32_t n;          // segments
@@ -87,8 +104,10 @@ struct Net {
     double* ibuf;
     uint32_t* spikes;
 
+    const uint64_t* tick;   // device tick counter (advanced by k_tick_advance after each tick)
+    StimDev stim;
+
     // segment static data
-    const uint32_t* seg_id;
     const float* radius;
     const float* length;
     const uint8_t* stype;
