@@ -41,7 +41,9 @@ PLUGIN_SYMBOLS = ("register_modifier_current", "register_modulator_release",
 EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence", "bchip_push_host_state",
                "bchip_run_ticks", "bchip_tick_count", "bchip_spike_counts", "bchip_delay_slots", "bchip_stats",
                "bchip_set_phase_timing", "bchip_stream", "bchip_plugin_load", "bchip_registered_modifier",
-               "bchip_modifier_supported")
+               "bchip_modifier_supported", "bchip_load_batch", "bchip_network_count", "bchip_set_stimuli",
+               "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
+               "bchip_set_graphs")
 
 
 class BionDataC(C.Structure):
@@ -57,6 +59,14 @@ class BchipStats(C.Structure):
                 ("lds_bytes_per_tile", C.c_uint32), ("_pad", C.c_uint32),
                 ("unsupported_modifiers", C.c_uint64), ("unsupported_synapses", C.c_uint64),
                 ("order_hazard_synapses", C.c_uint64), ("last_phase_ms", C.c_float * 6)]
+
+
+class BchipSchedule(C.Structure):
+    _fields_ = [("modifiers_block", C.c_uint32), ("diffusion_tiles", C.c_uint32), ("pre_grid", C.c_uint32),
+                ("plast_block", C.c_uint32), ("rrp_block", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 assert C.sizeof(BionDataC) == 168
@@ -99,6 +109,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_plugin_load.argtypes = [C.c_char_p]
     lib.bchip_registered_modifier.argtypes = [C.c_uint8]
     lib.bchip_modifier_supported.argtypes = [C.c_uint8]
+    lib.bchip_load_batch.argtypes = [P, C.POINTER(C.POINTER(BionDataC)), C.c_uint32]
+    lib.bchip_network_count.restype = C.c_uint32
+    lib.bchip_network_count.argtypes = [P]
+    lib.bchip_set_stimuli.argtypes = [P, C.c_uint32] + [P, C.c_uint32] * 5
+    lib.bchip_batch_read.argtypes = [P, C.c_uint32, P, P, P]
+    lib.bchip_get_schedule.argtypes = [P, C.POINTER(BchipSchedule)]
+    lib.bchip_set_schedule.argtypes = [P, C.POINTER(BchipSchedule)]
+    lib.bchip_autotune.argtypes = [P, C.c_float, C.c_uint32]
+    lib.bchip_set_graphs.argtypes = [P, C.c_int]
     _lib = lib
     return lib
 
@@ -160,6 +179,68 @@ class Simulation:
         b.num_neurons, b.num_segments, b.num_connections, b.num_synapses = [len(a) for a in arrs]
         rc = int(self.lib.bchip_load_bion_data(self.h, C.byref(b)))
         return rc
+
+    def load_batch(self, nets: list[bion.Network]) -> int:
+        """Load independent networks as one batch (network k's results equal a separate load)."""
+        keep, ptrs = [], []
+        for net in nets:
+            hdr = bion.make_header(net)
+            b = BionDataC()
+            C.memmove(b.header, hdr.ctypes.data, 100)
+            arrs = [np.ascontiguousarray(a) for a in (net.neurons, net.segments, net.connections, net.synapses)]
+            b.neurons, b.segments, b.connections, b.synapses = [a.ctypes.data if len(a) else None for a in arrs]
+            b.num_neurons, b.num_segments, b.num_connections, b.num_synapses = [len(a) for a in arrs]
+            keep.append((b, arrs))
+            ptrs.append(C.pointer(b))
+        arr = (C.POINTER(BionDataC) * len(ptrs))(*ptrs)
+        self._batch_sizes = [(len(n.segments), len(n.synapses)) for n in nets]
+        return int(self.lib.bchip_load_batch(self.h, arr, len(ptrs)))
+
+    def network_count(self) -> int:
+        return int(self.lib.bchip_network_count(self.h))
+
+    def set_stimuli(self, net: int, current=None, ramp=None, vclamp=None, ion=None, lig=None) -> None:
+        """Unbounded stimuli of one network (structured arrays of the STIM_DT dtypes)."""
+        args, keep = [], []
+        for arr, dt in ((current, bion.CURRENT_STIM_DT), (ramp, bion.RAMP_DT), (vclamp, bion.VCLAMP_DT),
+                        (ion, bion.ION_INJ_DT), (lig, bion.LIG_INJ_DT)):
+            a = np.ascontiguousarray(arr if arr is not None else np.zeros(0, dt), dtype=dt)
+            keep.append(a)   # alive until the call returns (the library copies)
+            args += [a.ctypes.data if len(a) else None, len(a)]
+        if self.lib.bchip_set_stimuli(self.h, net, *args) != 0:
+            raise RuntimeError("bchip_set_stimuli failed")
+
+    def batch_read(self, net: int):
+        """(segments, synapses, spikes) of network `net`, in its own indices."""
+        ns, ny = self._batch_sizes[net]
+        segs = np.zeros(ns, bion.SEGMENT_DT)
+        syns = np.zeros(ny, bion.SYNAPSE_DT)
+        spk = np.zeros(ns, np.uint32)
+        if self.lib.bchip_batch_read(self.h, net, segs.ctypes.data, syns.ctypes.data if ny else None,
+                                     spk.ctypes.data) != 0:
+            raise RuntimeError("bchip_batch_read failed")
+        return segs, syns, spk
+
+    def schedule(self) -> BchipSchedule:
+        s = BchipSchedule()
+        self.lib.bchip_get_schedule(self.h, C.byref(s))
+        return s
+
+    def set_schedule(self, **fields) -> None:
+        s = self.schedule()
+        for f, v in fields.items():
+            setattr(s, f, v)
+        if self.lib.bchip_set_schedule(self.h, C.byref(s)) != 0:
+            raise ValueError(f"bad schedule {fields}")
+
+    def autotune(self, dt: float, ticks_per_candidate: int = 3) -> int:
+        r = int(self.lib.bchip_autotune(self.h, dt, ticks_per_candidate))
+        if r < 0:
+            raise RuntimeError("bchip_autotune failed")
+        return r
+
+    def set_graphs(self, on: bool) -> None:
+        self.lib.bchip_set_graphs(self.h, 1 if on else 0)
 
     def set_coherence(self, mode: int) -> None:
         if self.lib.bchip_set_coherence(self.h, mode) != 0:

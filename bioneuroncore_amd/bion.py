@@ -295,6 +295,32 @@ def config1(seed: int = 0xB10C0001) -> Network:
     return build_network(64, 8, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21], 4.0, "ampa", 5, seed)
 
 
+CONFIG4_TYPES = [NAV12, KVDR, K2P, NALCN, PUMP, CAV21]
+
+
+def config4_neurons(n_networks: int = 512, seed: int = 0xB10C0004, lo: float = 2_000.0,
+                    hi: float = 200_000.0) -> np.ndarray:
+    """Neuron counts of the batched sweep: segment counts log-uniform in [lo, hi], 8 segments per
+    neuron (mean ~43k segments, ~22M in total at 512 networks)."""
+    rng = np.random.default_rng(seed)
+    seg = np.exp(rng.uniform(np.log(lo), np.log(hi), n_networks))
+    return np.maximum(1, np.round(seg / 8.0)).astype(np.int64)
+
+
+def config4_network(k: int, n_neurons: int, seed: int = 0xB10C0004) -> Network:
+    """Network k of the batched sweep: built like the 64-neuron chain network (soma + 7-segment
+    chain, 0.5 AMPA synapses per segment, delays 1-5 ticks) with its own seed, and its own
+    parameter set: every channel type's g_max scaled by one U(0.5, 2) factor."""
+    s = seed + 1 + k
+    net = build_network(int(n_neurons), 8, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21], 4.0, "ampa", 5, s)
+    rng = np.random.default_rng(s ^ 0x5EED)
+    dens = net.segments["modifiers"]["density"]
+    types = net.segments["modifiers"]["type"]
+    for t in CONFIG4_TYPES:
+        dens[types == t] *= rng.uniform(0.5, 2.0)
+    return net
+
+
 CONFIG2_TYPES = [NAV12, NAV16, KV11, KV31, KV7, CAV21, HCN1, SK, BK, K2P, PUMP, NALCN]
 
 

@@ -217,7 +217,33 @@ struct Sim {
     };
     DBuf sb_cur, sb_ramp, sb_vc, sb_ion, sb_lig, sb_ent, sb_segs[2], sb_heads;
     uint32_t* stim_head = nullptr;
-    std::vector<std::pair<uint32_t, uint32_t>> id2dev;   // (Segment.id, device index), sorted
+    std::vector<std::pair<uint64_t, uint32_t>> id2dev;   // (network << 32 | Segment.id, device index), sorted
+    // Networks of a batch (a plain load is one network): host index offsets into the union,
+    // and each network's stimuli (unbounded; Segment.id matched within the network).
+    std::vector<uint32_t> net_seg_off{0, 0}, net_conn_off{0, 0}, net_syn_off{0, 0};
+    struct NetStim {
+        std::vector<CurrentStimulus> cur;
+        std::vector<RampStimulus> ramp;
+        std::vector<VoltageClamp> vc;
+        std::vector<IonInjection> ion;
+        std::vector<LigandInjection> lig;
+    };
+    std::vector<NetStim> net_stim{NetStim{}};
+    bool inj_committed = false;   // injections are in the device lists (cleared after their tick)
+    // Launch schedule (bchip_autotune picks it; results do not depend on it).
+    BchipSchedule sched{256, 1, 8192, 256, 256};
+    // Captured tick sequence (bchip_run_ticks): replayed while the launch inputs are unchanged.
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_ticks = 0;
+    Net graph_net{};
+    float graph_dt = 0.0f;
+    BchipSchedule graph_sched{};
+    bool use_graphs = true;
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_ticks = 0;
+    }
     int coherence = BCHIP_COHERENCE_COMPAT;
     bool host_stale = false;
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
@@ -276,6 +302,7 @@ struct Sim {
     ~Sim() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
         free_device();
         free_stim_buffers();
         delete stim_host;
@@ -598,7 +625,12 @@ static int build_device(Sim& S) {
     d.stim.head = S.stim_head;
     d.stim.inj_tick = ~0ull;
     S.id2dev.resize(n);
-    for (uint32_t di = 0; di < n; di++) S.id2dev[di] = {sg[S.dev2host[di]].id, di};
+    for (uint32_t di = 0; di < n; di++) {
+        const uint32_t h = S.dev2host[di];
+        const uint64_t net = (uint64_t)(std::upper_bound(S.net_seg_off.begin(), S.net_seg_off.end(), h) -
+                                        S.net_seg_off.begin() - 1);
+        S.id2dev[di] = {(net << 32) | sg[h].id, di};
+    }
     std::sort(S.id2dev.begin(), S.id2dev.end());
     d.V = S.dalloc<double>(npad);
     d.in = S.dalloc<double>((size_t)7 * npad);
@@ -810,13 +842,15 @@ struct StimTrip {
     bool operator<(const StimTrip& o) const { return dev != o.dev ? dev < o.dev : key < o.key; }
 };
 
-// Appends the entries of one kind, matched by Segment.id through a sorted (id, device) map.
+// Appends network net's entries of one kind, matched by Segment.id through the sorted
+// (network << 32 | id, device) map; index0 is where they start in the combined payload array.
 template <class T>
-static void stim_match(const std::vector<T>& v, uint32_t kind, uint32_t index0,
-                       const std::vector<std::pair<uint32_t, uint32_t>>& ids, std::vector<StimTrip>& out) {
+static void stim_match(const std::vector<T>& v, uint32_t kind, uint64_t net, size_t index0,
+                       const std::vector<std::pair<uint64_t, uint32_t>>& ids, std::vector<StimTrip>& out) {
     for (size_t k = 0; k < v.size(); k++) {
-        auto lo = std::lower_bound(ids.begin(), ids.end(), std::make_pair(v[k].segment_id, 0u));
-        for (; lo != ids.end() && lo->first == v[k].segment_id; ++lo)
+        const uint64_t key = (net << 32) | v[k].segment_id;
+        auto lo = std::lower_bound(ids.begin(), ids.end(), std::make_pair(key, 0u));
+        for (; lo != ids.end() && lo->first == key; ++lo)
             out.push_back({lo->second, (kind << 28) | (uint32_t)(index0 + k)});
     }
 }
@@ -863,41 +897,54 @@ static void stim_commit(Sim& S, std::vector<StimTrip>& trips, uint64_t inj_tick)
         if (q.intracellular) d.nt_in_live = 1;
 }
 
-// gpu_backend_upload_stimuli payload -> lists (Segment.id matching of the CPU tick).
-static void stim_from_upload(Sim& S) {
-    const GpuStimulusData& g = *S.stim_host;
-    S.st_cur.assign(g.current_stim, g.current_stim + g.current_stim_count);
-    S.st_ramp.assign(g.ramps, g.ramps + g.ramp_count);
-    S.st_vc.assign(g.vclamps, g.vclamps + g.vclamp_count);
-    S.st_ion.assign(g.ion_inj, g.ion_inj + g.ion_inj_count);
-    S.st_lig.assign(g.lig_inj, g.lig_inj + g.lig_inj_count);
+// All networks' stimuli -> one set of lists (Segment.id matching of the CPU tick, per network).
+template <class T>
+static void stim_cat(std::vector<T>& dst, const std::vector<T>& src, std::vector<size_t>& at) {
+    at.push_back(dst.size());
+    dst.insert(dst.end(), src.begin(), src.end());
+}
+static void stim_rebuild(Sim& S) {
+    S.st_cur.clear(); S.st_ramp.clear(); S.st_vc.clear(); S.st_ion.clear(); S.st_lig.clear();
     std::vector<StimTrip> trips;
-    stim_match(S.st_cur, SK_CUR, 0, S.id2dev, trips);
-    stim_match(S.st_ramp, SK_RAMP, 0, S.id2dev, trips);
-    stim_match(S.st_ion, SK_ION, 0, S.id2dev, trips);
-    stim_match(S.st_lig, SK_LIG, 0, S.id2dev, trips);
-    stim_match(S.st_vc, SK_VCLAMP, 0, S.id2dev, trips);
-    stim_commit(S, trips, (S.st_ion.empty() && S.st_lig.empty()) ? ~0ull : S.tick);
+    std::vector<size_t> a, b, c, e, f;
+    for (uint64_t k = 0; k < S.net_stim.size(); k++) {
+        const Sim::NetStim& ns = S.net_stim[k];
+        stim_cat(S.st_cur, ns.cur, a);
+        stim_cat(S.st_ramp, ns.ramp, b);
+        stim_cat(S.st_ion, ns.ion, c);
+        stim_cat(S.st_lig, ns.lig, e);
+        stim_cat(S.st_vc, ns.vc, f);
+        stim_match(ns.cur, SK_CUR, k, a.back(), S.id2dev, trips);
+        stim_match(ns.ramp, SK_RAMP, k, b.back(), S.id2dev, trips);
+        stim_match(ns.ion, SK_ION, k, c.back(), S.id2dev, trips);
+        stim_match(ns.lig, SK_LIG, k, e.back(), S.id2dev, trips);
+        stim_match(ns.vc, SK_VCLAMP, k, f.back(), S.id2dev, trips);
+    }
+    const bool inj = !S.st_ion.empty() || !S.st_lig.empty();
+    stim_commit(S, trips, inj ? S.tick : ~0ull);
+    S.inj_committed = inj;
+}
+// After a tick has run: committed injections were one-shot (simulation_host.cpp:304-306); the
+// device lists keep them but their tick has passed, and they leave the next rebuild.
+static void stim_after_tick(Sim& S) {
+    if (!S.inj_committed) return;
+    for (Sim::NetStim& ns : S.net_stim) { ns.ion.clear(); ns.lig.clear(); }
+    S.inj_committed = false;
 }
 
 // ------------------------------------------------------------------ tick
-static void run_tick(Sim& S, float dt) {
-    Net& d = S.d;
-    if (S.coherence == BCHIP_COHERENCE_COMPAT && !S.host_stale) xfer(S, true);
-    if (S.coherence == BCHIP_COHERENCE_COMPAT && S.host_stale) { xfer(S, false); S.host_stale = false; xfer(S, true); }
-    if (S.stim_dirty) {   // injections of this upload apply on the tick about to run
-        stim_from_upload(S);
-        S.stim_dirty = false;
-    }
-    S.tick++;
-    const uint32_t gs = (d.n + 255) / 256, gy = (d.ns + 255) / 256;
-    const bool tm = S.timing != 0;
+// The kernel sequence of one tick on S.stream (no host synchronisation or copies, so it can be
+// captured). With `tm`, events bracket the phases.
+static void launch_tick(Sim& S, float dt, bool tm) {
+    const Net& d = S.d;
+    const BchipSchedule& sc = S.sched;
+    const uint32_t gy = (d.ns + 255) / 256;
     if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
-    k_modifiers<<<gs, 256, 0, S.stream>>>(d, dt);
+    k_modifiers<<<(d.n + sc.modifiers_block - 1) / sc.modifiers_block, sc.modifiers_block, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
         if (S.class_ntiles[0]) {
-            constexpr int W = diff_tiles_per_block(64);
+            const uint32_t W = sc.diffusion_tiles;
             k_diffusion<64><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, W * sizeof(DiffLds<64>), S.stream>>>(
                 d, S.tiles[0], dt);
         }
@@ -907,17 +954,41 @@ static void run_tick(Sim& S, float dt) {
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
         k_pre_summary<<<(d.nwatched + 255) / 256, 256, 0, S.stream>>>(d);
-        k_syn_pre<<<std::min<uint32_t>(gy, 8192), 256, 0, S.stream>>>(d, dt);
+        k_syn_pre<<<std::min<uint32_t>(gy, sc.pre_grid), 256, 0, S.stream>>>(d, dt);
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[3], S.stream));
+    const uint32_t gs = (d.n + 255) / 256;   // the post pass's windows are 256 segments
     if (S.receptors_only) k_syn_post_rec<<<gs, 256, 0, S.stream>>>(d, dt);
     else k_syn_post<<<gs, 256, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[4], S.stream));
-    if (d.ns) k_syn_plast<<<gy, 256, 0, S.stream>>>(d, dt);
+    if (d.ns) k_syn_plast<<<(d.ns + sc.plast_block - 1) / sc.plast_block, sc.plast_block, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[5], S.stream));
-    if (d.ns) k_syn_rrp<<<gy, 256, 0, S.stream>>>(d, dt);
+    if (d.ns) k_syn_rrp<<<(d.ns + sc.rrp_block - 1) / sc.rrp_block, sc.rrp_block, 0, S.stream>>>(d, dt);
     k_tick_advance<<<1, 64, 0, S.stream>>>(S.tick_dev);
     if (tm) HIPCHK(hipEventRecord(S.ev[6], S.stream));
+}
+
+static void before_ticks(Sim& S) {
+    if (S.coherence == BCHIP_COHERENCE_COMPAT && !S.host_stale) xfer(S, true);
+    if (S.coherence == BCHIP_COHERENCE_COMPAT && S.host_stale) { xfer(S, false); S.host_stale = false; xfer(S, true); }
+    if (S.stim_dirty) {   // injections of this upload apply on the tick about to run
+        stim_rebuild(S);
+        S.stim_dirty = false;
+    }
+}
+
+static void after_ticks(Sim& S, uint32_t count) {
+    S.tick += count;
+    S.ticks_run += count;
+    stim_after_tick(S);
+    S.host_stale = true;
+    if (S.coherence == BCHIP_COHERENCE_COMPAT) sync_host(S);
+}
+
+static void run_tick(Sim& S, float dt) {
+    before_ticks(S);
+    const bool tm = S.timing != 0;
+    launch_tick(S, dt, tm);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(S.stream));
     if (tm)
@@ -926,14 +997,57 @@ static void run_tick(Sim& S, float dt) {
             HIPCHK(hipEventElapsedTime(&ms, S.ev[k], S.ev[k + 1]));
             S.last_ms[k] = ms;
         }
-    S.ticks_run++;
-    S.host_stale = true;
-    if (S.coherence == BCHIP_COHERENCE_COMPAT) sync_host(S);
+    after_ticks(S, 1);
+}
+
+// count ticks as replays of a captured sequence of up to kGraphTicks ticks (recaptured when the
+// launch inputs change). Only in FAST coherence without phase timing; otherwise tick by tick.
+constexpr uint32_t kGraphTicks = 16;
+static void run_ticks(Sim& S, float dt, uint32_t count) {
+    if (!count) return;
+    if (!S.use_graphs || S.timing || S.coherence != BCHIP_COHERENCE_FAST) {
+        for (uint32_t k = 0; k < count; k++) run_tick(S, dt);
+        return;
+    }
+    before_ticks(S);
+    const uint32_t g = std::min(count, kGraphTicks);
+    const bool same = S.graph && S.graph_ticks == g && S.graph_dt == dt &&
+                      memcmp(&S.graph_net, &S.d, sizeof(Net)) == 0 &&
+                      memcmp(&S.graph_sched, &S.sched, sizeof(BchipSchedule)) == 0;
+    if (!same) {
+        S.drop_graph();
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t k = 0; k < g; k++) launch_tick(S, dt, false);
+        HIPCHK(hipStreamEndCapture(S.stream, &gr));
+        hipError_t e = hipGraphInstantiate(&S.graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHK(e);
+        S.graph_ticks = g;
+        S.graph_net = S.d;
+        S.graph_dt = dt;
+        S.graph_sched = S.sched;
+    }
+    uint32_t done = 0;
+    for (; done + g <= count; done += g) HIPCHK(hipGraphLaunch(S.graph, S.stream));
+    for (; done < count; done++) launch_tick(S, dt, false);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(S.stream));
+    after_ticks(S, count);
+}
+
+// A plain load is one network.
+static void plain_networks(Sim& S, const BionData& b) {
+    S.net_seg_off = {0, b.num_segments};
+    S.net_conn_off = {0, b.num_connections};
+    S.net_syn_off = {0, b.num_synapses};
+    S.net_stim.resize(1);   // network 0 keeps its stimuli across reloads
 }
 
 static int load_into(Sim& S, BionData* src_owned) {
     HIPCHK(hipSetDevice(S.device));
     HIPCHK(hipStreamSynchronize(S.stream));
+    S.drop_graph();
     S.free_device();
     free_bion(&S.host);
     S.host = *src_owned;
@@ -999,6 +1113,7 @@ int gpu_backend_load_bion(void* sim, const char* path) {
     int rc = bc::read_bion(path, &b);
     if (rc != 0) return rc;
     try {
+        bc::plain_networks(S, b);
         return bc::load_into(S, &b);
     } catch (...) {
         bc::free_bion(&b);
@@ -1033,11 +1148,15 @@ void gpu_backend_upload_stimuli(void* sim, const GpuStimulusData* src) {
     if (!sim || !src) return;
     Sim& S = *static_cast<Sim*>(sim);
     memcpy(S.stim_host, src, sizeof(GpuStimulusData));
+    GpuStimulusData& g = *S.stim_host;
     auto cap = [](uint32_t& c) { if (c > BC_GPU_MAX_STIMULI) c = BC_GPU_MAX_STIMULI; };
-    cap(S.stim_host->current_stim_count); cap(S.stim_host->vclamp_count); cap(S.stim_host->ramp_count);
-    cap(S.stim_host->ion_inj_count); cap(S.stim_host->lig_inj_count);
-    for (uint32_t k = 0; k < S.stim_host->lig_inj_count; k++)   // sources intra_nt from step 1 on
-        if (S.stim_host->lig_inj[k].intracellular) S.d.nt_in_live = 1;
+    cap(g.current_stim_count); cap(g.vclamp_count); cap(g.ramp_count); cap(g.ion_inj_count); cap(g.lig_inj_count);
+    Sim::NetStim& ns = S.net_stim[0];
+    ns.cur.assign(g.current_stim, g.current_stim + g.current_stim_count);
+    ns.ramp.assign(g.ramps, g.ramps + g.ramp_count);
+    ns.vc.assign(g.vclamps, g.vclamps + g.vclamp_count);
+    ns.ion.assign(g.ion_inj, g.ion_inj + g.ion_inj_count);
+    ns.lig.assign(g.lig_inj, g.lig_inj + g.lig_inj_count);
     S.stim_present = true;
     S.stim_dirty = true;
 }
@@ -1110,6 +1229,7 @@ int bchip_load_bion_data(void* sim, const BionData* src) {
     for (uint32_t i = 0; i < b.num_segments; i++)
         if (b.segments[i].modifier_count > BC_MAX_MODIFIERS) { bc::free_bion(&b); return BION_ERR_BAD_MODCOUNT; }
     try {
+        bc::plain_networks(S, b);
         return bc::load_into(S, &b);
     } catch (...) {
         bc::free_bion(&b);
@@ -1152,8 +1272,200 @@ int bchip_run_ticks(void* sim, float dt, uint32_t count) {
     if (!S.loaded) return -1;
     try {
         (void)hipSetDevice(S.device);
-        for (uint32_t k = 0; k < count; k++) bc::run_tick(S, dt);
+        bc::run_ticks(S, dt, count);
     } catch (...) { return -1; }
+    return 0;
+}
+
+int bchip_load_batch(void* sim, const BionData* const* nets, uint32_t count) {
+    if (!sim || !nets || count == 0) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    uint64_t tn = 0, ts = 0, tc = 0, ty = 0;
+    for (uint32_t k = 0; k < count; k++) {
+        const BionData* x = nets[k];
+        if (!x || !x->segments || x->num_segments == 0) return -1;
+        tn += x->num_neurons; ts += x->num_segments; tc += x->num_connections; ty += x->num_synapses;
+    }
+    if (ts >= (1ull << 32) || tc >= (1ull << 32) || ty >= (1ull << 31)) return BION_ERR_ALLOC;
+    BionData b;
+    memset(&b, 0, sizeof(b));
+    b.header = nets[0]->header;
+    b.num_neurons = (uint32_t)tn; b.num_segments = (uint32_t)ts;
+    b.num_connections = (uint32_t)tc; b.num_synapses = (uint32_t)ty;
+    b.header.num_neurons = b.num_neurons; b.header.num_segments = b.num_segments;
+    b.header.num_intra_connections = b.num_connections; b.header.num_synapses = b.num_synapses;
+    b.header.num_nuclei = 0;
+    try {
+        b.neurons = new Neuron[tn ? tn : 1];
+        b.segments = new Segment[ts];
+        b.connections = new IntraConnection[tc ? tc : 1];
+        b.synapses = new Synapse[ty ? ty : 1];
+    } catch (...) {
+        bc::free_bion(&b);
+        return BION_ERR_ALLOC;
+    }
+    std::vector<uint32_t> so{0}, co{0}, yo{0};
+    uint32_t on = 0;
+    for (uint32_t k = 0; k < count; k++) {
+        const BionData* x = nets[k];
+        const uint32_t s0 = so.back(), c0 = co.back(), y0 = yo.back(), n = x->num_segments;
+        for (uint32_t i = 0; i < x->num_neurons; i++) {
+            Neuron r = x->neurons[i];
+            r.segment_start += s0;
+            r.soma_index += s0;
+            b.neurons[on + i] = r;
+        }
+        memcpy(b.segments + s0, x->segments, sizeof(Segment) * n);
+        for (uint32_t i = 0; i < x->num_connections; i++) {
+            IntraConnection c = x->connections[i];
+            if (c.segment_a >= n || c.segment_b >= n) { bc::free_bion(&b); return BION_ERR_BAD_CONN; }
+            c.segment_a += s0;
+            c.segment_b += s0;
+            b.connections[c0 + i] = c;
+        }
+        for (uint32_t i = 0; i < x->num_synapses; i++) {
+            const Synapse& y = x->synapses[i];
+            if (y.pre_segment >= n || y.post_segment >= n || y.type >= 3 ||
+                y.post_receptor_count > BC_MAX_POST_RECEPTORS || y.pre_autoreceptor_count > BC_MAX_PRE_AUTORECEPTORS) {
+                bc::free_bion(&b);
+                return BION_ERR_BAD_SYNAPSE;
+            }
+            Synapse& z = b.synapses[y0 + i];
+            z = y;
+            z.pre_segment += s0;
+            z.post_segment += s0;
+        }
+        for (uint32_t i = 0; i < n; i++)
+            if (x->segments[i].modifier_count > BC_MAX_MODIFIERS) { bc::free_bion(&b); return BION_ERR_BAD_MODCOUNT; }
+        on += x->num_neurons;
+        so.push_back(s0 + n);
+        co.push_back(c0 + x->num_connections);
+        yo.push_back(y0 + x->num_synapses);
+    }
+    try {
+        S.net_seg_off = so;
+        S.net_conn_off = co;
+        S.net_syn_off = yo;
+        S.net_stim.assign(count, Sim::NetStim{});
+        S.stim_present = false;
+        return bc::load_into(S, &b);
+    } catch (...) {
+        bc::free_bion(&b);
+        return BION_ERR_ALLOC;
+    }
+}
+
+uint32_t bchip_network_count(void* sim) {
+    return sim ? (uint32_t)static_cast<Sim*>(sim)->net_stim.size() : 0;
+}
+
+int bchip_set_stimuli(void* sim, uint32_t net, const CurrentStimulus* cur, uint32_t ncur,
+                      const RampStimulus* ramp, uint32_t nramp, const VoltageClamp* vc, uint32_t nvc,
+                      const IonInjection* ion, uint32_t nion, const LigandInjection* lig, uint32_t nlig) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (net >= S.net_stim.size()) return -1;
+    if ((ncur && !cur) || (nramp && !ramp) || (nvc && !vc) || (nion && !ion) || (nlig && !lig)) return -1;
+    Sim::NetStim& ns = S.net_stim[net];
+    ns.cur.assign(cur, cur + ncur);
+    ns.ramp.assign(ramp, ramp + nramp);
+    ns.vc.assign(vc, vc + nvc);
+    ns.ion.assign(ion, ion + nion);
+    ns.lig.assign(lig, lig + nlig);
+    S.stim_present = true;
+    S.stim_dirty = true;
+    return 0;
+}
+
+int bchip_batch_read(void* sim, uint32_t net, Segment* segs, Synapse* syns, uint32_t* spikes) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded || net + 1 >= S.net_seg_off.size()) return -1;
+    try {
+        (void)hipSetDevice(S.device);
+        bc::sync_host(S);
+    } catch (...) { return -1; }
+    const uint32_t s0 = S.net_seg_off[net], s1 = S.net_seg_off[net + 1];
+    const uint32_t y0 = S.net_syn_off[net], y1 = S.net_syn_off[net + 1];
+    if (segs) memcpy(segs, S.host.segments + s0, sizeof(Segment) * (s1 - s0));
+    if (syns)
+        for (uint32_t i = y0; i < y1; i++) {
+            syns[i - y0] = S.host.synapses[i];
+            syns[i - y0].pre_segment -= s0;
+            syns[i - y0].post_segment -= s0;
+        }
+    if (spikes) {
+        std::vector<uint32_t> dev(S.d.n);
+        if (hipMemcpy(dev.data(), S.d.spikes, (size_t)S.d.n * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        for (uint32_t h = s0; h < s1; h++) spikes[h - s0] = dev[S.host2dev[h]];
+    }
+    return 0;
+}
+
+int bchip_get_schedule(void* sim, BchipSchedule* out) {
+    if (!sim || !out) return -1;
+    *out = static_cast<Sim*>(sim)->sched;
+    return 0;
+}
+
+int bchip_set_schedule(void* sim, const BchipSchedule* in) {
+    if (!sim || !in) return -1;
+    auto blk = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
+    if (!blk(in->modifiers_block) || !blk(in->plast_block) || !blk(in->rrp_block)) return -1;
+    if (in->diffusion_tiles != 1 && in->diffusion_tiles != 2 && in->diffusion_tiles != 4) return -1;
+    if (in->pre_grid < 1 || in->pre_grid > (1u << 20)) return -1;
+    static_cast<Sim*>(sim)->sched = *in;
+    return 0;
+}
+
+int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate) {
+    if (!sim || ticks_per_candidate == 0) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return -1;
+    // field, candidates, the phase its kernel is timed in (last_phase_ms index)
+    struct Field { uint32_t BchipSchedule::*f; std::vector<uint32_t> cand; int phase; };
+    const std::vector<Field> fields = {
+        {&BchipSchedule::modifiers_block, {64, 128, 256}, 0},
+        {&BchipSchedule::diffusion_tiles, {1, 2, 4}, 1},
+        {&BchipSchedule::pre_grid, {1024, 4096, 16384, 65536}, 2},
+        {&BchipSchedule::plast_block, {64, 128, 256}, 4},
+        {&BchipSchedule::rrp_block, {64, 128, 256}, 5},
+    };
+    int ran = 0;
+    const int timing = S.timing;
+    try {
+        (void)hipSetDevice(S.device);
+        S.timing = 1;
+        for (const Field& fd : fields) {
+            if (fd.phase == 1 && (!S.d.has_conn || !S.class_ntiles[0])) continue;
+            if (fd.phase >= 2 && !S.d.ns) continue;
+            uint32_t best = S.sched.*(fd.f);
+            float best_ms = 1e30f;
+            for (uint32_t c : fd.cand) {
+                S.sched.*(fd.f) = c;
+                float acc = 0.0f;
+                for (uint32_t t = 0; t < ticks_per_candidate; t++) {
+                    bc::run_tick(S, dt);
+                    acc += S.last_ms[fd.phase];
+                    ran++;
+                }
+                if (acc < best_ms) { best_ms = acc; best = c; }
+            }
+            S.sched.*(fd.f) = best;
+        }
+    } catch (...) {
+        S.timing = timing;
+        return -1;
+    }
+    S.timing = timing;
+    return ran;
+}
+
+int bchip_set_graphs(void* sim, int on) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    S.use_graphs = on != 0;
+    if (!S.use_graphs) S.drop_graph();
     return 0;
 }
 

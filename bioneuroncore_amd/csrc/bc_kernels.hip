@@ -167,9 +167,8 @@ static_assert(sizeof(DiffLds<64>::ion) + sizeof(((DiffLds<64>*)nullptr)->u) >= 1
               "transmitter staging needs 18 floats per segment in ion + u");
 
 // Tiles per workgroup: 64-lane tiles are single wavefronts that synchronise at wavefront level,
-// so several share a workgroup (fewer workgroups per CU for the same resident wavefronts).
-constexpr int kDiffTiles64 = 1;
-__host__ __device__ constexpr int diff_tiles_per_block(int ts) { return ts == 64 ? kDiffTiles64 : 1; }
+// so up to four can share a workgroup (blockDim.x / 64 of them: a launch-schedule choice).
+constexpr int kDiffMaxTiles64 = 4;
 
 // Barrier of one tile. A 64-lane tile is one wavefront, whose LDS accesses complete in issue
 // order: a wavefront-scope fence pair around a wave barrier orders them for the compiler.
@@ -210,12 +209,12 @@ __device__ static inline BPair bsel(int k, const BPair& b1, const BPair& b2) {
 }
 
 template <int TS>
-__global__ __launch_bounds__(TS * diff_tiles_per_block(TS)) void k_diffusion(Net d, TileSet ts, float dt) {
+__global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffusion(Net d, TileSet ts, float dt) {
     // 64-lane tiles: each wavefront of the workgroup owns its own tile and LDS slice.
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t sub = TS == 64 ? threadIdx.x >> 6 : 0;
     DiffLds<TS>& L = *reinterpret_cast<DiffLds<TS>*>(lds_raw + sub * sizeof(DiffLds<TS>));
-    const uint32_t tile = blockIdx.x * diff_tiles_per_block(TS) + sub;
+    const uint32_t tile = blockIdx.x * (TS == 64 ? blockDim.x >> 6 : 1) + sub;
     if (tile >= ts.count) return;   // whole wavefront: no workgroup barrier follows for TS = 64
     const uint32_t s0 = ts.seg0[tile];
     const uint32_t nseg = ts.nseg[tile];

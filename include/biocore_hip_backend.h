@@ -81,6 +81,38 @@ int bchip_plugin_load(const char* path);                       // dlopen + bioco
 int bchip_registered_modifier(uint8_t type);
 int bchip_modifier_supported(uint8_t type);
 
+// ---- batches: independent networks advanced together by one tick launch sequence.
+// The networks are copied into one device state (a disjoint union; every record keeps its own
+// network's values, including Segment.id and Synapse.id); network k's results equal a
+// separate load of network k bit for bit. A plain load is a batch of one.
+int bchip_load_batch(void* sim, const BionData* const* nets, uint32_t count);
+uint32_t bchip_network_count(void* sim);
+// Stimuli of network `net` (arrays copied, no size limit, Segment.id matched within that
+// network), replacing its previous ones; injections apply on the next tick only, as the CPU
+// tick's one-shot injections. gpu_backend_upload_stimuli sets those of network 0.
+int bchip_set_stimuli(void* sim, uint32_t net, const CurrentStimulus* cur, uint32_t ncur,
+                      const RampStimulus* ramp, uint32_t nramp, const VoltageClamp* vc, uint32_t nvc,
+                      const IonInjection* ion, uint32_t nion, const LigandInjection* lig, uint32_t nlig);
+// Copies network `net`'s current records into caller arrays sized for that network (its own
+// segment indices in Synapse pre/post), and its spike counts (may be NULL).
+int bchip_batch_read(void* sim, uint32_t net, Segment* segs, Synapse* syns, uint32_t* spikes);
+
+// ---- launch schedule: workgroup shapes of the tick's kernels. Results do not depend on it.
+typedef struct BchipSchedule {
+    uint32_t modifiers_block;   // step-1 workgroup size: 64, 128 or 256
+    uint32_t diffusion_tiles;   // 64-segment tiles per step-2.5 workgroup: 1, 2 or 4
+    uint32_t pre_grid;          // workgroups of the grid-stride pre-side pass
+    uint32_t plast_block;       // plasticity-pass workgroup size: 64, 128 or 256
+    uint32_t rrp_block;         // vesicle-pool-pass workgroup size: 64, 128 or 256
+} BchipSchedule;
+int bchip_get_schedule(void* sim, BchipSchedule* out);
+int bchip_set_schedule(void* sim, const BchipSchedule* in);   // -1 if a field is out of range
+// Times each candidate of each schedule field on ticks_per_candidate live ticks (coordinate
+// descent; the ticks advance the simulation) and keeps the fastest. Returns the ticks run.
+int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate);
+// bchip_run_ticks replays a captured hipGraph of up to 16 ticks in FAST coherence (on by default).
+int bchip_set_graphs(void* sim, int on);
+
 #ifdef __cplusplus
 }
 // C++ registration entry points (plugin_manager.cpp:49-75).

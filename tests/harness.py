@@ -30,7 +30,8 @@ class Stimuli:
     """Stimulus arrays in the reference's core_run_tick form (pointer + mutable count)."""
 
     def __init__(self, current=None, vclamp=None, ramp=None, ion=None, lig=None):
-        cap = 256
+        # the CPU tick takes arrays of any length (the 256 cap belongs to GpuStimulusData)
+        cap = max([256] + [len(a) for a in (current, vclamp, ramp, ion, lig) if a is not None])
         self.cur = np.zeros(cap, bion.CURRENT_STIM_DT)
         self.vc = np.zeros(cap, bion.VCLAMP_DT)
         self.rp = np.zeros(cap, bion.RAMP_DT)
