@@ -146,6 +146,60 @@ def cpu_baseline(budget_s=12.0, neurons=250):
                       f"{k} ticks in {el:.1f}s, same builder and stimuli as the GPU workload"}
 
 
+def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004):
+    """The batched sweep: n_networks networks built like the 64-neuron chain network, segment
+    counts log-uniform in 2k-200k, each with its own seed and g_max scaling and a 200 pA soma
+    clamp. Plain: one simulation per network, ticked one network at a time (each tick is that
+    network's own launch sequence and synchronisation). Batched: all networks in one
+    simulation, autotuned schedule, ticks replayed from a captured hipGraph. Both are bit-exact
+    to the per-network CPU tick; throughput counts segment-ticks of all networks."""
+    neurons = bion.config4_neurons(n_networks, seed)
+    t0 = time.time()
+    nets = [bion.config4_network(k, int(n), seed) for k, n in enumerate(neurons)]
+    stims = [bion.soma_current(net, 200.0) for net in nets]
+    n_seg = int(sum(len(n.segments) for n in nets))
+    n_syn = int(sum(len(n.synapses) for n in nets))
+    print(f"# batch: {n_networks} networks, {n_seg} segments, {n_syn} synapses built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    sims = []
+    for net, st in zip(nets, stims):
+        s = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
+        if s.load_network(net) != 0:
+            raise SystemExit("plain load failed")
+        s.set_stimuli(0, current=st)
+        s.set_graphs(False)
+        sims.append(s)
+    for s in sims:
+        for _ in range(2):
+            s.run_tick(dt)
+    t0 = time.perf_counter()
+    for _ in range(ticks):
+        for s in sims:
+            s.run_tick(dt)
+    plain = n_seg * ticks / (time.perf_counter() - t0)
+    for s in sims:
+        s.close()
+    del sims
+    b = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
+    t0 = time.time()
+    if b.load_batch(nets) != 0:
+        raise SystemExit("batch load failed")
+    print(f"# batch loaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    del nets
+    for k, st in enumerate(stims):
+        b.set_stimuli(k, current=st)
+    tuned = b.autotune(dt, 2)
+    b.run_ticks(dt, 16)
+    t0 = time.perf_counter()
+    b.run_ticks(dt, ticks)
+    batched = n_seg * ticks / (time.perf_counter() - t0)
+    sched = b.schedule().as_dict()
+    b.close()
+    return {"networks": n_networks, "segments": n_seg, "synapses": n_syn, "ticks": ticks,
+            "plain_segment_ticks_per_s": plain, "batched_segment_ticks_per_s": batched,
+            "batched_vs_plain": batched / plain, "autotune_ticks": tuned, "schedule": sched}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +207,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--neurons", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-networks", type=int, default=64,
+                    help="networks in the batched-vs-plain comparison (512 = the full batched sweep)")
+    ap.add_argument("--batch-ticks", type=int, default=64)
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 1 GiB device-to-device copies (the reference kernel for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -167,9 +225,12 @@ def main():
         print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs one GPU; multi-GPU sharding is described in DESIGN.md, "
-                                   "not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; runs on more than one are not launched",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.batch_only:
+        print(json.dumps(batched_vs_plain(args.batch_networks, args.batch_ticks)), flush=True)
+        return
     t0 = time.time()
     net, stims = build_workload(args.neurons, args.warmup + args.steps)
     print(f"# built {len(net.segments)} segments, {len(net.synapses)} synapses in {time.time() - t0:.1f}s",
@@ -220,6 +281,8 @@ def main():
                           "unit": "GB/s", "frac": tick_gbs / HBM_PEAK_GBS},
         "phase_ms": {PHASES[k]: float(mean[k]) for k in range(len(PHASES))},
     }
+    if args.batch_networks > 0:
+        out["batched"] = batched_vs_plain(args.batch_networks, args.batch_ticks)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -321,6 +321,16 @@ def config4_network(k: int, n_neurons: int, seed: int = 0xB10C0004) -> Network:
     return net
 
 
+def soma_current(net: Network, amp_pA: float) -> np.ndarray:
+    """A constant current clamp on every soma for the whole run."""
+    somas = np.nonzero(net.segments["segment_type"] == 1)[0]
+    cur = np.zeros(len(somas), CURRENT_STIM_DT)
+    cur["segment_id"] = net.segments["id"][somas]
+    cur["amplitude_pA"] = amp_pA
+    cur["duration_ms"] = 1e9
+    return cur
+
+
 CONFIG2_TYPES = [NAV12, NAV16, KV11, KV31, KV7, CAV21, HCN1, SK, BK, K2P, PUMP, NALCN]
 
 
