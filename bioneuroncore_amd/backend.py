@@ -43,7 +43,7 @@ EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence",
                "bchip_set_phase_timing", "bchip_stream", "bchip_plugin_load", "bchip_registered_modifier",
                "bchip_modifier_supported", "bchip_load_batch", "bchip_network_count", "bchip_set_stimuli",
                "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
-               "bchip_set_graphs")
+               "bchip_set_graphs", "bchip_set_jacobi")
 
 
 class BionDataC(C.Structure):
@@ -118,6 +118,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_set_schedule.argtypes = [P, C.POINTER(BchipSchedule)]
     lib.bchip_autotune.argtypes = [P, C.c_float, C.c_uint32]
     lib.bchip_set_graphs.argtypes = [P, C.c_int]
+    lib.bchip_set_jacobi.argtypes = [P, C.c_double, C.c_int]
     _lib = lib
     return lib
 
@@ -238,6 +239,11 @@ class Simulation:
         if r < 0:
             raise RuntimeError("bchip_autotune failed")
         return r
+
+    def set_jacobi(self, tol: float, max_sweeps: int = 200) -> None:
+        """Residual-terminated Jacobi (tol <= 0: the reference's fixed 50 sweeps)."""
+        if self.lib.bchip_set_jacobi(self.h, tol, max_sweeps) != 0:
+            raise ValueError("bad Jacobi settings")
 
     def set_graphs(self, on: bool) -> None:
         self.lib.bchip_set_graphs(self.h, 1 if on else 0)

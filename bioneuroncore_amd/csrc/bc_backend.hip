@@ -239,6 +239,9 @@ struct Sim {
     float graph_dt = 0.0f;
     BchipSchedule graph_sched{};
     bool use_graphs = true;
+    bool jac_res = false;   // residual-terminated Jacobi (bchip_set_jacobi)
+    double jac_tol = 0.0;
+    int jac_max = 50;
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -391,6 +394,11 @@ static int build_device(Sim& S) {
     S.ntiles = (uint32_t)tile_seg0.size();
     for (uint32_t t = 0; t < S.ntiles; t++)
         for (uint32_t k = 0; k < tile_nseg[t]; k++) seg_tile[tile_seg0[t] + k] = t;
+    std::vector<uint8_t> isl_lane0(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t di = S.host2dev[i];
+        isl_lane0[di] = (uint8_t)(comp_start[root[i]] - tile_seg0[seg_tile[di]]);
+    }
     tile_conn0.resize(S.ntiles);
     {
         uint32_t acc = 0;
@@ -670,6 +678,9 @@ static int build_device(Sim& S) {
             S.tiles[k] = TileSet{ts0 + o, tns + o, tc0 + o, tnc + o, tmi + o, S.class_ntiles[k]};
         }
     }
+    d.isl_lane0 = S.dupload(isl_lane0);
+    d.jac_tol = S.jac_res ? S.jac_tol : 0.0;
+    d.jac_max = S.jac_res ? S.jac_max : 50;
     d.conn_ab = S.dupload(conn_ab);
     d.conn_intra = S.dupload(conn_intra);
     d.csr_off = S.dupload(csr_off);
@@ -933,6 +944,21 @@ static void stim_after_tick(Sim& S) {
 }
 
 // ------------------------------------------------------------------ tick
+template <bool kRes>
+static void launch_diffusion(Sim& S, float dt) {
+    const Net& d = S.d;
+    auto lds = [](size_t tile, uint32_t ts, uint32_t w) { return w * (tile + (kRes ? 2 * ts * 8 : 0)); };
+    if (S.class_ntiles[0]) {
+        const uint32_t W = S.sched.diffusion_tiles;
+        k_diffusion<64, kRes><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, lds(sizeof(DiffLds<64>), 64, W), S.stream>>>(
+            d, S.tiles[0], dt);
+    }
+    if (S.class_ntiles[1])
+        k_diffusion<128, kRes><<<S.class_ntiles[1], 128, lds(sizeof(DiffLds<128>), 128, 1), S.stream>>>(d, S.tiles[1], dt);
+    if (S.class_ntiles[2])
+        k_diffusion<256, kRes><<<S.class_ntiles[2], 256, lds(sizeof(DiffLds<256>), 256, 1), S.stream>>>(d, S.tiles[2], dt);
+}
+
 // The kernel sequence of one tick on S.stream (no host synchronisation or copies, so it can be
 // captured). With `tm`, events bracket the phases.
 static void launch_tick(Sim& S, float dt, bool tm) {
@@ -943,13 +969,8 @@ static void launch_tick(Sim& S, float dt, bool tm) {
     k_modifiers<<<(d.n + sc.modifiers_block - 1) / sc.modifiers_block, sc.modifiers_block, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
-        if (S.class_ntiles[0]) {
-            const uint32_t W = sc.diffusion_tiles;
-            k_diffusion<64><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, W * sizeof(DiffLds<64>), S.stream>>>(
-                d, S.tiles[0], dt);
-        }
-        if (S.class_ntiles[1]) k_diffusion<128><<<S.class_ntiles[1], 128, sizeof(DiffLds<128>), S.stream>>>(d, S.tiles[1], dt);
-        if (S.class_ntiles[2]) k_diffusion<256><<<S.class_ntiles[2], 256, sizeof(DiffLds<256>), S.stream>>>(d, S.tiles[2], dt);
+        if (S.jac_res) launch_diffusion<true>(S, dt);
+        else launch_diffusion<false>(S, dt);
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
@@ -1459,6 +1480,18 @@ int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate) {
     }
     S.timing = timing;
     return ran;
+}
+
+int bchip_set_jacobi(void* sim, double tol, int max_sweeps) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (tol > 0.0 && (max_sweeps < 1 || max_sweeps > 100000)) return -1;
+    S.jac_res = tol > 0.0;
+    S.jac_tol = S.jac_res ? tol : 0.0;
+    S.jac_max = S.jac_res ? max_sweeps : 50;
+    S.d.jac_tol = S.jac_tol;
+    S.d.jac_max = S.jac_max;
+    return 0;
 }
 
 int bchip_set_graphs(void* sim, int on) {

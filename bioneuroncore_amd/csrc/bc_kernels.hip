@@ -208,12 +208,25 @@ __device__ static inline BPair bsel(int k, const BPair& b1, const BPair& b2) {
     return b1;
 }
 
+// All lanes of a tile: pred holds on every lane (a barrier for multi-wavefront tiles).
 template <int TS>
+__device__ __attribute__((always_inline)) inline bool tile_all(bool pred) {
+    if (TS == 64) return __ballot(!pred) == 0;
+    return __syncthreads_and(pred) != 0;
+}
+
+// kRes: residual-terminated sweeps (an extension; the reference sweeps 50 times). Each island
+// (cell) stops after the first sweep whose largest update is <= d.jac_tol, or after d.jac_max
+// sweeps; the islands' residuals are max-reduced in LDS beyond the tile's DiffLds.
+template <int TS, bool kRes>
 __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffusion(Net d, TileSet ts, float dt) {
     // 64-lane tiles: each wavefront of the workgroup owns its own tile and LDS slice.
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t sub = TS == 64 ? threadIdx.x >> 6 : 0;
     DiffLds<TS>& L = *reinterpret_cast<DiffLds<TS>*>(lds_raw + sub * sizeof(DiffLds<TS>));
+    unsigned long long* ires = nullptr;   // [2][TS]: residual (double bits) per island, by first lane
+    if (kRes)
+        ires = reinterpret_cast<unsigned long long*>(lds_raw + (blockDim.x / TS) * sizeof(DiffLds<TS>)) + sub * 2 * TS;
     const uint32_t tile = blockIdx.x * (TS == 64 ? blockDim.x >> 6 : 1) + sub;
     if (tile >= ts.count) return;   // whole wavefront: no workgroup barrier follows for TS = 64
     const uint32_t s0 = ts.seg0[tile];
@@ -315,6 +328,41 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 }
             }
         const double ydiag = rcp_prep(diag);   // diag is fixed across the sweeps
+        if constexpr (kRes) {
+            const uint32_t il = act ? d.isl_lane0[g] : t;
+            ires[t] = 0;
+            ires[TS + t] = 0;
+            tile_sync<TS>();
+            bool done = !act;
+            for (int it = 0; it < d.jac_max; it++) {
+                const int rb = it & 1;
+                if (!done) {
+                    double pn = 0.0;
+#pragma unroll
+                    for (int q = 0; q < D; q++) pn += pg[q] * L.u.p.phi[cur][po[q]];
+                    for (uint32_t e = pe_rest; e < e1; e++) {
+                        uint32_t en = L.ent[e];
+                        double gg = gphi[ent_conn(en)];
+                        if (gg <= 0.0) continue;
+                        pn += gg * L.u.p.phi[cur][ent_other(L, en)];
+                    }
+                    double ph = (diag > 1e-9) ? div_rcp(pn - ibuf, diag, ydiag) : 0.0;
+                    if (ph > 10.0) ph = 10.0;
+                    if (ph < -10.0) ph = -10.0;
+                    L.u.p.phi[cur ^ 1][t] = ph;
+                    atomicMax(&ires[rb * TS + il], (unsigned long long)__double_as_longlong(fabs(ph - L.u.p.phi[cur][t])));
+                } else if (act) {
+                    L.u.p.phi[cur ^ 1][t] = L.u.p.phi[cur][t];
+                }
+                tile_sync<TS>();
+                if (!done) done = __longlong_as_double((long long)ires[rb * TS + il]) <= d.jac_tol;
+                ires[(rb ^ 1) * TS + t] = 0;
+                cur ^= 1;
+                if (tile_all<TS>(done)) break;
+            }
+            tile_sync<TS>();
+            return;
+        }
         if constexpr (TS == 64) {
             if (maxinc <= 4) {   // registers + ds_bpermute, as the Jacobi sweeps below
                 int pa[D];
@@ -471,6 +519,41 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     jdeg++;
                     if (jdeg == D) je_rest = e + 1;
                 }
+            if constexpr (kRes) {   // residual-terminated, as the phi sweeps
+                const uint32_t il = act ? d.isl_lane0[g] : t;
+                ires[t] = 0;
+                ires[TS + t] = 0;
+                tile_sync<TS>();
+                bool done = !act;
+                int vc = 0;
+                for (int it = 0; it < d.jac_max; it++) {
+                    const int rb = it & 1;
+                    if (!done) {
+                        float off = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < D; q++) off += jg[q] * L.u.i.vf[vc][jo[q]];
+                        for (uint32_t e = je_rest; e < e1; e++) {
+                            uint32_t en = L.ent[e];
+                            uint32_t c = ent_conn(en);
+                            if (!ent_intra(en)) continue;
+                            off += L.u.i.cg[c] * L.u.i.vf[vc][ent_other(L, en)];
+                        }
+                        const float v = div_wide(base + off, den, rden);
+                        L.u.i.vf[vc ^ 1][t] = v;
+                        atomicMax(&ires[rb * TS + il], (unsigned long long)__double_as_longlong((double)fabsf(v - vnew)));
+                        vnew = v;
+                    } else if (act) {
+                        L.u.i.vf[vc ^ 1][t] = vnew;
+                    }
+                    tile_sync<TS>();
+                    if (!done) done = __longlong_as_double((long long)ires[rb * TS + il]) <= d.jac_tol;
+                    ires[(rb ^ 1) * TS + t] = 0;
+                    vc ^= 1;
+                    if (tile_all<TS>(done)) break;
+                }
+                tile_sync<TS>();
+                return;
+            }
             if constexpr (TS == 64) {
                 if (maxinc <= 4) {
                     // The tile is one wavefront and every incidence has a register slot: the

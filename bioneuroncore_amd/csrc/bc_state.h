@@ -91,6 +91,10 @@ struct Net {
     // dead; intra_nt stays +0.0 (and its diffusion an exact no-op) until something sources it.
     uint32_t nt_ex_live;
     uint32_t nt_in_live;
+    // Residual-terminated Jacobi (an extension; off: the reference's fixed 50 sweeps)
+    double jac_tol;
+    int32_t jac_max;
+    const uint8_t* isl_lane0;   // per segment: tile-local index of its island's first segment
 
     // segment dynamic state
     double* V;

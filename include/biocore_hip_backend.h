@@ -112,6 +112,10 @@ int bchip_set_schedule(void* sim, const BchipSchedule* in);   // -1 if a field i
 int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate);
 // bchip_run_ticks replays a captured hipGraph of up to 16 ticks in FAST coherence (on by default).
 int bchip_set_graphs(void* sim, int on);
+// Residual-terminated Jacobi for both potential solves of step 2.5 (an extension; the reference
+// always sweeps 50 times): each connected cell stops after the first sweep whose largest update
+// is <= tol (mV), or after max_sweeps. tol <= 0 restores the reference's fixed 50 sweeps.
+int bchip_set_jacobi(void* sim, double tol, int max_sweeps);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
 // synapses without presynaptic autoreceptors; no glia (GLAST), gap junctions, consolidation,
 // mechano stimuli or plugins. Anything outside that set is reported by
 // oracle_unsupported_count() and computed as the reference's "unknown ID" default (0 current).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -650,7 +651,49 @@ struct Ctx {
     uint32_t delay_syn = UINT32_MAX;
     std::vector<uint32_t> spikes;
     int substeps_override = 0;
+    // Residual-terminated Jacobi (an extension, not in the reference, which always sweeps 50
+    // times): each connected cell stops sweeping after the first sweep whose largest update is
+    // <= jac_tol, or after jac_max sweeps.
+    bool jac_on = false;
+    double jac_tol = 0.0;
+    int jac_max = 50;
+    std::vector<uint32_t> comp;   // cell (component over all connections) of each segment
+    uint32_t ncomp = 0;
 };
+
+static uint32_t uf_root(std::vector<uint32_t>& p, uint32_t x) {
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+}
+static void cells(Ctx& c, uint32_t n, const IntraConnection* cn, uint32_t nc) {
+    std::vector<uint32_t> p(n);
+    for (uint32_t i = 0; i < n; i++) p[i] = i;
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
+        if (a >= n || b >= n) continue;
+        a = uf_root(p, a);
+        b = uf_root(p, b);
+        if (a != b) p[a > b ? a : b] = a < b ? a : b;
+    }
+    c.comp.assign(n, 0);
+    std::vector<uint32_t> id(n, UINT32_MAX);
+    c.ncomp = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t r = uf_root(p, i);
+        if (id[r] == UINT32_MAX) id[r] = c.ncomp++;
+        c.comp[i] = id[r];
+    }
+}
+// After a sweep: cells whose largest update was <= tol are done. Returns true when all are.
+static bool cells_settle(Ctx& c, std::vector<double>& res, std::vector<uint8_t>& done) {
+    bool all = true;
+    for (uint32_t k = 0; k < c.ncomp; k++) {
+        if (!done[k] && res[k] <= c.jac_tol) done[k] = 1;
+        if (!done[k]) all = false;
+        res[k] = 0.0;
+    }
+    return all;
+}
 
 static void solve_phi(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint32_t nc) {
     c.phi.assign(n, 0.0);
@@ -678,7 +721,9 @@ static void solve_phi(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
         c.diag[a] += g;
         c.diag[b] += g;
     }
-    for (int it = 0; it < 50; it++) {
+    std::vector<double> res(c.jac_on ? c.ncomp : 0, 0.0);
+    std::vector<uint8_t> done(c.jac_on ? c.ncomp : 0, 0);
+    for (int it = 0; it < (c.jac_on ? c.jac_max : 50); it++) {
         c.phi_new.assign(n, 0.0);
         for (uint32_t i = 0; i < nc; i++) {
             uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
@@ -689,10 +734,14 @@ static void solve_phi(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
             c.phi_new[b] += g * c.phi[a];
         }
         for (uint32_t i = 0; i < n; i++) {
-            c.phi[i] = (c.diag[i] > 1e-9) ? (c.phi_new[i] - sg[i].I_buffer) / c.diag[i] : 0.0;
-            if (c.phi[i] > 10.0) c.phi[i] = 10.0;
-            if (c.phi[i] < -10.0) c.phi[i] = -10.0;
+            if (c.jac_on && done[c.comp[i]]) continue;
+            double v = (c.diag[i] > 1e-9) ? (c.phi_new[i] - sg[i].I_buffer) / c.diag[i] : 0.0;
+            if (v > 10.0) v = 10.0;
+            if (v < -10.0) v = -10.0;
+            if (c.jac_on) res[c.comp[i]] = std::max(res[c.comp[i]], fabs(v - c.phi[i]));
+            c.phi[i] = v;
         }
+        if (c.jac_on && cells_settle(c, res, done)) break;
     }
 }
 
@@ -726,6 +775,7 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
     c.vold.assign(n, 0.0f);
     c.vnew.assign(n, 0.0f);
     c.off.assign(n, 0.0f);
+    if (c.jac_on) cells(c, n, cn, nc);
     solve_phi(c, sg, n, cn, nc);
     const std::vector<double>& phi = c.phi;
 
@@ -763,7 +813,9 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
             c.inpd[b] -= inp - gax * dV;
         }
         for (uint32_t i = 0; i < n; i++) { c.vold[i] = sg[i].V_m; c.vnew[i] = sg[i].V_m; }
-        for (int it = 0; it < 50; it++) {
+        std::vector<double> res(c.jac_on ? c.ncomp : 0, 0.0);
+        std::vector<uint8_t> done(c.jac_on ? c.ncomp : 0, 0);
+        for (int it = 0; it < (c.jac_on ? c.jac_max : 50); it++) {
             std::fill(c.off.begin(), c.off.end(), 0.0f);
             for (uint32_t i = 0; i < nc; i++) {
                 uint32_t a = cn[i].segment_a, b = cn[i].segment_b;
@@ -773,10 +825,14 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
                 c.off[b] += c.goff[i] * c.vnew[a];
             }
             for (uint32_t i = 0; i < n; i++) {
+                if (c.jac_on && done[c.comp[i]]) continue;
                 float cdt = capacitance(&sg[i]) / dts;
                 if (cdt < 1e-9f) cdt = 1e-9f;
-                c.vnew[i] = (cdt * c.vold[i] + c.inpd[i] + c.off[i]) / (cdt + c.gdiag[i]);
+                float v = (cdt * c.vold[i] + c.inpd[i] + c.off[i]) / (cdt + c.gdiag[i]);
+                if (c.jac_on) res[c.comp[i]] = std::max(res[c.comp[i]], (double)fabsf(v - c.vnew[i]));
+                c.vnew[i] = v;
             }
+            if (c.jac_on && cells_settle(c, res, done)) break;
         }
         for (uint32_t i = 0; i < n; i++) sg[i].V_m = c.vnew[i];
         for (uint32_t i = 0; i < n; i++)
@@ -1184,6 +1240,154 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
 
 }  // namespace orc
 
+// ---- FP64 restatement of the tick of passive networks (no modifiers, synapses or stimuli):
+// step 2.5 (include/diffusion_kernel.h:41-364) and step 4 (src/simulation_host.cpp:596-649)
+// with every quantity in double, for error reports of the reference's mixed precision. With no
+// modifiers I_buffer is 0, so step 4 only clamps V_m; transmitters stay 0.
+namespace orc {
+struct F64 {
+    uint32_t n = 0, nc = 0;
+    std::vector<double> V, in, ex, atp, temp, txn, area, vol, cap, rad, len;
+    std::vector<uint32_t> ca, cb;
+    std::vector<uint8_t> intra;
+    Ctx cells;   // residual rule and cell ids
+};
+static inline double bern64(double x) {
+    if (fabs(x) < 1e-3) return 1.0 - 0.5 * x;
+    if (x > 50.0) return 0.0;
+    if (x < -50.0) return -x;
+    return x / (exp(x) - 1.0);
+}
+static void f64_tick(F64& s, double dt) {
+    const uint32_t n = s.n, nc = s.nc;
+    Ctx& c = s.cells;
+    const double pi = (double)kPi, evr = (double)kEvr;
+    if (nc > 0) {
+        std::vector<double> sigma(n), diag(n), phi(n, 0.0), acc(n), g(nc), ac(nc), dist(nc);
+        for (uint32_t i = 0; i < n; i++) {
+            double sm = 0.0;
+            for (int k = 0; k < 7; k++) sm += (double)kZ[k] * kZ[k] * kDiff[k] * s.ex[(size_t)i * 7 + k];
+            sigma[i] = (double)kSigmaE * sm;
+            diag[i] = sigma[i] * s.area[i];
+        }
+        for (uint32_t e = 0; e < nc; e++) {
+            uint32_t a = s.ca[e], b = s.cb[e];
+            double rmin = std::min(s.rad[a], s.rad[b]);
+            ac[e] = std::max(pi * rmin * rmin, 1e-6);
+            dist[e] = std::max((s.len[a] + s.len[b]) * 0.5, 0.1);
+            g[e] = 0.5 * (sigma[a] + sigma[b]) * ac[e] / dist[e];
+            diag[a] += g[e];
+            diag[b] += g[e];
+        }
+        std::vector<double> res(c.jac_on ? c.ncomp : 0, 0.0);
+        std::vector<uint8_t> done(c.jac_on ? c.ncomp : 0, 0);
+        for (int it = 0; it < (c.jac_on ? c.jac_max : 50); it++) {
+            std::fill(acc.begin(), acc.end(), 0.0);
+            for (uint32_t e = 0; e < nc; e++) {
+                if (g[e] <= 0.0) continue;
+                acc[s.ca[e]] += g[e] * phi[s.cb[e]];
+                acc[s.cb[e]] += g[e] * phi[s.ca[e]];
+            }
+            for (uint32_t i = 0; i < n; i++) {
+                if (c.jac_on && done[c.comp[i]]) continue;
+                double v = diag[i] > 1e-9 ? acc[i] / diag[i] : 0.0;
+                v = std::min(10.0, std::max(-10.0, v));
+                if (c.jac_on) res[c.comp[i]] = std::max(res[c.comp[i]], fabs(v - phi[i]));
+                phi[i] = v;
+            }
+            if (c.jac_on && cells_settle(c, res, done)) break;
+        }
+        const double dts = dt / 5.0;
+        std::vector<double> dacc((size_t)n * 7), gdiag(n), inpd(n), gax(nc), vold(n), vnew(n), off(n);
+        for (int sub = 0; sub < 5; sub++) {
+            std::fill(dacc.begin(), dacc.end(), 0.0);
+            std::fill(gdiag.begin(), gdiag.end(), 0.0);
+            std::fill(inpd.begin(), inpd.end(), 0.0);
+            for (uint32_t e = 0; e < nc; e++) {
+                if (!s.intra[e]) continue;
+                uint32_t a = s.ca[e], b = s.cb[e];
+                double sg = 0.0;
+                for (int k = 0; k < 7; k++)
+                    sg += (double)kZ[k] * kZ[k] * kDiff[k] * 0.5 * (s.in[(size_t)a * 7 + k] + s.in[(size_t)b * 7 + k]);
+                gax[e] = (double)kSigmaE * sg * ac[e] / dist[e];
+                gdiag[a] += gax[e];
+                gdiag[b] += gax[e];
+                double dV = (s.V[a] + phi[a]) - (s.V[b] + phi[b]);
+                double inp = 0.0;
+                for (int k = 0; k < 7; k++) {
+                    double x = (double)kZ[k] * kFrtMv * dV;
+                    double fl = (double)kDiff[k] * ac[e] * dts / dist[e] *
+                                (s.in[(size_t)a * 7 + k] * bern64(-x) - s.in[(size_t)b * 7 + k] * bern64(x));
+                    dacc[(size_t)a * 7 + k] -= fl / std::max(s.vol[a], 1e-6);
+                    dacc[(size_t)b * 7 + k] += fl / std::max(s.vol[b], 1e-6);
+                    inp += (double)kZ[k] * kChargeToPa * fl / dts;
+                }
+                inpd[a] += inp - gax[e] * dV;
+                inpd[b] -= inp - gax[e] * dV;
+            }
+            vold = s.V;
+            vnew = s.V;
+            std::fill(res.begin(), res.end(), 0.0);
+            std::fill(done.begin(), done.end(), 0);
+            for (int it = 0; it < (c.jac_on ? c.jac_max : 50); it++) {
+                std::fill(off.begin(), off.end(), 0.0);
+                for (uint32_t e = 0; e < nc; e++) {
+                    if (!s.intra[e]) continue;
+                    off[s.ca[e]] += gax[e] * vnew[s.cb[e]];
+                    off[s.cb[e]] += gax[e] * vnew[s.ca[e]];
+                }
+                for (uint32_t i = 0; i < n; i++) {
+                    if (c.jac_on && done[c.comp[i]]) continue;
+                    double cdt = std::max(s.cap[i] / dts, 1e-9);
+                    double v = (cdt * vold[i] + inpd[i] + off[i]) / (cdt + gdiag[i]);
+                    if (c.jac_on) res[c.comp[i]] = std::max(res[c.comp[i]], fabs(v - vnew[i]));
+                    vnew[i] = v;
+                }
+                if (c.jac_on && cells_settle(c, res, done)) break;
+            }
+            s.V = vnew;
+            for (size_t q = 0; q < dacc.size(); q++) s.in[q] += dacc[q];
+        }
+        std::vector<double> dext((size_t)n * 7);
+        for (int sub = 0; sub < 5; sub++) {
+            std::fill(dext.begin(), dext.end(), 0.0);
+            for (uint32_t e = 0; e < nc; e++) {
+                uint32_t a = s.ca[e], b = s.cb[e];
+                for (int k = 0; k < 7; k++) {
+                    double x = (double)kZ[k] * kFrtMv * (phi[a] - phi[b]);
+                    double fl = (double)kDiff[k] * ac[e] * dts / dist[e] *
+                                (s.ex[(size_t)a * 7 + k] * bern64(-x) - s.ex[(size_t)b * 7 + k] * bern64(x));
+                    dext[(size_t)a * 7 + k] -= fl / (std::max(s.vol[a], 1e-6) * evr);
+                    dext[(size_t)b * 7 + k] += fl / (std::max(s.vol[b], 1e-6) * evr);
+                }
+            }
+            for (size_t q = 0; q < dext.size(); q++) s.ex[q] += dext[q];
+        }
+        std::vector<double> datp(n, 0.0), dtemp(n, 0.0), dtxn(n, 0.0);
+        for (uint32_t e = 0; e < nc; e++) {
+            if (!s.intra[e]) continue;
+            uint32_t a = s.ca[e], b = s.cb[e];
+            double va = std::max(s.vol[a], 1e-6), vb = std::max(s.vol[b], 1e-6);
+            double d1 = s.atp[a] - s.atp[b], d2 = s.temp[a] - s.temp[b], d3 = s.txn[a] - s.txn[b];
+            if (fabs(d1) > 1e-12) { double fl = 0.14 * ac[e] * d1 * dt / dist[e]; datp[a] -= fl / va; datp[b] += fl / vb; }
+            if (fabs(d2) > 0.001) { double fl = 0.0014 * ac[e] * d2 * dt / dist[e]; dtemp[a] -= fl / va; dtemp[b] += fl / vb; }
+            if (fabs(d3) > 1e-9) { double fl = 0.01 * ac[e] * d3 * dt / dist[e]; dtxn[a] -= fl / va; dtxn[b] += fl / vb; }
+        }
+        for (uint32_t i = 0; i < n; i++) {
+            for (int k = 0; k < 7; k++) {
+                double& x = s.ex[(size_t)i * 7 + k];
+                x += ((double)kBath[k] - x) * dt;
+                x = std::min(1000.0, std::max(1e-9, x));
+            }
+            s.atp[i] += datp[i];
+            s.temp[i] += dtemp[i];
+            s.txn[i] += dtxn[i];
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) s.V[i] = std::min(150.0, std::max(-120.0, s.V[i]));
+}
+}  // namespace orc
+
 extern "C" {
 
 void* oracle_create(void) { return new orc::Ctx(); }
@@ -1225,6 +1429,62 @@ double oracle_modifier_current(uint8_t type, Modifier* m, Segment* s, float dt) 
 }
 
 float oracle_xorshift(uint32_t seed) { return orc::xorshift_draw(seed); }
+
+// Residual-terminated Jacobi for both potential solves (tol <= 0: the reference's fixed 50).
+void oracle_set_jacobi(void* ctx, double tol, int max_sweeps) {
+    orc::Ctx* c = static_cast<orc::Ctx*>(ctx);
+    c->jac_on = tol > 0.0;
+    c->jac_tol = tol;
+    c->jac_max = max_sweeps > 0 ? max_sweeps : 50;
+}
+
+// FP64 restatement for passive networks (see orc::f64_tick).
+void* oracle_fp64_create(const Segment* sg, uint32_t n, const IntraConnection* cn, uint32_t nc) {
+    orc::F64* s = new orc::F64();
+    s->n = n;
+    s->nc = nc;
+    s->V.resize(n); s->in.resize((size_t)n * 7); s->ex.resize((size_t)n * 7);
+    s->atp.resize(n); s->temp.resize(n); s->txn.resize(n); s->area.resize(n); s->vol.resize(n);
+    s->cap.resize(n); s->rad.resize(n); s->len.resize(n);
+    const double pi = (double)orc::kPi;
+    for (uint32_t i = 0; i < n; i++) {
+        const Segment& x = sg[i];
+        const bool sph = orc::is_sphere(&x);
+        const double r = x.radius, l = x.length;
+        s->V[i] = x.V_m;
+        for (int k = 0; k < 7; k++) { s->in[(size_t)i * 7 + k] = x.intra_ions[k]; s->ex[(size_t)i * 7 + k] = x.extra_ions[k]; }
+        s->atp[i] = x.ATP_level; s->temp[i] = x.temperature; s->txn[i] = x.transcription_signal;
+        s->rad[i] = r; s->len[i] = l;
+        s->area[i] = sph ? 4.0 * pi * r * r : 2.0 * pi * r * l;
+        s->vol[i] = sph ? (4.0 / 3.0) * pi * r * r * r : pi * r * r * l;
+        s->cap[i] = std::max(s->area[i] * 0.01, 1e-4);
+    }
+    for (uint32_t e = 0; e < nc; e++) {
+        if (cn[e].segment_a >= n || cn[e].segment_b >= n) continue;
+        s->ca.push_back(cn[e].segment_a);
+        s->cb.push_back(cn[e].segment_b);
+        s->intra.push_back(cn[e].is_intracellular != 0);
+    }
+    s->nc = (uint32_t)s->ca.size();
+    orc::cells(s->cells, n, cn, nc);
+    return s;
+}
+void oracle_fp64_set_jacobi(void* h, double tol, int max_sweeps) {
+    oracle_set_jacobi(&static_cast<orc::F64*>(h)->cells, tol, max_sweeps);
+}
+void oracle_fp64_run_ticks(void* h, float dt, uint32_t ticks) {
+    for (uint32_t t = 0; t < ticks; t++) orc::f64_tick(*static_cast<orc::F64*>(h), dt);
+}
+// V (n), intra (n x 7), extra (n x 7), ATP, temperature (n each).
+void oracle_fp64_read(void* h, double* V, double* in, double* ex, double* atp, double* temp) {
+    const orc::F64& s = *static_cast<orc::F64*>(h);
+    memcpy(V, s.V.data(), s.n * 8);
+    memcpy(in, s.in.data(), s.in.size() * 8);
+    memcpy(ex, s.ex.data(), s.ex.size() * 8);
+    memcpy(atp, s.atp.data(), s.n * 8);
+    memcpy(temp, s.temp.data(), s.n * 8);
+}
+void oracle_fp64_destroy(void* h) { delete static_cast<orc::F64*>(h); }
 float oracle_bernoulli(float x) { return orc::bernoulli(x); }
 
 }  // extern "C"
