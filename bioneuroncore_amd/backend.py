@@ -43,7 +43,8 @@ EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence",
                "bchip_set_phase_timing", "bchip_stream", "bchip_plugin_load", "bchip_registered_modifier",
                "bchip_modifier_supported", "bchip_load_batch", "bchip_network_count", "bchip_set_stimuli",
                "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
-               "bchip_set_graphs", "bchip_set_jacobi")
+               "bchip_set_graphs", "bchip_set_jacobi", "bchip_set_cable_mode", "bchip_set_coupling_scale",
+               "bchip_adjoint")
 
 
 class BionDataC(C.Structure):
@@ -119,6 +120,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_autotune.argtypes = [P, C.c_float, C.c_uint32]
     lib.bchip_set_graphs.argtypes = [P, C.c_int]
     lib.bchip_set_jacobi.argtypes = [P, C.c_double, C.c_int]
+    lib.bchip_set_cable_mode.argtypes = [P, C.c_int]
+    lib.bchip_set_coupling_scale.argtypes = [P, P, C.c_uint32]
+    lib.bchip_adjoint.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P]
     _lib = lib
     return lib
 
@@ -244,6 +248,38 @@ class Simulation:
         """Residual-terminated Jacobi (tol <= 0: the reference's fixed 50 sweeps)."""
         if self.lib.bchip_set_jacobi(self.h, tol, max_sweeps) != 0:
             raise ValueError("bad Jacobi settings")
+
+    def set_cable_mode(self, on: bool) -> None:
+        if self.lib.bchip_set_cable_mode(self.h, 1 if on else 0) != 0:
+            raise ValueError("cable mode needs a network without synapses")
+
+    def set_coupling_scale(self, scale) -> None:
+        """Per-connection axial scale (None: 1, the reference)."""
+        if scale is None:
+            rc = self.lib.bchip_set_coupling_scale(self.h, None, 0)
+        else:
+            s = np.ascontiguousarray(scale, dtype=np.float32)
+            rc = self.lib.bchip_set_coupling_scale(self.h, s.ctypes.data, len(s))
+        if rc != 0:
+            raise ValueError("bad coupling scale")
+
+    def adjoint(self, dt: float, ticks: int, checkpoint_every: int, soma, target, grads: bool = True):
+        """Cable-mode loss and gradients: (loss, d loss/d density [n, 32], d loss/d coupling scale [nc]).
+        soma: host segment index per cell; target: (ticks, cells) V_m after each tick."""
+        soma = np.ascontiguousarray(soma, dtype=np.uint32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        assert target.shape == (ticks, len(soma))
+        loss = C.c_double(0.0)
+        b = self.bion_data()
+        n, nc = len(b.segments), len(b.connections)
+        gd = np.zeros((n, 32)) if grads else None
+        gs = np.zeros(max(nc, 1)) if grads else None
+        rc = self.lib.bchip_adjoint(self.h, dt, ticks, checkpoint_every, soma.ctypes.data, len(soma),
+                                    target.ctypes.data, C.byref(loss), gd.ctypes.data if grads else None,
+                                    gs.ctypes.data if grads else None)
+        if rc != 0:
+            raise RuntimeError(f"bchip_adjoint failed ({rc})")
+        return loss.value, gd, (gs[:nc] if grads else None)
 
     def set_graphs(self, on: bool) -> None:
         self.lib.bchip_set_graphs(self.h, 1 if on else 0)

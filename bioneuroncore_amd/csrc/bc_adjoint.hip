@@ -1,0 +1,538 @@
+// bc_adjoint.hip — reverse-mode derivative of the cable-mode tick (an extension; the reference
+// has no adjoint). The differentiated tick: step 1 with the gating ODEs and GHK currents of
+// Nav1.2 (142), Kv-DR (123), K2P (107), NALCN (133) and HCN1 (129) at held ion
+// concentrations; step 2.5 in cable mode (phi_e = 0, five substeps of axial SG flux + chord
+// term and 50 Jacobi sweeps of V); step 4. Parameters: each modifier's density (g_max) and each
+// connection's conductance scale. Local partial derivatives of the channels come from
+// restating them over dual numbers; the tick-level reverse pass (transposed Jacobi sweeps,
+// flux and chord terms, step-4 clamps) is written out below.
+#pragma once
+
+namespace bc {
+namespace adj {
+
+// ---- dual numbers: value + N partial derivatives (double) ----
+template <int N>
+struct Dn {
+    double v, d[N];
+};
+template <int N>
+BC_HD Dn<N> cst(double v) {
+    Dn<N> r;
+    r.v = v;
+    for (int k = 0; k < N; k++) r.d[k] = 0.0;
+    return r;
+}
+template <int N>
+BC_HD Dn<N> var(double v, int slot) {
+    Dn<N> r = cst<N>(v);
+    r.d[slot] = 1.0;
+    return r;
+}
+template <int N> BC_HD Dn<N> operator+(Dn<N> a, Dn<N> b) { for (int k = 0; k < N; k++) a.d[k] += b.d[k]; a.v += b.v; return a; }
+template <int N> BC_HD Dn<N> operator-(Dn<N> a, Dn<N> b) { for (int k = 0; k < N; k++) a.d[k] -= b.d[k]; a.v -= b.v; return a; }
+template <int N> BC_HD Dn<N> operator-(Dn<N> a) { for (int k = 0; k < N; k++) a.d[k] = -a.d[k]; a.v = -a.v; return a; }
+template <int N> BC_HD Dn<N> operator+(Dn<N> a, double b) { a.v += b; return a; }
+template <int N> BC_HD Dn<N> operator+(double b, Dn<N> a) { a.v += b; return a; }
+template <int N> BC_HD Dn<N> operator-(Dn<N> a, double b) { a.v -= b; return a; }
+template <int N> BC_HD Dn<N> operator-(double b, Dn<N> a) { return cst<N>(b) - a; }
+template <int N> BC_HD Dn<N> operator*(Dn<N> a, Dn<N> b) {
+    Dn<N> r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < N; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int N> BC_HD Dn<N> operator*(Dn<N> a, double b) { a.v *= b; for (int k = 0; k < N; k++) a.d[k] *= b; return a; }
+template <int N> BC_HD Dn<N> operator*(double b, Dn<N> a) { return a * b; }
+template <int N> BC_HD Dn<N> operator/(Dn<N> a, Dn<N> b) {
+    Dn<N> r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < N; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    return r;
+}
+template <int N> BC_HD Dn<N> operator/(Dn<N> a, double b) { return a * (1.0 / b); }
+template <int N> BC_HD Dn<N> operator/(double a, Dn<N> b) { return cst<N>(a) / b; }
+template <int N> BC_HD Dn<N> dexp(Dn<N> a) {
+    Dn<N> r;
+    r.v = bc::exp_t(a.v, ExpTabGlobal{});
+    for (int k = 0; k < N; k++) r.d[k] = r.v * a.d[k];
+    return r;
+}
+template <int N> BC_HD Dn<N> clamp01(Dn<N> a) {   // the gates' clamp; derivative 0 where it binds
+    if (a.v < 0.0) return cst<N>(0.0);
+    if (a.v > 1.0) return cst<N>(1.0);
+    return a;
+}
+
+// GHK flux (run_tick_common.h:274-312) of a monovalent / divalent ion at held concentrations.
+template <int N>
+BC_HD Dn<N> ghk(Dn<N> V, double ci, double co, bool di) {
+    const double rt = (double)kGas * kTemp;
+    Dn<N> vn = (di ? 2.0 : 1.0) * kFaraday * V * 0.001 / rt;
+    const double k = (di ? 4.0 : 1.0) * kFaraday / rt;
+    if (fabs(vn.v) < 1e-6) return cst<N>((di ? 2.0 : 1.0) * kFaraday * (ci - co));
+    if (vn.v > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn.v < -20.0) return k * kFaraday * V * 0.001 * co;
+    Dn<N> e = dexp(-vn);
+    return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
+}
+template <int N>
+BC_HD Dn<N> vtrapd(Dn<N> x, double y) {
+    if (fabs(x.v / y) < 1e-6) return y * (1.0 - x / (2.0 * y));
+    return x / (1.0 - dexp(-x / y));
+}
+
+// One channel at one segment: value and partials used by the reverse pass.
+struct ChanAd {
+    double r;          // returned current (the tick adds it to I_buffer)
+    double dr[3];      // d r / d (V, gate0', gate1')
+    double r1;         // d r / d density
+    double dg[2][2];   // d gate_q' / d V, d gate_q' / d gate_q
+    int ng;            // gates
+};
+struct SegAd {         // held segment quantities the channels read
+    double in[3], ex[3];
+    double area, pkc, pka, camp;
+};
+BC_HD bool adjoint_supported(uint8_t t) {
+    return t == M_NAV12 || t == M_KVDR || t == M_K2P || t == M_NALCN || t == M_HCN1;
+}
+
+// The current of each type as a function of (V, new gates), per unit density (x dens outside).
+// Each follows the corresponding forward function of bc_device.h (same formulas and branches).
+BC_HD Dn<3> current1(uint8_t t, Dn<3> V, Dn<3> g0, Dn<3> g1, const double* st, const SegAd& s) {
+    if (t == M_NAV12) {   // ch_nav with the Nav1.2 constants
+        Dn<3> op = g0 * g0 * g0 * g1;
+        double ttx = st[5] > 0.0 ? st[5] : 0.0;
+        op = op / (1.0 + ttx / 10.0);
+        op = op * (1.0 - 0.7 * (s.pkc + s.pka) * 0.5);
+        if (op.v < 0.0) op = cst<3>(0.0);
+        const double P = p_single(20.8f, 1.0f, kCbarNa);
+        return -(P * ghk(V, s.in[0], s.ex[0], false) * s.area * op * (double)kAtoPa);
+    }
+    if (t == M_KVDR) {
+        const double P = p_single(12.0f, 1.0f, kCbarK);
+        return -(P * ghk(V, s.in[1], s.ex[1], false) * s.area * g0 * (double)kAtoPa);
+    }
+    if (t == M_K2P) {
+        const double gamma = st[0] > 0.0 ? st[0] : 100.0;
+        const double po = st[1] > 0.0 ? st[1] : 0.012;
+        const double P = st[0] > 0.0 ? p_single((float)gamma, 1.0f, kCbarK) : p_single(100.0f, 1.0f, kCbarK);
+        return -(P * ghk(V, s.in[1], s.ex[1], false) * s.area * po * (double)kAtoPa);
+    }
+    if (t == M_NALCN) {   // -(ina - ik + ica), as the forward function
+        const double gamma = st[0] > 0.0 ? st[0] : 27.0;
+        const P3 pp = st[0] > 0.0 ? p_nalcn3((float)gamma) : p_nalcn3(27.0f);
+        Dn<3> ina = pp.na * ghk(V, s.in[0], s.ex[0], false) * s.area * (double)kAtoPa;
+        Dn<3> ik = pp.k * ghk(V, s.in[1], s.ex[1], false) * s.area * (double)kAtoPa;
+        Dn<3> ica = pp.ca * ghk(V, s.in[2], s.ex[2], true) * s.area * (double)kAtoPa;
+        return -(ina - ik + ica);
+    }
+    if (t == M_HCN1) {
+        const float gamma = st[3] > 0.0 ? (float)st[3] : 0.84f;
+        const P3 pp = st[3] > 0.0 ? p_partial3(gamma, 0.2f, 0.8f, 0.0f) : p_partial3(0.84f, 0.2f, 0.8f, 0.0f);
+        Dn<3> in = pp.na * ghk(V, s.in[0], s.ex[0], false) * s.area * g0 * (double)kAtoPa;
+        Dn<3> ik = pp.k * ghk(V, s.in[1], s.ex[1], false) * s.area * g0 * (double)kAtoPa;
+        return -(in + ik);
+    }
+    return cst<3>(0.0);
+}
+
+// Gate updates of each type: gate_q' as a function of (V, gate_q) (slots 0: V, 1: gate).
+BC_HD int gates_of(uint8_t t) { return t == M_NAV12 ? 2 : (t == M_KVDR || t == M_HCN1) ? 1 : 0; }
+BC_HD Dn<2> gate_update(uint8_t t, int q, double Vv, double gv, const double* st, const SegAd& s, float dt) {
+    Dn<2> V = var<2>(Vv, 0), g = var<2>(gv, 1);
+    if (t == M_NAV12) {
+        const double am_off = 23.0, am_k = 6.0, ah_off = 41.5, bh_off = 66.5, h_off = 63.5;
+        if (q == 0) {
+            Dn<2> am = 0.182 * vtrapd(V + am_off, am_k), bm = 0.124 * vtrapd(-(V + am_off), am_k);
+            Dn<2> minf = am / (am + bm), mtau = 1.0 / (am + bm);
+            return clamp01(minf + (g - minf) * dexp(-(double)dt / mtau));
+        }
+        Dn<2> hinf = 1.0 / (1.0 + dexp((V + h_off) / 6.2));
+        Dn<2> ah = 0.024 * vtrapd(V + ah_off, 5.0), bh = 0.0091 * vtrapd(-(V + bh_off), 5.0);
+        Dn<2> htau = 1.0 / (ah + bh);
+        return clamp01(hinf + (g - hinf) * dexp(-(double)dt / htau));
+    }
+    if (t == M_KVDR) {
+        const double vh = st[1] != 0.0 ? st[1] : -30.0, kk = st[2] > 0.0 ? st[2] : 6.0;
+        Dn<2> tau;
+        if (st[4] > 0.0) tau = cst<2>(st[4]);
+        else {
+            tau = 14.6 * dexp(-(V - vh) / 30.1);
+            if (tau.v < 0.3) tau = cst<2>(0.3);
+            if (tau.v > 200.0) tau = cst<2>(200.0);
+        }
+        Dn<2> mi = 1.0 / (1.0 + dexp(-(V - vh) / (kk + 1e-6)));
+        return clamp01(g + (mi - g) * (double)dt / (tau + 1e-6));
+    }
+    if (t == M_HCN1) {
+        const double vh = st[1] != 0.0 ? st[1] : -69.0, k = st[2] > 0.0 ? st[2] : 6.0;
+        const double t_open = st[4] > 0.0 ? st[4] : 84.0, t_close = st[5] > 0.0 ? st[5] : 300.0;
+        const double ve = vh + 15.0 * s.camp / (s.camp + 0.5);
+        Dn<2> mi = 1.0 / (1.0 + dexp((V - ve) / k));
+        Dn<2> tau = V.v < ve ? t_open * dexp((V - ve) / 15.0) : cst<2>(t_close);
+        return clamp01(g + (mi - g) * (double)dt / tau);
+    }
+    return cst<2>(gv);
+}
+
+BC_HD ChanAd channel_ad(uint8_t t, double V, const double* st, const SegAd& s, double dens, float dt) {
+    ChanAd c;
+    c.ng = gates_of(t);
+    double gnew[2] = {st[0], st[1]};
+    for (int q = 0; q < 2; q++) {
+        c.dg[q][0] = 0.0;
+        c.dg[q][1] = 1.0;
+    }
+    for (int q = 0; q < c.ng; q++) {
+        Dn<2> gq = gate_update(t, q, V, st[q], st, s, dt);
+        gnew[q] = gq.v;
+        c.dg[q][0] = gq.d[0];
+        c.dg[q][1] = gq.d[1];
+    }
+    Dn<3> r1 = current1(t, var<3>(V, 0), var<3>(gnew[0], 1), var<3>(gnew[1], 2), st, s);
+    c.r1 = r1.v;
+    c.r = dens * r1.v;
+    for (int k = 0; k < 3; k++) c.dr[k] = dens * r1.d[k];
+    return c;
+}
+
+BC_DEV double bern_d(float x) {   // dB/dx of the SG Bernoulli function (the forward's branches)
+    if (fabsf(x) < 1e-3f) return -0.5;
+    if (x > 50.0f) return 0.0;
+    if (x < -50.0f) return -1.0;
+    double e = bc::exp_t((double)x, ExpTabGlobal{});
+    double em1 = e - 1.0;
+    return (em1 - x * e) / (em1 * em1);
+}
+
+}  // namespace adj
+
+// Reverse-mode state of the adjoint pass.
+struct AdjNet {
+    const double* recV;      // recorded pre-tick V of the tick being reversed [n]
+    const double* recG;      // recorded pre-tick gate states [slot][2][n]
+    double* lamV;            // adjoint of V after the tick in, before the tick out [n]
+    double* lamG;            // adjoint of the gate states [slot][2][n]
+    double* gdens;           // d loss / d density [slot][n] (accumulated)
+    double* gscale;          // d loss / d connection scale [device connection] (accumulated)
+    float* xs;               // scratch: 51 Jacobi iterates per segment of every tile [tile][51][TS]
+    uint32_t tile0;          // first tile of this size class (scratch offset)
+};
+
+// One tick of the reverse pass for one tile of cells (lane = segment = connection index).
+template <int TS>
+__global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a, uint64_t tick, float dt) {
+    __shared__ double sV[5][TS];       // V at the start of each substep
+    __shared__ double sLam[TS];        // adjoint exchanged between lanes
+    __shared__ float sMu[TS];
+    __shared__ double sDinp[TS];
+    __shared__ float sG[TS], sCt[TS], sDV[TS], sInp1[TS];
+    __shared__ uint32_t sAB[TS];
+    __shared__ uint8_t sIntra[TS];
+    __shared__ uint16_t sEnt[2 * TS];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t s0 = ts.seg0[tile], nseg = ts.nseg[tile], c0 = ts.conn0[tile], nconn = ts.nconn[tile];
+    const uint32_t t = threadIdx.x;
+    const bool act = t < nseg, cact = t < nconn;
+    const uint32_t g = s0 + t;
+    const size_t P = d.npad;
+    const uint32_t E0 = d.csr_off[s0];
+    const uint32_t nent = d.csr_off[s0 + nseg] - E0;
+    uint32_t e0 = 0, e1 = 0;
+    if (act) { e0 = d.csr_off[g] - E0; e1 = d.csr_off[g + 1] - E0; }
+    float* xs = a.xs + (size_t)(a.tile0 + tile) * 51 * TS;
+
+    // ---- recompute step 1 (held ions) and keep the channels' partials ----
+    double V = act ? a.recV[g] : 0.0;
+    adj::SegAd sa;
+    double ibuf = 0.0, area = 0.0;
+    float cap = 1.0f, volf = 1.0f;
+    if (act) {
+        for (int k = 0; k < 3; k++) { sa.in[k] = d.in[k * P + g]; sa.ex[k] = d.ex[k * P + g]; }
+        area = d.area[g];
+        sa.area = area;
+        sa.pkc = d.pkc[g]; sa.pka = d.pka[g]; sa.camp = d.camp[g];
+        cap = d.cap[g];
+        volf = (float)d.vol[g];
+        if (volf < 1e-6f) volf = 1e-6f;
+    }
+    // the channels' partials are evaluated twice (value here, partials in the step-1 reverse)
+    // rather than kept: eight slots of them would not fit in registers
+    const uint32_t cnt = act ? d.mcount[g] : 0;
+    const bool my = act && (d.sflags[g] & SF_MYELIN);
+    const double V0 = V;
+    auto slot_ad = [&](uint32_t j, bool& live) -> adj::ChanAd {
+        const uint8_t ty = d.mtype[j * P + g];
+        live = !(my && ty != 4 && myelin_blocked(ty));
+        double st[7];
+        for (int q = 0; q < 7; q++) st[q] = d.mstate[((size_t)j * 7 + q) * P + g];
+        st[0] = a.recG[((size_t)j * 2 + 0) * P + g];
+        st[1] = a.recG[((size_t)j * 2 + 1) * P + g];
+        return adj::channel_ad(ty, V0, st, sa, d.mdens[j * P + g], dt);
+    };
+    for (uint32_t j = 0; j < cnt; j++) {
+        bool live;
+        const adj::ChanAd c = slot_ad(j, live);
+        if (live) ibuf += c.r;
+    }
+    bool vclamped = false;
+    const uint32_t sh = (act && d.stim.any) ? d.stim.head[g] : ~0u;
+    if (sh != ~0u) {   // current pulses and ramps add to I_buffer; an active clamp fixes V
+        const uint32_t sc = d.stim.ent[sh];
+        for (uint32_t e = 1; e <= sc; e++) {
+            const uint32_t en = d.stim.ent[sh + e];
+            const uint32_t kind = en >> 28, x = en & 0x0FFFFFFFu;
+            if (kind == SK_CUR) {
+                const CurrentStimulus q = d.stim.cur[x];
+                uint64_t dur = (uint64_t)(q.duration_ms / dt);
+                if (tick >= q.start_tick && tick < q.start_tick + dur) ibuf += q.amplitude_pA;
+            } else if (kind == SK_RAMP) {
+                const RampStimulus q = d.stim.ramp[x];
+                if (tick >= q.start_tick && tick < q.end_tick) {
+                    float frac = (q.end_tick > q.start_tick)
+                                     ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick) : 1.0f;
+                    ibuf += q.start_pA + (q.end_pA - q.start_pA) * frac;
+                }
+            } else if (kind == SK_VCLAMP) {
+                const VoltageClamp q = d.stim.vc[x];
+                if (tick >= q.start_tick && tick < q.end_tick) vclamped = true;
+            }
+        }
+    }
+
+    // ---- recompute step 2.5 in cable mode: axial terms per connection, V per substep ----
+    for (uint32_t e = t; e < nent; e += TS) {
+        uint32_t en = d.csr_ent[E0 + e];
+        uint32_t c = en >> 1;
+        sEnt[e] = (uint16_t)((c << 2) | ((d.conn_intra[c0 + c] != 0) << 1) | (en & 1));
+    }
+    uint32_t ca_ = 0, cb_ = 0;
+    float G = 0.0f, csc = 1.0f, kin[7];
+    bool cintra = false;
+    const float dts = dt / 5.0f;
+    if (cact) {
+        const uint32_t ab = d.conn_ab[c0 + t];
+        sAB[t] = ab;
+        ca_ = ab & 0xFFFF;
+        cb_ = ab >> 16;
+        cintra = d.conn_intra[c0 + t] != 0;
+        sIntra[t] = cintra;
+        float ra = d.radius[s0 + ca_], rb = d.radius[s0 + cb_];
+        float rmin = ra < rb ? ra : rb;
+        float cac = kPi * rmin * rmin;
+        if (cac < 1e-6f) cac = 1e-6f;
+        float cdist = (d.length[s0 + ca_] + d.length[s0 + cb_]) * 0.5f;
+        if (cdist < 0.1f) cdist = 0.1f;
+        float sgi = 0.0f;
+        for (int k = 0; k < 7; k++) {
+            float cbar = 0.5f * (d.in[k * P + s0 + ca_] + d.in[k * P + s0 + cb_]);
+            sgi += kZ[k] * kZ[k] * kDiff[k] * cbar;
+        }
+        G = kSigmaE * sgi * cac / cdist;
+        csc = d.conn_scale ? d.conn_scale[c0 + t] : 1.0f;
+        for (int k = 0; k < 7; k++) kin[k] = kDiff[k] * cac * dts / cdist;
+    }
+    const float gax = G * csc;
+    float cdt = cap / dts;
+    if (cdt < 1e-9f) cdt = 1e-9f;
+    __syncthreads();
+    float gdiag = 0.0f;
+    if (cact) sG[t] = cintra ? gax : 0.0f;
+    __syncthreads();
+    if (act)
+        for (uint32_t e = e0; e < e1; e++) {
+            const uint32_t en = sEnt[e];
+            if ((en >> 1) & 1) gdiag += sG[en >> 2];
+        }
+    const float den = cdt + gdiag;
+    // forward of one substep from sV[sub]: axial terms into sCt/sDinp/sDV/sInp1; returns the
+    // Jacobi iterates in xs (x^0 .. x^50) when `keep`, and the new V
+    auto substep = [&](int sub, bool keep) -> float {
+        if (cact) {
+            float inp = 0.0f, dV = 0.0f;
+            double dinp = 0.0;   // d inp / d dV in double: inp - G dV nearly cancels
+            if (cintra) {
+                dV = (sV[sub][ca_] + 0.0) - (sV[sub][cb_] + 0.0);
+                for (int k = 0; k < 7; k++) {
+                    float ca = d.in[k * P + s0 + ca_], cb = d.in[k * P + s0 + cb_];
+                    float x = kZ[k] * kFrtMv * dV;
+                    float fl = kin[k] * (ca * bernoulli(-x, ExpTabGlobal{}) - cb * bernoulli(x, ExpTabGlobal{}));
+                    const double dfl = (double)kin[k] * (-ca * adj::bern_d(-x) - cb * adj::bern_d(x)) * kZ[k] * kFrtMv;
+                    inp += kZ[k] * kChargeToPa * fl / dts;
+                    dinp += (double)kZ[k] * kChargeToPa * dfl / dts;
+                }
+            }
+            sInp1[t] = inp;
+            sDinp[t] = dinp * csc;
+            sDV[t] = dV;
+            sCt[t] = inp * csc - gax * dV;
+        }
+        __syncthreads();
+        float inpd = 0.0f, vold = 0.0f;
+        if (act) {
+            for (uint32_t e = e0; e < e1; e++) {
+                const uint32_t en = sEnt[e];
+                if (!((en >> 1) & 1)) continue;
+                const uint32_t c = en >> 2;
+                if (en & 1) inpd -= sCt[c]; else inpd += sCt[c];
+            }
+            vold = (float)sV[sub][t];
+            sMu[t] = vold;
+            if (keep) xs[t] = vold;
+        }
+        __syncthreads();
+        const float base = cdt * vold + inpd;
+        float v = vold;
+        for (int it = 0; it < 50; it++) {
+            float off = 0.0f;
+            if (act)
+                for (uint32_t e = e0; e < e1; e++) {
+                    const uint32_t en = sEnt[e];
+                    if (!((en >> 1) & 1)) continue;
+                    const uint32_t c = en >> 2, ab = sAB[c];
+                    const uint32_t o = (en & 1) ? (ab & 0xFFFF) : (ab >> 16);
+                    off += sG[c] * sMu[o];
+                }
+            __syncthreads();
+            if (act) {
+                v = (base + off) / den;
+                sMu[t] = v;
+                if (keep) xs[(size_t)(it + 1) * TS + t] = v;
+            }
+            __syncthreads();
+        }
+        return v;
+    };
+    if (act) sV[0][t] = V;
+    __syncthreads();
+    for (int sub = 0; sub < 5; sub++) {
+        float v = substep(sub, false);
+        if (sub < 4) {
+            if (act) sV[sub + 1][t] = v;
+            __syncthreads();
+        } else {
+            V = v;   // V after step 2.5
+        }
+    }
+
+    // ---- reverse: step 4 ----
+    double lam = act ? a.lamV[g] : 0.0;
+    double lam_ibuf = 0.0;
+    if (act) {
+        double ib = ibuf;
+        bool ib_c = false;
+        if (ib != ib) { ib = 0.0; ib_c = true; }
+        if (ib > 10000.0f) { ib = 10000.0f; ib_c = true; }
+        if (ib < -10000.0f) { ib = -10000.0f; ib_c = true; }
+        float dVs = ib / cap * dt;
+        bool dv_c = dVs > 50.0f || dVs < -50.0f;
+        if (dv_c) dVs = dVs > 0 ? 50.0f : -50.0f;
+        double Vn = V + dVs;
+        bool v_c = Vn < -120.0f || Vn > 150.0f || Vn != Vn || vclamped;
+        if (v_c) lam = 0.0;
+        lam_ibuf = (ib_c || dv_c) ? 0.0 : lam * (double)dt / (double)cap;
+    }
+
+    // ---- reverse: step 2.5, substeps in reverse ----
+    double lam_s = 0.0;   // this connection lane's scale gradient
+    for (int sub = 4; sub >= 0; sub--) {
+        substep(sub, true);   // axial terms of this substep and its iterates in xs
+        // transposed sweeps: lam holds d loss / d x^{k+1}
+        double lb = 0.0, lg = 0.0;
+        for (int it = 49; it >= 0; it--) {
+            const double mu = act ? lam / den : 0.0;
+            if (act) sLam[t] = mu;
+            __syncthreads();
+            double nl = 0.0;
+            if (act) {
+                lb += mu;
+                for (uint32_t e = e0; e < e1; e++) {
+                    const uint32_t en = sEnt[e];
+                    if (!((en >> 1) & 1)) continue;
+                    const uint32_t c = en >> 2, ab = sAB[c];
+                    const uint32_t o = (en & 1) ? (ab & 0xFFFF) : (ab >> 16);
+                    nl += sG[c] * sLam[o];
+                }
+            }
+            if (cact && cintra) {   // d x^{k+1}_i / d g = (x^k_other - x^{k+1}_i) / D_i at both ends
+                const float* xk = xs + (size_t)it * TS;
+                const float* xk1 = xs + (size_t)(it + 1) * TS;
+                lg += sLam[ca_] * ((double)xk[cb_] - xk1[ca_]) + sLam[cb_] * ((double)xk[ca_] - xk1[cb_]);
+            }
+            __syncthreads();
+            lam = nl;
+        }
+        // x^0 = V(sub start) and base = cdt * V + inpd, so d loss / d inpd = lb
+        if (act) sLam[t] = lb;
+        __syncthreads();
+        double lam_dv = 0.0;
+        if (cact && cintra) {
+            const double lct = sLam[ca_] - sLam[cb_];   // inpd_a += cterm, inpd_b -= cterm
+            lam_dv = lct * (sDinp[t] - (double)gax);    // cterm = s * inp(dV) - s * G * dV
+            lg += -lct * sDV[t];
+            lam_s += lg * G + lct * sInp1[t];
+        }
+        __syncthreads();   // every read of d loss / d inpd is done
+        sLam[t] = lam_dv;
+        __syncthreads();
+        if (act) {
+            lam = lam + lb * cdt;
+            for (uint32_t e = e0; e < e1; e++) {
+                const uint32_t en = sEnt[e];
+                if (!((en >> 1) & 1)) continue;
+                const double ldv = sLam[en >> 2];
+                lam += (en & 1) ? -ldv : ldv;   // dV = V_a - V_b
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- reverse: step 1 ----
+    if (act) {
+        for (uint32_t j = 0; j < cnt; j++) {
+            bool live;
+            const adj::ChanAd c = slot_ad(j, live);
+            if (!live) continue;
+            const double lr = lam_ibuf;
+            lam += lr * c.dr[0];
+            a.gdens[(size_t)j * P + g] += lr * c.r1;
+            for (int q = 0; q < c.ng; q++) {
+                double* lg = &a.lamG[((size_t)j * 2 + q) * P + g];
+                const double lgn = *lg + lr * c.dr[1 + q];
+                lam += lgn * c.dg[q][0];
+                *lg = lgn * c.dg[q][1];
+            }
+        }
+        a.lamV[g] = lam;
+    }
+    if (cact) a.gscale[c0 + t] += lam_s;
+}
+
+// Records the pre-tick V and gate states (states 0 and 1 of every slot) of one tick.
+__global__ __launch_bounds__(256) void k_adj_record(Net d, double* recV, double* recG) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const size_t P = d.npad;
+    recV[i] = d.V[i];
+    for (uint32_t j = 0; j < d.nslot; j++)
+        for (int q = 0; q < 2; q++) recG[((size_t)j * 2 + q) * P + i] = d.mstate[((size_t)j * 7 + q) * P + i];
+}
+// Loss terms of one tick: acc[c] += (V(soma of cell c) - target[c])^2.
+__global__ __launch_bounds__(256) void k_adj_loss(const double* V, const uint32_t* soma, uint32_t ncells,
+                                                  const float* target, double* acc) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const double e = V[soma[c]] - (double)target[c];
+    acc[c] += e * e;
+}
+// d loss / d V(after the tick) at the somas: coef * (V - target).
+__global__ __launch_bounds__(256) void k_adj_seed(const double* V, const uint32_t* soma, uint32_t ncells,
+                                                  const float* target, double coef, double* lamV) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    lamV[soma[c]] += coef * (V[soma[c]] - (double)target[c]);
+}
+
+}  // namespace bc

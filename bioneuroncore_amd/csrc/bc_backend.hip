@@ -16,6 +16,7 @@
 #include "../../include/biocore_abi.h"
 #include "../../include/biocore_hip_backend.h"
 #include "bc_kernels.hip"
+#include "bc_adjoint.hip"
 
 namespace bc {
 
@@ -193,7 +194,7 @@ struct Sim {
     bool loaded = false;
     Net d{};
     std::vector<void*> dmem;
-    std::vector<uint32_t> dev2host, host2dev, slot2syn;
+    std::vector<uint32_t> dev2host, host2dev, slot2syn, conn2dev;
     uint32_t ntiles = 0;
     uint32_t class_tile0[kTileClasses] = {}, class_ntiles[kTileClasses] = {};
     TileSet tiles[kTileClasses] = {};
@@ -239,6 +240,8 @@ struct Sim {
     float graph_dt = 0.0f;
     BchipSchedule graph_sched{};
     bool use_graphs = true;
+    bool cable = false;               // bchip_set_cable_mode
+    float* conn_scale_dev = nullptr;  // bchip_set_coupling_scale (device order), in dmem
     bool jac_res = false;   // residual-terminated Jacobi (bchip_set_jacobi)
     double jac_tol = 0.0;
     int jac_max = 50;
@@ -406,6 +409,7 @@ static int build_device(Sim& S) {
     }
     // connections placed per tile in global index order
     std::vector<uint32_t> conn_ab(nc), conn_local(nc);
+    S.conn2dev.assign(nc, 0);
     std::vector<uint8_t> conn_intra(nc);
     {
         std::vector<uint32_t> fill(S.ntiles, 0);
@@ -415,6 +419,7 @@ static int build_device(Sim& S) {
             uint32_t loc = fill[t]++;
             conn_local[i] = loc;
             uint32_t pos = tile_conn0[t] + loc;
+            S.conn2dev[i] = pos;
             conn_ab[pos] = (da - tile_seg0[t]) | ((db - tile_seg0[t]) << 16);
             conn_intra[pos] = cn[i].is_intracellular ? 1 : 0;
         }
@@ -679,6 +684,10 @@ static int build_device(Sim& S) {
         }
     }
     d.isl_lane0 = S.dupload(isl_lane0);
+    if (d.ns) S.cable = false;   // a network with synapses is not a cable: the mode ends with it
+    d.cable = S.cable ? 1 : 0;
+    S.conn_scale_dev = nullptr;
+    d.conn_scale = nullptr;
     d.jac_tol = S.jac_res ? S.jac_tol : 0.0;
     d.jac_max = S.jac_res ? S.jac_max : 50;
     d.conn_ab = S.dupload(conn_ab);
@@ -1084,6 +1093,155 @@ static int load_into(Sim& S, BionData* src_owned) {
     if (S.stim_present) S.stim_dirty = true;
     S.loaded = true;
     return 0;
+}
+
+// Reverse pass over `ticks` cable-mode ticks from the current state (see bchip_adjoint).
+static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
+                       uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale) {
+    if (!soma || !target || ticks == 0 || ncells == 0) return -1;
+    if (!S.loaded || !S.cable || S.d.ns) return -1;
+    Net& d = S.d;
+    const uint32_t n = d.n, P = d.npad, nslot = d.nslot;
+    if (nslot > 8) return -2;
+    for (uint32_t h = 0; h < n; h++) {
+        const Segment& sg = S.host.segments[h];
+        for (uint32_t j = 0; j < sg.modifier_count; j++)
+            if (!adj::adjoint_supported(sg.modifiers[j].type)) return -2;
+    }
+    for (uint32_t c = 0; c < ncells; c++)
+        if (soma[c] >= n) return -1;
+    const bool backward = grad_density || grad_scale;
+    const uint32_t K = checkpoint_every ? checkpoint_every : ticks;
+    const uint32_t nck = (ticks + K - 1) / K;
+    const uint32_t nconn = (uint32_t)S.host.num_connections;
+    std::vector<void*> tmp;
+    auto alloc = [&](size_t bytes) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 64)));
+        tmp.push_back(p);
+        HIPCHK(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 64), S.stream));
+        return p;
+    };
+    int rc = 0;
+    try {
+        (void)hipSetDevice(S.device);
+        before_ticks(S);
+        const size_t vB = (size_t)P * 8, mB = (size_t)nslot * 7 * P * 8, gB = (size_t)nslot * 2 * P * 8;
+        char* ck = static_cast<char*>(alloc((size_t)(nck + 1) * (vB + mB + 8)));
+        auto save = [&](uint32_t k) {
+            char* b = ck + (size_t)k * (vB + mB + 8);
+            HIPCHK(hipMemcpyAsync(b, d.V, vB, hipMemcpyDeviceToDevice, S.stream));
+            HIPCHK(hipMemcpyAsync(b + vB, d.mstate, mB, hipMemcpyDeviceToDevice, S.stream));
+            HIPCHK(hipMemcpyAsync(b + vB + mB, S.tick_dev, 8, hipMemcpyDeviceToDevice, S.stream));
+        };
+        auto restore = [&](uint32_t k) {
+            const char* b = ck + (size_t)k * (vB + mB + 8);
+            HIPCHK(hipMemcpyAsync(d.V, b, vB, hipMemcpyDeviceToDevice, S.stream));
+            HIPCHK(hipMemcpyAsync(d.mstate, b + vB, mB, hipMemcpyDeviceToDevice, S.stream));
+            HIPCHK(hipMemcpyAsync(S.tick_dev, b + vB + mB, 8, hipMemcpyDeviceToDevice, S.stream));
+        };
+        std::vector<uint32_t> soma_dev(ncells);
+        for (uint32_t c = 0; c < ncells; c++) soma_dev[c] = S.host2dev[soma[c]];
+        uint32_t* dsoma = static_cast<uint32_t*>(alloc((size_t)ncells * 4));
+        float* dtarget = static_cast<float*>(alloc((size_t)ticks * ncells * 4));
+        double* acc = static_cast<double*>(alloc((size_t)ncells * 8));
+        HIPCHK(hipMemcpyAsync(dsoma, soma_dev.data(), (size_t)ncells * 4, hipMemcpyHostToDevice, S.stream));
+        HIPCHK(hipMemcpyAsync(dtarget, target, (size_t)ticks * ncells * 4, hipMemcpyHostToDevice, S.stream));
+        const uint32_t gc = (ncells + 255) / 256;
+        // forward: checkpoints every K ticks, loss after every tick
+        for (uint32_t t = 0; t < ticks; t++) {
+            if (t % K == 0) save(t / K);
+            launch_tick(S, dt, false);
+            k_adj_loss<<<gc, 256, 0, S.stream>>>(d.V, dsoma, ncells, dtarget + (size_t)t * ncells, acc);
+        }
+        save(nck);   // the state after the last tick, restored at the end
+        // recomputing a span also counts its spikes again, rewrites I_buffer and re-applies a
+        // ligand injection of its first tick: keep those fields as the forward left them
+        const size_t spB = (size_t)P * 4, ntB = (size_t)15 * P * 4;
+        char* fin = static_cast<char*>(alloc(spB + 2 * ntB + vB));
+        auto keep_final = [&](bool out) {
+            void* dev[4] = {d.spikes, d.nt, d.nt_in, d.ibuf};
+            const size_t sz[4] = {spB, ntB, ntB, vB};
+            size_t o = 0;
+            for (int q = 0; q < 4; q++) {
+                HIPCHK(hipMemcpyAsync(out ? dev[q] : fin + o, out ? fin + o : dev[q], sz[q],
+                                      hipMemcpyDeviceToDevice, S.stream));
+                o += sz[q];
+            }
+        };
+        keep_final(false);
+        HIPCHK(hipGetLastError());
+        std::vector<double> hacc(ncells);
+        HIPCHK(hipMemcpyAsync(hacc.data(), acc, (size_t)ncells * 8, hipMemcpyDeviceToHost, S.stream));
+        HIPCHK(hipStreamSynchronize(S.stream));
+        double L = 0.0;
+        for (double x : hacc) L += x;
+        L /= (double)ticks * ncells;
+        if (loss) *loss = L;
+        if (backward) {
+            double* rec = static_cast<double*>(alloc((size_t)(K + 1) * (vB + gB)));
+            AdjNet a{};
+            a.lamV = static_cast<double*>(alloc(vB));
+            a.lamG = static_cast<double*>(alloc(gB));
+            a.gdens = static_cast<double*>(alloc((size_t)nslot * P * 8));
+            a.gscale = static_cast<double*>(alloc((size_t)std::max<uint32_t>(nconn, 1) * 8));
+            size_t xs_off[kTileClasses], xs_total = 0;
+            for (int c = 0; c < kTileClasses; c++) {
+                xs_off[c] = xs_total;
+                xs_total += (size_t)S.class_ntiles[c] * 51 * kTileSize[c];
+            }
+            float* xs = static_cast<float*>(alloc(xs_total * 4));
+            const double coef = 2.0 / ((double)ticks * ncells);
+            const uint32_t gs = (n + 255) / 256;
+            for (int seg = (int)nck - 1; seg >= 0; seg--) {
+                const uint32_t c0 = (uint32_t)seg * K, c1 = std::min(ticks, c0 + K);
+                restore((uint32_t)seg);
+                for (uint32_t t = c0; t < c1; t++) {
+                    double* r = rec + (size_t)(t - c0) * (P + (size_t)nslot * 2 * P);
+                    k_adj_record<<<gs, 256, 0, S.stream>>>(d, r, r + P);
+                    launch_tick(S, dt, false);
+                }
+                double* rl = rec + (size_t)(c1 - c0) * (P + (size_t)nslot * 2 * P);
+                k_adj_record<<<gs, 256, 0, S.stream>>>(d, rl, rl + P);
+                for (uint32_t t = c1; t-- > c0;) {
+                    const double* post = rec + (size_t)(t + 1 - c0) * (P + (size_t)nslot * 2 * P);
+                    k_adj_seed<<<gc, 256, 0, S.stream>>>(post, dsoma, ncells, dtarget + (size_t)t * ncells, coef, a.lamV);
+                    const double* pre = rec + (size_t)(t - c0) * (P + (size_t)nslot * 2 * P);
+                    a.recV = pre;
+                    a.recG = pre + P;
+                    for (int c = 0; c < kTileClasses; c++) {
+                        if (!S.class_ntiles[c]) continue;
+                        a.xs = xs + xs_off[c];
+                        a.tile0 = 0;
+                        if (c == 0) k_adjoint_tick<64><<<S.class_ntiles[0], 64, 0, S.stream>>>(d, S.tiles[0], a, t, dt);
+                        if (c == 1) k_adjoint_tick<128><<<S.class_ntiles[1], 128, 0, S.stream>>>(d, S.tiles[1], a, t, dt);
+                        if (c == 2) k_adjoint_tick<256><<<S.class_ntiles[2], 256, 0, S.stream>>>(d, S.tiles[2], a, t, dt);
+                    }
+                }
+                HIPCHK(hipGetLastError());
+            }
+            std::vector<double> gd((size_t)nslot * P), gsc(std::max<uint32_t>(nconn, 1));
+            HIPCHK(hipMemcpyAsync(gd.data(), a.gdens, gd.size() * 8, hipMemcpyDeviceToHost, S.stream));
+            HIPCHK(hipMemcpyAsync(gsc.data(), a.gscale, gsc.size() * 8, hipMemcpyDeviceToHost, S.stream));
+            HIPCHK(hipStreamSynchronize(S.stream));
+            if (grad_density) {
+                memset(grad_density, 0, (size_t)n * BC_MAX_MODIFIERS * 8);
+                for (uint32_t h = 0; h < n; h++)
+                    for (uint32_t j = 0; j < nslot; j++)
+                        grad_density[(size_t)h * BC_MAX_MODIFIERS + j] = gd[(size_t)j * P + S.host2dev[h]];
+            }
+            if (grad_scale)
+                for (uint32_t i = 0; i < nconn; i++) grad_scale[i] = gsc[S.conn2dev[i]];
+            restore(nck);
+            keep_final(true);
+        }
+        HIPCHK(hipStreamSynchronize(S.stream));
+        after_ticks(S, ticks);
+    } catch (...) {
+        rc = -1;
+    }
+    for (void* p : tmp) (void)hipFree(p);
+    return rc;
 }
 
 }  // namespace bc
@@ -1492,6 +1650,40 @@ int bchip_set_jacobi(void* sim, double tol, int max_sweeps) {
     S.d.jac_tol = S.jac_tol;
     S.d.jac_max = S.jac_max;
     return 0;
+}
+
+int bchip_set_cable_mode(void* sim, int on) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (on && S.loaded && S.d.ns) return -1;   // receptors would move ions: not a cable
+    S.cable = on != 0;
+    S.d.cable = S.cable ? 1 : 0;
+    return 0;
+}
+
+int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    if (!S.loaded) return -1;
+    if (!scale) { S.d.conn_scale = nullptr; return 0; }
+    if (n != S.host.num_connections) return -1;
+    try {
+        (void)hipSetDevice(S.device);
+        std::vector<float> dev(std::max<uint32_t>(n, 1), 1.0f);
+        for (uint32_t i = 0; i < n; i++) dev[S.conn2dev[i]] = scale[i];
+        if (!S.conn_scale_dev) S.conn_scale_dev = S.dalloc<float>(dev.size());
+        HIPCHK(hipMemcpyAsync(S.conn_scale_dev, dev.data(), dev.size() * 4, hipMemcpyHostToDevice, S.stream));
+        HIPCHK(hipStreamSynchronize(S.stream));
+        S.d.conn_scale = S.conn_scale_dev;
+    } catch (...) { return -1; }
+    return 0;
+}
+
+int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
+                  uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale) {
+    if (!sim) return -1;
+    return bc::adjoint_run(*static_cast<Sim*>(sim), dt, ticks, checkpoint_every, soma, ncells, target, loss,
+                           grad_density, grad_scale);
 }
 
 int bchip_set_graphs(void* sim, int on) {

@@ -79,6 +79,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         m.a7 = s.area;
         m.av = d.mav[j * P + i];
         itot += dispatch(t, m, s, dt);
+        if (d.cable) {   // concentrations held: every channel sees the stored ones
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                s.in[k] = d.in[k * P + i];
+                s.ex[k] = d.ex[k * P + i];
+            }
+            s.atp = d.atp[i];
+        }
     }
     // stimuli (simulation_host.cpp:247-300): this segment's list, in the reference's order
     const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
@@ -121,12 +129,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     }
     d.V[i] = s.V;
+    if (!d.cable) {
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
-        d.in[k * P + i] = s.in[k];
-        d.ex[k * P + i] = s.ex[k];
+        for (int k = 0; k < 7; k++) {
+            d.in[k * P + i] = s.in[k];
+            d.ex[k * P + i] = s.ex[k];
+        }
+        d.atp[i] = s.atp;
     }
-    d.atp[i] = s.atp;
     d.ibuf[i] = itot;
 }
 
@@ -419,7 +429,8 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
             cur ^= 1;
         }
     };
-    if (dsel == 2) phi_sweeps(std::integral_constant<int, 2>{});
+    if (d.cable) {   // phi_e = 0 (both planes hold zeros)
+    } else if (dsel == 2) phi_sweeps(std::integral_constant<int, 2>{});
     else if (dsel == 3) phi_sweeps(std::integral_constant<int, 3>{});
     else phi_sweeps(std::integral_constant<int, 4>{});
 
@@ -456,6 +467,8 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 sgi += kZ[k] * kZ[k] * kDiff[k] * cbar;
             }
             float gax = kSigmaE * sgi * cac / cdist;
+            const float csc = d.conn_scale ? d.conn_scale[c0 + t] : 1.0f;
+            if (d.conn_scale) gax *= csc;
             L.u.i.cg[t] = gax;
             float dV = (L.u.i.V[a] + pha) - (L.u.i.V[b] + phb);
             BPair b1, b2;
@@ -466,6 +479,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 float ca = L.ion[k][a], cb = L.ion[k][b];
                 BPair bb = bsel(k, b1, b2);
                 float fl = kin[k] * (ca * bb.m - cb * bb.p);
+                if (d.conn_scale) fl *= csc;
                 cfin[k * TS + t] = fl;
                 inp += kZ[k] * kChargeToPa * fl / dts;
             }
@@ -609,12 +623,17 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
         if (act) {
             vcur = vnew;
             L.u.i.V[t] = vcur;
+            if (!d.cable)
 #pragma unroll
-            for (int k = 0; k < 7; k++) L.ion[k][t] += (float)dacc[k];
+                for (int k = 0; k < 7; k++) L.ion[k][t] += (float)dacc[k];
         }
         tile_sync<TS>();
     }
     // intracellular ions are final: store them, then reuse the plane for extracellular ions
+    if (d.cable) {   // concentrations, ATP, temperature and transmitters are held
+        if (act) d.V[g] = vcur;
+        return;
+    }
     if (act) {
         d.V[g] = vcur;
 #pragma unroll

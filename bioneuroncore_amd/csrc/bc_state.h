@@ -95,6 +95,12 @@ struct Net {
     double jac_tol;
     int32_t jac_max;
     const uint8_t* isl_lane0;   // per segment: tile-local index of its island's first segment
+    // Cable mode (an extension, the tick the adjoint differentiates): ion concentrations and
+    // ATP held at their values, phi_e = 0; the axial current is the intracellular SG flux and
+    // chord term at those concentrations. conn_scale (per connection, device order; null = 1)
+    // multiplies the axial conductance and the intracellular ion fluxes of each connection.
+    uint32_t cable;
+    const float* conn_scale;
 
     // segment dynamic state
     double* V;

@@ -116,6 +116,25 @@ int bchip_set_graphs(void* sim, int on);
 // always sweeps 50 times): each connected cell stops after the first sweep whose largest update
 // is <= tol (mV), or after max_sweeps. tol <= 0 restores the reference's fixed 50 sweeps.
 int bchip_set_jacobi(void* sim, double tol, int max_sweeps);
+// Cable mode (an extension; the tick bchip_adjoint differentiates): ion concentrations and ATP
+// are held at their values and phi_e = 0; the axial current is the intracellular SG flux and
+// chord term at those concentrations; ion injections are ignored, ligand injections apply.
+// Networks with synapses are refused (-1), and loading one ends the mode. The mode and the
+// scales below stay set across ticks; a load resets the scales to 1.
+int bchip_set_cable_mode(void* sim, int on);
+// Per-connection scale (connection order, n = num_connections) of the axial conductance and the
+// intracellular ion fluxes; NULL restores 1 (the reference). Scale 1 changes no bit.
+int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n);
+// Gradient of the cable-mode tick run for `ticks` ticks from the current state, for the loss
+// mean over ticks and cells of (V_m of host segment soma[c] after the tick - target[tick][c])^2
+// (target: ticks x ncells). State is checkpointed every checkpoint_every ticks and each span
+// is recomputed once for the reverse pass. grad_density: num_segments x 32 (d loss / d density
+// of modifier slot j of segment h); grad_scale: num_connections (d loss / d coupling scale).
+// With both NULL only the forward pass and the loss run. The simulation ends `ticks` ticks
+// later, as after bchip_run_ticks. Returns 0, -1 (arguments or state: cable mode off,
+// synapses present), -2 (a modifier type outside Nav1.2/Kv-DR/K2P/NALCN/HCN1, or > 8 slots).
+int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
+                  uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale);
 
 #ifdef __cplusplus
 }

@@ -659,6 +659,11 @@ struct Ctx {
     int jac_max = 50;
     std::vector<uint32_t> comp;   // cell (component over all connections) of each segment
     uint32_t ncomp = 0;
+    // Cable mode (an extension): ion concentrations, ATP, temperature and transmitters are held
+    // (every channel sees the stored concentrations, ion injections are ignored), phi_e = 0, and
+    // step 2.5 only moves V. scale: per-connection factor on axial conductance and flux (empty: 1).
+    bool cable = false;
+    std::vector<float> scale;
 };
 
 static uint32_t uf_root(std::vector<uint32_t>& p, uint32_t x) {
@@ -776,8 +781,10 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
     c.vnew.assign(n, 0.0f);
     c.off.assign(n, 0.0f);
     if (c.jac_on) cells(c, n, cn, nc);
-    solve_phi(c, sg, n, cn, nc);
+    if (c.cable) c.phi.assign(n, 0.0);
+    else solve_phi(c, sg, n, cn, nc);
     const std::vector<double>& phi = c.phi;
+    const bool scaled = c.scale.size() >= nc;
 
     const int kSub = 5;
     const float dts = dt / kSub;
@@ -796,6 +803,7 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
                 sig += kZ[ion] * kZ[ion] * kDiff[ion] * cbar;
             }
             float gax = kSigmaE * sig * g.ac / g.dist;
+            if (scaled) gax *= c.scale[i];
             c.goff[i] = gax;
             c.gdiag[a] += gax;
             c.gdiag[b] += gax;
@@ -805,6 +813,7 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
                 float ca = sg[a].intra_ions[ion], cb = sg[b].intra_ions[ion];
                 float x = kZ[ion] * kFrtMv * dV;
                 float fl = kDiff[ion] * g.ac * dts / g.dist * (ca * bernoulli(-x) - cb * bernoulli(x));
+                if (scaled) fl *= c.scale[i];
                 c.dacc[(size_t)a * 7 + ion] -= (double)fl / g.va;
                 c.dacc[(size_t)b * 7 + ion] += (double)fl / g.vb;
                 inp += kZ[ion] * kChargeToPa * fl / dts;
@@ -835,9 +844,11 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
             if (c.jac_on && cells_settle(c, res, done)) break;
         }
         for (uint32_t i = 0; i < n; i++) sg[i].V_m = c.vnew[i];
-        for (uint32_t i = 0; i < n; i++)
-            for (int ion = 0; ion < 7; ion++) sg[i].intra_ions[ion] += (float)c.dacc[(size_t)i * 7 + ion];
+        if (!c.cable)
+            for (uint32_t i = 0; i < n; i++)
+                for (int ion = 0; ion < 7; ion++) sg[i].intra_ions[ion] += (float)c.dacc[(size_t)i * 7 + ion];
     }
+    if (c.cable) return;
 
     for (int sub = 0; sub < kSub; sub++) {
         std::fill(c.dext.begin(), c.dext.end(), 0.0);
@@ -1162,7 +1173,18 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;
-            itot += dispatch(t, &s->modifiers[j], s, dt);
+            if (c.cable) {   // held concentrations: every channel sees the stored ones
+                double in[BC_ION_MAX], ex[BC_ION_MAX];
+                const float atp = s->ATP_level;
+                memcpy(in, s->intra_ions, sizeof in);
+                memcpy(ex, s->extra_ions, sizeof ex);
+                itot += dispatch(t, &s->modifiers[j], s, dt);
+                memcpy(s->intra_ions, in, sizeof in);
+                memcpy(s->extra_ions, ex, sizeof ex);
+                s->ATP_level = atp;
+            } else {
+                itot += dispatch(t, &s->modifiers[j], s, dt);
+            }
         }
         if (st.cur && st.ncur)
             for (uint32_t k = 0; k < *st.ncur; k++) {
@@ -1186,7 +1208,7 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
         if (st.ii && st.nii)
             for (uint32_t k = 0; k < *st.nii; k++) {
                 const IonInjection& q = st.ii[k];
-                if (q.segment_id == s->id && q.ion_idx >= 0 && q.ion_idx < 7)
+                if (q.segment_id == s->id && q.ion_idx >= 0 && q.ion_idx < 7 && !c.cable)
                     (q.extracellular ? s->extra_ions : s->intra_ions)[q.ion_idx] += q.amount_mM;
             }
         if (st.li && st.nli)
@@ -1388,6 +1410,176 @@ static void f64_tick(F64& s, double dt) {
 }
 }  // namespace orc
 
+// ---- FP64 restatement of the cable-mode tick (an extension of the backend with no reference
+// counterpart): held ion concentrations and ATP, phi_e = 0. Step 1 with Nav1.2, Kv-DR, K2P,
+// NALCN and HCN1 (hh_biological.cpp:193-277, 2763-2798, 3063-3085, 3609-3641;
+// modifier_helpers.h:1442-1484) and current pulses/ramps (simulation_host.cpp:247-300); step 2.5
+// as five substeps of axial Scharfetter-Gummel flux + chord term and 50 Jacobi sweeps of V
+// (diffusion_kernel.h:151-238) with each connection's conductance and flux times its scale;
+// step 4 (simulation_host.cpp:596-649). Every quantity is double. Its central differences are
+// the reference gradient of the backend's adjoint.
+namespace orc {
+struct Cab64 {
+    struct Ch { uint8_t type; double dens, st[7]; };
+    uint32_t n = 0, nc = 0;
+    uint64_t tick = 0;
+    std::vector<double> V, cap, area, vol, rad, len, in, ex, pkc, pka, camp, scale;
+    std::vector<uint32_t> ca, cb, ids, ch_off;
+    std::vector<uint8_t> intra;
+    std::vector<Ch> ch;
+    std::vector<CurrentStimulus> cur;
+    std::vector<RampStimulus> ramp;
+    std::vector<VoltageClamp> vc;
+};
+static inline double ghk64(double V, double ci, double co, bool di) {   // run_tick_common.h:274-312
+    const double z = di ? 2.0 : 1.0, rt = (double)kGas * kTemp;
+    const double vn = z * kFaraday * V * 0.001 / rt, k = z * z * kFaraday / rt;
+    if (fabs(vn) < 1e-6) return z * kFaraday * (ci - co);
+    if (vn > 20.0) return k * kFaraday * V * 0.001 * ci;
+    if (vn < -20.0) return k * kFaraday * V * 0.001 * co;
+    const double e = exp(-vn);
+    return k * kFaraday * V * 0.001 * (ci - co * e) / (1.0 - e);
+}
+static inline double vtrap64(double x, double y) {
+    if (fabs(x / y) < 1e-6) return y * (1.0 - x / (2.0 * y));
+    return x / (1.0 - exp(-x / y));
+}
+static inline double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+// One channel of segment i: updates its gates and returns its current (pA).
+static double cab_channel(Cab64& s, uint32_t i, Cab64::Ch& c, double dt) {
+    const double V = s.V[i], A = s.area[i], N = c.dens * A;
+    const double* ci = &s.in[(size_t)i * 7];
+    const double* co = &s.ex[(size_t)i * 7];
+    double* st = c.st;
+    switch (c.type) {
+        case T_NAV12: {
+            const double am = 0.182 * vtrap64(V + 23.0, 6.0), bm = 0.124 * vtrap64(-(V + 23.0), 6.0);
+            const double ah = 0.024 * vtrap64(V + 41.5, 5.0), bh = 0.0091 * vtrap64(-(V + 66.5), 5.0);
+            const double minf = am / (am + bm), hinf = 1.0 / (1.0 + exp((V + 63.5) / 6.2));
+            st[0] = clamp01(minf + (st[0] - minf) * exp(-dt * (am + bm)));
+            st[1] = clamp01(hinf + (st[1] - hinf) * exp(-dt * (ah + bh)));
+            double op = st[0] * st[0] * st[0] * st[1];
+            op /= 1.0 + (st[5] > 0.0 ? st[5] : 0.0) / 10.0;
+            op *= 1.0 - 0.7 * (s.pkc[i] + s.pka[i]) * 0.5;
+            if (op < 0.0) op = 0.0;
+            return -(p_single(20.8f, 1.0f, kCbarNa) * ghk64(V, ci[0], co[0], false) * N * op * kAtoPa);
+        }
+        case T_KVDR: {
+            const double vh = st[1] != 0.0 ? st[1] : -30.0, kk = st[2] > 0.0 ? st[2] : 6.0;
+            double tau = st[4];
+            if (!(tau > 0.0)) tau = std::min(200.0, std::max(0.3, 14.6 * exp(-(V - vh) / 30.1)));
+            const double mi = 1.0 / (1.0 + exp(-(V - vh) / (kk + 1e-6)));
+            st[0] = clamp01(st[0] + (mi - st[0]) * dt / (tau + 1e-6));
+            return -(p_single(12.0f, 1.0f, kCbarK) * ghk64(V, ci[1], co[1], false) * N * st[0] * kAtoPa);
+        }
+        case T_K2P: {
+            const double P = p_single(st[0] > 0.0 ? (float)st[0] : 100.0f, 1.0f, kCbarK);
+            const double po = st[1] > 0.0 ? st[1] : 0.012;
+            return -(P * ghk64(V, ci[1], co[1], false) * N * po * kAtoPa);
+        }
+        case T_NALCN: {
+            double pna, pk, pca;
+            p_nalcn(st[0] > 0.0 ? (float)st[0] : 27.0f, &pna, &pk, &pca);
+            const double ina = pna * ghk64(V, ci[0], co[0], false) * N * kAtoPa;
+            const double ik = pk * ghk64(V, ci[1], co[1], false) * N * kAtoPa;
+            const double ica = pca * ghk64(V, ci[2], co[2], true) * N * kAtoPa;
+            return -(ina - ik + ica);
+        }
+        case T_HCN1: {
+            const double vh = st[1] != 0.0 ? st[1] : -69.0, kk = st[2] > 0.0 ? st[2] : 6.0;
+            const double to = st[4] > 0.0 ? st[4] : 84.0, tc = st[5] > 0.0 ? st[5] : 300.0;
+            const double ve = vh + 15.0 * s.camp[i] / (s.camp[i] + 0.5);
+            const double mi = 1.0 / (1.0 + exp((V - ve) / kk));
+            const double tau = V < ve ? to * exp((V - ve) / 15.0) : tc;
+            st[0] = clamp01(st[0] + (mi - st[0]) * dt / tau);
+            double pna, pk, pca;
+            p_partial(st[3] > 0.0 ? (float)st[3] : 0.84f, 0.2f, 0.8f, 0.0f, &pna, &pk, &pca);
+            return -(pna * ghk64(V, ci[0], co[0], false) * N * st[0] * kAtoPa +
+                     pk * ghk64(V, ci[1], co[1], false) * N * st[0] * kAtoPa);
+        }
+        default: return 0.0;
+    }
+}
+static void cab_tick(Cab64& s, double dt) {
+    const uint32_t n = s.n, nc = s.nc;
+    const uint64_t tick = s.tick;
+    std::vector<double> ib(n, 0.0);
+    for (uint32_t i = 0; i < n; i++) {
+        for (uint32_t j = s.ch_off[i]; j < s.ch_off[i + 1]; j++) ib[i] += cab_channel(s, i, s.ch[j], dt);
+        for (const CurrentStimulus& q : s.cur)
+            if (q.segment_id == s.ids[i] && tick >= q.start_tick &&
+                tick < q.start_tick + (uint64_t)(q.duration_ms / (float)dt))
+                ib[i] += q.amplitude_pA;
+        for (const RampStimulus& q : s.ramp)
+            if (q.segment_id == s.ids[i] && tick >= q.start_tick && tick < q.end_tick) {
+                const double f = q.end_tick > q.start_tick
+                                     ? (double)(tick - q.start_tick) / (double)(q.end_tick - q.start_tick) : 1.0;
+                ib[i] += q.start_pA + (q.end_pA - q.start_pA) * f;
+            }
+    }
+    const double dts = dt / 5.0, pi = (double)kPi;
+    std::vector<double> G(nc), inpd(n), gdiag(n), vold(n), off(n);
+    for (uint32_t e = 0; e < nc; e++) {
+        const uint32_t a = s.ca[e], b = s.cb[e];
+        double sg = 0.0;
+        for (int k = 0; k < 7; k++)
+            sg += (double)kZ[k] * kZ[k] * kDiff[k] * 0.5 * (s.in[(size_t)a * 7 + k] + s.in[(size_t)b * 7 + k]);
+        const double rmin = std::min(s.rad[a], s.rad[b]);
+        const double ac = std::max(pi * rmin * rmin, 1e-6), dist = std::max((s.len[a] + s.len[b]) * 0.5, 0.1);
+        G[e] = s.intra[e] ? (double)kSigmaE * sg * ac / dist * s.scale[e] : 0.0;
+    }
+    std::fill(gdiag.begin(), gdiag.end(), 0.0);
+    for (uint32_t e = 0; e < nc; e++) {
+        gdiag[s.ca[e]] += G[e];
+        gdiag[s.cb[e]] += G[e];
+    }
+    for (int sub = 0; sub < 5; sub++) {
+        std::fill(inpd.begin(), inpd.end(), 0.0);
+        for (uint32_t e = 0; e < nc; e++) {
+            if (!s.intra[e]) continue;
+            const uint32_t a = s.ca[e], b = s.cb[e];
+            const double rmin = std::min(s.rad[a], s.rad[b]);
+            const double ac = std::max(pi * rmin * rmin, 1e-6), dist = std::max((s.len[a] + s.len[b]) * 0.5, 0.1);
+            const double dV = s.V[a] - s.V[b];
+            double inp = 0.0;
+            for (int k = 0; k < 7; k++) {
+                const double x = (double)kZ[k] * kFrtMv * dV;
+                const double fl = (double)kDiff[k] * ac * dts / dist *
+                                  (s.in[(size_t)a * 7 + k] * bern64(-x) - s.in[(size_t)b * 7 + k] * bern64(x));
+                inp += (double)kZ[k] * kChargeToPa * fl * s.scale[e] / dts;
+            }
+            inpd[a] += inp - G[e] * dV;
+            inpd[b] -= inp - G[e] * dV;
+        }
+        vold = s.V;
+        for (int it = 0; it < 50; it++) {
+            std::fill(off.begin(), off.end(), 0.0);
+            for (uint32_t e = 0; e < nc; e++) {
+                off[s.ca[e]] += G[e] * s.V[s.cb[e]];
+                off[s.cb[e]] += G[e] * s.V[s.ca[e]];
+            }
+            for (uint32_t i = 0; i < n; i++) {
+                const double cdt = std::max(s.cap[i] / dts, 1e-9);
+                s.V[i] = (cdt * vold[i] + inpd[i] + off[i]) / (cdt + gdiag[i]);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        double x = ib[i];
+        if (x != x) x = 0.0;
+        x = std::min(10000.0, std::max(-10000.0, x));
+        double dV = x / s.cap[i] * dt;
+        dV = std::min(50.0, std::max(-50.0, dV));
+        double v = s.V[i] + dV;
+        for (const VoltageClamp& q : s.vc)
+            if (q.segment_id == s.ids[i] && tick >= q.start_tick && tick < q.end_tick) v = q.target_mV;
+        if (v != v) v = -65.0;
+        s.V[i] = std::min(150.0, std::max(-120.0, v));
+    }
+    s.tick++;
+}
+}  // namespace orc
+
 extern "C" {
 
 void* oracle_create(void) { return new orc::Ctx(); }
@@ -1438,6 +1630,14 @@ void oracle_set_jacobi(void* ctx, double tol, int max_sweeps) {
     c->jac_max = max_sweeps > 0 ? max_sweeps : 50;
 }
 
+// Cable mode and per-connection coupling scale (see orc::Ctx::cable); scale may be NULL.
+void oracle_set_cable(void* ctx, int on, const float* scale, uint32_t nc) {
+    orc::Ctx* c = static_cast<orc::Ctx*>(ctx);
+    c->cable = on != 0;
+    if (scale) c->scale.assign(scale, scale + nc);
+    else c->scale.clear();
+}
+
 // FP64 restatement for passive networks (see orc::f64_tick).
 void* oracle_fp64_create(const Segment* sg, uint32_t n, const IntraConnection* cn, uint32_t nc) {
     orc::F64* s = new orc::F64();
@@ -1485,6 +1685,73 @@ void oracle_fp64_read(void* h, double* V, double* in, double* ex, double* atp, d
     memcpy(temp, s.temp.data(), s.n * 8);
 }
 void oracle_fp64_destroy(void* h) { delete static_cast<orc::F64*>(h); }
+
+// FP64 cable-mode restatement (see orc::cab_tick). scale: per connection (NULL: all 1).
+void* oracle_cable64_create(const Segment* sg, uint32_t n, const IntraConnection* cn, uint32_t nc,
+                            const float* scale) {
+    orc::Cab64* s = new orc::Cab64();
+    const double pi = (double)orc::kPi;
+    s->n = n;
+    s->ch_off.push_back(0);
+    for (uint32_t i = 0; i < n; i++) {
+        const Segment& x = sg[i];
+        const bool sph = orc::is_sphere(&x);
+        const double r = x.radius, l = x.length;
+        s->ids.push_back(x.id);
+        s->V.push_back(x.V_m);
+        s->rad.push_back(r);
+        s->len.push_back(l);
+        s->area.push_back(sph ? 4.0 * pi * r * r : 2.0 * pi * r * l);
+        s->vol.push_back(sph ? (4.0 / 3.0) * pi * r * r * r : pi * r * r * l);
+        s->cap.push_back(std::max(s->area.back() * 0.01, 1e-4));
+        for (int k = 0; k < 7; k++) { s->in.push_back(x.intra_ions[k]); s->ex.push_back(x.extra_ions[k]); }
+        s->pkc.push_back(orc::first_state_of(&x, orc::T_PKC, 0, 0.0f));
+        s->pka.push_back(orc::first_state_of(&x, orc::T_CAMP_PKA, 1, 0.0f));
+        s->camp.push_back(orc::first_state_of(&x, orc::T_CAMP_PKA, 0, 0.1f));
+        for (uint32_t j = 0; j < x.modifier_count; j++) {
+            orc::Cab64::Ch c;
+            c.type = x.modifiers[j].type;
+            c.dens = x.modifiers[j].density;
+            for (int q = 0; q < 7; q++) c.st[q] = x.modifiers[j].state[q];
+            s->ch.push_back(c);
+        }
+        s->ch_off.push_back((uint32_t)s->ch.size());
+    }
+    for (uint32_t e = 0; e < nc; e++) {
+        if (cn[e].segment_a >= n || cn[e].segment_b >= n) continue;
+        s->ca.push_back(cn[e].segment_a);
+        s->cb.push_back(cn[e].segment_b);
+        s->intra.push_back(cn[e].is_intracellular != 0);
+        s->scale.push_back(scale ? scale[e] : 1.0);
+    }
+    s->nc = (uint32_t)s->ca.size();
+    return s;
+}
+void oracle_cable64_set_stimuli(void* h, const CurrentStimulus* cur, uint32_t ncur, const RampStimulus* ramp,
+                                uint32_t nramp, const VoltageClamp* vc, uint32_t nvc) {
+    orc::Cab64& s = *static_cast<orc::Cab64*>(h);
+    s.cur.assign(cur, cur + ncur);
+    s.ramp.assign(ramp, ramp + nramp);
+    s.vc.assign(vc, vc + nvc);
+}
+// Runs `ticks` ticks; returns mean over ticks and cells of (V(soma[c]) - target[t][c])^2 and,
+// when trace is not NULL, V(soma[c]) after each tick (ticks x ncells).
+double oracle_cable64_loss(void* h, float dt, uint32_t ticks, const uint32_t* soma, uint32_t ncells,
+                           const float* target, double* trace) {
+    orc::Cab64& s = *static_cast<orc::Cab64*>(h);
+    double acc = 0.0;
+    for (uint32_t t = 0; t < ticks; t++) {
+        orc::cab_tick(s, dt);
+        for (uint32_t c = 0; c < ncells; c++) {
+            const double v = s.V[soma[c]];
+            if (trace) trace[(size_t)t * ncells + c] = v;
+            const double e = v - (double)target[(size_t)t * ncells + c];
+            acc += e * e;
+        }
+    }
+    return acc / ((double)ticks * ncells);
+}
+void oracle_cable64_destroy(void* h) { delete static_cast<orc::Cab64*>(h); }
 float oracle_bernoulli(float x) { return orc::bernoulli(x); }
 
 }  // extern "C"

@@ -72,8 +72,48 @@ def oracle_lib():
         lib.oracle_xorshift.argtypes = [C.c_uint32]
         lib.oracle_bernoulli.restype = C.c_float
         lib.oracle_bernoulli.argtypes = [C.c_float]
+        lib.oracle_set_cable.argtypes = [P, C.c_int, P, C.c_uint32]
         _oracle = lib
     return _oracle
+
+
+class Cable64:
+    """The oracle's FP64 restatement of the cable-mode tick (held ions, phi_e = 0) of one network
+    or a concatenation of networks; scale: per-connection conductance scale."""
+
+    def __init__(self, segments, connections, scale=None, current=None, ramp=None, vclamp=None):
+        lib = oracle_lib()
+        if not hasattr(lib, "_cable64"):
+            lib.oracle_cable64_create.restype = P
+            lib.oracle_cable64_create.argtypes = [P, C.c_uint32, P, C.c_uint32, P]
+            lib.oracle_cable64_set_stimuli.argtypes = [P, P, C.c_uint32, P, C.c_uint32, P, C.c_uint32]
+            lib.oracle_cable64_loss.restype = C.c_double
+            lib.oracle_cable64_loss.argtypes = [P, C.c_float, C.c_uint32, P, C.c_uint32, P, P]
+            lib.oracle_cable64_destroy.argtypes = [P]
+            lib._cable64 = True
+        self.lib = lib
+        self._keep = [np.ascontiguousarray(segments), np.ascontiguousarray(connections)]
+        sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
+        self._keep.append(sc)
+        self.h = lib.oracle_cable64_create(_ptr(self._keep[0]), len(segments), _ptr(self._keep[1]),
+                                           len(connections), _ptr(sc))
+        st = [np.ascontiguousarray(a) if a is not None else None for a in (current, ramp, vclamp)]
+        self._keep += st
+        lib.oracle_cable64_set_stimuli(self.h, *[x for a in st for x in (_ptr(a), 0 if a is None else len(a))])
+
+    def __del__(self):
+        try:
+            self.lib.oracle_cable64_destroy(self.h)
+        except Exception:
+            pass
+
+    def loss(self, dt, ticks, soma, target, trace=False):
+        soma = np.ascontiguousarray(soma, dtype=np.uint32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        tr = np.zeros((ticks, len(soma))) if trace else None
+        L = self.lib.oracle_cable64_loss(self.h, dt, ticks, _ptr(soma), len(soma), target.ctypes.data_as(P),
+                                         tr.ctypes.data_as(P) if trace else None)
+        return (L, tr) if trace else L
 
 
 def ref_available() -> bool:
@@ -101,6 +141,12 @@ class OracleSim:
             self.lib.oracle_destroy(self.ctx)
         except Exception:
             pass
+
+    def set_cable(self, on: bool, scale=None):
+        """Cable mode (held concentrations, phi_e = 0) and per-connection coupling scale."""
+        self._scale = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
+        self.lib.oracle_set_cable(self.ctx, 1 if on else 0, _ptr(self._scale),
+                                  0 if scale is None else len(self._scale))
 
     def step(self, dt: float, stim: Stimuli | None = None):
         n = self.net
