@@ -200,6 +200,55 @@ def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004):
             "batched_vs_plain": batched / plain, "autotune_ticks": tuned, "schedule": sched}
 
 
+def adjoint_cost(n_cells, ticks, every, dt=0.025, seed=0xB10C0005, graphs=True):
+    """The adjoint fit: n_cells cells of 200 segments (soma + branched dendrite; Nav1.2, Kv-DR,
+    K2P, NALCN, HCN1) in one batch, cable mode, a 50 pA soma current step, loss = MSE of the
+    somatic V_m against the trace of the same cells with every g_max scaled by U(0.8, 1.25).
+    Forward: `ticks` ticks replayed from captured hipGraphs. Adjoint: bchip_adjoint with
+    checkpoints every `every` ticks (forward pass + loss, then per span a recorded recompute and
+    the reverse kernel), gradients for every density and coupling scale."""
+    rng = np.random.default_rng(seed)
+    cells = [bion.config5_cell(seed + k, n_seg=200) for k in range(n_cells)]
+    pert = [c.copy() for c in cells]
+    for c in pert:
+        c.segments["modifiers"]["density"][:, :5] *= rng.uniform(0.8, 1.25, (len(c.segments), 5))
+    soma, off = [], 0
+    for c in cells:
+        soma.append(off + int(np.nonzero(c.segments["segment_type"] == 1)[0][0]))
+        off += len(c.segments)
+    soma = np.array(soma, np.uint32)
+
+    def sim_of(nets):
+        s = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
+        if s.load_batch([n.copy() for n in nets]) != 0:
+            raise SystemExit("adjoint workload load failed")
+        s.set_cable_mode(True)
+        s.set_graphs(graphs)
+        for k, n in enumerate(nets):
+            s.set_stimuli(k, current=bion.soma_current(n, 50.0))
+        return s
+    s = sim_of(pert)
+    _, _, _, target = s.adjoint(dt, ticks, every, soma, np.zeros((ticks, n_cells), np.float32), grads=False,
+                                trace=True)
+    s.close()
+    s = sim_of(cells)
+    s.run_ticks(dt, 32)   # capture and warm the graphs, then time from the same start state
+    s.close()
+    s = sim_of(cells)
+    t0 = time.perf_counter()
+    s.run_ticks(dt, ticks)
+    fwd = time.perf_counter() - t0
+    s.close()
+    s = sim_of(cells)
+    t0 = time.perf_counter()
+    L, gd, gs = s.adjoint(dt, ticks, every, soma, target)
+    adj = time.perf_counter() - t0
+    s.close()
+    return {"cells": n_cells, "segments": off, "ticks": ticks, "checkpoint_every": every,
+            "forward_s": fwd, "adjoint_s": adj, "adjoint_vs_forward": adj / fwd, "loss": L,
+            "grad_density_norm": float(np.linalg.norm(gd)), "grad_scale_norm": float(np.linalg.norm(gs))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +260,12 @@ def main():
                     help="networks in the batched-vs-plain comparison (512 = the full batched sweep)")
     ap.add_argument("--batch-ticks", type=int, default=64)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain comparison")
+    ap.add_argument("--adjoint-cells", type=int, default=64)
+    ap.add_argument("--adjoint-ticks", type=int, default=20000)
+    ap.add_argument("--adjoint-every", type=int, default=500)
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward cost")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="tick by tick instead of captured hipGraphs in the adjoint leg (for kernel tracing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 1 GiB device-to-device copies (the reference kernel for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -230,6 +285,10 @@ def main():
         sys.exit(2)
     if args.batch_only:
         print(json.dumps(batched_vs_plain(args.batch_networks, args.batch_ticks)), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_cost(args.adjoint_cells, args.adjoint_ticks, args.adjoint_every,
+                                      graphs=not args.no_graphs)), flush=True)
         return
     t0 = time.time()
     net, stims = build_workload(args.neurons, args.warmup + args.steps)
@@ -283,6 +342,8 @@ def main():
     }
     if args.batch_networks > 0:
         out["batched"] = batched_vs_plain(args.batch_networks, args.batch_ticks)
+    if args.adjoint_cells > 0:
+        out["adjoint"] = adjoint_cost(args.adjoint_cells, args.adjoint_ticks, args.adjoint_every)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

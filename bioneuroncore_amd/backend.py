@@ -44,7 +44,7 @@ EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence",
                "bchip_modifier_supported", "bchip_load_batch", "bchip_network_count", "bchip_set_stimuli",
                "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
                "bchip_set_graphs", "bchip_set_jacobi", "bchip_set_cable_mode", "bchip_set_coupling_scale",
-               "bchip_adjoint")
+               "bchip_adjoint", "bchip_set_adjoint_memory")
 
 
 class BionDataC(C.Structure):
@@ -122,7 +122,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_set_jacobi.argtypes = [P, C.c_double, C.c_int]
     lib.bchip_set_cable_mode.argtypes = [P, C.c_int]
     lib.bchip_set_coupling_scale.argtypes = [P, P, C.c_uint32]
-    lib.bchip_adjoint.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P]
+    lib.bchip_adjoint.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P, P]
+    lib.bchip_set_adjoint_memory.argtypes = [P, C.c_uint64]
     _lib = lib
     return lib
 
@@ -263,8 +264,10 @@ class Simulation:
         if rc != 0:
             raise ValueError("bad coupling scale")
 
-    def adjoint(self, dt: float, ticks: int, checkpoint_every: int, soma, target, grads: bool = True):
-        """Cable-mode loss and gradients: (loss, d loss/d density [n, 32], d loss/d coupling scale [nc]).
+    def adjoint(self, dt: float, ticks: int, checkpoint_every: int, soma, target, grads: bool = True,
+                trace: bool = False):
+        """Cable-mode loss and gradients: (loss, d loss/d density [n, 32], d loss/d coupling scale [nc]),
+        plus the somatic V_m trace (ticks, cells) when trace=True.
         soma: host segment index per cell; target: (ticks, cells) V_m after each tick."""
         soma = np.ascontiguousarray(soma, dtype=np.uint32)
         target = np.ascontiguousarray(target, dtype=np.float32)
@@ -274,12 +277,18 @@ class Simulation:
         n, nc = len(b.segments), len(b.connections)
         gd = np.zeros((n, 32)) if grads else None
         gs = np.zeros(max(nc, 1)) if grads else None
+        tr = np.zeros((ticks, len(soma)), np.float32) if trace else None
         rc = self.lib.bchip_adjoint(self.h, dt, ticks, checkpoint_every, soma.ctypes.data, len(soma),
                                     target.ctypes.data, C.byref(loss), gd.ctypes.data if grads else None,
-                                    gs.ctypes.data if grads else None)
+                                    gs.ctypes.data if grads else None, tr.ctypes.data if trace else None)
         if rc != 0:
             raise RuntimeError(f"bchip_adjoint failed ({rc})")
-        return loss.value, gd, (gs[:nc] if grads else None)
+        out = (loss.value, gd, (gs[:nc] if grads else None))
+        return out + (tr,) if trace else out
+
+    def set_adjoint_memory(self, max_bytes: int) -> None:
+        """Cap on the adjoint's recorded-iterate memory (0: the reverse pass recomputes substeps)."""
+        self.lib.bchip_set_adjoint_memory(self.h, max_bytes)
 
     def set_graphs(self, on: bool) -> None:
         self.lib.bchip_set_graphs(self.h, 1 if on else 0)

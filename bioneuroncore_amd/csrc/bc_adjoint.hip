@@ -217,16 +217,17 @@ struct AdjNet {
     double* lamG;            // adjoint of the gate states [slot][2][n]
     double* gdens;           // d loss / d density [slot][n] (accumulated)
     double* gscale;          // d loss / d connection scale [device connection] (accumulated)
-    float* xs;               // scratch: 51 Jacobi iterates per segment of every tile [tile][51][TS]
-    uint32_t tile0;          // first tile of this size class (scratch offset)
+    const float* xrec;       // Jacobi iterates the forward recorded for this tick [5][51][npad],
+                             // or NULL: the kernel runs the substeps again
 };
 
 // One tick of the reverse pass for one tile of cells (lane = segment = connection index).
 template <int TS>
 __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a, uint64_t tick, float dt) {
-    __shared__ double sV[5][TS];       // V at the start of each substep
+    __shared__ float xs[51 * TS];      // Jacobi iterates x^0 .. x^50 of one substep
+    __shared__ double sV0[TS];         // V before the tick (start of the first substep)
+    __shared__ float sVf[4][TS];       // V at the start of substeps 1-4
     __shared__ double sLam[TS];        // adjoint exchanged between lanes
-    __shared__ float sMu[TS];
     __shared__ double sDinp[TS];
     __shared__ float sG[TS], sCt[TS], sDV[TS], sInp1[TS];
     __shared__ uint32_t sAB[TS];
@@ -242,7 +243,6 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
     const uint32_t nent = d.csr_off[s0 + nseg] - E0;
     uint32_t e0 = 0, e1 = 0;
     if (act) { e0 = d.csr_off[g] - E0; e1 = d.csr_off[g + 1] - E0; }
-    float* xs = a.xs + (size_t)(a.tile0 + tile) * 51 * TS;
 
     // ---- recompute step 1 (held ions) and keep the channels' partials ----
     double V = act ? a.recV[g] : 0.0;
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
         }
     }
 
-    // ---- recompute step 2.5 in cable mode: axial terms per connection, V per substep ----
+    // ---- step 2.5 in cable mode: axial terms per connection, V per substep ----
     for (uint32_t e = t; e < nent; e += TS) {
         uint32_t en = d.csr_ent[E0 + e];
         uint32_t c = en >> 1;
@@ -318,7 +318,6 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
         ca_ = ab & 0xFFFF;
         cb_ = ab >> 16;
         cintra = d.conn_intra[c0 + t] != 0;
-        sIntra[t] = cintra;
         float ra = d.radius[s0 + ca_], rb = d.radius[s0 + cb_];
         float rmin = ra < rb ? ra : rb;
         float cac = kPi * rmin * rmin;
@@ -337,83 +336,94 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
     const float gax = G * csc;
     float cdt = cap / dts;
     if (cdt < 1e-9f) cdt = 1e-9f;
+    if (cact) sG[t] = cintra ? gax : 0.0f;
+    if (act) sV0[t] = V;
     __syncthreads();
     float gdiag = 0.0f;
-    if (cact) sG[t] = cintra ? gax : 0.0f;
-    __syncthreads();
     if (act)
         for (uint32_t e = e0; e < e1; e++) {
             const uint32_t en = sEnt[e];
             if ((en >> 1) & 1) gdiag += sG[en >> 2];
         }
     const float den = cdt + gdiag;
-    // forward of one substep from sV[sub]: axial terms into sCt/sDinp/sDV/sInp1; returns the
-    // Jacobi iterates in xs (x^0 .. x^50) when `keep`, and the new V
-    auto substep = [&](int sub, bool keep) -> float {
+    // V of segment i at the start of substep sub, as the forward's dV reads it (the pre-tick V
+    // in double for the first substep, the float iterate after)
+    auto vstart = [&](int sub, uint32_t i) -> double { return sub == 0 ? sV0[i] : (double)sVf[sub - 1][i]; };
+    // axial terms of substep sub (sCt for the forward, sDinp/sDV/sInp1 for the reverse)
+    auto conn_terms = [&](int sub) {
         if (cact) {
-            float inp = 0.0f, dV = 0.0f;
-            double dinp = 0.0;   // d inp / d dV in double: inp - G dV nearly cancels
+            // inp as the forward sums it (each flux times the scale), inp1 unscaled
+            float inp = 0.0f, inp1 = 0.0f, dV = 0.0f;
+            double dinp = 0.0;   // d inp1 / d dV in double: inp - G dV nearly cancels
             if (cintra) {
-                dV = (sV[sub][ca_] + 0.0) - (sV[sub][cb_] + 0.0);
+                dV = vstart(sub, ca_) - vstart(sub, cb_);
                 for (int k = 0; k < 7; k++) {
                     float ca = d.in[k * P + s0 + ca_], cb = d.in[k * P + s0 + cb_];
                     float x = kZ[k] * kFrtMv * dV;
                     float fl = kin[k] * (ca * bernoulli(-x, ExpTabGlobal{}) - cb * bernoulli(x, ExpTabGlobal{}));
                     const double dfl = (double)kin[k] * (-ca * adj::bern_d(-x) - cb * adj::bern_d(x)) * kZ[k] * kFrtMv;
-                    inp += kZ[k] * kChargeToPa * fl / dts;
+                    inp1 += kZ[k] * kChargeToPa * fl / dts;
+                    inp += kZ[k] * kChargeToPa * (fl * csc) / dts;
                     dinp += (double)kZ[k] * kChargeToPa * dfl / dts;
                 }
             }
-            sInp1[t] = inp;
+            sInp1[t] = inp1;
             sDinp[t] = dinp * csc;
             sDV[t] = dV;
-            sCt[t] = inp * csc - gax * dV;
+            sCt[t] = inp - gax * dV;
         }
+    };
+    // the forward's 50 sweeps of substep sub into xs (x^0 .. x^50); ends with a barrier
+    auto sweeps = [&](int sub) {
+        conn_terms(sub);
         __syncthreads();
         float inpd = 0.0f, vold = 0.0f;
         if (act) {
             for (uint32_t e = e0; e < e1; e++) {
                 const uint32_t en = sEnt[e];
                 if (!((en >> 1) & 1)) continue;
-                const uint32_t c = en >> 2;
-                if (en & 1) inpd -= sCt[c]; else inpd += sCt[c];
+                if (en & 1) inpd -= sCt[en >> 2]; else inpd += sCt[en >> 2];
             }
-            vold = (float)sV[sub][t];
-            sMu[t] = vold;
-            if (keep) xs[t] = vold;
+            vold = (float)vstart(sub, t);
+            xs[t] = vold;
         }
         __syncthreads();
         const float base = cdt * vold + inpd;
-        float v = vold;
         for (int it = 0; it < 50; it++) {
-            float off = 0.0f;
-            if (act)
+            if (act) {
+                float off = 0.0f;
                 for (uint32_t e = e0; e < e1; e++) {
                     const uint32_t en = sEnt[e];
                     if (!((en >> 1) & 1)) continue;
                     const uint32_t c = en >> 2, ab = sAB[c];
                     const uint32_t o = (en & 1) ? (ab & 0xFFFF) : (ab >> 16);
-                    off += sG[c] * sMu[o];
+                    off += sG[c] * xs[it * TS + o];
                 }
-            __syncthreads();
-            if (act) {
-                v = (base + off) / den;
-                sMu[t] = v;
-                if (keep) xs[(size_t)(it + 1) * TS + t] = v;
+                xs[(it + 1) * TS + t] = (base + off) / den;
             }
             __syncthreads();
         }
-        return v;
     };
-    if (act) sV[0][t] = V;
-    __syncthreads();
-    for (int sub = 0; sub < 5; sub++) {
-        float v = substep(sub, false);
-        if (sub < 4) {
-            if (act) sV[sub + 1][t] = v;
+    // the recorded iterates of substep sub into xs; ends with a barrier
+    auto load_iterates = [&](int sub) {
+        for (int it = 0; it <= 50; it++)
+            xs[it * TS + t] = act ? a.xrec[((size_t)sub * 51 + it) * P + g] : 0.0f;
+        __syncthreads();
+    };
+    if (a.xrec) {   // substep start values from the recording
+        for (int sub = 1; sub < 5; sub++)
+            if (act) sVf[sub - 1][t] = a.xrec[((size_t)(sub - 1) * 51 + 50) * P + g];
+        if (act) V = a.xrec[((size_t)4 * 51 + 50) * P + g];
+        __syncthreads();
+    } else {        // ... or from running the substeps again
+        for (int sub = 0; sub < 5; sub++) {
+            sweeps(sub);
+            if (sub < 4) {
+                if (act) sVf[sub][t] = xs[50 * TS + t];
+            } else if (act) {
+                V = xs[50 * TS + t];   // V after step 2.5
+            }
             __syncthreads();
-        } else {
-            V = v;   // V after step 2.5
         }
     }
 
@@ -438,7 +448,12 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
     // ---- reverse: step 2.5, substeps in reverse ----
     double lam_s = 0.0;   // this connection lane's scale gradient
     for (int sub = 4; sub >= 0; sub--) {
-        substep(sub, true);   // axial terms of this substep and its iterates in xs
+        if (a.xrec) {
+            load_iterates(sub);
+            conn_terms(sub);
+        } else {
+            sweeps(sub);
+        }
         // transposed sweeps: lam holds d loss / d x^{k+1}
         double lb = 0.0, lg = 0.0;
         for (int it = 49; it >= 0; it--) {
@@ -457,8 +472,8 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
                 }
             }
             if (cact && cintra) {   // d x^{k+1}_i / d g = (x^k_other - x^{k+1}_i) / D_i at both ends
-                const float* xk = xs + (size_t)it * TS;
-                const float* xk1 = xs + (size_t)(it + 1) * TS;
+                const float* xk = xs + it * TS;
+                const float* xk1 = xs + (it + 1) * TS;
                 lg += sLam[ca_] * ((double)xk[cb_] - xk1[ca_]) + sLam[cb_] * ((double)xk[ca_] - xk1[cb_]);
             }
             __syncthreads();
@@ -470,7 +485,7 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
         double lam_dv = 0.0;
         if (cact && cintra) {
             const double lct = sLam[ca_] - sLam[cb_];   // inpd_a += cterm, inpd_b -= cterm
-            lam_dv = lct * (sDinp[t] - (double)gax);    // cterm = s * inp(dV) - s * G * dV
+            lam_dv = lct * (sDinp[t] - (double)gax);     // cterm = s * inp(dV) - s * G * dV
             lg += -lct * sDV[t];
             lam_s += lg * G + lct * sInp1[t];
         }
@@ -519,13 +534,15 @@ __global__ __launch_bounds__(256) void k_adj_record(Net d, double* recV, double*
     for (uint32_t j = 0; j < d.nslot; j++)
         for (int q = 0; q < 2; q++) recG[((size_t)j * 2 + q) * P + i] = d.mstate[((size_t)j * 7 + q) * P + i];
 }
-// Loss terms of one tick: acc[c] += (V(soma of cell c) - target[c])^2.
+// Loss terms of one tick: acc[c] += (V(soma of cell c) - target[c])^2; trace[c] = that V.
 __global__ __launch_bounds__(256) void k_adj_loss(const double* V, const uint32_t* soma, uint32_t ncells,
-                                                  const float* target, double* acc) {
+                                                  const float* target, double* acc, float* trace) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
-    const double e = V[soma[c]] - (double)target[c];
+    const double v = V[soma[c]];
+    const double e = v - (double)target[c];
     acc[c] += e * e;
+    if (trace) trace[c] = (float)v;
 }
 // d loss / d V(after the tick) at the somas: coef * (V - target).
 __global__ __launch_bounds__(256) void k_adj_seed(const double* V, const uint32_t* soma, uint32_t ncells,

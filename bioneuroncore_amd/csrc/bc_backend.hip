@@ -241,7 +241,8 @@ struct Sim {
     BchipSchedule graph_sched{};
     bool use_graphs = true;
     bool cable = false;               // bchip_set_cable_mode
-    float* conn_scale_dev = nullptr;  // bchip_set_coupling_scale (device order), in dmem
+    float* conn_scale_dev = nullptr;
+    uint64_t adj_record_cap = ~0ull;   // bchip_set_adjoint_memory  // bchip_set_coupling_scale (device order), in dmem
     bool jac_res = false;   // residual-terminated Jacobi (bchip_set_jacobi)
     double jac_tol = 0.0;
     int jac_max = 50;
@@ -1097,9 +1098,10 @@ static int load_into(Sim& S, BionData* src_owned) {
 
 // Reverse pass over `ticks` cable-mode ticks from the current state (see bchip_adjoint).
 static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
-                       uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale) {
+                       uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
+                       float* trace) {
     if (!soma || !target || ticks == 0 || ncells == 0) return -1;
-    if (!S.loaded || !S.cable || S.d.ns) return -1;
+    if (!S.loaded || !S.cable || S.d.ns || S.jac_res) return -1;
     Net& d = S.d;
     const uint32_t n = d.n, P = d.npad, nslot = d.nslot;
     if (nslot > 8) return -2;
@@ -1149,11 +1151,15 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
         HIPCHK(hipMemcpyAsync(dtarget, target, (size_t)ticks * ncells * 4, hipMemcpyHostToDevice, S.stream));
         const uint32_t gc = (ncells + 255) / 256;
         // forward: checkpoints every K ticks, loss after every tick
+        float* dtrace = trace ? static_cast<float*>(alloc((size_t)ticks * ncells * 4)) : nullptr;
         for (uint32_t t = 0; t < ticks; t++) {
             if (t % K == 0) save(t / K);
             launch_tick(S, dt, false);
-            k_adj_loss<<<gc, 256, 0, S.stream>>>(d.V, dsoma, ncells, dtarget + (size_t)t * ncells, acc);
+            k_adj_loss<<<gc, 256, 0, S.stream>>>(d.V, dsoma, ncells, dtarget + (size_t)t * ncells, acc,
+                                                 dtrace ? dtrace + (size_t)t * ncells : nullptr);
         }
+        if (dtrace)
+            HIPCHK(hipMemcpyAsync(trace, dtrace, (size_t)ticks * ncells * 4, hipMemcpyDeviceToHost, S.stream));
         save(nck);   // the state after the last tick, restored at the end
         // recomputing a span also counts its spikes again, rewrites I_buffer and re-applies a
         // ligand injection of its first tick: keep those fields as the forward left them
@@ -1185,12 +1191,15 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
             a.lamG = static_cast<double*>(alloc(gB));
             a.gdens = static_cast<double*>(alloc((size_t)nslot * P * 8));
             a.gscale = static_cast<double*>(alloc((size_t)std::max<uint32_t>(nconn, 1) * 8));
-            size_t xs_off[kTileClasses], xs_total = 0;
-            for (int c = 0; c < kTileClasses; c++) {
-                xs_off[c] = xs_total;
-                xs_total += (size_t)S.class_ntiles[c] * 51 * kTileSize[c];
-            }
-            float* xs = static_cast<float*>(alloc(xs_total * 4));
+            // The forward records each tick's Jacobi iterates (5 x 51 floats per segment) when
+            // a span's worth fits in half the free memory; otherwise the reverse kernel runs
+            // the substeps again from the tick's recorded start state.
+            const size_t xr_tick = (size_t)5 * 51 * P;
+            size_t mfree = 0, mtotal = 0;
+            HIPCHK(hipMemGetInfo(&mfree, &mtotal));
+            float* xrec = nullptr;
+            const size_t xr_bytes = (size_t)K * xr_tick * 4;
+            if (xr_bytes <= mfree / 2 && xr_bytes <= S.adj_record_cap) xrec = static_cast<float*>(alloc(xr_bytes));
             const double coef = 2.0 / ((double)ticks * ncells);
             const uint32_t gs = (n + 255) / 256;
             for (int seg = (int)nck - 1; seg >= 0; seg--) {
@@ -1199,7 +1208,9 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
                 for (uint32_t t = c0; t < c1; t++) {
                     double* r = rec + (size_t)(t - c0) * (P + (size_t)nslot * 2 * P);
                     k_adj_record<<<gs, 256, 0, S.stream>>>(d, r, r + P);
+                    d.xrec = xrec ? xrec + (size_t)(t - c0) * xr_tick : nullptr;
                     launch_tick(S, dt, false);
+                    d.xrec = nullptr;
                 }
                 double* rl = rec + (size_t)(c1 - c0) * (P + (size_t)nslot * 2 * P);
                 k_adj_record<<<gs, 256, 0, S.stream>>>(d, rl, rl + P);
@@ -1209,10 +1220,9 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
                     const double* pre = rec + (size_t)(t - c0) * (P + (size_t)nslot * 2 * P);
                     a.recV = pre;
                     a.recG = pre + P;
+                    a.xrec = xrec ? xrec + (size_t)(t - c0) * xr_tick : nullptr;
                     for (int c = 0; c < kTileClasses; c++) {
                         if (!S.class_ntiles[c]) continue;
-                        a.xs = xs + xs_off[c];
-                        a.tile0 = 0;
                         if (c == 0) k_adjoint_tick<64><<<S.class_ntiles[0], 64, 0, S.stream>>>(d, S.tiles[0], a, t, dt);
                         if (c == 1) k_adjoint_tick<128><<<S.class_ntiles[1], 128, 0, S.stream>>>(d, S.tiles[1], a, t, dt);
                         if (c == 2) k_adjoint_tick<256><<<S.class_ntiles[2], 256, 0, S.stream>>>(d, S.tiles[2], a, t, dt);
@@ -1240,6 +1250,7 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
     } catch (...) {
         rc = -1;
     }
+    S.d.xrec = nullptr;
     for (void* p : tmp) (void)hipFree(p);
     return rc;
 }
@@ -1680,10 +1691,17 @@ int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n) {
 }
 
 int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
-                  uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale) {
+                  uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
+                  float* trace) {
     if (!sim) return -1;
     return bc::adjoint_run(*static_cast<Sim*>(sim), dt, ticks, checkpoint_every, soma, ncells, target, loss,
-                           grad_density, grad_scale);
+                           grad_density, grad_scale, trace);
+}
+
+int bchip_set_adjoint_memory(void* sim, uint64_t max_bytes) {
+    if (!sim) return -1;
+    static_cast<Sim*>(sim)->adj_record_cap = max_bytes;
+    return 0;
 }
 
 int bchip_set_graphs(void* sim, int on) {

@@ -509,6 +509,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
             }
             vold = (float)vcur;
             L.u.i.vf[0][t] = vold;
+            if (d.xrec) d.xrec[(size_t)sub * 51 * P + g] = vold;   // the adjoint's recording
         }
         tile_sync<TS>();
         float vnew = vold;
@@ -590,6 +591,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                         }
                         float nv = div_wide(base + off, den, rden);
                         vi = act ? nv : vi;
+                        if (d.xrec && act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
                     }
                     vnew = vi;
                     return;
@@ -612,6 +614,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     }
                     vnew = div_wide(base + off, den, rden);
                     L.u.i.vf[vc ^ 1][t] = vnew;
+                    if (d.xrec) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = vnew;
                 }
                 tile_sync<TS>();
                 vc ^= 1;

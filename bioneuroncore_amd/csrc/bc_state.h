@@ -101,6 +101,7 @@ struct Net {
     // multiplies the axial conductance and the intracellular ion fluxes of each connection.
     uint32_t cable;
     const float* conn_scale;
+    float* xrec;         // cable mode: the Jacobi iterates of the tick [5][51][npad] (NULL: none)
 
     // segment dynamic state
     double* V;

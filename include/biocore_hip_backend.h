@@ -131,10 +131,18 @@ int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n);
 // is recomputed once for the reverse pass. grad_density: num_segments x 32 (d loss / d density
 // of modifier slot j of segment h); grad_scale: num_connections (d loss / d coupling scale).
 // With both NULL only the forward pass and the loss run. The simulation ends `ticks` ticks
-// later, as after bchip_run_ticks. Returns 0, -1 (arguments or state: cable mode off,
-// synapses present), -2 (a modifier type outside Nav1.2/Kv-DR/K2P/NALCN/HCN1, or > 8 slots).
+// later, as after bchip_run_ticks. trace (ticks x ncells, may be NULL) receives the somatic V_m
+// after each tick. The reverse pass reads the Jacobi iterates the forward recorded for the span
+// being reversed (5 x 51 floats per segment and tick) when checkpoint_every ticks of them fit in
+// half the free device memory, and otherwise runs each tick's substeps again. Returns 0, -1
+// (arguments or state: cable mode off, synapses present, residual-terminated Jacobi on), -2 (a
+// modifier type outside Nav1.2/Kv-DR/K2P/NALCN/HCN1, or more than 8 modifier slots).
 int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
-                  uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale);
+                  uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
+                  float* trace);
+// Upper bound on the device memory bchip_adjoint may use for recorded Jacobi iterates (bytes;
+// default unbounded, i.e. half the free memory). 0 makes the reverse pass run the substeps again.
+int bchip_set_adjoint_memory(void* sim, uint64_t max_bytes);
 
 #ifdef __cplusplus
 }

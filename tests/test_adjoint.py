@@ -182,11 +182,14 @@ def test_gradients_match_gpu_forward_differences():
         assert abs(a - f) < 2e-2 * abs(f)
 
 
-def test_checkpoint_interval_and_end_state():
+def test_checkpoint_interval_recording_and_end_state():
+    """Checkpoint intervals, and recorded iterates vs substeps run again in the reverse pass."""
     cells, scales = _cells(), _scales(_cells())
     ref = None
-    for K in (TICKS, 16, 7, 1):
+    for K, rec in ((TICKS, True), (16, True), (7, False), (1, True), (16, False)):
         sim = _gpu(cells, scales)
+        if not rec:
+            sim.set_adjoint_memory(0)
         L, gd, gs = sim.adjoint(DT, TICKS, K, _soma(cells), _target())
         ends = [sim.batch_read(k) for k in range(len(cells))]
         assert sim.tick_count() == TICKS
