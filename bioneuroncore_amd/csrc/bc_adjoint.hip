@@ -213,6 +213,8 @@ BC_DEV double bern_d(float x) {   // dB/dx of the SG Bernoulli function (the for
 struct AdjNet {
     const double* recV;      // recorded pre-tick V of the tick being reversed [n]
     const double* recG;      // recorded pre-tick gate states [slot][2][n]
+    const double* recIb;     // recorded I_buffer of the tick being reversed [n]
+    double* lamIb;           // d loss / d I_buffer of the tick being reversed [n]
     double* lamV;            // adjoint of V after the tick in, before the tick out [n]
     double* lamG;            // adjoint of the gate states [slot][2][n]
     double* gdens;           // d loss / d density [slot][n] (accumulated)
@@ -221,13 +223,16 @@ struct AdjNet {
                              // or NULL: the kernel runs the substeps again
 };
 
-// One tick of the reverse pass for one tile of cells (lane = segment = connection index).
+// One tick of the reverse pass through steps 4 and 2.5 for one tile of cells (lane = segment =
+// connection index).
 template <int TS>
 __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a, uint64_t tick, float dt) {
     __shared__ float xs[51 * TS];      // Jacobi iterates x^0 .. x^50 of one substep
     __shared__ double sV0[TS];         // V before the tick (start of the first substep)
     __shared__ float sVf[4][TS];       // V at the start of substeps 1-4
     __shared__ double sLam[TS];        // adjoint exchanged between lanes
+    __shared__ double sMu2[2 * TS];    // lam / D of the transposed sweeps, two planes
+    __shared__ float muh[50 * TS];     // lam / D of every sweep of the substep
     __shared__ double sDinp[TS];
     __shared__ float sG[TS], sCt[TS], sDV[TS], sInp1[TS];
     __shared__ uint32_t sAB[TS];
@@ -244,61 +249,19 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
     uint32_t e0 = 0, e1 = 0;
     if (act) { e0 = d.csr_off[g] - E0; e1 = d.csr_off[g + 1] - E0; }
 
-    // ---- recompute step 1 (held ions) and keep the channels' partials ----
+    // ---- step 4 inputs: pre-tick V, this tick's I_buffer (recorded), clamps ----
     double V = act ? a.recV[g] : 0.0;
-    adj::SegAd sa;
-    double ibuf = 0.0, area = 0.0;
-    float cap = 1.0f, volf = 1.0f;
-    if (act) {
-        for (int k = 0; k < 3; k++) { sa.in[k] = d.in[k * P + g]; sa.ex[k] = d.ex[k * P + g]; }
-        area = d.area[g];
-        sa.area = area;
-        sa.pkc = d.pkc[g]; sa.pka = d.pka[g]; sa.camp = d.camp[g];
-        cap = d.cap[g];
-        volf = (float)d.vol[g];
-        if (volf < 1e-6f) volf = 1e-6f;
-    }
-    // the channels' partials are evaluated twice (value here, partials in the step-1 reverse)
-    // rather than kept: eight slots of them would not fit in registers
-    const uint32_t cnt = act ? d.mcount[g] : 0;
-    const bool my = act && (d.sflags[g] & SF_MYELIN);
-    const double V0 = V;
-    auto slot_ad = [&](uint32_t j, bool& live) -> adj::ChanAd {
-        const uint8_t ty = d.mtype[j * P + g];
-        live = !(my && ty != 4 && myelin_blocked(ty));
-        double st[7];
-        for (int q = 0; q < 7; q++) st[q] = d.mstate[((size_t)j * 7 + q) * P + g];
-        st[0] = a.recG[((size_t)j * 2 + 0) * P + g];
-        st[1] = a.recG[((size_t)j * 2 + 1) * P + g];
-        return adj::channel_ad(ty, V0, st, sa, d.mdens[j * P + g], dt);
-    };
-    for (uint32_t j = 0; j < cnt; j++) {
-        bool live;
-        const adj::ChanAd c = slot_ad(j, live);
-        if (live) ibuf += c.r;
-    }
+    const double ibuf = act ? a.recIb[g] : 0.0;
+    const float cap = act ? d.cap[g] : 1.0f;
     bool vclamped = false;
     const uint32_t sh = (act && d.stim.any) ? d.stim.head[g] : ~0u;
-    if (sh != ~0u) {   // current pulses and ramps add to I_buffer; an active clamp fixes V
+    if (sh != ~0u) {   // an active voltage clamp fixes V after step 4
         const uint32_t sc = d.stim.ent[sh];
         for (uint32_t e = 1; e <= sc; e++) {
             const uint32_t en = d.stim.ent[sh + e];
-            const uint32_t kind = en >> 28, x = en & 0x0FFFFFFFu;
-            if (kind == SK_CUR) {
-                const CurrentStimulus q = d.stim.cur[x];
-                uint64_t dur = (uint64_t)(q.duration_ms / dt);
-                if (tick >= q.start_tick && tick < q.start_tick + dur) ibuf += q.amplitude_pA;
-            } else if (kind == SK_RAMP) {
-                const RampStimulus q = d.stim.ramp[x];
-                if (tick >= q.start_tick && tick < q.end_tick) {
-                    float frac = (q.end_tick > q.start_tick)
-                                     ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick) : 1.0f;
-                    ibuf += q.start_pA + (q.end_pA - q.start_pA) * frac;
-                }
-            } else if (kind == SK_VCLAMP) {
-                const VoltageClamp q = d.stim.vc[x];
-                if (tick >= q.start_tick && tick < q.end_tick) vclamped = true;
-            }
+            if ((en >> 28) != SK_VCLAMP) continue;
+            const VoltageClamp q = d.stim.vc[en & 0x0FFFFFFFu];
+            if (tick >= q.start_tick && tick < q.end_tick) vclamped = true;
         }
     }
 
@@ -405,9 +368,12 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
         }
     };
     // the recorded iterates of substep sub into xs; ends with a barrier
-    auto load_iterates = [&](int sub) {
-        for (int it = 0; it <= 50; it++)
-            xs[it * TS + t] = act ? a.xrec[((size_t)sub * 51 + it) * P + g] : 0.0f;
+    auto load_iterates = [&](int sub) {   // all 51 loads in flight before the first store
+        float r[51];
+#pragma unroll
+        for (int it = 0; it <= 50; it++) r[it] = act ? a.xrec[((size_t)sub * 51 + it) * P + g] : 0.0f;
+#pragma unroll
+        for (int it = 0; it <= 50; it++) xs[it * TS + t] = r[it];
         __syncthreads();
     };
     if (a.xrec) {   // substep start values from the recording
@@ -446,6 +412,32 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
     }
 
     // ---- reverse: step 2.5, substeps in reverse ----
+    // The first kRegNb intracellular neighbours of each lane stay in registers for the
+    // transposed sweeps (g and lane; padding reads the lane itself with g = 0), the rest is
+    // walked from the incidence list.
+    constexpr int kRegNb = 4;
+    double nbg[kRegNb];
+    uint32_t nbo[kRegNb];
+#pragma unroll
+    for (int q = 0; q < kRegNb; q++) { nbg[q] = 0.0; nbo[q] = t; }
+    uint32_t e_rest = e1;
+    {
+        int nn = 0;
+        if (act)
+            for (uint32_t e = e0; e < e1; e++) {
+                const uint32_t en = sEnt[e];
+                if (!((en >> 1) & 1)) continue;
+                if (nn == kRegNb) { e_rest = e; break; }
+                const uint32_t c = en >> 2, ab = sAB[c];
+                const double gq = sG[c];
+                const uint32_t oq = (en & 1) ? (ab & 0xFFFF) : (ab >> 16);
+#pragma unroll
+                for (int q = 0; q < kRegNb; q++)
+                    if (q == nn) { nbg[q] = gq; nbo[q] = oq; }
+                nn++;
+            }
+    }
+    const double rden = 1.0 / (double)den;
     double lam_s = 0.0;   // this connection lane's scale gradient
     for (int sub = 4; sub >= 0; sub--) {
         if (a.xrec) {
@@ -454,31 +446,36 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
         } else {
             sweeps(sub);
         }
-        // transposed sweeps: lam holds d loss / d x^{k+1}
+        // transposed sweeps: lam holds d loss / d x^{k+1}; mu = lam / D alternates between two
+        // LDS planes, so one barrier per sweep suffices, and is kept (as float) for the
+        // conductance terms summed after the sweeps
         double lb = 0.0, lg = 0.0;
-        for (int it = 49; it >= 0; it--) {
-            const double mu = act ? lam / den : 0.0;
-            if (act) sLam[t] = mu;
+        for (int it = 49, pl = 0; it >= 0; it--, pl ^= 1) {
+            double* mus = sMu2 + pl * TS;
+            const double mu = act ? lam * rden : 0.0;
+            mus[t] = mu;
+            muh[it * TS + t] = (float)mu;
             __syncthreads();
             double nl = 0.0;
-            if (act) {
-                lb += mu;
-                for (uint32_t e = e0; e < e1; e++) {
-                    const uint32_t en = sEnt[e];
-                    if (!((en >> 1) & 1)) continue;
-                    const uint32_t c = en >> 2, ab = sAB[c];
-                    const uint32_t o = (en & 1) ? (ab & 0xFFFF) : (ab >> 16);
-                    nl += sG[c] * sLam[o];
-                }
+#pragma unroll
+            for (int q = 0; q < kRegNb; q++) nl += nbg[q] * mus[nbo[q]];
+            for (uint32_t e = e_rest; e < e1; e++) {
+                const uint32_t en = sEnt[e];
+                if (!((en >> 1) & 1)) continue;
+                const uint32_t c = en >> 2, ab = sAB[c];
+                nl += (double)sG[c] * mus[(en & 1) ? (ab & 0xFFFF) : (ab >> 16)];
             }
-            if (cact && cintra) {   // d x^{k+1}_i / d g = (x^k_other - x^{k+1}_i) / D_i at both ends
-                const float* xk = xs + it * TS;
-                const float* xk1 = xs + (it + 1) * TS;
-                lg += sLam[ca_] * ((double)xk[cb_] - xk1[ca_]) + sLam[cb_] * ((double)xk[ca_] - xk1[cb_]);
-            }
-            __syncthreads();
+            lb += mu;
             lam = nl;
         }
+        __syncthreads();   // the last sweep's reads of the mu planes are done
+        if (cact && cintra)   // d x^{k+1}_i / d g = (x^k_other - x^{k+1}_i) / D_i at both ends
+            for (int it = 0; it < 50; it++) {
+                const float* xk = xs + it * TS;
+                const float* xk1 = xs + (it + 1) * TS;
+                lg += (double)muh[it * TS + ca_] * ((double)xk[cb_] - xk1[ca_]) +
+                      (double)muh[it * TS + cb_] * ((double)xk[ca_] - xk1[cb_]);
+            }
         // x^0 = V(sub start) and base = cdt * V + inpd, so d loss / d inpd = lb
         if (act) sLam[t] = lb;
         __syncthreads();
@@ -504,33 +501,56 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
         __syncthreads();
     }
 
-    // ---- reverse: step 1 ----
-    if (act) {
-        for (uint32_t j = 0; j < cnt; j++) {
-            bool live;
-            const adj::ChanAd c = slot_ad(j, live);
-            if (!live) continue;
-            const double lr = lam_ibuf;
-            lam += lr * c.dr[0];
-            a.gdens[(size_t)j * P + g] += lr * c.r1;
-            for (int q = 0; q < c.ng; q++) {
-                double* lg = &a.lamG[((size_t)j * 2 + q) * P + g];
-                const double lgn = *lg + lr * c.dr[1 + q];
-                lam += lgn * c.dg[q][0];
-                *lg = lgn * c.dg[q][1];
-            }
-        }
+    if (act) {   // d loss / d V before the tick through step 2.5; the channels' part follows
         a.lamV[g] = lam;
+        a.lamIb[g] = lam_ibuf;
     }
     if (cact) a.gscale[c0 + t] += lam_s;
 }
 
-// Records the pre-tick V and gate states (states 0 and 1 of every slot) of one tick.
-__global__ __launch_bounds__(256) void k_adj_record(Net d, double* recV, double* recG) {
+// One tick of the reverse pass through step 1 (the channels), per segment: the gating ODEs and
+// GHK currents at the tick's pre-tick V and gates, their partials from dual numbers.
+__global__ __launch_bounds__(256) void k_adjoint_channels(Net d, AdjNet a, float dt) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.n) return;
+    const size_t P = d.npad;
+    adj::SegAd sa;
+    for (int k = 0; k < 3; k++) { sa.in[k] = d.in[k * P + g]; sa.ex[k] = d.ex[k * P + g]; }
+    sa.area = d.area[g];
+    sa.pkc = d.pkc[g]; sa.pka = d.pka[g]; sa.camp = d.camp[g];
+    const double V0 = a.recV[g];
+    const double lr = a.lamIb[g];
+    double lam = a.lamV[g];
+    const uint32_t cnt = d.mcount[g];
+    const bool my = d.sflags[g] & SF_MYELIN;
+    for (uint32_t j = 0; j < cnt; j++) {
+        const uint8_t ty = d.mtype[j * P + g];
+        if (my && ty != 4 && myelin_blocked(ty)) continue;
+        double st[7];
+        for (int q = 0; q < 7; q++) st[q] = d.mstate[((size_t)j * 7 + q) * P + g];
+        st[0] = a.recG[((size_t)j * 2 + 0) * P + g];
+        st[1] = a.recG[((size_t)j * 2 + 1) * P + g];
+        const adj::ChanAd c = adj::channel_ad(ty, V0, st, sa, d.mdens[j * P + g], dt);
+        lam += lr * c.dr[0];
+        a.gdens[(size_t)j * P + g] += lr * c.r1;
+        for (int q = 0; q < c.ng; q++) {
+            double* lg = &a.lamG[((size_t)j * 2 + q) * P + g];
+            const double lgn = *lg + lr * c.dr[1 + q];
+            lam += lgn * c.dg[q][0];
+            *lg = lgn * c.dg[q][1];
+        }
+    }
+    a.lamV[g] = lam;
+}
+
+// Records the pre-tick V and gate states (states 0 and 1 of every slot) of one tick and, when
+// recIb is not NULL, the I_buffer the previous tick computed.
+__global__ __launch_bounds__(256) void k_adj_record(Net d, double* recV, double* recG, double* recIb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n) return;
     const size_t P = d.npad;
     recV[i] = d.V[i];
+    if (recIb) recIb[i] = d.ibuf[i];
     for (uint32_t j = 0; j < d.nslot; j++)
         for (int q = 0; q < 2; q++) recG[((size_t)j * 2 + q) * P + i] = d.mstate[((size_t)j * 7 + q) * P + i];
 }

@@ -1185,10 +1185,13 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
         L /= (double)ticks * ncells;
         if (loss) *loss = L;
         if (backward) {
-            double* rec = static_cast<double*>(alloc((size_t)(K + 1) * (vB + gB)));
+            // per tick of a span: pre-tick V, gate states, and the tick's I_buffer
+            const size_t rstride = (size_t)2 * P + (size_t)nslot * 2 * P;
+            double* rec = static_cast<double*>(alloc((size_t)(K + 1) * rstride * 8));
             AdjNet a{};
             a.lamV = static_cast<double*>(alloc(vB));
             a.lamG = static_cast<double*>(alloc(gB));
+            a.lamIb = static_cast<double*>(alloc(vB));
             a.gdens = static_cast<double*>(alloc((size_t)nslot * P * 8));
             a.gscale = static_cast<double*>(alloc((size_t)std::max<uint32_t>(nconn, 1) * 8));
             // The forward records each tick's Jacobi iterates (5 x 51 floats per segment) when
@@ -1205,21 +1208,26 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
             for (int seg = (int)nck - 1; seg >= 0; seg--) {
                 const uint32_t c0 = (uint32_t)seg * K, c1 = std::min(ticks, c0 + K);
                 restore((uint32_t)seg);
+                // record j: V, gates before tick c0 + j; I_buffer of tick c0 + j (written by
+                // the record of the tick after)
+                auto recp = [&](uint32_t j) { return rec + (size_t)j * rstride; };
+                auto ibp = [&](uint32_t j) { return recp(j) + P + (size_t)nslot * 2 * P; };
                 for (uint32_t t = c0; t < c1; t++) {
-                    double* r = rec + (size_t)(t - c0) * (P + (size_t)nslot * 2 * P);
-                    k_adj_record<<<gs, 256, 0, S.stream>>>(d, r, r + P);
+                    double* r = recp(t - c0);
+                    k_adj_record<<<gs, 256, 0, S.stream>>>(d, r, r + P, t > c0 ? ibp(t - c0 - 1) : nullptr);
                     d.xrec = xrec ? xrec + (size_t)(t - c0) * xr_tick : nullptr;
                     launch_tick(S, dt, false);
                     d.xrec = nullptr;
                 }
-                double* rl = rec + (size_t)(c1 - c0) * (P + (size_t)nslot * 2 * P);
-                k_adj_record<<<gs, 256, 0, S.stream>>>(d, rl, rl + P);
+                double* rl = recp(c1 - c0);
+                k_adj_record<<<gs, 256, 0, S.stream>>>(d, rl, rl + P, ibp(c1 - c0 - 1));
                 for (uint32_t t = c1; t-- > c0;) {
-                    const double* post = rec + (size_t)(t + 1 - c0) * (P + (size_t)nslot * 2 * P);
+                    const double* post = recp(t + 1 - c0);
                     k_adj_seed<<<gc, 256, 0, S.stream>>>(post, dsoma, ncells, dtarget + (size_t)t * ncells, coef, a.lamV);
-                    const double* pre = rec + (size_t)(t - c0) * (P + (size_t)nslot * 2 * P);
+                    const double* pre = recp(t - c0);
                     a.recV = pre;
                     a.recG = pre + P;
+                    a.recIb = ibp(t - c0);
                     a.xrec = xrec ? xrec + (size_t)(t - c0) * xr_tick : nullptr;
                     for (int c = 0; c < kTileClasses; c++) {
                         if (!S.class_ntiles[c]) continue;
@@ -1227,6 +1235,7 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
                         if (c == 1) k_adjoint_tick<128><<<S.class_ntiles[1], 128, 0, S.stream>>>(d, S.tiles[1], a, t, dt);
                         if (c == 2) k_adjoint_tick<256><<<S.class_ntiles[2], 256, 0, S.stream>>>(d, S.tiles[2], a, t, dt);
                     }
+                    k_adjoint_channels<<<gs, 256, 0, S.stream>>>(d, a, dt);
                 }
                 HIPCHK(hipGetLastError());
             }
