@@ -954,19 +954,21 @@ static void stim_after_tick(Sim& S) {
 }
 
 // ------------------------------------------------------------------ tick
-template <bool kRes>
+template <bool kRes, bool kRec = false>
 static void launch_diffusion(Sim& S, float dt) {
     const Net& d = S.d;
     auto lds = [](size_t tile, uint32_t ts, uint32_t w) { return w * (tile + (kRes ? 2 * ts * 8 : 0)); };
     if (S.class_ntiles[0]) {
         const uint32_t W = S.sched.diffusion_tiles;
-        k_diffusion<64, kRes><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, lds(sizeof(DiffLds<64>), 64, W), S.stream>>>(
-            d, S.tiles[0], dt);
+        k_diffusion<64, kRes, kRec><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, lds(sizeof(DiffLds<64>), 64, W),
+                                      S.stream>>>(d, S.tiles[0], dt);
     }
     if (S.class_ntiles[1])
-        k_diffusion<128, kRes><<<S.class_ntiles[1], 128, lds(sizeof(DiffLds<128>), 128, 1), S.stream>>>(d, S.tiles[1], dt);
+        k_diffusion<128, kRes, kRec><<<S.class_ntiles[1], 128, lds(sizeof(DiffLds<128>), 128, 1), S.stream>>>(
+            d, S.tiles[1], dt);
     if (S.class_ntiles[2])
-        k_diffusion<256, kRes><<<S.class_ntiles[2], 256, lds(sizeof(DiffLds<256>), 256, 1), S.stream>>>(d, S.tiles[2], dt);
+        k_diffusion<256, kRes, kRec><<<S.class_ntiles[2], 256, lds(sizeof(DiffLds<256>), 256, 1), S.stream>>>(
+            d, S.tiles[2], dt);
 }
 
 // The kernel sequence of one tick on S.stream (no host synchronisation or copies, so it can be
@@ -980,6 +982,7 @@ static void launch_tick(Sim& S, float dt, bool tm) {
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
         if (S.jac_res) launch_diffusion<true>(S, dt);
+        else if (d.xrec) launch_diffusion<false, true>(S, dt);
         else launch_diffusion<false>(S, dt);
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));

@@ -228,7 +228,8 @@ __device__ __attribute__((always_inline)) inline bool tile_all(bool pred) {
 // kRes: residual-terminated sweeps (an extension; the reference sweeps 50 times). Each island
 // (cell) stops after the first sweep whose largest update is <= d.jac_tol, or after d.jac_max
 // sweeps; the islands' residuals are max-reduced in LDS beyond the tile's DiffLds.
-template <int TS, bool kRes>
+// kRec: cable mode for the adjoint, which records every Jacobi iterate in d.xrec.
+template <int TS, bool kRes, bool kRec = false>
 __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffusion(Net d, TileSet ts, float dt) {
     // 64-lane tiles: each wavefront of the workgroup owns its own tile and LDS slice.
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -509,7 +510,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
             }
             vold = (float)vcur;
             L.u.i.vf[0][t] = vold;
-            if (d.xrec) d.xrec[(size_t)sub * 51 * P + g] = vold;   // the adjoint's recording
+            if (kRec) d.xrec[(size_t)sub * 51 * P + g] = vold;   // the adjoint's recording
         }
         tile_sync<TS>();
         float vnew = vold;
@@ -591,7 +592,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                         }
                         float nv = div_wide(base + off, den, rden);
                         vi = act ? nv : vi;
-                        if (d.xrec && act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
+                        if (kRec && act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
                     }
                     vnew = vi;
                     return;
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     }
                     vnew = div_wide(base + off, den, rden);
                     L.u.i.vf[vc ^ 1][t] = vnew;
-                    if (d.xrec) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = vnew;
+                    if (kRec) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = vnew;
                 }
                 tile_sync<TS>();
                 vc ^= 1;
