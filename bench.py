@@ -200,6 +200,32 @@ def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004):
             "batched_vs_plain": batched / plain, "autotune_ticks": tuned, "schedule": sched}
 
 
+def electrodiffusion_stress(n_cables, ticks, dt=0.025, tol=1e-6, max_sweeps=200):
+    """The electrodiffusion stress network: n_cables unbranched 100-segment passive cables with
+    linear +-20% ramps of all ions, ATP and temperature; residual-terminated Jacobi (tol mV, at
+    most max_sweeps sweeps) in both potential solves; ticks replayed from captured hipGraphs.
+    Its per-field error against the FP64 restatement after 500 ticks is a test
+    (tests/test_residual_jacobi.py). Model bytes: 273 B per segment (dynamic state read and
+    written, statics read) + 12 B per connection."""
+    net = bion.config3(n_cables=n_cables, seg_per_cable=100)
+    n_seg, n_conn = len(net.segments), len(net.connections)
+    s = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
+    if s.load_network(net) != 0:
+        raise SystemExit("electrodiffusion stress load failed")
+    del net
+    s.set_jacobi(tol, max_sweeps)
+    s.run_ticks(dt, 16)
+    t0 = time.perf_counter()
+    s.run_ticks(dt, ticks)
+    el = time.perf_counter() - t0
+    s.close()
+    model = 273 * n_seg + 12 * n_conn
+    return {"segments": n_seg, "connections": n_conn, "ticks": ticks, "jacobi_tol_mV": tol,
+            "max_sweeps": max_sweeps, "ms_per_tick": el / ticks * 1e3,
+            "segment_ticks_per_s": n_seg * ticks / el, "model_bytes_per_tick": model,
+            "model_GBps": model * ticks / el / 1e9}
+
+
 def adjoint_cost(n_cells, ticks, every, dt=0.025, seed=0xB10C0005, graphs=True):
     """The adjoint fit: n_cells cells of 200 segments (soma + branched dendrite; Nav1.2, Kv-DR,
     K2P, NALCN, HCN1) in one batch, cable mode, a 50 pA soma current step, loss = MSE of the
@@ -260,6 +286,9 @@ def main():
                     help="networks in the batched-vs-plain comparison (512 = the full batched sweep)")
     ap.add_argument("--batch-ticks", type=int, default=64)
     ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain comparison")
+    ap.add_argument("--stress-cables", type=int, default=10_000,
+                    help="100-segment cables in the electrodiffusion stress leg (0: skip)")
+    ap.add_argument("--stress-ticks", type=int, default=500)
     ap.add_argument("--adjoint-cells", type=int, default=64)
     ap.add_argument("--adjoint-ticks", type=int, default=20000)
     ap.add_argument("--adjoint-every", type=int, default=500)
@@ -342,6 +371,8 @@ def main():
     }
     if args.batch_networks > 0:
         out["batched"] = batched_vs_plain(args.batch_networks, args.batch_ticks)
+    if args.stress_cables > 0:
+        out["electrodiffusion"] = electrodiffusion_stress(args.stress_cables, args.stress_ticks)
     if args.adjoint_cells > 0:
         out["adjoint"] = adjoint_cost(args.adjoint_cells, args.adjoint_ticks, args.adjoint_every)
     if not args.no_cpu:
