@@ -44,7 +44,8 @@ EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence",
                "bchip_modifier_supported", "bchip_load_batch", "bchip_network_count", "bchip_set_stimuli",
                "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
                "bchip_set_graphs", "bchip_set_jacobi", "bchip_set_cable_mode", "bchip_set_coupling_scale",
-               "bchip_adjoint", "bchip_set_adjoint_memory")
+               "bchip_adjoint", "bchip_set_adjoint_memory", "bchip_read_bion", "bchip_free_bion",
+               "bchip_write_bion")
 
 
 class BionDataC(C.Structure):

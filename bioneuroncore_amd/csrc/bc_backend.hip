@@ -30,128 +30,9 @@ namespace bc {
         }                                                                                \
     } while (0)
 
-// ------------------------------------------------------------------ BION v14 file I/O
-static void free_bion(BionData* b) {
-    delete[] b->neurons;
-    delete[] b->segments;
-    delete[] b->connections;
-    delete[] b->synapses;
-    delete[] reinterpret_cast<uint8_t*>(b->nuclei);
-    memset(b, 0, sizeof(BionData));
-}
-
-// Follows the reference's host loader (src/bion_loader_host.cpp:26-107, 358-425): same error
-// codes, same validation, same intra_connection_ids fill. Only the current v14 record sizes
-// are accepted; older versions return BION_ERR_BAD_VERSION.
-static int read_bion(const char* path, BionData* b) {
-    memset(b, 0, sizeof(BionData));
-    FILE* f = fopen(path, "rb");
-    if (!f) return BION_ERR_FILE_OPEN;
-    NetworkHeader h;
-    memset(&h, 0, sizeof(h));
-    if (fread(&h, 68, 1, f) != 1) { fclose(f); return BION_ERR_FILE_READ; }
-    if (h.version >= 5) {
-        size_t ext = h.version >= 10 ? (sizeof(NetworkHeader) - 68) : 16;
-        if (fread(reinterpret_cast<uint8_t*>(&h) + 68, ext, 1, f) != 1) { fclose(f); return BION_ERR_FILE_READ; }
-    }
-    if (h.magic != BION_MAGIC_U32) { fclose(f); return BION_ERR_BAD_MAGIC; }
-    if (h.version != BION_VERSION_CURRENT ||
-        (h.file_sz_segment && h.file_sz_segment != sizeof(Segment)) ||
-        (h.file_sz_synapse && h.file_sz_synapse != sizeof(Synapse)) ||
-        (h.file_sz_intraconnection && h.file_sz_intraconnection != sizeof(IntraConnection)) ||
-        (h.file_sz_neuron && h.file_sz_neuron != sizeof(Neuron))) {
-        fclose(f);
-        return BION_ERR_BAD_VERSION;
-    }
-    if (h.num_segments == 0) { fclose(f); return BION_ERR_ZERO_SEGS; }
-    b->header = h;
-    b->num_neurons = h.num_neurons;
-    b->num_segments = h.num_segments;
-    b->num_connections = h.num_intra_connections;
-    b->num_synapses = h.num_synapses;
-    b->neurons = new (std::nothrow) Neuron[b->num_neurons ? b->num_neurons : 1];
-    b->segments = new (std::nothrow) Segment[b->num_segments];
-    b->connections = new (std::nothrow) IntraConnection[b->num_connections ? b->num_connections : 1];
-    b->synapses = new (std::nothrow) Synapse[b->num_synapses ? b->num_synapses : 1];
-    if (!b->neurons || !b->segments || !b->connections || !b->synapses) {
-        free_bion(b); fclose(f); return BION_ERR_ALLOC;
-    }
-    auto rd = [&](void* dst, size_t sz, size_t n) { return n == 0 || fread(dst, sz, n, f) == n; };
-    if (!rd(b->neurons, sizeof(Neuron), b->num_neurons) || !rd(b->segments, sizeof(Segment), b->num_segments) ||
-        !rd(b->connections, sizeof(IntraConnection), b->num_connections) ||
-        !rd(b->synapses, sizeof(Synapse), b->num_synapses)) {
-        free_bion(b); fclose(f); return BION_ERR_FILE_READ;
-    }
-    if (h.num_nuclei > 0) {
-        size_t nsz = h.file_sz_nucleus ? h.file_sz_nucleus : sizeof(Nucleus);
-        if (nsz == sizeof(Nucleus)) {
-            uint8_t* raw = new (std::nothrow) uint8_t[(size_t)h.num_nuclei * nsz];
-            if (raw && fread(raw, nsz, h.num_nuclei, f) == h.num_nuclei) {
-                b->nuclei = reinterpret_cast<Nucleus*>(raw);
-                b->num_nuclei = h.num_nuclei;
-            } else {
-                delete[] raw;
-            }
-        }
-    }
-    fclose(f);
-    for (uint32_t i = 0; i < b->num_connections; i++)
-        if (b->connections[i].segment_a >= b->num_segments || b->connections[i].segment_b >= b->num_segments) {
-            free_bion(b); return BION_ERR_BAD_CONN;
-        }
-    for (uint32_t i = 0; i < b->num_synapses; i++) {
-        const Synapse& y = b->synapses[i];
-        if (y.pre_segment >= b->num_segments || y.post_segment >= b->num_segments) { free_bion(b); return BION_ERR_BAD_SYNAPSE; }
-        if (y.pre_autoreceptor_count > BC_MAX_PRE_AUTORECEPTORS || y.post_receptor_count > BC_MAX_POST_RECEPTORS) {
-            free_bion(b); return BION_ERR_BAD_MODCOUNT;
-        }
-        if (y.type >= 3) { free_bion(b); return BION_ERR_BAD_SYNAPSE; }
-    }
-    for (uint32_t i = 0; i < b->num_neurons; i++)
-        if ((uint64_t)b->neurons[i].segment_start + b->neurons[i].segment_count > b->num_segments) {
-            free_bion(b); return BION_ERR_BAD_NEURON;
-        }
-    for (uint32_t i = 0; i < b->num_segments; i++)
-        if (b->segments[i].modifier_count > BC_MAX_MODIFIERS) { free_bion(b); return BION_ERR_BAD_MODCOUNT; }
-    for (uint32_t i = 0; i < b->num_segments; i++) b->segments[i].intra_connection_count = 0;
-    for (uint32_t i = 0; i < b->num_connections; i++) {
-        uint32_t a = b->connections[i].segment_a, c = b->connections[i].segment_b;
-        if (b->segments[a].intra_connection_count < BC_MAX_INTRACONN_PER_SEG)
-            b->segments[a].intra_connection_ids[b->segments[a].intra_connection_count++] = i;
-        if (b->segments[c].intra_connection_count < BC_MAX_INTRACONN_PER_SEG)
-            b->segments[c].intra_connection_ids[b->segments[c].intra_connection_count++] = i;
-    }
-    return 0;
-}
-
-static int write_bion(const char* path, const BionData* b) {   // bion_save, src/bion_loader_host.cpp:449-504
-    FILE* f = fopen(path, "wb");
-    if (!f) return BION_ERR_FILE_OPEN;
-    NetworkHeader h = b->header;
-    h.magic = BION_MAGIC_U32;
-    h.version = BION_VERSION_CURRENT;
-    h.num_neurons = b->num_neurons;
-    h.num_segments = b->num_segments;
-    h.num_intra_connections = b->num_connections;
-    h.num_synapses = b->num_synapses;
-    h.num_nuclei = b->nuclei ? b->num_nuclei : 0;
-    h.nucleus_version = 1;
-    h.file_sz_segment = sizeof(Segment);
-    h.file_sz_synapse = sizeof(Synapse);
-    h.file_sz_nucleus = sizeof(Nucleus);
-    h.file_sz_intraconnection = sizeof(IntraConnection);
-    h.file_sz_neuron = sizeof(Neuron);
-    h.file_sz_modifier = sizeof(Modifier);
-    bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
-    auto wr = [&](const void* p, size_t sz, size_t n) { if (ok && n) ok = fwrite(p, sz, n, f) == n; };
-    wr(b->neurons, sizeof(Neuron), b->num_neurons);
-    wr(b->segments, sizeof(Segment), b->num_segments);
-    wr(b->connections, sizeof(IntraConnection), b->num_connections);
-    wr(b->synapses, sizeof(Synapse), b->num_synapses);
-    if (b->nuclei) wr(b->nuclei, sizeof(Nucleus), b->num_nuclei);
-    fclose(f);
-    return ok ? 0 : BION_ERR_WRITE_FAIL;
-}
+}  // namespace bc
+#include "bc_bion.h"
+namespace bc {
 
 // ------------------------------------------------------------------ host-side static precompute
 static inline bool seg_sphere(const Segment& s) { return s.segment_type == 1 || s.segment_type == 3 || s.segment_type == 4; }
@@ -1418,6 +1299,13 @@ int bchip_load_bion_data(void* sim, const BionData* src) {
     memcpy(b.segments, src->segments, sizeof(Segment) * b.num_segments);
     if (b.num_connections) memcpy(b.connections, src->connections, sizeof(IntraConnection) * b.num_connections);
     if (b.num_synapses) memcpy(b.synapses, src->synapses, sizeof(Synapse) * b.num_synapses);
+    if (src->nuclei && src->num_nuclei) {
+        b.nuclei = new (std::nothrow) Nucleus[src->num_nuclei];
+        if (b.nuclei) {
+            memcpy(b.nuclei, src->nuclei, sizeof(Nucleus) * src->num_nuclei);
+            b.num_nuclei = src->num_nuclei;
+        }
+    }
     for (uint32_t i = 0; i < b.num_connections; i++)
         if (b.connections[i].segment_a >= b.num_segments || b.connections[i].segment_b >= b.num_segments) {
             bc::free_bion(&b); return BION_ERR_BAD_CONN;
@@ -1437,6 +1325,24 @@ int bchip_load_bion_data(void* sim, const BionData* src) {
         bc::free_bion(&b);
         return BION_ERR_ALLOC;
     }
+}
+
+int bchip_read_bion(const char* path, BionData* out) {
+    if (!path || !out) return -1;
+    try {
+        return bc::read_bion(path, out);
+    } catch (...) {
+        return BION_ERR_ALLOC;
+    }
+}
+
+void bchip_free_bion(BionData* data) {
+    if (data) bc::free_bion(data);
+}
+
+int bchip_write_bion(const char* path, const BionData* data) {
+    if (!path || !data || !data->segments) return -1;
+    return bc::write_bion(path, data);
 }
 
 int bchip_save_bion(void* sim, const char* path) {

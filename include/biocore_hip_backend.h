@@ -68,6 +68,12 @@ typedef struct BchipStats {
 
 int bchip_load_bion_data(void* sim, const BionData* src);   // load from in-memory AoS (copied)
 int bchip_save_bion(void* sim, const char* path);            // BION v14 save (bion_save layout)
+// Host-only BION file access (no device involved): bion_load_host (src/bion_loader_host.cpp:26),
+// reading versions 3-14 with the reference's record migrations; bion_unload_host (:428);
+// bion_save (:449, v14). Same error codes.
+int bchip_read_bion(const char* path, BionData* out);
+void bchip_free_bion(BionData* data);
+int bchip_write_bion(const char* path, const BionData* data);
 int bchip_set_coherence(void* sim, int mode);
 int bchip_push_host_state(void* sim);                        // upload AoS edits (FAST mode)
 int bchip_run_ticks(void* sim, float dt, uint32_t count);

@@ -195,9 +195,12 @@ def test_load_bion_file_and_error_codes(tmp_path):
     bad.write_bytes(b"\0" * 200)
     assert sim.load_bion(str(bad)) == -2
     raw = bytearray(open(p, "rb").read())
-    raw[4:8] = (13).to_bytes(4, "little")
+    raw[4:8] = (15).to_bytes(4, "little")
+    (tmp_path / "v15.bion").write_bytes(bytes(raw))
+    assert sim.load_bion(str(tmp_path / "v15.bion")) == -3
+    raw[4:8] = (13).to_bytes(4, "little")   # v13 header whose sizes name v14 records
     (tmp_path / "v13.bion").write_bytes(bytes(raw))
-    assert sim.load_bion(str(tmp_path / "v13.bion")) == -3
+    assert sim.load_bion(str(tmp_path / "v13.bion")) == 0
     out = str(tmp_path / "saved.bion")
     assert sim.load_bion(p) == 0
     sim.run_ticks(DT, 5)
