@@ -254,6 +254,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
 
     double ibuf = 0.0, diag = 0.0, sg = 0.0, vcur = 0.0;
     float area = 0.0f, cap = 0.0f, volf = 0.0f;
+    double rvol = 0.0;   // RN(1 / volf): divisions by volf below are div_f24 / div_wide
     uint32_t e0 = 0, e1 = 0;
     if (act) {
         vcur = d.V[g];
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
         cap = d.cap[g];
         volf = (float)d.vol[g];
         if (volf < 1e-6f) volf = 1e-6f;
+        rvol = 1.0 / (double)volf;
         e0 = d.csr_off[g] - E0;
         e1 = d.csr_off[g + 1] - E0;
     }
@@ -458,6 +460,8 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
     float kin[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) kin[k] = kDiff[k] * cac * dts / cdist;
+    const double rdts = 1.0 / (double)dts;                  // divisions by dts and cdist in the
+    const double rcdist = cact ? 1.0 / (double)cdist : 0.0;  // substeps are div_wide
     for (int sub = 0; sub < kSub; sub++) {
         if (cact && cintra) {
             const uint32_t a = ca_, b = cb_;
@@ -467,7 +471,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 float cbar = 0.5f * (L.ion[k][a] + L.ion[k][b]);
                 sgi += kZ[k] * kZ[k] * kDiff[k] * cbar;
             }
-            float gax = kSigmaE * sgi * cac / cdist;
+            float gax = div_wide(kSigmaE * sgi * cac, cdist, rcdist);
             const float csc = d.conn_scale ? d.conn_scale[c0 + t] : 1.0f;
             if (d.conn_scale) gax *= csc;
             L.u.i.cg[t] = gax;
@@ -482,7 +486,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 float fl = kin[k] * (ca * bb.m - cb * bb.p);
                 if (d.conn_scale) fl *= csc;
                 cfin[k * TS + t] = fl;
-                inp += kZ[k] * kChargeToPa * fl / dts;
+                inp += div_wide(kZ[k] * kChargeToPa * fl, dts, rdts);
             }
             L.u.i.cterm[t] = inp - gax * dV;
         }
@@ -501,11 +505,11 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 if (en & 1) {
                     inpd -= L.u.i.cterm[c];
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dacc[k] += (double)cfin[k * TS + c] / volf;
+                    for (int k = 0; k < 7; k++) dacc[k] += div_f24((double)cfin[k * TS + c], (double)volf, rvol);
                 } else {
                     inpd += L.u.i.cterm[c];
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dacc[k] -= (double)cfin[k * TS + c] / volf;
+                    for (int k = 0; k < 7; k++) dacc[k] -= div_f24((double)cfin[k * TS + c], (double)volf, rvol);
                 }
             }
             vold = (float)vcur;
@@ -584,12 +588,11 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
 #pragma unroll
                         for (int q = 0; q < D; q++)
                             jv[q] = __int_as_float(__builtin_amdgcn_ds_bpermute(ja[q], __float_as_int(vi)));
+                        // padded slots hold g = 0 and read the lane itself: off starts at +0, so
+                        // it is never -0 and adding 0 * v (v finite) leaves it unchanged
                         float off = 0.0f;
 #pragma unroll
-                        for (int q = 0; q < D; q++) {
-                            float s = off + jg[q] * jv[q];
-                            off = q < jdeg ? s : off;
-                        }
+                        for (int q = 0; q < D; q++) off = off + jg[q] * jv[q];
                         float nv = div_wide(base + off, den, rden);
                         vi = act ? nv : vi;
                         if (kRec && act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
@@ -670,15 +673,16 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
 #pragma unroll
             for (int k = 0; k < 7; k++) dx[k] = 0.0;
             const float ve = volf * kEvr;
+            const double rve = 1.0 / (double)ve;
             for (uint32_t e = e0; e < e1; e++) {
                 uint32_t en = L.ent[e];
                 uint32_t c = ent_conn(en);
                 if (en & 1) {
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dx[k] += L.u.flux[k][c] / ve;
+                    for (int k = 0; k < 7; k++) dx[k] += div_f24(L.u.flux[k][c], (double)ve, rve);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 7; k++) dx[k] -= L.u.flux[k][c] / ve;
+                    for (int k = 0; k < 7; k++) dx[k] -= div_f24(L.u.flux[k][c], (double)ve, rve);
                 }
             }
             // Only this lane reads its own ex during the gather; neighbours read cbuf.
@@ -716,25 +720,25 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     float dC = st[k * TS + a] - st[k * TS + b];
                     if (dC < 1e-12f && dC > -1e-12f) continue;
                     float fl = dcoef[base_sp + k] * ac * dC * dt / dist;
-                    if (role_b) dsp[k] += fl / volf;
-                    else dsp[k] -= fl / volf;
+                    if (role_b) dsp[k] += div_wide(fl, volf, rvol);
+                    else dsp[k] -= div_wide(fl, volf, rvol);
                 }
             }
             if (do_scal && intra) {
                 float d1 = st[15 * TS + a] - st[15 * TS + b];
                 if (d1 > 1e-12f || d1 < -1e-12f) {
                     float fl = 0.14f * ac * d1 * dt / dist;
-                    if (role_b) datp += fl / volf; else datp -= fl / volf;
+                    if (role_b) datp += div_wide(fl, volf, rvol); else datp -= div_wide(fl, volf, rvol);
                 }
                 float d2 = st[16 * TS + a] - st[16 * TS + b];
                 if (d2 > 0.001f || d2 < -0.001f) {
                     float fl = 0.0014f * ac * d2 * dt / dist;
-                    if (role_b) dtemp += fl / volf; else dtemp -= fl / volf;
+                    if (role_b) dtemp += div_wide(fl, volf, rvol); else dtemp -= div_wide(fl, volf, rvol);
                 }
                 float d3 = st[17 * TS + a] - st[17 * TS + b];
                 if (d3 > 1e-9f || d3 < -1e-9f) {
                     float fl = 0.01f * ac * d3 * dt / dist;
-                    if (role_b) dtxn += fl / volf; else dtxn -= fl / volf;
+                    if (role_b) dtxn += div_wide(fl, volf, rvol); else dtxn -= div_wide(fl, volf, rvol);
                 }
             }
         }

@@ -440,4 +440,25 @@ BC_HD float div_wide(float a, float b, double rb) {
     return r;
 }
 
+// a / b in double, correctly rounded, for a divisor b whose significand has at most 24 bits (a
+// widened float) and rb = RN(1 / b). Let a = A 2^x (A a 53-bit integer), b = B 2^y (B < 2^24)
+// and M a rounding midpoint (an odd 54-bit significand). a / b = M would need A 2^k =
+// (2j+1) B with 2j+1 >= 2^53 > odd(A): impossible; and a - M b is a nonzero multiple of the
+// lattice unit, so a / b lies at least 2^-76 (relative) from every midpoint. q0 = RN(a rb) is
+// within ~2^-52 of a / b and the corrected q0 + (a - b q0) rb within ~2^-104 as long as the
+// remainder a - b q0 (~2^-53 |a|) is not subnormal, so both round alike: the result is
+// RN(a / b) whenever it is a normal number and |a| >= 2^-968. Other cases (zero, subnormal,
+// infinite or NaN results, tiny numerators) take the IEEE division, which also fixes the sign
+// of a zero quotient.
+BC_HD double div_f24(double a, double b, double rb) {
+    const double q0 = a * rb;
+    const double q = __builtin_fma(__builtin_fma(-b, q0, a), rb, q0);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (!(__builtin_amdgcn_class(q, 0x108) && __builtin_fabs(a) >= 0x1p-968)) return keep_branch(a) / b;
+#else
+    if (!(__builtin_isnormal(q) && __builtin_fabs(a) >= 0x1p-968)) return a / b;
+#endif
+    return q;
+}
+
 }  // namespace bc

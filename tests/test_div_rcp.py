@@ -1,5 +1,5 @@
-"""bc::div_rcp and bc::div_wide (bioneuroncore_amd/csrc/bc_libm.h) return the IEEE quotient bit
-for bit: random float and double operands over wide exponent ranges (subnormal operands and
+"""bc::div_rcp, bc::div_wide and bc::div_f24 (bioneuroncore_amd/csrc/bc_libm.h) return the IEEE
+quotient bit for bit: random float and double operands over wide exponent ranges (subnormal operands and
 quotients near the underflow and overflow thresholds included), divisors with all-ones and
 near-power-of-two significands, and zero/subnormal/infinite edge cases."""
 import os
@@ -51,7 +51,27 @@ int main() {
         float w = bc::div_wide(a, b, 1.0 / (double)b);
         if (memcmp(&r, &w, 4) && !(std::isnan(r) && std::isnan(w))) bade++;
     }
-    printf("%ld %ld %ld %ld\n", badf, badd, bade, badw);
+    // div_f24: double numerators of any exponent, divisors that are widened floats
+    long bad24 = 0;
+    for (long i = 0; i < 40000000L; i++) {
+        uint64_t Ea = (i % 5 == 0) ? g() % 2047 : g() % 400 + 823, Ma = g() & 0xFFFFFFFFFFFFFull;
+        if (i % 7 == 0) Ma = (g() & 0xFFFFFFull) << 29;   // numerators that are floats too
+        uint64_t da = (Ea << 52) | Ma | ((g() & 1ull) << 63);
+        uint32_t eb = (uint32_t)(i % 4 == 0 ? g() % 255 : g() % 60 + 97);
+        uint32_t mb = (i % 3 == 0) ? 0x7FFFFF : (i % 3 == 1 ? (g() & 0xF) : (g() & 0x7FFFFF));
+        uint32_t bb = (eb << 23) | mb | ((g() & 1u) << 31);
+        double A; float bf; memcpy(&A, &da, 8); memcpy(&bf, &bb, 4);
+        const double B = bf;
+        double R = A / B, Q = bc::div_f24(A, B, 1.0 / B);
+        if (memcmp(&R, &Q, 8) && !(std::isnan(R) && std::isnan(Q))) bad24++;
+    }
+    const double ea24[] = {0.0, -0.0, 5e-324, -1e-310, INFINITY, -INFINITY, 1.7e308, 2.3e-308, 5.0};
+    const float eb24[] = {1.0f, -3.0f, 1e-30f, 7e30f, 1e-38f, 3e-39f, 0.0f, -0.0f, INFINITY, 1e-45f};
+    for (double a : ea24) for (float b : eb24) {
+        double R = a / (double)b, Q = bc::div_f24(a, (double)b, 1.0 / (double)b);
+        if (memcmp(&R, &Q, 8) && !(std::isnan(R) && std::isnan(Q))) bad24++;
+    }
+    printf("%ld %ld %ld %ld %ld\n", badf, badd, bade, badw, bad24);
 }
 '''
 
@@ -64,6 +84,6 @@ def test_div_rcp_is_ieee_division(tmp_path):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     subprocess.run([hipcc, "-O2", "-ffp-contract=off", "-I", os.path.join(ROOT, "bioneuroncore_amd", "csrc"),
                     "-o", str(exe), str(src)], check=True, capture_output=True)
-    badf, badd, bade, badw = map(int, subprocess.run([str(exe)], capture_output=True, text=True,
-                                                     check=True).stdout.split())
-    assert (badf, badd, bade, badw) == (0, 0, 0, 0)
+    badf, badd, bade, badw, bad24 = map(int, subprocess.run([str(exe)], capture_output=True, text=True,
+                                                            check=True).stdout.split())
+    assert (badf, badd, bade, badw, bad24) == (0, 0, 0, 0, 0)
