@@ -45,7 +45,7 @@ EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence",
                "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
                "bchip_set_graphs", "bchip_set_jacobi", "bchip_set_cable_mode", "bchip_set_coupling_scale",
                "bchip_adjoint", "bchip_set_adjoint_memory", "bchip_read_bion", "bchip_free_bion",
-               "bchip_write_bion")
+               "bchip_write_bion", "bchip_set_stimulus_convention")
 
 
 class BionDataC(C.Structure):
@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_set_coupling_scale.argtypes = [P, P, C.c_uint32]
     lib.bchip_adjoint.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P, P]
     lib.bchip_set_adjoint_memory.argtypes = [P, C.c_uint64]
+    lib.bchip_set_stimulus_convention.argtypes = [P, C.c_int]
     _lib = lib
     return lib
 
@@ -286,6 +287,11 @@ class Simulation:
             raise RuntimeError(f"bchip_adjoint failed ({rc})")
         out = (loss.value, gd, (gs[:nc] if grads else None))
         return out + (tr,) if trace else out
+
+    def set_stimulus_convention(self, convention: int) -> None:
+        """0: stimulus windows matched at the tick being run (CPU/CUDA); 1: one tick later (SYCL)."""
+        if self.lib.bchip_set_stimulus_convention(self.h, convention) != 0:
+            raise ValueError("convention must be 0 or 1")
 
     def set_adjoint_memory(self, max_bytes: int) -> None:
         """Cap on the adjoint's recorded-iterate memory (0: the reverse pass recomputes substeps)."""

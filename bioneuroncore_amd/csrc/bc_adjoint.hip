@@ -261,7 +261,7 @@ __global__ __launch_bounds__(TS) void k_adjoint_tick(Net d, TileSet ts, AdjNet a
             const uint32_t en = d.stim.ent[sh + e];
             if ((en >> 28) != SK_VCLAMP) continue;
             const VoltageClamp q = d.stim.vc[en & 0x0FFFFFFFu];
-            if (tick >= q.start_tick && tick < q.end_tick) vclamped = true;
+            if (tick + d.stim.tick_shift >= q.start_tick && tick + d.stim.tick_shift < q.end_tick) vclamped = true;
         }
     }
 

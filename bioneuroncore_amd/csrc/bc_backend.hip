@@ -123,7 +123,8 @@ struct Sim {
     bool use_graphs = true;
     bool cable = false;               // bchip_set_cable_mode
     float* conn_scale_dev = nullptr;
-    uint64_t adj_record_cap = ~0ull;   // bchip_set_adjoint_memory  // bchip_set_coupling_scale (device order), in dmem
+    uint64_t adj_record_cap = ~0ull;   // bchip_set_adjoint_memory
+    uint32_t stim_tick_shift = 0;      // bchip_set_stimulus_convention  // bchip_set_coupling_scale (device order), in dmem
     bool jac_res = false;   // residual-terminated Jacobi (bchip_set_jacobi)
     double jac_tol = 0.0;
     int jac_max = 50;
@@ -517,6 +518,7 @@ static int build_device(Sim& S) {
     S.stim_head = S.dalloc<uint32_t>(npad);
     HIPCHK(hipMemsetAsync(S.stim_head, 0xFF, (size_t)npad * 4, S.stream));
     memset(&d.stim, 0, sizeof(d.stim));
+    d.stim.tick_shift = S.stim_tick_shift;
     d.stim.head = S.stim_head;
     d.stim.inj_tick = ~0ull;
     S.id2dev.resize(n);
@@ -1614,6 +1616,14 @@ int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every
     if (!sim) return -1;
     return bc::adjoint_run(*static_cast<Sim*>(sim), dt, ticks, checkpoint_every, soma, ncells, target, loss,
                            grad_density, grad_scale, trace);
+}
+
+int bchip_set_stimulus_convention(void* sim, int convention) {
+    if (!sim || convention < 0 || convention > 1) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    S.stim_tick_shift = (uint32_t)convention;
+    S.d.stim.tick_shift = S.stim_tick_shift;
+    return 0;
 }
 
 int bchip_set_adjoint_memory(void* sim, uint64_t max_bytes) {

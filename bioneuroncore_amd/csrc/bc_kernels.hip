@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     // stimuli (simulation_host.cpp:247-300): this segment's list, in the reference's order
     const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
+    const uint64_t wtick = tick + d.stim.tick_shift;   // time windows; injections use the tick
     if (sh != ~0u) {
         const uint32_t cnt = d.stim.ent[sh];
         for (uint32_t e = 1; e <= cnt; e++) {
@@ -98,12 +99,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (kind == SK_CUR) {
                 const CurrentStimulus q = d.stim.cur[x];
                 uint64_t dur = (uint64_t)(q.duration_ms / dt);
-                if (tick >= q.start_tick && tick < q.start_tick + dur) itot += q.amplitude_pA;
+                if (wtick >= q.start_tick && wtick < q.start_tick + dur) itot += q.amplitude_pA;
             } else if (kind == SK_RAMP) {
                 const RampStimulus q = d.stim.ramp[x];
-                if (tick >= q.start_tick && tick < q.end_tick) {
+                if (wtick >= q.start_tick && wtick < q.end_tick) {
                     float frac = (q.end_tick > q.start_tick)
-                                     ? (float)(tick - q.start_tick) / (float)(q.end_tick - q.start_tick)
+                                     ? (float)(wtick - q.start_tick) / (float)(q.end_tick - q.start_tick)
                                      : 1.0f;
                     float amp = q.start_pA + (q.end_pA - q.start_pA) * frac;
                     itot += amp;
@@ -1046,7 +1047,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
     if (prev <= -40.0f && V > -40.0f) d.spikes[i]++;
     const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
     if (sh != ~0u) {   // voltage clamps in index order: the last active one wins
-        const uint64_t tick = *d.tick;
+        const uint64_t tick = *d.tick + d.stim.tick_shift;
         const uint32_t cnt = d.stim.ent[sh];
         for (uint32_t e = 1; e <= cnt; e++) {
             const uint32_t en = d.stim.ent[sh + e];

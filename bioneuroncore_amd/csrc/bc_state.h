@@ -75,6 +75,7 @@ struct StimDev {
     const LigandInjection* lig;
     uint64_t inj_tick;      // one-shot injections apply on this tick only
     uint32_t any;           // some segment has a list
+    uint32_t tick_shift;    // time windows are matched against tick + tick_shift (0: the CPU tick)
 };
 
 struct Net {

@@ -146,6 +146,12 @@ int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n);
 int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
                   uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
                   float* trace);
+// Tick at which stimulus time windows (current pulses, ramps, voltage clamps) are matched.
+// 0 (default): the tick being run, as the CPU tick (src/simulation_host.cpp:186,247-300) and the
+// CUDA backend (src/gpu_backend_cuda.cu:252-255) match them. 1: that tick plus one, as the SYCL
+// backend does (src/simulation.cpp:201,234), for callers that were built against it. One-shot
+// ion and ligand injections apply on the next tick either way.
+int bchip_set_stimulus_convention(void* sim, int convention);
 // Upper bound on the device memory bchip_adjoint may use for recorded Jacobi iterates (bytes;
 // default unbounded, i.e. half the free memory). 0 makes the reverse pass run the substeps again.
 int bchip_set_adjoint_memory(void* sim, uint64_t max_bytes);
