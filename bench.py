@@ -146,7 +146,7 @@ def cpu_baseline(budget_s=12.0, neurons=250):
                       f"{k} ticks in {el:.1f}s, same builder and stimuli as the GPU workload"}
 
 
-def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004):
+def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004, graphs=True):
     """The batched sweep: n_networks networks built like the 64-neuron chain network, segment
     counts log-uniform in 2k-200k, each with its own seed and g_max scaling and a 200 pA soma
     clamp. Plain: one simulation per network, ticked one network at a time (each tick is that
@@ -188,6 +188,7 @@ def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004):
     del nets
     for k, st in enumerate(stims):
         b.set_stimuli(k, current=st)
+    b.set_graphs(graphs)
     tuned = b.autotune(dt, 2)
     b.run_ticks(dt, 16)
     t0 = time.perf_counter()
@@ -200,7 +201,7 @@ def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004):
             "batched_vs_plain": batched / plain, "autotune_ticks": tuned, "schedule": sched}
 
 
-def electrodiffusion_stress(n_cables, ticks, dt=0.025, tol=1e-6, max_sweeps=200):
+def electrodiffusion_stress(n_cables, ticks, dt=0.025, tol=1e-6, max_sweeps=200, graphs=True):
     """The electrodiffusion stress network: n_cables unbranched 100-segment passive cables with
     linear +-20% ramps of all ions, ATP and temperature; residual-terminated Jacobi (tol mV, at
     most max_sweeps sweeps) in both potential solves; ticks replayed from captured hipGraphs.
@@ -214,6 +215,7 @@ def electrodiffusion_stress(n_cables, ticks, dt=0.025, tol=1e-6, max_sweeps=200)
         raise SystemExit("electrodiffusion stress load failed")
     del net
     s.set_jacobi(tol, max_sweeps)
+    s.set_graphs(graphs)
     s.run_ticks(dt, 16)
     t0 = time.perf_counter()
     s.run_ticks(dt, ticks)
@@ -289,12 +291,14 @@ def main():
     ap.add_argument("--stress-cables", type=int, default=10_000,
                     help="100-segment cables in the electrodiffusion stress leg (0: skip)")
     ap.add_argument("--stress-ticks", type=int, default=500)
+    ap.add_argument("--stress-only", action="store_true", help="only the electrodiffusion stress leg")
     ap.add_argument("--adjoint-cells", type=int, default=64)
     ap.add_argument("--adjoint-ticks", type=int, default=20000)
     ap.add_argument("--adjoint-every", type=int, default=500)
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward cost")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="tick by tick instead of captured hipGraphs in the adjoint leg (for kernel tracing)")
+                    help="tick by tick instead of captured hipGraphs in the batched, stress and adjoint "
+                         "legs (for kernel tracing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 1 GiB device-to-device copies (the reference kernel for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -313,7 +317,12 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.batch_only:
-        print(json.dumps(batched_vs_plain(args.batch_networks, args.batch_ticks)), flush=True)
+        print(json.dumps(batched_vs_plain(args.batch_networks, args.batch_ticks, graphs=not args.no_graphs)),
+              flush=True)
+        return
+    if args.stress_only:
+        print(json.dumps(electrodiffusion_stress(args.stress_cables, args.stress_ticks, graphs=not args.no_graphs)),
+              flush=True)
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_cost(args.adjoint_cells, args.adjoint_ticks, args.adjoint_every,
