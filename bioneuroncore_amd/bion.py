@@ -70,8 +70,9 @@ BION_VERSION = 14
 PUMP, AMPA, NMDA, GABAA = 7, 8, 9, 11
 SK, BK, K2P, KV7, KVDR, HCN1, NALCN = 103, 104, 107, 112, 123, 129, 133
 CAV21, NAV16, NAV12, KV31, KV11 = 140, 141, 142, 150, 156
+CX36, CX43 = 114, 198   # connexins (gap junctions read their connexon count and open probability)
 SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11)
+                       NAV16, NAV12, KV31, KV11, CX36, CX43)
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -79,11 +80,13 @@ EXTRA_IONS = np.array([140.0, 3.0, 1.2, 130.0, 2.0, 0.0, 4.0e-5, 0.0], dtype=np.
 
 # Canonical densities (reference include/cpg_config.h:16-76, tests/test_cpu_gpu_parity_cuda.cpp:246-247).
 CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, PUMP: 1500.0,
-                 CAV21: 100.0, HCN1: 120.0, SK: 6.0, BK: 0.15, KV11: 0.346, KV31: 2.0, KV7: 2.0}
+                 CAV21: 100.0, HCN1: 120.0, SK: 6.0, BK: 0.15, KV11: 0.346, KV31: 2.0, KV7: 2.0,
+                 CX36: 400.0, CX43: 60.0}
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
-             KV31: {0: 0.0, 1: 1.0}, KV7: {0: 0.1, 6: 1.0}, SK: {0: 0.0}, BK: {0: 0.0}}
+             KV31: {0: 0.0, 1: 1.0}, KV7: {0: 0.1, 6: 1.0}, SK: {0: 0.0}, BK: {0: 0.0},
+             CX36: {0: 0.8, 1: 0.1, 4: 1.0, 5: 7.2, 6: 0.2}, CX43: {0: 0.6, 4: 1.0, 5: 7.1}}
 
 
 @dataclasses.dataclass
@@ -287,6 +290,34 @@ def build_network(n_neurons: int, seg_per_neuron: int, morphology: str, seg_type
         else:
             raise ValueError(receptor_mix)
     return Network(neurons, segs, conns, syns)
+
+
+def gap_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons with connexins (Cx36 and Cx43 on somas) and AMPA synapses, plus electrical
+    synapses: soma-to-soma gap junctions between neighbouring neurons (type 1; every other one
+    with a delay, i.e. filtered) and dendritic bridges (type 2), interleaved in index order with
+    the chemical synapses."""
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN], [CX36, CX43],
+                        3.0, "ampa", 4, seed)
+    rng = np.random.default_rng(seed + 1)
+    e = []
+    for k in range(n_neurons - 1):
+        e.append((k * seg_per_neuron, (k + 1) * seg_per_neuron, 1, 0.5 if k % 2 else 0.0))
+        d = 1 + k % (seg_per_neuron - 1)
+        e.append(((k + 1) * seg_per_neuron + d, k * seg_per_neuron + d, 2, 0.0))
+    gj = np.zeros(len(e), SYNAPSE_DT)
+    gj["pre_segment"] = [x[0] for x in e]
+    gj["post_segment"] = [x[1] for x in e]
+    gj["type"] = [x[2] for x in e]
+    gj["delay"] = [x[3] for x in e]
+    gj["depression"] = rng.uniform(0.0, 0.3, len(e)).astype(np.float32)
+    gj["weight"] = 1.0
+    syn = np.concatenate([net.synapses, gj])
+    order = rng.permutation(len(syn))
+    syn = syn[order]
+    syn["id"] = np.arange(1, len(syn) + 1, dtype=np.uint32)
+    net.synapses = syn
+    return net
 
 
 def config1(seed: int = 0xB10C0001) -> Network:

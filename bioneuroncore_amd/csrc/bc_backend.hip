@@ -448,6 +448,7 @@ static int build_device(Sim& S) {
     S.unsupported_syn = 0;
     S.hazard_syn = 0;
     S.receptors_only = true;
+    uint32_t ngap = 0;
     for (uint32_t k = 0; k < ns; k++) {
         const Synapse& y = sy[S.slot2syn[k]];
         y_pre[k] = S.host2dev[y.pre_segment];
@@ -456,7 +457,7 @@ static int build_device(Sim& S) {
         y_prew[k] = (uint32_t)watch_idx[y_pre[k]];
         y_id[k] = y.id;
         uint8_t fl = 0;
-        if (y.type == 1 || y.type == 2) { fl |= YF_SKIP; S.unsupported_syn++; }
+        if (y.type == 1 || y.type == 2) { fl |= YF_SKIP | YF_GAP; ngap++; }
         if (y.pre_autoreceptor_count > 0) S.unsupported_syn++;
         for (uint32_t j = 0; j < y.post_receptor_count; j++) {
             uint8_t rt = y.post_receptors[j].type;
@@ -549,6 +550,7 @@ static int build_device(Sim& S) {
     d.pka = S.dupload(pka);
     d.camp = S.dupload(camp);
     d.pp1 = S.dupload(pp1);
+    d.camkii = S.dupload(camkii);
     d.sflags = S.dupload(sflags);
     d.cav21_slot = S.dupload(cav21);
     d.mcount = S.dupload(mcount);
@@ -614,6 +616,46 @@ static int build_device(Sim& S) {
     d.y_camkii = S.dupload(y_ck);
     d.y_calcn = S.dupload(y_cn);
     d.y_relp = S.dalloc<float>(ns);
+    // ---- gap junctions: post end per slot, and each segment's junctions as presynaptic end ----
+    d.y_post = nullptr; d.y_igap = nullptr; d.y_sid = nullptr;
+    d.gap_off = nullptr; d.gap_slot = nullptr; d.gap_sid = nullptr;
+    if (ngap) {
+        std::vector<uint32_t> y_post(ns), y_sid(ns), goff(n + 1, 0), gslot, gsid;
+        for (uint32_t k = 0; k < ns; k++) {
+            y_sid[k] = S.slot2syn[k];
+            y_post[k] = S.host2dev[sy[S.slot2syn[k]].post_segment];
+        }
+        // the first synapse index at which each segment's Ca can change in step 3 (a receptor
+        // other than AMPA or GABA-A on an incoming synapse)
+        std::vector<uint32_t> first_ca(n, UINT32_MAX);
+        for (uint32_t i = 0; i < ns; i++) {
+            const Synapse& y = sy[i];
+            if (y.type != 0) continue;
+            for (uint32_t j = 0; j < y.post_receptor_count; j++)
+                if (y.post_receptors[j].type != M_AMPA && y.post_receptors[j].type != M_GABAA) {
+                    uint32_t& f = first_ca[S.host2dev[y.post_segment]];
+                    f = std::min(f, i);
+                }
+        }
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> out(n);   // (synapse, slot)
+        for (uint32_t k = 0; k < ns; k++) {
+            if (!(y_flags[k] & YF_GAP)) continue;
+            const uint32_t i = S.slot2syn[k];
+            out[y_pre[k]].push_back({i, k});
+            if (first_ca[y_pre[k]] < i || first_ca[y_post[k]] < i) S.hazard_syn++;
+        }
+        for (uint32_t s = 0; s < n; s++) {
+            std::sort(out[s].begin(), out[s].end());
+            goff[s + 1] = goff[s] + (uint32_t)out[s].size();
+            for (auto& e : out[s]) { gsid.push_back(e.first); gslot.push_back(e.second); }
+        }
+        d.y_post = S.dupload(y_post);
+        d.y_sid = S.dupload(y_sid);
+        d.gap_off = S.dupload(goff);
+        d.gap_slot = S.dupload(gslot);
+        d.gap_sid = S.dupload(gsid);
+        d.y_igap = S.dalloc<float>(ns);
+    }
     d.y_isyn = S.dalloc<float>(ns);
     d.y_pp1 = S.dupload(y_pp1);
     d.hist_base = S.dupload(hist_base);

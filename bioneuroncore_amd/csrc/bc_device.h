@@ -32,14 +32,14 @@ __device__ __constant__ static const float kBath[7] = {140.0f, 3.0f, 1.2f, 130.0
 enum : uint8_t {
     M_PUMP = 7, M_AMPA = 8, M_NMDA = 9, M_GABAA = 11, M_SK = 103, M_BK = 104, M_K2P = 107,
     M_KV7 = 112, M_KVDR = 123, M_HCN1 = 129, M_NALCN = 133, M_CAV21 = 140, M_NAV16 = 141,
-    M_NAV12 = 142, M_KV31 = 150, M_KV11 = 156,
+    M_NAV12 = 142, M_KV31 = 150, M_KV11 = 156, M_CX36 = 114, M_CX43 = 198,
 };
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
     switch (t) {
         case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
         case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
-        case M_NAV12: case M_KV31: case M_KV11:
+        case M_NAV12: case M_KV31: case M_KV11: case M_CX36: case M_CX43:
             return true;
         default:
             return false;
@@ -57,6 +57,7 @@ struct SegRegs {
     double vol;    // volume (double, run_tick_common.h:260-264)
     double cpm;    // F * vol / 1000, the charge-per-mole factor every current divides by
     float pkc, pka, camp;   // cascade activities read by Nav/Kv3.1/Kv7/HCN (constant on device)
+    float camkii;           // state[0] of the last CaMKII modifier, read by the connexins
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -737,6 +738,36 @@ BC_DEV double dispatch_receptor(uint8_t type, const ModSlot& m, SegRegs& s, floa
     return rr * (s2 * coag);
 }
 
+// Connexins Cx36 (114) and Cx43 (198) (hh_biological.cpp:3266-3300, 3310-3341; the two bodies
+// are the same): connexon turnover over hours, CaMKII/PKA phosphorylation over seconds and a
+// pH-gated open probability; no current. Gap junctions read state[0] and state[4].
+BC_DEV double ch_connexin(const ModSlot& m, SegRegs& s, float dt) {
+    double synth = (m.get(2) > 0.0f) ? m.get(2) : 0.001f;
+    double degr = (m.get(3) > 0.0f) ? m.get(3) : 0.0005f;
+    double pH = (m.get(5) > 4.0f) ? m.get(5) : 7.2f;
+    double n = m.get(0);
+    n += (synth - degr * n) * dt / 3600000.0f;
+    if (n < 0.0f) n = 0.0f;
+    if (n > 1.0f) n = 1.0f;
+    m.set(0, n);
+    double pka = m.get(6);
+    float p_target = s.camkii * 0.6f + pka * 0.3f;
+    double ph = m.get(1);
+    ph += (p_target - ph) * dt / 5000.0f;
+    if (ph < 0.0f) ph = 0.0f;
+    if (ph > 1.0f) ph = 1.0f;
+    m.set(1, ph);
+    float ph_f = (pH < 6.8f) ? (pH - 6.0f) / 0.8f : 1.0f;
+    if (ph_f < 0.0f) ph_f = 0.0f;
+    double p_cam = 1.0f - ph * 0.5f;
+    double p_pka = 1.0f + ph * 0.3f;
+    double po = (p_cam * 0.5f + p_pka * 0.5f) * ph_f;
+    if (po < 0.05f) po = 0.05f;
+    if (po > 2.0f) po = 2.0f;
+    m.set(4, po);
+    return 0.0;
+}
+
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
 // current and leave their state untouched, as the reference's device switch does.
 BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
@@ -769,6 +800,7 @@ BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
         }
         case M_KV31: return ch_kv31(m, s, dt);
         case M_KV11: return ch_kv11(m, s, dt);
+        case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
         default: return 0.0;
     }
 }

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     s.pkc = d.pkc[i];
     s.pka = d.pka[i];
     s.camp = d.camp[i];
+    s.camkii = d.camkii[i];
     seg_consts_init(s);
     ghk_cache_init(s);
     const bool my = d.sflags[i] & SF_MYELIN;
@@ -809,6 +810,60 @@ __global__ __launch_bounds__(256) void k_pre_summary(Net d) {
     d.presum[w] = ps;
 }
 
+// Gap junction of slot y (simulation_host.cpp:372-400): the slow Ca/CaM gate from both ends'
+// intracellular Ca (run_tick_common.h:112-122), the connexin conductance (run_tick_common.h:
+// 79-110) times the junction voltage, through a first-order filter of time constant `delay`
+// when it has one. The current leaves the presynaptic end and enters the postsynaptic end in
+// the post-side pass. Both ends' Ca is read at step-3 entry: exact unless an earlier synapse
+// changed it (counted as an order hazard at load).
+__device__ static void gap_junction(const Net& d, uint32_t y, float dt) {
+    const size_t P = d.npad, NS = d.ns;
+    const uint32_t pre = d.y_pre[y], post = d.y_post[y];
+    const float vj = d.V[pre] - d.V[post];
+    const float ca_pre = d.in[2 * P + pre], ca_post = d.in[2 * P + post];
+    float m = d.y_dep[y];
+    const float avg = 0.5f * (ca_pre + ca_post);
+    const float cam = (avg * avg) / (avg * avg + 0.0003f * 0.0003f);
+    m += (cam - m) * dt / 100.0f;
+    if (m < 0.0f) m = 0.0f;
+    if (m > 1.0f) m = 1.0f;
+    d.y_dep[y] = m;
+    uint32_t pm = d.mcount[pre], qm = d.mcount[post];
+    if (pm > 32) pm = 0;
+    if (qm > 32) qm = 0;
+    float g = 0.0f;
+    for (int cx = 0; cx < 2; cx++) {
+        const uint8_t ty = cx == 0 ? M_CX36 : M_CX43;
+        const float gamma = cx == 0 ? 14.0f : 100.0f;
+        float dens = 0.0f, f = 1.0f;
+        for (uint32_t j = 0; j < pm; j++)
+            if (d.mtype[j * P + pre] == ty) {
+                dens = d.mdens[j * P + pre];
+                f *= d.mstate[((size_t)j * 7 + 0) * P + pre] * d.mstate[((size_t)j * 7 + 4) * P + pre];
+            }
+        for (uint32_t j = 0; j < qm; j++)
+            if (d.mtype[j * P + post] == ty)
+                f *= d.mstate[((size_t)j * 7 + 0) * P + post] * d.mstate[((size_t)j * 7 + 4) * P + post];
+        if (cx == 1) {   // Cx43 closes with |Vj| (Boltzmann, half 40 mV, slope 5 mV)
+            const float a = vj < 0.0f ? -vj : vj;
+            const float fv = 1.0f / (1.0f + bc::exp((a - 40.0f) / 5.0f));
+            f *= fv;
+        }
+        if (f < 0.01f) f = 0.01f;
+        g += dens * f * gamma;
+    }
+    const float raw = g * 1e-3f * (1.0f - m) * vj;
+    float out = raw;
+    const float delay = d.y_delay[y];
+    if (delay > 0.001f) {
+        float flt = d.y_cleft[y];   // species 0 holds the filter state
+        flt += (raw - flt) * dt / delay;
+        d.y_cleft[y] = flt;
+        out = flt;
+    }
+    d.y_igap[y] = out;
+}
+
 __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     exp_table_stage();
     const uint64_t tick = *d.tick;
@@ -816,7 +871,10 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     const size_t NS = d.ns;
     for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < d.ns; y += gridDim.x * blockDim.x) {
         d.y_relp[y] = 0.0f;
-        if (d.y_flags[y] & YF_SKIP) continue;
+        if (d.y_flags[y] & YF_SKIP) {
+            if (d.y_flags[y] & YF_GAP) gap_junction(d, y, dt);
+            continue;
+        }
         const float delay = d.y_delay[y];
         // delayed arrivals (delay_buffer.h:31-43)
         uint8_t mask = d.d_mask[y];
@@ -948,6 +1006,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
             s.pkc = d.pkc[i];
             s.pka = d.pka[i];
             s.camp = d.camp[i];
+            s.camkii = d.camkii[i];
         }
         seg_consts_init(s);
         ghk_cache_init(s);
@@ -962,13 +1021,22 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
         const uint32_t wl = threadIdx.x & 63;
         uint32_t off = d.wave_base[lane >> 6];
         bool ran = false;
+        // electrical synapses whose presynaptic end is this segment take their current out of
+        // I_buffer in synapse-index order among the chain (simulation_host.cpp:372-400)
+        uint32_t go = 0, ge = 0;
+        if (d.gap_off) { go = d.gap_off[i]; ge = d.gap_off[i + 1]; }
         for (uint32_t q = 0; q < len; q++) {
             // the lanes still walking are lanes 0..cnt-1 of this wavefront (sorted by length)
             const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
             const uint32_t y = off + wl;
             off += cnt;
+            if (go < ge) {   // this segment's junctions as the presynaptic end, up to slot y
+                const uint32_t sid = d.y_sid[y];
+                for (; go < ge && d.gap_sid[go] <= sid; go++) ibuf -= d.y_igap[d.gap_slot[go]];
+            }
             const uint8_t yf = d.y_flags[y];
             if (yf & YF_SKIP) {
+                if (yf & YF_GAP) ibuf += d.y_igap[y];
                 if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
                 continue;
             }
@@ -995,6 +1063,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
             d.y_isyn[y] = isyn;
             if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
         }
+        for (; go < ge; go++) ibuf -= d.y_igap[d.gap_slot[go]];
         if (kReceptorsOnly) {
             // the same chain again, for the transmitters the receptors do not read
             uint32_t off2 = d.wave_base[lane >> 6];

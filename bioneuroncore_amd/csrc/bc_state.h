@@ -32,7 +32,8 @@ enum : uint8_t {
 // Synapse flag bits (static).
 enum : uint8_t {
     YF_INHIBITORY = 1,    // a post receptor is GABA-A / GABA-B / glycine
-    YF_SKIP = 2,          // pre/post out of range or electrical: nothing runs
+    YF_SKIP = 2,          // pre/post out of range or electrical: no chemical step runs
+    YF_GAP = 4,           // electrical (gap junction); also YF_SKIP
 };
 
 // Tiles of one size class, launched together.
@@ -130,6 +131,7 @@ struct Net {
     const float* pka;
     const float* camp;
     const float* pp1;
+    const float* camkii;        // state[0] of the last CaMKII modifier (0 if none)
     const uint8_t* sflags;
     const int8_t* cav21_slot;   // last Cav2.1 modifier slot, -1 if none
 
@@ -194,6 +196,13 @@ struct Net {
     uint64_t* d_arr;             // [4][ns]
     float* d_amt;                // [4][ns]
     uint8_t* d_nt;               // [4][ns]
+    // gap junctions (null when the network has none)
+    const uint32_t* y_post;      // device post segment of each slot
+    float* y_igap;               // junction current this tick (pA, into the post, out of the pre)
+    const uint32_t* y_sid;       // synapse index of each slot
+    const uint32_t* gap_off;     // [n+1] per segment: its electrical synapses as the presynaptic end
+    const uint32_t* gap_slot;    //   their slots
+    const uint32_t* gap_sid;     //   their synapse indices (ascending per segment)
 
     float nt_diff[kNtCount];     // transmitter diffusion coefficients
 };

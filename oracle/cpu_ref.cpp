@@ -10,9 +10,9 @@
 // identity is checked by tests/test_oracle_vs_reference.py against oracle/_ref (the reference
 // compiled from its own sources) and against the committed golden fixtures.
 //
-// Scope: modifier IDs {7,8,9,11,103,104,107,112,123,129,133,140,141,142,150,156}; chemical
-// synapses without presynaptic autoreceptors; no glia (GLAST), gap junctions, consolidation,
-// mechano stimuli or plugins. Anything outside that set is reported by
+// Scope: modifier IDs {7,8,9,11,103,104,107,112,114,123,129,133,140,141,142,150,156,198};
+// chemical synapses without presynaptic autoreceptors and electrical synapses (gap junctions);
+// no glia (GLAST), consolidation, mechano stimuli or plugins. Anything outside that set is reported by
 // oracle_unsupported_count() and computed as the reference's "unknown ID" default (0 current).
 #include <algorithm>
 #include <cmath>
@@ -48,6 +48,7 @@ enum : uint8_t {
     T_CALCINEURIN = 83, T_ECB = 86, T_SK = 103, T_BK = 104, T_K2P = 107, T_KV7 = 112,
     T_PP1 = 113, T_KVDR = 123, T_HCN1 = 129, T_NALCN = 133, T_CAMP_PKA = 135, T_PKC = 137,
     T_CAV21 = 140, T_NAV16 = 141, T_NAV12 = 142, T_GLAST = 147, T_KV31 = 150, T_KV11 = 156,
+    T_CX36 = 114, T_CX43 = 198,
 };
 
 static uint64_t g_unsupported = 0;
@@ -612,6 +613,34 @@ static double rec_nmda(Modifier* m, Segment* s, float dt) {   // receptors_biolo
     return r * nmda_cdi(m, s, dt);
 }
 
+// Connexin Cx36 (114) and Cx43 (198) (hh_biological.cpp:3266-3300, 3310-3341, identical
+// bodies): connexon turnover, CaMKII/PKA phosphorylation, pH-gated open probability. They
+// carry no current; gap junctions read state[0] (connexons) and state[4] (open probability).
+static double chan_connexin(Modifier* m, Segment* s, float dt) {
+    double synth = (m->state[2] > 0.0f) ? m->state[2] : 0.001f;
+    double degr = (m->state[3] > 0.0f) ? m->state[3] : 0.0005f;
+    double pH = (m->state[5] > 4.0f) ? m->state[5] : 7.2f;
+    m->state[0] += (synth - degr * m->state[0]) * dt / 3600000.0f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float camkii = 0.0f;
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == T_CAMKII) camkii = s->modifiers[j].state[0];
+    double pka = m->state[6];
+    float p_target = camkii * 0.6f + pka * 0.3f;
+    m->state[1] += (p_target - m->state[1]) * dt / 5000.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    float ph_f = (pH < 6.8f) ? (pH - 6.0f) / 0.8f : 1.0f;
+    if (ph_f < 0.0f) ph_f = 0.0f;
+    double p_cam = 1.0f - m->state[1] * 0.5f;
+    double p_pka = 1.0f + m->state[1] * 0.3f;
+    m->state[4] = (p_cam * 0.5f + p_pka * 0.5f) * ph_f;
+    if (m->state[4] < 0.05f) m->state[4] = 0.05f;
+    if (m->state[4] > 2.0f) m->state[4] = 2.0f;
+    return 0.0f;
+}
+
 static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
     switch (type) {
         case T_PUMP: return chan_pump(m, s, dt);
@@ -625,6 +654,7 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_KVDR: return chan_kvdr(m, s, dt);
         case T_HCN1: return chan_hcn1(m, s, dt);
         case T_NALCN: return chan_nalcn(m, s, dt);
+        case T_CX36: case T_CX43: return chan_connexin(m, s, dt);
         case T_CAV21: return chan_cav21(m, s, dt);
         case T_NAV16: return chan_nav(m, s, dt, kNav16, 20.7f, 5.0f, 0.2f);
         case T_NAV12: return chan_nav(m, s, dt, kNav12, 20.8f, 10.0f, 0.7f);
@@ -1016,6 +1046,57 @@ static void spine_plast(Synapse* y, float dt) {
     if (y->spine_calcineurin > 1.0f) y->spine_calcineurin = 1.0f;
 }
 
+// Gap-junction conductance (run_tick_common.h:79-110) and the slow Ca/CaM gate
+// (run_tick_common.h:112-122), with the junction's first-order filter when it has a delay.
+static float gap_g_eff(const Segment* pre, const Segment* post, float m_slow, float vj) {
+    uint8_t pm = pre->modifier_count, qm = post->modifier_count;
+    if (pm > 32) pm = 0;
+    if (qm > 32) qm = 0;
+    float g = 0.0f;
+    for (int cx = 0; cx < 2; cx++) {
+        const uint8_t ty = cx == 0 ? T_CX36 : T_CX43;
+        const float gamma = cx == 0 ? 14.0f : 100.0f;
+        float dens = 0.0f, f = 1.0f;
+        for (uint8_t j = 0; j < pm; j++)
+            if (pre->modifiers[j].type == ty) {
+                dens = pre->modifiers[j].density;
+                f *= pre->modifiers[j].state[0] * pre->modifiers[j].state[4];
+            }
+        for (uint8_t j = 0; j < qm; j++)
+            if (post->modifiers[j].type == ty) f *= post->modifiers[j].state[0] * post->modifiers[j].state[4];
+        if (cx == 1) {
+            float a = vj < 0.0f ? -vj : vj;
+            float fv = 1.0f / (1.0f + exp((a - 40.0f) / 5.0f));
+            f *= fv;
+        }
+        if (f < 0.01f) f = 0.01f;
+        g += dens * f * gamma;
+    }
+    return g * 1e-3f * (1.0f - m_slow);
+}
+static void gap_junction(Synapse* y, Segment* pre, Segment* post, float dt) {
+    float vj = pre->V_m - post->V_m;
+    float ca_pre = pre->intra_ions[2], ca_post = post->intra_ions[2];
+    float m = y->depression;
+    float avg = 0.5f * (ca_pre + ca_post);
+    float cam = (avg * avg) / (avg * avg + 0.0003f * 0.0003f);
+    m += (cam - m) * dt / 100.0f;
+    if (m < 0.0f) m = 0.0f;
+    if (m > 1.0f) m = 1.0f;
+    y->depression = m;
+    float raw = gap_g_eff(pre, post, m, vj) * vj;
+    if (y->delay > 0.001f) {
+        float flt = y->cleft_neurotransmitter[0];
+        flt += (raw - flt) * dt / y->delay;
+        y->cleft_neurotransmitter[0] = flt;
+        pre->I_buffer -= flt;
+        post->I_buffer += flt;
+    } else {
+        pre->I_buffer -= raw;
+        post->I_buffer += raw;
+    }
+}
+
 static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t ns, uint64_t tick, float dt) {
     if (ns != c.delay_syn) {
         c.delay.assign((size_t)ns * 4 * 16, 0);
@@ -1049,7 +1130,10 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
                 }
             }
         }
-        if (y->type == 1 || y->type == 2) { g_unsupported++; continue; }
+        if (y->type == 1 || y->type == 2) {   // gap junction (simulation_host.cpp:372-400)
+            gap_junction(y, pre, post, dt);
+            continue;
+        }
         if (y->pre_autoreceptor_count > 0) g_unsupported++;
         rrp_recover(y, pre, dt);
         bouton_step(pre, y, dt);

@@ -163,6 +163,20 @@ def _concat(nets):
     return bion.Network(np.concatenate(neurons), np.concatenate(segs), np.concatenate(conns), np.concatenate(syns))
 
 
+@pytest.mark.parametrize("seed", [5, 6])
+def test_gap_junctions_and_connexins(seed):
+    """Electrical synapses (filtered and unfiltered soma-soma junctions, dendritic bridges)
+    interleaved in index order with AMPA synapses, Cx36/Cx43 connexins on somas: every byte
+    equals the oracle's (which equals the compiled reference) over 300 ticks."""
+    net = bion.gap_network(seed)
+    cur = H.soma_clamp(net, 150.0)
+    cur["amplitude_pA"][::2] = 0.0
+    gst = backend.make_stimuli(current=cur)
+    _compare(net, 300, lambda t: H.Stimuli(current=cur), lambda t: gst)
+    seg, syn, spk, dl, stats = _gpu_run(net.copy(), 1, lambda t: gst)
+    assert stats.order_hazard_synapses == 0 and stats.unsupported_synapses == 0
+
+
 def test_compat_coherence_and_host_edits():
     """COMPAT mode: the AoS is current after every tick and host edits between ticks reach the
     device (as the daemon writes V_m into the AoS, src/daemon_server.cpp:1634-1657)."""

@@ -61,3 +61,16 @@ def test_immediate_release_and_no_connections():
     net.connections = net.connections[:0]   # step 2.5 skipped entirely
     clamp = H.soma_clamp(net, 250.0)
     _both(net, 200, lambda t: H.Stimuli(current=clamp))
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_gap_junctions_and_connexins(seed):
+    """Electrical synapses (filtered and unfiltered soma-soma junctions, dendritic bridges)
+    interleaved with AMPA synapses, Cx36/Cx43 connexins on somas."""
+    net = bion.gap_network(seed)
+    cur = H.soma_clamp(net, 150.0)
+    cur["amplitude_pA"][::2] = 0.0   # half the neurons driven only through the junctions
+    rs, ry, _ = _both(net, 300, lambda t: H.Stimuli(current=cur))
+    gj = np.nonzero(ry["type"] > 0)[0]
+    assert len(gj) and np.any(ry["depression"][gj] != net.synapses["depression"][gj])
+    assert np.any(ry["cleft_neurotransmitter"][gj, 0] != 0.0)   # the filtered junctions
