@@ -71,8 +71,12 @@ PUMP, AMPA, NMDA, GABAA = 7, 8, 9, 11
 SK, BK, K2P, KV7, KVDR, HCN1, NALCN = 103, 104, 107, 112, 123, 129, 133
 CAV21, NAV16, NAV12, KV31, KV11 = 140, 141, 142, 150, 156
 CX36, CX43 = 114, 198   # connexins (gap junctions read their connexon count and open probability)
+# Na+-coupled transmitter transporters: glial (GLAST marks a segment as glia for step 3's
+# capture) and neuronal.
+GLAST, GLYT1, GAT3, EAAT3, GLYT2, GAT1 = 147, 148, 149, 183, 184, 185
+TRANSPORTERS = (GLAST, GLYT1, GAT3, EAAT3, GLYT2, GAT1)
 SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43)
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -81,7 +85,8 @@ EXTRA_IONS = np.array([140.0, 3.0, 1.2, 130.0, 2.0, 0.0, 4.0e-5, 0.0], dtype=np.
 # Canonical densities (reference include/cpg_config.h:16-76, tests/test_cpu_gpu_parity_cuda.cpp:246-247).
 CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, PUMP: 1500.0,
                  CAV21: 100.0, HCN1: 120.0, SK: 6.0, BK: 0.15, KV11: 0.346, KV31: 2.0, KV7: 2.0,
-                 CX36: 400.0, CX43: 60.0}
+                 CX36: 400.0, CX43: 60.0, GLAST: 1.0, GLYT1: 1.0, GAT3: 1.0, EAAT3: 1.0,
+                 GLYT2: 1.0, GAT1: 1.0}
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -315,6 +320,60 @@ def gap_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Netw
     syn = np.concatenate([net.synapses, gj])
     order = rng.permutation(len(syn))
     syn = syn[order]
+    syn["id"] = np.arange(1, len(syn) + 1, dtype=np.uint32)
+    net.synapses = syn
+    return net
+
+
+def glia_network(seed: int, n_neurons: int = 16, seg_per_neuron: int = 6, n_glia: int = 5,
+                 glia_first: bool = False) -> Network:
+    """Chain neurons with AMPA+NMDA / GABA-A synapses and neuronal transporters (EAAT3, GlyT2,
+    GAT1 on somas), spread over depth z, plus n_glia single-segment glial cells carrying GLAST,
+    GlyT1, GAT3, the pump and K2P (glia_first: the glial cells come first in segment order).
+    Glial depths repeat in pairs, so some synapses have two equally near glia (the first in
+    segment order takes the capture); every fourth glial cell also receives synapses."""
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP],
+                        [CAV21, EAAT3, GLYT2, GAT1], 3.0, "ampa_nmda_gaba", 4, seed)
+    rng = np.random.default_rng(seed + 7)
+    segs = net.segments
+    segs["position"][:, 2] = np.repeat(rng.uniform(0.0, 400.0, n_neurons), seg_per_neuron) \
+        + np.tile(np.arange(seg_per_neuron) * 3.0, n_neurons)
+    g = np.zeros(n_glia, SEGMENT_DT)
+    _init_segment_state(g, rng)
+    g["segment_type"] = 1
+    g["radius"], g["length"] = 4.0, 8.0
+    g["neuron_id"] = n_neurons + np.arange(n_glia)
+    g["position"][:, 0] = -50.0
+    g["position"][:, 2] = np.repeat(rng.uniform(0.0, 400.0, (n_glia + 1) // 2), 2)[:n_glia]
+    gt = [GLAST, GLYT1, GAT3, PUMP, K2P]
+    _set_modifiers(g, gt, np.array([[CANON_DENSITY[t] for t in gt]] * n_glia, np.float64))
+    ng = n_neurons * seg_per_neuron
+    gneu = np.zeros(n_glia, NEURON_DT)
+    gneu["id"] = n_neurons + np.arange(n_glia)
+    gneu["segment_count"] = 1
+    gneu["threshold"] = -40.0
+    syn = net.synapses
+    gsyn = syn[rng.integers(0, len(syn), (n_glia + 3) // 4)].copy()
+    if glia_first:   # glia at 0..n_glia-1, neurons after them
+        net.segments = np.concatenate([g, segs])
+        gneu["segment_start"] = gneu["soma_index"] = np.arange(n_glia)
+        net.neurons["segment_start"] += n_glia
+        net.neurons["soma_index"] += n_glia
+        net.connections["segment_a"] += n_glia
+        net.connections["segment_b"] += n_glia
+        syn["pre_segment"] += n_glia
+        syn["post_segment"] += n_glia
+        gsyn["pre_segment"] += n_glia
+        gsyn["post_segment"] = np.arange(0, n_glia, 4)
+        net.neurons = np.concatenate([gneu, net.neurons])
+    else:
+        net.segments = np.concatenate([segs, g])
+        gneu["segment_start"] = gneu["soma_index"] = ng + np.arange(n_glia)
+        gsyn["post_segment"] = ng + np.arange(0, n_glia, 4)
+        net.neurons = np.concatenate([net.neurons, gneu])
+    net.segments["id"] = np.arange(len(net.segments), dtype=np.uint32)
+    syn = np.concatenate([syn, gsyn])
+    syn = syn[rng.permutation(len(syn))]
     syn["id"] = np.arange(1, len(syn) + 1, dtype=np.uint32)
     net.synapses = syn
     return net

@@ -10,9 +10,10 @@
 // identity is checked by tests/test_oracle_vs_reference.py against oracle/_ref (the reference
 // compiled from its own sources) and against the committed golden fixtures.
 //
-// Scope: modifier IDs {7,8,9,11,103,104,107,112,114,123,129,133,140,141,142,150,156,198};
-// chemical synapses without presynaptic autoreceptors and electrical synapses (gap junctions);
-// no glia (GLAST), consolidation, mechano stimuli or plugins. Anything outside that set is reported by
+// Scope: modifier IDs {7,8,9,11,103,104,107,112,114,123,129,133,140,141,142,147,148,149,150,
+// 156,183,184,185,198}; chemical synapses without presynaptic autoreceptors, electrical synapses
+// (gap junctions) and glial transmitter capture into GLAST-carrying segments; no consolidation,
+// mechano stimuli or plugins. Anything outside that set is reported by
 // oracle_unsupported_count() and computed as the reference's "unknown ID" default (0 current).
 #include <algorithm>
 #include <cmath>
@@ -48,7 +49,8 @@ enum : uint8_t {
     T_CALCINEURIN = 83, T_ECB = 86, T_SK = 103, T_BK = 104, T_K2P = 107, T_KV7 = 112,
     T_PP1 = 113, T_KVDR = 123, T_HCN1 = 129, T_NALCN = 133, T_CAMP_PKA = 135, T_PKC = 137,
     T_CAV21 = 140, T_NAV16 = 141, T_NAV12 = 142, T_GLAST = 147, T_KV31 = 150, T_KV11 = 156,
-    T_CX36 = 114, T_CX43 = 198,
+    T_CX36 = 114, T_CX43 = 198, T_GLYT1 = 148, T_GAT3 = 149, T_EAAT3 = 183, T_GLYT2 = 184,
+    T_GAT1 = 185,
 };
 
 static uint64_t g_unsupported = 0;
@@ -641,6 +643,49 @@ static double chan_connexin(Modifier* m, Segment* s, float dt) {
     return 0.0f;
 }
 
+// Na+-coupled transmitter transporters (modifier_helpers.h:418-455, hh_biological.cpp:725-780):
+// Michaelis-Menten uptake of one transmitter from the segment's extracellular pool, limited by
+// the transmitter, extracellular Na+ and (K+-countertransport) intracellular K+ available, then
+// both ion arrays floored at 1 nM. Returns the (depolarising) transporter current in pA.
+struct Transporter { int nt; float km, imax, rel_per_mm; int na_in, cl_in, k_out, net; };
+constexpr Transporter kGlast = {0, 10.0f, 2.0f, 667.0f, 3, 0, 1, 2};    // 147
+constexpr Transporter kGlyt1 = {7, 30.0f, 2.0f, 667.0f, 2, 1, 0, 1};    // 148
+constexpr Transporter kGat3 = {1, 0.5f, 2.0f, 667.0f, 2, 1, 0, 1};      // 149
+constexpr Transporter kEaat3 = {0, 20.0f, 2.0f, 667.0f, 3, 0, 1, 2};    // 183
+constexpr Transporter kGlyt2 = {7, 55.7f, 2.0f, 667.0f, 3, 1, 0, 2};    // 184
+constexpr Transporter kGat1 = {1, 4.7f, 2.0f, 667.0f, 3, 1, 0, 2};      // 185
+static double chan_transporter(Modifier* m, Segment* s, float dt, const Transporter& t) {
+    float area = eff_area(m, s);
+    float nt = s->neurotransmitter[t.nt];
+    if (nt < 0.0f) nt = 0.0f;
+    float frac = nt / (nt + t.km);
+    float I = t.imax * m->density * area * (1e-8f * 1e6f) * frac;
+    float vol = (float)volume(s);
+    double cpm = kFaraday * vol / 1000.0;
+    float tr = I * dt / ((float)t.net * cpm);
+    float nt_av = nt / t.rel_per_mm;
+    float na_av = s->extra_ions[0] * kEvr / (float)t.na_in;
+    if (tr > nt_av) tr = nt_av;
+    if (tr > na_av) tr = na_av;
+    if (t.k_out && tr > s->intra_ions[1]) tr = s->intra_ions[1];
+    if (tr < 0.0f) tr = 0.0f;
+    s->intra_ions[0] += (float)t.na_in * tr;
+    s->extra_ions[0] -= (float)t.na_in * tr / kEvr;
+    if (t.cl_in) {
+        s->intra_ions[3] += (float)t.cl_in * tr;
+        s->extra_ions[3] -= (float)t.cl_in * tr / kEvr;
+    }
+    if (t.k_out) {
+        s->intra_ions[1] -= (float)t.k_out * tr;
+        s->extra_ions[1] += (float)t.k_out * tr / kEvr;
+    }
+    s->neurotransmitter[t.nt] -= tr * t.rel_per_mm;
+    if (s->neurotransmitter[t.nt] < 0.0f) s->neurotransmitter[t.nt] = 0.0f;
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return I;
+}
+
 static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
     switch (type) {
         case T_PUMP: return chan_pump(m, s, dt);
@@ -660,6 +705,12 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_NAV12: return chan_nav(m, s, dt, kNav12, 20.8f, 10.0f, 0.7f);
         case T_KV31: return chan_kv31(m, s, dt);
         case T_KV11: return chan_kv11(m, s, dt);
+        case T_GLAST: return chan_transporter(m, s, dt, kGlast);
+        case T_GLYT1: return chan_transporter(m, s, dt, kGlyt1);
+        case T_GAT3: return chan_transporter(m, s, dt, kGat3);
+        case T_EAAT3: return chan_transporter(m, s, dt, kEaat3);
+        case T_GLYT2: return chan_transporter(m, s, dt, kGlyt2);
+        case T_GAT1: return chan_transporter(m, s, dt, kGat1);
         default: g_unsupported++; return 0.0;
     }
 }
@@ -1111,9 +1162,11 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
     for (uint32_t i = 0; i < n; i++)
         for (uint8_t j = 0; j < sg[i].modifier_count; j++)
             if (sg[i].modifiers[j].type == T_5HT7) sg[i].neurotransmitter[3] = 0.0f;
-    for (uint32_t g = 0; g < n; g++)
+    // glial segments (GLAST carriers), the first 256 in segment order (simulation_host.cpp:350-354)
+    uint32_t glia[256], nglia = 0;
+    for (uint32_t g = 0; g < n && nglia < 256; g++)
         for (uint8_t j = 0; j < sg[g].modifier_count; j++)
-            if (sg[g].modifiers[j].type == T_GLAST) g_unsupported++;
+            if (sg[g].modifiers[j].type == T_GLAST) { glia[nglia++] = g; break; }
 
     for (uint32_t i = 0; i < ns; i++) {
         Synapse* y = &sy[i];
@@ -1161,6 +1214,21 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
             }
         }
         for (int k = 0; k < 15; k++) post->neurotransmitter[k] += y->cleft_neurotransmitter[k];
+        if (nglia > 0)   // glial capture of glutamate, glycine, GABA (simulation_host.cpp:472-490)
+            for (int q = 0; q < 3; q++) {
+                const int nt = q == 0 ? 0 : (q == 1 ? 7 : 1);
+                if (y->cleft_neurotransmitter[nt] <= 0.0f) continue;
+                float best = 1e30f;
+                uint32_t g = glia[0];
+                for (uint32_t k = 0; k < nglia; k++) {
+                    float dz = fabsf(sg[glia[k]].position.z - post->position.z);
+                    if (dz < best) { best = dz; g = glia[k]; }
+                }
+                float tau = (y->tau_clearance > 0.0f) ? y->tau_clearance : 5.0f;
+                float cap = y->cleft_neurotransmitter[nt] * (1.0f - exp(-dt / tau));
+                sg[g].neurotransmitter[nt] += cap;
+                y->cleft_neurotransmitter[nt] -= cap;
+            }
         bool my = has_myelin(post);
         float isyn = 0.0f;
         for (int j = 0; j < y->post_receptor_count; j++) {
@@ -1172,6 +1240,7 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
         post->I_buffer += isyn;
         spine_ca(y, isyn, dt);
         for (int k = 0; k < 15; k++) {
+            if ((k == 0 || k == 7 || k == 1) && nglia > 0) continue;   // the glia took them up
             y->cleft_neurotransmitter[k] *= exp(-dt / y->tau_clearance);
             if (y->cleft_neurotransmitter[k] < 0.0f) y->cleft_neurotransmitter[k] = 0.0f;
         }
@@ -1259,12 +1328,15 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                                                     : 2.0f * kPi * s->radius * s->length;
             if (c.cable) {   // held concentrations: every channel sees the stored ones
                 double in[BC_ION_MAX], ex[BC_ION_MAX];
+                float nt[15];
                 const float atp = s->ATP_level;
                 memcpy(in, s->intra_ions, sizeof in);
                 memcpy(ex, s->extra_ions, sizeof ex);
+                memcpy(nt, s->neurotransmitter, sizeof nt);
                 itot += dispatch(t, &s->modifiers[j], s, dt);
                 memcpy(s->intra_ions, in, sizeof in);
                 memcpy(s->extra_ions, ex, sizeof ex);
+                memcpy(s->neurotransmitter, nt, sizeof nt);
                 s->ATP_level = atp;
             } else {
                 itot += dispatch(t, &s->modifiers[j], s, dt);

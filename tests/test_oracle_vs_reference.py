@@ -74,3 +74,33 @@ def test_gap_junctions_and_connexins(seed):
     gj = np.nonzero(ry["type"] > 0)[0]
     assert len(gj) and np.any(ry["depression"][gj] != net.synapses["depression"][gj])
     assert np.any(ry["cleft_neurotransmitter"][gj, 0] != 0.0)   # the filtered junctions
+
+
+@pytest.mark.parametrize("seed,n_glia,first", [(3, 5, False), (4, 6, True), (5, 270, False)])
+def test_glial_capture_and_transporters(seed, n_glia, first):
+    """Glial cells (GLAST, GlyT1, GAT3) capture glutamate and GABA from every cleft into the
+    nearest one in depth (ties: the first in segment order; only the first 256 glial segments
+    count), some glia also receive synapses; EAAT3/GlyT2/GAT1 on the neurons' somas."""
+    net = bion.glia_network(seed, n_glia=n_glia, glia_first=first)
+    cur = H.soma_clamp(net, 150.0)
+    rs, ry, sp = _both(net, 300, lambda t: H.Stimuli(current=cur))
+    glia = np.nonzero(net.segments["modifiers"]["type"][:, 0] == bion.GLAST)[0]
+    assert rs["neurotransmitter"][glia, 0].max() > 0.0 and rs["neurotransmitter"][glia, 1].max() > 0.0
+    assert sp.sum() > 0
+
+
+def test_transporters_take_up_injected_transmitters():
+    """Without synapses the transmitters persist: injected glutamate, GABA and glycine are taken
+    up by the neuronal and glial transporters while step 2.5 diffuses them."""
+    net = bion.glia_network(9, n_glia=4)
+    net.synapses = net.synapses[:0]
+    n = len(net.segments)
+    lig = np.zeros(12, bion.LIG_INJ_DT)
+    lig["segment_id"] = np.arange(12) * (n // 12)
+    lig["segment_id"][-3:] = n - 3 + np.arange(3)
+    lig["nt_idx"] = np.tile([0, 1, 7], 4)
+    lig["amount"] = np.linspace(0.5, 30.0, 12)
+    cur = H.soma_clamp(net, 150.0)
+    rs, _, _ = _both(net, 200, lambda t: H.Stimuli(current=cur, lig=lig if t % 40 == 2 else None))
+    got = rs["neurotransmitter"][lig["segment_id"], lig["nt_idx"]]
+    assert np.all(got < 5 * lig["amount"]) and np.all(got > 0.0)
