@@ -379,6 +379,29 @@ def glia_network(seed: int, n_neurons: int = 16, seg_per_neuron: int = 6, n_glia
     return net
 
 
+def segment_receptor_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons with AMPA+NMDA/GABA-A synapses whose dendrites also carry ionotropic
+    receptors as segment modifiers (AMPA and NMDA on every third segment, GABA-A on the next):
+    step 1 runs them on the segment's own transmitters (the NMDA carriers' glutamate bath and
+    the previous tick's synaptic deposits)."""
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21],
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    local = np.arange(len(segs)) % seg_per_neuron
+    mods = segs["modifiers"]
+    for i in np.nonzero(local > 0)[0]:
+        k = int(segs["modifier_count"][i])
+        if local[i] % 3 == 1:
+            mods["type"][i, k], mods["density"][i, k] = AMPA, 200.0
+            mods["type"][i, k + 1], mods["density"][i, k + 1] = NMDA, 40.0
+            mods["state"][i, k + 1, 2] = 1.0   # CDI gate open
+            segs["modifier_count"][i] = k + 2
+        elif local[i] % 3 == 2:
+            mods["type"][i, k], mods["density"][i, k] = GABAA, 150.0
+            segs["modifier_count"][i] = k + 1
+    return net
+
+
 def config1(seed: int = 0xB10C0001) -> Network:
     """64 neurons x (soma + 7-segment chain); Nav1.2/Kv-DR/K2P/NALCN/pump everywhere,
     Cav2.1 on somas; 256 AMPA synapses (soma -> uniform segment), delays 1-5 ticks."""

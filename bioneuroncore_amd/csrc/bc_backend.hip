@@ -137,6 +137,7 @@ struct Sim {
     bool host_stale = false;
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
     bool receptors_only = true;   // every receptor slot is AMPA, NMDA or GABA-A
+    bool nt_read = false;         // some segment modifier reads or takes up transmitters
     std::vector<uint8_t> seg_dispatched;   // [n*32] slot ran at least once (state[7] written)
     hipEvent_t ev[8] = {};
     float last_ms[8] = {};
@@ -344,6 +345,7 @@ static int build_device(Sim& S) {
     std::vector<uint8_t> mtype((size_t)nslot * npad, 0), mav((size_t)nslot * npad, 0);
     std::vector<double> mdens((size_t)nslot * npad, 0.0);
     S.unsupported_mod = 0;
+    S.nt_read = false;
     for (uint32_t di = 0; di < n; di++) {
         const Segment& s = sg[S.dev2host[di]];
         radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type;
@@ -365,9 +367,11 @@ static int build_device(Sim& S) {
             if (m.type == 82) camkii[di] = m.state[0];
             if (m.type == 83) calcn[di] = m.state[0];
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
+            if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
             if (!modifier_supported(m.type)) S.unsupported_mod++;
         }
         sflags[di] = fl;
+        if (fl & SF_NTREAD) S.nt_read = true;
     }
     // ---- synapses, slot order ----
     uint32_t nrec = 1;
@@ -435,6 +439,36 @@ static int build_device(Sim& S) {
             last_onto[sy[i].post_segment] = i;
         }
     }
+    // Glia: per network, its first 256 segments carrying GLAST (simulation_host.cpp:350-354).
+    // Every chemical synapse of such a network captures into the glial segment nearest its post
+    // segment in depth, a strictly nearer one replacing the first (simulation_host.cpp:477-482).
+    std::vector<uint32_t> y_glia(ns, UINT32_MAX);   // per slot: device index of that glial segment
+    uint32_t nglia_syn = 0;
+    for (size_t k = 0; k + 1 < S.net_seg_off.size(); k++) {
+        const uint32_t s0 = S.net_seg_off[k], s1 = S.net_seg_off[k + 1];
+        std::vector<uint32_t> glia;
+        for (uint32_t h = s0; h < s1 && glia.size() < 256; h++)
+            for (uint32_t j = 0; j < sg[h].modifier_count; j++)
+                if (sg[h].modifiers[j].type == M_GLAST) { glia.push_back(h); break; }
+        if (glia.empty()) continue;
+        std::vector<uint32_t> best_of(s1 - s0, UINT32_MAX);   // per post segment
+        for (uint32_t i = S.net_syn_off[k]; i < S.net_syn_off[k + 1]; i++) {
+            const Synapse& y = sy[i];
+            if (y.type == 1 || y.type == 2 || y.pre_segment >= n || y.post_segment >= n) continue;
+            uint32_t& b = best_of[y.post_segment - s0];
+            if (b == UINT32_MAX) {
+                const float pz = sg[y.post_segment].position.z;
+                float best = 1e30f;
+                b = glia[0];
+                for (uint32_t g : glia) {
+                    const float dz = fabsf(sg[g].position.z - pz);
+                    if (dz < best) { best = dz; b = g; }
+                }
+            }
+            y_glia[syn2slot[i]] = S.host2dev[b];
+            nglia_syn++;
+        }
+    }
     std::vector<uint32_t> y_pre(ns), y_id(ns), y_prew(ns), watched;
     std::vector<int64_t> watch_idx(n, -1);
     std::vector<uint8_t> y_flags(ns, 0), y_rcount(ns, 0), y_refr(ns, 0);
@@ -458,6 +492,7 @@ static int build_device(Sim& S) {
         y_id[k] = y.id;
         uint8_t fl = 0;
         if (y.type == 1 || y.type == 2) { fl |= YF_SKIP | YF_GAP; ngap++; }
+        if (y_glia[k] != UINT32_MAX) fl |= YF_GLIA;
         if (y.pre_autoreceptor_count > 0) S.unsupported_syn++;
         for (uint32_t j = 0; j < y.post_receptor_count; j++) {
             uint8_t rt = y.post_receptors[j].type;
@@ -498,11 +533,7 @@ static int build_device(Sim& S) {
         // when no earlier synapse targets that presynaptic segment.
         if (src[k] >= 0 && y.delay <= 0.0f && y.RRP_max > 0.0f) S.hazard_syn++;
     }
-    for (uint32_t i = 0; i < n; i++) {   // glial GLAST capture is not covered
-        const Segment& s = sg[i];
-        for (uint32_t j = 0; j < s.modifier_count; j++)
-            if (s.modifiers[j].type == 147) { S.unsupported_mod++; break; }
-    }
+    if (nglia_syn) S.receptors_only = false;   // captures merge into the full post-side walk
 
     // transmitter diffusion coefficients (types.h:436-443), host powf as the reference
     static const float kMass[15] = {147.0f, 103.0f, 153.0f, 176.0f, 146.0f, 267.0f, 169.0f, 75.0f,
@@ -655,6 +686,32 @@ static int build_device(Sim& S) {
         d.gap_slot = S.dupload(gslot);
         d.gap_sid = S.dupload(gsid);
         d.y_igap = S.dalloc<float>(ns);
+    }
+    // ---- glial captures: per receiving segment, its synapses in synapse-index order ----
+    d.y_cap = nullptr; d.y_capm = nullptr;
+    d.glia_off = nullptr; d.glia_slot = nullptr; d.glia_sid = nullptr;
+    if (nglia_syn) {
+        std::vector<uint32_t> coff(n + 1, 0), cslot(nglia_syn), csid(nglia_syn);
+        for (uint32_t k = 0; k < ns; k++)
+            if (y_glia[k] != UINT32_MAX) coff[y_glia[k] + 1]++;
+        for (uint32_t i = 0; i < n; i++) coff[i + 1] += coff[i];
+        std::vector<uint32_t> fill(coff.begin(), coff.end() - 1);
+        for (uint32_t i = 0; i < ns; i++) {
+            const uint32_t k = syn2slot[i];
+            if (y_glia[k] == UINT32_MAX) continue;
+            const uint32_t e = fill[y_glia[k]]++;
+            cslot[e] = k;
+            csid[e] = i;
+        }
+        if (!d.y_sid) {
+            std::vector<uint32_t> y_sid(S.slot2syn.begin(), S.slot2syn.end());
+            d.y_sid = S.dupload(y_sid);
+        }
+        d.glia_off = S.dupload(coff);
+        d.glia_slot = S.dupload(cslot);
+        d.glia_sid = S.dupload(csid);
+        d.y_cap = S.dalloc<float>((size_t)3 * ns);
+        d.y_capm = S.dalloc<uint8_t>(ns);
     }
     d.y_isyn = S.dalloc<float>(ns);
     d.y_pp1 = S.dupload(y_pp1);
@@ -903,7 +960,9 @@ static void launch_tick(Sim& S, float dt, bool tm) {
     const BchipSchedule& sc = S.sched;
     const uint32_t gy = (d.ns + 255) / 256;
     if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
-    k_modifiers<<<(d.n + sc.modifiers_block - 1) / sc.modifiers_block, sc.modifiers_block, 0, S.stream>>>(d, dt);
+    const uint32_t gm = (d.n + sc.modifiers_block - 1) / sc.modifiers_block;
+    if (S.nt_read) k_modifiers<true><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
+    else k_modifiers<false><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
         if (S.jac_res) launch_diffusion<true>(S, dt);

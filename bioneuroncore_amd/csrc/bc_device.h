@@ -33,13 +33,22 @@ enum : uint8_t {
     M_PUMP = 7, M_AMPA = 8, M_NMDA = 9, M_GABAA = 11, M_SK = 103, M_BK = 104, M_K2P = 107,
     M_KV7 = 112, M_KVDR = 123, M_HCN1 = 129, M_NALCN = 133, M_CAV21 = 140, M_NAV16 = 141,
     M_NAV12 = 142, M_KV31 = 150, M_KV11 = 156, M_CX36 = 114, M_CX43 = 198,
+    M_GLAST = 147, M_GLYT1 = 148, M_GAT3 = 149, M_EAAT3 = 183, M_GLYT2 = 184, M_GAT1 = 185,
 };
+
+// Modifiers that read the segment's extracellular transmitters in step 1 (the ionotropic
+// receptors) or also take them up (the transporters).
+__host__ __device__ static inline bool modifier_reads_nt(uint8_t t) {
+    return t == M_AMPA || t == M_NMDA || t == M_GABAA || (t >= M_GLAST && t <= M_GAT3) ||
+           (t >= M_EAAT3 && t <= M_GAT1);
+}
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
     switch (t) {
         case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
         case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
-        case M_NAV12: case M_KV31: case M_KV11: case M_CX36: case M_CX43:
+        case M_NAV12: case M_KV31: case M_KV11: case M_CX36: case M_CX43: case M_GLAST:
+        case M_GLYT1: case M_GAT3: case M_EAAT3: case M_GLYT2: case M_GAT1:
             return true;
         default:
             return false;
@@ -587,7 +596,7 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
         double g = m.get(m.av);
         if (g > 0.0f) gamma = g;
     }
-    float nt = s.nt[p.nt];
+    float nt = p.nt ? s.nt[1] : s.nt[0];   // glutamate (0) or GABA (1)
     float r_inf = nt / (nt + p.ec50);
     double s0 = m.get(0);
     float r = s0;
@@ -768,9 +777,54 @@ BC_DEV double ch_connexin(const ModSlot& m, SegRegs& s, float dt) {
     return 0.0;
 }
 
+// Na+-coupled transmitter transporters (modifier_helpers.h:418-455, hh_biological.cpp:725-780):
+// Michaelis-Menten uptake of transmitter NT from the segment's extracellular pool with NA Na+
+// (and CL Cl-) in and KO K+ out per cycle, NET charges; the turnover is limited by the
+// transmitter, the extracellular Na+ and, with K+ countertransport, the intracellular K+.
+// Returns the transporter current (pA). The charge-per-mole factor uses the volume rounded to
+// float, as the reference's helper does.
+template <int NT, int NA, int CL, int KO, int NET>
+BC_DEV double ch_transporter(const ModSlot& m, SegRegs& s, float dt, float km, float rel_per_mm) {
+    constexpr float kUaPerCm2ToPa = 1e-8f * 1e6f;
+    const float area = eff_area(m, s);
+    float nt = s.nt[NT];
+    if (nt < 0.0f) nt = 0.0f;
+    const float frac = nt / (nt + km);
+    // the density is a double in the record, so the product runs in double
+    const float I = (float)(2.0f * m.dens * area * kUaPerCm2ToPa * frac);
+    const double cpm = kFaraday * (double)(float)s.vol / 1000.0;
+    float tr = (float)((double)(I * dt) / ((double)(float)NET * cpm));
+    const float nt_av = nt / rel_per_mm;
+    const float na_av = (float)(s.ex[0] * (double)kEvr / (double)(float)NA);
+    if (tr > nt_av) tr = nt_av;
+    if (tr > na_av) tr = na_av;
+    if (KO && tr > s.in[1]) tr = (float)s.in[1];
+    if (tr < 0.0f) tr = 0.0f;
+    s.in[0] += (float)NA * tr;
+    s.ex[0] -= (float)NA * tr / kEvr;
+    if (CL) {
+        s.in[3] += (float)CL * tr;
+        s.ex[3] -= (float)CL * tr / kEvr;
+    }
+    if (KO) {
+        s.in[1] -= (float)KO * tr;
+        s.ex[1] += (float)KO * tr / kEvr;
+    }
+    float v = s.nt[NT] - tr * rel_per_mm;
+    if (v < 0.0f) v = 0.0f;
+    s.nt[NT] = v;
+    floor7(s.in);
+    floor7(s.ex);
+    return I;
+}
+
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
 // current and leave their state untouched, as the reference's device switch does.
+// kNtMods = false: the caller has no modifier that reads transmitters (no ionotropic receptor
+// or transporter; see modifier_reads_nt), and those cases are left out.
+template <bool kNtMods = true>
 BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
+    if (!kNtMods && modifier_reads_nt(type)) return 0.0;
     switch (type) {
         case M_PUMP: return ch_pump(m, s, dt);
         case M_AMPA: {
@@ -801,6 +855,12 @@ BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
         case M_KV31: return ch_kv31(m, s, dt);
         case M_KV11: return ch_kv11(m, s, dt);
         case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
+        case M_GLAST: return ch_transporter<0, 3, 0, 1, 2>(m, s, dt, 10.0f, 667.0f);
+        case M_GLYT1: return ch_transporter<7, 2, 1, 0, 1>(m, s, dt, 30.0f, 667.0f);
+        case M_GAT3: return ch_transporter<1, 2, 1, 0, 1>(m, s, dt, 0.5f, 667.0f);
+        case M_EAAT3: return ch_transporter<0, 3, 0, 1, 2>(m, s, dt, 20.0f, 667.0f);
+        case M_GLYT2: return ch_transporter<7, 3, 1, 0, 2>(m, s, dt, 55.7f, 667.0f);
+        case M_GAT1: return ch_transporter<1, 3, 1, 0, 2>(m, s, dt, 4.7f, 667.0f);
         default: return 0.0;
     }
 }

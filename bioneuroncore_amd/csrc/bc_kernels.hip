@@ -45,6 +45,9 @@ __device__ static inline uint64_t delay_ticks(float delay, float dt) {   // dela
 }
 
 // ------------------------------------------------------------------ step 1
+// kNt: some segment carries a modifier that reads or takes up transmitters (SF_NTREAD); the
+// variant without keeps those registers free.
+template <bool kNt>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_modifiers(Net d, float dt) {
     exp_table_stage();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,7 +70,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     s.camkii = d.camkii[i];
     seg_consts_init(s);
     ghk_cache_init(s);
-    const bool my = d.sflags[i] & SF_MYELIN;
+    const uint8_t sfl = d.sflags[i];
+    if (kNt && (sfl & SF_NTREAD)) {   // step 1 reads glutamate (0), GABA (1) and glycine (7) only
+        s.nt[0] = d.nt[0 * P + i];
+        s.nt[1] = d.nt[1 * P + i];
+        s.nt[7] = d.nt[7 * P + i];
+    }
+    const bool my = sfl & SF_MYELIN;
     const uint32_t cnt = d.mcount[i];
     double itot = 0.0;
     for (uint32_t j = 0; j < cnt; j++) {
@@ -79,7 +88,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         m.dens = d.mdens[j * P + i];
         m.a7 = s.area;
         m.av = d.mav[j * P + i];
-        itot += dispatch(t, m, s, dt);
+        itot += dispatch<kNt>(t, m, s, dt);
         if (d.cable) {   // concentrations held: every channel sees the stored ones
 #pragma unroll
             for (int k = 0; k < 7; k++) {
@@ -87,7 +96,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 s.ex[k] = d.ex[k * P + i];
             }
             s.atp = d.atp[i];
+            if (kNt && (sfl & SF_NTREAD)) {
+                s.nt[0] = d.nt[0 * P + i];
+                s.nt[1] = d.nt[1 * P + i];
+                s.nt[7] = d.nt[7 * P + i];
+            }
         }
+    }
+    if (kNt && (sfl & SF_NTREAD) && !d.cable) {   // the transporters' uptake (before injections land)
+        d.nt[0 * P + i] = s.nt[0];
+        d.nt[1 * P + i] = s.nt[1];
+        d.nt[7 * P + i] = s.nt[7];
     }
     // stimuli (simulation_host.cpp:247-300): this segment's list, in the reference's order
     const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
@@ -864,6 +883,30 @@ __device__ static void gap_junction(const Net& d, uint32_t y, float dt) {
     d.y_igap[y] = out;
 }
 
+// Glial capture (simulation_host.cpp:472-490): of the cleft's glutamate, glycine and GABA after
+// this tick's releases, the fraction 1 - exp(-dt / tau) (tau: the clearance time, 5 ms if not
+// positive) goes to the glial segment nearest the post segment. Stored per synapse here; the
+// post-side pass adds it to that segment's transmitters in synapse order and the plasticity
+// pass takes it out of the cleft.
+__device__ static void glial_capture(const Net& d, uint32_t y, float dt) {
+    const size_t NS = d.ns;
+    const uint16_t cm = d.y_cleft_mask[y];
+    float tau = d.y_tau[y];
+    tau = (tau > 0.0f) ? tau : 5.0f;
+    const double keep = 1.0f - bc::exp(-dt / tau);
+    uint8_t got = 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int nt = q == 0 ? 0 : (q == 1 ? 7 : 1);
+        if (!(cm & (1u << nt))) continue;   // outside the mask the cleft holds +0
+        const float c = d.y_cleft[nt * NS + y];
+        if (c <= 0.0f) continue;
+        d.y_cap[q * NS + y] = (float)(c * keep);
+        got |= 1u << q;
+    }
+    d.y_capm[y] = got;
+}
+
 __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     exp_table_stage();
     const uint64_t tick = *d.tick;
@@ -964,6 +1007,7 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
         }
         if (refr > 0) refr--;
         d.y_refr[y] = refr;
+        if (d.y_flags[y] & YF_GLIA) glial_capture(d, y, dt);
     }
 }
 
@@ -1025,6 +1069,17 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
         // I_buffer in synapse-index order among the chain (simulation_host.cpp:372-400)
         uint32_t go = 0, ge = 0;
         if (d.gap_off) { go = d.gap_off[i]; ge = d.gap_off[i + 1]; }
+        // glial captures this segment receives, merged in synapse-index order (each synapse
+        // captures after its own deposit)
+        uint32_t co = 0, ce = 0;
+        if (!kReceptorsOnly && d.glia_off) { co = d.glia_off[i]; ce = d.glia_off[i + 1]; }
+        auto take = [&](uint32_t e) {
+            const uint32_t sl = d.glia_slot[e];
+            const uint8_t m = d.y_capm[sl];
+            if (m & 1) s.nt[0] += d.y_cap[0 * NS + sl];
+            if (m & 2) s.nt[7] += d.y_cap[1 * NS + sl];
+            if (m & 4) s.nt[1] += d.y_cap[2 * NS + sl];
+        };
         for (uint32_t q = 0; q < len; q++) {
             // the lanes still walking are lanes 0..cnt-1 of this wavefront (sorted by length)
             const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
@@ -1041,9 +1096,15 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
                 continue;
             }
             const uint16_t cm = d.y_cleft_mask[y];
+            uint32_t sid = 0;
+            if (co < ce) {
+                sid = d.y_sid[y];
+                for (; co < ce && d.glia_sid[co] < sid; co++) take(co);
+            }
 #pragma unroll
             for (int k = 0; k < kNtWalk; k++)
                 if (cm & (1u << k)) s.nt[k] += d.y_cleft[k * NS + y];
+            if (co < ce && d.glia_sid[co] == sid) take(co++);
             float isyn = 0.0f;
             const uint32_t rc = d.y_rcount[y];
             const float parea = d.y_post_area[y];
@@ -1064,6 +1125,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
             if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
         }
         for (; go < ge; go++) ibuf -= d.y_igap[d.gap_slot[go]];
+        for (; co < ce; co++) take(co);
         if (kReceptorsOnly) {
             // the same chain again, for the transmitters the receptors do not read
             uint32_t off2 = d.wave_base[lane >> 6];
@@ -1186,12 +1248,20 @@ __global__ __launch_bounds__(256) void k_syn_plast(Net d, float dt) {
     {
         const float tau = d.y_tau[y];
         const double E = bc::exp_t(-dt / tau, ExpTabGlobal{});
+        // with glia, glutamate, glycine and GABA leave the cleft by capture only
+        const uint8_t capm = (yf & YF_GLIA) ? d.y_capm[y] : 0;
 #pragma unroll
         for (int k = 0; k < 15; k++)
             if (cm & (1u << k)) {
                 float v = d.y_cleft[k * NS + y];
-                v *= E;
-                if (v < 0.0f) v = 0.0f;
+                if ((yf & YF_GLIA) && (k == 0 || k == 7 || k == 1)) {
+                    const int q = k == 0 ? 0 : (k == 7 ? 1 : 2);
+                    if (!(capm & (1u << q))) continue;
+                    v -= d.y_cap[q * NS + y];
+                } else {
+                    v *= E;
+                    if (v < 0.0f) v = 0.0f;
+                }
                 d.y_cleft[k * NS + y] = v;
             }
     }

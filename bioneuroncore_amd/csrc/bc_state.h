@@ -27,6 +27,7 @@ enum : uint8_t {
     SF_BATH_NMDA = 2,     // carries an NMDA segment modifier: glutamate bath 1.0
     SF_BATH_5HT = 4,      // carries a 5-HT7 segment modifier: serotonin bath 0.0
     SF_WATCHED = 8,       // is the presynaptic segment of some synapse
+    SF_NTREAD = 16,       // a modifier reads (receptors) or takes up (transporters) transmitters
 };
 
 // Synapse flag bits (static).
@@ -34,6 +35,7 @@ enum : uint8_t {
     YF_INHIBITORY = 1,    // a post receptor is GABA-A / GABA-B / glycine
     YF_SKIP = 2,          // pre/post out of range or electrical: no chemical step runs
     YF_GAP = 4,           // electrical (gap junction); also YF_SKIP
+    YF_GLIA = 8,          // its network has glia: glutamate, glycine and GABA are captured
 };
 
 // Tiles of one size class, launched together.
@@ -203,6 +205,14 @@ struct Net {
     const uint32_t* gap_off;     // [n+1] per segment: its electrical synapses as the presynaptic end
     const uint32_t* gap_slot;    //   their slots
     const uint32_t* gap_sid;     //   their synapse indices (ascending per segment)
+    // glial capture (null when no network has glia): each chemical synapse of a network with
+    // GLAST segments hands part of its cleft glutamate (0), glycine (7) and GABA (1) to the
+    // glial segment nearest its post segment in depth
+    float* y_cap;                // [3][ns] amount captured this tick (species 0, 7, 1)
+    uint8_t* y_capm;             // bit q: species q was captured this tick
+    const uint32_t* glia_off;    // [n+1] per segment: the synapses whose captures it receives
+    const uint32_t* glia_slot;   //   their slots
+    const uint32_t* glia_sid;    //   their synapse indices (ascending per segment)
 
     float nt_diff[kNtCount];     // transmitter diffusion coefficients
 };

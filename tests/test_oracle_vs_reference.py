@@ -104,3 +104,11 @@ def test_transporters_take_up_injected_transmitters():
     rs, _, _ = _both(net, 200, lambda t: H.Stimuli(current=cur, lig=lig if t % 40 == 2 else None))
     got = rs["neurotransmitter"][lig["segment_id"], lig["nt_idx"]]
     assert np.all(got < 5 * lig["amount"]) and np.all(got > 0.0)
+
+
+def test_ionotropic_receptors_as_segment_modifiers():
+    """AMPA, NMDA and GABA-A as segment modifiers: step 1 reads the segment's transmitters."""
+    net = bion.segment_receptor_network(13)
+    cur = H.soma_clamp(net, 160.0)
+    rs, _, sp = _both(net, 300, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
