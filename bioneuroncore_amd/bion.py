@@ -75,8 +75,11 @@ CX36, CX43 = 114, 198   # connexins (gap junctions read their connexon count and
 # capture) and neuronal.
 GLAST, GLYT1, GAT3, EAAT3, GLYT2, GAT1 = 147, 148, 149, 183, 184, 185
 TRANSPORTERS = (GLAST, GLYT1, GAT3, EAAT3, GLYT2, GAT1)
+# Ion homeostasis: pump isoforms, Ca2+ extrusion and buffering, cation-chloride cotransport.
+HCN2, PUMP_A3, PUMP_GLIA, PMCA, NCX, KCC2, NKCC1, CA_BUFFER = 130, 197, 110, 98, 99, 100, 122, 102
+HOMEOSTASIS = (HCN2, PUMP_A3, PUMP_GLIA, PMCA, NCX, KCC2, NKCC1, CA_BUFFER)
 SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -86,7 +89,8 @@ EXTRA_IONS = np.array([140.0, 3.0, 1.2, 130.0, 2.0, 0.0, 4.0e-5, 0.0], dtype=np.
 CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, PUMP: 1500.0,
                  CAV21: 100.0, HCN1: 120.0, SK: 6.0, BK: 0.15, KV11: 0.346, KV31: 2.0, KV7: 2.0,
                  CX36: 400.0, CX43: 60.0, GLAST: 1.0, GLYT1: 1.0, GAT3: 1.0, EAAT3: 1.0,
-                 GLYT2: 1.0, GAT1: 1.0}
+                 GLYT2: 1.0, GAT1: 1.0, HCN2: 120.0, PUMP_A3: 1500.0, PUMP_GLIA: 1500.0, PMCA: 300.0,
+                 NCX: 30.0, KCC2: 0.093, NKCC1: 0.1, CA_BUFFER: 0.5}
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -376,6 +380,41 @@ def glia_network(seed: int, n_neurons: int = 16, seg_per_neuron: int = 6, n_glia
     syn = syn[rng.permutation(len(syn))]
     syn["id"] = np.arange(1, len(syn) + 1, dtype=np.uint32)
     net.synapses = syn
+    return net
+
+
+def homeostasis_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6, n_glia: int = 4) -> Network:
+    """Chain neurons whose somas carry the ion-homeostasis set (alpha3 pump, PMCA, NCX, KCC2,
+    NKCC1, Ca2+ buffer, HCN2) next to Cav2.1, dendrites with PMCA and the Ca2+ buffer, AMPA+NMDA /
+    GABA-A synapses (NMDA and Cav2.1 bring Ca2+ in), plus glial cells with the glial pump, K2P,
+    NKCC1, PMCA and the buffer."""
+    soma = [CAV21, PUMP_A3, PMCA, NCX, KCC2, NKCC1, CA_BUFFER, HCN2]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], soma,
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    rng = np.random.default_rng(seed + 3)
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        for j, (t, d) in enumerate(((PMCA, 300.0), (CA_BUFFER, 0.5))):
+            segs["modifiers"]["type"][i, k + j] = t
+            segs["modifiers"]["density"][i, k + j] = d
+        segs["modifier_count"][i] = k + 2
+    segs["intra_ions"][:, 2] *= rng.uniform(1.0, 20.0, len(segs))   # some Ca loads to remove
+    g = np.zeros(n_glia, SEGMENT_DT)
+    _init_segment_state(g, rng)
+    g["segment_type"], g["radius"], g["length"] = 1, 4.0, 8.0
+    g["neuron_id"] = n_neurons + np.arange(n_glia)
+    g["intra_ions"][:, 3] = 25.0
+    gt = [PUMP_GLIA, K2P, NKCC1, PMCA, CA_BUFFER]
+    _set_modifiers(g, gt, np.array([[CANON_DENSITY[t] for t in gt]] * n_glia, np.float64))
+    gneu = np.zeros(n_glia, NEURON_DT)
+    gneu["id"] = n_neurons + np.arange(n_glia)
+    gneu["segment_start"] = gneu["soma_index"] = len(segs) + np.arange(n_glia)
+    gneu["segment_count"] = 1
+    gneu["threshold"] = -40.0
+    net.segments = np.concatenate([segs, g])
+    net.segments["id"] = np.arange(len(net.segments), dtype=np.uint32)
+    net.neurons = np.concatenate([net.neurons, gneu])
     return net
 
 

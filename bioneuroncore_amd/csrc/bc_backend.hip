@@ -138,6 +138,7 @@ struct Sim {
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
     bool receptors_only = true;   // every receptor slot is AMPA, NMDA or GABA-A
     bool nt_read = false;         // some segment modifier reads or takes up transmitters
+    bool ext_mods = false;        // some segment modifier is outside the core channel set
     std::vector<uint8_t> seg_dispatched;   // [n*32] slot ran at least once (state[7] written)
     hipEvent_t ev[8] = {};
     float last_ms[8] = {};
@@ -346,6 +347,7 @@ static int build_device(Sim& S) {
     std::vector<double> mdens((size_t)nslot * npad, 0.0);
     S.unsupported_mod = 0;
     S.nt_read = false;
+    S.ext_mods = false;
     for (uint32_t di = 0; di < n; di++) {
         const Segment& s = sg[S.dev2host[di]];
         radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type;
@@ -368,7 +370,12 @@ static int build_device(Sim& S) {
             if (m.type == 83) calcn[di] = m.state[0];
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
+            if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
             if (!modifier_supported(m.type)) S.unsupported_mod++;
+            // PMCA's Hill powers are exact squares only for n = 2 and a float-valued K_m
+            if (m.type == M_PMCA && ((m.state[2] > 0.0f && m.state[2] != 2.0) ||
+                                     (m.state[1] > 0.0f && (double)(float)m.state[1] != m.state[1])))
+                S.unsupported_mod++;
         }
         sflags[di] = fl;
         if (fl & SF_NTREAD) S.nt_read = true;
@@ -961,8 +968,13 @@ static void launch_tick(Sim& S, float dt, bool tm) {
     const uint32_t gy = (d.ns + 255) / 256;
     if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
     const uint32_t gm = (d.n + sc.modifiers_block - 1) / sc.modifiers_block;
-    if (S.nt_read) k_modifiers<true><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
-    else k_modifiers<false><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
+    if (S.ext_mods) {
+        if (S.nt_read) k_modifiers<true, true><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
+        else k_modifiers<false, true><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
+    } else {
+        if (S.nt_read) k_modifiers<true, false><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
+        else k_modifiers<false, false><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
+    }
     if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
     if (d.has_conn) {
         if (S.jac_res) launch_diffusion<true>(S, dt);

@@ -34,7 +34,22 @@ enum : uint8_t {
     M_KV7 = 112, M_KVDR = 123, M_HCN1 = 129, M_NALCN = 133, M_CAV21 = 140, M_NAV16 = 141,
     M_NAV12 = 142, M_KV31 = 150, M_KV11 = 156, M_CX36 = 114, M_CX43 = 198,
     M_GLAST = 147, M_GLYT1 = 148, M_GAT3 = 149, M_EAAT3 = 183, M_GLYT2 = 184, M_GAT1 = 185,
+    M_HCN2 = 130, M_PUMP_A3 = 197, M_PUMP_GLIA = 110, M_PMCA = 98, M_NCX = 99, M_KCC2 = 100,
+    M_NKCC1 = 122, M_CA_BUFFER = 102,
 };
+
+// The core channel set: pump, ionotropic receptors, the K+/Na+/Ca2+/HCN/NALCN channels of the
+// tree and chain workloads. Every other covered ID is an extended type.
+__host__ __device__ static inline bool modifier_core(uint8_t t) {
+    switch (t) {
+        case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
+        case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
+        case M_NAV12: case M_KV31: case M_KV11:
+            return true;
+        default:
+            return false;
+    }
+}
 
 // Modifiers that read the segment's extracellular transmitters in step 1 (the ionotropic
 // receptors) or also take them up (the transporters).
@@ -48,7 +63,9 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
         case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
         case M_NAV12: case M_KV31: case M_KV11: case M_CX36: case M_CX43: case M_GLAST:
-        case M_GLYT1: case M_GAT3: case M_EAAT3: case M_GLYT2: case M_GAT1:
+        case M_GLYT1: case M_GAT3: case M_EAAT3: case M_GLYT2: case M_GAT1: case M_HCN2:
+        case M_PUMP_A3: case M_PUMP_GLIA: case M_PMCA: case M_NCX: case M_KCC2: case M_NKCC1:
+        case M_CA_BUFFER:
             return true;
         default:
             return false;
@@ -241,8 +258,7 @@ BC_DEV float bk_po(float V, float ca) {
     float num = L * o;
     return num / (num + c);
 }
-BC_DEV float pump_vnorm(float V, float na, float ko, float atp_level) {
-    const float kmna = 10.0f, kmk = 1.5f;
+BC_DEV float pump_vnorm(float V, float na, float ko, float atp_level, float kmna = 10.0f, float kmk = 1.5f) {
     float na3 = na * na * na, k2 = ko * ko;
     float kna3 = kmna * kmna * kmna, kk2 = kmk * kmk;
     float fna = na3 / (na3 + kna3), fk = k2 / (k2 + kk2);
@@ -263,9 +279,19 @@ BC_DEV float pump_vnorm(float V, float na, float ko, float atp_level) {
 }
 
 // ---- channel currents: return pA (reference sign convention) ----
-BC_DEV double ch_pump(const ModSlot& m, SegRegs& s, float dt) {   // hh_biological.cpp:57-97
+// Na/K-ATPase (7), its alpha3 isoform (197: K_m Na 25 mM, ouabain block by state[5] nM, IC50
+// 20 nM) and the glial alpha2 pump (110: K_m K 1 mM) (hh_biological.cpp:57-141, 3414-3445).
+enum : int { PUMP_A1 = 0, PUMP_A3 = 1, PUMP_GLIA = 2 };
+template <int kKind = PUMP_A1>
+BC_DEV double ch_pump(const ModSlot& m, SegRegs& s, float dt) {
     float area = eff_area(m, s);
-    float vn = pump_vnorm(s.V, s.in[0], s.ex[1], s.atp);
+    float vn = pump_vnorm(s.V, s.in[0], s.ex[1], s.atp, kKind == PUMP_A3 ? 25.0f : 10.0f,
+                          kKind == PUMP_GLIA ? 1.0f : 1.5f);
+    if (kKind == PUMP_A3) {
+        double s5 = m.get(5);
+        double oua = (s5 > 0.0f) ? s5 : 0.0f;
+        vn /= (1.0f + oua / 20.0f);
+    }
     double N = m.dens * area;
     float ip = N * vn * 75.0f * 1.602176634e-19f * kAtoPa;
     float vol = s.vol;
@@ -524,14 +550,19 @@ BC_DEV double ch_kv7(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologica
     return k_out(s, ia * kAtoPa, dt);
 }
 
-BC_DEV double ch_hcn1(const ModSlot& m, SegRegs& s, float dt) {   // modifier_helpers.h:1442-1484
+// HCN1 (129: V1/2 -69 mV, tau 84 ms, 0.84 pS) and HCN2 (kHcn2, 130: -95 mV, 557 ms, 1.54 pS)
+// (modifier_helpers.h:1442-1484, hh_biological.cpp:3598-3599)
+template <bool kHcn2 = false>
+BC_DEV double ch_hcn1(const ModSlot& m, SegRegs& s, float dt) {
+    constexpr float kVh = kHcn2 ? -95.0f : -69.0f, kTopen = kHcn2 ? 557.0f : 84.0f;
+    constexpr float kGamma = kHcn2 ? 1.54f : 0.84f;
     double V = s.V;
     float area = eff_area(m, s);
     double s1 = m.get(1), s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
-    float vh = (s1 != 0) ? s1 : -69.0f;
+    float vh = (s1 != 0) ? s1 : kVh;
     float k = (s2 > 0) ? s2 : 6.0f;
-    float gamma = (s3 > 0) ? s3 : 0.84f;
-    float t_open = (s4 > 0) ? s4 : 84.0f;
+    float gamma = (s3 > 0) ? s3 : kGamma;
+    float t_open = (s4 > 0) ? s4 : kTopen;
     float t_close = (s5 > 0) ? s5 : 300.0f;
     float camp = s.camp;
     float ve = vh + 15.0f * camp / (camp + 0.5f);
@@ -543,7 +574,7 @@ BC_DEV double ch_hcn1(const ModSlot& m, SegRegs& s, float dt) {   // modifier_he
     if (mv < 0.0f) mv = 0.0f;
     if (mv > 1.0f) mv = 1.0f;
     m.set(0, mv);
-    const P3 pp = (s3 > 0) ? p_partial3(gamma, 0.2f, 0.8f, 0.0f) : p_partial3(0.84f, 0.2f, 0.8f, 0.0f);
+    const P3 pp = (s3 > 0) ? p_partial3(gamma, 0.2f, 0.8f, 0.0f) : p_partial3(kGamma, 0.2f, 0.8f, 0.0f);
     double pna = pp.na, pk = pp.k;
     double N = m.dens * area;
     double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
@@ -777,6 +808,183 @@ BC_DEV double ch_connexin(const ModSlot& m, SegRegs& s, float dt) {
     return 0.0;
 }
 
+// PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
+// takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
+// covered) both are exact squares in double.
+BC_DEV double ch_pmca(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    double s1 = m.get(1);
+    double km = (s1 > 0.0f) ? s1 : 0.0003f;
+    float atp = s.atp;
+    if (atp < 0.0f) atp = 0.0f;
+    float f_atp = atp / (atp + 0.2f);
+    float ca = s.in[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float ca_pow = (double)ca * (double)ca;
+    float f_ca = ca_pow / (ca_pow + km * km);
+    double N = m.dens * area;
+    float ip = N * 100.0f * f_ca * f_atp * 1.602176634e-19f * 2.0f * kAtoPa;
+    float vol = s.vol;
+    double cpv = kFaraday * vol / 1000.0;
+    float tr = ip * dt / (2.0f * cpv);
+    float ca_av = s.in[2] - 1e-9f;
+    if (ca_av < 0.0f) ca_av = 0.0f;
+    if (tr > ca_av) tr = ca_av;
+    s.in[2] -= tr;
+    s.ex[2] += tr / kEvr;
+    floor7(s.in);
+    s.atp -= ip * dt * 1e-8f;
+    if (s.atp < 0.0f) s.atp = 0.0f;
+    return -ip;
+}
+
+// NCX (99, hh_biological.cpp:2616-2646, modifier_helpers.h:1181-1201): 3 Na+ in / 1 Ca2+ out,
+// voltage-dependent (eta 0.35), allosteric Ca2+ activation, either direction.
+BC_DEV double ch_ncx(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    float na_i = s.in[0]; if (na_i < 1e-9f) na_i = 1e-9f;
+    float na_o = s.ex[0]; if (na_o < 1e-9f) na_o = 1e-9f;
+    float ca_i = s.in[2]; if (ca_i < 1e-9f) ca_i = 1e-9f;
+    float ca_o = s.ex[2]; if (ca_o < 1e-9f) ca_o = 1e-9f;
+    float vv = (float)s.V * 1e-3f;
+    float frt = kFaraday / (kGas * kTemp);
+    float e_eta = bc::exp(0.35f * frt * vv);
+    float e_eta1 = bc::exp((0.35f - 1.0f) * frt * vv);
+    float nai3 = na_i * na_i * na_i;
+    float nao3 = na_o * na_o * na_o;
+    float fwd = e_eta1 * nao3 * ca_i;
+    float rev = e_eta * nai3 * ca_o;
+    float den = 1.0f + 0.27f * e_eta1;
+    float cr = 0.000125f / ca_i;
+    float allo = 1.0f / (1.0f + cr * cr);
+    float fn = allo * (fwd - rev) / (den * 530.2f);
+    double N = m.dens * area;
+    float I = N * 500.0f * fn * 1.602176634e-19f * kAtoPa;
+    float vol = s.vol;
+    double cpv = kFaraday * vol / 1000.0;
+    float tr = I * dt / cpv;
+    if (tr > ca_i) tr = ca_i;
+    if (tr < -na_i / 3.0f) tr = -na_i / 3.0f;
+    s.in[2] -= tr;
+    s.ex[2] += tr / kEvr;
+    s.in[0] += 3.0f * tr;
+    s.ex[0] -= 3.0f * tr / kEvr;
+    floor7(s.in);
+    return I;
+}
+
+// KCC2 (100, hh_biological.cpp:2648-2703): electroneutral K+-Cl- cotransport down the product
+// gradient, limited by what the losing side holds. No current.
+BC_DEV double ch_kcc2(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    double s1 = m.get(1), s2 = m.get(2);
+    double km_k = (s1 > 0.0f) ? s1 : 6.5f;
+    double km_cl = (s2 > 0.0f) ? s2 : 6.5f;
+    float k_i = s.in[1]; if (k_i < 1e-9f) k_i = 1e-9f;
+    float cl_i = s.in[3]; if (cl_i < 1e-9f) cl_i = 1e-9f;
+    float k_o = s.ex[1]; if (k_o < 1e-9f) k_o = 1e-9f;
+    float cl_o = s.ex[3]; if (cl_o < 1e-9f) cl_o = 1e-9f;
+    float f_k = k_i / (k_i + km_k);
+    float f_cl = cl_i / (cl_i + km_cl);
+    float drv = 1.0f - (k_o * cl_o) / (k_i * cl_i);
+    if (drv > 1.0f) drv = 1.0f;
+    if (drv < -1.0f) drv = -1.0f;
+    float flux = 5.0f * 1e-20f * m.dens * area * f_k * f_cl * drv * dt;
+    float dc = flux / (s.vol * 1e-18f);
+    if (dc > 0.0f) {
+        float ka = k_i - 1e-9f, cla = cl_i - 1e-9f;
+        float mn = (ka < cla) ? ka : cla;
+        if (mn < 0.0f) mn = 0.0f;
+        if (dc > mn) dc = mn;
+    } else {
+        float ka = (k_o - 1e-9f) * kEvr, cla = (cl_o - 1e-9f) * kEvr;
+        float mn = (ka < cla) ? ka : cla;
+        if (mn < 0.0f) mn = 0.0f;
+        float mag = -dc;
+        if (mag > mn) mag = mn;
+        dc = -mag;
+    }
+    s.in[1] -= dc;
+    s.in[3] -= dc;
+    s.ex[1] += dc / kEvr;
+    s.ex[3] += dc / kEvr;
+    floor7(s.in);
+    return 0.0;
+}
+
+// NKCC1 (122, hh_biological.cpp:2705-2760): Na+-K+-2Cl- uptake, trans-stimulated by external
+// Na+ and trans-inhibited by external K+. No current.
+BC_DEV double ch_nkcc1(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    double s1 = m.get(1), s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
+    double km_nai = (s1 > 0.0f) ? s1 : 10.0f;
+    double km_ki = (s2 > 0.0f) ? s2 : 15.0f;
+    double km_cli = (s3 > 0.0f) ? s3 : 50.0f;
+    double km_nao = (s4 > 0.0f) ? s4 : 0.5f;
+    double km_ko = (s5 > 0.0f) ? s5 : 0.2f;
+    float na_i = s.in[0], k_i = s.in[1], cl_i = s.in[3], na_o = s.ex[0], k_o = s.ex[1];
+    if (na_i < 1e-9f) na_i = 1e-9f;
+    if (k_i < 1e-9f) k_i = 1e-9f;
+    if (cl_i < 1e-9f) cl_i = 1e-9f;
+    if (na_o < 1e-9f) na_o = 1e-9f;
+    if (k_o < 1e-9f) k_o = 1e-9f;
+    float fna = na_i / (na_i + km_nai);
+    float fk = k_i / (k_i + km_ki);
+    float fcl = cl_i / (cl_i + km_cli);
+    float stim = na_o / (na_o + km_nao);
+    float inhib = 1.0f / (1.0f + k_o / km_ko);
+    float flux = 7.0f * 1e-20f * m.dens * area * fna * fk * fcl * stim * inhib * dt;
+    float dc = flux / (s.vol * 1e-18f);
+    float naa = (s.ex[0] - 1e-9f) * kEvr;
+    float ka = (s.ex[1] - 1e-9f) * kEvr;
+    float cla = (s.ex[3] - 1e-9f) * kEvr / 2.0f;
+    float mn = (naa < ka) ? naa : ka;
+    if (cla < mn) mn = cla;
+    if (mn < 0.0f) mn = 0.0f;
+    if (dc > mn) dc = mn;
+    s.in[0] += dc;
+    s.in[1] += dc;
+    s.in[3] += 2.0f * dc;
+    s.ex[0] -= dc / kEvr;
+    s.ex[1] -= dc / kEvr;
+    s.ex[3] -= 2.0f * dc / kEvr;
+    floor7(s.in);
+    return 0.0;
+}
+
+// Ca2+ buffer (102, hh_biological.cpp:2825-2865): first-order binding to a pool of capacity
+// state[0] (0.15 mM) x density; state[1] is the bound Ca (set to equilibrium when not positive).
+BC_DEV double ch_ca_buffer(const ModSlot& m, SegRegs& s, float dt) {
+    double s0 = m.get(0), s2 = m.get(2), s3 = m.get(3), s4 = m.get(4);
+    double bref = (s0 > 0.0f) ? s0 : 0.15f;
+    double btot = bref * m.dens;
+    double kd = (s2 > 0.0f) ? s2 : 0.0005f;
+    double kon = (s3 > 0.0f) ? s3 : 0.1f;
+    double koff = (s4 > 0.0f) ? s4 : (kd * kon);
+    double bound = m.get(1);
+    if (bound <= 0.0f && btot > 0.0f) {
+        float c0 = s.in[2];
+        bound = btot * c0 / (c0 + kd);
+    }
+    float cf = s.in[2];
+    if (cf < 1e-9f) cf = 1e-9f;
+    float bf = btot - bound;
+    if (bf < 0.0f) bf = 0.0f;
+    float db = (kon * cf * bf - koff * bound) * dt;
+    bound += db;
+    if (bound < 0.0f) bound = 0.0f;
+    if (bound > btot) bound = btot;
+    s.in[2] -= db;
+    if (s.in[2] < 1e-9f) {
+        float def = 1e-9f - s.in[2];
+        s.in[2] = 1e-9f;
+        bound -= def;
+        if (bound < 0.0f) bound = 0.0f;
+    }
+    m.set(1, bound);
+    return 0.0;
+}
+
 // Na+-coupled transmitter transporters (modifier_helpers.h:418-455, hh_biological.cpp:725-780):
 // Michaelis-Menten uptake of transmitter NT from the segment's extracellular pool with NA Na+
 // (and CL Cl-) in and KO K+ out per cycle, NET charges; the turnover is limited by the
@@ -820,11 +1028,36 @@ BC_DEV double ch_transporter(const ModSlot& m, SegRegs& s, float dt, float km, f
 
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
 // current and leave their state untouched, as the reference's device switch does.
+// The covered IDs outside the core set, in a switch of their own: the core networks' kernels
+// never see these cases in their register allocation.
+BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
+    switch (type) {
+        case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
+        case M_HCN2: return ch_hcn1<true>(m, s, dt);
+        case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
+        case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
+        case M_PMCA: return ch_pmca(m, s, dt);
+        case M_NCX: return ch_ncx(m, s, dt);
+        case M_KCC2: return ch_kcc2(m, s, dt);
+        case M_NKCC1: return ch_nkcc1(m, s, dt);
+        case M_CA_BUFFER: return ch_ca_buffer(m, s, dt);
+        case M_GLAST: return ch_transporter<0, 3, 0, 1, 2>(m, s, dt, 10.0f, 667.0f);
+        case M_GLYT1: return ch_transporter<7, 2, 1, 0, 1>(m, s, dt, 30.0f, 667.0f);
+        case M_GAT3: return ch_transporter<1, 2, 1, 0, 1>(m, s, dt, 0.5f, 667.0f);
+        case M_EAAT3: return ch_transporter<0, 3, 0, 1, 2>(m, s, dt, 20.0f, 667.0f);
+        case M_GLYT2: return ch_transporter<7, 3, 1, 0, 2>(m, s, dt, 55.7f, 667.0f);
+        case M_GAT1: return ch_transporter<1, 3, 1, 0, 2>(m, s, dt, 4.7f, 667.0f);
+        default: return 0.0;
+    }
+}
 // kNtMods = false: the caller has no modifier that reads transmitters (no ionotropic receptor
-// or transporter; see modifier_reads_nt), and those cases are left out.
-template <bool kNtMods = true>
+// or transporter; see modifier_reads_nt), and those cases are left out. kExt = false: only the
+// core channel set runs (see modifier_core), which keeps the kernel's register budget for the
+// common networks.
+template <bool kNtMods = true, bool kExt = true>
 BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
     if (!kNtMods && modifier_reads_nt(type)) return 0.0;
+    if (kExt && !modifier_core(type)) return dispatch_ext(type, m, s, dt);
     switch (type) {
         case M_PUMP: return ch_pump(m, s, dt);
         case M_AMPA: {
@@ -854,13 +1087,6 @@ BC_DEV double dispatch(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
         }
         case M_KV31: return ch_kv31(m, s, dt);
         case M_KV11: return ch_kv11(m, s, dt);
-        case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
-        case M_GLAST: return ch_transporter<0, 3, 0, 1, 2>(m, s, dt, 10.0f, 667.0f);
-        case M_GLYT1: return ch_transporter<7, 2, 1, 0, 1>(m, s, dt, 30.0f, 667.0f);
-        case M_GAT3: return ch_transporter<1, 2, 1, 0, 1>(m, s, dt, 0.5f, 667.0f);
-        case M_EAAT3: return ch_transporter<0, 3, 0, 1, 2>(m, s, dt, 20.0f, 667.0f);
-        case M_GLYT2: return ch_transporter<7, 3, 1, 0, 2>(m, s, dt, 55.7f, 667.0f);
-        case M_GAT1: return ch_transporter<1, 3, 1, 0, 2>(m, s, dt, 4.7f, 667.0f);
         default: return 0.0;
     }
 }

@@ -45,9 +45,10 @@ __device__ static inline uint64_t delay_ticks(float delay, float dt) {   // dela
 }
 
 // ------------------------------------------------------------------ step 1
-// kNt: some segment carries a modifier that reads or takes up transmitters (SF_NTREAD); the
-// variant without keeps those registers free.
-template <bool kNt>
+// kNt: some segment carries a modifier that reads or takes up transmitters (SF_NTREAD); kExt:
+// some segment carries a modifier outside the core set. The variants without keep those
+// registers free.
+template <bool kNt, bool kExt>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_modifiers(Net d, float dt) {
     exp_table_stage();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         m.dens = d.mdens[j * P + i];
         m.a7 = s.area;
         m.av = d.mav[j * P + i];
-        itot += dispatch<kNt>(t, m, s, dt);
+        itot += dispatch<kNt, kExt>(t, m, s, dt);
         if (d.cable) {   // concentrations held: every channel sees the stored ones
 #pragma unroll
             for (int k = 0; k < 7; k++) {

@@ -1,5 +1,5 @@
-// bc_libm.h — host/device double exp, log and pow that return the same bits as the
-// x86-64 glibc 2.35 routines the reference CPU tick calls (::exp, ::log, ::pow).
+// bc_libm.h — a host/device double exp that returns the same bits as the x86-64 glibc 2.35
+// routine the reference CPU tick calls (::exp), and correctly rounded division helpers.
 //
 // exp follows the published table-driven algorithm of glibc's exp (ARM optimized-routines
 // family, N = 128 table, degree-5 polynomial), with the FMA contractions of glibc's

@@ -50,7 +50,8 @@ enum : uint8_t {
     T_PP1 = 113, T_KVDR = 123, T_HCN1 = 129, T_NALCN = 133, T_CAMP_PKA = 135, T_PKC = 137,
     T_CAV21 = 140, T_NAV16 = 141, T_NAV12 = 142, T_GLAST = 147, T_KV31 = 150, T_KV11 = 156,
     T_CX36 = 114, T_CX43 = 198, T_GLYT1 = 148, T_GAT3 = 149, T_EAAT3 = 183, T_GLYT2 = 184,
-    T_GAT1 = 185,
+    T_GAT1 = 185, T_HCN2 = 130, T_PUMP_A3 = 197, T_PUMP_GLIA = 110, T_PMCA = 98, T_NCX = 99,
+    T_KCC2 = 100, T_NKCC1 = 122, T_CA_BUFFER = 102,
 };
 
 static uint64_t g_unsupported = 0;
@@ -207,8 +208,8 @@ static inline float bk_po(float V, float ca) {
     float num = L * o;
     return num / (num + c);
 }
-static inline float pump_vnorm(float V, float na, float ko, float atp_level) {
-    const float kmna = 10.0f, kmk = 1.5f;
+static inline float pump_vnorm(float V, float na, float ko, float atp_level, float kmna = 10.0f,
+                               float kmk = 1.5f) {
     float na3 = na * na * na, k2 = ko * ko;
     float kna3 = kmna * kmna * kmna, kk2 = kmk * kmk;
     float fna = na3 / (na3 + kna3), fk = k2 / (k2 + kk2);
@@ -229,9 +230,17 @@ static inline float pump_vnorm(float V, float na, float ko, float atp_level) {
 }
 
 // ---- channel currents (return pA, negative = outward) ----
-static double chan_pump(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:57-97
+// Na/K-ATPase (7), its alpha3 isoform (197: K_m Na 25 mM, ouabain block by state[5] nM, IC50
+// 20 nM) and the glial alpha2 pump (110: K_m K 1 mM) (hh_biological.cpp:57-141, 3414-3445).
+enum PumpKind { PUMP_A1, PUMP_A3, PUMP_GLIA };
+static double chan_pump(Modifier* m, Segment* s, float dt, PumpKind kind = PUMP_A1) {
     float area = eff_area(m, s);
-    float vn = pump_vnorm(s->V_m, s->intra_ions[0], s->extra_ions[1], s->ATP_level);
+    float vn = pump_vnorm(s->V_m, s->intra_ions[0], s->extra_ions[1], s->ATP_level,
+                          kind == PUMP_A3 ? 25.0f : 10.0f, kind == PUMP_GLIA ? 1.0f : 1.5f);
+    if (kind == PUMP_A3) {
+        double oua = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        vn /= (1.0f + oua / 20.0f);
+    }
     double N = m->density * area;
     float ip = N * vn * 75.0f * 1.602176634e-19f * kAtoPa;
     float vol = volume(s);
@@ -484,13 +493,16 @@ static double chan_kv7(Modifier* m, Segment* s, float dt) {   // hh_biological.c
     return k_out(s, ia * kAtoPa, dt);
 }
 
-static double chan_hcn1(Modifier* m, Segment* s, float dt) {   // modifier_helpers.h:1442-1484
+// HCN1 (129: V1/2 -69 mV, tau 84 ms, 0.84 pS) and HCN2 (130: -95 mV, 557 ms, 1.54 pS)
+// (modifier_helpers.h:1442-1484, hh_biological.cpp:3598-3599)
+static double chan_hcn1(Modifier* m, Segment* s, float dt, float vh0 = -69.0f, float topen0 = 84.0f,
+                        float gamma0 = 0.84f) {
     double V = s->V_m;
     float area = eff_area(m, s);
-    float vh = (m->state[1] != 0) ? m->state[1] : -69.0f;
+    float vh = (m->state[1] != 0) ? m->state[1] : vh0;
     float k = (m->state[2] > 0) ? m->state[2] : 6.0f;
-    float gamma = (m->state[3] > 0) ? m->state[3] : 0.84f;
-    float t_open = (m->state[4] > 0) ? m->state[4] : 84.0f;
+    float gamma = (m->state[3] > 0) ? m->state[3] : gamma0;
+    float t_open = (m->state[4] > 0) ? m->state[4] : topen0;
     float t_close = (m->state[5] > 0) ? m->state[5] : 300.0f;
     float camp = first_state_of(s, T_CAMP_PKA, 0, 0.1f);
     float ve = vh + 15.0f * camp / (camp + 0.5f);
@@ -643,6 +655,183 @@ static double chan_connexin(Modifier* m, Segment* s, float dt) {
     return 0.0f;
 }
 
+// PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
+// are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
+// default K_m squared, fit a double), which is how they are computed here.
+static double chan_pmca(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    double km = (m->state[1] > 0.0f) ? m->state[1] : 0.0003f;
+    double hill = (m->state[2] > 0.0f) ? m->state[2] : 2.0f;
+    if (hill != 2.0 || (m->state[1] > 0.0f && (double)(float)m->state[1] != m->state[1])) g_unsupported++;
+    float atp = s->ATP_level;
+    if (atp < 0.0f) atp = 0.0f;
+    float f_atp = atp / (atp + 0.2f);
+    float ca = s->intra_ions[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float ca_pow = (double)ca * (double)ca;
+    float f_ca = ca_pow / (ca_pow + km * km);
+    double N = m->density * area;
+    float ip = N * 100.0f * f_ca * f_atp * 1.602176634e-19f * 2.0f * kAtoPa;
+    float vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0;
+    float tr = ip * dt / (2.0f * cpv);
+    float ca_av = s->intra_ions[2] - 1e-9f;
+    if (ca_av < 0.0f) ca_av = 0.0f;
+    if (tr > ca_av) tr = ca_av;
+    s->intra_ions[2] -= tr;
+    s->extra_ions[2] += tr / kEvr;
+    floor_ions(s->intra_ions);
+    s->ATP_level -= ip * dt * 1e-8f;
+    if (s->ATP_level < 0.0f) s->ATP_level = 0.0f;
+    return -ip;
+}
+
+// NCX (99, hh_biological.cpp:2616-2646, modifier_helpers.h:1181-1201): 3 Na+ in / 1 Ca2+ out,
+// voltage-dependent, allosteric Ca activation, either direction.
+static double chan_ncx(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    float na_i = s->intra_ions[0]; if (na_i < 1e-9f) na_i = 1e-9f;
+    float na_o = s->extra_ions[0]; if (na_o < 1e-9f) na_o = 1e-9f;
+    float ca_i = s->intra_ions[2]; if (ca_i < 1e-9f) ca_i = 1e-9f;
+    float ca_o = s->extra_ions[2]; if (ca_o < 1e-9f) ca_o = 1e-9f;
+    float vv = (float)s->V_m * 1e-3f;
+    float frt = kFaraday / (kGas * kTemp);
+    float e_eta = exp(0.35f * frt * vv);
+    float e_eta1 = exp((0.35f - 1.0f) * frt * vv);
+    float nai3 = na_i * na_i * na_i;
+    float nao3 = na_o * na_o * na_o;
+    float fwd = e_eta1 * nao3 * ca_i;
+    float rev = e_eta * nai3 * ca_o;
+    float den = 1.0f + 0.27f * e_eta1;
+    float cr = 0.000125f / ca_i;
+    float allo = 1.0f / (1.0f + cr * cr);
+    float fn = allo * (fwd - rev) / (den * 530.2f);
+    double N = m->density * area;
+    float I = N * 500.0f * fn * 1.602176634e-19f * kAtoPa;
+    float vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0;
+    float tr = I * dt / cpv;
+    if (tr > ca_i) tr = ca_i;
+    if (tr < -na_i / 3.0f) tr = -na_i / 3.0f;
+    s->intra_ions[2] -= tr;
+    s->extra_ions[2] += tr / kEvr;
+    s->intra_ions[0] += 3.0f * tr;
+    s->extra_ions[0] -= 3.0f * tr / kEvr;
+    floor_ions(s->intra_ions);
+    return I;
+}
+
+// KCC2 (100, hh_biological.cpp:2648-2703): electroneutral K+-Cl- cotransport down the product
+// gradient, limited by the ions available on the losing side.
+static double chan_kcc2(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    double km_k = (m->state[1] > 0.0f) ? m->state[1] : 6.5f;
+    double km_cl = (m->state[2] > 0.0f) ? m->state[2] : 6.5f;
+    float k_i = s->intra_ions[1]; if (k_i < 1e-9f) k_i = 1e-9f;
+    float cl_i = s->intra_ions[3]; if (cl_i < 1e-9f) cl_i = 1e-9f;
+    float k_o = s->extra_ions[1]; if (k_o < 1e-9f) k_o = 1e-9f;
+    float cl_o = s->extra_ions[3]; if (cl_o < 1e-9f) cl_o = 1e-9f;
+    float f_k = k_i / (k_i + km_k);
+    float f_cl = cl_i / (cl_i + km_cl);
+    float drv = 1.0f - (k_o * cl_o) / (k_i * cl_i);
+    if (drv > 1.0f) drv = 1.0f;
+    if (drv < -1.0f) drv = -1.0f;
+    float flux = 5.0f * 1e-20f * m->density * area * f_k * f_cl * drv * dt;
+    float dc = flux / (volume(s) * 1e-18f);
+    if (dc > 0.0f) {
+        float ka = k_i - 1e-9f, cla = cl_i - 1e-9f;
+        float mn = (ka < cla) ? ka : cla;
+        if (mn < 0.0f) mn = 0.0f;
+        if (dc > mn) dc = mn;
+    } else {
+        float ka = (k_o - 1e-9f) * kEvr, cla = (cl_o - 1e-9f) * kEvr;
+        float mn = (ka < cla) ? ka : cla;
+        if (mn < 0.0f) mn = 0.0f;
+        float mag = -dc;
+        if (mag > mn) mag = mn;
+        dc = -mag;
+    }
+    s->intra_ions[1] -= dc;
+    s->intra_ions[3] -= dc;
+    s->extra_ions[1] += dc / kEvr;
+    s->extra_ions[3] += dc / kEvr;
+    floor_ions(s->intra_ions);
+    return 0.0f;
+}
+
+// NKCC1 (122, hh_biological.cpp:2705-2760): Na+-K+-2Cl- uptake with trans-stimulation by
+// external Na+ and trans-inhibition by external K+.
+static double chan_nkcc1(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    double km_nai = (m->state[1] > 0.0f) ? m->state[1] : 10.0f;
+    double km_ki = (m->state[2] > 0.0f) ? m->state[2] : 15.0f;
+    double km_cli = (m->state[3] > 0.0f) ? m->state[3] : 50.0f;
+    double km_nao = (m->state[4] > 0.0f) ? m->state[4] : 0.5f;
+    double km_ko = (m->state[5] > 0.0f) ? m->state[5] : 0.2f;
+    float na_i = s->intra_ions[0], k_i = s->intra_ions[1], cl_i = s->intra_ions[3];
+    float na_o = s->extra_ions[0], k_o = s->extra_ions[1];
+    if (na_i < 1e-9f) na_i = 1e-9f;
+    if (k_i < 1e-9f) k_i = 1e-9f;
+    if (cl_i < 1e-9f) cl_i = 1e-9f;
+    if (na_o < 1e-9f) na_o = 1e-9f;
+    if (k_o < 1e-9f) k_o = 1e-9f;
+    float fna = na_i / (na_i + km_nai);
+    float fk = k_i / (k_i + km_ki);
+    float fcl = cl_i / (cl_i + km_cli);
+    float stim = na_o / (na_o + km_nao);
+    float inhib = 1.0f / (1.0f + k_o / km_ko);
+    float flux = 7.0f * 1e-20f * m->density * area * fna * fk * fcl * stim * inhib * dt;
+    float dc = flux / (volume(s) * 1e-18f);
+    float naa = (s->extra_ions[0] - 1e-9f) * kEvr;
+    float ka = (s->extra_ions[1] - 1e-9f) * kEvr;
+    float cla = (s->extra_ions[3] - 1e-9f) * kEvr / 2.0f;
+    float mn = (naa < ka) ? naa : ka;
+    if (cla < mn) mn = cla;
+    if (mn < 0.0f) mn = 0.0f;
+    if (dc > mn) dc = mn;
+    s->intra_ions[0] += dc;
+    s->intra_ions[1] += dc;
+    s->intra_ions[3] += 2.0f * dc;
+    s->extra_ions[0] -= dc / kEvr;
+    s->extra_ions[1] -= dc / kEvr;
+    s->extra_ions[3] -= 2.0f * dc / kEvr;
+    floor_ions(s->intra_ions);
+    return 0.0f;
+}
+
+// Ca2+ buffer (102, hh_biological.cpp:2825-2865): first-order binding to a pool of capacity
+// state[0] (0.15 mM) x density; state[1] holds the bound Ca (initialised at equilibrium).
+static double chan_ca_buffer(Modifier* m, Segment* s, float dt) {
+    double bref = (m->state[0] > 0.0f) ? m->state[0] : 0.15f;
+    double btot = bref * m->density;
+    double kd = (m->state[2] > 0.0f) ? m->state[2] : 0.0005f;
+    double kon = (m->state[3] > 0.0f) ? m->state[3] : 0.1f;
+    double koff = (m->state[4] > 0.0f) ? m->state[4] : (kd * kon);
+    double bound = m->state[1];
+    if (bound <= 0.0f && btot > 0.0f) {
+        float c0 = s->intra_ions[2];
+        bound = btot * c0 / (c0 + kd);
+        m->state[1] = bound;
+    }
+    float cf = s->intra_ions[2];
+    if (cf < 1e-9f) cf = 1e-9f;
+    float bf = btot - bound;
+    if (bf < 0.0f) bf = 0.0f;
+    float db = (kon * cf * bf - koff * bound) * dt;
+    bound += db;
+    if (bound < 0.0f) bound = 0.0f;
+    if (bound > btot) bound = btot;
+    s->intra_ions[2] -= db;
+    if (s->intra_ions[2] < 1e-9f) {
+        float def = 1e-9f - s->intra_ions[2];
+        s->intra_ions[2] = 1e-9f;
+        bound -= def;
+        if (bound < 0.0f) bound = 0.0f;
+    }
+    m->state[1] = bound;
+    return 0.0f;
+}
+
 // Na+-coupled transmitter transporters (modifier_helpers.h:418-455, hh_biological.cpp:725-780):
 // Michaelis-Menten uptake of one transmitter from the segment's extracellular pool, limited by
 // the transmitter, extracellular Na+ and (K+-countertransport) intracellular K+ available, then
@@ -705,6 +894,14 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_NAV12: return chan_nav(m, s, dt, kNav12, 20.8f, 10.0f, 0.7f);
         case T_KV31: return chan_kv31(m, s, dt);
         case T_KV11: return chan_kv11(m, s, dt);
+        case T_HCN2: return chan_hcn1(m, s, dt, -95.0f, 557.0f, 1.54f);
+        case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
+        case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
+        case T_PMCA: return chan_pmca(m, s, dt);
+        case T_NCX: return chan_ncx(m, s, dt);
+        case T_KCC2: return chan_kcc2(m, s, dt);
+        case T_NKCC1: return chan_nkcc1(m, s, dt);
+        case T_CA_BUFFER: return chan_ca_buffer(m, s, dt);
         case T_GLAST: return chan_transporter(m, s, dt, kGlast);
         case T_GLYT1: return chan_transporter(m, s, dt, kGlyt1);
         case T_GAT3: return chan_transporter(m, s, dt, kGat3);

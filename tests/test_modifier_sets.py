@@ -1,0 +1,55 @@
+"""Extended modifier sets on the device against the oracle, bit for bit (the oracle equals the
+compiled reference on the same networks, tests/test_oracle_vs_reference.py):
+
+- ion homeostasis: the alpha3 and glial Na/K pumps, PMCA, NCX, KCC2, NKCC1, the Ca2+ buffer and
+  HCN2 (src/hh_biological.cpp:99-141, 2574-2865, 3414-3445, 3599);
+- a network that mixes them with the core channel set runs the extended step-1 kernel, and the
+  core-only networks keep the lean one (results identical either way).
+"""
+import numpy as np
+import pytest
+
+from bioneuroncore_amd import backend, bion
+import harness as H
+
+pytestmark = pytest.mark.gpu
+DT = 0.025
+
+
+def _compare(net, ticks, cur):
+    o = H.OracleSim(net.copy())
+    for _ in range(ticks):
+        o.step(DT, H.Stimuli(current=cur))
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net.copy()) == 0
+    sim.upload_stimuli(backend.make_stimuli(current=cur))
+    sim.run_ticks(DT, ticks)
+    out = sim.bion_data()
+    seg, syn, spk, stats = out.segments.copy(), out.synapses.copy(), sim.spikes(len(net.segments)), sim.stats()
+    sim.close()
+    print(f"unsupported={stats.unsupported_modifiers} spikes={int(spk.sum())}", flush=True)
+    assert stats.unsupported_modifiers == 0
+    assert H.first_mismatch(o.net.segments, seg) is None, H.first_mismatch(o.net.segments, seg)
+    assert H.first_mismatch(o.net.synapses, syn) is None, H.first_mismatch(o.net.synapses, syn)
+    np.testing.assert_array_equal(o.spikes(), spk)
+    return seg, spk
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_ion_homeostasis_modifiers(seed):
+    net = bion.homeostasis_network(seed)
+    cur = H.soma_clamp(net, 170.0)
+    cur = cur[cur["segment_id"] < 12 * 6]
+    _, spk = _compare(net, 300, cur)
+    assert spk.sum() > 0
+
+
+def test_pmca_hill_other_than_two_is_counted():
+    net = bion.homeostasis_network(33)
+    mt = net.segments["modifiers"]["type"]
+    i, j = np.nonzero(mt == bion.PMCA)
+    net.segments["modifiers"]["state"][i[0], j[0], 2] = 3.0
+    sim = backend.Simulation()
+    assert sim.load_network(net) == 0
+    assert sim.stats().unsupported_modifiers == 1
+    sim.close()

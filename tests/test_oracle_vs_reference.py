@@ -112,3 +112,13 @@ def test_ionotropic_receptors_as_segment_modifiers():
     cur = H.soma_clamp(net, 160.0)
     rs, _, sp = _both(net, 300, lambda t: H.Stimuli(current=cur))
     assert sp.sum() > 0
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_ion_homeostasis_modifiers(seed):
+    """The alpha3 and glial Na/K pumps, PMCA, NCX, KCC2, NKCC1, the Ca2+ buffer and HCN2."""
+    net = bion.homeostasis_network(seed)
+    cur = H.soma_clamp(net, 170.0)
+    cur = cur[cur["segment_id"] < 12 * 6]
+    rs, _, sp = _both(net, 300, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
