@@ -78,8 +78,11 @@ TRANSPORTERS = (GLAST, GLYT1, GAT3, EAAT3, GLYT2, GAT1)
 # Ion homeostasis: pump isoforms, Ca2+ extrusion and buffering, cation-chloride cotransport.
 HCN2, PUMP_A3, PUMP_GLIA, PMCA, NCX, KCC2, NKCC1, CA_BUFFER = 130, 197, 110, 98, 99, 100, 122, 102
 HOMEOSTASIS = (HCN2, PUMP_A3, PUMP_GLIA, PMCA, NCX, KCC2, NKCC1, CA_BUFFER)
+# Further voltage-gated and leak channels of the tadpole spinal network's cell types.
+CAV13, CLC2, KV42, KV21, NAP, NAP_TAD, KIR, NAV12DEV = 1, 2, 85, 196, 105, 195, 106, 193
+CHANNELS2 = (CAV13, CLC2, KV42, KV21, NAP, NAP_TAD, KIR, NAV12DEV)
 SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -90,7 +93,8 @@ CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, P
                  CAV21: 100.0, HCN1: 120.0, SK: 6.0, BK: 0.15, KV11: 0.346, KV31: 2.0, KV7: 2.0,
                  CX36: 400.0, CX43: 60.0, GLAST: 1.0, GLYT1: 1.0, GAT3: 1.0, EAAT3: 1.0,
                  GLYT2: 1.0, GAT1: 1.0, HCN2: 120.0, PUMP_A3: 1500.0, PUMP_GLIA: 1500.0, PMCA: 300.0,
-                 NCX: 30.0, KCC2: 0.093, NKCC1: 0.1, CA_BUFFER: 0.5}
+                 NCX: 30.0, KCC2: 0.093, NKCC1: 0.1, CA_BUFFER: 0.5, CAV13: 2.0, CLC2: 0.8,
+                 KV42: 41.51, KV21: 2.595, NAP: 3.0, NAP_TAD: 10.6, KIR: 0.35, NAV12DEV: 60.0}
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -415,6 +419,21 @@ def homeostasis_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6,
     net.segments = np.concatenate([segs, g])
     net.segments["id"] = np.arange(len(net.segments), dtype=np.uint32)
     net.neurons = np.concatenate([net.neurons, gneu])
+    return net
+
+
+def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5) -> Network:
+    """Chain neurons built on the developmental Nav1.2 with Kv2.1, Kv4.2, Kv1.1, K2P, NALCN and
+    the pump on every segment; somas add Cav1.3, Cav2.1, NaP (Nav1.6 and tadpole), Kir2.1, the
+    ClC-2 leak and HCN2; AMPA+NMDA / GABA-A synapses."""
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12DEV, KV21, KV42, KV11, K2P, NALCN, PUMP],
+                        [CAV13, CAV21, NAP, NAP_TAD, KIR, CLC2, HCN2], 3.0, "ampa_nmda_gaba", 4, seed,
+                        lognormal_sigma=0.2)
+    mods = net.segments["modifiers"]
+    for t, init in ((NAV12DEV, {0: 0.02, 1: 0.7, 3: 0.7}), (KV21, {0: 0.05, 1: 0.9}), (KV42, {0: 0.1, 1: 0.5})):
+        i, j = np.nonzero(mods["type"] == t)
+        for k, v in init.items():
+            mods["state"][i, j, k] = v
     return net
 
 

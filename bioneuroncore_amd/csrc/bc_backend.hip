@@ -339,7 +339,8 @@ static int build_device(Sim& S) {
     d.nt_ex_live = ns == 0;
     d.nt_in_live = 1;   // until the first upload of the state decides
     std::vector<float> radius(npad, 0), length(npad, 0), area(npad, 0), cap(npad, 1), pkc(npad, 0), pka(npad, 0),
-        camp(npad, 0.1f), pp1(npad, 0.5f), camkii(npad, 0), calcn(npad, 0);
+        camp(npad, 0.1f), pp1(npad, 0.5f), camkii(npad, 0), calcn(npad, 0),
+        camkii01(npad, 0);
     std::vector<double> vol(npad, 0);
     std::vector<uint8_t> stype(npad, 0), sflags(npad, 0), mcount(npad, 0);
     std::vector<int8_t> cav21(npad, -1);
@@ -366,7 +367,10 @@ static int build_device(Sim& S) {
             if (m.type == 137 && !seen_pkc) { pkc[di] = m.state[0]; seen_pkc = true; }
             if (m.type == 135 && !seen_pka) { pka[di] = m.state[1]; camp[di] = m.state[0]; seen_pka = true; }
             if (m.type == 113) pp1[di] = m.state[3];
-            if (m.type == 82) camkii[di] = m.state[0];
+            if (m.type == 82) {
+                camkii[di] = m.state[0];
+                camkii01[di] = m.state[0] + m.state[1];
+            }
             if (m.type == 83) calcn[di] = m.state[0];
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
@@ -589,6 +593,7 @@ static int build_device(Sim& S) {
     d.camp = S.dupload(camp);
     d.pp1 = S.dupload(pp1);
     d.camkii = S.dupload(camkii);
+    d.camkii01 = S.dupload(camkii01);
     d.sflags = S.dupload(sflags);
     d.cav21_slot = S.dupload(cav21);
     d.mcount = S.dupload(mcount);

@@ -35,7 +35,8 @@ enum : uint8_t {
     M_NAV12 = 142, M_KV31 = 150, M_KV11 = 156, M_CX36 = 114, M_CX43 = 198,
     M_GLAST = 147, M_GLYT1 = 148, M_GAT3 = 149, M_EAAT3 = 183, M_GLYT2 = 184, M_GAT1 = 185,
     M_HCN2 = 130, M_PUMP_A3 = 197, M_PUMP_GLIA = 110, M_PMCA = 98, M_NCX = 99, M_KCC2 = 100,
-    M_NKCC1 = 122, M_CA_BUFFER = 102,
+    M_NKCC1 = 122, M_CA_BUFFER = 102, M_CAV13 = 1, M_CLC2 = 2, M_KV42 = 85, M_KV21 = 196,
+    M_NAP = 105, M_NAP_TAD = 195, M_KIR = 106, M_NAV12DEV = 193,
 };
 
 // The core channel set: pump, ionotropic receptors, the K+/Na+/Ca2+/HCN/NALCN channels of the
@@ -65,7 +66,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case M_NAV12: case M_KV31: case M_KV11: case M_CX36: case M_CX43: case M_GLAST:
         case M_GLYT1: case M_GAT3: case M_EAAT3: case M_GLYT2: case M_GAT1: case M_HCN2:
         case M_PUMP_A3: case M_PUMP_GLIA: case M_PMCA: case M_NCX: case M_KCC2: case M_NKCC1:
-        case M_CA_BUFFER:
+        case M_CA_BUFFER: case M_CAV13: case M_CLC2: case M_KV42: case M_KV21: case M_NAP:
+        case M_NAP_TAD: case M_KIR: case M_NAV12DEV:
             return true;
         default:
             return false;
@@ -84,6 +86,7 @@ struct SegRegs {
     double cpm;    // F * vol / 1000, the charge-per-mole factor every current divides by
     float pkc, pka, camp;   // cascade activities read by Nav/Kv3.1/Kv7/HCN (constant on device)
     float camkii;           // state[0] of the last CaMKII modifier, read by the connexins
+    float camkii01;         // its state[0] + state[1], read by Kv4.2
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -808,6 +811,242 @@ BC_DEV double ch_connexin(const ModSlot& m, SegRegs& s, float dt) {
     return 0.0;
 }
 
+// Cav1.3 (1, hh_biological.cpp:142-185): L-type m^2 h, fixed time constants, Ca-dependent h;
+// the extracellular Ca falls with the intracellular, as the reference writes it.
+BC_DEV double ch_cav13(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4);
+    double vh = (s3 != 0.0f) ? -fabs(s3) : -35.0f;
+    double k = (s4 > 0.0f) ? s4 : 7.0f;
+    double gamma = (s2 > 0.0f) ? s2 : 25.0f;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (k + 1e-6f)));
+    double g0 = m.get(0), g1 = m.get(1);
+    g0 += (mi - g0) * dt / 2.0f;
+    float ca = s.in[2] * kMmToUm;
+    float hi = 1.0f / (1.0f + bc::exp((V + 52.0f) / 8.0f));
+    hi *= 1.0f / (1.0f + ca / 3.0f);
+    g1 += (hi - g1) * dt / 100.0f;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    float ci = s.in[2], co = s.ex[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double op = g0 * g0 * g1;
+    double flux = ghk_di(s, V, ci, co);
+    double ica = P * flux * (m.dens * area) * op * kAtoPa;
+    double cpm = s.cpm;
+    s.in[2] -= (ica * dt) / (2.0f * cpm);
+    s.ex[2] -= (ica * dt) / (2.0f * cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ica;
+}
+
+// ClC-2 leak (2, hh_biological.cpp:390-414): state[0] forced to 1 below 0.5, GHK anion current;
+// the charge-per-mole factor is a float in this one.
+BC_DEV double ch_cl(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double g0 = m.get(0);
+    if (g0 < 0.5f) { g0 = 1.0f; m.set(0, g0); }
+    double s2 = m.get(2);
+    double gamma = (s2 > 0.0f) ? s2 : 2.5f;
+    float ci = s.in[3], co = s.ex[3];
+    double P = p_single(gamma, 1.0f, kCbarCl);
+    double flux = ghk_an(s, V, ci, co);
+    double icl = P * flux * (m.dens * area) * g0 * kAtoPa;
+    float vol = s.vol;
+    float cpm = kFaraday * vol / 1000.0;
+    s.in[3] += (icl * dt) / cpm;
+    s.ex[3] -= (icl * dt) / (cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -icl;
+}
+
+// Kv4.2 A-type (85, hh_biological.cpp:789-836, modifier_helpers.h:463-472): m^3 h with V-dependent
+// recovery; CaMKII (state[0] + state[1] of the last CaMKII modifier) phosphorylates state[3].
+BC_DEV double ch_kv42(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s4 = m.get(4), s5 = m.get(5);
+    double vh = (s4 != 0.0f) ? -fabs(s4) : -10.0f;
+    double k = (s5 > 0.0f) ? s5 : 13.5f;
+    double gamma = (s2 > 0.0f) ? s2 : 5.0f;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (k + 1e-6f)));
+    double g0 = m.get(0), g1 = m.get(1), g3 = m.get(3);
+    g0 += (mi - g0) * dt / 1.5f;
+    float hi = 1.0f / (1.0f + bc::exp((V + 65.0f) / 8.0f));
+    float th = 15.0f + (170.0f - 15.0f) / (1.0f + bc::exp((V - -60.0f) / 8.0f));
+    g1 += (hi - g1) * dt / th;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    g3 += (s.camkii01 * 0.5f - g3) * dt / 5000.0f;
+    if (g3 < 0.0f) g3 = 0.0f;
+    if (g3 > 1.0f) g3 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    m.set(3, g3);
+    double geff = gamma * (1.0f - g3 * 0.7f);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(geff, 1.0f, kCbarK);
+    double op = g0 * g0 * g0 * g1;
+    double flux = ghk_mono(s, V, ki, ko);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+
+// Kv2.1 delayed rectifier (196, hh_biological.cpp:1392-1428, modifier_helpers.h:584-620): m^4 h
+// with a U-shaped inactivation, TEA block by state[5].
+BC_DEV double ch_kv21(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s5 = m.get(5);
+    double gamma = (s2 > 0.0f) ? s2 : 8.0f;
+    double g0 = m.get(0), g1 = m.get(1);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - 0.0f) / 9.4f));
+    float tm = 15.0f + (73.0f - 15.0f) / (1.0f + bc::exp((V - 0.0f) / 10.0f));
+    g0 += (mi - g0) * dt / tm;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    float left = 1.0f / (1.0f + bc::exp(-(V - -26.5f) / 5.2f));
+    float right = 1.0f / (1.0f + bc::exp((V - 15.0f) / 10.0f));
+    float hi = 1.0f - (1.0f - 0.2f) * left * right;
+    float th = 4000.0f + (300.0f - 4000.0f) / (1.0f + bc::exp((V - -80.0f) / 15.0f));
+    g1 += (hi - g1) * dt / th;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    double op = g0 * g0 * g0 * g0 * g1;
+    double tea = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + tea / 10.0f);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+
+// Persistent Na+: Nav1.6 NaP (105, hh_biological.cpp:2951-2978) and the tadpole NaP (kTadpole,
+// 195: Nav1.2 gamma, V1/2 -40 mV, slow inactivation in state[6], Na+ drawn from the extracellular
+// pool, hh_biological.cpp:2980-3016).
+template <bool kTadpole>
+BC_DEV double ch_nap(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double mv = m.get(0);
+    double s1 = m.get(1), s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
+    double vh = (s1 != 0.0f) ? s1 : (kTadpole ? -40.0f : -58.7f);
+    double km = (s2 > 0.0f) ? s2 : 6.0f;
+    double gamma = (s3 > 0.0f) ? s3 : (kTadpole ? 20.8f : 20.7f);
+    double tau = (s4 > 0.0f) ? s4 : 5.0f;
+    double po = (s5 > 0.0f) ? s5 : 0.016f;
+    float vhf = vh, kmf = km;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vhf) / kmf));
+    mv += (mi - mv) * dt / tau;
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m.set(0, mv);
+    float nai = s.in[0], nao = s.ex[0];
+    double P = p_single(gamma, 1.0f, kCbarNa);
+    double flux = ghk_mono(s, V, nai, nao);
+    if (!kTadpole) {
+        float ina = (float)(P * flux * (m.dens * area) * (mv * po) * kAtoPa);
+        s.in[0] -= (ina * dt) / s.cpm;
+        floor7(s.in);
+        return -ina;
+    }
+    float si = 1.0f / (1.0f + bc::exp((V - -50.0f) / 6.0f));
+    double sv = m.get(6);
+    sv += (si - sv) * dt / 500.0f;
+    if (sv < 0.0f) sv = 0.0f;
+    if (sv > 1.0f) sv = 1.0f;
+    m.set(6, sv);
+    double ina = P * flux * (m.dens * area) * (mv * po * sv) * kAtoPa;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ina;
+}
+
+// Kir2.1 (106, hh_biological.cpp:3018-3037, modifier_helpers.h:870-891): Woodhull Mg2+ block and
+// a three-site spermine block.
+BC_DEV float kir_f_open(float V) {
+    float vn = kFaraday * V * 0.001f / (kGas * kTemp);
+    float kd_mg = 0.3f * bc::exp(-0.6f * 2.0f * vn);
+    float mg = 0.5f / kd_mg;
+    float kd_spm = 0.0775f * bc::exp(-0.5f * 4.0f * vn);
+    float r = 0.001f / kd_spm;
+    float spm = 0.0f, rk = r;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { spm += rk; rk *= r; }
+    return 1.0f / (1.0f + mg + spm);
+}
+BC_DEV double ch_kir(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s3 = m.get(3);
+    double gamma = (s3 > 0.0f) ? s3 : 30.0f;
+    float ki = s.in[1], ko = s.ex[1];
+    float fo = kir_f_open(V);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
+    return k_out(s, P * flux * (m.dens * area) * fo * kAtoPa, dt);
+}
+
+// Developmental Nav1.2 (193, hh_biological.cpp:314-357, modifier_helpers.h:221-241): m^3 with
+// fast and slow inactivation (slow fraction state[4], default 0.7; slow recovery 500 ms), the
+// inactivation V1/2 from state[2] (default 45: -45 mV), TTX and PKA/PKC block.
+BC_DEV double ch_nav12dev(const ModSlot& m, SegRegs& s, float dt) {
+    const NavK g = {16.0f, 6.0f, 41.5f, 66.5f, 45.0f};
+    float area = eff_area(m, s);
+    if (s.V < -200.0f) s.V = -200.0f;
+    if (s.V > 200.0f) s.V = 200.0f;
+    float V = s.V;
+    float am = nav_am(g, V), bm = nav_bm(g, V);
+    float minf = am / (am + bm);
+    float mtau = 1.0f / (am + bm);
+    double mv = minf + (m.get(0) - minf) * bc::exp(-dt / mtau);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
+    double vhalf = (s2 > 0.0f) ? s2 : 45.0f;
+    float vhf = vhalf;
+    float hi = 1.0f / (1.0f + bc::exp((V + vhf) / 6.2f));
+    float htau = 1.0f / (nav_ah(g, V) + nav_bh(g, V));
+    double hf = hi + (m.get(1) - hi) * bc::exp(-dt / htau);
+    double ts = (hi < s3) ? htau : 500.0f;
+    double hs = hi + (s3 - hi) * bc::exp(-dt / ts);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hf < 0.0f) hf = 0.0f;
+    if (hf > 1.0f) hf = 1.0f;
+    if (hs < 0.0f) hs = 0.0f;
+    if (hs > 1.0f) hs = 1.0f;
+    m.set(0, mv);
+    m.set(1, hf);
+    m.set(3, hs);
+    double sf = (s4 > 0.0f) ? s4 : 0.7f;
+    float he = (1.0f - sf) * hf + sf * hs;
+    float op = mv * mv * mv * he;
+    double ttx = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + ttx / 10.0f);
+    op *= (1.0f - 0.7f * (s.pkc + s.pka) * 0.5f);
+    if (op < 0.0f) op = 0.0f;
+    double P = p_single(20.8f, 1.0f, kCbarNa);
+    double flux = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double ina = P * flux * (m.dens * area) * op * kAtoPa;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ina;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -1034,6 +1273,14 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
     switch (type) {
         case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
         case M_HCN2: return ch_hcn1<true>(m, s, dt);
+        case M_CAV13: return ch_cav13(m, s, dt);
+        case M_CLC2: return ch_cl(m, s, dt);
+        case M_KV42: return ch_kv42(m, s, dt);
+        case M_KV21: return ch_kv21(m, s, dt);
+        case M_NAP: return ch_nap<false>(m, s, dt);
+        case M_NAP_TAD: return ch_nap<true>(m, s, dt);
+        case M_KIR: return ch_kir(m, s, dt);
+        case M_NAV12DEV: return ch_nav12dev(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

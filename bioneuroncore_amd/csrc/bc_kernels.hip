@@ -69,6 +69,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     s.pka = d.pka[i];
     s.camp = d.camp[i];
     s.camkii = d.camkii[i];
+    s.camkii01 = d.camkii01[i];
     seg_consts_init(s);
     ghk_cache_init(s);
     const uint8_t sfl = d.sflags[i];
@@ -1052,6 +1053,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
             s.pka = d.pka[i];
             s.camp = d.camp[i];
             s.camkii = d.camkii[i];
+            s.camkii01 = d.camkii01[i];
         }
         seg_consts_init(s);
         ghk_cache_init(s);

@@ -134,6 +134,7 @@ struct Net {
     const float* camp;
     const float* pp1;
     const float* camkii;        // state[0] of the last CaMKII modifier (0 if none)
+    const float* camkii01;      // its state[0] + state[1] (0 if none)
     const uint8_t* sflags;
     const int8_t* cav21_slot;   // last Cav2.1 modifier slot, -1 if none
 

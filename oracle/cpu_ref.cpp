@@ -51,7 +51,8 @@ enum : uint8_t {
     T_CAV21 = 140, T_NAV16 = 141, T_NAV12 = 142, T_GLAST = 147, T_KV31 = 150, T_KV11 = 156,
     T_CX36 = 114, T_CX43 = 198, T_GLYT1 = 148, T_GAT3 = 149, T_EAAT3 = 183, T_GLYT2 = 184,
     T_GAT1 = 185, T_HCN2 = 130, T_PUMP_A3 = 197, T_PUMP_GLIA = 110, T_PMCA = 98, T_NCX = 99,
-    T_KCC2 = 100, T_NKCC1 = 122, T_CA_BUFFER = 102,
+    T_KCC2 = 100, T_NKCC1 = 122, T_CA_BUFFER = 102, T_CAV13 = 1, T_CL = 2, T_KV42 = 85,
+    T_KV21 = 196, T_NAP = 105, T_NAP_TAD = 195, T_KIR = 106, T_NAV12DEV = 193,
 };
 
 static uint64_t g_unsupported = 0;
@@ -655,6 +656,250 @@ static double chan_connexin(Modifier* m, Segment* s, float dt) {
     return 0.0f;
 }
 
+// Cav1.3 (1, hh_biological.cpp:142-185): L-type m^2 h, fixed time constants, Ca-dependent h.
+// Extracellular Ca decreases with the intracellular change, as the reference writes it.
+static double chan_cav13(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double vh = (m->state[3] != 0.0f) ? -fabs(m->state[3]) : -35.0f;
+    double k = (m->state[4] > 0.0f) ? m->state[4] : 7.0f;
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 25.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (k + 1e-6f)));
+    m->state[0] += (mi - m->state[0]) * dt / 2.0f;
+    float ca = s->intra_ions[2] * kMmToUm;
+    float hi = 1.0f / (1.0f + exp((V + 52.0f) / 8.0f));
+    hi *= 1.0f / (1.0f + ca / 3.0f);
+    m->state[1] += (hi - m->state[1]) * dt / 100.0f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    float ci = s->intra_ions[2], co = s->extra_ions[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double op = m->state[0] * m->state[0] * m->state[1];
+    double flux = ghk_di(V, ci, co);
+    double ia = P * flux * (m->density * area) * op;
+    double ica = ia * kAtoPa;
+    double cpm = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->extra_ions[2] -= (ica * dt) / (2.0f * cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ica;
+}
+
+// ClC-2 leak (2, hh_biological.cpp:390-414): open state[0] forced to 1 when below 0.5, GHK
+// anion current; the charge-per-mole factor is a float here.
+static double chan_cl(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    if (m->state[0] < 0.5f) m->state[0] = 1.0f;
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 2.5f;
+    float ci = s->intra_ions[3], co = s->extra_ions[3];
+    double P = p_single(gamma, 1.0f, kCbarCl);
+    double flux = ghk_an(V, ci, co);
+    double ia = P * flux * (m->density * area) * m->state[0];
+    double icl = ia * kAtoPa;
+    float vol = volume(s);
+    float cpm = kFaraday * vol / 1000.0;
+    s->intra_ions[3] += (icl * dt) / cpm;
+    s->extra_ions[3] -= (icl * dt) / (cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -icl;
+}
+
+// Kv4.2 A-type (85, hh_biological.cpp:789-836, modifier_helpers.h:463-472): m^3 h, V-dependent
+// recovery, CaMKII (state[0] + state[1] of the last CaMKII modifier) phosphorylation in state[3].
+static double chan_kv42(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double vh = (m->state[4] != 0.0f) ? -fabs(m->state[4]) : -10.0f;
+    double k = (m->state[5] > 0.0f) ? m->state[5] : 13.5f;
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 5.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (k + 1e-6f)));
+    m->state[0] += (mi - m->state[0]) * dt / 1.5f;
+    float hi = 1.0f / (1.0f + exp((V + 65.0f) / 8.0f));
+    float th = 15.0f + (170.0f - 15.0f) / (1.0f + exp((V - -60.0f) / 8.0f));
+    m->state[1] += (hi - m->state[1]) * dt / th;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    float ck = 0.0f;
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == T_CAMKII) ck = s->modifiers[j].state[0] + s->modifiers[j].state[1];
+    m->state[3] += (ck * 0.5f - m->state[3]) * dt / 5000.0f;
+    if (m->state[3] < 0.0f) m->state[3] = 0.0f;
+    if (m->state[3] > 1.0f) m->state[3] = 1.0f;
+    double geff = gamma * (1.0f - m->state[3] * 0.7f);
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(geff, 1.0f, kCbarK);
+    double op = m->state[0] * m->state[0] * m->state[0] * m->state[1];
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * op;
+    double ik = ia * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+// Kv2.1 delayed rectifier (196, hh_biological.cpp:1392-1428, modifier_helpers.h:584-620): m^4 h
+// with a U-shaped inactivation, TEA block by state[5].
+static double chan_kv21(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 8.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - 0.0f) / 9.4f));
+    float tm = 15.0f + (73.0f - 15.0f) / (1.0f + exp((V - 0.0f) / 10.0f));
+    m->state[0] += (mi - m->state[0]) * dt / tm;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float left = 1.0f / (1.0f + exp(-(V - -26.5f) / 5.2f));
+    float right = 1.0f / (1.0f + exp((V - 15.0f) / 10.0f));
+    float hi = 1.0f - (1.0f - 0.2f) * left * right;
+    float th = 4000.0f + (300.0f - 4000.0f) / (1.0f + exp((V - -80.0f) / 15.0f));
+    m->state[1] += (hi - m->state[1]) * dt / th;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    double op = m->state[0] * m->state[0] * m->state[0] * m->state[0] * m->state[1];
+    double tea = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + tea / 10.0f);
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * op;
+    double ik = ia * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+// Persistent Na+ (105: Nav1.6 NaP, hh_biological.cpp:2951-2978) and the tadpole NaP (195:
+// Nav1.2 gamma, V1/2 -40 mV, slow inactivation in state[6], Na+ taken from the extracellular
+// pool, hh_biological.cpp:2980-3016).
+static double chan_nap(Modifier* m, Segment* s, float dt, bool tadpole) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double mv = m->state[0];
+    double vh = (m->state[1] != 0.0f) ? m->state[1] : (tadpole ? -40.0f : -58.7f);
+    double km = (m->state[2] > 0.0f) ? m->state[2] : 6.0f;
+    double gamma = (m->state[3] > 0.0f) ? m->state[3] : (tadpole ? 20.8f : 20.7f);
+    double tau = (m->state[4] > 0.0f) ? m->state[4] : 5.0f;
+    double po = (m->state[5] > 0.0f) ? m->state[5] : 0.016f;
+    float vhf = vh, kmf = km;
+    float mi = 1.0f / (1.0f + exp(-(V - vhf) / kmf));
+    mv += (mi - mv) * dt / tau;
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m->state[0] = mv;
+    float nai = s->intra_ions[0], nao = s->extra_ions[0];
+    double P = p_single(gamma, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, nai, nao);
+    if (!tadpole) {
+        double ia = P * flux * (m->density * area) * (mv * po);
+        float ina = (float)(ia * kAtoPa);
+        double cpv = kFaraday * volume(s) / 1000.0f;
+        s->intra_ions[0] -= (ina * dt) / cpv;
+        floor_ions(s->intra_ions);
+        return -ina;
+    }
+    float si = 1.0f / (1.0f + exp((V - -50.0f) / 6.0f));
+    double sv = m->state[6];
+    sv += (si - sv) * dt / 500.0f;
+    if (sv < 0.0f) sv = 0.0f;
+    if (sv > 1.0f) sv = 1.0f;
+    m->state[6] = sv;
+    double ia = P * flux * (m->density * area) * (mv * po * sv);
+    double ina = ia * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ina;
+}
+
+// Kir2.1 (106, hh_biological.cpp:3018-3037, modifier_helpers.h:870-891): Woodhull Mg2+ and
+// three-site spermine block.
+static inline float kir_f_open(float V) {
+    float vn = kFaraday * V * 0.001f / (kGas * kTemp);
+    float kd_mg = 0.3f * exp(-0.6f * 2.0f * vn);
+    float mg = 0.5f / kd_mg;
+    float kd_spm = 0.0775f * exp(-0.5f * 4.0f * vn);
+    float r = 0.001f / kd_spm;
+    float spm = 0.0f, rk = r;
+    for (int k = 0; k < 3; k++) { spm += rk; rk *= r; }
+    return 1.0f / (1.0f + mg + spm);
+}
+static double chan_kir(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[3] > 0.0f) ? m->state[3] : 30.0f;
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    float fo = kir_f_open(V);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    double ia = P * flux * (m->density * area) * fo;
+    double ik = ia * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+
+// Developmental Nav1.2 (193, hh_biological.cpp:314-357, modifier_helpers.h:221-241): m^3 with
+// fast and slow inactivation (fraction state[4], default 0.7; slow recovery 500 ms), V1/2 of
+// inactivation from state[2] (default 45, i.e. -45 mV), TTX and PKA/PKC block.
+static double chan_nav12dev(Modifier* m, Segment* s, float dt) {
+    static constexpr NavGates kDev = {16.0f, 6.0f, 41.5f, 66.5f, 45.0f};
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    if (s->V_m < -200.0f) s->V_m = -200.0f;
+    if (s->V_m > 200.0f) s->V_m = 200.0f;
+    V = s->V_m;
+    double mv = nav_minf(kDev, V) + (m->state[0] - nav_minf(kDev, V)) * exp(-dt / nav_mtau(kDev, V));
+    double vhalf = (m->state[2] > 0.0f) ? m->state[2] : 45.0f;
+    float vhf = vhalf;
+    float hi = 1.0f / (1.0f + exp((V + vhf) / 6.2f));
+    double hf = hi + (m->state[1] - hi) * exp(-dt / nav_htau(kDev, V));
+    double ts = (hi < m->state[3]) ? nav_htau(kDev, V) : 500.0f;
+    double hs = hi + (m->state[3] - hi) * exp(-dt / ts);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hf < 0.0f) hf = 0.0f;
+    if (hf > 1.0f) hf = 1.0f;
+    if (hs < 0.0f) hs = 0.0f;
+    if (hs > 1.0f) hs = 1.0f;
+    m->state[0] = mv;
+    m->state[1] = hf;
+    m->state[3] = hs;
+    double sf = (m->state[4] > 0.0f) ? m->state[4] : 0.7f;
+    float he = (1.0f - sf) * hf + sf * hs;
+    float op = mv * mv * mv * he;
+    double ttx = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + ttx / 10.0f);
+    float pkc = first_state_of(s, T_PKC, 0, 0.0f);
+    float pka = first_state_of(s, T_CAMP_PKA, 1, 0.0f);
+    op *= (1.0f - 0.7f * (pkc + pka) * 0.5f);
+    if (op < 0.0f) op = 0.0f;
+    double P = p_single(20.8f, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double ia = P * flux * (m->density * area) * op;
+    double ina = ia * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ina;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -895,6 +1140,14 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_KV31: return chan_kv31(m, s, dt);
         case T_KV11: return chan_kv11(m, s, dt);
         case T_HCN2: return chan_hcn1(m, s, dt, -95.0f, 557.0f, 1.54f);
+        case T_CAV13: return chan_cav13(m, s, dt);
+        case T_CL: return chan_cl(m, s, dt);
+        case T_KV42: return chan_kv42(m, s, dt);
+        case T_KV21: return chan_kv21(m, s, dt);
+        case T_NAP: return chan_nap(m, s, dt, false);
+        case T_NAP_TAD: return chan_nap(m, s, dt, true);
+        case T_KIR: return chan_kir(m, s, dt);
+        case T_NAV12DEV: return chan_nav12dev(m, s, dt);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);

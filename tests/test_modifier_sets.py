@@ -3,6 +3,8 @@ compiled reference on the same networks, tests/test_oracle_vs_reference.py):
 
 - ion homeostasis: the alpha3 and glial Na/K pumps, PMCA, NCX, KCC2, NKCC1, the Ca2+ buffer and
   HCN2 (src/hh_biological.cpp:99-141, 2574-2865, 3414-3445, 3599);
+- the channel set of the spinal cell types: developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, the two
+  persistent Na+ channels, Kir2.1 and the ClC-2 leak;
 - a network that mixes them with the core channel set runs the extended step-1 kernel, and the
   core-only networks keep the lean one (results identical either way).
 """
@@ -53,3 +55,13 @@ def test_pmca_hill_other_than_two_is_counted():
     assert sim.load_network(net) == 0
     assert sim.stats().unsupported_modifiers == 1
     sim.close()
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_channel_set_of_the_spinal_cell_types(seed):
+    """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2
+    (src/hh_biological.cpp:142-185, 314-414, 789-836, 1392-1428, 2951-3037)."""
+    net = bion.channel_network(seed)
+    cur = H.soma_clamp(net, 120.0)
+    _, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0

@@ -122,3 +122,12 @@ def test_ion_homeostasis_modifiers(seed):
     cur = cur[cur["segment_id"] < 12 * 6]
     rs, _, sp = _both(net, 300, lambda t: H.Stimuli(current=cur))
     assert sp.sum() > 0
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_channel_set_of_the_spinal_cell_types(seed):
+    """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2."""
+    net = bion.channel_network(seed)
+    cur = H.soma_clamp(net, 120.0)
+    rs, _, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
