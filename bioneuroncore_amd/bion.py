@@ -81,8 +81,11 @@ HOMEOSTASIS = (HCN2, PUMP_A3, PUMP_GLIA, PMCA, NCX, KCC2, NKCC1, CA_BUFFER)
 # Further voltage-gated and leak channels of the tadpole spinal network's cell types.
 CAV13, CLC2, KV42, KV21, NAP, NAP_TAD, KIR, NAV12DEV = 1, 2, 85, 196, 105, 195, 106, 193
 CHANNELS2 = (CAV13, CLC2, KV42, KV21, NAP, NAP_TAD, KIR, NAV12DEV)
+# Receptors: further ionotropic types, and the Gi/o- (17-40) and Gs-coupled (41-51) families.
+KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B, GABAB = 10, 12, 13, 14, 97, 194, 17
+RECEPTORS2 = (KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B) + tuple(range(17, 52))
 SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -95,6 +98,7 @@ CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, P
                  GLYT2: 1.0, GAT1: 1.0, HCN2: 120.0, PUMP_A3: 1500.0, PUMP_GLIA: 1500.0, PMCA: 300.0,
                  NCX: 30.0, KCC2: 0.093, NKCC1: 0.1, CA_BUFFER: 0.5, CAV13: 2.0, CLC2: 0.8,
                  KV42: 41.51, KV21: 2.595, NAP: 3.0, NAP_TAD: 10.6, KIR: 0.35, NAV12DEV: 60.0}
+CANON_DENSITY.update({t: 40.0 for t in RECEPTORS2})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -422,10 +426,11 @@ def homeostasis_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6,
     return net
 
 
-def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5) -> Network:
+def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, myelin: bool = False) -> Network:
     """Chain neurons built on the developmental Nav1.2 with Kv2.1, Kv4.2, Kv1.1, K2P, NALCN and
     the pump on every segment; somas add Cav1.3, Cav2.1, NaP (Nav1.6 and tadpole), Kir2.1, the
-    ClC-2 leak and HCN2; AMPA+NMDA / GABA-A synapses."""
+    ClC-2 leak and HCN2; AMPA+NMDA / GABA-A synapses. myelin: every third segment also carries
+    myelin, which blocks the listed channel types and the ionotropic receptors."""
     net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12DEV, KV21, KV42, KV11, K2P, NALCN, PUMP],
                         [CAV13, CAV21, NAP, NAP_TAD, KIR, CLC2, HCN2], 3.0, "ampa_nmda_gaba", 4, seed,
                         lognormal_sigma=0.2)
@@ -434,6 +439,54 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5) -> 
         i, j = np.nonzero(mods["type"] == t)
         for k, v in init.items():
             mods["state"][i, j, k] = v
+    if myelin:
+        segs = net.segments
+        for i in range(0, len(segs), 3):
+            k = int(segs["modifier_count"][i])
+            mods["type"][i, k], mods["density"][i, k] = 4, 1.0
+            segs["modifier_count"][i] = k + 1
+    return net
+
+
+def receptor_network(seed: int, n_neurons: int = 14, seg_per_neuron: int = 5) -> Network:
+    """Chain neurons whose synapses carry, by index, glycine, nicotinic, 5-HT3+kainate, GluN2B
+    NMDA, GABA-B, D1 and alpha7+D2 receptors besides AMPA+NMDA and GABA-A (so glycine,
+    acetylcholine, serotonin, GABA and dopamine are released); somas carry Gi/o (GABA-B, 5-HT1A,
+    D2, M2, mGluR2, CB1, H3, MT1) and Gs (D1, 5-HT7, beta1, A2A, H2) receptors as segment
+    modifiers, dendrites glycine and GluN2B receptors."""
+    soma = [17, 18, 22, 25, 29, 36, 38, 39, 44, 43, 48, 46, 50]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21] + soma,
+                        4.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    mods = segs["modifiers"]
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    for j in range(6, 6 + len(soma)):
+        mods["density"][soma_idx, j] = 40.0
+    mods["active_vars"][soma_idx[::2], 6] = 3            # EC50/gamma from state[3]/state[4]
+    mods["state"][soma_idx[::2], 6, 3] = 0.4
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        mods["type"][i, k], mods["density"][i, k] = GLYCINE, 100.0
+        mods["type"][i, k + 1], mods["density"][i, k + 1] = NMDA2B, 20.0
+        mods["state"][i, k + 1, 2] = 1.0
+        segs["modifier_count"][i] = k + 2
+    syn = net.synapses
+    pr = syn["post_receptors"]
+    kinds = [[GLYCINE], [NICOTINIC], [HT3, KAINATE], [NMDA2B], [GABAB], [44], [ALPHA7, 22]]
+    for y in range(len(syn)):
+        c = y % 9
+        if c >= len(kinds):
+            continue
+        ts = kinds[c]
+        pr["type"][y, :] = 0
+        pr["density"][y, :] = 0.0
+        pr["state"][y, :, :] = 0.0
+        for j, t in enumerate(ts):
+            pr["type"][y, j] = t
+            pr["density"][y, j] = 300.0 if t not in (GABAB, 44, 22) else 30.0
+        if NMDA2B in ts:
+            pr["state"][y, ts.index(NMDA2B), 2] = 1.0
+        syn["post_receptor_count"][y] = len(ts)
     return net
 
 

@@ -376,6 +376,8 @@ static int build_device(Sim& S) {
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
             if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
             if (!modifier_supported(m.type)) S.unsupported_mod++;
+            // G-protein receptors read state[active_vars + 1]: past the record's state array
+            if (m.type >= 17 && m.type <= 51 && m.active_vars >= 7) S.unsupported_mod++;
             // PMCA's Hill powers are exact squares only for n = 2 and a float-valued K_m
             if (m.type == M_PMCA && ((m.state[2] > 0.0f && m.state[2] != 2.0) ||
                                      (m.state[1] > 0.0f && (double)(float)m.state[1] != m.state[1])))
@@ -513,6 +515,7 @@ static int build_device(Sim& S) {
             y_rdens[(size_t)j * ns + k] = y.post_receptors[j].density;
             for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
             if (!modifier_supported(rt)) S.unsupported_mod++;
+            if (rt >= 17 && rt <= 51 && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
             if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }
         y_flags[k] = fl;
@@ -820,10 +823,7 @@ static void xfer(Sim& S, bool to_device) {
             Modifier& m = s.modifiers[j];
             for (int q = 0; q < 7; q++) m.state[q] = ms[((size_t)j * 7 + q) * npad + di];
             uint8_t t = m.type;
-            bool blocked = my && t != 4 &&
-                           ((t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194) || t == M_K2P ||
-                            t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 || t == M_KVDR ||
-                            t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK);
+            bool blocked = my && t != 4 && myelin_blocks(t);
             if (ran && !blocked) m.state[7] = seg_area(s);
         }
     }
@@ -839,7 +839,7 @@ static void xfer(Sim& S, bool to_device) {
             Modifier& m = y.post_receptors[j];
             for (int q = 0; q < 7; q++) m.state[q] = yrs[((size_t)j * 7 + q) * ns + k];
             uint8_t t = m.type;
-            bool blocked = my && (t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194);
+            bool blocked = my && ionotropic_type(t);
             if (ran && !(y.type == 1 || y.type == 2) && !blocked) m.state[7] = y.post_area;
         }
     }

@@ -45,32 +45,45 @@ __host__ __device__ static inline bool modifier_core(uint8_t t) {
     switch (t) {
         case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
         case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
-        case M_NAV12: case M_KV31: case M_KV11:
+        case M_NAV12: case M_KV31: case M_KV11: case 4:
             return true;
         default:
             return false;
     }
 }
 
+// Ionotropic types (kernel_dispatch.h:283-289) and the channel types myelin blocks
+// (kernel_dispatch.h:299-309) among the covered IDs; myelin blocks no pump or transporter.
+__host__ __device__ static inline bool ionotropic_type(uint8_t t) {
+    return t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194;
+}
+__host__ __device__ static inline bool myelin_blocks(uint8_t t) {
+    return ionotropic_type(t) || t == M_K2P || t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 ||
+           t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK || t == M_KV42 ||
+           t == M_NAP || t == M_NAP_TAD || t == M_HCN2;
+}
+
 // Modifiers that read the segment's extracellular transmitters in step 1 (the ionotropic
 // receptors) or also take them up (the transporters).
 __host__ __device__ static inline bool modifier_reads_nt(uint8_t t) {
     return t == M_AMPA || t == M_NMDA || t == M_GABAA || (t >= M_GLAST && t <= M_GAT3) ||
-           (t >= M_EAAT3 && t <= M_GAT1);
+           (t >= M_EAAT3 && t <= M_GAT1) || (t >= 10 && t <= 51) || t == 97 || t == 194;
 }
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
     switch (t) {
         case M_PUMP: case M_AMPA: case M_NMDA: case M_GABAA: case M_SK: case M_BK: case M_K2P:
         case M_KV7: case M_KVDR: case M_HCN1: case M_NALCN: case M_CAV21: case M_NAV16:
+        case 4:   // myelin: no current, no state (hh_biological.cpp:450-453); blocks others
         case M_NAV12: case M_KV31: case M_KV11: case M_CX36: case M_CX43: case M_GLAST:
         case M_GLYT1: case M_GAT3: case M_EAAT3: case M_GLYT2: case M_GAT1: case M_HCN2:
         case M_PUMP_A3: case M_PUMP_GLIA: case M_PMCA: case M_NCX: case M_KCC2: case M_NKCC1:
         case M_CA_BUFFER: case M_CAV13: case M_CLC2: case M_KV42: case M_KV21: case M_NAP:
-        case M_NAP_TAD: case M_KIR: case M_NAV12DEV:
+        case M_NAP_TAD: case M_KIR: case M_NAV12DEV: case 10: case 12: case 13: case 14: case 97:
+        case 194:
             return true;
         default:
-            return false;
+            return t >= 17 && t <= 51;   // the Gi/o- and Gs-coupled receptor families
     }
 }
 
@@ -620,17 +633,38 @@ BC_DEV double ch_nalcn(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologi
 // Ionotropic receptor (modifier_helpers.h:1213-1276).
 struct IonoK { float gamma, ec50, t_rise, t_decay; int nt; float pna, pk, pca; bool anion; float t_desens; };
 
+// Transmitter k of the segment (k a constant after inlining: the array stays in registers).
+BC_DEV float nt_pick(const SegRegs& s, int k) {
+    switch (k) {
+        case 0: return s.nt[0];
+        case 1: return s.nt[1];
+        case 2: return s.nt[2];
+        case 3: return s.nt[3];
+        case 4: return s.nt[4];
+        case 5: return s.nt[5];
+        case 6: return s.nt[6];
+        case 7: return s.nt[7];
+        case 8: return s.nt[8];
+        case 9: return s.nt[9];
+        case 11: return s.nt[11];
+        default: return s.nt[10];
+    }
+}
+// state[k] of a modifier for k <= 7: state[7] is the effective area the caller set (a register
+// here); larger k, past the record's state array, are refused at load.
+BC_DEV double state_any(const ModSlot& m, uint32_t k) { return k < 7 ? m.get(k) : (double)m.a7; }
+
 template <bool kSegV = false>
 BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     float area = eff_area(m, s);
     double V = s.V;
     double cpm = s.cpm;
     float gamma = p.gamma;
-    if (m.av > 0 && m.av < 7) {
-        double g = m.get(m.av);
+    if (m.av > 0) {
+        double g = state_any(m, m.av);
         if (g > 0.0f) gamma = g;
     }
-    float nt = p.nt ? s.nt[1] : s.nt[0];   // glutamate (0) or GABA (1)
+    float nt = nt_pick(s, p.nt);
     float r_inf = nt / (nt + p.ec50);
     double s0 = m.get(0);
     float r = s0;
@@ -677,9 +711,10 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
     return -itot;
 }
 
-template <bool kSegV = false>   // kSegV: also requires mg_cache_init
-BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_biological.cpp:48-55
-    const IonoK k = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
+// NMDA (9) and the GluN2B NMDA (k2b, 194: rise 10 ms, decay 150 ms).
+template <bool kSegV = false, bool k2b = false>   // kSegV: also requires mg_cache_init
+BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_biological.cpp:48-62
+    const IonoK k = {50.0f, 1.0f, k2b ? 10.0f : 5.0f, k2b ? 150.0f : 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
     double I = iono<kSegV>(m, s, dt, k);
     double emg = (kSegV || s.V == s.vkmg) ? s.emg : bc::exp(-0.062f * s.V);
     float mg = 1.0f / (1.0f + (1.0f / 3.57f) * emg);
@@ -1267,12 +1302,88 @@ BC_DEV double ch_transporter(const ModSlot& m, SegRegs& s, float dt, float km, f
 
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
 // current and leave their state untouched, as the reference's device switch does.
+// Gi/o-coupled receptors (17-40, modifier_helpers.h:1332-1363): occupancy of transmitter NT
+// drives a GIRK gate (state[0]) with desensitisation (state[1]); EC50 and gamma from
+// state[active_vars] and state[active_vars + 1] when positive.
+template <int NT>
+BC_DEV double ch_gio(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    double V = s.V;
+    const double sa = state_any(m, m.av), sb = state_any(m, m.av + 1u);
+    float ec50 = (sa > 0.0f) ? sa : 1.0f;
+    float gamma = (sb > 0.0f) ? sb : 35.0f;
+    float nt = s.nt[NT];
+    float act = nt / (nt + ec50);
+    float r = m.get(0);
+    float tau = (act > r) ? 70.0f : 1500.0f;
+    r += (act - r) * dt / tau;
+    if (r < 0.0f) r = 0.0f;
+    if (r > 1.0f) r = 1.0f;
+    m.set(0, r);
+    float d = m.get(1);
+    d += (act - d) * dt / 40000.0f;
+    if (d < 0.0f) d = 0.0f;
+    if (d > 1.0f) d = 1.0f;
+    m.set(1, d);
+    float fo = kir_f_open(V) * r * (1.0f - 0.55f * d);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, s.in[1], s.ex[1]);
+    double ik = P * flux * (m.dens * area) * fo * kAtoPa;
+    s.in[1] -= (ik * dt) / s.cpm;
+    floor7(s.in);
+    return -ik;
+}
+
+// Gs-coupled receptors (41-51, modifier_helpers.h:1379-1398): activation (state[0]) and
+// desensitisation (state[1]) signals of transmitter NT; no current.
+template <int NT>
+BC_DEV double ch_gs(const ModSlot& m, SegRegs& s, float dt) {
+    const double sa = state_any(m, m.av), sb = state_any(m, m.av + 1u);
+    float ec50 = (sa > 0.0f) ? sa : 1.0f;
+    float tdes = (sb > 0.0f) ? sb : 70000.0f;
+    float nt = s.nt[NT];
+    float occ = nt / (nt + ec50);
+    float act = m.get(0);
+    float tau = (occ > act) ? 70.0f : 1500.0f;
+    act += (occ - act) * dt / tau;
+    if (act < 0.0f) act = 0.0f;
+    if (act > 1.0f) act = 1.0f;
+    m.set(0, act);
+    float d = m.get(1);
+    d += (occ - d) * dt / tdes;
+    if (d < 0.0f) d = 0.0f;
+    if (d > 1.0f) d = 1.0f;
+    m.set(1, d);
+    return 0.0;
+}
+
 // The covered IDs outside the core set, in a switch of their own: the core networks' kernels
 // never see these cases in their register allocation.
 BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
     switch (type) {
         case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
         case M_HCN2: return ch_hcn1<true>(m, s, dt);
+        case 10: return iono(m, s, dt, IonoK{3.0f, 0.5f, 1.0f, 10.0f, 0, 0.5f, 0.5f, 0.0f, false, 5.0f});
+        case 12: return iono(m, s, dt, IonoK{60.0f, 0.5f, 0.3f, 5.0f, 7, 0.0f, 0.0f, 0.0f, true, 2000.0f});
+        case 13: return iono(m, s, dt, IonoK{30.0f, 0.5f, 1.0f, 10.0f, 4, 0.48f, 0.48f, 0.04f, false, 2000.0f});
+        case 14: return iono(m, s, dt, IonoK{2.0f, 0.3f, 1.0f, 10.0f, 3, 0.40f, 0.40f, 0.20f, false, 800.0f});
+        case 97: return iono(m, s, dt, IonoK{40.0f, 0.5f, 0.5f, 3.0f, 4, 0.12f, 0.17f, 0.71f, false, 50.0f});
+        case 194: return rec_nmda<false, true>(m, s, dt);
+        case 17: return ch_gio<1>(m, s, dt);
+        case 18: case 19: case 20: case 21: return ch_gio<3>(m, s, dt);
+        case 22: case 23: case 24: return ch_gio<2>(m, s, dt);
+        case 25: case 26: return ch_gio<4>(m, s, dt);
+        case 27: case 28: return ch_gio<5>(m, s, dt);
+        case 29: case 30: case 31: case 32: case 33: case 34: return ch_gio<0>(m, s, dt);
+        case 35: return ch_gio<6>(m, s, dt);
+        case 36: case 37: return ch_gio<9>(m, s, dt);
+        case 38: return ch_gio<8>(m, s, dt);
+        case 39: case 40: return ch_gio<11>(m, s, dt);
+        case 41: case 42: case 43: return ch_gs<3>(m, s, dt);
+        case 44: case 45: case 51: return ch_gs<2>(m, s, dt);
+        case 46: case 47: return ch_gs<5>(m, s, dt);
+        case 48: case 49: return ch_gs<6>(m, s, dt);
+        case 50: return ch_gs<8>(m, s, dt);
         case M_CAV13: return ch_cav13(m, s, dt);
         case M_CLC2: return ch_cl(m, s, dt);
         case M_KV42: return ch_kv42(m, s, dt);

@@ -23,14 +23,8 @@
 
 namespace bc {
 
-__device__ static inline bool is_ionotropic(uint8_t t) {
-    return t <= 2 || (t >= 8 && t <= 16) || t == 97 || t == 134 || t == 194;
-}
-// kernel_dispatch.h:301-309 restricted to the covered IDs (the rest run as 0-current anyway).
-__device__ static inline bool myelin_blocked(uint8_t t) {
-    return is_ionotropic(t) || t == M_K2P || t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 ||
-           t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK;
-}
+__device__ static inline bool is_ionotropic(uint8_t t) { return ionotropic_type(t); }
+__device__ static inline bool myelin_blocked(uint8_t t) { return myelin_blocks(t); }
 __device__ static inline uint8_t nt_of_receptor(uint8_t rt) {   // run_tick_common.h:241-248
     if (rt == 11 || rt == 17) return 1;
     if (rt == 12) return 7;
@@ -73,10 +67,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     seg_consts_init(s);
     ghk_cache_init(s);
     const uint8_t sfl = d.sflags[i];
-    if (kNt && (sfl & SF_NTREAD)) {   // step 1 reads glutamate (0), GABA (1) and glycine (7) only
-        s.nt[0] = d.nt[0 * P + i];
-        s.nt[1] = d.nt[1 * P + i];
-        s.nt[7] = d.nt[7 * P + i];
+    if (kNt && (sfl & SF_NTREAD)) {   // receptors read transmitters 0-11, transporters 0, 1 and 7
+#pragma unroll
+        for (int k = 0; k < 12; k++) s.nt[k] = d.nt[k * P + i];
     }
     const bool my = sfl & SF_MYELIN;
     const uint32_t cnt = d.mcount[i];

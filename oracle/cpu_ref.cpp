@@ -561,6 +561,14 @@ struct Iono { float gamma, ec50, t_rise, t_decay; int nt; float pna, pk, pca; bo
 static constexpr Iono kAmpa = {8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f};
 static constexpr Iono kNmda = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
 static constexpr Iono kGabaa = {28.0f, 1.0f, 1.0f, 5.0f, 1, 0.0f, 0.0f, 0.0f, true, 2000.0f};
+// receptors_biological.cpp:57-100 with units.h:160-275: kainate (10), glycine (12), nicotinic
+// alpha4beta2 (13), 5-HT3 (14), alpha7 (97); GluN2B NMDA (194) is NMDA with slower kinetics.
+static constexpr Iono kKainate = {3.0f, 0.5f, 1.0f, 10.0f, 0, 0.5f, 0.5f, 0.0f, false, 5.0f};
+static constexpr Iono kGlycine = {60.0f, 0.5f, 0.3f, 5.0f, 7, 0.0f, 0.0f, 0.0f, true, 2000.0f};
+static constexpr Iono kNicotinic = {30.0f, 0.5f, 1.0f, 10.0f, 4, 0.48f, 0.48f, 0.04f, false, 2000.0f};
+static constexpr Iono k5ht3 = {2.0f, 0.3f, 1.0f, 10.0f, 3, 0.40f, 0.40f, 0.20f, false, 800.0f};
+static constexpr Iono kAlpha7 = {40.0f, 0.5f, 0.5f, 3.0f, 4, 0.12f, 0.17f, 0.71f, false, 50.0f};
+static constexpr Iono kNmda2b = {50.0f, 1.0f, 10.0f, 150.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
 
 static double iono_current(Modifier* m, Segment* s, float dt, const Iono& p) {
     float area = eff_area(m, s);
@@ -621,11 +629,91 @@ static double nmda_cdi(Modifier* m, Segment* s, float dt) {   // modifier_helper
     float coag = 10.0f / (10.0f + 1.0f);
     return m->state[2] * coag;
 }
-static double rec_nmda(Modifier* m, Segment* s, float dt) {   // receptors_biological.cpp:48-55
-    double I = iono_current(m, s, dt, kNmda);
+static double rec_nmda(Modifier* m, Segment* s, float dt, const Iono& k = kNmda) {   // receptors_biological.cpp:48-62
+    double I = iono_current(m, s, dt, k);
     float mg = 1.0f / (1.0f + (1.0f / 3.57f) * exp(-0.062f * s->V_m));
     double r = I * mg;
     return r * nmda_cdi(m, s, dt);
+}
+
+// Gi/o-coupled receptors (17-40, modifier_helpers.h:1332-1363, receptors_biological.cpp:163-186):
+// transmitter occupancy drives a GIRK gate (state[0]) with desensitisation (state[1]); EC50 and
+// gamma come from state[active_vars] and state[active_vars + 1] when positive. The transmitter
+// each one reads: GABA-B GABA; 5-HT1A/B/D/F serotonin; D2-4 dopamine; M2/M4 acetylcholine;
+// A1/A3 adenosine; mGluR2-8 glutamate; alpha2 noradrenaline; CB1/2 endocannabinoid; H3
+// histamine; MT1/2 melatonin.
+static int gio_nt(uint8_t t) {
+    if (t == 17) return 1;
+    if (t <= 21) return 3;
+    if (t <= 24) return 2;
+    if (t <= 26) return 4;
+    if (t <= 28) return 5;
+    if (t <= 34) return 0;
+    if (t == 35) return 6;
+    if (t <= 37) return 9;
+    if (t == 38) return 8;
+    return 11;
+}
+static float kir_f_open(float V);
+static double chan_gio(Modifier* m, Segment* s, float dt, int nt_index) {
+    float area = eff_area(m, s);
+    double V = s->V_m;
+    double cpm = kFaraday * volume(s) / 1000.0;
+    if (m->active_vars >= 7) g_unsupported++;
+    float ec50 = (m->state[m->active_vars] > 0.0f) ? m->state[m->active_vars] : 1.0f;
+    float gamma = (m->state[m->active_vars + 1] > 0.0f) ? m->state[m->active_vars + 1] : 35.0f;
+    float nt = s->neurotransmitter[nt_index];
+    float act = nt / (nt + ec50);
+    float r = m->state[0];
+    float tau = (act > r) ? 70.0f : 1500.0f;
+    r += (act - r) * dt / tau;
+    if (r < 0.0f) r = 0.0f;
+    if (r > 1.0f) r = 1.0f;
+    m->state[0] = r;
+    float d = m->state[1];
+    d += (act - d) * dt / 40000.0f;
+    if (d < 0.0f) d = 0.0f;
+    if (d > 1.0f) d = 1.0f;
+    m->state[1] = d;
+    float fo = kir_f_open(V) * r * (1.0f - 0.55f * d);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    double ia = P * flux * (m->density * area) * fo;
+    double ik = ia * kAtoPa;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    floor_ions(s->intra_ions);
+    return -ik;
+}
+
+// Gs-coupled receptors (41-51, modifier_helpers.h:1379-1398): occupancy drives the activation
+// (state[0]) and desensitisation (state[1]) signals; no current. 5-HT4/6/7 read serotonin,
+// D1/D5 and TAAR1 dopamine, A2A/A2B adenosine, beta1/2 noradrenaline, H2 histamine.
+static int gs_nt(uint8_t t) {
+    if (t <= 43) return 3;
+    if (t <= 45) return 2;
+    if (t <= 47) return 5;
+    if (t <= 49) return 6;
+    if (t == 50) return 8;
+    return 2;
+}
+static double chan_gs(Modifier* m, Segment* s, float dt, int nt_index) {
+    if (m->active_vars >= 7) g_unsupported++;
+    float ec50 = (m->state[m->active_vars] > 0.0f) ? m->state[m->active_vars] : 1.0f;
+    float tdes = (m->state[m->active_vars + 1] > 0.0f) ? m->state[m->active_vars + 1] : 70000.0f;
+    float nt = s->neurotransmitter[nt_index];
+    float occ = nt / (nt + ec50);
+    float act = m->state[0];
+    float tau = (occ > act) ? 70.0f : 1500.0f;
+    act += (occ - act) * dt / tau;
+    if (act < 0.0f) act = 0.0f;
+    if (act > 1.0f) act = 1.0f;
+    m->state[0] = act;
+    float d = m->state[1];
+    d += (occ - d) * dt / tdes;
+    if (d < 0.0f) d = 0.0f;
+    if (d > 1.0f) d = 1.0f;
+    m->state[1] = d;
+    return 0.0f;
 }
 
 // Connexin Cx36 (114) and Cx43 (198) (hh_biological.cpp:3266-3300, 3310-3341, identical
@@ -1122,6 +1210,7 @@ static double chan_transporter(Modifier* m, Segment* s, float dt, const Transpor
 
 static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
     switch (type) {
+        case T_MYELIN: return 0.0;   // hh_biological.cpp:450-453
         case T_PUMP: return chan_pump(m, s, dt);
         case T_AMPA: return iono_current(m, s, dt, kAmpa);
         case T_NMDA: return rec_nmda(m, s, dt);
@@ -1140,6 +1229,12 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_KV31: return chan_kv31(m, s, dt);
         case T_KV11: return chan_kv11(m, s, dt);
         case T_HCN2: return chan_hcn1(m, s, dt, -95.0f, 557.0f, 1.54f);
+        case 10: return iono_current(m, s, dt, kKainate);
+        case T_GLYCINE: return iono_current(m, s, dt, kGlycine);
+        case T_NICOTINIC: return iono_current(m, s, dt, kNicotinic);
+        case T_5HT3: return iono_current(m, s, dt, k5ht3);
+        case 97: return iono_current(m, s, dt, kAlpha7);
+        case 194: return rec_nmda(m, s, dt, kNmda2b);
         case T_CAV13: return chan_cav13(m, s, dt);
         case T_CL: return chan_cl(m, s, dt);
         case T_KV42: return chan_kv42(m, s, dt);
@@ -1161,7 +1256,11 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_EAAT3: return chan_transporter(m, s, dt, kEaat3);
         case T_GLYT2: return chan_transporter(m, s, dt, kGlyt2);
         case T_GAT1: return chan_transporter(m, s, dt, kGat1);
-        default: g_unsupported++; return 0.0;
+        default:
+            if (type >= 17 && type <= 40) return chan_gio(m, s, dt, gio_nt(type));
+            if (type >= 41 && type <= 51) return chan_gs(m, s, dt, gs_nt(type));
+            g_unsupported++;
+            return 0.0;
     }
 }
 
@@ -1772,7 +1871,8 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
             uint8_t t = s->modifiers[j].type;
             if (my && t != T_MYELIN && (is_ionotropic(t) || t == T_K2P || t == T_NALCN || t == T_KV31 ||
                                          t == T_KV11 || t == T_KV7 || t == T_KVDR || t == T_CAV21 ||
-                                         t == T_HCN1 || t == T_SK || t == T_BK))
+                                         t == T_HCN1 || t == T_SK || t == T_BK || t == T_KV42 ||
+                                         t == T_NAP || t == T_NAP_TAD || t == T_HCN2))
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;

@@ -57,11 +57,23 @@ def test_pmca_hill_other_than_two_is_counted():
     sim.close()
 
 
-@pytest.mark.parametrize("seed", [41, 42])
-def test_channel_set_of_the_spinal_cell_types(seed):
+@pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
+def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2
-    (src/hh_biological.cpp:142-185, 314-414, 789-836, 1392-1428, 2951-3037)."""
-    net = bion.channel_network(seed)
+    (src/hh_biological.cpp:142-185, 314-414, 789-836, 1392-1428, 2951-3037); myelin blocks the
+    listed types (include/kernel_dispatch.h:299-309)."""
+    net = bion.channel_network(seed, myelin=myelin)
     cur = H.soma_clamp(net, 120.0)
+    _, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0
+
+
+@pytest.mark.parametrize("seed", [51, 52])
+def test_receptor_families(seed):
+    """Glycine, nicotinic, 5-HT3, kainate, alpha7 and GluN2B receptors on synapses and segments;
+    Gi/o- and Gs-coupled receptors as segment and synaptic modifiers
+    (src/receptors_biological.cpp:40-200, include/modifier_helpers.h:1332-1398)."""
+    net = bion.receptor_network(seed)
+    cur = H.soma_clamp(net, 150.0)
     _, spk = _compare(net, 400, cur)
     assert spk.sum() > 0

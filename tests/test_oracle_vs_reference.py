@@ -124,10 +124,22 @@ def test_ion_homeostasis_modifiers(seed):
     assert sp.sum() > 0
 
 
-@pytest.mark.parametrize("seed", [41, 42])
-def test_channel_set_of_the_spinal_cell_types(seed):
-    """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2."""
-    net = bion.channel_network(seed)
+@pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
+def test_channel_set_of_the_spinal_cell_types(seed, myelin):
+    """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2;
+    with myelin on every third segment in one case."""
+    net = bion.channel_network(seed, myelin=myelin)
     cur = H.soma_clamp(net, 120.0)
     rs, _, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
     assert sp.sum() > 0
+
+
+@pytest.mark.parametrize("seed", [51, 52])
+def test_receptor_families(seed):
+    """Glycine, nicotinic, 5-HT3, kainate, alpha7 and GluN2B receptors on synapses and segments;
+    Gi/o- and Gs-coupled receptors as segment and synaptic modifiers."""
+    net = bion.receptor_network(seed)
+    cur = H.soma_clamp(net, 150.0)
+    rs, ry, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
+    assert np.any(ry["cleft_neurotransmitter"][:, [2, 3, 4, 7]] > 0.0)
