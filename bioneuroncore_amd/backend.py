@@ -45,7 +45,7 @@ EXT_SYMBOLS = ("bchip_load_bion_data", "bchip_save_bion", "bchip_set_coherence",
                "bchip_batch_read", "bchip_get_schedule", "bchip_set_schedule", "bchip_autotune",
                "bchip_set_graphs", "bchip_set_jacobi", "bchip_set_cable_mode", "bchip_set_coupling_scale",
                "bchip_adjoint", "bchip_set_adjoint_memory", "bchip_read_bion", "bchip_free_bion",
-               "bchip_write_bion", "bchip_set_stimulus_convention")
+               "bchip_write_bion", "bchip_set_stimulus_convention", "bchip_set_protein_mapping")
 
 
 class BionDataC(C.Structure):
@@ -126,6 +126,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_adjoint.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P, P]
     lib.bchip_set_adjoint_memory.argtypes = [P, C.c_uint64]
     lib.bchip_set_stimulus_convention.argtypes = [P, C.c_int]
+    lib.bchip_set_protein_mapping.argtypes = [P, C.c_int]
     _lib = lib
     return lib
 
@@ -178,15 +179,23 @@ class Simulation:
     def load_bion(self, path: str) -> int:
         return int(self.lib.gpu_backend_load_bion(self.h, path.encode()))
 
-    def load_network(self, net: bion.Network) -> int:
+    def load_network(self, net: bion.Network, nuclei: np.ndarray | None = None) -> int:
         hdr = bion.make_header(net)
         b = BionDataC()
         C.memmove(b.header, hdr.ctypes.data, 100)
         arrs = [np.ascontiguousarray(a) for a in (net.neurons, net.segments, net.connections, net.synapses)]
         b.neurons, b.segments, b.connections, b.synapses = [a.ctypes.data if len(a) else None for a in arrs]
         b.num_neurons, b.num_segments, b.num_connections, b.num_synapses = [len(a) for a in arrs]
+        if nuclei is not None and len(nuclei):
+            nuc = np.ascontiguousarray(nuclei, dtype=bion.NUCLEUS_DT)
+            b.nuclei, b.num_nuclei = nuc.ctypes.data, len(nuc)
         rc = int(self.lib.bchip_load_bion_data(self.h, C.byref(b)))
         return rc
+
+    def set_protein_mapping(self, on: bool) -> None:
+        """Nucleus proteins set the release parameters of chemical synapses (see the C header)."""
+        if self.lib.bchip_set_protein_mapping(self.h, 1 if on else 0) != 0:
+            raise RuntimeError("bchip_set_protein_mapping failed")
 
     def load_batch(self, nets: list[bion.Network]) -> int:
         """Load independent networks as one batch (network k's results equal a separate load)."""

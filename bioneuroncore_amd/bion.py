@@ -48,6 +48,14 @@ HEADER_DT = np.dtype([
     ("file_sz_segment", "<u4"), ("file_sz_synapse", "<u4"), ("file_sz_nucleus", "<u4"),
     ("file_sz_intraconnection", "<u4"), ("file_sz_neuron", "<u4"), ("file_sz_modifier", "<u4")])
 NUCLEUS_SIZE = 500
+# Nucleus record (reference include/types.h: gene activity and protein levels of 57 genes).
+NUCLEUS_DT = np.dtype([("neuron_id", "<u4"), ("enabled", "u1"), ("consolidation_mode", "u1"), ("pad", "u1", 2),
+                       ("consolidation_interval_ticks", "<u4"), ("last_consolidation_tick", "<u4"),
+                       ("gene_activity", "<f4", 57), ("protein_level", "<f4", 57),
+                       ("protein_synthesis_rate", "<f4"), ("pcreb_level", "<f4"), ("erk_level", "<f4"),
+                       ("nfkb_level", "<f4"), ("mtorc1_level", "<f4"), ("amino_acid_level", "<f4"),
+                       ("res", "u1", 4)])
+assert NUCLEUS_DT.itemsize == NUCLEUS_SIZE
 CURRENT_STIM_DT = np.dtype([("segment_id", "<u4"), ("amplitude_pA", "<f4"), ("duration_ms", "<f4"),
                             ("start_tick", "<u8")])
 VCLAMP_DT = np.dtype([("segment_id", "<u4"), ("target_mV", "<f4"), ("start_tick", "<u8"),
@@ -84,7 +92,8 @@ CHANNELS2 = (CAV13, CLC2, KV42, KV21, NAP, NAP_TAD, KIR, NAV12DEV)
 # Receptors: further ionotropic types, and the Gi/o- (17-40) and Gs-coupled (41-51) families.
 KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B, GABAB = 10, 12, 13, 14, 97, 194, 17
 RECEPTORS2 = (KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B) + tuple(range(17, 52))
-SUPPORTED_MODIFIERS = (PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
+MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
+SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
                        NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2
 
 # Physiological ion means (reference include/cpg_config.h:400-412).

@@ -138,6 +138,7 @@ struct Sim {
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
     bool receptors_only = true;   // every receptor slot is AMPA, NMDA or GABA-A
     bool nt_read = false;         // some segment modifier reads or takes up transmitters
+    bool protein_map = false;     // nucleus proteins set the release parameters (see release_params)
     bool ext_mods = false;        // some segment modifier is outside the core channel set
     std::vector<uint8_t> seg_dispatched;   // [n*32] slot ran at least once (state[7] written)
     hipEvent_t ev[8] = {};
@@ -208,6 +209,49 @@ struct Sim {
 static uint32_t uf_find(std::vector<uint32_t>& p, uint32_t x) {
     while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
     return x;
+}
+
+// Release parameters of every synapse slot: RRP_max, release probability and the vesicle
+// priming rate RRP_recovery + CaMKII(pre) 0.02 + calcineurin(pre) 0.01 (kernel_dispatch.h:
+// 612-631). With protein mapping on (the reference applies it while its consolidation runs,
+// simulation_host.cpp:407-414, kernel_dispatch.h:651-663), a chemical synapse whose presynaptic
+// neuron has a nucleus takes RRP_max, the release probability and RRP_recovery from that
+// nucleus's Munc13-1, synaptotagmin-1 and complexin levels; the nucleus state does not change
+// on the device, so the mapped values are constant and computed here once.
+static void protein_mapped(const Synapse& y, const BionData& B, float& rmax, float& prel, float& rrec) {
+    rmax = y.RRP_max; prel = y.release_probability; rrec = y.RRP_recovery;
+    if (y.type == 1 || y.type == 2 || !B.nuclei) return;
+    const uint32_t nid = B.segments[y.pre_segment].neuron_id;
+    if (nid >= B.num_nuclei) return;
+    const float* pl = B.nuclei[nid].protein_level;
+    float munc13 = pl[42];
+    rmax = 10.0f + munc13 * 40.0f;
+    float syt1 = pl[40];
+    prel = 0.3f + syt1 * 0.7f;
+    float cplx1 = pl[44];
+    rrec = 0.02f + munc13 * 0.08f - cplx1 * 0.01f;
+    if (rrec < 0.005f) rrec = 0.005f;
+    if (rrec > 0.2f) rrec = 0.2f;
+}
+static void release_params(const Sim& S, std::vector<float>& rmax, std::vector<float>& prel,
+                           std::vector<float>& prim0) {
+    const BionData& B = S.host;
+    const uint32_t ns = B.num_synapses;
+    rmax.resize(ns); prel.resize(ns); prim0.resize(ns);
+    for (uint32_t k = 0; k < ns; k++) {
+        const Synapse& y = B.synapses[S.slot2syn[k]];
+        float rrec;
+        if (S.protein_map) protein_mapped(y, B, rmax[k], prel[k], rrec);
+        else { rmax[k] = y.RRP_max; prel[k] = y.release_probability; rrec = y.RRP_recovery; }
+        const Segment& pre = B.segments[y.pre_segment];
+        float ck = 0.0f, cn = 0.0f;
+        for (uint32_t j = 0; j < pre.modifier_count; j++) {
+            if (pre.modifiers[j].type == 82) ck = pre.modifiers[j].state[0];
+            if (pre.modifiers[j].type == 83) cn = pre.modifiers[j].state[0];
+        }
+        float a = rrec + ck * 0.02f;
+        prim0[k] = a + cn * 0.01f;
+    }
 }
 
 static int build_device(Sim& S) {
@@ -636,9 +680,10 @@ static int build_device(Sim& S) {
     d.y_delay = S.dupload(y_delay);
     d.y_pre_area = S.dupload(y_pa);
     d.y_post_area = S.dupload(y_qa);
-    d.y_prel = S.dupload(y_prel);
     d.y_amount = S.dupload(y_amt);
     d.y_tau = S.dupload(y_tau);
+    release_params(S, y_rmax, y_prel, y_prim0);
+    d.y_prel = S.dupload(y_prel);
     d.y_rrp_max = S.dupload(y_rmax);
     d.y_prim0 = S.dupload(y_prim0);
     d.y_cleft_mask = S.dupload(y_mask);
@@ -829,6 +874,7 @@ static void xfer(Sim& S, bool to_device) {
     }
     for (uint32_t k = 0; k < ns; k++) {
         Synapse& y = B.synapses[S.slot2syn[k]];
+        if (ran && S.protein_map) protein_mapped(y, B, y.RRP_max, y.release_probability, y.RRP_recovery);
         y.weight = yw[k]; y.facilitation = yf[k]; y.depression = yd[k]; y.RRP = yr[k]; y.bouton_Ca = yb[k];
         y.spine_Ca = ys[k]; y.spine_camkii = yk[k]; y.spine_calcineurin = yc[k]; y.refractory = yrf[k];
         for (int q = 0; q < 15; q++) y.cleft_neurotransmitter[q] = ycl[(size_t)q * ns + k];
@@ -1698,6 +1744,22 @@ int bchip_set_jacobi(void* sim, double tol, int max_sweeps) {
     S.jac_max = S.jac_res ? max_sweeps : 50;
     S.d.jac_tol = S.jac_tol;
     S.d.jac_max = S.jac_max;
+    return 0;
+}
+
+int bchip_set_protein_mapping(void* sim, int on) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    S.protein_map = on != 0;
+    if (!S.loaded || !S.d.ns) return 0;
+    HIPCHK(hipSetDevice(S.device));
+    std::vector<float> rmax, prel, prim0;
+    release_params(S, rmax, prel, prim0);
+    const size_t b = sizeof(float) * rmax.size();
+    HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_rrp_max), rmax.data(), b, hipMemcpyHostToDevice, S.stream));
+    HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_prel), prel.data(), b, hipMemcpyHostToDevice, S.stream));
+    HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_prim0), prim0.data(), b, hipMemcpyHostToDevice, S.stream));
+    HIPCHK(hipStreamSynchronize(S.stream));
     return 0;
 }
 

@@ -152,6 +152,14 @@ int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every
 // backend does (src/simulation.cpp:201,234), for callers that were built against it. One-shot
 // ion and ligand injections apply on the next tick either way.
 int bchip_set_stimulus_convention(void* sim, int convention);
+// Protein -> release-parameter mapping (on != 0): every chemical synapse whose presynaptic
+// neuron has a nucleus in the loaded BionData takes RRP_max = 10 + 40 Munc13-1, release
+// probability = 0.3 + 0.7 synaptotagmin-1 and RRP_recovery = 0.02 + 0.08 Munc13-1 - 0.01
+// complexin (clamped to [0.005, 0.2]) from that nucleus's protein levels, as the CPU tick does
+// while its consolidation runs (src/simulation_host.cpp:407-414, include/kernel_dispatch.h:
+// 651-663). Nucleus state is not advanced on the device. Takes effect from the next tick; the
+// records show the mapped values once a tick has run. Returns 0, or -1 (no simulation).
+int bchip_set_protein_mapping(void* sim, int on);
 // Upper bound on the device memory bchip_adjoint may use for recorded Jacobi iterates (bytes;
 // default unbounded, i.e. half the free memory). 0 makes the reverse pass run the substeps again.
 int bchip_set_adjoint_memory(void* sim, uint64_t max_bytes);

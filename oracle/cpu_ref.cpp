@@ -1294,7 +1294,24 @@ struct Ctx {
     // step 2.5 only moves V. scale: per-connection factor on axial conductance and flux (empty: 1).
     bool cable = false;
     std::vector<float> scale;
+    // Protein -> release-parameter mapping (simulation_host.cpp:407-414), active while the
+    // reference's consolidation runs: nucleus k's protein levels set the vesicle parameters of
+    // every chemical synapse whose presynaptic segment belongs to neuron k.
+    std::vector<Nucleus> nuclei;
 };
+
+// kernel_dispatch.h:651-663: Munc13-1 (gene 42) sets RRP_max and the recovery rate, which
+// complexin (44) lowers; synaptotagmin-1 (40) the release probability.
+static void protein_map(Synapse* y, const float* protein) {
+    float munc13 = protein[42];
+    y->RRP_max = 10.0f + munc13 * 40.0f;
+    float syt1 = protein[40];
+    y->release_probability = 0.3f + syt1 * 0.7f;
+    float cplx1 = protein[44];
+    y->RRP_recovery = 0.02f + munc13 * 0.08f - cplx1 * 0.01f;
+    if (y->RRP_recovery < 0.005f) y->RRP_recovery = 0.005f;
+    if (y->RRP_recovery > 0.2f) y->RRP_recovery = 0.2f;
+}
 
 static uint32_t uf_root(std::vector<uint32_t>& p, uint32_t x) {
     while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
@@ -1737,6 +1754,8 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
             continue;
         }
         if (y->pre_autoreceptor_count > 0) g_unsupported++;
+        if (!c.nuclei.empty() && pre->neuron_id < c.nuclei.size())
+            protein_map(y, c.nuclei[pre->neuron_id].protein_level);
         rrp_recover(y, pre, dt);
         bouton_step(pre, y, dt);
         if (y->refractory == 0) {
@@ -2337,6 +2356,12 @@ void oracle_set_jacobi(void* ctx, double tol, int max_sweeps) {
 }
 
 // Cable mode and per-connection coupling scale (see orc::Ctx::cable); scale may be NULL.
+void oracle_set_protein_mapping(void* ctx, const Nucleus* nuclei, uint32_t count) {
+    orc::Ctx* c = static_cast<orc::Ctx*>(ctx);
+    if (nuclei && count) c->nuclei.assign(nuclei, nuclei + count);
+    else c->nuclei.clear();
+}
+
 void oracle_set_cable(void* ctx, int on, const float* scale, uint32_t nc) {
     orc::Ctx* c = static_cast<orc::Ctx*>(ctx);
     c->cable = on != 0;

@@ -73,6 +73,7 @@ def oracle_lib():
         lib.oracle_bernoulli.restype = C.c_float
         lib.oracle_bernoulli.argtypes = [C.c_float]
         lib.oracle_set_cable.argtypes = [P, C.c_int, P, C.c_uint32]
+        lib.oracle_set_protein_mapping.argtypes = [P, P, C.c_uint32]
         _oracle = lib
     return _oracle
 
@@ -147,6 +148,11 @@ class OracleSim:
         self._scale = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
         self.lib.oracle_set_cable(self.ctx, 1 if on else 0, _ptr(self._scale),
                                   0 if scale is None else len(self._scale))
+
+    def set_protein_mapping(self, nuclei=None):
+        """Protein -> release-parameter mapping from these nucleus records (None: off)."""
+        self._nuclei = None if nuclei is None else np.ascontiguousarray(nuclei)
+        self.lib.oracle_set_protein_mapping(self.ctx, _ptr(self._nuclei), 0 if nuclei is None else len(nuclei))
 
     def step(self, dt: float, stim: Stimuli | None = None):
         n = self.net
