@@ -92,9 +92,13 @@ CHANNELS2 = (CAV13, CLC2, KV42, KV21, NAP, NAP_TAD, KIR, NAV12DEV)
 # Receptors: further ionotropic types, and the Gi/o- (17-40) and Gs-coupled (41-51) families.
 KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B, GABAB = 10, 12, 13, 14, 97, 194, 17
 RECEPTORS2 = (KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B) + tuple(range(17, 52))
+# Gq-coupled receptors (52-62, 70), P2X (15) and P2X2 (134), the network-builder receptor
+# variants (63, 64, 67, 68, 71-74), HCN3 (131) and HCN4 (132).
+P2X, P2X2, HCN3, HCN4 = 15, 134, 131, 132
+RECEPTORS3 = tuple(range(52, 63)) + (70, P2X, P2X2, 63, 64, 67, 68, 71, 72, 73, 74, HCN3, HCN4)
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -107,7 +111,8 @@ CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, P
                  GLYT2: 1.0, GAT1: 1.0, HCN2: 120.0, PUMP_A3: 1500.0, PUMP_GLIA: 1500.0, PMCA: 300.0,
                  NCX: 30.0, KCC2: 0.093, NKCC1: 0.1, CA_BUFFER: 0.5, CAV13: 2.0, CLC2: 0.8,
                  KV42: 41.51, KV21: 2.595, NAP: 3.0, NAP_TAD: 10.6, KIR: 0.35, NAV12DEV: 60.0}
-CANON_DENSITY.update({t: 40.0 for t in RECEPTORS2})
+CANON_DENSITY.update({t: 40.0 for t in RECEPTORS2 + RECEPTORS3})
+CANON_DENSITY.update({HCN3: 120.0, HCN4: 120.0})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -463,7 +468,7 @@ def receptor_network(seed: int, n_neurons: int = 14, seg_per_neuron: int = 5) ->
     acetylcholine, serotonin, GABA and dopamine are released); somas carry Gi/o (GABA-B, 5-HT1A,
     D2, M2, mGluR2, CB1, H3, MT1) and Gs (D1, 5-HT7, beta1, A2A, H2) receptors as segment
     modifiers, dendrites glycine and GluN2B receptors."""
-    soma = [17, 18, 22, 25, 29, 36, 38, 39, 44, 43, 48, 46, 50]
+    soma = [17, 18, 22, 25, 29, 36, 38, 39, 44, 43, 48, 46, 50, 52, 55, 57, 60, 62, 70, 63, 64, 67, 74, HCN3, HCN4]
     net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21] + soma,
                         4.0, "ampa_nmda_gaba", 4, seed)
     segs = net.segments
@@ -481,9 +486,10 @@ def receptor_network(seed: int, n_neurons: int = 14, seg_per_neuron: int = 5) ->
         segs["modifier_count"][i] = k + 2
     syn = net.synapses
     pr = syn["post_receptors"]
-    kinds = [[GLYCINE], [NICOTINIC], [HT3, KAINATE], [NMDA2B], [GABAB], [44], [ALPHA7, 22]]
+    kinds = [[GLYCINE], [NICOTINIC], [HT3, KAINATE], [NMDA2B], [GABAB], [44], [ALPHA7, 22], [P2X2, 71, 72],
+             [73, 68, P2X, 57]]
     for y in range(len(syn)):
-        c = y % 9
+        c = y % 11
         if c >= len(kinds):
             continue
         ts = kinds[c]
@@ -492,7 +498,7 @@ def receptor_network(seed: int, n_neurons: int = 14, seg_per_neuron: int = 5) ->
         pr["state"][y, :, :] = 0.0
         for j, t in enumerate(ts):
             pr["type"][y, j] = t
-            pr["density"][y, j] = 300.0 if t not in (GABAB, 44, 22) else 30.0
+            pr["density"][y, j] = 300.0 if t not in (GABAB, 44, 22, 57) else 30.0
         if NMDA2B in ts:
             pr["state"][y, ts.index(NMDA2B), 2] = 1.0
         syn["post_receptor_count"][y] = len(ts)

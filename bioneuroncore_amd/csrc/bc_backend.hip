@@ -421,7 +421,7 @@ static int build_device(Sim& S) {
             if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
             if (!modifier_supported(m.type)) S.unsupported_mod++;
             // G-protein receptors read state[active_vars + 1]: past the record's state array
-            if (m.type >= 17 && m.type <= 51 && m.active_vars >= 7) S.unsupported_mod++;
+            if (gprotein_type(m.type) && m.active_vars >= 7) S.unsupported_mod++;
             // PMCA's Hill powers are exact squares only for n = 2 and a float-valued K_m
             if (m.type == M_PMCA && ((m.state[2] > 0.0f && m.state[2] != 2.0) ||
                                      (m.state[1] > 0.0f && (double)(float)m.state[1] != m.state[1])))
@@ -559,7 +559,7 @@ static int build_device(Sim& S) {
             y_rdens[(size_t)j * ns + k] = y.post_receptors[j].density;
             for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
             if (!modifier_supported(rt)) S.unsupported_mod++;
-            if (rt >= 17 && rt <= 51 && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
+            if (gprotein_type(rt) && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
             if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }
         y_flags[k] = fl;

@@ -63,11 +63,17 @@ __host__ __device__ static inline bool myelin_blocks(uint8_t t) {
            t == M_NAP || t == M_NAP_TAD || t == M_HCN2;
 }
 
+// G-protein-coupled receptor types: they read state[active_vars + 1].
+__host__ __device__ static inline bool gprotein_type(uint8_t t) {
+    return (t >= 17 && t <= 62) || t == 70 || t == 74;
+}
+
 // Modifiers that read the segment's extracellular transmitters in step 1 (the ionotropic
 // receptors) or also take them up (the transporters).
 __host__ __device__ static inline bool modifier_reads_nt(uint8_t t) {
     return t == M_AMPA || t == M_NMDA || t == M_GABAA || (t >= M_GLAST && t <= M_GAT3) ||
-           (t >= M_EAAT3 && t <= M_GAT1) || (t >= 10 && t <= 51) || t == 97 || t == 194;
+           (t >= M_EAAT3 && t <= M_GAT1) || (t >= 10 && t <= 64) || (t >= 67 && t <= 68) ||
+           (t >= 70 && t <= 74) || t == 97 || t == 134 || t == 194;
 }
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
@@ -80,10 +86,11 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case M_PUMP_A3: case M_PUMP_GLIA: case M_PMCA: case M_NCX: case M_KCC2: case M_NKCC1:
         case M_CA_BUFFER: case M_CAV13: case M_CLC2: case M_KV42: case M_KV21: case M_NAP:
         case M_NAP_TAD: case M_KIR: case M_NAV12DEV: case 10: case 12: case 13: case 14: case 97:
-        case 194:
+        case 194: case 15: case 134: case 63: case 64: case 67: case 68: case 70: case 71: case 72:
+        case 73: case 74: case 131: case 132:
             return true;
         default:
-            return t >= 17 && t <= 51;   // the Gi/o- and Gs-coupled receptor families
+            return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
     }
 }
 
@@ -568,10 +575,11 @@ BC_DEV double ch_kv7(const ModSlot& m, SegRegs& s, float dt) {   // hh_biologica
 
 // HCN1 (129: V1/2 -69 mV, tau 84 ms, 0.84 pS) and HCN2 (kHcn2, 130: -95 mV, 557 ms, 1.54 pS)
 // (modifier_helpers.h:1442-1484, hh_biological.cpp:3598-3599)
-template <bool kHcn2 = false>
+template <int kHcn = 1>   // 1-4: HCN1-HCN4 (hh_biological.cpp:3598-3601, modifier_helpers.h:1064-1079)
 BC_DEV double ch_hcn1(const ModSlot& m, SegRegs& s, float dt) {
-    constexpr float kVh = kHcn2 ? -95.0f : -69.0f, kTopen = kHcn2 ? 557.0f : 84.0f;
-    constexpr float kGamma = kHcn2 ? 1.54f : 0.84f;
+    constexpr float kVh = kHcn == 1 ? -69.0f : kHcn == 2 ? -95.0f : kHcn == 3 ? -77.0f : -100.0f;
+    constexpr float kTopen = kHcn == 1 ? 84.0f : kHcn == 2 ? 557.0f : kHcn == 3 ? 1408.0f : 3586.0f;
+    constexpr float kGamma = kHcn == 1 ? 0.84f : kHcn == 2 ? 1.54f : kHcn == 3 ? 0.54f : 0.51f;
     double V = s.V;
     float area = eff_area(m, s);
     double s1 = m.get(1), s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5);
@@ -1302,6 +1310,18 @@ BC_DEV double ch_transporter(const ModSlot& m, SegRegs& s, float dt, float km, f
 
 // Modifier dispatch (kernel_dispatch.h:337-570) over the covered IDs; unknown IDs give 0
 // current and leave their state untouched, as the reference's device switch does.
+// The network-builder NMDA variants (63, 72; receptors_biological.cpp:222-229, 312-319): NMDA
+// gating without Ca-dependent inactivation, Mg2+ block by the extracellular Mg2+ (1 mM when below
+// 0.01).
+BC_DEV double rec_nmda_mg(const ModSlot& m, SegRegs& s, float dt) {
+    const IonoK k = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
+    double I = iono(m, s, dt, k);
+    float mg = s.ex[4];
+    if (mg < 0.01f) mg = 1.0f;
+    float block = 1.0f / (1.0f + (mg / 3.57f) * bc::exp(-0.062f * s.V));
+    return I * block;
+}
+
 // Gi/o-coupled receptors (17-40, modifier_helpers.h:1332-1363): occupancy of transmitter NT
 // drives a GIRK gate (state[0]) with desensitisation (state[1]); EC50 and gamma from
 // state[active_vars] and state[active_vars + 1] when positive.
@@ -1362,7 +1382,22 @@ BC_DEV double ch_gs(const ModSlot& m, SegRegs& s, float dt) {
 BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
     switch (type) {
         case M_CX36: case M_CX43: return ch_connexin(m, s, dt);
-        case M_HCN2: return ch_hcn1<true>(m, s, dt);
+        case M_HCN2: return ch_hcn1<2>(m, s, dt);
+        case 131: return ch_hcn1<3>(m, s, dt);
+        case 132: return ch_hcn1<4>(m, s, dt);
+        case 15: return iono(m, s, dt, IonoK{15.0f, 0.5f, 1.0f, 20.0f, 10, 0.22f, 0.22f, 0.56f, false, 20.0f});
+        case 134: return iono(m, s, dt, IonoK{15.0f, 0.5f, 1.0f, 20.0f, 10, 0.22f, 0.22f, 0.56f, false, 5000.0f});
+        case 64: case 71: return iono(m, s, dt, IonoK{8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f});
+        case 73: return iono(m, s, dt, IonoK{28.0f, 1.0f, 1.0f, 5.0f, 1, 0.0f, 0.0f, 0.0f, true, 2000.0f});
+        case 67: return iono(m, s, dt, IonoK{40.0f, 0.5f, 0.5f, 3.0f, 4, 0.12f, 0.17f, 0.71f, false, 50.0f});
+        case 68: return iono(m, s, dt, IonoK{30.0f, 0.5f, 1.0f, 10.0f, 4, 0.48f, 0.48f, 0.04f, false, 2000.0f});
+        case 74: return ch_gio<1>(m, s, dt);
+        case 63: case 72: return rec_nmda_mg(m, s, dt);
+        case 52: case 53: case 54: return ch_gs<3>(m, s, dt);   // Gq: the Gs signal's form
+        case 55: case 56: return ch_gs<6>(m, s, dt);
+        case 57: case 58: case 59: case 70: return ch_gs<4>(m, s, dt);
+        case 60: case 61: return ch_gs<0>(m, s, dt);
+        case 62: return ch_gs<8>(m, s, dt);
         case 10: return iono(m, s, dt, IonoK{3.0f, 0.5f, 1.0f, 10.0f, 0, 0.5f, 0.5f, 0.0f, false, 5.0f});
         case 12: return iono(m, s, dt, IonoK{60.0f, 0.5f, 0.3f, 5.0f, 7, 0.0f, 0.0f, 0.0f, true, 2000.0f});
         case 13: return iono(m, s, dt, IonoK{30.0f, 0.5f, 1.0f, 10.0f, 4, 0.48f, 0.48f, 0.04f, false, 2000.0f});

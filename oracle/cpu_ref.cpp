@@ -569,6 +569,9 @@ static constexpr Iono kNicotinic = {30.0f, 0.5f, 1.0f, 10.0f, 4, 0.48f, 0.48f, 0
 static constexpr Iono k5ht3 = {2.0f, 0.3f, 1.0f, 10.0f, 3, 0.40f, 0.40f, 0.20f, false, 800.0f};
 static constexpr Iono kAlpha7 = {40.0f, 0.5f, 0.5f, 3.0f, 4, 0.12f, 0.17f, 0.71f, false, 50.0f};
 static constexpr Iono kNmda2b = {50.0f, 1.0f, 10.0f, 150.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
+// P2X (15) and P2X2 (134) on ATP (receptors_biological.cpp:102-111, units.h:173, 275-292).
+static constexpr Iono kP2x = {15.0f, 0.5f, 1.0f, 20.0f, 10, 0.22f, 0.22f, 0.56f, false, 20.0f};
+static constexpr Iono kP2x2 = {15.0f, 0.5f, 1.0f, 20.0f, 10, 0.22f, 0.22f, 0.56f, false, 5000.0f};
 
 static double iono_current(Modifier* m, Segment* s, float dt, const Iono& p) {
     float area = eff_area(m, s);
@@ -634,6 +637,17 @@ static double rec_nmda(Modifier* m, Segment* s, float dt, const Iono& k = kNmda)
     float mg = 1.0f / (1.0f + (1.0f / 3.57f) * exp(-0.062f * s->V_m));
     double r = I * mg;
     return r * nmda_cdi(m, s, dt);
+}
+
+// The network-builder NMDA variants (63, 72; receptors_biological.cpp:222-229, 312-319): NMDA
+// gating without the Ca-dependent inactivation, Mg2+ block by the extracellular Mg2+ (1 mM when
+// below 0.01).
+static double rec_nmda_mg(Modifier* m, Segment* s, float dt) {
+    double I = iono_current(m, s, dt, kNmda);
+    float mg = s->extra_ions[4];
+    if (mg < 0.01f) mg = 1.0f;
+    float block = 1.0f / (1.0f + (mg / 3.57f) * exp(-0.062f * s->V_m));
+    return I * block;
 }
 
 // Gi/o-coupled receptors (17-40, modifier_helpers.h:1332-1363, receptors_biological.cpp:163-186):
@@ -1235,6 +1249,16 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_5HT3: return iono_current(m, s, dt, k5ht3);
         case 97: return iono_current(m, s, dt, kAlpha7);
         case 194: return rec_nmda(m, s, dt, kNmda2b);
+        case 15: return iono_current(m, s, dt, kP2x);
+        case 134: return iono_current(m, s, dt, kP2x2);
+        case 64: case 71: return iono_current(m, s, dt, kAmpa);
+        case 73: return iono_current(m, s, dt, kGabaa);
+        case 67: return iono_current(m, s, dt, kAlpha7);
+        case 68: return iono_current(m, s, dt, kNicotinic);
+        case 74: return chan_gio(m, s, dt, 1);
+        case 63: case 72: return rec_nmda_mg(m, s, dt);
+        case 131: return chan_hcn1(m, s, dt, -77.0f, 1408.0f, 0.54f);
+        case 132: return chan_hcn1(m, s, dt, -100.0f, 3586.0f, 0.51f);
         case T_CAV13: return chan_cav13(m, s, dt);
         case T_CL: return chan_cl(m, s, dt);
         case T_KV42: return chan_kv42(m, s, dt);
@@ -1259,6 +1283,11 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         default:
             if (type >= 17 && type <= 40) return chan_gio(m, s, dt, gio_nt(type));
             if (type >= 41 && type <= 51) return chan_gs(m, s, dt, gs_nt(type));
+            // Gq-coupled (52-62 and the muscarinic 70; modifier_helpers.h:1600-1621): the Gs
+            // signal's form and constants. 5-HT2A/B/C serotonin, alpha1A/B noradrenaline,
+            // M1/M3/M5 and 70 acetylcholine, mGluR1/5 glutamate, H1 histamine.
+            if ((type >= 52 && type <= 62) || type == 70)
+                return chan_gs(m, s, dt, type <= 54 ? 3 : type <= 56 ? 6 : type <= 59 || type == 70 ? 4 : type <= 61 ? 0 : 8);
             g_unsupported++;
             return 0.0;
     }
