@@ -96,9 +96,13 @@ RECEPTORS2 = (KAINATE, GLYCINE, NICOTINIC, HT3, ALPHA7, NMDA2B) + tuple(range(17
 # variants (63, 64, 67, 68, 71-74), HCN3 (131) and HCN4 (132).
 P2X, P2X2, HCN3, HCN4 = 15, 134, 131, 132
 RECEPTORS3 = tuple(range(52, 63)) + (70, P2X, P2X2, 63, 64, 67, 68, 71, 72, 73, 74, HCN3, HCN4)
+NAV11, NAV13, NAV17, NAV18, NAV19 = 199, 177, 169, 170, 171
+NAV_VARIANTS = (NAV11, NAV13, NAV17, NAV18, NAV19)
+KV12, KV13, KV14, KV16, KV22 = 157, 154, 206, 207, 205
+KV_VARIANTS = (KV12, KV13, KV14, KV16, KV22)
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -112,7 +116,8 @@ CANON_DENSITY = {NAV12: 15.6, NAV16: 15.6, KVDR: 3.84, K2P: 4.17, NALCN: 0.11, P
                  NCX: 30.0, KCC2: 0.093, NKCC1: 0.1, CA_BUFFER: 0.5, CAV13: 2.0, CLC2: 0.8,
                  KV42: 41.51, KV21: 2.595, NAP: 3.0, NAP_TAD: 10.6, KIR: 0.35, NAV12DEV: 60.0}
 CANON_DENSITY.update({t: 40.0 for t in RECEPTORS2 + RECEPTORS3})
-CANON_DENSITY.update({HCN3: 120.0, HCN4: 120.0})
+CANON_DENSITY.update({HCN3: 120.0, HCN4: 120.0, NAV11: 8.0, NAV13: 8.0, NAV17: 8.0, NAV18: 8.0, NAV19: 2.0,
+                      KV12: 2.0, KV13: 2.0, KV14: 2.0, KV16: 2.0, KV22: 2.0})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -446,7 +451,8 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, mye
     ClC-2 leak and HCN2; AMPA+NMDA / GABA-A synapses. myelin: every third segment also carries
     myelin, which blocks the listed channel types and the ionotropic receptors."""
     net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12DEV, KV21, KV42, KV11, K2P, NALCN, PUMP],
-                        [CAV13, CAV21, NAP, NAP_TAD, KIR, CLC2, HCN2], 3.0, "ampa_nmda_gaba", 4, seed,
+                        [CAV13, CAV21, NAP, NAP_TAD, KIR, CLC2, HCN2] + list(NAV_VARIANTS) + list(KV_VARIANTS), 3.0,
+                        "ampa_nmda_gaba", 4, seed,
                         lognormal_sigma=0.2)
     mods = net.segments["modifiers"]
     for t, init in ((NAV12DEV, {0: 0.02, 1: 0.7, 3: 0.7}), (KV21, {0: 0.05, 1: 0.9}), (KV42, {0: 0.1, 1: 0.5})):

@@ -60,7 +60,8 @@ __host__ __device__ static inline bool ionotropic_type(uint8_t t) {
 __host__ __device__ static inline bool myelin_blocks(uint8_t t) {
     return ionotropic_type(t) || t == M_K2P || t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 ||
            t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK || t == M_KV42 ||
-           t == M_NAP || t == M_NAP_TAD || t == M_HCN2;
+           t == M_NAP || t == M_NAP_TAD || t == M_HCN2 || t == 131 || t == 132 || t == 154 || t == 157 ||
+           t == 205 || t == 206 || t == 207;
 }
 
 // G-protein-coupled receptor types: they read state[active_vars + 1].
@@ -87,7 +88,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case M_CA_BUFFER: case M_CAV13: case M_CLC2: case M_KV42: case M_KV21: case M_NAP:
         case M_NAP_TAD: case M_KIR: case M_NAV12DEV: case 10: case 12: case 13: case 14: case 97:
         case 194: case 15: case 134: case 63: case 64: case 67: case 68: case 70: case 71: case 72:
-        case 73: case 74: case 131: case 132:
+        case 73: case 74: case 131: case 132: case 169: case 170: case 171: case 177: case 199:
+        case 154: case 157: case 205: case 206: case 207:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1090,6 +1092,174 @@ BC_DEV double ch_nav12dev(const ModSlot& m, SegRegs& s, float dt) {
     return -ina;
 }
 
+// Nav1.3 (177), Nav1.7 (169), Nav1.8 (170), Nav1.9 (171) (hh_biological.cpp nav13/17/18/19):
+// m^3 h with Boltzmann steady states and fixed time constants, TTX block by state[5].
+struct NavSimpleK { float vm, km, vh, kh, tm, th, ttx, gamma; };
+BC_DEV double ch_nav_simple(const ModSlot& m, SegRegs& s, float dt, const NavSimpleK& g) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - g.vm) / g.km));
+    float hi = 1.0f / (1.0f + bc::exp((V - g.vh) / g.kh));
+    double g0 = m.get(0), g1 = m.get(1);
+    g0 += (mi - g0) * (dt / g.tm);
+    g1 += (hi - g1) * (dt / g.th);
+    if (g0 < 0) g0 = 0;
+    if (g0 > 1) g0 = 1;
+    if (g1 < 0) g1 = 0;
+    if (g1 > 1) g1 = 1;
+    m.set(0, g0);
+    m.set(1, g1);
+    double op = g0 * g0 * g0 * g1;
+    double s5 = m.get(5);
+    double ttx = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + ttx / g.ttx);
+    double P = p_single(g.gamma, 1.0f, kCbarNa);
+    double flux = ghk_mono(s, V, s.in[0], s.ex[0]);
+    float I = (float)(P * flux * (m.dens * area) * op * kAtoPa);
+    s.in[0] -= (I * dt) / s.cpm;
+    s.ex[0] += (I * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -I;
+}
+
+// Nav1.1 (199, hh_biological.cpp nav11_current, modifier_helpers.h:186-205): HH rates with the
+// Nav1.1 offsets, inactivation slope 5 mV, TTX block; no PKA/PKC term.
+BC_DEV double ch_nav11(const ModSlot& m, SegRegs& s, float dt) {
+    const NavK g = {30.0f, 6.5f, 45.0f, 65.0f, 65.0f};
+    float area = eff_area(m, s);
+    if (s.V < -200.0f) s.V = -200.0f;
+    if (s.V > 200.0f) s.V = 200.0f;
+    float V = s.V;
+    float am = nav_am(g, V), bm = nav_bm(g, V);
+    float minf = am / (am + bm);
+    float mtau = 1.0f / (am + bm);
+    float hinf = 1.0f / (1.0f + bc::exp((V + 65.0f) / 5.0f));
+    float htau = 1.0f / (nav_ah(g, V) + nav_bh(g, V));
+    double mv = minf + (m.get(0) - minf) * bc::exp(-dt / mtau);
+    double hv = hinf + (m.get(1) - hinf) * bc::exp(-dt / htau);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m.set(0, mv);
+    m.set(1, hv);
+    float op = mv * mv * mv * hv;
+    double s5 = m.get(5);
+    double ttx = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + ttx / 10.0f);
+    double P = p_single(17.0f, 1.0f, kCbarNa);
+    double flux = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double ina = P * flux * (m.dens * area) * op * kAtoPa;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ina;
+}
+
+// Kv1.2 (157), Kv1.3 (154), Kv1.4 (206), Kv1.6 (207), Kv2.2 (205) (hh_biological.cpp kv12/kv13/
+// kv14/kv16/kv22, modifier_helpers.h:518-558, units.h:354-364): Kv1.2 and Kv1.3 m^4 h with the
+// delayed-rectifier activation time constant (PKC block of Kv1.2; DTX/TEA and ShK toxin blocks),
+// Kv1.4 m^3 h in ten substeps with a V-dependent recovery, Kv1.6 m^3, Kv2.2 m^3 h; pore blockers
+// from the states.
+template <bool kKv12>
+BC_DEV double ch_kv1x(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    const float vh = kKv12 ? -27.0f : -30.0f, k = 6.0f, gamma = kKv12 ? 16.0f : 19.0f;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (k + 1e-6f)));
+    float tm = kdr_tau(V, vh);
+    double g0 = m.get(0), g1 = m.get(1);
+    g0 += (mi - g0) * dt / tm;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    float hi = 1.0f / (1.0f + bc::exp((V + 40.0f) / 6.0f));
+    float th = kKv12 ? 400.0f : 3000.0f;
+    g1 += (hi - g1) * dt / th;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    double op = g0 * g0 * g0 * g0 * g1;
+    if (kKv12) {
+        op *= (1.0f - 0.8f * s.pkc);
+        if (op < 0.0f) op = 0.0f;
+        double s5 = m.get(5), s6 = m.get(6);
+        double dtx = (s5 > 0.0f) ? s5 : 0.0f;
+        double tea = (s6 > 0.0f) ? s6 : 0.0f;
+        op /= (1.0f + dtx / 5.0f) * (1.0f + tea / 170.0f);
+    } else {
+        double s5 = m.get(5);
+        double shk = (s5 > 0.0f) ? s5 : 0.0f;
+        op /= (1.0f + shk / 11.0f);
+    }
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+BC_DEV double ch_kv14(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4);
+    double gamma = (s2 > 0.0f) ? s2 : 9.3f;
+    double mv = m.get(0), hv = m.get(1);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - -21.7f) / 8.0f));
+    float hi = 1.0f / (1.0f + bc::exp((V - -51.8f) / 3.2f));
+    float tm = 2.0f;
+    float th = 50.0f + (1500.0f - 50.0f) / (1.0f + bc::exp((V - -80.0f) / 10.0f));
+    float sub = dt / 10.0f;
+    for (int k = 0; k < 10; k++) {
+        mv += (mi - mv) * sub / tm;
+        hv += (hi - hv) * sub / th;
+    }
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m.set(0, mv);
+    m.set(1, hv);
+    float op = mv * mv * mv * hv;
+    double ap4 = (s3 > 0.0f) ? s3 : 0.0f;
+    double tea = (s4 > 0.0f) ? s4 : 0.0f;
+    op /= (1.0f + ap4 / 13.0f) * (1.0f + tea / 100.0f);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, s.in[1], s.ex[1]);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+template <bool kKv22>
+BC_DEV double ch_kv16_22(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4);
+    double gamma = (s2 > 0.0f) ? s2 : (kKv22 ? 14.5f : 8.2f);
+    double mv = m.get(0);
+    float mi = kKv22 ? 1.0f / (1.0f + bc::exp(-(V - 13.2f) / 9.0f)) : 1.0f / (1.0f + bc::exp(-(V - -8.1f) / 13.0f));
+    mv += (mi - mv) * dt / (kKv22 ? 32.0f : 20.0f);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m.set(0, mv);
+    float op;
+    if (kKv22) {
+        double hv = m.get(1);
+        float hi = 1.0f / (1.0f + bc::exp((V - -16.6f) / 6.0f));
+        hv += (hi - hv) * dt / 2000.0f;
+        if (hv < 0.0f) hv = 0.0f;
+        if (hv > 1.0f) hv = 1.0f;
+        m.set(1, hv);
+        op = mv * mv * mv * hv;
+    } else {
+        op = mv * mv * mv;
+    }
+    double tea = (s3 > 0.0f) ? s3 : 0.0f;
+    double ap4 = (s4 > 0.0f) ? s4 : 0.0f;
+    op /= (1.0f + tea / (kKv22 ? 2.6f : 1.7f)) * (1.0f + ap4 / (kKv22 ? 1.5f : 0.3f));
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, s.in[1], s.ex[1]);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -1427,6 +1597,16 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case M_NAP_TAD: return ch_nap<true>(m, s, dt);
         case M_KIR: return ch_kir(m, s, dt);
         case M_NAV12DEV: return ch_nav12dev(m, s, dt);
+        case 177: return ch_nav_simple(m, s, dt, NavSimpleK{-22.0f, 6.5f, -64.0f, 6.5f, 0.5f, 0.65f, 4.0f, 20.8f});
+        case 169: return ch_nav_simple(m, s, dt, NavSimpleK{-24.0f, 5.0f, -71.0f, 6.0f, 0.5f, 1.2f, 10.0f, 19.5f});
+        case 170: return ch_nav_simple(m, s, dt, NavSimpleK{-20.0f, 7.0f, -30.0f, 5.0f, 1.5f, 8.0f, 60000.0f, 20.0f});
+        case 171: return ch_nav_simple(m, s, dt, NavSimpleK{-47.0f, 5.5f, -44.0f, 6.0f, 10.0f, 43.0f, 39000.0f, 16.0f});
+        case 199: return ch_nav11(m, s, dt);
+        case 157: return ch_kv1x<true>(m, s, dt);
+        case 154: return ch_kv1x<false>(m, s, dt);
+        case 206: return ch_kv14(m, s, dt);
+        case 207: return ch_kv16_22<false>(m, s, dt);
+        case 205: return ch_kv16_22<true>(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

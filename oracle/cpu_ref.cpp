@@ -1002,6 +1002,172 @@ static double chan_nav12dev(Modifier* m, Segment* s, float dt) {
     return -ina;
 }
 
+// Nav1.3 (177), Nav1.7 (169), Nav1.8 (170), Nav1.9 (171) (hh_biological.cpp nav13/17/18/19):
+// m^3 h with Boltzmann steady states and fixed time constants, TTX block by state[5].
+struct NavSimple { float vm, km, vh, kh, tm, th, ttx, gamma; };
+static constexpr NavSimple kNav13 = {-22.0f, 6.5f, -64.0f, 6.5f, 0.5f, 0.65f, 4.0f, 20.8f};
+static constexpr NavSimple kNav17 = {-24.0f, 5.0f, -71.0f, 6.0f, 0.5f, 1.2f, 10.0f, 19.5f};
+static constexpr NavSimple kNav18 = {-20.0f, 7.0f, -30.0f, 5.0f, 1.5f, 8.0f, 60000.0f, 20.0f};
+static constexpr NavSimple kNav19 = {-47.0f, 5.5f, -44.0f, 6.0f, 10.0f, 43.0f, 39000.0f, 16.0f};
+static double chan_nav_simple(Modifier* m, Segment* s, float dt, const NavSimple& g) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + exp(-(V - g.vm) / g.km));
+    float hi = 1.0f / (1.0f + exp((V - g.vh) / g.kh));
+    m->state[0] += (mi - m->state[0]) * (dt / g.tm);
+    m->state[1] += (hi - m->state[1]) * (dt / g.th);
+    if (m->state[0] < 0) m->state[0] = 0;
+    if (m->state[0] > 1) m->state[0] = 1;
+    if (m->state[1] < 0) m->state[1] = 0;
+    if (m->state[1] > 1) m->state[1] = 1;
+    double op = m->state[0] * m->state[0] * m->state[0] * m->state[1];
+    double ttx = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + ttx / g.ttx);
+    double P = p_single(g.gamma, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double ia = P * flux * (m->density * area) * op;
+    float I = (float)(ia * kAtoPa);
+    double cpv = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[0] -= (I * dt) / cpv;
+    s->extra_ions[0] += (I * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -I;
+}
+
+// Nav1.1 (199, hh_biological.cpp nav11_current, modifier_helpers.h:186-205): HH rates with the
+// Nav1.1 offsets, inactivation slope 5 mV, TTX block; no PKA/PKC term.
+static double chan_nav11(Modifier* m, Segment* s, float dt) {
+    static constexpr NavGates k11 = {30.0f, 6.5f, 45.0f, 65.0f, 65.0f};
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    if (s->V_m < -200.0f) s->V_m = -200.0f;
+    if (s->V_m > 200.0f) s->V_m = 200.0f;
+    V = s->V_m;
+    float hinf = 1.0f / (1.0f + exp((V + 65.0f) / 5.0f));
+    double mv = nav_minf(k11, V) + (m->state[0] - nav_minf(k11, V)) * exp(-dt / nav_mtau(k11, V));
+    double hv = hinf + (m->state[1] - hinf) * exp(-dt / nav_htau(k11, V));
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m->state[0] = mv;
+    m->state[1] = hv;
+    float op = mv * mv * mv * hv;
+    double ttx = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + ttx / 10.0f);
+    double P = p_single(17.0f, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double ia = P * flux * (m->density * area) * op;
+    double ina = ia * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ina;
+}
+
+// Kv1.2 (157), Kv1.3 (154), Kv1.4 (206), Kv1.6 (207), Kv2.2 (205) (hh_biological.cpp kv12/kv13/
+// kv14/kv16/kv22, modifier_helpers.h:518-558, units.h:354-364): Kv1.2 and Kv1.3 m^4 h with the
+// delayed-rectifier activation time constant (PKC block of Kv1.2; DTX/TEA and ShK toxin blocks),
+// Kv1.4 m^3 h in ten substeps with a V-dependent recovery, Kv1.6 m^3, Kv2.2 m^3 h; pore blockers
+// from the states.
+static double k_efflux(Segment* s, double ik, float dt) {
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    return -ik;
+}
+static double chan_kv1x(Modifier* m, Segment* s, float dt, bool kv12) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    const float vh = kv12 ? -27.0f : -30.0f, k = 6.0f, gamma = kv12 ? 16.0f : 19.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (k + 1e-6f)));
+    float tm = kdr_tau(V, vh);
+    m->state[0] += (mi - m->state[0]) * dt / tm;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float hi = 1.0f / (1.0f + exp((V + 40.0f) / 6.0f));
+    float th = kv12 ? 400.0f : 3000.0f;
+    m->state[1] += (hi - m->state[1]) * dt / th;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    double op = m->state[0] * m->state[0] * m->state[0] * m->state[0] * m->state[1];
+    if (kv12) {
+        float pkc = first_state_of(s, T_PKC, 0, 0.0f);
+        op *= (1.0f - 0.8f * pkc);
+        if (op < 0.0f) op = 0.0f;
+        double dtx = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        double tea = (m->state[6] > 0.0f) ? m->state[6] : 0.0f;
+        op /= (1.0f + dtx / 5.0f) * (1.0f + tea / 170.0f);
+    } else {
+        double shk = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        op /= (1.0f + shk / 11.0f);
+    }
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    return k_efflux(s, P * flux * (m->density * area) * op * kAtoPa, dt);
+}
+static double chan_kv14(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 9.3f;
+    double mv = m->state[0], hv = m->state[1];
+    float mi = 1.0f / (1.0f + exp(-(V - -21.7f) / 8.0f));
+    float hi = 1.0f / (1.0f + exp((V - -51.8f) / 3.2f));
+    float tm = 2.0f;
+    float th = 50.0f + (1500.0f - 50.0f) / (1.0f + exp((V - -80.0f) / 10.0f));
+    float sub = dt / 10.0f;
+    for (int k = 0; k < 10; k++) {
+        mv += (mi - mv) * sub / tm;
+        hv += (hi - hv) * sub / th;
+    }
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m->state[0] = mv;
+    m->state[1] = hv;
+    float op = mv * mv * mv * hv;
+    double ap4 = (m->state[3] > 0.0f) ? m->state[3] : 0.0f;
+    double tea = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    op /= (1.0f + ap4 / 13.0f) * (1.0f + tea / 100.0f);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    return k_efflux(s, P * flux * (m->density * area) * op * kAtoPa, dt);
+}
+static double chan_kv16_22(Modifier* m, Segment* s, float dt, bool kv22) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : (kv22 ? 14.5f : 8.2f);
+    double mv = m->state[0], hv = m->state[1];
+    float mi = kv22 ? 1.0f / (1.0f + exp(-(V - 13.2f) / 9.0f)) : 1.0f / (1.0f + exp(-(V - -8.1f) / 13.0f));
+    mv += (mi - mv) * dt / (kv22 ? 32.0f : 20.0f);
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    m->state[0] = mv;
+    float op;
+    if (kv22) {
+        float hi = 1.0f / (1.0f + exp((V - -16.6f) / 6.0f));
+        hv += (hi - hv) * dt / 2000.0f;
+        if (hv < 0.0f) hv = 0.0f;
+        if (hv > 1.0f) hv = 1.0f;
+        m->state[1] = hv;
+        op = mv * mv * mv * hv;
+    } else {
+        op = mv * mv * mv;
+    }
+    double tea = (m->state[3] > 0.0f) ? m->state[3] : 0.0f;
+    double ap4 = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    op /= (1.0f + tea / (kv22 ? 2.6f : 1.7f)) * (1.0f + ap4 / (kv22 ? 1.5f : 0.3f));
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    return k_efflux(s, P * flux * (m->density * area) * op * kAtoPa, dt);
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1267,6 +1433,16 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_NAP_TAD: return chan_nap(m, s, dt, true);
         case T_KIR: return chan_kir(m, s, dt);
         case T_NAV12DEV: return chan_nav12dev(m, s, dt);
+        case 177: return chan_nav_simple(m, s, dt, kNav13);
+        case 169: return chan_nav_simple(m, s, dt, kNav17);
+        case 170: return chan_nav_simple(m, s, dt, kNav18);
+        case 171: return chan_nav_simple(m, s, dt, kNav19);
+        case 199: return chan_nav11(m, s, dt);
+        case 157: return chan_kv1x(m, s, dt, true);
+        case 154: return chan_kv1x(m, s, dt, false);
+        case 206: return chan_kv14(m, s, dt);
+        case 207: return chan_kv16_22(m, s, dt, false);
+        case 205: return chan_kv16_22(m, s, dt, true);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
@@ -1920,7 +2096,9 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
             if (my && t != T_MYELIN && (is_ionotropic(t) || t == T_K2P || t == T_NALCN || t == T_KV31 ||
                                          t == T_KV11 || t == T_KV7 || t == T_KVDR || t == T_CAV21 ||
                                          t == T_HCN1 || t == T_SK || t == T_BK || t == T_KV42 ||
-                                         t == T_NAP || t == T_NAP_TAD || t == T_HCN2))
+                                         t == T_NAP || t == T_NAP_TAD || t == T_HCN2 || t == 131 ||
+                                         t == 132 || t == 154 || t == 157 || t == 205 || t == 206 ||
+                                         t == 207))
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;
