@@ -61,7 +61,7 @@ __host__ __device__ static inline bool myelin_blocks(uint8_t t) {
     return ionotropic_type(t) || t == M_K2P || t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 ||
            t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK || t == M_KV42 ||
            t == M_NAP || t == M_NAP_TAD || t == M_HCN2 || t == 131 || t == 132 || t == 154 || t == 157 ||
-           t == 205 || t == 206 || t == 207;
+           t == 205 || t == 206 || t == 207 || t == 202 || t == 203 || t == 204;
 }
 
 // G-protein-coupled receptor types: they read state[active_vars + 1].
@@ -89,7 +89,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case M_NAP_TAD: case M_KIR: case M_NAV12DEV: case 10: case 12: case 13: case 14: case 97:
         case 194: case 15: case 134: case 63: case 64: case 67: case 68: case 70: case 71: case 72:
         case 73: case 74: case 131: case 132: case 169: case 170: case 171: case 177: case 199:
-        case 154: case 157: case 205: case 206: case 207:
+        case 154: case 157: case 205: case 206: case 207: case 202: case 203: case 204: case 160:
+        case 161: case 166: case 167: case 208: case 178: case 176:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1260,6 +1261,127 @@ BC_DEV double ch_kv16_22(const ModSlot& m, SegRegs& s, float dt) {
     return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
 }
 
+// Kv3.2 (203), Kv3.3 (204), Kv4.3 (202) (hh_biological.cpp kv32/kv33/kv43, modifier_helpers.h:
+// 475-515): m^3 (h) with Boltzmann steady states, Kv4.3 in ten substeps with a V-dependent
+// recovery; two pore blockers from state[3] and state[4].
+struct KvGenK { float vm, km, vh, kh, tm, th, gamma, ic3, ic4; bool inact, sub; };
+BC_DEV double ch_kvgen(const ModSlot& m, SegRegs& s, float dt, const KvGenK& g) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4);
+    double gamma = (s2 > 0.0f) ? s2 : g.gamma;
+    double mv = m.get(0), hv = m.get(1);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - g.vm) / g.km));
+    float op;
+    if (!g.inact) {
+        mv += (mi - mv) * dt / g.tm;
+        if (mv < 0.0f) mv = 0.0f;
+        if (mv > 1.0f) mv = 1.0f;
+        m.set(0, mv);
+        op = mv * mv * mv;
+    } else {
+        float hi = 1.0f / (1.0f + bc::exp((V - g.vh) / g.kh));
+        if (g.sub) {
+            float th = 27.6f + (142.0f - 27.6f) / (1.0f + bc::exp((V - -60.0f) / 8.0f));
+            float sub = dt / 10.0f;
+            for (int k = 0; k < 10; k++) {
+                mv += (mi - mv) * sub / g.tm;
+                hv += (hi - hv) * sub / th;
+            }
+        } else {
+            mv += (mi - mv) * dt / g.tm;
+            hv += (hi - hv) * dt / g.th;
+        }
+        if (mv < 0.0f) mv = 0.0f;
+        if (mv > 1.0f) mv = 1.0f;
+        if (hv < 0.0f) hv = 0.0f;
+        if (hv > 1.0f) hv = 1.0f;
+        m.set(0, mv);
+        m.set(1, hv);
+        op = mv * mv * mv * hv;
+    }
+    double b3 = (s3 > 0.0f) ? s3 : 0.0f;
+    double b4 = (s4 > 0.0f) ? s4 : 0.0f;
+    op /= (1.0f + b3 / g.ic3) * (1.0f + b4 / g.ic4);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, s.in[1], s.ex[1]);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+
+// Leak-like K+ channels (hh_biological.cpp girk/katp/thik1/thik2/kir22): GIRK (160) open as
+// G-beta-gamma state[4], tertiapin block state[5]; KATP (161) closed by ATP state[4],
+// glibenclamide state[5]; THIK-1 (166) Po 0.5 with halothane block state[5]; THIK-2 (167)
+// Po 0.5; Kir2.2 (208) Woodhull block with Ba2+ state[4].
+template <int T>
+BC_DEV double ch_kleak(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    float po;
+    double gamma;
+    if (T == 160) {
+        double gbg = m.get(4);
+        if (gbg < 0.0f) gbg = 0.0f;
+        if (gbg > 1.0f) gbg = 1.0f;
+        po = gbg;
+        double s5 = m.get(5);
+        double tq = (s5 > 0.0f) ? s5 : 0.0f;
+        po /= (1.0f + tq / 8.0f);
+        gamma = 35.0f;
+    } else if (T == 161) {
+        double atp = m.get(4);
+        if (atp < 0.0f) atp = 0.0f;
+        po = 1.0f / (1.0f + atp / 15.0f);
+        double s5 = m.get(5);
+        double gl = (s5 > 0.0f) ? s5 : 0.0f;
+        po /= (1.0f + gl / 3.0f);
+        gamma = 75.0f;
+    } else if (T == 166 || T == 167) {
+        po = 0.5f;
+        if (T == 166) {
+            double s5 = m.get(5);
+            double hal = (s5 > 0.0f) ? s5 : 0.0f;
+            po /= (1.0f + hal / 2.8f);
+        }
+        gamma = 5.0f;
+    } else {
+        double s3 = m.get(3), s4 = m.get(4);
+        gamma = (s3 > 0.0f) ? s3 : 36.0f;
+        po = kir_f_open(V);
+        double ba = (s4 > 0.0f) ? s4 : 0.0f;
+        po /= (1.0f + ba / 2.8f);
+    }
+    double P = p_single(gamma, 1.0f, kCbarK);
+    // Kir2.2 passes the concentrations rounded to float
+    double flux = (T == 208) ? ghk_mono(s, V, (float)s.in[1], (float)s.ex[1]) : ghk_mono(s, V, s.in[1], s.ex[1]);
+    return k_out(s, P * flux * (m.dens * area) * po * kAtoPa, dt);
+}
+
+// Kir7.1 (178, always open, 0.05 pS) and KCNQ4 (kKcnq4, 176: m with tau 212 ms, 2.31 pS)
+// (hh_biological.cpp kir71/kcnq4): float current, both ion arrays floored.
+template <bool kKcnq4>
+BC_DEV double ch_kir71_kcnq4(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double op = 1.0f;
+    if (kKcnq4) {
+        float mi = 1.0f / (1.0f + bc::exp(-(V - (-78.0f)) / 14.0f));
+        double g0 = m.get(0);
+        g0 += (mi - g0) * (dt / 212.0f);
+        if (g0 < 0) g0 = 0;
+        if (g0 > 1) g0 = 1;
+        m.set(0, g0);
+        op = g0;
+    }
+    double P = p_single(kKcnq4 ? 2.31f : 0.05f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, s.in[1], s.ex[1]);
+    float I = (float)(P * flux * (m.dens * area) * op * kAtoPa);
+    s.in[1] -= (I * dt) / s.cpm;
+    s.ex[1] += (I * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -I;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -1607,6 +1729,16 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 206: return ch_kv14(m, s, dt);
         case 207: return ch_kv16_22<false>(m, s, dt);
         case 205: return ch_kv16_22<true>(m, s, dt);
+        case 203: return ch_kvgen(m, s, dt, KvGenK{10.0f, 8.0f, 0.0f, 1.0f, 12.0f, 0.0f, 15.0f, 63.0f, 25.0f, false, false});
+        case 204: return ch_kvgen(m, s, dt, KvGenK{1.5f, 8.0f, -27.9f, 8.0f, 4.0f, 200.0f, 35.0f, 67.5f, 100.0f, true, false});
+        case 202: return ch_kvgen(m, s, dt, KvGenK{-31.7f, 8.0f, -47.4f, 5.9f, 1.7f, 0.0f, 4.4f, 488.0f, 81.0f, true, true});
+        case 160: return ch_kleak<160>(m, s, dt);
+        case 161: return ch_kleak<161>(m, s, dt);
+        case 166: return ch_kleak<166>(m, s, dt);
+        case 167: return ch_kleak<167>(m, s, dt);
+        case 208: return ch_kleak<208>(m, s, dt);
+        case 178: return ch_kir71_kcnq4<false>(m, s, dt);
+        case 176: return ch_kir71_kcnq4<true>(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

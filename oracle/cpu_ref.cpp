@@ -1168,6 +1168,123 @@ static double chan_kv16_22(Modifier* m, Segment* s, float dt, bool kv22) {
     return k_efflux(s, P * flux * (m->density * area) * op * kAtoPa, dt);
 }
 
+// Kv3.2 (203), Kv3.3 (204), Kv4.3 (202) (hh_biological.cpp kv32/kv33/kv43, modifier_helpers.h:
+// 475-515): m^3 (h) with Boltzmann steady states, Kv4.3 in ten substeps with a V-dependent
+// recovery; two pore blockers from state[3] and state[4].
+struct KvGen { float vm, km, vh, kh, tm, th, gamma, ic3, ic4; bool inact, sub; };
+static constexpr KvGen kKv32 = {10.0f, 8.0f, 0.0f, 1.0f, 12.0f, 0.0f, 15.0f, 63.0f, 25.0f, false, false};
+static constexpr KvGen kKv33 = {1.5f, 8.0f, -27.9f, 8.0f, 4.0f, 200.0f, 35.0f, 67.5f, 100.0f, true, false};
+static constexpr KvGen kKv43 = {-31.7f, 8.0f, -47.4f, 5.9f, 1.7f, 0.0f, 4.4f, 488.0f, 81.0f, true, true};
+static double chan_kvgen(Modifier* m, Segment* s, float dt, const KvGen& g) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : g.gamma;
+    double mv = m->state[0], hv = m->state[1];
+    float mi = 1.0f / (1.0f + exp(-(V - g.vm) / g.km));
+    float op;
+    if (!g.inact) {
+        mv += (mi - mv) * dt / g.tm;
+        if (mv < 0.0f) mv = 0.0f;
+        if (mv > 1.0f) mv = 1.0f;
+        m->state[0] = mv;
+        op = mv * mv * mv;
+    } else {
+        float hi = 1.0f / (1.0f + exp((V - g.vh) / g.kh));
+        if (g.sub) {
+            float th = 27.6f + (142.0f - 27.6f) / (1.0f + exp((V - -60.0f) / 8.0f));
+            float sub = dt / 10.0f;
+            for (int k = 0; k < 10; k++) {
+                mv += (mi - mv) * sub / g.tm;
+                hv += (hi - hv) * sub / th;
+            }
+        } else {
+            mv += (mi - mv) * dt / g.tm;
+            hv += (hi - hv) * dt / g.th;
+        }
+        if (mv < 0.0f) mv = 0.0f;
+        if (mv > 1.0f) mv = 1.0f;
+        if (hv < 0.0f) hv = 0.0f;
+        if (hv > 1.0f) hv = 1.0f;
+        m->state[0] = mv;
+        m->state[1] = hv;
+        op = mv * mv * mv * hv;
+    }
+    double b3 = (m->state[3] > 0.0f) ? m->state[3] : 0.0f;
+    double b4 = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    op /= (1.0f + b3 / g.ic3) * (1.0f + b4 / g.ic4);
+    double P = p_single(gamma, 1.0f, kCbarK);
+    double flux = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    return k_efflux(s, P * flux * (m->density * area) * op * kAtoPa, dt);
+}
+
+// Leak-like K+ channels (hh_biological.cpp girk/katp/thik1/thik2/kir22): GIRK (160) open as
+// G-beta-gamma state[4], tertiapin block state[5]; KATP (161) closed by ATP state[4],
+// glibenclamide state[5]; THIK-1 (166) Po 0.5 with halothane block state[5]; THIK-2 (167)
+// Po 0.5; Kir2.2 (208) Woodhull block with Ba2+ state[4].
+static double chan_kleak(Modifier* m, Segment* s, float dt, uint8_t t) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float po;
+    double gamma;
+    if (t == 160) {
+        double gbg = m->state[4];
+        if (gbg < 0.0f) gbg = 0.0f;
+        if (gbg > 1.0f) gbg = 1.0f;
+        po = gbg;
+        double tq = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        po /= (1.0f + tq / 8.0f);
+        gamma = 35.0f;
+    } else if (t == 161) {
+        double atp = m->state[4];
+        if (atp < 0.0f) atp = 0.0f;
+        po = 1.0f / (1.0f + atp / 15.0f);
+        double gl = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        po /= (1.0f + gl / 3.0f);
+        gamma = 75.0f;
+    } else if (t == 166 || t == 167) {
+        po = 0.5f;
+        if (t == 166) {
+            double hal = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+            po /= (1.0f + hal / 2.8f);
+        }
+        gamma = 5.0f;
+    } else {   // 208
+        gamma = (m->state[3] > 0.0f) ? m->state[3] : 36.0f;
+        po = kir_f_open(V);
+        double ba = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+        po /= (1.0f + ba / 2.8f);
+    }
+    double P = p_single(gamma, 1.0f, kCbarK);
+    // Kir2.2 passes the concentrations rounded to float
+    double flux = (t == 208) ? ghk_mono(V, (float)s->intra_ions[1], (float)s->extra_ions[1])
+                             : ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    return k_efflux(s, P * flux * (m->density * area) * po * kAtoPa, dt);
+}
+
+// Kir7.1 (178, always open, 0.05 pS) and KCNQ4 (176, m with tau 212 ms, 2.31 pS)
+// (hh_biological.cpp kir71/kcnq4): float current, both ion arrays floored.
+static double chan_kir71_kcnq4(Modifier* m, Segment* s, float dt, bool kcnq4) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double op = 1.0f;
+    if (kcnq4) {
+        float mi = 1.0f / (1.0f + exp(-(V - (-78.0f)) / 14.0f));
+        m->state[0] += (mi - m->state[0]) * (dt / 212.0f);
+        if (m->state[0] < 0) m->state[0] = 0;
+        if (m->state[0] > 1) m->state[0] = 1;
+        op = m->state[0];
+    }
+    double P = p_single(kcnq4 ? 2.31f : 0.05f, 1.0f, kCbarK);
+    double flux = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    float I = (float)(P * flux * (m->density * area) * op * kAtoPa);
+    double cpv = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[1] -= (I * dt) / cpv;
+    s->extra_ions[1] += (I * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -I;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1443,6 +1560,12 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 206: return chan_kv14(m, s, dt);
         case 207: return chan_kv16_22(m, s, dt, false);
         case 205: return chan_kv16_22(m, s, dt, true);
+        case 203: return chan_kvgen(m, s, dt, kKv32);
+        case 204: return chan_kvgen(m, s, dt, kKv33);
+        case 202: return chan_kvgen(m, s, dt, kKv43);
+        case 160: case 161: case 166: case 167: case 208: return chan_kleak(m, s, dt, type);
+        case 178: return chan_kir71_kcnq4(m, s, dt, false);
+        case 176: return chan_kir71_kcnq4(m, s, dt, true);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
@@ -2098,7 +2221,7 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                                          t == T_HCN1 || t == T_SK || t == T_BK || t == T_KV42 ||
                                          t == T_NAP || t == T_NAP_TAD || t == T_HCN2 || t == 131 ||
                                          t == 132 || t == 154 || t == 157 || t == 205 || t == 206 ||
-                                         t == 207))
+                                         t == 207 || t == 202 || t == 203 || t == 204))
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;
