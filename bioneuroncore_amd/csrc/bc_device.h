@@ -61,7 +61,8 @@ __host__ __device__ static inline bool myelin_blocks(uint8_t t) {
     return ionotropic_type(t) || t == M_K2P || t == M_NALCN || t == M_KV31 || t == M_KV11 || t == M_KV7 ||
            t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK || t == M_KV42 ||
            t == M_NAP || t == M_NAP_TAD || t == M_HCN2 || t == 131 || t == 132 || t == 154 || t == 157 ||
-           t == 205 || t == 206 || t == 207 || t == 202 || t == 203 || t == 204;
+           t == 205 || t == 206 || t == 207 || t == 202 || t == 203 || t == 204 || t == 168 || t == 200 ||
+           t == 201 || t == 191;
 }
 
 // G-protein-coupled receptor types: they read state[active_vars + 1].
@@ -90,7 +91,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 194: case 15: case 134: case 63: case 64: case 67: case 68: case 70: case 71: case 72:
         case 73: case 74: case 131: case 132: case 169: case 170: case 171: case 177: case 199:
         case 154: case 157: case 205: case 206: case 207: case 202: case 203: case 204: case 160:
-        case 161: case 166: case 167: case 208: case 178: case 176:
+        case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
+        case 191:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1382,6 +1384,78 @@ BC_DEV double ch_kir71_kcnq4(const ModSlot& m, SegRegs& s, float dt) {
     return -I;
 }
 
+// Cav2.2 N-type (168, hh_biological.cpp cav22_current): m^2 h, omega-conotoxin GVIA block
+// state[5]; the intracellular Ca changes by the full current (no factor 2).
+BC_DEV double ch_cav22(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double g0 = m.get(0), g1 = m.get(1), s5 = m.get(5);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - -20.0f) / (6.0f + 1e-6f)));
+    g0 += (mi - g0) * dt / 1.7f;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    float hi = 1.0f / (1.0f + bc::exp((V + 60.0f) / 6.0f));
+    g1 += (hi - g1) * dt / 150.0f;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    double op = g0 * g0 * g1;
+    double gvia = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + gvia / 0.5f);
+    double P = p_single(2.76f, 2.0f, kCbarCa);
+    double flux = ghk_di(s, V, s.in[2], s.ex[2]);
+    double ica = P * flux * (m.dens * area) * op * kAtoPa;
+    s.in[2] -= (ica * dt) / s.cpm;
+    s.ex[2] += (ica * dt) / (s.cpm * kEvr);
+    floor7(s.ex);
+    floor7(s.in);
+    return -ica;
+}
+
+// T-type Cav3.1 (200), Cav3.2 (201), Cav3.3 (191) (hh_biological.cpp:2033-2143, modifier_helpers.h:
+// 649-737, 1032-1036): m^3 h in ten substeps, a bell-shaped activation time constant and a
+// V-dependent recovery, Ni2+/mibefradil/TTA-P2/Z944 blocks from states 3-6; the intracellular Ca
+// changes with a float charge-per-mole factor and is not floored.
+struct CavTK { float vm, km, vh, kh, gamma, thf, ths, tmp, ni, mib; };
+BC_DEV double ch_cavt(const ModSlot& m, SegRegs& s, float dt, const CavTK& g) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4), s5 = m.get(5), s6 = m.get(6);
+    double gamma = (s2 > 0.0f) ? s2 : g.gamma;
+    double mv = m.get(0), hv = m.get(1);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - g.vm) / g.km));
+    float hi = 1.0f / (1.0f + bc::exp((V - g.vh) / g.kh));
+    float dv = V - -20.0f;
+    float gg = bc::exp(-(dv * dv) / (2.0f * 22.0f * 22.0f));
+    float tm = 1.25f + (g.tmp - 1.25f) * gg;
+    float th = g.thf + (g.ths - g.thf) / (1.0f + bc::exp((V - -65.0f) / 8.0f));
+    float sub = dt / 10.0f;
+    for (int k = 0; k < 10; k++) {
+        mv += (mi - mv) * sub / tm;
+        hv += (hi - hv) * sub / th;
+    }
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m.set(0, mv);
+    m.set(1, hv);
+    float op = mv * mv * mv * hv;
+    double ni = (s3 > 0.0f) ? s3 : 0.0f;
+    double mib = (s4 > 0.0f) ? s4 : 0.0f;
+    double tta = (s5 > 0.0f) ? s5 : 0.0f;
+    double z = (s6 > 0.0f) ? s6 : 0.0f;
+    op /= (1.0f + ni / g.ni) * (1.0f + mib / g.mib) * (1.0f + tta / 82.0f) * (1.0f + z / 100.0f);
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double flux = ghk_di(s, V, s.in[2], s.ex[2]);
+    double ica = P * flux * (m.dens * area) * op * kAtoPa;
+    float vol = s.vol;
+    float cpm = kFaraday * vol / 1000.0;
+    s.in[2] -= (ica * dt) / (2.0f * cpm);
+    return -ica;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -1739,6 +1813,10 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 208: return ch_kleak<208>(m, s, dt);
         case 178: return ch_kir71_kcnq4<false>(m, s, dt);
         case 176: return ch_kir71_kcnq4<true>(m, s, dt);
+        case 168: return ch_cav22(m, s, dt);
+        case 200: return ch_cavt(m, s, dt, CavTK{-45.0f, 5.5f, -77.0f, 6.0f, 7.5f, 25.0f, 250.0f, 15.0f, 250.0f, 1.6f});
+        case 201: return ch_cavt(m, s, dt, CavTK{-46.0f, 6.0f, -72.0f, 6.0f, 9.0f, 16.0f, 180.0f, 15.0f, 13.0f, 3.0f});
+        case 191: return ch_cavt(m, s, dt, CavTK{-43.8f, 6.9f, -72.5f, 6.6f, 11.0f, 97.0f, 400.0f, 30.0f, 200.0f, 1.6f});
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

@@ -1285,6 +1285,78 @@ static double chan_kir71_kcnq4(Modifier* m, Segment* s, float dt, bool kcnq4) {
     return -I;
 }
 
+// Cav2.2 N-type (168, hh_biological.cpp cav22_current): m^2 h, omega-conotoxin GVIA block
+// state[5]; the intracellular Ca changes by the full current (no factor 2).
+static double chan_cav22(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + exp(-(V - -20.0f) / (6.0f + 1e-6f)));
+    m->state[0] += (mi - m->state[0]) * dt / 1.7f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float hi = 1.0f / (1.0f + exp((V + 60.0f) / 6.0f));
+    m->state[1] += (hi - m->state[1]) * dt / 150.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    double op = m->state[0] * m->state[0] * m->state[1];
+    double gvia = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + gvia / 0.5f);
+    double P = p_single(2.76f, 2.0f, kCbarCa);
+    double flux = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double ica = P * flux * (m->density * area) * op * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[2] -= (ica * dt) / cpv;
+    s->extra_ions[2] += (ica * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    floor_ions(s->intra_ions);
+    return -ica;
+}
+
+// T-type Cav3.1 (200), Cav3.2 (201), Cav3.3 (191) (hh_biological.cpp:2033-2143, modifier_helpers.h:
+// 649-737, 1032-1036): m^3 h in ten substeps, a bell-shaped activation time constant and a
+// V-dependent recovery, Ni2+/mibefradil/TTA-P2/Z944 blocks from states 3-6; the intracellular Ca
+// changes with a float charge-per-mole factor and is not floored.
+struct CavT { float vm, km, vh, kh, gamma, thf, ths, tmp, ni, mib; };
+static constexpr CavT kCav31 = {-45.0f, 5.5f, -77.0f, 6.0f, 7.5f, 25.0f, 250.0f, 15.0f, 250.0f, 1.6f};
+static constexpr CavT kCav32 = {-46.0f, 6.0f, -72.0f, 6.0f, 9.0f, 16.0f, 180.0f, 15.0f, 13.0f, 3.0f};
+static constexpr CavT kCav33 = {-43.8f, 6.9f, -72.5f, 6.6f, 11.0f, 97.0f, 400.0f, 30.0f, 200.0f, 1.6f};
+static double chan_cavt(Modifier* m, Segment* s, float dt, const CavT& g) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : g.gamma;
+    double mv = m->state[0], hv = m->state[1];
+    float mi = 1.0f / (1.0f + exp(-(V - g.vm) / g.km));
+    float hi = 1.0f / (1.0f + exp((V - g.vh) / g.kh));
+    float dv = V - -20.0f;
+    float gg = exp(-(dv * dv) / (2.0f * 22.0f * 22.0f));
+    float tm = 1.25f + (g.tmp - 1.25f) * gg;
+    float th = g.thf + (g.ths - g.thf) / (1.0f + exp((V - -65.0f) / 8.0f));
+    float sub = dt / 10.0f;
+    for (int k = 0; k < 10; k++) {
+        mv += (mi - mv) * sub / tm;
+        hv += (hi - hv) * sub / th;
+    }
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m->state[0] = mv;
+    m->state[1] = hv;
+    float op = mv * mv * mv * hv;
+    double ni = (m->state[3] > 0.0f) ? m->state[3] : 0.0f;
+    double mib = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    double tta = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    double z = (m->state[6] > 0.0f) ? m->state[6] : 0.0f;
+    op /= (1.0f + ni / g.ni) * (1.0f + mib / g.mib) * (1.0f + tta / 82.0f) * (1.0f + z / 100.0f);
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double flux = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double ica = P * flux * (m->density * area) * op * kAtoPa;
+    float vol = volume(s);
+    float cpm = kFaraday * vol / 1000.0;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    return -ica;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1566,6 +1638,10 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 160: case 161: case 166: case 167: case 208: return chan_kleak(m, s, dt, type);
         case 178: return chan_kir71_kcnq4(m, s, dt, false);
         case 176: return chan_kir71_kcnq4(m, s, dt, true);
+        case 168: return chan_cav22(m, s, dt);
+        case 200: return chan_cavt(m, s, dt, kCav31);
+        case 201: return chan_cavt(m, s, dt, kCav32);
+        case 191: return chan_cavt(m, s, dt, kCav33);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
@@ -2221,7 +2297,8 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                                          t == T_HCN1 || t == T_SK || t == T_BK || t == T_KV42 ||
                                          t == T_NAP || t == T_NAP_TAD || t == T_HCN2 || t == 131 ||
                                          t == 132 || t == 154 || t == 157 || t == 205 || t == 206 ||
-                                         t == 207 || t == 202 || t == 203 || t == 204))
+                                         t == 207 || t == 202 || t == 203 || t == 204 || t == 168 ||
+                                         t == 200 || t == 201 || t == 191))
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;
