@@ -102,9 +102,10 @@ KV12, KV13, KV14, KV16, KV22 = 157, 154, 206, 207, 205
 KV_VARIANTS = (KV12, KV13, KV14, KV16, KV22)
 K_MORE = (203, 204, 202, 160, 161, 166, 167, 208, 178, 176)   # Kv3.2/3.3/4.3, GIRK, KATP, THIK-1/2, Kir2.2/7.1, KCNQ4
 CAV_MORE = (168, 200, 201, 191)   # Cav2.2, Cav3.1, Cav3.2, Cav3.3
+CATION_MORE = (210, 162, 211, 175, 182)   # TRPC3, TRPC4, TRPM7, TRPV3, ENaC
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -122,6 +123,7 @@ CANON_DENSITY.update({HCN3: 120.0, HCN4: 120.0, NAV11: 8.0, NAV13: 8.0, NAV17: 8
                       KV12: 2.0, KV13: 2.0, KV14: 2.0, KV16: 2.0, KV22: 2.0})
 CANON_DENSITY.update({t: 0.1 for t in K_MORE})
 CANON_DENSITY.update({t: 2.0 for t in CAV_MORE})
+CANON_DENSITY.update({t: 0.05 for t in CATION_MORE})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -467,9 +469,13 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, mye
     segs = net.segments
     for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:   # the rest of the K+ set on dendrites
         k = int(segs["modifier_count"][i])
-        for j, t in enumerate(K_MORE[6:] + CAV_MORE):
-            mods["type"][i, k + j], mods["density"][i, k + j] = t, (0.1 if t in K_MORE else 2.0)
-        segs["modifier_count"][i] = k + len(K_MORE) - 6 + len(CAV_MORE)
+        extra = K_MORE[6:] + CAV_MORE + CATION_MORE
+        for j, t in enumerate(extra):
+            mods["type"][i, k + j], mods["density"][i, k + j] = t, CANON_DENSITY[t]
+        segs["modifier_count"][i] = k + len(extra)
+    for t, k, v in ((210, 4, 0.4), (162, 4, 0.3), (175, 4, 30.0)):   # DAG, Gq, ligand
+        i, j = np.nonzero(mods["type"] == t)
+        mods["state"][i, j, k] = v
     i, j = np.nonzero(mods["type"] == 160)
     mods["state"][i, j, 4] = 0.3    # GIRK: some G-beta-gamma
     i, j = np.nonzero(mods["type"] == 161)

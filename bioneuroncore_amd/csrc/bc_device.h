@@ -92,7 +92,7 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 73: case 74: case 131: case 132: case 169: case 170: case 171: case 177: case 199:
         case 154: case 157: case 205: case 206: case 207: case 202: case 203: case 204: case 160:
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
-        case 191:
+        case 191: case 210: case 162: case 211: case 175: case 182:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1456,6 +1456,94 @@ BC_DEV double ch_cavt(const ModSlot& m, SegRegs& s, float dt, const CavTK& g) {
     return -ica;
 }
 
+// TRPC3 (210, DAG state[4], Pyr3 block state[5]), TRPC4 (162, Gq state[4], ML204 block
+// state[5]) and TRPM7 (211, closed by intracellular Mg2+, NS8593 block state[5])
+// (hh_biological.cpp trpc3/trpc4/trpm7): Na+ and Ca2+ currents from partial permeabilities; the
+// Ca2+ change uses the charge-per-mole factor without the valence.
+template <int T>
+BC_DEV double ch_trp_naca(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    float po;
+    double s5 = m.get(5);
+    if (T == 211) {
+        float mg = s.in[4];
+        float r = mg / 0.7f;
+        po = 1.0f / (1.0f + r * r);
+        double ns = (s5 > 0.0f) ? s5 : 0.0f;
+        po /= (1.0f + ns / 1.6f);
+    } else {
+        double a = m.get(4);
+        if (a < 0.0f) a = 0.0f;
+        if (a > 1.0f) a = 1.0f;
+        po = a;
+        double b = (s5 > 0.0f) ? s5 : 0.0f;
+        po /= (1.0f + b / (T == 210 ? 0.7f : 2.0f));
+    }
+    const P3 pp = p_partial3(T == 211 ? 40.0f : T == 210 ? 60.0f : 40.0f, 1.0f, 0.0f,
+                             T == 211 ? 0.34f : T == 210 ? 1.62f : 4.0f);
+    double N = m.dens * area;
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double fca = ghk_di(s, V, s.in[2], s.ex[2]);
+    double ina = pp.na * fna * N * po * kAtoPa;
+    double ica = pp.ca * fca * N * po * kAtoPa;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    s.in[2] -= (ica * dt) / s.cpm;
+    s.ex[2] += (ica * dt) / (s.cpm * kEvr);
+    floor7(s.ex);
+    floor7(s.in);
+    return -(ina + ica);
+}
+
+// TRPV3 (175, hh_biological.cpp trpv3_current): ligand state[4] (EC50 50) drives the gate with
+// tau 20 ms; float Na+ and Ca2+ currents.
+BC_DEV double ch_trpv3(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s4 = m.get(4);
+    double lig = (s4 > 0.0f) ? s4 : 0.0f;
+    float r = lig / (lig + 50.0f);
+    double g0 = m.get(0);
+    g0 += (r - g0) * (dt / 20.0f);
+    if (g0 < 0) g0 = 0;
+    if (g0 > 1) g0 = 1;
+    m.set(0, g0);
+    const P3 pp = p_partial3(185.0f, 0.5f, 0.0f, 0.5f);
+    double N = m.dens * area;
+    double fca = ghk_di(s, V, s.in[2], s.ex[2]);
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    float ica = (float)(pp.ca * fca * N * g0 * kAtoPa);
+    float ina = (float)(pp.na * fna * N * g0 * kAtoPa);
+    double cpm = s.cpm;
+    s.in[2] -= (ica * dt) / (2.0f * cpm);
+    s.in[0] -= (ina * dt) / cpm;
+    s.ex[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    s.ex[0] += (ina * dt) / (cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(ica + ina);
+}
+
+// ENaC (182, hh_biological.cpp enac_current): Po 0.5, amiloride block state[5] (IC50 100),
+// 4.5 pS; float current.
+BC_DEV double ch_enac(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    float op = 0.5f;
+    double s5 = m.get(5);
+    double am = (s5 > 0.0f) ? s5 : 0.0f;
+    op /= (1.0f + am / 100.0f);
+    double P = p_single(4.5f, 1.0f, kCbarNa);
+    double flux = ghk_mono(s, V, s.in[0], s.ex[0]);
+    float I = (float)(P * flux * (m.dens * area) * op * kAtoPa);
+    s.in[0] -= (I * dt) / s.cpm;
+    s.ex[0] += (I * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -I;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -1817,6 +1905,11 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 200: return ch_cavt(m, s, dt, CavTK{-45.0f, 5.5f, -77.0f, 6.0f, 7.5f, 25.0f, 250.0f, 15.0f, 250.0f, 1.6f});
         case 201: return ch_cavt(m, s, dt, CavTK{-46.0f, 6.0f, -72.0f, 6.0f, 9.0f, 16.0f, 180.0f, 15.0f, 13.0f, 3.0f});
         case 191: return ch_cavt(m, s, dt, CavTK{-43.8f, 6.9f, -72.5f, 6.6f, 11.0f, 97.0f, 400.0f, 30.0f, 200.0f, 1.6f});
+        case 210: return ch_trp_naca<210>(m, s, dt);
+        case 162: return ch_trp_naca<162>(m, s, dt);
+        case 211: return ch_trp_naca<211>(m, s, dt);
+        case 175: return ch_trpv3(m, s, dt);
+        case 182: return ch_enac(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

@@ -1357,6 +1357,97 @@ static double chan_cavt(Modifier* m, Segment* s, float dt, const CavT& g) {
     return -ica;
 }
 
+// TRPC3 (210, DAG state[4], Pyr3 block state[5]), TRPC4 (162, Gq state[4], ML204 block
+// state[5]) and TRPM7 (211, closed by intracellular Mg2+, NS8593 block state[5])
+// (hh_biological.cpp trpc3/trpc4/trpm7): Na+ and Ca2+ currents from partial permeabilities; the
+// Ca2+ change uses the charge-per-mole factor without the valence.
+static double chan_trp_naca(Modifier* m, Segment* s, float dt, uint8_t t) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float po;
+    float gamma, pca;
+    if (t == 211) {
+        float mg = s->intra_ions[4];
+        float r = mg / 0.7f;
+        po = 1.0f / (1.0f + r * r);
+        double ns = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        po /= (1.0f + ns / 1.6f);
+        gamma = 40.0f;
+        pca = 0.34f;
+    } else {
+        double a = m->state[4];
+        if (a < 0.0f) a = 0.0f;
+        if (a > 1.0f) a = 1.0f;
+        po = a;
+        double b = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        po /= (1.0f + b / (t == 210 ? 0.7f : 2.0f));
+        gamma = t == 210 ? 60.0f : 40.0f;
+        pca = t == 210 ? 1.62f : 4.0f;
+    }
+    double pna, pk, pc;
+    p_partial(gamma, 1.0f, 0.0f, pca, &pna, &pk, &pc);
+    double N = m->density * area;
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double fca = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double ina = pna * fna * N * po * kAtoPa;
+    double ica = pc * fca * N * po * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    s->intra_ions[2] -= (ica * dt) / cpv;
+    s->extra_ions[2] += (ica * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    floor_ions(s->intra_ions);
+    return -(ina + ica);
+}
+
+// TRPV3 (175, hh_biological.cpp trpv3_current): ligand state[4] (EC50 50) drives the gate with
+// tau 20 ms; float Na+ and Ca2+ currents.
+static double chan_trpv3(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double lig = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    float r = lig / (lig + 50.0f);
+    m->state[0] += (r - m->state[0]) * (dt / 20.0f);
+    if (m->state[0] < 0) m->state[0] = 0;
+    if (m->state[0] > 1) m->state[0] = 1;
+    double op = m->state[0];
+    double pna, pk, pca;
+    p_partial(185.0f, 0.5f, 0.0f, 0.5f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fca = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    float ica = (float)(pca * fca * N * op * kAtoPa);
+    float ina = (float)(pna * fna * N * op * kAtoPa);
+    double cpm = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->extra_ions[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    s->extra_ions[0] += (ina * dt) / (cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(ica + ina);
+}
+
+// ENaC (182, hh_biological.cpp enac_current): Po 0.5, amiloride block state[5] (IC50 100),
+// 4.5 pS; float current.
+static double chan_enac(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float op = 0.5f;
+    double am = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    op /= (1.0f + am / 100.0f);
+    double P = p_single(4.5f, 1.0f, kCbarNa);
+    double flux = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    float I = (float)(P * flux * (m->density * area) * op * kAtoPa);
+    double cpv = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[0] -= (I * dt) / cpv;
+    s->extra_ions[0] += (I * dt) / (cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -I;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1642,6 +1733,9 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 200: return chan_cavt(m, s, dt, kCav31);
         case 201: return chan_cavt(m, s, dt, kCav32);
         case 191: return chan_cavt(m, s, dt, kCav33);
+        case 210: case 162: case 211: return chan_trp_naca(m, s, dt, type);
+        case 175: return chan_trpv3(m, s, dt);
+        case 182: return chan_enac(m, s, dt);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
