@@ -102,7 +102,8 @@ KV12, KV13, KV14, KV16, KV22 = 157, 154, 206, 207, 205
 KV_VARIANTS = (KV12, KV13, KV14, KV16, KV22)
 K_MORE = (203, 204, 202, 160, 161, 166, 167, 208, 178, 176)   # Kv3.2/3.3/4.3, GIRK, KATP, THIK-1/2, Kir2.2/7.1, KCNQ4
 CAV_MORE = (168, 200, 201, 191)   # Cav2.2, Cav3.1, Cav3.2, Cav3.3
-CATION_MORE = (210, 162, 211, 175, 182)   # TRPC3, TRPC4, TRPM7, TRPV3, ENaC
+CATION_MORE = (210, 162, 211, 175, 182, 172, 173, 174, 179, 163, 164, 180)   # TRPC3/4/5/6-7, TRPM4/7/8,
+# TRPV1/V3, TRPA1, ENaC, CALHM1/3
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
                        NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE
@@ -473,7 +474,8 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, mye
         for j, t in enumerate(extra):
             mods["type"][i, k + j], mods["density"][i, k + j] = t, CANON_DENSITY[t]
         segs["modifier_count"][i] = k + len(extra)
-    for t, k, v in ((210, 4, 0.4), (162, 4, 0.3), (175, 4, 30.0)):   # DAG, Gq, ligand
+    for t, k, v in ((210, 4, 0.4), (162, 4, 0.3), (175, 4, 30.0), (172, 4, 0.5), (173, 4, 20.0),
+                    (174, 4, 20.0), (179, 4, 5.0), (163, 4, 0.3)):   # DAG, Gq, ligands
         i, j = np.nonzero(mods["type"] == t)
         mods["state"][i, j, k] = v
     i, j = np.nonzero(mods["type"] == 160)

@@ -92,7 +92,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 73: case 74: case 131: case 132: case 169: case 170: case 171: case 177: case 199:
         case 154: case 157: case 205: case 206: case 207: case 202: case 203: case 204: case 160:
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
-        case 191: case 210: case 162: case 211: case 175: case 182:
+        case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
+        case 179: case 163: case 164: case 180:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1544,6 +1545,125 @@ BC_DEV double ch_enac(const ModSlot& m, SegRegs& s, float dt) {
     return -I;
 }
 
+// Ligand-gated TRP channels with float Na+/Ca2+ currents (hh_biological.cpp trpv1_channel/
+// trpm8/trpa1/trpc67): the ligand state[4] sets the gate target (EC50 ec), a Ca-driven
+// desensitisation in state[1] (TRPV1 172, TRPM8 173, TRPA1 174) and, for TRPA1, Ca2+
+// co-activation; TRPC6/7 (179) has DAG gating only.
+struct TrpLigK { float ec, tau, tdes, gamma, fna, fca; bool des, ca_act; };
+BC_DEV double ch_trp_lig(const ModSlot& m, SegRegs& s, float dt, const TrpLigK& g) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s4 = m.get(4);
+    double lig = (s4 > 0.0f) ? s4 : 0.0f;
+    float r = lig / (lig + g.ec);
+    double g0 = m.get(0), op;
+    if (g.des) {
+        double g1 = m.get(1);
+        float ca = s.in[2] * kMmToUm;
+        float ca_act = ca / (ca + 6.0f);
+        float d_inf = ca / (ca + 0.5f);
+        g0 += (r - g0) * (dt / g.tau);
+        g1 += (d_inf - g1) * (dt / g.tdes);
+        if (g0 < 0) g0 = 0;
+        if (g0 > 1) g0 = 1;
+        if (g1 < 0) g1 = 0;
+        if (g1 > 1) g1 = 1;
+        m.set(1, g1);
+        double act = g.ca_act ? (double)fminf(1.0f, g0 + ca_act) : g0;
+        op = act * (1.0f - g1);
+    } else {
+        g0 += (r - g0) * (dt / g.tau);
+        if (g0 < 0) g0 = 0;
+        if (g0 > 1) g0 = 1;
+        op = g0;
+    }
+    m.set(0, g0);
+    const P3 pp = p_partial3(g.gamma, g.fna, 0.0f, g.fca);
+    double N = m.dens * area;
+    double fca = ghk_di(s, V, s.in[2], s.ex[2]);
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    float ica = (float)(pp.ca * fca * N * op * kAtoPa);
+    float ina = (float)(pp.na * fna * N * op * kAtoPa);
+    double cpm = s.cpm;
+    s.in[2] -= (ica * dt) / (2.0f * cpm);
+    s.in[0] -= (ina * dt) / cpm;
+    s.ex[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    s.ex[0] += (ina * dt) / (cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(ica + ina);
+}
+
+// TRPC5 (163, Gq state[4] x intracellular Ca2+ co-activation, HC-070 block state[5]) and TRPM4
+// (kM4, 164: Ca2+-activated Na+/K+ channel, 9-phenanthrol block state[5]) (hh_biological.cpp
+// trpc5/trpm4).
+template <bool kM4>
+BC_DEV double ch_trpc5_m4(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    float cu = s.in[2] * 1000.0f;
+    double s5 = m.get(5);
+    float po;
+    if (kM4) {
+        po = cu / (cu + 0.4f);
+        double ph = (s5 > 0.0f) ? s5 : 0.0f;
+        po /= (1.0f + ph / 18.0f);
+    } else {
+        double gq = m.get(4);
+        if (gq < 0.0f) gq = 0.0f;
+        if (gq > 1.0f) gq = 1.0f;
+        float ca_act = cu / (cu + 0.5f);
+        po = gq * ca_act;
+        double hc = (s5 > 0.0f) ? s5 : 0.0f;
+        po /= (1.0f + hc / 9.3f);
+    }
+    const P3 pp = kM4 ? p_partial3(25.0f, 1.0f, 1.0f, 0.0f) : p_partial3(50.0f, 1.0f, 0.0f, 5.0f);
+    double N = m.dens * area;
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double ina = pp.na * fna * N * po * kAtoPa;
+    double f2 = kM4 ? ghk_mono(s, V, s.in[1], s.ex[1]) : ghk_di(s, V, s.in[2], s.ex[2]);
+    double i2 = (kM4 ? pp.k : pp.ca) * f2 * N * po * kAtoPa;
+    constexpr int k2 = kM4 ? 1 : 2;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    s.in[k2] -= (i2 * dt) / s.cpm;
+    s.ex[k2] += (i2 * dt) / (s.cpm * kEvr);
+    floor7(s.ex);
+    floor7(s.in);
+    return -(ina + i2);
+}
+
+// CALHM1/3 (180, hh_biological.cpp calhm13_current): depolarisation-activated gate (V1/2 30.5,
+// tau 20 ms), float Na+/K+/Ca2+ currents.
+BC_DEV double ch_calhm(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - 30.5f) / 8.0f));
+    double g0 = m.get(0);
+    g0 += (mi - g0) * (dt / 20.0f);
+    if (g0 < 0) g0 = 0;
+    if (g0 > 1) g0 = 1;
+    m.set(0, g0);
+    const P3 pp = p_partial3(25.0f, 1.0f, 1.17f, 10.0f);
+    double N = m.dens * area;
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double fk = ghk_mono(s, V, s.in[1], s.ex[1]);
+    double fca = ghk_di(s, V, s.in[2], s.ex[2]);
+    float ina = (float)(pp.na * fna * N * g0 * kAtoPa);
+    float ik = (float)(pp.k * fk * N * g0 * kAtoPa);
+    float ica = (float)(pp.ca * fca * N * g0 * kAtoPa);
+    double cpv = s.cpm;
+    s.in[0] -= (ina * dt) / cpv;
+    s.in[1] -= (ik * dt) / cpv;
+    s.in[2] -= (ica * dt) / (2.0f * cpv);
+    s.ex[0] += (ina * dt) / (cpv * kEvr);
+    s.ex[1] += (ik * dt) / (cpv * kEvr);
+    s.ex[2] += (ica * dt) / (2.0f * cpv * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(ina + ik + ica);
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -1910,6 +2030,13 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 211: return ch_trp_naca<211>(m, s, dt);
         case 175: return ch_trpv3(m, s, dt);
         case 182: return ch_enac(m, s, dt);
+        case 172: return ch_trp_lig(m, s, dt, TrpLigK{0.7f, 10.0f, 150.0f, 70.0f, 0.09f, 0.91f, true, false});
+        case 173: return ch_trp_lig(m, s, dt, TrpLigK{30.0f, 20.0f, 200.0f, 63.0f, 0.5f, 0.5f, true, false});
+        case 174: return ch_trp_lig(m, s, dt, TrpLigK{30.0f, 30.0f, 300.0f, 90.0f, 0.54f, 0.46f, true, true});
+        case 179: return ch_trp_lig(m, s, dt, TrpLigK{10.0f, 50.0f, 0.0f, 35.0f, 0.17f, 0.83f, false, false});
+        case 163: return ch_trpc5_m4<false>(m, s, dt);
+        case 164: return ch_trpc5_m4<true>(m, s, dt);
+        case 180: return ch_calhm(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

@@ -1448,6 +1448,127 @@ static double chan_enac(Modifier* m, Segment* s, float dt) {
     return -I;
 }
 
+// Ligand-gated TRP channels with float Na+/Ca2+ currents (hh_biological.cpp trpv1_channel/
+// trpm8/trpa1/trpc67): the ligand state[4] sets the gate target (EC50 ec), a Ca-driven
+// desensitisation in state[1] (TRPV1 172, TRPM8 173, TRPA1 174) and, for TRPA1, Ca2+
+// co-activation; TRPC6/7 (179) has DAG gating only.
+struct TrpLig { float ec, tau, tdes, gamma, fna, fca; bool des, ca_act; };
+static constexpr TrpLig kTrpv1 = {0.7f, 10.0f, 150.0f, 70.0f, 0.09f, 0.91f, true, false};
+static constexpr TrpLig kTrpm8 = {30.0f, 20.0f, 200.0f, 63.0f, 0.5f, 0.5f, true, false};
+static constexpr TrpLig kTrpa1 = {30.0f, 30.0f, 300.0f, 90.0f, 0.54f, 0.46f, true, true};
+static constexpr TrpLig kTrpc67 = {10.0f, 50.0f, 0.0f, 35.0f, 0.17f, 0.83f, false, false};
+static double chan_trp_lig(Modifier* m, Segment* s, float dt, const TrpLig& g) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double lig = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    float r = lig / (lig + g.ec);
+    double op;
+    if (g.des) {
+        float ca = s->intra_ions[2] * kMmToUm;
+        float ca_act = ca / (ca + 6.0f);
+        float d_inf = ca / (ca + 0.5f);
+        m->state[0] += (r - m->state[0]) * (dt / g.tau);
+        m->state[1] += (d_inf - m->state[1]) * (dt / g.tdes);
+        if (m->state[0] < 0) m->state[0] = 0;
+        if (m->state[0] > 1) m->state[0] = 1;
+        if (m->state[1] < 0) m->state[1] = 0;
+        if (m->state[1] > 1) m->state[1] = 1;
+        double act = g.ca_act ? (double)fminf(1.0f, m->state[0] + ca_act) : m->state[0];
+        op = act * (1.0f - m->state[1]);
+    } else {
+        m->state[0] += (r - m->state[0]) * (dt / g.tau);
+        if (m->state[0] < 0) m->state[0] = 0;
+        if (m->state[0] > 1) m->state[0] = 1;
+        op = m->state[0];
+    }
+    double pna, pk, pca;
+    p_partial(g.gamma, g.fna, 0.0f, g.fca, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fca = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    float ica = (float)(pca * fca * N * op * kAtoPa);
+    float ina = (float)(pna * fna * N * op * kAtoPa);
+    double cpm = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->extra_ions[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    s->extra_ions[0] += (ina * dt) / (cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(ica + ina);
+}
+
+// TRPC5 (163, Gq state[4] x intracellular Ca2+ co-activation, HC-070 block state[5]) and TRPM4
+// (164, Ca2+-activated Na+/K+ channel, 9-phenanthrol block state[5]) (hh_biological.cpp
+// trpc5/trpm4).
+static double chan_trpc5_m4(Modifier* m, Segment* s, float dt, bool m4) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float cu = s->intra_ions[2] * 1000.0f;
+    float po;
+    if (m4) {
+        po = cu / (cu + 0.4f);
+        double ph = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        po /= (1.0f + ph / 18.0f);
+    } else {
+        double gq = m->state[4];
+        if (gq < 0.0f) gq = 0.0f;
+        if (gq > 1.0f) gq = 1.0f;
+        float ca_act = cu / (cu + 0.5f);
+        po = gq * ca_act;
+        double hc = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+        po /= (1.0f + hc / 9.3f);
+    }
+    double pna, pk, pca;
+    if (m4) p_partial(25.0f, 1.0f, 1.0f, 0.0f, &pna, &pk, &pca);
+    else p_partial(50.0f, 1.0f, 0.0f, 5.0f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double ina = pna * fna * N * po * kAtoPa;
+    double cpv = kFaraday * volume(s) / 1000.0;
+    const int k2 = m4 ? 1 : 2;
+    double f2 = m4 ? ghk_mono(V, s->intra_ions[1], s->extra_ions[1]) : ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double i2 = (m4 ? pk : pca) * f2 * N * po * kAtoPa;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    s->intra_ions[k2] -= (i2 * dt) / cpv;
+    s->extra_ions[k2] += (i2 * dt) / (cpv * kEvr);
+    floor_ions(s->extra_ions);
+    floor_ions(s->intra_ions);
+    return -(ina + i2);
+}
+
+// CALHM1/3 (180, hh_biological.cpp calhm13_current): depolarisation-activated gate (V1/2 30.5,
+// tau 20 ms), float Na+/K+/Ca2+ currents.
+static double chan_calhm(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + exp(-(V - 30.5f) / 8.0f));
+    m->state[0] += (mi - m->state[0]) * (dt / 20.0f);
+    if (m->state[0] < 0) m->state[0] = 0;
+    if (m->state[0] > 1) m->state[0] = 1;
+    double op = m->state[0];
+    double pna, pk, pca;
+    p_partial(25.0f, 1.0f, 1.17f, 10.0f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double fk = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    double fca = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    float ina = (float)(pna * fna * N * op * kAtoPa);
+    float ik = (float)(pk * fk * N * op * kAtoPa);
+    float ica = (float)(pca * fca * N * op * kAtoPa);
+    double cpv = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->intra_ions[1] -= (ik * dt) / cpv;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpv);
+    s->extra_ions[0] += (ina * dt) / (cpv * kEvr);
+    s->extra_ions[1] += (ik * dt) / (cpv * kEvr);
+    s->extra_ions[2] += (ica * dt) / (2.0f * cpv * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(ina + ik + ica);
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1736,6 +1857,13 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 210: case 162: case 211: return chan_trp_naca(m, s, dt, type);
         case 175: return chan_trpv3(m, s, dt);
         case 182: return chan_enac(m, s, dt);
+        case 172: return chan_trp_lig(m, s, dt, kTrpv1);
+        case 173: return chan_trp_lig(m, s, dt, kTrpm8);
+        case 174: return chan_trp_lig(m, s, dt, kTrpa1);
+        case 179: return chan_trp_lig(m, s, dt, kTrpc67);
+        case 163: return chan_trpc5_m4(m, s, dt, false);
+        case 164: return chan_trpc5_m4(m, s, dt, true);
+        case 180: return chan_calhm(m, s, dt);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
