@@ -104,9 +104,10 @@ K_MORE = (203, 204, 202, 160, 161, 166, 167, 208, 178, 176)   # Kv3.2/3.3/4.3, G
 CAV_MORE = (168, 200, 201, 191)   # Cav2.2, Cav3.1, Cav3.2, Cav3.3
 CATION_MORE = (210, 162, 211, 175, 182, 172, 173, 174, 179, 163, 164, 180)   # TRPC3/4/5/6-7, TRPM4/7/8,
 # TRPV1/V3, TRPA1, ENaC, CALHM1/3
+MISC_MORE = (159, 101, 96)   # Kv3.4, Ca2+ leak, post-inhibitory-rebound T-type Ca2+
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE
+                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -125,6 +126,7 @@ CANON_DENSITY.update({HCN3: 120.0, HCN4: 120.0, NAV11: 8.0, NAV13: 8.0, NAV17: 8
 CANON_DENSITY.update({t: 0.1 for t in K_MORE})
 CANON_DENSITY.update({t: 2.0 for t in CAV_MORE})
 CANON_DENSITY.update({t: 0.05 for t in CATION_MORE})
+CANON_DENSITY.update({159: 1.0, 101: 0.005, 96: 2.0})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -470,7 +472,7 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, mye
     segs = net.segments
     for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:   # the rest of the K+ set on dendrites
         k = int(segs["modifier_count"][i])
-        extra = K_MORE[6:] + CAV_MORE + CATION_MORE
+        extra = K_MORE[6:] + CAV_MORE + CATION_MORE + MISC_MORE
         for j, t in enumerate(extra):
             mods["type"][i, k + j], mods["density"][i, k + j] = t, CANON_DENSITY[t]
         segs["modifier_count"][i] = k + len(extra)

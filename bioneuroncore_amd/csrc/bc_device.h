@@ -62,7 +62,7 @@ __host__ __device__ static inline bool myelin_blocks(uint8_t t) {
            t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK || t == M_KV42 ||
            t == M_NAP || t == M_NAP_TAD || t == M_HCN2 || t == 131 || t == 132 || t == 154 || t == 157 ||
            t == 205 || t == 206 || t == 207 || t == 202 || t == 203 || t == 204 || t == 168 || t == 200 ||
-           t == 201 || t == 191;
+           t == 201 || t == 191 || t == 159 || t == 96;
 }
 
 // G-protein-coupled receptor types: they read state[active_vars + 1].
@@ -75,7 +75,7 @@ __host__ __device__ static inline bool gprotein_type(uint8_t t) {
 __host__ __device__ static inline bool modifier_reads_nt(uint8_t t) {
     return t == M_AMPA || t == M_NMDA || t == M_GABAA || (t >= M_GLAST && t <= M_GAT3) ||
            (t >= M_EAAT3 && t <= M_GAT1) || (t >= 10 && t <= 64) || (t >= 67 && t <= 68) ||
-           (t >= 70 && t <= 74) || t == 97 || t == 134 || t == 194;
+           (t >= 70 && t <= 74) || t == 96 || t == 97 || t == 134 || t == 194;
 }
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
@@ -93,7 +93,7 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 154: case 157: case 205: case 206: case 207: case 202: case 203: case 204: case 160:
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
-        case 179: case 163: case 164: case 180:
+        case 179: case 163: case 164: case 180: case 159: case 101: case 96:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1664,6 +1664,97 @@ BC_DEV double ch_calhm(const ModSlot& m, SegRegs& s, float dt) {
     return -(ina + ik + ica);
 }
 
+// Kv3.4 (159, hh_biological.cpp kv34_current, modifier_helpers.h:564-576): m h hs with PKC
+// relief of the fast inactivation, 4-AP/TEA blocks from states 5 and 6; float K+ in the flux.
+BC_DEV double ch_kv34(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double g0 = m.get(0), g1 = m.get(1), g2 = m.get(2), s5 = m.get(5), s6 = m.get(6);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - 19.0f) / 11.3f));
+    float tm = 40.0f - 37.7f / (1.0f + bc::exp(-V / 28.0f));
+    g0 += (mi - g0) * dt / tm;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    float hi = 1.0f / (1.0f + bc::exp((V + 15.0f) / 7.4f));
+    float th = 12.0f + 165.6f * bc::exp(-V / 11.2f);
+    if (th > 2000.0f) th = 2000.0f;
+    g1 += (hi - g1) * dt / th;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    double he = g1 + (1.0f - g1) * s.pkc * 0.9f;
+    float hsi = 1.0f / (1.0f + bc::exp((V + 30.0f) / 6.0f));
+    g2 += (hsi - g2) * dt / 2000.0f;
+    if (g2 < 0.0f) g2 = 0.0f;
+    if (g2 > 1.0f) g2 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    m.set(2, g2);
+    double op = g0 * he * g2;
+    double ap4 = (s5 > 0.0f) ? s5 : 0.0f;
+    double tea = (s6 > 0.0f) ? s6 : 0.0f;
+    op /= (1.0f + ap4 / 500.0f) * (1.0f + tea / 1000.0f);
+    float ki = s.in[1], ko = s.ex[1];
+    double P = p_single(20.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, V, ki, ko);
+    return k_out(s, P * flux * (m.dens * area) * op * kAtoPa, dt);
+}
+
+// Ca2+/Na+ leak (101, hh_biological.cpp ca_leak_current): always open, 50 pS split 1:2 Na:Ca,
+// float concentrations in the fluxes, the volume rounded to float, intracellular change only.
+BC_DEV double ch_ca_leak(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    float V = s.V;
+    float ci = s.in[2], co = s.ex[2];
+    float ni = s.in[0], no = s.ex[0];
+    const P3 pp = p_partial3(50.0f, 1.0f, 0.0f, 2.0f);
+    double N = m.dens * area;
+    double fna = ghk_mono(s, V, ni, no);
+    double fca = ghk_di(s, V, ci, co);
+    double ina = pp.na * fna * N * kAtoPa;
+    double ica = pp.ca * fca * N * kAtoPa;
+    float vol = s.vol;
+    double cpv = kFaraday * vol / 1000.0f;
+    s.in[0] -= (ina * dt) / cpv;
+    s.in[2] -= (ica * dt) / (2.0f * cpv);
+    floor7(s.in);
+    return -(ina + ica);
+}
+
+// Post-inhibitory-rebound T-type Ca2+ (96, hh_biological.cpp pir_ttype_current): m^3 h in ten
+// substeps, serotonin and dopamine reduce it by up to 70%; float Ca in the flux, the
+// intracellular Ca changes with a float charge-per-mole factor and is not floored.
+BC_DEV double ch_pir_ttype(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2);
+    double gamma = (s2 > 0.0f) ? s2 : 7.2f;
+    double mv = m.get(0), hv = m.get(1);
+    float mi = 1.0f / (1.0f + bc::exp(-(V - -48.4f) / 5.0f));
+    float hi = 1.0f / (1.0f + bc::exp((V - -75.6f) / 6.5f));
+    float th = 15.0f + (150.0f - 15.0f) / (1.0f + bc::exp((V - -65.0f) / 8.0f));
+    float sub = dt / 10.0f;
+    for (int k = 0; k < 10; k++) {
+        mv += (mi - mv) * sub / 3.0f;
+        hv += (hi - hv) * sub / th;
+    }
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m.set(0, mv);
+    m.set(1, hv);
+    float ci = s.in[2], co = s.ex[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    float ht = s.nt[3], da = s.nt[2];
+    double mod = (1.0 - 0.7 / (1.0 + (ht + da) / 0.01f));
+    double flux = ghk_di(s, V, ci, co);
+    double ica = P * flux * (m.dens * area) * mod * (mv * mv * mv * hv) * kAtoPa;
+    float vol = s.vol;
+    float cpm = kFaraday * vol / 1000.0;
+    s.in[2] -= (ica * dt) / (2.0f * cpm);
+    return -ica;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -2037,6 +2128,9 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 163: return ch_trpc5_m4<false>(m, s, dt);
         case 164: return ch_trpc5_m4<true>(m, s, dt);
         case 180: return ch_calhm(m, s, dt);
+        case 159: return ch_kv34(m, s, dt);
+        case 101: return ch_ca_leak(m, s, dt);
+        case 96: return ch_pir_ttype(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

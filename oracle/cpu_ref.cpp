@@ -1569,6 +1569,95 @@ static double chan_calhm(Modifier* m, Segment* s, float dt) {
     return -(ina + ik + ica);
 }
 
+// Kv3.4 (159, hh_biological.cpp kv34_current, modifier_helpers.h:564-576): m h hs with PKC
+// relief of the fast inactivation, 4-AP/TEA blocks from states 5 and 6; float K+ in the flux.
+static double chan_kv34(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float mi = 1.0f / (1.0f + exp(-(V - 19.0f) / 11.3f));
+    float tm = 40.0f - 37.7f / (1.0f + exp(-V / 28.0f));
+    m->state[0] += (mi - m->state[0]) * dt / tm;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float hi = 1.0f / (1.0f + exp((V + 15.0f) / 7.4f));
+    float th = 12.0f + 165.6f * exp(-V / 11.2f);
+    if (th > 2000.0f) th = 2000.0f;
+    m->state[1] += (hi - m->state[1]) * dt / th;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    float pkc = first_state_of(s, T_PKC, 0, 0.0f);
+    double he = m->state[1] + (1.0f - m->state[1]) * pkc * 0.9f;
+    float hsi = 1.0f / (1.0f + exp((V + 30.0f) / 6.0f));
+    m->state[2] += (hsi - m->state[2]) * dt / 2000.0f;
+    if (m->state[2] < 0.0f) m->state[2] = 0.0f;
+    if (m->state[2] > 1.0f) m->state[2] = 1.0f;
+    double op = m->state[0] * he * m->state[2];
+    double ap4 = (m->state[5] > 0.0f) ? m->state[5] : 0.0f;
+    double tea = (m->state[6] > 0.0f) ? m->state[6] : 0.0f;
+    op /= (1.0f + ap4 / 500.0f) * (1.0f + tea / 1000.0f);
+    float ki = s->intra_ions[1], ko = s->extra_ions[1];
+    double P = p_single(20.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(V, ki, ko);
+    return k_efflux(s, P * flux * (m->density * area) * op * kAtoPa, dt);
+}
+
+// Ca2+/Na+ leak (101, hh_biological.cpp ca_leak_current): always open, 50 pS split 1:2 Na:Ca,
+// float concentrations in the fluxes, the volume rounded to float, intracellular change only.
+static double chan_ca_leak(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    float V = s->V_m;
+    float ci = s->intra_ions[2], co = s->extra_ions[2];
+    float ni = s->intra_ions[0], no = s->extra_ions[0];
+    double pna, pk, pca;
+    p_partial(50.0f, 1.0f, 0.0f, 2.0f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fna = ghk_mono(V, ni, no);
+    double fca = ghk_di(V, ci, co);
+    double ina = pna * fna * N * kAtoPa;
+    double ica = pca * fca * N * kAtoPa;
+    float vol = volume(s);
+    double cpv = kFaraday * vol / 1000.0f;
+    s->intra_ions[0] -= (ina * dt) / cpv;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpv);
+    floor_ions(s->intra_ions);
+    return -(ina + ica);
+}
+
+// Post-inhibitory-rebound T-type Ca2+ (96, hh_biological.cpp pir_ttype_current): m^3 h in ten
+// substeps, serotonin and dopamine reduce it by up to 70%; float Ca in the flux, the
+// intracellular Ca changes with a float charge-per-mole factor and is not floored.
+static double chan_pir_ttype(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 7.2f;
+    double mv = m->state[0], hv = m->state[1];
+    float mi = 1.0f / (1.0f + exp(-(V - -48.4f) / 5.0f));
+    float hi = 1.0f / (1.0f + exp((V - -75.6f) / 6.5f));
+    float th = 15.0f + (150.0f - 15.0f) / (1.0f + exp((V - -65.0f) / 8.0f));
+    float sub = dt / 10.0f;
+    for (int k = 0; k < 10; k++) {
+        mv += (mi - mv) * sub / 3.0f;
+        hv += (hi - hv) * sub / th;
+    }
+    if (mv < 0.0f) mv = 0.0f;
+    if (mv > 1.0f) mv = 1.0f;
+    if (hv < 0.0f) hv = 0.0f;
+    if (hv > 1.0f) hv = 1.0f;
+    m->state[0] = mv;
+    m->state[1] = hv;
+    float ci = s->intra_ions[2], co = s->extra_ions[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    float ht = s->neurotransmitter[3], da = s->neurotransmitter[2];
+    double mod = (1.0 - 0.7 / (1.0 + (ht + da) / 0.01f));
+    double flux = ghk_di(V, ci, co);
+    double ia = P * flux * (m->density * area) * mod * (mv * mv * mv * hv);
+    double ica = ia * kAtoPa;
+    float vol = volume(s);
+    float cpm = kFaraday * vol / 1000.0;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    return -ica;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1864,6 +1953,9 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 163: return chan_trpc5_m4(m, s, dt, false);
         case 164: return chan_trpc5_m4(m, s, dt, true);
         case 180: return chan_calhm(m, s, dt);
+        case 159: return chan_kv34(m, s, dt);
+        case 101: return chan_ca_leak(m, s, dt);
+        case 96: return chan_pir_ttype(m, s, dt);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
@@ -2520,7 +2612,7 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                                          t == T_NAP || t == T_NAP_TAD || t == T_HCN2 || t == 131 ||
                                          t == 132 || t == 154 || t == 157 || t == 205 || t == 206 ||
                                          t == 207 || t == 202 || t == 203 || t == 204 || t == 168 ||
-                                         t == 200 || t == 201 || t == 191))
+                                         t == 200 || t == 201 || t == 191 || t == 159 || t == 96))
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;
