@@ -62,7 +62,8 @@ __host__ __device__ static inline bool myelin_blocks(uint8_t t) {
            t == M_KVDR || t == M_CAV21 || t == M_HCN1 || t == M_SK || t == M_BK || t == M_KV42 ||
            t == M_NAP || t == M_NAP_TAD || t == M_HCN2 || t == 131 || t == 132 || t == 154 || t == 157 ||
            t == 205 || t == 206 || t == 207 || t == 202 || t == 203 || t == 204 || t == 168 || t == 200 ||
-           t == 201 || t == 191 || t == 159 || t == 96;
+           t == 201 || t == 191 || t == 159 || t == 96 || t == 84 || t == 152 || t == 151 || t == 153 ||
+           t == 158 || t == 121 || t == 209;
 }
 
 // G-protein-coupled receptor types: they read state[active_vars + 1].
@@ -93,7 +94,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 154: case 157: case 205: case 206: case 207: case 202: case 203: case 204: case 160:
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
-        case 179: case 163: case 164: case 180: case 159: case 101: case 96:
+        case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
+        case 145: case 146: case 152: case 186:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -113,6 +115,7 @@ struct SegRegs {
     float pkc, pka, camp;   // cascade activities read by Nav/Kv3.1/Kv7/HCN (constant on device)
     float camkii;           // state[0] of the last CaMKII modifier, read by the connexins
     float camkii01;         // its state[0] + state[1], read by Kv4.2
+    float cabp4;            // state[0] of the first CaBP4 modifier, set before each Cav1.4 runs
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -1755,6 +1758,158 @@ BC_DEV double ch_pir_ttype(const ModSlot& m, SegRegs& s, float dt) {
     return -ica;
 }
 
+// L-type Ca2+ Cav1.2 (84, hh_biological.cpp:599-646, modifier_helpers.h:624-626): m^2 times a
+// slow voltage-dependent (state[1]) and a Ca-dependent (state[5]) inactivation; PKA doubles the
+// current at full activity.
+BC_DEV double ch_ca_l(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2), s3 = m.get(3), s4 = m.get(4);
+    double vh = (s3 != 0.0f) ? -fabs(s3) : -11.0f;
+    double k = (s4 > 0.0f) ? s4 : 6.9f;
+    double gamma = (s2 > 0.0f) ? s2 : 25.0f;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (k + 1e-6f)));
+    double g0 = m.get(0), g1 = m.get(1), g5 = m.get(5);
+    g0 += (mi - g0) * dt / 3.0f;
+    float vi = 1.0f / (1.0f + bc::exp((V + 37.3f) / 8.0f));
+    g1 += (vi - g1) * dt / 150.0f;
+    float ca = s.in[2] * kMmToUm;
+    float cdi = 1.0f / (1.0f + ca / 1.0f);
+    g5 += (cdi - g5) * dt / 35.0f;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g5 < 0.0f) g5 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    if (g5 > 1.0f) g5 = 1.0f;
+    m.set(0, g0);
+    m.set(1, g1);
+    m.set(5, g5);
+    float ci = s.in[2], co = s.ex[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double op = g0 * g0 * g1 * g5;
+    float boost = 1.0f + 1.0f * s.pka;
+    double flux = ghk_di(s, V, ci, co);
+    double ica = P * flux * (m.dens * area) * op * boost * kAtoPa;
+    s.in[2] -= (ica * dt) / (2.0f * s.cpm);
+    s.ex[2] -= (ica * dt) / (2.0f * s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ica;
+}
+
+// Cav1.4 (144, hh_biological.cpp:648-682, modifier_helpers.h:283-313): m^2 without inactivation;
+// bound CaBP4 (s.cabp4) shifts V1/2 by -10 mV.
+BC_DEV double ch_cav14(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2);
+    double gamma = (s2 > 0.0f) ? s2 : 4.0f;
+    float vh = -15.0f + -10.0f * s.cabp4;
+    float mi = 1.0f / (1.0f + bc::exp(-(V - vh) / (7.0f + 1e-6f)));
+    double g0 = m.get(0);
+    g0 += (mi - g0) * dt / 2.0f;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    m.set(0, g0);
+    float ci = s.in[2], co = s.ex[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double flux = ghk_di(s, V, ci, co);
+    double ica = P * flux * (m.dens * area) * (g0 * g0) * kAtoPa;
+    s.in[2] -= (ica * dt) / (2.0f * s.cpm);
+    s.ex[2] -= (ica * dt) / (2.0f * s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -ica;
+}
+
+// CaBP4 (145, hh_biological.cpp:684-693): no current; state[0] binds towards 1 with tau 1 ms.
+BC_DEV double ch_cabp4(const ModSlot& m, float dt) {
+    double g0 = m.get(0);
+    g0 += (1.0f - g0) * dt / 1.0f;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    m.set(0, g0);
+    return 0.0;
+}
+
+// CaCC / ANO2 (146, hh_biological.cpp:695-722, modifier_helpers.h:318-327): Ca-activated Cl-
+// channel, Hill 2 with EC50 4 uM (the reference's pow(x, 2) of a float is the exact square in
+// double), GHK anion current; only the intracellular Cl changes.
+BC_DEV double ch_cacc(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s2 = m.get(2);
+    double gamma = (s2 > 0.0f) ? s2 : 2.0f;
+    float cu = s.in[2] * kMmToUm;
+    if (cu < 0.0f) cu = 0.0f;
+    float cn = (double)cu * (double)cu;
+    float mi = cn / (cn + 16.0f);
+    double g0 = m.get(0);
+    g0 += (mi - g0) * dt / 2.0f;
+    if (g0 < 0.0f) g0 = 0.0f;
+    if (g0 > 1.0f) g0 = 1.0f;
+    m.set(0, g0);
+    float ci = s.in[3], co = s.ex[3];
+    double P = p_single(gamma, 1.0f, kCbarCl);
+    double flux = ghk_an(s, V, ci, co);
+    double icl = P * flux * (m.dens * area) * g0 * kAtoPa;
+    s.in[3] += (icl * dt) / s.cpm;
+    floor7(s.in);
+    return -icl;
+}
+
+// CAN / TRPM4-5 (152, hh_biological.cpp:3643-3682): Ca-activated (Hill 2, Kd 0.5 uM) Na+/K+
+// channel with slow Ca-dependent desensitisation in state[1]; float currents.
+BC_DEV double ch_can(const ModSlot& m, SegRegs& s, float dt) {
+    constexpr P3 pp = p_partial3(30.0f, 1.0f, 1.0f, 0.0f);
+    float V = s.V;
+    float area = eff_area(m, s);
+    float ca = s.in[2];
+    const float kd = 5e-4f;
+    float po = (ca * ca) / (ca * ca + kd * kd);
+    float di = ca / (ca + 0.01f);
+    double g1 = m.get(1);
+    g1 += (di - g1) * dt / 2000.0f;
+    if (g1 < 0.0f) g1 = 0.0f;
+    if (g1 > 1.0f) g1 = 1.0f;
+    m.set(1, g1);
+    po *= (1.0f - g1);
+    double N = m.dens * area;
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    double fk = ghk_mono(s, V, s.in[1], s.ex[1]);
+    float ina = (float)(pp.na * fna * N * po * kAtoPa);
+    float ik = (float)(pp.k * fk * N * po * kAtoPa);
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.in[1] -= (ik * dt) / s.cpm;
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    s.ex[1] += (ik * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(ina - ik);
+}
+
+// AQP4 (186, hh_biological.cpp:4114-4129, modifier_helpers.h:395-396): water follows the osmotic
+// difference of the summed (float) concentrations and dilutes every intracellular ion by a
+// factor clamped to [0.95, 1]; no current, no floor.
+BC_DEV double ch_aqp4(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    float si = 0.0f, se = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        si += s.in[i];
+        se += s.ex[i];
+    }
+    float dpi = si - se;
+    double J = m.dens * 24e-14f * area * dpi * 1e-6f;
+    float f = 1.0f / (1.0f + J * dt);
+    if (f < 0.95f) f = 0.95f;
+    if (f > 1.0f) f = 1.0f;
+#pragma unroll
+    for (int i = 0; i < 7; i++) s.in[i] *= f;
+    return 0.0;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill Ca2+ extrusion, ATP-dependent. The reference
 // takes the Hill powers with pow; for the default n = 2 (the only one the loader accepts as
 // covered) both are exact squares in double.
@@ -2131,6 +2286,12 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 159: return ch_kv34(m, s, dt);
         case 101: return ch_ca_leak(m, s, dt);
         case 96: return ch_pir_ttype(m, s, dt);
+        case 84: return ch_ca_l(m, s, dt);
+        case 144: return ch_cav14(m, s, dt);
+        case 145: return ch_cabp4(m, dt);
+        case 146: return ch_cacc(m, s, dt);
+        case 152: return ch_can(m, s, dt);
+        case 186: return ch_aqp4(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

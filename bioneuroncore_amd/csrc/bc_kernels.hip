@@ -83,6 +83,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         m.dens = d.mdens[j * P + i];
         m.a7 = s.area;
         m.av = d.mav[j * P + i];
+        if (kExt && t == 144) {   // Cav1.4 reads the first CaBP4's state[0] as it stands now
+            s.cabp4 = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 145) {
+                    s.cabp4 = d.mstate[(size_t)q * 7 * P + i];
+                    break;
+                }
+        }
         itot += dispatch<kNt, kExt>(t, m, s, dt);
         if (d.cable) {   // concentrations held: every channel sees the stored ones
 #pragma unroll

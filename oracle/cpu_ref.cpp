@@ -1658,6 +1658,151 @@ static double chan_pir_ttype(Modifier* m, Segment* s, float dt) {
     return -ica;
 }
 
+// L-type Ca2+ Cav1.2 (84, hh_biological.cpp:599-646, modifier_helpers.h:624-626): m^2 times a
+// slow voltage-dependent (state[1]) and a Ca-dependent (state[5]) inactivation, PKA (state[1]
+// of the first cAMP/PKA modifier) doubles the current at full activity.
+static double chan_ca_l(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double vh = (m->state[3] != 0.0f) ? -fabs(m->state[3]) : -11.0f;
+    double k = (m->state[4] > 0.0f) ? m->state[4] : 6.9f;
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 25.0f;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (k + 1e-6f)));
+    m->state[0] += (mi - m->state[0]) * dt / 3.0f;
+    float vi = 1.0f / (1.0f + exp((V + 37.3f) / 8.0f));
+    m->state[1] += (vi - m->state[1]) * dt / 150.0f;
+    float ca = s->intra_ions[2] * kMmToUm;
+    float ci_ = 1.0f / (1.0f + ca / 1.0f);
+    m->state[5] += (ci_ - m->state[5]) * dt / 35.0f;
+    for (int q : {0, 1, 5}) {
+        if (m->state[q] < 0.0f) m->state[q] = 0.0f;
+        if (m->state[q] > 1.0f) m->state[q] = 1.0f;
+    }
+    float ci = s->intra_ions[2], co = s->extra_ions[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double op = m->state[0] * m->state[0] * m->state[1] * m->state[5];
+    float boost = 1.0f + 1.0f * first_state_of(s, T_CAMP_PKA, 1, 0.0f);
+    double flux = ghk_di(V, ci, co);
+    double ia = P * flux * (m->density * area) * op * boost;
+    double ica = ia * kAtoPa;
+    double cpm = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->extra_ions[2] -= (ica * dt) / (2.0f * cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ica;
+}
+
+// Cav1.4 (144, hh_biological.cpp:648-682, modifier_helpers.h:283-313): m^2 without
+// inactivation; CaBP4 (state[0] of the first CaBP4 modifier, as it stands when this one runs)
+// shifts V1/2 by -10 mV.
+static double chan_cav14(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 4.0f;
+    float bound = first_state_of(s, 145, 0, 0.0f);
+    float vh = -15.0f + -10.0f * bound;
+    float mi = 1.0f / (1.0f + exp(-(V - vh) / (7.0f + 1e-6f)));
+    m->state[0] += (mi - m->state[0]) * dt / 2.0f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float ci = s->intra_ions[2], co = s->extra_ions[2];
+    double P = p_single(gamma, 2.0f, kCbarCa);
+    double op = m->state[0] * m->state[0];
+    double flux = ghk_di(V, ci, co);
+    double ia = P * flux * (m->density * area) * op;
+    double ica = ia * kAtoPa;
+    double cpm = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->extra_ions[2] -= (ica * dt) / (2.0f * cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -ica;
+}
+
+// CaBP4 (145, hh_biological.cpp:684-693): no current; state[0] binds towards 1 with tau 1 ms.
+static double chan_cabp4(Modifier* m, Segment*, float dt) {
+    m->state[0] += (1.0f - m->state[0]) * dt / 1.0f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    return 0.0f;
+}
+
+// CaCC / ANO2 (146, hh_biological.cpp:695-722, modifier_helpers.h:318-327): Ca-activated Cl-
+// channel, Hill 2 with EC50 4 uM (pow(x, 2) of a float is exact in double), GHK anion current;
+// only the intracellular Cl changes.
+static double chan_cacc(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double gamma = (m->state[2] > 0.0f) ? m->state[2] : 2.0f;
+    float cu = s->intra_ions[2] * kMmToUm;
+    if (cu < 0.0f) cu = 0.0f;
+    float cn = pow(cu, 2.0f);
+    float en = pow(4.0f, 2.0f);
+    float mi = cn / (cn + en);
+    m->state[0] += (mi - m->state[0]) * dt / 2.0f;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    float ci = s->intra_ions[3], co = s->extra_ions[3];
+    double P = p_single(gamma, 1.0f, kCbarCl);
+    double flux = ghk_an(V, ci, co);
+    double ia = P * flux * (m->density * area) * m->state[0];
+    double icl = ia * kAtoPa;
+    double cpm = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[3] += (icl * dt) / cpm;
+    floor_ions(s->intra_ions);
+    return -icl;
+}
+
+// CAN / TRPM4-5 (152, hh_biological.cpp:3643-3682): Ca-activated (Hill 2, Kd 0.5 uM) Na+/K+
+// channel with slow Ca-dependent desensitisation in state[1]; float currents.
+static double chan_can(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    float ca = s->intra_ions[2];
+    float kd = 5e-4f;
+    float po = (ca * ca) / (ca * ca + kd * kd);
+    float di = ca / (ca + 0.01f);
+    m->state[1] += (di - m->state[1]) * dt / 2000.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    po *= (1.0f - m->state[1]);
+    double pna, pk, pca;
+    p_partial(30.0f, 1.0f, 1.0f, 0.0f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    double fk = ghk_mono(V, s->intra_ions[1], s->extra_ions[1]);
+    float ina = (float)(pna * fna * N * po * kAtoPa);
+    float ik = (float)(pk * fk * N * po * kAtoPa);
+    double cpm = kFaraday * volume(s) / 1000.0;
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    s->extra_ions[0] += (ina * dt) / (cpm * kEvr);
+    s->extra_ions[1] += (ik * dt) / (cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(ina - ik);
+}
+
+// AQP4 (186, hh_biological.cpp:4114-4129, modifier_helpers.h:395-396): water follows the
+// osmotic difference of the summed (float) concentrations and dilutes every intracellular ion
+// by a factor clamped to [0.95, 1]; no current, no floor.
+static double chan_aqp4(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    float si = 0.0f, se = 0.0f;
+    for (int i = 0; i < 7; i++) {
+        si += s->intra_ions[i];
+        se += s->extra_ions[i];
+    }
+    float dpi = si - se;
+    double J = m->density * 24e-14f * area * dpi * 1e-6f;
+    float f = 1.0f / (1.0f + J * dt);
+    if (f < 0.95f) f = 0.95f;
+    if (f > 1.0f) f = 1.0f;
+    for (int i = 0; i < 7; i++) s->intra_ions[i] *= f;
+    return 0.0f;
+}
+
 // PMCA (98, hh_biological.cpp:2574-2614): Hill-n Ca2+ extrusion, ATP-dependent. The Hill powers
 // are glibc pow calls; with the default n = 2 both are exact squares (a float squared, and the
 // default K_m squared, fit a double), which is how they are computed here.
@@ -1956,6 +2101,12 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 159: return chan_kv34(m, s, dt);
         case 101: return chan_ca_leak(m, s, dt);
         case 96: return chan_pir_ttype(m, s, dt);
+        case 84: return chan_ca_l(m, s, dt);
+        case 144: return chan_cav14(m, s, dt);
+        case 145: return chan_cabp4(m, s, dt);
+        case 146: return chan_cacc(m, s, dt);
+        case 152: return chan_can(m, s, dt);
+        case 186: return chan_aqp4(m, s, dt);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
@@ -2612,7 +2763,9 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                                          t == T_NAP || t == T_NAP_TAD || t == T_HCN2 || t == 131 ||
                                          t == 132 || t == 154 || t == 157 || t == 205 || t == 206 ||
                                          t == 207 || t == 202 || t == 203 || t == 204 || t == 168 ||
-                                         t == 200 || t == 201 || t == 191 || t == 159 || t == 96))
+                                         t == 200 || t == 201 || t == 191 || t == 159 || t == 96 ||
+                                         t == 84 || t == 152 || t == 151 || t == 153 || t == 158 ||
+                                         t == 121 || t == 209))
                 continue;
             s->modifiers[j].state[7] = is_sphere(s) ? 4.0f * kPi * s->radius * s->radius
                                                     : 2.0f * kPi * s->radius * s->length;
