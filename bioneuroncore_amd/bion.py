@@ -105,7 +105,7 @@ CAV_MORE = (168, 200, 201, 191)   # Cav2.2, Cav3.1, Cav3.2, Cav3.3
 CATION_MORE = (210, 162, 211, 175, 182, 172, 173, 174, 179, 163, 164, 180)   # TRPC3/4/5/6-7, TRPM4/7/8,
 # TRPV1/V3, TRPA1, ENaC, CALHM1/3
 MISC_MORE = (159, 101, 96)   # Kv3.4, Ca2+ leak, post-inhibitory-rebound T-type Ca2+
-CA_ACT = (84, 144, 145, 146, 152, 186)   # L-type Cav1.2, Cav1.4, CaBP4, CaCC, CAN, AQP4
+CA_ACT = (84, 144, 145, 146, 152, 186, 190)   # L-type Cav1.2, Cav1.4, CaBP4, CaCC, CAN, AQP4, TMC1
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
                        NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT
@@ -128,7 +128,7 @@ CANON_DENSITY.update({t: 0.1 for t in K_MORE})
 CANON_DENSITY.update({t: 2.0 for t in CAV_MORE})
 CANON_DENSITY.update({t: 0.05 for t in CATION_MORE})
 CANON_DENSITY.update({159: 1.0, 101: 0.005, 96: 2.0})
-CANON_DENSITY.update({84: 2.0, 144: 5.0, 145: 1.0, 146: 1.0, 152: 0.05, 186: 1e11})
+CANON_DENSITY.update({84: 2.0, 144: 5.0, 145: 1.0, 146: 1.0, 152: 0.05, 186: 1e11, 190: 0.05})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -480,7 +480,7 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, mye
         segs["modifier_count"][i] = k + len(extra)
     for n, i in enumerate(np.nonzero(segs["segment_type"] == 2)[0][1::2]):   # Ca-activated set
         k = int(segs["modifier_count"][i])
-        extra = CA_ACT if n % 2 else (84, 145, 144, 146, 152, 186)   # CaBP4 before/after Cav1.4
+        extra = CA_ACT if n % 2 else (84, 145, 144, 146, 152, 186, 190)   # CaBP4 before/after Cav1.4
         for j, t in enumerate(extra):
             mods["type"][i, k + j], mods["density"][i, k + j] = t, CANON_DENSITY[t]
         segs["modifier_count"][i] = k + len(extra)
@@ -490,6 +490,8 @@ def channel_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 5, mye
         mods["state"][i, j, k] = v
     i, j = np.nonzero(mods["type"] == 160)
     mods["state"][i, j, 4] = 0.3    # GIRK: some G-beta-gamma
+    i, j = np.nonzero(mods["type"] == 190)
+    mods["state"][i[::2], j[::2], 4] = 0.5   # TMC1: half the bundles deflected
     i, j = np.nonzero(mods["type"] == 161)
     mods["state"][i, j, 4] = 5.0    # KATP: partly closed by ATP
     if myelin:

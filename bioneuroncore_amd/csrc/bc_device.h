@@ -95,7 +95,7 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
         case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
-        case 145: case 146: case 152: case 186:
+        case 145: case 146: case 152: case 186: case 190:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -1889,6 +1889,43 @@ BC_DEV double ch_can(const ModSlot& m, SegRegs& s, float dt) {
     return -(ina - ik);
 }
 
+// TMC1 (190, hh_biological.cpp tmc1_current): hair-cell transduction channel, open fraction
+// relaxes to 0.1 + 0.9 stim/(stim + 0.3) (stim = state[4]), fast (state[1]) and slow (state[2])
+// Ca-dependent adaptation; Ca2+ and Na+ currents from partial permeabilities, as floats.
+BC_DEV double ch_tmc1(const ModSlot& m, SegRegs& s, float dt) {
+    constexpr P3 pp = p_partial3(80.0f, 0.21f, 0.0f, 0.79f);
+    float V = s.V;
+    float area = eff_area(m, s);
+    double s4 = m.get(4);
+    double stim = (s4 > 0.0f) ? s4 : 0.0f;
+    float pi = 0.1f + 0.9f * (stim / (stim + 0.3f));
+    double g0 = m.get(0), g1 = m.get(1), g2 = m.get(2);
+    g0 += (pi - g0) * (dt / 1.0f);
+    float ca = s.in[2] * kMmToUm;
+    float ad = ca / (ca + 5.0f);
+    g1 += (ad - g1) * (dt / 2.0f);
+    g2 += (ad - g2) * (dt / 45.0f);
+    g0 = g0 < 0 ? 0.0 : (g0 > 1 ? 1.0 : g0);
+    g1 = g1 < 0 ? 0.0 : (g1 > 1 ? 1.0 : g1);
+    g2 = g2 < 0 ? 0.0 : (g2 > 1 ? 1.0 : g2);
+    m.set(0, g0);
+    m.set(1, g1);
+    m.set(2, g2);
+    double open = g0 * (1.0f - g1) * (1.0f - g2);
+    double N = m.dens * area;
+    double fca = ghk_di(s, V, s.in[2], s.ex[2]);
+    double fna = ghk_mono(s, V, s.in[0], s.ex[0]);
+    float ica = (float)(pp.ca * fca * N * open * kAtoPa);
+    float ina = (float)(pp.na * fna * N * open * kAtoPa);
+    s.in[2] -= (ica * dt) / (2.0f * s.cpm);
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.ex[2] += (ica * dt) / (2.0f * s.cpm * kEvr);
+    s.ex[0] += (ina * dt) / (s.cpm * kEvr);
+    floor7(s.in);
+    floor7(s.ex);
+    return -(ica + ina);
+}
+
 // AQP4 (186, hh_biological.cpp:4114-4129, modifier_helpers.h:395-396): water follows the osmotic
 // difference of the summed (float) concentrations and dilutes every intracellular ion by a
 // factor clamped to [0.95, 1]; no current, no floor.
@@ -2292,6 +2329,7 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 146: return ch_cacc(m, s, dt);
         case 152: return ch_can(m, s, dt);
         case 186: return ch_aqp4(m, s, dt);
+        case 190: return ch_tmc1(m, s, dt);
         case M_PUMP_A3: return ch_pump<PUMP_A3>(m, s, dt);
         case M_PUMP_GLIA: return ch_pump<PUMP_GLIA>(m, s, dt);
         case M_PMCA: return ch_pmca(m, s, dt);

@@ -1784,6 +1784,41 @@ static double chan_can(Modifier* m, Segment* s, float dt) {
     return -(ina - ik);
 }
 
+// TMC1 (190, hh_biological.cpp tmc1_current): hair-cell transduction channel, open fraction
+// relaxes to 0.1 + 0.9 stim/(stim + 0.3) (stim = state[4]), fast (state[1]) and slow (state[2])
+// Ca-dependent adaptation; Ca2+ and Na+ currents from partial permeabilities, as floats.
+static double chan_tmc1(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    float area = eff_area(m, s);
+    double stim = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    float pi = 0.1f + 0.9f * (stim / (stim + 0.3f));
+    m->state[0] += (pi - m->state[0]) * (dt / 1.0f);
+    float ca = s->intra_ions[2] * kMmToUm;
+    float ad = ca / (ca + 5.0f);
+    m->state[1] += (ad - m->state[1]) * (dt / 2.0f);
+    m->state[2] += (ad - m->state[2]) * (dt / 45.0f);
+    for (int q = 0; q < 3; q++) {
+        if (m->state[q] < 0) m->state[q] = 0;
+        if (m->state[q] > 1) m->state[q] = 1;
+    }
+    double open = m->state[0] * (1.0f - m->state[1]) * (1.0f - m->state[2]);
+    double pna, pk, pca;
+    p_partial(80.0f, 0.21f, 0.0f, 0.79f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double fca = ghk_di(V, s->intra_ions[2], s->extra_ions[2]);
+    double fna = ghk_mono(V, s->intra_ions[0], s->extra_ions[0]);
+    float ica = (float)(pca * fca * N * open * kAtoPa);
+    float ina = (float)(pna * fna * N * open * kAtoPa);
+    double cpm = kFaraday * volume(s) / 1000.0f;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->extra_ions[2] += (ica * dt) / (2.0f * cpm * kEvr);
+    s->extra_ions[0] += (ina * dt) / (cpm * kEvr);
+    floor_ions(s->intra_ions);
+    floor_ions(s->extra_ions);
+    return -(ica + ina);
+}
+
 // AQP4 (186, hh_biological.cpp:4114-4129, modifier_helpers.h:395-396): water follows the
 // osmotic difference of the summed (float) concentrations and dilutes every intracellular ion
 // by a factor clamped to [0.95, 1]; no current, no floor.
@@ -2107,6 +2142,7 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 146: return chan_cacc(m, s, dt);
         case 152: return chan_can(m, s, dt);
         case 186: return chan_aqp4(m, s, dt);
+        case 190: return chan_tmc1(m, s, dt);
         case T_PUMP_A3: return chan_pump(m, s, dt, PUMP_A3);
         case T_PUMP_GLIA: return chan_pump(m, s, dt, PUMP_GLIA);
         case T_PMCA: return chan_pmca(m, s, dt);
