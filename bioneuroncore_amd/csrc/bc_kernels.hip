@@ -249,6 +249,47 @@ __device__ __attribute__((always_inline)) inline bool tile_all(bool pred) {
     return __syncthreads_and(pred) != 0;
 }
 
+// Exact early end of the fixed-count (50-sweep) Jacobi loops. A sweep is a pure function of the
+// tile's iterate: every other input (diagonal, conductances, right-hand side) is fixed across the
+// sweeps, and islands never cross a tile. So when sweep j returns x_j equal to x_{j-1} bit for bit
+// on every lane of the tile, every later sweep returns x_j; when x_j equals x_{j-2}, the iterates
+// alternate x_j, x_{j-1} from then on and sweep 50 returns x_j for even 50 - j, else x_{j-1}.
+// `it` is j - 1. d1: this lane's x_j differs from x_{j-1}; d2: from x_{j-2}. The result is
+// uniform over the tile: 0 go on, 1 fixed point, 2 two-cycle.
+// For tiles of several wavefronts, each wavefront publishes its two ballots in sflag (two
+// sweep-parity buffers of TS/64 words) and the barrier that ends the sweep orders them before
+// every wavefront reads all of them; a buffer is rewritten two sweeps later, after the next
+// barrier, when every wavefront has read it. The TS = 64 form takes no barrier and no sflag.
+template <int TS>
+__device__ __attribute__((always_inline)) inline int sweep_settled(bool d1, bool d2, int it,
+                                                                   uint32_t* sflag = nullptr) {
+    const uint64_t b1 = __ballot(d1), b2 = __ballot(d2);
+    uint32_t any = (b1 != 0) | ((b2 != 0) << 1);
+    if (TS > 64) {
+        uint32_t* f = sflag + (it & 1) * (TS / 64);
+        if ((threadIdx.x & 63) == 0) f[threadIdx.x >> 6] = any;
+        __syncthreads();
+        any = 0;
+#pragma unroll
+        for (int w = 0; w < TS / 64; w++) any |= f[w];
+    }
+    if (!(any & 1)) return 1;
+    if (it >= 1 && !(any & 2)) return 2;
+    return 0;
+}
+BC_DEV bool bits_ne(double a, double b) { return __double_as_longlong(a) != __double_as_longlong(b); }
+BC_DEV bool bits_ne(float a, float b) { return __float_as_uint(a) != __float_as_uint(b); }
+// Register form for a one-wavefront tile: nx = x_j, p1 = x_{j-1}, p2 = x_{j-2}.
+template <class T>
+__device__ __attribute__((always_inline)) inline int sweep_period(T nx, T p1, T p2, int it) {
+    return sweep_settled<64>(bits_ne(nx, p1), bits_ne(nx, p2), it);
+}
+// The value sweep 50 would return, given the settled state found after sweep j = it + 1.
+template <class T>
+__device__ __attribute__((always_inline)) inline T sweep_final(int settled, T nx, T p1, int it) {
+    return (settled == 2 && ((50 - (it + 1)) & 1)) ? p1 : nx;
+}
+
 // kRes: residual-terminated sweeps (an extension; the reference sweeps 50 times). Each island
 // (cell) stops after the first sweep whose largest update is <= d.jac_tol, or after d.jac_max
 // sweeps; the islands' residuals are max-reduced in LDS beyond the tile's DiffLds.
@@ -275,6 +316,11 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
     const size_t P = d.npad;
     const uint32_t E0 = d.csr_off[s0];
     const uint32_t nent = d.csr_off[s0 + nseg] - E0;
+    uint32_t* sflag = nullptr;   // settle flags of the sweeps (tiles of several wavefronts)
+    if constexpr (TS > 64) {
+        __shared__ uint32_t sflag_[2 * (TS / 64)];
+        sflag = sflag_;
+    }
 
     double ibuf = 0.0, diag = 0.0, sg = 0.0, vcur = 0.0;
     float area = 0.0f, cap = 0.0f, volf = 0.0f;
@@ -405,7 +451,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 int pa[D];
 #pragma unroll
                 for (int q = 0; q < D; q++) pa[q] = (int)((q < pdeg ? po[q] : t) * 4);
-                double pi = 0.0;
+                double pi = 0.0, pp = 0.0;   // x_{j-1}, x_{j-2}
                 for (int it = 0; it < 50; it++) {
                     const int lo = (int)(uint32_t)__double_as_longlong(pi);
                     const int hi = (int)(uint32_t)((uint64_t)__double_as_longlong(pi) >> 32);
@@ -425,7 +471,14 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     double ph = (diag > 1e-9) ? div_rcp(pn - ibuf, diag, ydiag) : 0.0;
                     if (ph > 10.0) ph = 10.0;
                     if (ph < -10.0) ph = -10.0;
-                    pi = act ? ph : pi;
+                    const double nx = act ? ph : pi;
+                    const int pfx = sweep_period(nx, pi, pp, it);
+                    if (pfx) {
+                        pi = sweep_final(pfx, nx, pi, it);
+                        break;
+                    }
+                    pp = pi;
+                    pi = nx;
                 }
                 if (act) L.u.p.phi[0][t] = pi;
                 cur = 0;
@@ -433,7 +486,9 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 return;
             }
         }
+        double p1 = 0.0, p2 = 0.0;   // this lane's x_{j-1}, x_{j-2}
         for (int it = 0; it < 50; it++) {
+            double ph = p1;
             if (act) {
                 double pv[D];
 #pragma unroll
@@ -447,13 +502,21 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     if (gg <= 0.0) continue;
                     pn += gg * L.u.p.phi[cur][ent_other(L, en)];
                 }
-                double ph = (diag > 1e-9) ? div_rcp(pn - ibuf, diag, ydiag) : 0.0;
+                ph = (diag > 1e-9) ? div_rcp(pn - ibuf, diag, ydiag) : 0.0;
                 if (ph > 10.0) ph = 10.0;
                 if (ph < -10.0) ph = -10.0;
                 L.u.p.phi[cur ^ 1][t] = ph;
             }
-            tile_sync<TS>();
+            // the reduction is also the barrier before the next sweep reads the new plane
+            const int settled = sweep_settled<TS>(bits_ne(ph, p1), bits_ne(ph, p2), it, sflag);
+            if (TS == 64) tile_sync<TS>();
             cur ^= 1;
+            if (settled) {   // phi[cur] holds x_j, phi[cur ^ 1] x_{j-1}
+                if (sweep_final(settled, 1, 0, it) == 0) cur ^= 1;
+                break;
+            }
+            p2 = p1;
+            p1 = ph;
         }
     };
     if (d.cable) {   // phi_e = 0 (both planes hold zeros)
@@ -606,7 +669,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     int ja[D];
 #pragma unroll
                     for (int q = 0; q < D; q++) ja[q] = (int)((q < jdeg ? jo[q] : t) * 4);
-                    float vi = vold;
+                    float vi = vold, vp = vold;   // x_{j-1}, x_{j-2}
                     for (int it = 0; it < 50; it++) {
                         float jv[D];
 #pragma unroll
@@ -618,15 +681,27 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
 #pragma unroll
                         for (int q = 0; q < D; q++) off = off + jg[q] * jv[q];
                         float nv = div_wide(base + off, den, rden);
-                        vi = act ? nv : vi;
-                        if (kRec && act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
+                        const float nx = act ? nv : vi;
+                        if (kRec) {   // the adjoint's recording keeps every sweep
+                            if (act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
+                        } else {
+                            const int settled = sweep_period(nx, vi, vp, it);
+                            if (settled) {
+                                vi = sweep_final(settled, nx, vi, it);
+                                break;
+                            }
+                        }
+                        vp = vi;
+                        vi = nx;
                     }
                     vnew = vi;
                     return;
                 }
             }
             int vc = 0;
+            float vp = vnew;   // this lane's x_{j-2} (vnew holds x_{j-1})
             for (int it = 0; it < 50; it++) {
+                float nx = vnew;
                 if (act) {
                     float jv[D];
 #pragma unroll
@@ -640,11 +715,23 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                         if (!ent_intra(en)) continue;
                         off += L.u.i.cg[c] * L.u.i.vf[vc][ent_other(L, en)];
                     }
-                    vnew = div_wide(base + off, den, rden);
-                    L.u.i.vf[vc ^ 1][t] = vnew;
-                    if (kRec) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = vnew;
+                    nx = div_wide(base + off, den, rden);
+                    L.u.i.vf[vc ^ 1][t] = nx;
+                    if (kRec) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nx;
                 }
-                tile_sync<TS>();
+                if (kRec) {   // the adjoint's recording keeps every sweep
+                    tile_sync<TS>();
+                } else {
+                    // the reduction is also the barrier before the next sweep reads the new plane
+                    const int settled = sweep_settled<TS>(bits_ne(nx, vnew), bits_ne(nx, vp), it, sflag);
+                    if (TS == 64) tile_sync<TS>();
+                    if (settled) {
+                        vnew = sweep_final(settled, nx, vnew, it);
+                        break;
+                    }
+                }
+                vp = vnew;
+                vnew = nx;
                 vc ^= 1;
             }
         };

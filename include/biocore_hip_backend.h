@@ -26,7 +26,8 @@ void* gpu_backend_create(void);
 void* gpu_backend_create_on_device(int device_index);
 // NULL-safe                                                      (gpu_backend_cuda.cu:334-336)
 void gpu_backend_destroy(void* sim);
-// 0 ok; -1 NULL args; BION_ERR_* loader codes (bion_loader_host.cpp:14-24); v14 files only;
+// 0 ok; -1 NULL args; BION_ERR_* loader codes (bion_loader_host.cpp:14-24); versions 3-14
+// (older records migrated as bion_loader_host.cpp:99-340 does);
 // BCHIP_ERR_ISLAND_TOO_LARGE when a connected island exceeds one workgroup tile
 //                                                                (gpu_backend_cuda.cu:338-341)
 int gpu_backend_load_bion(void* sim, const char* path);
