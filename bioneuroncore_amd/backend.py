@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libb
 COHERENCE_COMPAT = 0
 COHERENCE_FAST = 1
 ERR_ISLAND_TOO_LARGE = -20
+ERR_UNSUPPORTED = -21
 
 STIM_DT = np.dtype([
     ("current_stim_count", "<u4"), ("vclamp_count", "<u4"), ("ramp_count", "<u4"),
@@ -60,7 +61,8 @@ class BchipStats(C.Structure):
                 ("num_tiles", C.c_uint32), ("modifier_slots", C.c_uint32), ("receptor_slots", C.c_uint32),
                 ("lds_bytes_per_tile", C.c_uint32), ("_pad", C.c_uint32),
                 ("unsupported_modifiers", C.c_uint64), ("unsupported_synapses", C.c_uint64),
-                ("order_hazard_synapses", C.c_uint64), ("last_phase_ms", C.c_float * 6)]
+                ("order_hazard_synapses", C.c_uint64), ("last_phase_ms", C.c_float * 6),
+                ("ordered_events", C.c_uint32), ("ordered_segments", C.c_uint32)]
 
 
 class BchipSchedule(C.Structure):
@@ -127,6 +129,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_set_adjoint_memory.argtypes = [P, C.c_uint64]
     lib.bchip_set_stimulus_convention.argtypes = [P, C.c_int]
     lib.bchip_set_protein_mapping.argtypes = [P, C.c_int]
+    lib.bchip_allow_unsupported.argtypes = [P, C.c_int]
     _lib = lib
     return lib
 
@@ -178,6 +181,11 @@ class Simulation:
 
     def load_bion(self, path: str) -> int:
         return int(self.lib.gpu_backend_load_bion(self.h, path.encode()))
+
+    def allow_unsupported(self, on: bool = True) -> None:
+        """Load networks that need work the device does not do (off: such loads return
+        ERR_UNSUPPORTED)."""
+        self.lib.bchip_allow_unsupported(self.h, 1 if on else 0)
 
     def load_network(self, net: bion.Network, nuclei: np.ndarray | None = None) -> int:
         hdr = bion.make_header(net)

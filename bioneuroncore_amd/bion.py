@@ -367,6 +367,47 @@ def gap_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Netw
     return net
 
 
+def hazard_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6, glia: bool = False) -> Network:
+    """A recurrent network whose step-3 results depend on synapse order across post chains:
+    somas (the presynaptic segments) receive AMPA+NMDA / GABA-A synapses; a third of all
+    chemical synapses release immediately (delay 0, or 0.01 ms, which rounds to 0 ticks at
+    dt = 0.025 ms), so their vesicle recovery reads a presynaptic Ca that earlier synapses onto
+    that soma changed; soma-to-soma gap junctions (filtered and not) are interleaved in index
+    order behind those synapses. With `glia`, two glial (GLAST) segments capture cleft
+    glutamate and GABA."""
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN], [CAV21, CX36, CX43],
+                        8.0, "ampa_nmda_gaba", 3, seed)
+    rng = np.random.default_rng(seed + 7)
+    y = net.synapses
+    onto_soma = rng.random(len(y)) < 0.6
+    y["post_segment"][onto_soma] = rng.integers(0, n_neurons, int(onto_soma.sum())) * seg_per_neuron
+    now = rng.random(len(y)) < 0.35
+    y["delay"][now] = np.where(rng.random(int(now.sum())) < 0.7, 0.0, 0.01).astype(np.float32)
+    e = []
+    for k in range(n_neurons - 1):
+        e.append((k * seg_per_neuron, (k + 1) * seg_per_neuron, 1, 0.5 if k % 2 else 0.0))
+    gj = np.zeros(len(e), SYNAPSE_DT)
+    gj["pre_segment"] = [x[0] for x in e]
+    gj["post_segment"] = [x[1] for x in e]
+    gj["type"] = [x[2] for x in e]
+    gj["delay"] = [x[3] for x in e]
+    gj["depression"] = rng.uniform(0.0, 0.3, len(e)).astype(np.float32)
+    gj["weight"] = 1.0
+    syn = np.concatenate([y, gj])
+    syn = syn[rng.permutation(len(syn))]
+    syn["id"] = np.arange(1, len(syn) + 1, dtype=np.uint32)
+    net.synapses = syn
+    if glia:   # two dendritic segments become glial capture sites, segments spread over depth
+        s = net.segments
+        s["position"][:, 2] = rng.uniform(0.0, 400.0, len(s)).astype(np.float32)
+        for g in (2, seg_per_neuron + 3):
+            c = s["modifier_count"][g]
+            s["modifiers"][g, c]["type"] = GLAST
+            s["modifiers"][g, c]["density"] = CANON_DENSITY.get(GLAST, 1.0)
+            s["modifier_count"][g] = c + 1
+    return net
+
+
 def glia_network(seed: int, n_neurons: int = 16, seg_per_neuron: int = 6, n_glia: int = 5,
                  glia_first: bool = False) -> Network:
     """Chain neurons with AMPA+NMDA / GABA-A synapses and neuronal transporters (EAAT3, GlyT2,

@@ -98,6 +98,7 @@ struct Sim {
         size_t cap = 0;
     };
     DBuf sb_cur, sb_ramp, sb_vc, sb_ion, sb_lig, sb_ent, sb_segs[2], sb_heads;
+    DBuf sb_segord, sb_yhaz, sb_oev, sb_ofl, sb_ohist, sb_oglia, sb_opost, sb_oseg;
     uint32_t* stim_head = nullptr;
     std::vector<std::pair<uint64_t, uint32_t>> id2dev;   // (network << 32 | Segment.id, device index), sorted
     // Networks of a batch (a plain load is one network): host index offsets into the union,
@@ -136,6 +137,19 @@ struct Sim {
     int coherence = BCHIP_COHERENCE_COMPAT;
     bool host_stale = false;
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
+    bool allow_unsupported = false;   // bchip_allow_unsupported
+    // Ordered pass planning (see plan_ordered): per slot the Ca-history source, the effective
+    // RRP_max, flags and glial target; per segment the Ca-history base and flags; the chemical
+    // synapses that become hazards when their release is immediate, and the gap junction hazards.
+    std::vector<int32_t> h_src;
+    std::vector<float> h_rmax;
+    std::vector<uint8_t> h_yflags, h_sflags;
+    std::vector<uint32_t> h_glia, h_hist_base, h_cand, h_gaphaz;
+    float ord_dt = NAN;   // dt the ordered pass was planned for (NaN: not planned)
+    uint32_t ord_events = 0, ord_segments = 0;
+    std::vector<uint8_t> o_segord, o_yhaz, o_fl;
+    std::vector<uint32_t> o_ev, o_glia, o_post, o_seg;
+    std::vector<int32_t> o_hist;
     bool receptors_only = true;   // every receptor slot is AMPA, NMDA or GABA-A
     bool nt_read = false;         // some segment modifier reads or takes up transmitters
     bool protein_map = false;     // nucleus proteins set the release parameters (see release_params)
@@ -183,7 +197,8 @@ struct Sim {
         return static_cast<T*>(b.p);
     }
     void free_stim_buffers() {
-        for (DBuf* b : {&sb_cur, &sb_ramp, &sb_vc, &sb_ion, &sb_lig, &sb_ent, &sb_segs[0], &sb_segs[1], &sb_heads}) {
+        for (DBuf* b : {&sb_cur, &sb_ramp, &sb_vc, &sb_ion, &sb_lig, &sb_ent, &sb_segs[0], &sb_segs[1], &sb_heads,
+                        &sb_segord, &sb_yhaz, &sb_oev, &sb_ofl, &sb_ohist, &sb_oglia, &sb_opost, &sb_oseg}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->cap = 0;
@@ -538,6 +553,9 @@ static int build_device(Sim& S) {
     std::vector<float> y_cleft((size_t)15 * ns, 0.0f);
     S.unsupported_syn = 0;
     S.hazard_syn = 0;
+    S.h_cand.clear();
+    S.h_gaphaz.clear();
+    S.ord_dt = NAN;
     S.receptors_only = true;
     uint32_t ngap = 0;
     for (uint32_t k = 0; k < ns; k++) {
@@ -587,9 +605,10 @@ static int build_device(Sim& S) {
         for (uint32_t j = 0; j < post.modifier_count; j++)
             if (post.modifiers[j].type == 86) { mask |= 1u << 9; S.unsupported_syn++; break; }
         y_mask[k] = mask;
-        // Immediate releases read the presynaptic Ca before the post-side pass; exact only
-        // when no earlier synapse targets that presynaptic segment.
-        if (src[k] >= 0 && y.delay <= 0.0f && y.RRP_max > 0.0f) S.hazard_syn++;
+        if (mask & ~3u) sflags[S.host2dev[y.post_segment]] |= SF_NTX;
+        // An immediate release whose vesicle recovery reads a presynaptic Ca that an earlier
+        // synapse may change: the ordered pass completes it (plan_ordered, per dt).
+        if (src[k] >= 0 && !(fl & YF_SKIP)) S.h_cand.push_back(k);
     }
     if (nglia_syn) S.receptors_only = false;   // captures merge into the full post-side walk
 
@@ -716,24 +735,22 @@ static int build_device(Sim& S) {
             y_sid[k] = S.slot2syn[k];
             y_post[k] = S.host2dev[sy[S.slot2syn[k]].post_segment];
         }
-        // the first synapse index at which each segment's Ca can change in step 3 (a receptor
-        // other than AMPA or GABA-A on an incoming synapse)
+        // the first synapse index at which each segment's Ca can change in step 3: an incoming
+        // chemical synapse with a receptor (every receptor call floors the ions at 1e-9,
+        // modifier_helpers.h:1117-1121, and NMDA-type ones move Ca)
         std::vector<uint32_t> first_ca(n, UINT32_MAX);
         for (uint32_t i = 0; i < ns; i++) {
             const Synapse& y = sy[i];
-            if (y.type != 0) continue;
-            for (uint32_t j = 0; j < y.post_receptor_count; j++)
-                if (y.post_receptors[j].type != M_AMPA && y.post_receptors[j].type != M_GABAA) {
-                    uint32_t& f = first_ca[S.host2dev[y.post_segment]];
-                    f = std::min(f, i);
-                }
+            if (y.type == 1 || y.type == 2 || y.post_receptor_count == 0) continue;
+            uint32_t& f = first_ca[S.host2dev[y.post_segment]];
+            f = std::min(f, i);
         }
         std::vector<std::vector<std::pair<uint32_t, uint32_t>>> out(n);   // (synapse, slot)
         for (uint32_t k = 0; k < ns; k++) {
             if (!(y_flags[k] & YF_GAP)) continue;
             const uint32_t i = S.slot2syn[k];
             out[y_pre[k]].push_back({i, k});
-            if (first_ca[y_pre[k]] < i || first_ca[y_post[k]] < i) S.hazard_syn++;
+            if (first_ca[y_pre[k]] < i || first_ca[y_post[k]] < i) S.h_gaphaz.push_back(k);
         }
         for (uint32_t s = 0; s < n; s++) {
             std::sort(out[s].begin(), out[s].end());
@@ -783,7 +800,105 @@ static int build_device(Sim& S) {
     d.d_nt = S.dalloc<uint8_t>((size_t)4 * ns);
     S.lds_bytes = sizeof(DiffLds<256>);
     S.seg_dispatched.assign((size_t)n * 32, 0);
+    // kept for planning the ordered pass per dt
+    S.h_src = std::move(src);
+    S.h_rmax = std::move(y_rmax);
+    S.h_yflags = std::move(y_flags);
+    S.h_sflags = std::move(sflags);
+    S.h_hist_base = std::move(hist_base);
+    if (nglia_syn) S.h_glia = std::move(y_glia);
+    else S.h_glia.clear();
+    // hazards at any dt: the gap junctions, and the candidates whose release is always immediate
+    S.hazard_syn = S.h_gaphaz.size();
+    for (uint32_t k : S.h_cand)
+        if (sy[S.slot2syn[k]].delay <= 0.0f && S.h_rmax[k] > 0.0f) S.hazard_syn++;
+    d.seg_ord = nullptr; d.y_haz = nullptr;
+    d.ord_ev = nullptr; d.ord_fl = nullptr; d.ord_hist = nullptr; d.ord_glia = nullptr; d.ord_post = nullptr;
+    d.ord_seg = nullptr; d.ord_n = 0; d.ord_nseg = 0;
     return 0;
+}
+
+// ------------------------------------------------------------------ ordered pass planning
+static uint64_t host_delay_ticks(float delay, float dt) {   // delay_buffer.h:61-64
+    if (delay <= 0.0f || dt <= 0.0f) return 0;
+    return (uint64_t)((delay / dt) + 0.5f);
+}
+
+// The hazards at this dt, the ordered segments they make and the events touching those, in
+// synapse-index order (k_syn_ordered). Runs when dt differs from the last plan; with no hazard
+// the ordered pass is off (null pointers).
+static void plan_ordered(Sim& S, float dt) {
+    if (S.ord_dt == dt) return;
+    Net& d = S.d;
+    const BionData& B = S.host;
+    const uint32_t n = d.n, ns = d.ns;
+    S.ord_dt = dt;
+    d.seg_ord = nullptr; d.y_haz = nullptr;
+    d.ord_ev = nullptr; d.ord_fl = nullptr; d.ord_hist = nullptr; d.ord_glia = nullptr; d.ord_post = nullptr;
+    d.ord_seg = nullptr; d.ord_n = 0; d.ord_nseg = 0;
+    S.ord_events = 0;
+    S.ord_segments = 0;
+    if (!ns) return;
+    std::vector<uint8_t>& haz = S.o_yhaz;
+    std::vector<uint8_t>& ser = S.o_segord;
+    haz.assign(ns, 0);
+    ser.assign(n, 0);
+    uint64_t nhaz = 0;
+    for (uint32_t k : S.h_cand) {
+        const Synapse& y = B.synapses[S.slot2syn[k]];
+        if (!(S.h_rmax[k] > 0.0f) || host_delay_ticks(y.delay, dt) != 0) continue;
+        haz[k] = HZ_RELEASE;
+        ser[S.host2dev[y.post_segment]] = 1;
+        if (!S.h_glia.empty() && S.h_glia[k] != UINT32_MAX) ser[S.h_glia[k]] = 1;
+        nhaz++;
+    }
+    for (uint32_t k : S.h_gaphaz) {
+        const Synapse& y = B.synapses[S.slot2syn[k]];
+        haz[k] = HZ_GAP;
+        ser[S.host2dev[y.pre_segment]] = 1;
+        ser[S.host2dev[y.post_segment]] = 1;
+        nhaz++;
+    }
+    S.hazard_syn = nhaz;
+    if (!nhaz) return;
+    std::vector<uint32_t> syn2slot(ns), rank(n, 0);
+    for (uint32_t k = 0; k < ns; k++) syn2slot[S.slot2syn[k]] = k;
+    S.o_ev.clear(); S.o_fl.clear(); S.o_hist.clear(); S.o_glia.clear(); S.o_post.clear(); S.o_seg.clear();
+    for (uint32_t i = 0; i < ns; i++) {
+        const uint32_t k = syn2slot[i];
+        const Synapse& y = B.synapses[i];
+        const uint32_t post = S.host2dev[y.post_segment], pre = S.host2dev[y.pre_segment];
+        const uint32_t q = rank[post]++;   // position in the post segment's chain
+        const bool gap = S.h_yflags[k] & YF_GAP;
+        const uint32_t g = S.h_glia.empty() ? UINT32_MAX : S.h_glia[k];
+        uint8_t f = 0;
+        if (ser[post]) f |= EV_POST;
+        if (gap && ser[pre]) f |= EV_PREGAP;
+        if (g != UINT32_MAX && ser[g]) f |= EV_GLIA;
+        if (haz[k] == HZ_RELEASE) f |= EV_HAZ;
+        if (haz[k] == HZ_GAP) f |= EV_GAPHAZ;
+        if (!f) continue;
+        S.o_ev.push_back(k);
+        S.o_fl.push_back(f);
+        S.o_hist.push_back((f & EV_POST) && (S.h_sflags[post] & SF_WATCHED) ? (int32_t)(S.h_hist_base[post] + q) : -1);
+        S.o_glia.push_back(g);
+        S.o_post.push_back(post);
+    }
+    for (uint32_t i = 0; i < n; i++)
+        if (ser[i]) S.o_seg.push_back(i);
+    d.seg_ord = S.dput(S.sb_segord, ser);
+    d.y_haz = S.dput(S.sb_yhaz, haz);
+    d.ord_ev = S.dput(S.sb_oev, S.o_ev);
+    d.ord_fl = S.dput(S.sb_ofl, S.o_fl);
+    d.ord_hist = S.dput(S.sb_ohist, S.o_hist);
+    d.ord_glia = S.dput(S.sb_oglia, S.o_glia);
+    d.ord_post = S.dput(S.sb_opost, S.o_post);
+    d.ord_seg = S.dput(S.sb_oseg, S.o_seg);
+    d.ord_n = (uint32_t)S.o_ev.size();
+    d.ord_nseg = (uint32_t)S.o_seg.size();
+    S.ord_events = d.ord_n;
+    S.ord_segments = d.ord_nseg;
+    HIPCHK(hipStreamSynchronize(S.stream));
 }
 
 // ------------------------------------------------------------------ coherence
@@ -1039,8 +1154,10 @@ static void launch_tick(Sim& S, float dt, bool tm) {
     }
     if (tm) HIPCHK(hipEventRecord(S.ev[3], S.stream));
     const uint32_t gs = (d.n + 255) / 256;   // the post pass's windows are 256 segments
-    if (S.receptors_only) k_syn_post_rec<<<gs, 256, 0, S.stream>>>(d, dt);
+    if (S.receptors_only && d.nrec <= 2) k_syn_post_rec<true><<<gs, 256, 0, S.stream>>>(d, dt);
+    else if (S.receptors_only) k_syn_post_rec<false><<<gs, 256, 0, S.stream>>>(d, dt);
     else k_syn_post<<<gs, 256, 0, S.stream>>>(d, dt);
+    if (d.ord_n) k_syn_ordered<<<1, 64, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[4], S.stream));
     if (d.ns) k_syn_plast<<<(d.ns + sc.plast_block - 1) / sc.plast_block, sc.plast_block, 0, S.stream>>>(d, dt);
     if (tm) HIPCHK(hipEventRecord(S.ev[5], S.stream));
@@ -1068,6 +1185,7 @@ static void after_ticks(Sim& S, uint32_t count) {
 
 static void run_tick(Sim& S, float dt) {
     before_ticks(S);
+    plan_ordered(S, dt);
     const bool tm = S.timing != 0;
     launch_tick(S, dt, tm);
     HIPCHK(hipGetLastError());
@@ -1091,6 +1209,7 @@ static void run_ticks(Sim& S, float dt, uint32_t count) {
         return;
     }
     before_ticks(S);
+    plan_ordered(S, dt);
     const uint32_t g = std::min(count, kGraphTicks);
     const bool same = S.graph && S.graph_ticks == g && S.graph_dt == dt &&
                       memcmp(&S.graph_net, &S.d, sizeof(Net)) == 0 &&
@@ -1138,6 +1257,18 @@ static int load_into(Sim& S, BionData* src_owned) {
     S.host_stale = false;
     int rc = build_device(S);
     if (rc != 0) { S.free_device(); free_bion(&S.host); S.loaded = false; return rc; }
+    if ((S.unsupported_mod || S.unsupported_syn) && !S.allow_unsupported) {
+        fprintf(stderr,
+                "biocore_hip: load refused: %llu modifier/receptor instance(s) outside the device set and "
+                "%llu synapse(s) with autoreceptors or eCB synthesis (bchip_allow_unsupported loads them "
+                "as zero current / skipped)\n",
+                (unsigned long long)S.unsupported_mod, (unsigned long long)S.unsupported_syn);
+        HIPCHK(hipStreamSynchronize(S.stream));
+        S.free_device();
+        free_bion(&S.host);
+        S.loaded = false;
+        return BCHIP_ERR_UNSUPPORTED;
+    }
     xfer(S, true);
     S.st_segs[0].clear();   // the lists named device segments of the previous network
     S.st_segs[1].clear();
@@ -1747,6 +1878,12 @@ int bchip_set_jacobi(void* sim, double tol, int max_sweeps) {
     return 0;
 }
 
+int bchip_allow_unsupported(void* sim, int on) {
+    if (!sim) return -1;
+    static_cast<Sim*>(sim)->allow_unsupported = on != 0;
+    return 0;
+}
+
 int bchip_set_protein_mapping(void* sim, int on) {
     if (!sim) return -1;
     Sim& S = *static_cast<Sim*>(sim);
@@ -1760,6 +1897,8 @@ int bchip_set_protein_mapping(void* sim, int on) {
     HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_prel), prel.data(), b, hipMemcpyHostToDevice, S.stream));
     HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_prim0), prim0.data(), b, hipMemcpyHostToDevice, S.stream));
     HIPCHK(hipStreamSynchronize(S.stream));
+    S.h_rmax = std::move(rmax);   // the hazards depend on the effective RRP_max
+    S.ord_dt = NAN;
     return 0;
 }
 
@@ -1877,6 +2016,8 @@ int bchip_stats(void* sim, BchipStats* out) {
     out->order_hazard_synapses = S.hazard_syn;
     out->lds_bytes_per_tile = (uint32_t)S.lds_bytes;
     for (int k = 0; k < 6; k++) out->last_phase_ms[k] = S.last_ms[k];
+    out->ordered_events = S.ord_events;
+    out->ordered_segments = S.ord_segments;
     return 0;
 }
 

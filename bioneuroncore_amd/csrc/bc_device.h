@@ -170,7 +170,8 @@ struct ModSlot {
     __device__ void set(int k, double v) const { st[(size_t)k * stride] = v; }
 };
 
-BC_DEV float eff_area(const ModSlot& m, const SegRegs& s) { return (m.a7 > 0.0f) ? m.a7 : s.area; }
+template <class M>
+BC_DEV float eff_area(const M& m, const SegRegs& s) { return (m.a7 > 0.0f) ? m.a7 : s.area; }
 template <int N>
 BC_DEV void floor_n(double* a) {
 #pragma unroll
@@ -754,7 +755,31 @@ BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_
 // ions 4-6 (untouched here) once after its walk, which floor7 per call amounts to. One body with the three parameter sets
 // selected per lane (the same operations as iono<true> with the constants of
 // dispatch(): AMPA and NMDA lanes of a wavefront run it together.
-BC_DEV double dispatch_receptor(uint8_t type, const ModSlot& m, SegRegs& s, float dt) {
+// Receptor slot whose states 0-3 were loaded ahead (the post-side walk issues every load of a
+// chain step at once); a state beyond 3 (a gamma override index) is read from memory.
+struct RegSlot {
+    double st[4];
+    const double* mem;   // state 0 of this receptor, stride `stride`
+    size_t stride;
+    double dens;
+    float a7;
+    uint8_t av;
+    __device__ double get(int k) const {
+        if (k == 0) return st[0];
+        if (k == 1) return st[1];
+        if (k == 2) return st[2];
+        if (k == 3) return st[3];
+        return mem[(size_t)k * stride];
+    }
+    __device__ void set(int k, double v) {
+        if (k == 0) st[0] = v;
+        else if (k == 1) st[1] = v;
+        else st[2] = v;   // the receptors below set states 0, 1 (AMPA, GABA-A) and 2 (NMDA)
+    }
+};
+
+template <class Slot>
+BC_DEV double dispatch_receptor(uint8_t type, Slot& m, SegRegs& s, float dt) {
     const bool nmda = type == M_NMDA, gaba = type == M_GABAA;
     const float gamma0 = nmda ? 50.0f : (gaba ? 28.0f : 8.0f);
     const float ec50 = gaba || nmda ? 1.0f : 0.5f;

@@ -997,6 +997,37 @@ __device__ static void glial_capture(const Net& d, uint32_t y, float dt) {
     d.y_capm[y] = got;
 }
 
+// Immediate release of probability p (simulation_host.cpp:412-456): vesicle recovery with the
+// presynaptic Ca the reference would read at this synapse (rrp_do_recovery, kernel_dispatch.h:
+// 612-631), depletion (rrp_do_release, :636-645) and the cleft deposit; y_relp = -1 marks it done.
+__device__ static void release_now(const Net& d, uint32_t y, float p, float ca, float dt) {
+    const size_t NS = d.ns;
+    float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
+    if (!(rmax <= 0.0f || rrp >= rmax)) {
+        float caf = ca / (0.0005f + ca);
+        float prim = d.y_prim0[y];
+        prim *= caf;
+        float rec = (rmax - rrp) * prim * dt;
+        rrp += rec;
+        if (rrp > rmax) rrp = rmax;
+    }
+    float amt;
+    float amount = d.y_amount[y];
+    if (rmax <= 0.0f) {
+        amt = amount * p;
+    } else {
+        float av = amount;
+        if (av > rrp) av = rrp;
+        amt = av * p;
+        rrp -= amt;
+        if (rrp < 0) rrp = 0;
+    }
+    d.y_rrp[y] = rrp;
+    uint8_t nt = d.y_rcount[y] > 0 ? nt_of_receptor(d.y_rtype[y]) : 0;
+    d.y_cleft[nt * NS + y] += amt;
+    d.y_relp[y] = -1.0f;   // finished here
+}
+
 __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     exp_table_stage();
     const uint64_t tick = *d.tick;
@@ -1004,8 +1035,9 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     const size_t NS = d.ns;
     for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < d.ns; y += gridDim.x * blockDim.x) {
         d.y_relp[y] = 0.0f;
+        const uint8_t hz = d.y_haz ? d.y_haz[y] : 0;   // resolved in the ordered pass
         if (d.y_flags[y] & YF_SKIP) {
-            if (d.y_flags[y] & YF_GAP) gap_junction(d, y, dt);
+            if ((d.y_flags[y] & YF_GAP) && hz != HZ_GAP) gap_junction(d, y, dt);
             continue;
         }
         const float delay = d.y_delay[y];
@@ -1063,47 +1095,53 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
             if (xorshift_draw(seed) < p) {
                 d.y_relp[y] = p;
                 refr = (uint8_t)(uint32_t)(20.0f / dt + 0.5f);
-                if (delay_ticks(delay, dt) == 0) {
-                    // Immediate release: vesicle pool now (pre Ca = value at step-3 start).
-                    const uint32_t pre = d.y_pre[y];
-                    const size_t P = d.npad;
-                    float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
-                    if (!(rmax <= 0.0f || rrp >= rmax)) {
-                        float ca = d.in[2 * P + pre];
-                        float caf = ca / (0.0005f + ca);
-                        float prim = d.y_prim0[y];
-                        prim *= caf;
-                        float rec = (rmax - rrp) * prim * dt;
-                        rrp += rec;
-                        if (rrp > rmax) rrp = rmax;
-                    }
-                    float amt;
-                    float amount = d.y_amount[y];
-                    if (rmax <= 0.0f) {
-                        amt = amount * p;
-                    } else {
-                        float av = amount;
-                        if (av > rrp) av = rrp;
-                        amt = av * p;
-                        rrp -= amt;
-                        if (rrp < 0) rrp = 0;
-                    }
-                    d.y_rrp[y] = rrp;
-                    uint8_t nt = d.y_rcount[y] > 0 ? nt_of_receptor(d.y_rtype[y]) : 0;
-                    d.y_cleft[nt * NS + y] += amt;
-                    d.y_relp[y] = -1.0f;   // finished here
-                }
+                if (delay_ticks(delay, dt) == 0 && hz != HZ_RELEASE)   // pre Ca = value at step-3 start
+                    release_now(d, y, p, (float)d.in[2 * (size_t)d.npad + d.y_pre[y]], dt);
             }
         }
         if (refr > 0) refr--;
         d.y_refr[y] = refr;
-        if (d.y_flags[y] & YF_GLIA) glial_capture(d, y, dt);
+        if ((d.y_flags[y] & YF_GLIA) && hz != HZ_RELEASE) glial_capture(d, y, dt);
     }
+}
+
+// Step 4 of segment i (simulation_host.cpp:596-649): I_buffer clamp, dV = I / C_m * dt clamped
+// to +-50 mV, spike count, voltage clamps in index order (the last active one wins), V clamp.
+__device__ static double step4(const Net& d, uint32_t i, double ibuf, double V, float dt) {
+    if (ibuf != ibuf) ibuf = 0.0f;
+    if (ibuf > 10000.0f) ibuf = 10000.0f;
+    if (ibuf < -10000.0f) ibuf = -10000.0f;
+    float cm = d.cap[i];
+    float prev = V;
+    float dV = ibuf / cm * dt;
+    if (dV > 50.0f) dV = 50.0f;
+    if (dV < -50.0f) dV = -50.0f;
+    V += dV;
+    if (prev <= -40.0f && V > -40.0f) d.spikes[i]++;
+    const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
+    if (sh != ~0u) {
+        const uint64_t tick = *d.tick + d.stim.tick_shift;
+        const uint32_t cnt = d.stim.ent[sh];
+        for (uint32_t e = 1; e <= cnt; e++) {
+            const uint32_t en = d.stim.ent[sh + e];
+            if ((en >> 28) != SK_VCLAMP) continue;
+            const VoltageClamp q = d.stim.vc[en & 0x0FFFFFFFu];
+            if (tick >= q.start_tick && tick < q.end_tick) V = q.target_mV;
+        }
+    }
+    if (V != V) V = -65.0f;
+    else {
+        if (V < -120.0f) V = -120.0f;
+        if (V > 150.0f) V = 150.0f;
+    }
+    return V;
 }
 
 // ------------------------------------------------------------------ step 3 post-side + step 4
 // kReceptorsOnly: every receptor slot holds AMPA, NMDA or GABA-A (a leaner dispatch).
-template <bool kReceptorsOnly>
+// kPre (with kReceptorsOnly, at most two receptors per synapse): each chain step issues all
+// of its loads together.
+template <bool kReceptorsOnly, bool kPre = false>
 __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, float dt) {
     // kStage: the block's 256-segment window is written back through LDS once every lane is
     // done, one coalesced row per field; lanes walk the window in chain-length order and finish
@@ -1118,6 +1156,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
     const uint32_t i = d.walk[lane];
     const uint32_t pos = i - blockIdx.x * blockDim.x;   // the walk permutes inside the window
     const uint8_t fl = d.sflags[i];
+    const bool ord = d.seg_ord && d.seg_ord[i];   // walked (and stepped) by the ordered pass
     double ibuf = d.ibuf[i];
     double V = d.V[i];
     if (d.ns > 0) {
@@ -1129,9 +1168,9 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
         SegRegs s;
         s.V = V;
 #pragma unroll
-        for (int k = 0; k < 7; k++) {
+        for (int k = 0; k < 7; k++) {   // the receptors read ions 0-3 only
             s.in[k] = k < kIonWalk ? d.in[k * P + i] : 1.0;
-            s.ex[k] = d.ex[k * P + i];
+            s.ex[k] = k < kIonWalk ? d.ex[k * P + i] : 1.0;
         }
         s.area = d.area[i];
         s.vol = d.vol[i];
@@ -1159,11 +1198,11 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
         // electrical synapses whose presynaptic end is this segment take their current out of
         // I_buffer in synapse-index order among the chain (simulation_host.cpp:372-400)
         uint32_t go = 0, ge = 0;
-        if (d.gap_off) { go = d.gap_off[i]; ge = d.gap_off[i + 1]; }
+        if (d.gap_off && !ord) { go = d.gap_off[i]; ge = d.gap_off[i + 1]; }
         // glial captures this segment receives, merged in synapse-index order (each synapse
         // captures after its own deposit)
         uint32_t co = 0, ce = 0;
-        if (!kReceptorsOnly && d.glia_off) { co = d.glia_off[i]; ce = d.glia_off[i + 1]; }
+        if (!kReceptorsOnly && d.glia_off && !ord) { co = d.glia_off[i]; ce = d.glia_off[i + 1]; }
         auto take = [&](uint32_t e) {
             const uint32_t sl = d.glia_slot[e];
             const uint8_t m = d.y_capm[sl];
@@ -1176,9 +1215,61 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
             const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
             const uint32_t y = off + wl;
             off += cnt;
+            if (ord) continue;   // the ordered pass walks this chain
             if (go < ge) {   // this segment's junctions as the presynaptic end, up to slot y
                 const uint32_t sid = d.y_sid[y];
                 for (; go < ge && d.gap_sid[go] <= sid; go++) ibuf -= d.y_igap[d.gap_slot[go]];
+            }
+            if constexpr (kPre) {
+                // every load of the step at once (one memory round trip per step): the slot
+                // arrays cover every slot and the receptor arrays d.nrec <= 2 receptors, so the
+                // loads a lane does not use are still in bounds
+                const uint8_t yf = d.y_flags[y];
+                const uint16_t cm = d.y_cleft_mask[y];
+                const float cl0 = d.y_cleft[y], cl1 = d.y_cleft[NS + y];
+                const uint32_t rc = d.y_rcount[y];
+                const float parea = d.y_post_area[y];
+                RegSlot rs[2];
+                uint8_t rt[2];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    rt[j] = 0;
+                    if (j < (int)d.nrec) {
+                        const size_t b = (size_t)j * 7 * NS + y;
+                        rt[j] = d.y_rtype[j * NS + y];
+                        rs[j].av = d.y_rav[j * NS + y];
+                        rs[j].dens = d.y_rdens[j * NS + y];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) rs[j].st[k] = d.y_rstate[b + (size_t)k * NS];
+                        rs[j].mem = d.y_rstate + b;
+                        rs[j].stride = NS;
+                        rs[j].a7 = parea;
+                    }
+                }
+                if (yf & YF_SKIP) {
+                    if (yf & YF_GAP) ibuf += d.y_igap[y];
+                    if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
+                    continue;
+                }
+                if (cm & 1u) s.nt[0] += cl0;
+                if (cm & 2u) s.nt[1] += cl1;
+                float isyn = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    if ((uint32_t)j >= rc) break;
+                    if (my && is_ionotropic(rt[j])) continue;
+                    isyn += dispatch_receptor(rt[j], rs[j], s, dt);
+                    // the receptor set state 0, state 1 (AMPA, GABA-A: desensitisation) or 2 (NMDA)
+                    double* st = d.y_rstate + (size_t)j * 7 * NS + y;
+                    st[0] = rs[j].st[0];
+                    if (rt[j] == M_NMDA) st[2 * NS] = rs[j].st[2];
+                    else st[NS] = rs[j].st[1];
+                    ran = true;
+                }
+                ibuf += isyn;
+                d.y_isyn[y] = isyn;
+                if (fl & SF_WATCHED) d.ca_hist[hb + q] = s.in[2];
+                continue;
             }
             const uint8_t yf = d.y_flags[y];
             if (yf & YF_SKIP) {
@@ -1217,14 +1308,15 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
         }
         for (; go < ge; go++) ibuf -= d.y_igap[d.gap_slot[go]];
         for (; co < ce; co++) take(co);
-        if (kReceptorsOnly) {
-            // the same chain again, for the transmitters the receptors do not read
+        // the same chain again, for the transmitters the receptors do not read; skipped by a
+        // wavefront none of whose segments receives a synapse with such a transmitter live
+        if (kReceptorsOnly && __ballot(fl & SF_NTX) != 0) {
             uint32_t off2 = d.wave_base[lane >> 6];
             for (uint32_t q = 0; q < len; q++) {
                 const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
                 const uint32_t y = off2 + wl;
                 off2 += cnt;
-                if (d.y_flags[y] & YF_SKIP) continue;
+                if (ord || (d.y_flags[y] & YF_SKIP)) continue;
                 const uint32_t cm = d.y_cleft_mask[y];
                 if (!(cm & ~3u)) continue;
 #pragma unroll
@@ -1256,33 +1348,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
         if (fl & SF_BATH_NMDA) d.nt[0 * P + i] = 1.0f;
         if (fl & SF_BATH_5HT) d.nt[3 * P + i] = 0.0f;
     }
-    // ---- step 4 (simulation_host.cpp:596-649) ----
-    if (ibuf != ibuf) ibuf = 0.0f;
-    if (ibuf > 10000.0f) ibuf = 10000.0f;
-    if (ibuf < -10000.0f) ibuf = -10000.0f;
-    float cm = d.cap[i];
-    float prev = V;
-    float dV = ibuf / cm * dt;
-    if (dV > 50.0f) dV = 50.0f;
-    if (dV < -50.0f) dV = -50.0f;
-    V += dV;
-    if (prev <= -40.0f && V > -40.0f) d.spikes[i]++;
-    const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
-    if (sh != ~0u) {   // voltage clamps in index order: the last active one wins
-        const uint64_t tick = *d.tick + d.stim.tick_shift;
-        const uint32_t cnt = d.stim.ent[sh];
-        for (uint32_t e = 1; e <= cnt; e++) {
-            const uint32_t en = d.stim.ent[sh + e];
-            if ((en >> 28) != SK_VCLAMP) continue;
-            const VoltageClamp q = d.stim.vc[en & 0x0FFFFFFFu];
-            if (tick >= q.start_tick && tick < q.end_tick) V = q.target_mV;
-        }
-    }
-    if (V != V) V = -65.0f;
-    else {
-        if (V < -120.0f) V = -120.0f;
-        if (V > 150.0f) V = 150.0f;
-    }
+    if (!ord) V = step4(d, i, ibuf, V, dt);
     if (kStage) o_V[pos] = V;
     else d.V[i] = V;
     }   // lane < n
@@ -1302,15 +1368,127 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
 }
 
 // AMPA/NMDA/GABA-A receptors only: four wavefronts per SIMD fit in the register budget.
+// kPre: at most two receptors per synapse, each chain step's loads issued together.
+template <bool kPre>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_syn_post_rec(
     Net d, float dt) {
     exp_table_stage();
-    syn_post<true>(d, dt);
+    syn_post<true, kPre>(d, dt);
 }
 // Any receptor type (full modifier dispatch).
 __global__ __launch_bounds__(256) void k_syn_post(Net d, float dt) {
     exp_table_stage();
     syn_post<false>(d, dt);
+}
+
+// ------------------------------------------------------------------ step 3, ordered pass
+// One lane walks, in synapse-index order, every event that touches an ordered segment (the
+// post pass skipped those), as the reference's synapse loop does (simulation_host.cpp:356-592):
+// gap junctions computed with both ends' Ca as of their position, currents into and out of
+// I_buffer, immediate releases completed with the presynaptic Ca after the last earlier synapse
+// onto that segment (its Ca-history entry: the parallel post pass wrote the entries of other
+// chains, this walk those of the ordered ones, each before any later synapse reads it), glial
+// captures, the post side with the full receptor dispatch, and the Ca history. Then step 4 of
+// the ordered segments. Segment state stays in memory between events.
+__global__ __launch_bounds__(64) void k_syn_ordered(Net d, float dt) {
+    exp_table_stage();
+    if (threadIdx.x != 0) return;
+    const size_t P = d.npad, NS = d.ns;
+    auto take = [&](uint32_t y, float* nt) {   // a glial capture into transmitters nt[0], nt[7], nt[1]
+        const uint8_t m = d.y_capm[y];
+        if (m & 1) nt[0] += d.y_cap[0 * NS + y];
+        if (m & 2) nt[7] += d.y_cap[1 * NS + y];
+        if (m & 4) nt[1] += d.y_cap[2 * NS + y];
+    };
+    for (uint32_t e = 0; e < d.ord_n; e++) {
+        const uint32_t y = d.ord_ev[e];
+        const uint8_t f = d.ord_fl[e];
+        const uint8_t yf = d.y_flags[y];
+        const uint32_t post = d.ord_post[e];
+        const int32_t h = d.ord_hist[e];
+        if (yf & YF_SKIP) {
+            if (yf & YF_GAP) {
+                if (f & EV_GAPHAZ) gap_junction(d, y, dt);
+                const float ig = d.y_igap[y];
+                if (f & EV_PREGAP) d.ibuf[d.y_pre[y]] -= ig;
+                if (f & EV_POST) d.ibuf[post] += ig;
+            }
+            if (h >= 0) d.ca_hist[h] = d.in[2 * P + post];
+            continue;
+        }
+        if (f & EV_HAZ) {
+            const float p = d.y_relp[y];
+            if (p > 0.0f) release_now(d, y, p, (float)d.ca_hist[d.y_src[y]], dt);
+            if (yf & YF_GLIA) glial_capture(d, y, dt);
+        }
+        const bool glia_here = (f & EV_GLIA) && d.ord_glia[e] == post;
+        if (f & EV_POST) {
+            SegRegs s;
+            s.V = d.V[post];
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                s.in[k] = d.in[k * P + post];
+                s.ex[k] = d.ex[k * P + post];
+            }
+            s.area = d.area[post];
+            s.vol = d.vol[post];
+            s.atp = d.atp[post];
+            s.pkc = d.pkc[post];
+            s.pka = d.pka[post];
+            s.camp = d.camp[post];
+            s.camkii = d.camkii[post];
+            s.camkii01 = d.camkii01[post];
+            seg_consts_init(s);
+            ghk_cache_init(s);
+            mg_cache_init(s);
+#pragma unroll
+            for (int k = 0; k < 15; k++) s.nt[k] = d.nt[k * P + post];
+            const uint16_t cm = d.y_cleft_mask[y];
+#pragma unroll
+            for (int k = 0; k < 15; k++)
+                if (cm & (1u << k)) s.nt[k] += d.y_cleft[k * NS + y];
+            if (glia_here) take(y, s.nt);
+            const bool my = d.sflags[post] & SF_MYELIN;
+            float isyn = 0.0f;
+            const uint32_t rc = d.y_rcount[y];
+            for (uint32_t j = 0; j < rc; j++) {
+                uint8_t rt = d.y_rtype[j * NS + y];
+                if (my && is_ionotropic(rt)) continue;
+                ModSlot m;
+                m.st = d.y_rstate + (size_t)j * 7 * NS + y;
+                m.stride = NS;
+                m.dens = d.y_rdens[j * NS + y];
+                m.a7 = d.y_post_area[y];
+                m.av = d.y_rav[j * NS + y];
+                isyn += dispatch(rt, m, s, dt);
+            }
+            d.ibuf[post] += isyn;
+            d.y_isyn[y] = isyn;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                d.in[k * P + post] = s.in[k];
+                d.ex[k * P + post] = s.ex[k];
+            }
+            d.atp[post] = s.atp;
+#pragma unroll
+            for (int k = 0; k < 15; k++) d.nt[k * P + post] = s.nt[k];
+            if (h >= 0) d.ca_hist[h] = s.in[2];
+        }
+        if ((f & EV_GLIA) && !((f & EV_POST) && glia_here)) {
+            const uint32_t g = d.ord_glia[e];
+            float nt[15];
+#pragma unroll
+            for (int k = 0; k < 15; k++) nt[k] = d.nt[k * P + g];
+            take(y, nt);
+            d.nt[0 * P + g] = nt[0];
+            d.nt[1 * P + g] = nt[1];
+            d.nt[7 * P + g] = nt[7];
+        }
+    }
+    for (uint32_t k = 0; k < d.ord_nseg; k++) {
+        const uint32_t i = d.ord_seg[k];
+        d.V[i] = step4(d, i, d.ibuf[i], d.V[i], dt);
+    }
 }
 
 // ------------------------------------------------------------------ step 3, synapse plasticity

@@ -28,6 +28,7 @@ enum : uint8_t {
     SF_BATH_5HT = 4,      // carries a 5-HT7 segment modifier: serotonin bath 0.0
     SF_WATCHED = 8,       // is the presynaptic segment of some synapse
     SF_NTREAD = 16,       // a modifier reads (receptors) or takes up (transporters) transmitters
+    SF_NTX = 32,          // an incoming synapse carries a cleft transmitter other than glutamate/GABA
 };
 
 // Synapse flag bits (static).
@@ -215,7 +216,32 @@ struct Net {
     const uint32_t* glia_slot;   //   their slots
     const uint32_t* glia_sid;    //   their synapse indices (ascending per segment)
 
+    // Ordered pass (k_syn_ordered; null / 0 when no synapse needs it). A synapse whose effect
+    // depends on another post chain's progress within step 3 - an immediate release whose
+    // vesicle recovery reads a presynaptic Ca that an earlier synapse's receptors change, or a
+    // gap junction behind such a synapse - makes its post chain (and the segments its effect
+    // reaches) "ordered": the parallel post pass skips them, and one lane then walks every
+    // event touching them in synapse-index order.
+    const uint8_t* seg_ord;      // per segment: 1 = walked by the ordered pass
+    const uint8_t* y_haz;        // per slot: HZ_RELEASE or HZ_GAP (resolved in the ordered pass)
+    const uint32_t* ord_ev;      // events in synapse-index order: the slot
+    const uint8_t* ord_fl;       //   EV_* flags
+    const int32_t* ord_hist;     //   Ca-history entry written after the post side (-1: none)
+    const uint32_t* ord_glia;    //   glial segment receiving the capture (EV_GLIA)
+    const uint32_t* ord_post;    //   the post segment (device index)
+    const uint32_t* ord_seg;     // the ordered segments, for step 4 after the events
+    uint32_t ord_n, ord_nseg;
+
     float nt_diff[kNtCount];     // transmitter diffusion coefficients
+};
+
+enum : uint8_t { HZ_RELEASE = 1, HZ_GAP = 2 };
+enum : uint8_t {
+    EV_POST = 1,     // the post segment is ordered: cleft, captures, receptors, I_buffer, Ca history
+    EV_PREGAP = 2,   // gap junction whose presynaptic end is ordered: its current leaves that end
+    EV_GLIA = 4,     // the capture goes to an ordered glial segment
+    EV_HAZ = 8,      // immediate release completed here (vesicle recovery with the Ca history)
+    EV_GAPHAZ = 16,  // gap junction computed here, with both ends' Ca as of its position
 };
 
 }  // namespace bc

@@ -28,7 +28,8 @@ void* gpu_backend_create_on_device(int device_index);
 void gpu_backend_destroy(void* sim);
 // 0 ok; -1 NULL args; BION_ERR_* loader codes (bion_loader_host.cpp:14-24); versions 3-14
 // (older records migrated as bion_loader_host.cpp:99-340 does);
-// BCHIP_ERR_ISLAND_TOO_LARGE when a connected island exceeds one workgroup tile
+// BCHIP_ERR_ISLAND_TOO_LARGE when a connected island exceeds one workgroup tile;
+// BCHIP_ERR_UNSUPPORTED when the network needs work the device does not do (see below)
 //                                                                (gpu_backend_cuda.cu:338-341)
 int gpu_backend_load_bion(void* sim, const char* path);
 // One tick (steps 1, 2.5, 3, 4); synchronous on return; NULL-safe (gpu_backend_cuda.cu:343-346)
@@ -56,15 +57,29 @@ void register_modulator_release(uint8_t type, ReleaseModulatorFunc func);   // p
 #define BCHIP_COHERENCE_FAST 1     // AoS refreshed only by gpu_backend_bion_data / bchip_save_bion;
                                    // host edits reach the device only via bchip_push_host_state
 #define BCHIP_ERR_ISLAND_TOO_LARGE -20
+// A load (gpu_backend_load_bion, bchip_load_bion_data, bchip_load_batch) returns this when the
+// network needs work the device does not do - modifier or receptor types outside the device set
+// (BchipStats.unsupported_modifiers), presynaptic autoreceptors or eCB synthesis
+// (unsupported_synapses) - instead of running those parts as zero current; a warning names the
+// counts on stderr and bchip_stats reports them. bchip_allow_unsupported(sim, 1) loads such
+// networks anyway (those parts then do nothing on the device).
+#define BCHIP_ERR_UNSUPPORTED -21
+int bchip_allow_unsupported(void* sim, int on);
 
 typedef struct BchipStats {
     uint32_t num_segments, num_connections, num_synapses, num_tiles;
     uint32_t modifier_slots, receptor_slots, lds_bytes_per_tile, _pad;
     uint64_t unsupported_modifiers;   // modifier/receptor instances outside the device set (run as 0 current)
     uint64_t unsupported_synapses;    // electrical synapses, autoreceptors, eCB (not run on device)
-    uint64_t order_hazard_synapses;   // immediate-release synapses whose pre segment is an earlier post
+    uint64_t order_hazard_synapses;   // synapses whose step-3 effect depends on another post chain's
+                                      // progress (immediate releases behind a synapse onto their
+                                      // presynaptic segment, gap junctions behind a Ca-changing synapse),
+                                      // completed exactly by the ordered pass; before the first tick: those
+                                      // that are hazards at any dt
     float last_phase_ms[6];           // step 1, 2.5, 3-pre, 3-post+4, 3-plasticity, 3-rrp of the last
                                       // timed tick
+    uint32_t ordered_events;          // synapse events the ordered pass walks (last planned dt)
+    uint32_t ordered_segments;        // segments it walks (the parallel post pass skips them)
 } BchipStats;
 
 int bchip_load_bion_data(void* sim, const BionData* src);   // load from in-memory AoS (copied)

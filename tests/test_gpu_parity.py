@@ -174,7 +174,8 @@ def test_gap_junctions_and_connexins(seed):
     gst = backend.make_stimuli(current=cur)
     _compare(net, 300, lambda t: H.Stimuli(current=cur), lambda t: gst)
     seg, syn, spk, dl, stats = _gpu_run(net.copy(), 1, lambda t: gst)
-    assert stats.order_hazard_synapses == 0 and stats.unsupported_synapses == 0
+    # junctions behind synapses that may change either end's Ca ran in the ordered pass
+    assert stats.unsupported_synapses == 0 and stats.order_hazard_synapses > 0
 
 
 def test_compat_coherence_and_host_edits():
@@ -221,4 +222,87 @@ def test_load_bion_file_and_error_codes(tmp_path):
     assert sim.save_bion(out) == 0
     back = bion.read_bion(out)
     np.testing.assert_array_equal(back.segments["V_m"], sim.bion_data().segments["V_m"])
+    sim.close()
+
+
+@pytest.mark.parametrize("glia", [False, True])
+def test_order_dependent_step3_bitexact(glia):
+    """Immediate releases behind synapses onto their presynaptic soma and gap junctions behind
+    Ca-changing synapses: the ordered pass reproduces the reference's synapse-index order
+    (hazard_network), byte for byte over 400 ticks."""
+    net = bion.hazard_network(3, glia=glia)
+    clamp = H.soma_clamp(net, 200.0)
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net.copy()) == 0
+    assert sim.stats().order_hazard_synapses > 0   # gap junctions and delay-0 releases: any dt
+    sim.close()
+    st = backend.make_stimuli(current=clamp)
+    o = _compare(net, 400, lambda t: H.Stimuli(current=clamp), lambda t: st)
+    now = (o.net.synapses["delay"] <= 0.01) & (o.net.synapses["type"] == 0)
+    assert (o.net.synapses["RRP"][now] < o.net.synapses["RRP_max"][now]).any()
+
+
+def test_ordered_pass_planned_per_dt():
+    """A 0.01 ms delay is immediate at dt = 0.025 ms and delayed at dt = 0.01 ms: the ordered
+    pass is planned for the dt of each tick."""
+    net = bion.hazard_network(5)
+    clamp = H.soma_clamp(net, 200.0)
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net.copy()) == 0
+    sim.upload_stimuli(backend.make_stimuli(current=clamp))
+    sim.run_tick(0.025)
+    a = sim.stats()
+    sim.run_tick(0.01)
+    b = sim.stats()
+    sim.close()
+    assert a.order_hazard_synapses > b.order_hazard_synapses > 0
+    assert a.ordered_events > 0
+    o = H.OracleSim(net.copy())
+    o.step(0.025, H.Stimuli(current=clamp))
+    o.step(0.01, H.Stimuli(current=clamp))
+    g = _gpu_two_dt(net.copy(), clamp)
+    assert H.first_mismatch(o.net.segments, g[0]) is None
+    assert H.first_mismatch(o.net.synapses, g[1]) is None
+
+
+def _gpu_two_dt(net, clamp):
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net) == 0
+    sim.upload_stimuli(backend.make_stimuli(current=clamp))
+    sim.run_tick(0.025)
+    sim.run_tick(0.01)
+    out = sim.bion_data()
+    res = (out.segments.copy(), out.synapses.copy())
+    sim.close()
+    return res
+
+
+def test_unsupported_work_is_refused(tmp_path):
+    """A network that needs work the device does not do loads only on request: a modifier type
+    outside the device set (through the drop-in file load) and a presynaptic autoreceptor
+    (in-memory load) return ERR_UNSUPPORTED, with the counts in the stats, unless
+    bchip_allow_unsupported was set."""
+    lib = backend.load_library()
+    bad = next(t for t in range(1, 212) if not lib.bchip_modifier_supported(t))
+    net = bion.config1(seed=9)
+    c = net.segments["modifier_count"][3]
+    net.segments["modifiers"]["type"][3, c] = bad
+    net.segments["modifier_count"][3] = c + 1
+    path = str(tmp_path / "unsupported.bion")
+    bion.write_bion(path, net)
+    h = lib.gpu_backend_create()
+    try:
+        assert lib.gpu_backend_load_bion(h, path.encode()) == backend.ERR_UNSUPPORTED
+        lib.bchip_allow_unsupported(h, 1)
+        assert lib.gpu_backend_load_bion(h, path.encode()) == 0
+    finally:
+        lib.gpu_backend_destroy(h)
+    net = bion.config1(seed=9)
+    net.synapses["pre_autoreceptor_count"][0] = 1
+    net.synapses["pre_autoreceptors"]["type"][0, 0] = 17
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net.copy()) == backend.ERR_UNSUPPORTED
+    assert sim.stats().unsupported_synapses == 1
+    sim.allow_unsupported(True)
+    assert sim.load_network(net) == 0
     sim.close()

@@ -52,6 +52,9 @@ def test_pmca_hill_other_than_two_is_counted():
     i, j = np.nonzero(mt == bion.PMCA)
     net.segments["modifiers"]["state"][i[0], j[0], 2] = 3.0
     sim = backend.Simulation()
+    assert sim.load_network(net.copy()) == backend.ERR_UNSUPPORTED   # refused by default
+    assert sim.stats().unsupported_modifiers == 1
+    sim.allow_unsupported(True)
     assert sim.load_network(net) == 0
     assert sim.stats().unsupported_modifiers == 1
     sim.close()

@@ -143,3 +143,16 @@ def test_receptor_families(seed):
     rs, ry, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
     assert sp.sum() > 0
     assert np.any(ry["cleft_neurotransmitter"][:, [2, 3, 4, 7]] > 0.0)
+
+
+@pytest.mark.parametrize("glia", [False, True])
+def test_order_dependent_step3(glia):
+    """Immediate releases whose vesicle recovery reads a presynaptic Ca that earlier synapses onto
+    that soma changed, and gap junctions behind such synapses (hazard_network), with and
+    without glial capture."""
+    net = bion.hazard_network(3, glia=glia)
+    clamp = H.soma_clamp(net, 200.0)
+    _, ry, sp = _both(net, 300, lambda t: H.Stimuli(current=clamp))
+    assert sp.sum() > 0
+    now = (ry["delay"] <= 0.01) & (ry["type"] == 0)
+    assert (ry["RRP"][now] < ry["RRP_max"][now]).any()   # immediate releases happened
