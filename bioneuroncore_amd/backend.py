@@ -20,6 +20,7 @@ COHERENCE_COMPAT = 0
 COHERENCE_FAST = 1
 ERR_ISLAND_TOO_LARGE = -20
 ERR_UNSUPPORTED = -21
+ERR_SCRATCH = -22
 
 STIM_DT = np.dtype([
     ("current_stim_count", "<u4"), ("vclamp_count", "<u4"), ("ramp_count", "<u4"),
@@ -126,6 +127,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_set_cable_mode.argtypes = [P, C.c_int]
     lib.bchip_set_coupling_scale.argtypes = [P, P, C.c_uint32]
     lib.bchip_adjoint.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P, P]
+    lib.bchip_adjoint_noalloc.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P, P, P]
     lib.bchip_set_adjoint_memory.argtypes = [P, C.c_uint64]
     lib.bchip_set_stimulus_convention.argtypes = [P, C.c_int]
     lib.bchip_set_protein_mapping.argtypes = [P, C.c_int]
@@ -284,7 +286,7 @@ class Simulation:
             raise ValueError("bad coupling scale")
 
     def adjoint(self, dt: float, ticks: int, checkpoint_every: int, soma, target, grads: bool = True,
-                trace: bool = False):
+                trace: bool = False, noalloc: bool = False):
         """Cable-mode loss and gradients: (loss, d loss/d density [n, 32], d loss/d coupling scale [nc]),
         plus the somatic V_m trace (ticks, cells) when trace=True.
         soma: host segment index per cell; target: (ticks, cells) V_m after each tick."""
@@ -297,11 +299,12 @@ class Simulation:
         gd = np.zeros((n, 32)) if grads else None
         gs = np.zeros(max(nc, 1)) if grads else None
         tr = np.zeros((ticks, len(soma)), np.float32) if trace else None
-        rc = self.lib.bchip_adjoint(self.h, dt, ticks, checkpoint_every, soma.ctypes.data, len(soma),
-                                    target.ctypes.data, C.byref(loss), gd.ctypes.data if grads else None,
-                                    gs.ctypes.data if grads else None, tr.ctypes.data if trace else None)
+        fn = self.lib.bchip_adjoint_noalloc if noalloc else self.lib.bchip_adjoint
+        rc = fn(self.h, dt, ticks, checkpoint_every, soma.ctypes.data, len(soma),
+                target.ctypes.data, C.byref(loss), gd.ctypes.data if grads else None,
+                gs.ctypes.data if grads else None, tr.ctypes.data if trace else None)
         if rc != 0:
-            raise RuntimeError(f"bchip_adjoint failed ({rc})")
+            raise RuntimeError(f"bchip_adjoint failed ({rc})", rc)
         out = (loss.value, gd, (gs[:nc] if grads else None))
         return out + (tr,) if trace else out
 

@@ -138,6 +138,8 @@ struct Sim {
     bool host_stale = false;
     uint64_t unsupported_mod = 0, unsupported_syn = 0, hazard_syn = 0;
     bool allow_unsupported = false;   // bchip_allow_unsupported
+    uint32_t* host2dev_dev = nullptr;   // device copies of host2dev and conn2dev (the adjoint's
+    uint32_t* conn2dev_dev = nullptr;   // soma mapping and gradient order)
     // Ordered pass planning (see plan_ordered): per slot the Ca-history source, the effective
     // RRP_max, flags and glial target; per segment the Ca-history base and flags; the chemical
     // synapses that become hazards when their release is immediate, and the gap junction hazards.
@@ -800,6 +802,8 @@ static int build_device(Sim& S) {
     d.d_nt = S.dalloc<uint8_t>((size_t)4 * ns);
     S.lds_bytes = sizeof(DiffLds<256>);
     S.seg_dispatched.assign((size_t)n * 32, 0);
+    S.host2dev_dev = S.dupload(S.host2dev);
+    S.conn2dev_dev = S.dupload(S.conn2dev);
     // kept for planning the ordered pass per dt
     S.h_src = std::move(src);
     S.h_rmax = std::move(y_rmax);
@@ -1277,8 +1281,100 @@ static int load_into(Sim& S, BionData* src_owned) {
     return 0;
 }
 
-// Reverse pass over `ticks` cable-mode ticks from the current state (see bchip_adjoint).
-static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
+__global__ void k_fill_zero(uint64_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+__global__ void k_map_u32(uint32_t* p, uint32_t n, const uint32_t* table) {   // p[i] = table[p[i]]
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = table[p[i]];
+}
+// grad[h][j] (h host segment, j < 32) = gd[j][host2dev[h]] for j < nslot, else 0
+__global__ void k_grad_density_out(double* out, const double* gd, uint32_t n, uint32_t P, uint32_t nslot,
+                                   const uint32_t* host2dev) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * (uint32_t)BC_MAX_MODIFIERS) return;
+    const uint32_t h = e / BC_MAX_MODIFIERS, j = e % BC_MAX_MODIFIERS;
+    out[e] = j < nslot ? gd[(size_t)j * P + host2dev[h]] : 0.0;
+}
+__global__ void k_gather_f64(double* out, const double* in, uint32_t n, const uint32_t* idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[idx[i]];
+}
+
+// Scratch of bchip_adjoint_noalloc: 64 MiB, one copy per GPU the library runs on, held until
+// the process ends. In host code `scratch` names a host-side stand-in, not device memory: the
+// kernels and copies take the device address hipGetSymbolAddress gives for the current device,
+// looked up once per device by scratch_setup (at simulation creation) and kept in g_scratch_ptr.
+// Every call on one GPU shares its copy; it keeps the previous call's contents, and calls that
+// overlap on one GPU race.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+static int scratch_setup(int device) {   // `device` is current; 0 or -1
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+    g_scratch_ptr[device] = ptr;
+    return 0;
+}
+static void* scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// Buffers of one adjoint call, every one zeroed when handed out: one hipMalloc each, freed when
+// the call ends (bchip_adjoint), or consecutive 256-byte-aligned pieces of the file-scope
+// scratch array (bchip_adjoint_noalloc), cleared by a fill kernel.
+static size_t adj_round(size_t bytes) { return (std::max<size_t>(bytes, 64) + 255) & ~(size_t)255; }
+struct AdjHeap {
+    Sim& S;
+    std::vector<void*> tmp;
+    explicit AdjHeap(Sim& s) : S(s) {}
+    bool fits(size_t) const { return true; }
+    void* get(size_t bytes) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, adj_round(bytes)));
+        tmp.push_back(p);
+        HIPCHK(hipMemsetAsync(p, 0, adj_round(bytes), S.stream));
+        return p;
+    }
+    // the optional recorded iterates: within half the free memory and the caller's cap
+    bool room(size_t bytes) const {
+        size_t mfree = 0, mtotal = 0;
+        HIPCHK(hipMemGetInfo(&mfree, &mtotal));
+        return bytes <= mfree / 2 && bytes <= S.adj_record_cap;
+    }
+    ~AdjHeap() {
+        for (void* p : tmp) (void)hipFree(p);
+    }
+};
+struct AdjScratch {
+    Sim& S;
+    char* base;
+    size_t cap, used = 0;
+    AdjScratch(Sim& s, void* b, size_t c) : S(s), base(static_cast<char*>(b)), cap(c) {}
+    bool fits(size_t need) const { return need <= cap; }
+    void* get(size_t bytes) {
+        const size_t b = adj_round(bytes);
+        if (used + b > cap) throw std::runtime_error("scratch");   // excluded by fits() before the run
+        char* p = base + used;
+        used += b;
+        k_fill_zero<<<(uint32_t)std::min<size_t>((b / 8 + 255) / 256, 4096), 256, 0, S.stream>>>(
+            reinterpret_cast<uint64_t*>(p), b / 8);
+        return p;
+    }
+    bool room(size_t bytes) const { return adj_round(bytes) <= cap - used && bytes <= S.adj_record_cap; }
+};
+
+// Reverse pass over `ticks` cable-mode ticks from the current state (see bchip_adjoint); Mem
+// supplies the device buffers (AdjHeap or AdjScratch).
+template <class Mem>
+static int adjoint_run(Sim& S, Mem& mem, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
                        uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
                        float* trace) {
     if (!soma || !target || ticks == 0 || ncells == 0) return -1;
@@ -1297,20 +1393,26 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
     const uint32_t K = checkpoint_every ? checkpoint_every : ticks;
     const uint32_t nck = (ticks + K - 1) / K;
     const uint32_t nconn = (uint32_t)S.host.num_connections;
-    std::vector<void*> tmp;
-    auto alloc = [&](size_t bytes) {
-        void* p = nullptr;
-        HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 64)));
-        tmp.push_back(p);
-        HIPCHK(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 64), S.stream));
-        return p;
-    };
+    const size_t vB = (size_t)P * 8, mB = (size_t)nslot * 7 * P * 8, gB = (size_t)nslot * 2 * P * 8;
+    const size_t spB = (size_t)P * 4, ntB = (size_t)15 * P * 4;
+    const size_t rstride = (size_t)2 * P + (size_t)nslot * 2 * P;
+    const size_t xr_tick = (size_t)5 * 51 * P;
+    {   // every buffer the run takes apart from the optional recorded iterates, before any effect
+        size_t need = adj_round((size_t)(nck + 1) * (vB + mB + 8)) + adj_round((size_t)ncells * 4) +
+                      adj_round((size_t)ticks * ncells * 4) + adj_round((size_t)ncells * 8) +
+                      (trace ? adj_round((size_t)ticks * ncells * 4) : 0) + adj_round(spB + 2 * ntB + vB);
+        if (backward)
+            need += adj_round((size_t)(K + 1) * rstride * 8) + 2 * adj_round(vB) + adj_round(gB) +
+                    adj_round((size_t)nslot * P * 8) + adj_round((size_t)std::max<uint32_t>(nconn, 1) * 8) +
+                    (grad_density ? adj_round((size_t)n * BC_MAX_MODIFIERS * 8) : 0) +
+                    (grad_scale ? adj_round((size_t)std::max<uint32_t>(nconn, 1) * 8) : 0);
+        if (!mem.fits(need)) return BCHIP_ERR_SCRATCH;
+    }
     int rc = 0;
     try {
         (void)hipSetDevice(S.device);
         before_ticks(S);
-        const size_t vB = (size_t)P * 8, mB = (size_t)nslot * 7 * P * 8, gB = (size_t)nslot * 2 * P * 8;
-        char* ck = static_cast<char*>(alloc((size_t)(nck + 1) * (vB + mB + 8)));
+        char* ck = static_cast<char*>(mem.get((size_t)(nck + 1) * (vB + mB + 8)));
         auto save = [&](uint32_t k) {
             char* b = ck + (size_t)k * (vB + mB + 8);
             HIPCHK(hipMemcpyAsync(b, d.V, vB, hipMemcpyDeviceToDevice, S.stream));
@@ -1323,16 +1425,16 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
             HIPCHK(hipMemcpyAsync(d.mstate, b + vB, mB, hipMemcpyDeviceToDevice, S.stream));
             HIPCHK(hipMemcpyAsync(S.tick_dev, b + vB + mB, 8, hipMemcpyDeviceToDevice, S.stream));
         };
-        std::vector<uint32_t> soma_dev(ncells);
-        for (uint32_t c = 0; c < ncells; c++) soma_dev[c] = S.host2dev[soma[c]];
-        uint32_t* dsoma = static_cast<uint32_t*>(alloc((size_t)ncells * 4));
-        float* dtarget = static_cast<float*>(alloc((size_t)ticks * ncells * 4));
-        double* acc = static_cast<double*>(alloc((size_t)ncells * 8));
-        HIPCHK(hipMemcpyAsync(dsoma, soma_dev.data(), (size_t)ncells * 4, hipMemcpyHostToDevice, S.stream));
-        HIPCHK(hipMemcpyAsync(dtarget, target, (size_t)ticks * ncells * 4, hipMemcpyHostToDevice, S.stream));
+        // the cells' somas as device segment indices
+        uint32_t* dsoma = static_cast<uint32_t*>(mem.get((size_t)ncells * 4));
+        HIPCHK(hipMemcpyAsync(dsoma, soma, (size_t)ncells * 4, hipMemcpyHostToDevice, S.stream));
         const uint32_t gc = (ncells + 255) / 256;
+        k_map_u32<<<gc, 256, 0, S.stream>>>(dsoma, ncells, S.host2dev_dev);
+        float* dtarget = static_cast<float*>(mem.get((size_t)ticks * ncells * 4));
+        double* acc = static_cast<double*>(mem.get((size_t)ncells * 8));
+        HIPCHK(hipMemcpyAsync(dtarget, target, (size_t)ticks * ncells * 4, hipMemcpyHostToDevice, S.stream));
         // forward: checkpoints every K ticks, loss after every tick
-        float* dtrace = trace ? static_cast<float*>(alloc((size_t)ticks * ncells * 4)) : nullptr;
+        float* dtrace = trace ? static_cast<float*>(mem.get((size_t)ticks * ncells * 4)) : nullptr;
         for (uint32_t t = 0; t < ticks; t++) {
             if (t % K == 0) save(t / K);
             launch_tick(S, dt, false);
@@ -1344,8 +1446,7 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
         save(nck);   // the state after the last tick, restored at the end
         // recomputing a span also counts its spikes again, rewrites I_buffer and re-applies a
         // ligand injection of its first tick: keep those fields as the forward left them
-        const size_t spB = (size_t)P * 4, ntB = (size_t)15 * P * 4;
-        char* fin = static_cast<char*>(alloc(spB + 2 * ntB + vB));
+        char* fin = static_cast<char*>(mem.get(spB + 2 * ntB + vB));
         auto keep_final = [&](bool out) {
             void* dev[4] = {d.spikes, d.nt, d.nt_in, d.ibuf};
             const size_t sz[4] = {spB, ntB, ntB, vB};
@@ -1358,32 +1459,33 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
         };
         keep_final(false);
         HIPCHK(hipGetLastError());
-        std::vector<double> hacc(ncells);
-        HIPCHK(hipMemcpyAsync(hacc.data(), acc, (size_t)ncells * 8, hipMemcpyDeviceToHost, S.stream));
-        HIPCHK(hipStreamSynchronize(S.stream));
-        double L = 0.0;
-        for (double x : hacc) L += x;
+        double L = 0.0;   // the per-cell sums in cell order, read in pieces
+        for (uint32_t c0 = 0; c0 < ncells; c0 += 64) {
+            double part[64];
+            const uint32_t m = std::min<uint32_t>(64, ncells - c0);
+            HIPCHK(hipMemcpyAsync(part, acc + c0, (size_t)m * 8, hipMemcpyDeviceToHost, S.stream));
+            HIPCHK(hipStreamSynchronize(S.stream));
+            for (uint32_t k = 0; k < m; k++) L += part[k];
+        }
         L /= (double)ticks * ncells;
         if (loss) *loss = L;
         if (backward) {
             // per tick of a span: pre-tick V, gate states, and the tick's I_buffer
-            const size_t rstride = (size_t)2 * P + (size_t)nslot * 2 * P;
-            double* rec = static_cast<double*>(alloc((size_t)(K + 1) * rstride * 8));
+            double* rec = static_cast<double*>(mem.get((size_t)(K + 1) * rstride * 8));
             AdjNet a{};
-            a.lamV = static_cast<double*>(alloc(vB));
-            a.lamG = static_cast<double*>(alloc(gB));
-            a.lamIb = static_cast<double*>(alloc(vB));
-            a.gdens = static_cast<double*>(alloc((size_t)nslot * P * 8));
-            a.gscale = static_cast<double*>(alloc((size_t)std::max<uint32_t>(nconn, 1) * 8));
+            a.lamV = static_cast<double*>(mem.get(vB));
+            a.lamG = static_cast<double*>(mem.get(gB));
+            a.lamIb = static_cast<double*>(mem.get(vB));
+            a.gdens = static_cast<double*>(mem.get((size_t)nslot * P * 8));
+            a.gscale = static_cast<double*>(mem.get((size_t)std::max<uint32_t>(nconn, 1) * 8));
+            double* gd_out = grad_density ? static_cast<double*>(mem.get((size_t)n * BC_MAX_MODIFIERS * 8)) : nullptr;
+            double* gs_out = grad_scale ? static_cast<double*>(mem.get((size_t)std::max<uint32_t>(nconn, 1) * 8)) : nullptr;
             // The forward records each tick's Jacobi iterates (5 x 51 floats per segment) when
-            // a span's worth fits in half the free memory; otherwise the reverse kernel runs
-            // the substeps again from the tick's recorded start state.
-            const size_t xr_tick = (size_t)5 * 51 * P;
-            size_t mfree = 0, mtotal = 0;
-            HIPCHK(hipMemGetInfo(&mfree, &mtotal));
-            float* xrec = nullptr;
+            // a span's worth fits (AdjHeap: half the free memory; AdjScratch: the rest of the
+            // scratch); otherwise the reverse kernel runs the substeps again from the tick's
+            // recorded start state. Both give the same gradients bit for bit.
             const size_t xr_bytes = (size_t)K * xr_tick * 4;
-            if (xr_bytes <= mfree / 2 && xr_bytes <= S.adj_record_cap) xrec = static_cast<float*>(alloc(xr_bytes));
+            float* xrec = mem.room(xr_bytes) ? static_cast<float*>(mem.get(xr_bytes)) : nullptr;
             const double coef = 2.0 / ((double)ticks * ncells);
             const uint32_t gs = (n + 255) / 256;
             for (int seg = (int)nck - 1; seg >= 0; seg--) {
@@ -1420,18 +1522,18 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
                 }
                 HIPCHK(hipGetLastError());
             }
-            std::vector<double> gd((size_t)nslot * P), gsc(std::max<uint32_t>(nconn, 1));
-            HIPCHK(hipMemcpyAsync(gd.data(), a.gdens, gd.size() * 8, hipMemcpyDeviceToHost, S.stream));
-            HIPCHK(hipMemcpyAsync(gsc.data(), a.gscale, gsc.size() * 8, hipMemcpyDeviceToHost, S.stream));
-            HIPCHK(hipStreamSynchronize(S.stream));
-            if (grad_density) {
-                memset(grad_density, 0, (size_t)n * BC_MAX_MODIFIERS * 8);
-                for (uint32_t h = 0; h < n; h++)
-                    for (uint32_t j = 0; j < nslot; j++)
-                        grad_density[(size_t)h * BC_MAX_MODIFIERS + j] = gd[(size_t)j * P + S.host2dev[h]];
+            // gradients in the caller's order: segment h, slot j (0 past nslot); connection i
+            if (gd_out) {
+                k_grad_density_out<<<(n * BC_MAX_MODIFIERS + 255) / 256, 256, 0, S.stream>>>(
+                    gd_out, a.gdens, n, P, nslot, S.host2dev_dev);
+                HIPCHK(hipMemcpyAsync(grad_density, gd_out, (size_t)n * BC_MAX_MODIFIERS * 8, hipMemcpyDeviceToHost,
+                                      S.stream));
             }
-            if (grad_scale)
-                for (uint32_t i = 0; i < nconn; i++) grad_scale[i] = gsc[S.conn2dev[i]];
+            if (gs_out && nconn) {
+                k_gather_f64<<<(nconn + 255) / 256, 256, 0, S.stream>>>(gs_out, a.gscale, nconn, S.conn2dev_dev);
+                HIPCHK(hipMemcpyAsync(grad_scale, gs_out, (size_t)nconn * 8, hipMemcpyDeviceToHost, S.stream));
+            }
+            HIPCHK(hipStreamSynchronize(S.stream));
             restore(nck);
             keep_final(true);
         }
@@ -1441,7 +1543,6 @@ static int adjoint_run(Sim& S, float dt, uint32_t ticks, uint32_t checkpoint_eve
         rc = -1;
     }
     S.d.xrec = nullptr;
-    for (void* p : tmp) (void)hipFree(p);
     return rc;
 }
 
@@ -1470,6 +1571,7 @@ void* gpu_backend_create_on_device(int device_index) {
         Sim* s = new Sim();
         s->device = device_index;
         HIPCHK(hipSetDevice(device_index));
+        (void)bc::scratch_setup(device_index);   // bchip_adjoint_noalloc refuses (-1) if this failed
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
         s->stim_host = new GpuStimulusData();
@@ -1933,8 +2035,25 @@ int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every
                   uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
                   float* trace) {
     if (!sim) return -1;
-    return bc::adjoint_run(*static_cast<Sim*>(sim), dt, ticks, checkpoint_every, soma, ncells, target, loss,
-                           grad_density, grad_scale, trace);
+    Sim& S = *static_cast<Sim*>(sim);
+    bc::AdjHeap mem(S);
+    return bc::adjoint_run(S, mem, dt, ticks, checkpoint_every, soma, ncells, target, loss, grad_density,
+                           grad_scale, trace);
+}
+
+int bchip_adjoint_noalloc(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
+                          uint32_t ncells, const float* target, double* loss, double* grad_density,
+                          double* grad_scale, float* trace) {
+    if (!sim) return -1;
+    Sim& S = *static_cast<Sim*>(sim);
+    // building stimulus lists (a pending upload) and refreshing the host records (COMPAT)
+    // allocate: refused here
+    if (S.coherence != BCHIP_COHERENCE_FAST || S.stim_dirty) return -1;
+    void* base = bc::scratch_of(S.device);
+    if (!base) return -1;
+    bc::AdjScratch mem(S, base, bc::kScratchBytes);
+    return bc::adjoint_run(S, mem, dt, ticks, checkpoint_every, soma, ncells, target, loss, grad_density,
+                           grad_scale, trace);
 }
 
 int bchip_set_stimulus_convention(void* sim, int convention) {

@@ -64,6 +64,7 @@ void register_modulator_release(uint8_t type, ReleaseModulatorFunc func);   // p
 // counts on stderr and bchip_stats reports them. bchip_allow_unsupported(sim, 1) loads such
 // networks anyway (those parts then do nothing on the device).
 #define BCHIP_ERR_UNSUPPORTED -21
+#define BCHIP_ERR_SCRATCH -22   // bchip_adjoint_noalloc: the call needs more than its fixed array
 int bchip_allow_unsupported(void* sim, int on);
 
 typedef struct BchipStats {
@@ -162,6 +163,21 @@ int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n);
 int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
                   uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
                   float* trace);
+// No counterpart in the reference: for callers that cannot allocate device memory (code under
+// HIP graph capture, serving and training loops, harnesses that forbid hipMalloc). Same as
+// bchip_adjoint, with every buffer of the call (checkpoints, recordings, adjoint state, staging)
+// taken from a fixed 64 MiB device array the library holds per GPU instead of allocated; it
+// calls no hipMalloc/hipFree and allocates no host memory. The recorded iterates are used when
+// they fit in what the call leaves of the array. Requires FAST coherence and no stimulus upload
+// pending since the last tick (building the stimulus lists allocates): -1 otherwise.
+// BCHIP_ERR_SCRATCH (nothing run, state unchanged) when the call needs more than the array
+// holds: checkpoints ((ticks / checkpoint_every + 2) x (8 + 56 x slots) bytes per segment), the
+// targets (4 x ticks x cells bytes), the span record (16 x (checkpoint_every + 1) x (1 + slots)
+// bytes per segment) and the gradients (256 bytes per segment, 8 per connection). Calls that
+// overlap on one GPU share the array and race; each call clears what it uses.
+int bchip_adjoint_noalloc(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
+                          uint32_t ncells, const float* target, double* loss, double* grad_density,
+                          double* grad_scale, float* trace);
 // Tick at which stimulus time windows (current pulses, ramps, voltage clamps) are matched.
 // 0 (default): the tick being run, as the CPU tick (src/simulation_host.cpp:186,247-300) and the
 // CUDA backend (src/gpu_backend_cuda.cu:252-255) match them. 1: that tick plus one, as the SYCL
