@@ -587,6 +587,51 @@ def receptor_network(seed: int, n_neurons: int = 14, seg_per_neuron: int = 5) ->
     return net
 
 
+GPCR_REST = (19, 20, 21, 23, 24, 26, 27, 28, 30, 31, 32, 33, 34, 35, 37, 40, 41, 42, 45, 47, 49, 51, 53, 54,
+             56, 58, 59, 61)   # the G-protein-coupled IDs receptor_network does not place
+
+
+def gpcr_network(seed: int, n_neurons: int = 8, seg_per_neuron: int = 5) -> Network:
+    """Chain neurons without synapses whose somas carry the first half of GPCR_REST and whose
+    dendrites (local segments 1 and 3) carry the second half, as segment modifiers; with no
+    synapse the segments' transmitter levels persist and diffuse, so ligand injections
+    (gpcr_ligands) reach every receptor's transmitter. Every other soma's first receptor takes
+    parameter states (active_vars 3)."""
+    half = len(GPCR_REST) // 2
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP],
+                        list(GPCR_REST[:half]), 0.0, "ampa", 1, seed)
+    rng = np.random.default_rng(seed + 3)
+    segs = net.segments
+    mods = segs["modifiers"]
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    for j in range(5, 5 + half):
+        mods["density"][soma_idx, j] = rng.uniform(20.0, 60.0, len(soma_idx))
+    mods["active_vars"][soma_idx[::2], 5] = 3
+    mods["state"][soma_idx[::2], 5, 3] = 0.4
+    local = np.arange(len(segs)) % seg_per_neuron
+    for i in np.nonzero((local == 1) | (local == 3))[0]:
+        k = int(segs["modifier_count"][i])
+        for q, t in enumerate(GPCR_REST[half:]):
+            mods["type"][i, k + q] = t
+            mods["density"][i, k + q] = rng.uniform(20.0, 60.0)
+        segs["modifier_count"][i] = k + len(GPCR_REST) - half
+    return net
+
+
+def gpcr_ligands(net: Network, tick: int, seed: int = 5) -> np.ndarray:
+    """The tick's extracellular ligand injections of gpcr_network: every 4th tick each of the 15
+    transmitters onto a few segments (amounts 0.05-0.5)."""
+    if tick % 4:
+        return np.zeros(0, LIG_INJ_DT)
+    rng = np.random.default_rng(seed * 100003 + tick)
+    n = len(net.segments)
+    out = np.zeros(15 * 3, LIG_INJ_DT)
+    out["nt_idx"] = np.repeat(np.arange(15), 3)
+    out["segment_id"] = net.segments["id"][rng.integers(0, n, len(out))]
+    out["amount"] = rng.uniform(0.05, 0.5, len(out)).astype(np.float32)
+    return out
+
+
 def segment_receptor_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
     """Chain neurons with AMPA+NMDA/GABA-A synapses whose dendrites also carry ionotropic
     receptors as segment modifiers (AMPA and NMDA on every third segment, GABA-A on the next):

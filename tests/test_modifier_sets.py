@@ -80,3 +80,22 @@ def test_receptor_families(seed):
     cur = H.soma_clamp(net, 150.0)
     _, spk = _compare(net, 400, cur)
     assert spk.sum() > 0
+
+
+def test_remaining_gpcr_families():
+    """bion.GPCR_REST as segment modifiers reading injected transmitters (gpcr_network)."""
+    net = bion.gpcr_network(4)
+    clamp = H.soma_clamp(net, 120.0)
+    o = H.OracleSim(net.copy())
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net.copy()) == 0
+    assert sim.stats().unsupported_modifiers == 0
+    for t in range(240):
+        lig = bion.gpcr_ligands(net, t)
+        o.step(DT, H.Stimuli(current=clamp, lig=lig))
+        sim.upload_stimuli(backend.make_stimuli(current=clamp, lig=lig))
+        sim.run_tick(DT)
+    g = sim.bion_data()
+    assert H.first_mismatch(o.net.segments, g.segments) is None, H.first_mismatch(o.net.segments, g.segments)
+    np.testing.assert_array_equal(o.spikes(), sim.spikes(len(net.segments)))
+    sim.close()

@@ -156,3 +156,14 @@ def test_order_dependent_step3(glia):
     assert sp.sum() > 0
     now = (ry["delay"] <= 0.01) & (ry["type"] == 0)
     assert (ry["RRP"][now] < ry["RRP_max"][now]).any()   # immediate releases happened
+
+
+def test_remaining_gpcr_families():
+    """The G-protein-coupled receptor IDs receptor_network does not place (bion.GPCR_REST) as
+    segment modifiers, in a network without synapses whose segments receive ligand injections
+    of all 15 transmitters: each receptor's transmitter and response against the reference."""
+    net = bion.gpcr_network(3)
+    clamp = H.soma_clamp(net, 120.0)
+    rs, _, sp = _both(net, 240, lambda t: H.Stimuli(current=clamp, lig=bion.gpcr_ligands(net, t)))
+    assert sp.sum() > 0
+    assert (rs["neurotransmitter"] > 0).all(axis=0).sum() >= 10
