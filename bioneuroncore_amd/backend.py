@@ -63,7 +63,8 @@ class BchipStats(C.Structure):
                 ("lds_bytes_per_tile", C.c_uint32), ("_pad", C.c_uint32),
                 ("unsupported_modifiers", C.c_uint64), ("unsupported_synapses", C.c_uint64),
                 ("order_hazard_synapses", C.c_uint64), ("last_phase_ms", C.c_float * 6),
-                ("ordered_events", C.c_uint32), ("ordered_segments", C.c_uint32)]
+                ("ordered_events", C.c_uint32), ("ordered_segments", C.c_uint32),
+                ("big_islands", C.c_uint32), ("_pad2", C.c_uint32)]
 
 
 class BchipSchedule(C.Structure):
@@ -132,6 +133,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.bchip_set_stimulus_convention.argtypes = [P, C.c_int]
     lib.bchip_set_protein_mapping.argtypes = [P, C.c_int]
     lib.bchip_allow_unsupported.argtypes = [P, C.c_int]
+    lib.bchip_eval_math.argtypes = [C.c_int, P, P, P, C.c_uint32]
     _lib = lib
     return lib
 

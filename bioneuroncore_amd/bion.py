@@ -367,6 +367,70 @@ def gap_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Netw
     return net
 
 
+def spinal_glia_network(seed: int, n_spinal: int = 4, per_spinal: int = 12, seg_per_neuron: int = 6,
+                        glia_per_spinal: int = 2, synapses: bool = True) -> Network:
+    """One connected island far larger than a tile, built as the reference's spinal network
+    couples its glia (include/cpg_config.h:657-713): chain neurons in n_spinal spinal segments,
+    single-segment glial cells (GLAST, pump, K2P) each coupled extracellularly to every segment of
+    the neurons of its own and the neighbouring spinal segments, and each soma coupled
+    extracellularly to the soma of the same index in the next spinal segment. With `synapses`,
+    AMPA+NMDA / GABA-A synapses between somas (delays 1-4 ticks); without, the extracellular
+    transmitters diffuse over the coupling."""
+    nn = n_spinal * per_spinal
+    net = build_network(nn, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21],
+                        4.0 if synapses else 0.0, "ampa_nmda_gaba", 4, seed)
+    rng = np.random.default_rng(seed + 11)
+    ng = n_spinal * glia_per_spinal
+    g = np.zeros(ng, SEGMENT_DT)
+    _init_segment_state(g, rng)
+    g["segment_type"] = 1
+    g["radius"], g["length"] = 4.0, 8.0
+    g["neuron_id"] = nn + np.arange(ng)
+    gt = [GLAST, PUMP, K2P]
+    _set_modifiers(g, gt, np.array([[CANON_DENSITY[t] for t in gt]] * ng, np.float64))
+    n0 = len(net.segments)
+    net.segments = np.concatenate([net.segments, g])
+    net.segments["id"] = np.arange(len(net.segments), dtype=np.uint32)
+    gneu = np.zeros(ng, NEURON_DT)
+    gneu["id"] = nn + np.arange(ng)
+    gneu["segment_start"] = gneu["soma_index"] = n0 + np.arange(ng)
+    gneu["segment_count"] = 1
+    gneu["threshold"] = -40.0
+    net.neurons = np.concatenate([net.neurons, gneu])
+    e = []
+    span = per_spinal * seg_per_neuron
+    for k in range(ng):
+        s = k // glia_per_spinal
+        for ds in (-1, 0, 1):
+            if 0 <= s + ds < n_spinal:
+                e += [(n0 + k, x) for x in range((s + ds) * span, (s + ds + 1) * span)]
+    for s in range(n_spinal - 1):
+        e += [(s * span + j * seg_per_neuron, (s + 1) * span + j * seg_per_neuron) for j in range(per_spinal)]
+    ext = np.zeros(len(e), CONN_DT)
+    ext["segment_a"] = [x[0] for x in e]
+    ext["segment_b"] = [x[1] for x in e]
+    ext["is_intracellular"] = 0
+    net.connections = np.concatenate([net.connections, ext])
+    return net
+
+
+def long_cell(seed: int, n_seg: int = 1000, n_small: int = 6) -> Network:
+    """A 1000-compartment cell (soma + binary-branching dendrite; Nav1.2, Kv-DR, K2P, NALCN, pump)
+    followed by n_small 8-segment chain neurons, no synapses: one island beyond any tile next to
+    islands that tile."""
+    big = build_network(1, n_seg, "tree", [NAV12, KVDR, K2P, NALCN, PUMP], [], 0.0, "ampa", 1, seed)
+    small = build_network(n_small, 8, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [], 0.0, "ampa", 1, seed + 1)
+    small.segments["id"] += n_seg
+    small.segments["neuron_id"] += 1
+    small.neurons["id"] += 1
+    small.neurons["segment_start"] += n_seg
+    small.neurons["soma_index"] += n_seg
+    small.connections["segment_a"] += n_seg
+    small.connections["segment_b"] += n_seg
+    return Network(np.concatenate([big.neurons, small.neurons]), np.concatenate([big.segments, small.segments]),
+                   np.concatenate([big.connections, small.connections]), np.zeros(0, SYNAPSE_DT))
+
+
 def hazard_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6, glia: bool = False) -> Network:
     """A recurrent network whose step-3 results depend on synapse order across post chains:
     somas (the presynaptic segments) receive AMPA+NMDA / GABA-A synapses; a third of all

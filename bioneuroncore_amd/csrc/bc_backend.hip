@@ -16,6 +16,7 @@
 #include "../../include/biocore_abi.h"
 #include "../../include/biocore_hip_backend.h"
 #include "bc_kernels.hip"
+#include "bc_bigisland.hip"
 #include "bc_adjoint.hip"
 
 namespace bc {
@@ -78,6 +79,7 @@ struct Sim {
     std::vector<uint32_t> dev2host, host2dev, slot2syn, conn2dev;
     uint32_t ntiles = 0;
     uint32_t class_tile0[kTileClasses] = {}, class_ntiles[kTileClasses] = {};
+    BigSet big{};   // islands larger than a tile (k_diffusion_big); big.count = 0 without
     TileSet tiles[kTileClasses] = {};
     size_t lds_bytes = 0;
     uint64_t tick = 0;
@@ -293,10 +295,9 @@ static int build_device(Sim& S) {
     std::vector<uint32_t> comps;
     for (uint32_t i = 0; i < n; i++) {
         if (root[i] != i) continue;
-        int cls = -1;
-        for (int k = 0; k < kTileClasses && cls < 0; k++)
+        int cls = kTileClasses;   // past every tile class: a big island (one workgroup each)
+        for (int k = 0; k < kTileClasses && cls == kTileClasses; k++)
             if (comp_size[i] <= (uint32_t)kTileSize[k] && comp_conns[i] <= (uint32_t)kTileSize[k]) cls = k;
-        if (cls < 0) return BCHIP_ERR_ISLAND_TOO_LARGE;
         comp_class[i] = cls;
         comps.push_back(i);
     }
@@ -329,6 +330,7 @@ static int build_device(Sim& S) {
         };
         for (uint32_t r : comps) {
             int cls = comp_class[r];
+            if (cls == kTileClasses) break;   // the big islands follow
             uint32_t cap = kTileSize[cls];
             if (cls != cur_cls) {
                 flush();
@@ -341,24 +343,35 @@ static int build_device(Sim& S) {
         flush();
     }
     S.ntiles = (uint32_t)tile_seg0.size();
-    for (uint32_t t = 0; t < S.ntiles; t++)
+    // each big island is a pseudo-tile past the real ones (placement and incidence lists below
+    // treat it as a tile; only k_diffusion_big runs on it)
+    uint32_t nbig = 0;
+    for (uint32_t r : comps)
+        if (comp_class[r] == kTileClasses) {
+            tile_seg0.push_back(comp_start[r]);
+            tile_nseg.push_back(comp_size[r]);
+            tile_nconn.push_back(comp_conns[r]);
+            nbig++;
+        }
+    const uint32_t nall = S.ntiles + nbig;
+    for (uint32_t t = 0; t < nall; t++)
         for (uint32_t k = 0; k < tile_nseg[t]; k++) seg_tile[tile_seg0[t] + k] = t;
     std::vector<uint8_t> isl_lane0(n);
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t di = S.host2dev[i];
         isl_lane0[di] = (uint8_t)(comp_start[root[i]] - tile_seg0[seg_tile[di]]);
     }
-    tile_conn0.resize(S.ntiles);
+    tile_conn0.resize(nall);
     {
         uint32_t acc = 0;
-        for (uint32_t t = 0; t < S.ntiles; t++) { tile_conn0[t] = acc; acc += tile_nconn[t]; }
+        for (uint32_t t = 0; t < nall; t++) { tile_conn0[t] = acc; acc += tile_nconn[t]; }
     }
     // connections placed per tile in global index order
     std::vector<uint32_t> conn_ab(nc), conn_local(nc);
     S.conn2dev.assign(nc, 0);
     std::vector<uint8_t> conn_intra(nc);
     {
-        std::vector<uint32_t> fill(S.ntiles, 0);
+        std::vector<uint32_t> fill(nall, 0);
         for (uint32_t i = 0; i < nc; i++) {
             uint32_t da = S.host2dev[cn[i].segment_a], db = S.host2dev[cn[i].segment_b];
             uint32_t t = seg_tile[da];
@@ -385,7 +398,7 @@ static int build_device(Sim& S) {
         }
     }
     // per tile: the largest incidence count of its segments (selects the sweep variant)
-    std::vector<uint32_t> tile_maxinc(S.ntiles, 0);
+    std::vector<uint32_t> tile_maxinc(nall, 0);
     for (uint32_t i = 0; i < n; i++) {
         uint32_t& m = tile_maxinc[seg_tile[i]];
         m = std::max<uint32_t>(m, csr_off[i + 1] - csr_off[i]);
@@ -681,6 +694,47 @@ static int build_device(Sim& S) {
         }
     }
     d.isl_lane0 = S.dupload(isl_lane0);
+    // ---- big islands: bounds, connection ends, scratch planes ----
+    S.big = BigSet{};
+    if (nbig) {
+        BigSet& b = S.big;
+        b.count = nbig;
+        b.seg_base = tile_seg0[S.ntiles];
+        b.conn_base = tile_conn0[S.ntiles];
+        b.nseg_all = n - b.seg_base;
+        b.nconn_all = nc - b.conn_base;
+        std::vector<uint32_t> s0(tile_seg0.begin() + S.ntiles, tile_seg0.end()),
+            sn(tile_nseg.begin() + S.ntiles, tile_nseg.end()), c0(tile_conn0.begin() + S.ntiles, tile_conn0.end()),
+            cnn(tile_nconn.begin() + S.ntiles, tile_nconn.end()), ca(std::max<uint32_t>(b.nconn_all, 1)),
+            cb(std::max<uint32_t>(b.nconn_all, 1));
+        for (uint32_t i = 0; i < nc; i++) {
+            const uint32_t pos = S.conn2dev[i];
+            if (pos < b.conn_base) continue;
+            ca[pos - b.conn_base] = S.host2dev[cn[i].segment_a];
+            cb[pos - b.conn_base] = S.host2dev[cn[i].segment_b];
+        }
+        b.seg0 = S.dupload(s0);
+        b.nseg = S.dupload(sn);
+        b.conn0 = S.dupload(c0);
+        b.nconn = S.dupload(cnn);
+        b.ca = S.dupload(ca);
+        b.cb = S.dupload(cb);
+        const size_t SN = b.nseg_all, CN = std::max<uint32_t>(b.nconn_all, 1);
+        b.phi = S.dalloc<double>(2 * SN);
+        b.diag = S.dalloc<double>(SN);
+        b.vf = S.dalloc<float>(2 * SN);
+        b.base = S.dalloc<float>(SN);
+        b.den = S.dalloc<float>(SN);
+        b.dacc = S.dalloc<double>(7 * SN);
+        b.dsp = S.dalloc<float>(33 * SN);
+        b.gphi = S.dalloc<double>(CN);
+        b.cac = S.dalloc<float>(CN);
+        b.cdist = S.dalloc<float>(CN);
+        b.flux = S.dalloc<float>(7 * CN);
+        b.cg = S.dalloc<float>(CN);
+        b.cterm = S.dalloc<float>(CN);
+        b.eflux = S.dalloc<double>(7 * CN);
+    }
     if (d.ns) S.cable = false;   // a network with synapses is not a cable: the mode ends with it
     d.cable = S.cable ? 1 : 0;
     S.conn_scale_dev = nullptr;
@@ -1128,6 +1182,7 @@ static void launch_diffusion(Sim& S, float dt) {
     if (S.class_ntiles[2])
         k_diffusion<256, kRes, kRec><<<S.class_ntiles[2], 256, lds(sizeof(DiffLds<256>), 256, 1), S.stream>>>(
             d, S.tiles[2], dt);
+    if (!kRec && S.big.count) k_diffusion_big<kRes><<<S.big.count, kBigThreads, 0, S.stream>>>(d, S.big, dt);
 }
 
 // The kernel sequence of one tick on S.stream (no host synchronisation or copies, so it can be
@@ -1381,7 +1436,7 @@ static int adjoint_run(Sim& S, Mem& mem, float dt, uint32_t ticks, uint32_t chec
     if (!S.loaded || !S.cable || S.d.ns || S.jac_res) return -1;
     Net& d = S.d;
     const uint32_t n = d.n, P = d.npad, nslot = d.nslot;
-    if (nslot > 8) return -2;
+    if (nslot > 8 || S.big.count) return -2;   // the reverse kernels run on tiles only
     for (uint32_t h = 0; h < n; h++) {
         const Segment& sg = S.host.segments[h];
         for (uint32_t j = 0; j < sg.modifier_count; j++)
@@ -1980,6 +2035,58 @@ int bchip_set_jacobi(void* sim, double tol, int max_sweeps) {
     return 0;
 }
 
+// Device evaluation of the tick's exact-arithmetic helpers (see bchip_eval_math).
+__global__ void k_eval_math(int fn, const double* a, const double* b, double* out, uint32_t n) {
+    bc::exp_table_stage();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double r = 0.0;
+        switch (fn) {
+            case 0: r = bc::exp(a[i]); break;
+            case 1: r = bc::exp_t(a[i], bc::ExpTabGlobal{}); break;
+            case 2: r = bc::div_rcp(a[i], b[i], bc::rcp_prep(b[i])); break;
+            case 3: {
+                const float x = (float)a[i], y = (float)b[i];
+                r = bc::div_wide(x, y, 1.0 / (double)y);
+                break;
+            }
+            case 4: {
+                const double y = (double)(float)b[i];
+                r = bc::div_f24(a[i], y, 1.0 / y);
+                break;
+            }
+            case 5: r = bc::bernoulli((float)a[i], bc::ExpTabGlobal{}); break;
+            default: break;
+        }
+        out[i] = r;
+    }
+}
+
+int bchip_eval_math(int fn, const double* a, const double* b, double* out, uint32_t n) {
+    if (fn < 0 || fn > 5 || !a || !out || ((fn >= 2 && fn <= 4) && !b)) return -1;
+    if (!n) return 0;
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    int rc = 0;
+    try {
+        const size_t B = (size_t)n * 8;
+        HIPCHK(hipMalloc(&da, B));
+        HIPCHK(hipMalloc(&dout, B));
+        HIPCHK(hipMemcpy(da, a, B, hipMemcpyHostToDevice));
+        if (b) {
+            HIPCHK(hipMalloc(&db, B));
+            HIPCHK(hipMemcpy(db, b, B, hipMemcpyHostToDevice));
+        }
+        k_eval_math<<<std::min<uint32_t>((n + 255) / 256, 8192), 256>>>(fn, da, db, dout, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(out, dout, B, hipMemcpyDeviceToHost));
+    } catch (...) {
+        rc = -1;
+    }
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dout);
+    return rc;
+}
+
 int bchip_allow_unsupported(void* sim, int on) {
     if (!sim) return -1;
     static_cast<Sim*>(sim)->allow_unsupported = on != 0;
@@ -2137,6 +2244,7 @@ int bchip_stats(void* sim, BchipStats* out) {
     for (int k = 0; k < 6; k++) out->last_phase_ms[k] = S.last_ms[k];
     out->ordered_events = S.ord_events;
     out->ordered_segments = S.ord_segments;
+    out->big_islands = S.big.count;
     return 0;
 }
 

@@ -28,7 +28,7 @@ void* gpu_backend_create_on_device(int device_index);
 void gpu_backend_destroy(void* sim);
 // 0 ok; -1 NULL args; BION_ERR_* loader codes (bion_loader_host.cpp:14-24); versions 3-14
 // (older records migrated as bion_loader_host.cpp:99-340 does);
-// BCHIP_ERR_ISLAND_TOO_LARGE when a connected island exceeds one workgroup tile;
+// islands of any size (one beyond a 256-segment tile runs in a workgroup of its own);
 // BCHIP_ERR_UNSUPPORTED when the network needs work the device does not do (see below)
 //                                                                (gpu_backend_cuda.cu:338-341)
 int gpu_backend_load_bion(void* sim, const char* path);
@@ -56,7 +56,7 @@ void register_modulator_release(uint8_t type, ReleaseModulatorFunc func);   // p
 #define BCHIP_COHERENCE_COMPAT 0   // AoS uploaded before and refreshed after every tick
 #define BCHIP_COHERENCE_FAST 1     // AoS refreshed only by gpu_backend_bion_data / bchip_save_bion;
                                    // host edits reach the device only via bchip_push_host_state
-#define BCHIP_ERR_ISLAND_TOO_LARGE -20
+#define BCHIP_ERR_ISLAND_TOO_LARGE -20   // no longer returned: islands of any size load
 // A load (gpu_backend_load_bion, bchip_load_bion_data, bchip_load_batch) returns this when the
 // network needs work the device does not do - modifier or receptor types outside the device set
 // (BchipStats.unsupported_modifiers), presynaptic autoreceptors or eCB synthesis
@@ -66,6 +66,13 @@ void register_modulator_release(uint8_t type, ReleaseModulatorFunc func);   // p
 #define BCHIP_ERR_UNSUPPORTED -21
 #define BCHIP_ERR_SCRATCH -22   // bchip_adjoint_noalloc: the call needs more than its fixed array
 int bchip_allow_unsupported(void* sim, int on);
+// Device evaluation, on the current device, of the exact-arithmetic helpers the tick kernels use
+// (for checking them against host glibc and IEEE division): out[i] = fn(a[i], b[i]) with fn
+// 0 exp (table staged in LDS), 1 exp (table read from global memory), 2 a / b in double through
+// the reciprocal (div_rcp), 3 float(a) / float(b) through the double reciprocal (div_wide),
+// 4 a / double(float(b)) (div_f24), 5 the Scharfetter-Gummel Bernoulli function of float(a)
+// (b unused for 0, 1 and 5). Returns 0 or -1.
+int bchip_eval_math(int fn, const double* a, const double* b, double* out, uint32_t n);
 
 typedef struct BchipStats {
     uint32_t num_segments, num_connections, num_synapses, num_tiles;
@@ -81,6 +88,8 @@ typedef struct BchipStats {
                                       // timed tick
     uint32_t ordered_events;          // synapse events the ordered pass walks (last planned dt)
     uint32_t ordered_segments;        // segments it walks (the parallel post pass skips them)
+    uint32_t big_islands;             // islands beyond one tile (one workgroup each in step 2.5)
+    uint32_t _pad2;
 } BchipStats;
 
 int bchip_load_bion_data(void* sim, const BionData* src);   // load from in-memory AoS (copied)
@@ -159,7 +168,8 @@ int bchip_set_coupling_scale(void* sim, const float* scale, uint32_t n);
 // being reversed (5 x 51 floats per segment and tick) when checkpoint_every ticks of them fit in
 // half the free device memory, and otherwise runs each tick's substeps again. Returns 0, -1
 // (arguments or state: cable mode off, synapses present, residual-terminated Jacobi on), -2 (a
-// modifier type outside Nav1.2/Kv-DR/K2P/NALCN/HCN1, or more than 8 modifier slots).
+// modifier type outside Nav1.2/Kv-DR/K2P/NALCN/HCN1, more than 8 modifier slots, or an island
+// beyond one 256-segment tile).
 int bchip_adjoint(void* sim, float dt, uint32_t ticks, uint32_t checkpoint_every, const uint32_t* soma,
                   uint32_t ncells, const float* target, double* loss, double* grad_density, double* grad_scale,
                   float* trace);

@@ -167,3 +167,26 @@ def test_remaining_gpcr_families():
     rs, _, sp = _both(net, 240, lambda t: H.Stimuli(current=clamp, lig=bion.gpcr_ligands(net, t)))
     assert sp.sum() > 0
     assert (rs["neurotransmitter"] > 0).all(axis=0).sum() >= 10
+
+
+@pytest.mark.parametrize("synapses", [True, False])
+def test_islands_beyond_a_tile(synapses):
+    """One island of ~300 segments and ~1700 connections, glia coupled extracellularly to the
+    neurons of three spinal segments as the reference's spinal network builder does
+    (include/cpg_config.h:657-713), with synapses and, without, with transmitter diffusion over
+    the coupling; and a 1000-compartment cell next to small neurons."""
+    net = bion.spinal_glia_network(3, synapses=synapses)
+    clamp = H.soma_clamp(net, 180.0)
+
+    def stim(t):
+        lig = None
+        if not synapses and t % 10 == 2:
+            lig = np.zeros(2, bion.LIG_INJ_DT)
+            lig["segment_id"], lig["nt_idx"], lig["amount"] = [5, 40], [0, 3], 0.5
+        return H.Stimuli(current=clamp, lig=lig)
+    _, _, sp = _both(net, 150, stim)
+    assert sp.sum() > 0
+    cell = bion.long_cell(4)
+    clamp = H.soma_clamp(cell, 150.0)
+    _, _, sp = _both(cell, 150, lambda t: H.Stimuli(current=clamp))
+    assert sp.sum() > 0
