@@ -21,6 +21,10 @@
 #include <cstring>
 #include <vector>
 #include "../include/biocore_abi.h"
+// The record fields' own float type; f64.h (the FP64 build) promotes every other float.
+#ifndef ORC_F64
+typedef float abi_float;
+#endif
 
 namespace orc {
 
@@ -2207,7 +2211,7 @@ struct Ctx {
 
 // kernel_dispatch.h:651-663: Munc13-1 (gene 42) sets RRP_max and the recovery rate, which
 // complexin (44) lowers; synaptotagmin-1 (40) the release probability.
-static void protein_map(Synapse* y, const float* protein) {
+static void protein_map(Synapse* y, const abi_float* protein) {
     float munc13 = protein[42];
     y->RRP_max = 10.0f + munc13 * 40.0f;
     float syt1 = protein[40];
@@ -2481,7 +2485,7 @@ static void diffusion(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint
 
 // ---- step 3 helpers (delay_buffer.h, kernel_dispatch.h:612-663, modifier_helpers.h:764-836) ----
 #pragma pack(push, 1)
-struct Delayed { uint64_t arrival; float amount; uint8_t nt; uint8_t pad[3]; };
+struct Delayed { uint64_t arrival; abi_float amount; uint8_t nt; uint8_t pad[3]; };
 #pragma pack(pop)
 static_assert(sizeof(Delayed) == 16, "delay slot");
 
@@ -2807,7 +2811,7 @@ static void run_tick(Ctx& c, Segment* sg, uint32_t n, IntraConnection* cn, uint3
                                                     : 2.0f * kPi * s->radius * s->length;
             if (c.cable) {   // held concentrations: every channel sees the stored ones
                 double in[BC_ION_MAX], ex[BC_ION_MAX];
-                float nt[15];
+                abi_float nt[15];
                 const float atp = s->ATP_level;
                 memcpy(in, s->intra_ions, sizeof in);
                 memcpy(ex, s->extra_ions, sizeof ex);
@@ -3394,5 +3398,13 @@ double oracle_cable64_loss(void* h, float dt, uint32_t ticks, const uint32_t* so
 }
 void oracle_cable64_destroy(void* h) { delete static_cast<orc::Cab64*>(h); }
 float oracle_bernoulli(float x) { return orc::bernoulli(x); }
+// Array forms for checking device evaluations: host glibc ::exp (the function the reference
+// calls), and the Bernoulli function of diffusion_kernel.h:108-113.
+void oracle_exp_n(const double* x, double* y, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) y[i] = ::exp(x[i]);
+}
+void oracle_bernoulli_n(const double* x, double* y, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) y[i] = orc::bernoulli((float)x[i]);
+}
 
 }  // extern "C"

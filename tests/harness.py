@@ -194,6 +194,48 @@ def _ref_child(conn, net, ticks, dt, stim_fn, record):
     conn.close()
 
 
+def _ref_consolidation_child(conn, net, nuclei, ticks, dt, stim_fn):
+    lib = _ref_lib()
+    for f in ("core_init_consolidation", "core_start_consolidation"):
+        getattr(lib, f).restype = None
+    lib.core_set_consolidation_interval.argtypes = [C.c_uint32]
+    lib.core_init_consolidation.argtypes = [P, C.c_uint32, P, C.c_uint32]
+    lib.core_start_consolidation.argtypes = [P, C.c_uint32, P, P, C.c_uint32, P]
+    n = net
+    tick = np.zeros(1, np.uint64)
+    nsyn = np.array([len(n.synapses)], np.uint32)
+    nuc = np.ascontiguousarray(nuclei)
+    lib.core_set_consolidation_interval(0)   # the thread sleeps: nucleus proteins stay as given
+    lib.core_init_consolidation(_ptr(n.neurons), len(n.neurons), nuc.ctypes.data_as(P), len(nuc))
+    lib.core_start_consolidation(_ptr(n.segments), len(n.segments), _ptr(n.synapses), nsyn.ctypes.data_as(P),
+                                 len(n.synapses), tick.ctypes.data_as(P))
+    mech = np.zeros(4, np.uint8)
+    mcount = np.zeros(1, np.uint32)
+    for t in range(ticks):
+        stim = stim_fn(t) if stim_fn else None
+        st = stim.args() if stim is not None else [None] * 10
+        lib.core_run_tick(_ptr(n.segments), len(n.segments), _ptr(n.connections), len(n.connections),
+                          _ptr(n.synapses), len(n.synapses), n.mode, tick.ctypes.data_as(P), dt, *st,
+                          mech.ctypes.data_as(P), mcount.ctypes.data_as(P))
+    lib.core_stop_consolidation()
+    conn.send((n.segments, n.synapses))
+    conn.close()
+
+
+def run_reference_consolidating(net: bion.Network, nuclei: np.ndarray, ticks: int, dt: float, stim_fn=None):
+    """The compiled reference with its consolidation started on these nuclei and its interval 0
+    (the consolidation thread never runs a cycle, so the protein levels stay as given while the
+    tick applies the protein -> release-parameter mapping), in a forked child; returns
+    (segments, synapses)."""
+    ctx = mp.get_context("fork")
+    a, b = ctx.Pipe(duplex=False)
+    p = ctx.Process(target=_ref_consolidation_child, args=(b, net.copy(), nuclei, ticks, dt, stim_fn))
+    p.start()
+    out = a.recv()
+    p.join()
+    return out
+
+
 def run_reference(net: bion.Network, ticks: int, dt: float, stim_fn=None, record=None):
     """Run the compiled reference for `ticks` ticks in a forked child; returns
     (segments, synapses, spike_counts, per-tick records)."""
@@ -204,6 +246,35 @@ def run_reference(net: bion.Network, ticks: int, dt: float, stim_fn=None, record
     out = a.recv()
     p.join()
     return out
+
+
+_REF_INPROC_USED = False
+
+
+def run_reference_inproc(net: bion.Network, ticks: int, dt: float, stim_fn=None):
+    """The compiled reference stepping `net` in place in this process (no fork: networks of tens
+    of GB do not travel through a pipe). Its delay ring and spike-counter pointer are process
+    statics, so this may run once per process; returns the spike counts, or None when the
+    reference is absent or was already used here."""
+    global _REF_INPROC_USED
+    if _REF_INPROC_USED or not ref_available():
+        return None
+    _REF_INPROC_USED = True
+    lib = _ref_lib()
+    n = net
+    tick = np.zeros(1, np.uint64)
+    spikes = np.zeros(len(n.segments), np.uint32)
+    lib.core_enable_spike_count(spikes.ctypes.data_as(P), len(spikes))
+    mech = np.zeros(4, np.uint8)
+    mcount = np.zeros(1, np.uint32)
+    for t in range(ticks):
+        stim = stim_fn(t) if stim_fn else None
+        st = stim.args() if stim is not None else [None] * 10
+        lib.core_run_tick(_ptr(n.segments), len(n.segments), _ptr(n.connections), len(n.connections),
+                          _ptr(n.synapses), len(n.synapses), n.mode, tick.ctypes.data_as(P), dt, *st,
+                          mech.ctypes.data_as(P), mcount.ctypes.data_as(P))
+    lib.core_enable_spike_count(None, 0)
+    return spikes
 
 
 def run_oracle(net: bion.Network, ticks: int, dt: float, stim_fn=None, record=None):

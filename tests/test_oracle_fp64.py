@@ -51,3 +51,19 @@ def test_fp64_agrees_with_mixed_precision():
     assert err["V_m"] < 1e-4 and err["ATP"] < 1e-4 and err["temperature"] < 1e-6
     assert max(err[f"intra[{k}]"] for k in range(7)) < 1e-6
     assert max(err[f"extra[{k}]"] for k in range(7)) < 1e-9
+
+
+def test_fp64_build_agrees_with_passive_restatement():
+    """The oracle's FP64 build (every float quantity in double) against the independent FP64
+    restatement of passive networks on 100-segment cables with +-20% ramps, 100 ticks: V_m and
+    every ion to 1e-8 relative; ATP to float storage (the build keeps the record's float ATP)."""
+    import f64full
+    net = bion.config3(n_cables=4, seg_per_cable=100)
+    ref = fp64.fp64_run(net.copy(), 100, 0.025)
+    b = net.copy()
+    f64full.run(b, 100, 0.025)
+    err = fp64.field_errors(b.segments, ref)
+    print({k: f"{v:.1e}" for k, v in err.items()})
+    assert err["V_m"] < 1e-8 and err["temperature"] < 1e-8
+    assert max(v for k, v in err.items() if k.startswith(("intra", "extra"))) < 1e-8
+    assert err["ATP"] < 1e-5

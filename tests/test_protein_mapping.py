@@ -4,7 +4,8 @@ synapse whose presynaptic neuron has a nucleus takes RRP_max, the release probab
 RRP_recovery from that nucleus's Munc13-1 (gene 42), synaptotagmin-1 (40) and complexin (44).
 
 - CPU: the oracle's mapped parameters equal the formula evaluated independently in float32,
-  only for chemical synapses of neurons with a nucleus;
+  only for chemical synapses of neurons with a nucleus; and the oracle equals the compiled
+  reference run with its consolidation started (interval 0: nucleus proteins fixed) byte for byte;
 - GPU: the device with mapping switched on mid-run equals the oracle bit for bit.
 """
 import numpy as np
@@ -54,6 +55,24 @@ def test_oracle_mapping_closed_form():
     assert np.any(o.net.synapses["RRP_max"][chem] != net.synapses["RRP_max"][chem])
     assert np.any(o.net.synapses["RRP_recovery"] == np.float32(0.005))
     assert np.any(o.net.synapses["RRP_recovery"] == np.float32(0.2))
+
+
+@pytest.mark.skipif(not H.ref_available(), reason="oracle/_ref not built (reference absent)")
+def test_oracle_mapping_against_the_reference():
+    """The compiled reference with its consolidation started on the nuclei (interval 0, so the
+    thread never changes them) against the oracle with the mapping on: every byte of segments and
+    synapses over 200 ticks, releases included."""
+    net, nuc = _net_and_nuclei(62)
+    clamp = H.soma_clamp(net, 180.0)
+    rs, ry = H.run_reference_consolidating(net, nuc, 200, DT, lambda t: H.Stimuli(current=clamp))
+    o = H.OracleSim(net.copy())
+    o.set_protein_mapping(nuc)
+    for _ in range(200):
+        o.step(DT, H.Stimuli(current=clamp))
+    assert H.first_mismatch(rs, o.net.segments) is None, H.first_mismatch(rs, o.net.segments)
+    assert H.first_mismatch(ry, o.net.synapses) is None, H.first_mismatch(ry, o.net.synapses)
+    chem = net.synapses["type"] == 0
+    assert np.any(ry["RRP_max"][chem] != net.synapses["RRP_max"][chem])   # the mapping took effect
 
 
 @pytest.mark.gpu
