@@ -15,6 +15,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import resource
 import sys
 import time
 
@@ -102,6 +103,27 @@ def measured_traffic(kernel, neurons):
     return None
 
 
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a SIMD for 4 cycles (FP32 and FP64
+# alike without packed math); 2.4 GHz peak engine clock.
+VALU_SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
+
+
+def measured_valu(kernel, neurons):
+    """VALU instructions per launch of `kernel` (wavefront instructions, SQ_INSTS_VALU) from the
+    committed rocprofv3 counter summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_tick_config2_3p2M.md")
+    if neurons != 100_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == kernel:
+            try:
+                return float(cells[3])
+            except ValueError:
+                return None
+    return None
+
+
 def build_workload(neurons, ticks, seed=0xB10C0002):
     net = bion.config2(n_neurons=neurons, seed=seed)
     pulses = bion.poisson_soma_pulses(net, 5.0, ticks, 0.025, 1000.0, seed + 1)
@@ -146,46 +168,67 @@ def cpu_baseline(budget_s=12.0, neurons=250):
                       f"{k} ticks in {el:.1f}s, same builder and stimuli as the GPU workload"}
 
 
-def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004, graphs=True):
+def _peak_rss_gb():
+    return resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6   # kB -> GB
+
+
+def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004, graphs=True, plain_ticks=None):
     """The batched sweep: n_networks networks built like the 64-neuron chain network, segment
     counts log-uniform in 2k-200k, each with its own seed and g_max scaling and a 200 pA soma
     clamp. Plain: one simulation per network, ticked one network at a time (each tick is that
-    network's own launch sequence and synchronisation). Batched: all networks in one
-    simulation, autotuned schedule, ticks replayed from a captured hipGraph. Both are bit-exact
-    to the per-network CPU tick; throughput counts segment-ticks of all networks."""
+    network's own launch sequence and synchronisation), for plain_ticks ticks (a rate: every
+    network's tick costs the same at any tick count). Batched: all networks in one simulation,
+    autotuned schedule, `ticks` ticks replayed from captured hipGraphs. Both are bit-exact to the
+    per-network CPU tick; throughput counts segment-ticks of all networks. Each network is built,
+    loaded and dropped in turn for the plain leg and built again for the batch, so the host holds
+    at most the networks plus one copy of their records."""
     neurons = bion.config4_neurons(n_networks, seed)
+    plain_ticks = plain_ticks or ticks
     t0 = time.time()
-    nets = [bion.config4_network(k, int(n), seed) for k, n in enumerate(neurons)]
-    stims = [bion.soma_current(net, 200.0) for net in nets]
-    n_seg = int(sum(len(n.segments) for n in nets))
-    n_syn = int(sum(len(n.synapses) for n in nets))
-    print(f"# batch: {n_networks} networks, {n_seg} segments, {n_syn} synapses built in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
-    sims = []
-    for net, st in zip(nets, stims):
+    sims, stims = [], []
+    n_seg = n_syn = 0
+    for k, nn in enumerate(neurons):
+        net = bion.config4_network(k, int(nn), seed)
+        st = bion.soma_current(net, 200.0)
+        n_seg += len(net.segments)
+        n_syn += len(net.synapses)
         s = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
         if s.load_network(net) != 0:
             raise SystemExit("plain load failed")
+        del net
         s.set_stimuli(0, current=st)
         s.set_graphs(False)
         sims.append(s)
+        stims.append(st)
+        if (k + 1) % 64 == 0:
+            print(f"# plain: {k + 1} networks loaded ({time.time() - t0:.0f}s, peak RSS {_peak_rss_gb():.1f} GB)",
+                  file=sys.stderr, flush=True)
+    print(f"# batch: {n_networks} networks, {n_seg} segments, {n_syn} synapses built and loaded one by one "
+          f"in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     for s in sims:
         for _ in range(2):
             s.run_tick(dt)
     t0 = time.perf_counter()
-    for _ in range(ticks):
+    for _ in range(plain_ticks):
         for s in sims:
             s.run_tick(dt)
-    plain = n_seg * ticks / (time.perf_counter() - t0)
+    plain = n_seg * plain_ticks / (time.perf_counter() - t0)
     for s in sims:
         s.close()
     del sims
-    b = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
     t0 = time.time()
+    nets = []
+    for k, nn in enumerate(neurons):
+        nets.append(bion.config4_network(k, int(nn), seed))
+        if (k + 1) % 64 == 0:
+            print(f"# batch: {k + 1} networks built ({time.time() - t0:.0f}s, peak RSS {_peak_rss_gb():.1f} GB)",
+                  file=sys.stderr, flush=True)
+    b = backend.Simulation(device=0, coherence=backend.COHERENCE_FAST)
     if b.load_batch(nets) != 0:
         raise SystemExit("batch load failed")
-    print(f"# batch loaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     del nets
+    print(f"# batch built and loaded in {time.time() - t0:.1f}s (peak RSS {_peak_rss_gb():.1f} GB)",
+          file=sys.stderr, flush=True)
     for k, st in enumerate(stims):
         b.set_stimuli(k, current=st)
     b.set_graphs(graphs)
@@ -197,7 +240,7 @@ def batched_vs_plain(n_networks, ticks, dt=0.025, seed=0xB10C0004, graphs=True):
     sched = b.schedule().as_dict()
     b.close()
     return {"networks": n_networks, "segments": n_seg, "synapses": n_syn, "ticks": ticks,
-            "plain_segment_ticks_per_s": plain, "batched_segment_ticks_per_s": batched,
+            "plain_ticks": plain_ticks, "plain_segment_ticks_per_s": plain, "batched_segment_ticks_per_s": batched,
             "batched_vs_plain": batched / plain, "autotune_ticks": tuned, "schedule": sched}
 
 
@@ -284,9 +327,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--neurons", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch-networks", type=int, default=64,
-                    help="networks in the batched-vs-plain comparison (512 = the full batched sweep)")
-    ap.add_argument("--batch-ticks", type=int, default=64)
+    ap.add_argument("--batch-networks", type=int, default=512,
+                    help="networks in the batched-vs-plain comparison (512: the batched sweep's ~20M segments)")
+    ap.add_argument("--batch-ticks", type=int, default=1000, help="ticks of the batched run")
+    ap.add_argument("--plain-ticks", type=int, default=50, help="ticks of the one-network-at-a-time run")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-vs-plain comparison")
     ap.add_argument("--stress-cables", type=int, default=10_000,
                     help="100-segment cables in the electrodiffusion stress leg (0: skip)")
@@ -317,8 +361,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.batch_only:
-        print(json.dumps(batched_vs_plain(args.batch_networks, args.batch_ticks, graphs=not args.no_graphs)),
-              flush=True)
+        print(json.dumps(batched_vs_plain(args.batch_networks, args.batch_ticks, graphs=not args.no_graphs,
+                                          plain_ticks=args.plain_ticks)), flush=True)
         return
     if args.stress_only:
         print(json.dumps(electrodiffusion_stress(args.stress_cables, args.stress_ticks, graphs=not args.no_graphs)),
@@ -362,6 +406,8 @@ def main():
     name = PHASES[dom]
     achieved = algo[name] / (mean[dom] * 1e-3) / 1e9
     traffic = measured_traffic(name, args.neurons)
+    valu = measured_valu(name, args.neurons)
+    valu_frac = (valu * VALU_CYCLES / VALU_SIMDS / CLOCK_HZ) / (mean[dom] * 1e-3) if valu else None
     tick_gbs = tick_bytes / (ms_step * 1e-3) / 1e9
     out = {
         "metric": "segment-ticks/s @3.2M seg (HBM roofline %); batched vs plain; adjoint/fwd cost",
@@ -373,13 +419,17 @@ def main():
                    "neurons": args.neurons, "segments": n_seg},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": algo[name], "launch_ms": float(mean[dom])},
+                     "algorithmic_bytes": algo[name], "launch_ms": float(mean[dom]),
+                     # what limits it: the share of the launch its VALU instructions occupy every
+                     # SIMD (SQ_INSTS_VALU x 4 cycles / 1024 SIMDs / clock / duration)
+                     "limiter": "valu" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",
+                     "valu_frac": valu_frac, "valu_insts": valu},
         "tick_roofline": {"model_bytes_per_tick": tick_bytes, "achieved": tick_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": tick_gbs / HBM_PEAK_GBS},
         "phase_ms": {PHASES[k]: float(mean[k]) for k in range(len(PHASES))},
     }
     if args.batch_networks > 0:
-        out["batched"] = batched_vs_plain(args.batch_networks, args.batch_ticks)
+        out["batched"] = batched_vs_plain(args.batch_networks, args.batch_ticks, plain_ticks=args.plain_ticks)
     if args.stress_cables > 0:
         out["electrodiffusion"] = electrodiffusion_stress(args.stress_cables, args.stress_ticks)
     if args.adjoint_cells > 0:
