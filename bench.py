@@ -399,6 +399,8 @@ def main():
         phase += np.array(sim.stats().last_phase_ms[:])
     el = time.perf_counter() - t0
     sim.set_phase_timing(False)
+    sim.close()   # its host records and device state are not needed by the other legs
+    del sim, stims
     ms_step = el / args.steps * 1e3
     value = n_seg * args.steps / el
     mean = phase / args.steps
