@@ -107,8 +107,9 @@ CATION_MORE = (210, 162, 211, 175, 182, 172, 173, 174, 179, 163, 164, 180)   # T
 MISC_MORE = (159, 101, 96)   # Kv3.4, Ca2+ leak, post-inhibitory-rebound T-type Ca2+
 CA_ACT = (84, 144, 145, 146, 152, 186, 190)   # L-type Cav1.2, Cav1.4, CaBP4, CaCC, CAN, AQP4, TMC1
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
+ECB_SYNTHESIS = 86   # 2-AG synthesis; step 3 deposits it into the clefts of incoming synapses
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT
+                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -679,6 +680,65 @@ def gpcr_network(seed: int, n_neurons: int = 8, seg_per_neuron: int = 5) -> Netw
             mods["type"][i, k + q] = t
             mods["density"][i, k + q] = rng.uniform(20.0, 60.0)
         segs["modifier_count"][i] = k + len(GPCR_REST) - half
+    return net
+
+
+# The reference's built-in presynaptic release modulators (receptors_biological.cpp:455-491):
+# 5-HT1A, 5-HT1B, D2, A1, A2A, D1, mGluR2, mGluR3, CB1, H3, nicotinic, alpha2.
+RELEASE_MODULATORS = (18, 19, 22, 27, 46, 44, 29, 30, 36, 38, 13, 35)
+
+
+def modulated_network(seed: int, n_neurons: int = 16, seg_per_neuron: int = 6, autoreceptors: bool = True,
+                      ecb: bool = True) -> Network:
+    """Chain neurons with somatic Cav2.1 and 8 synapses per neuron whose first post receptor is
+    AMPA, 5-HT3, nicotinic, D1 or D2 (so releases put glutamate, serotonin, acetylcholine or
+    dopamine into the cleft). With `autoreceptors`, each synapse carries 0-4 presynaptic
+    autoreceptors drawn from RELEASE_MODULATORS and one ID without a modulator (125), with
+    densities 0.5-15 (some factors reach zero or below), random state[0] and state[7]; clefts hold
+    adenosine, noradrenaline, histamine and eCB at load and pre areas vary, so every modulator
+    sees its transmitter. With `ecb`, local segments 2 and 5 of every neuron and the first soma
+    carry eCB synthesis (86) with random 2-AG levels and rates, so their incoming synapses take
+    the retrograde deposits in synapse order. A few synapses start refractory."""
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], [CAV21], 8.0,
+                        "ampa", 5, seed)
+    rng = np.random.default_rng(seed + 11)
+    y = net.synapses
+    ny = len(y)
+    rt = np.array([AMPA, 14, 13, 44, 22], np.uint8)[rng.integers(0, 5, ny)]
+    pr = y["post_receptors"]
+    pr["type"][:, 0] = rt
+    pr["density"][:, 0] = np.where((rt == 44) | (rt == 22), 40.0, 500.0)
+    y["pre_area"] = rng.uniform(0.05, 0.3, ny).astype(np.float32)
+    refr = rng.random(ny) < 0.1
+    y["refractory"][refr] = rng.integers(1, 40, int(refr.sum()))
+    if autoreceptors:
+        pool = np.array(RELEASE_MODULATORS + (125,), np.uint8)
+        na = rng.integers(0, 5, ny)
+        ar = y["pre_autoreceptors"]
+        for k in range(ny):
+            for j in range(int(na[k])):
+                ar["type"][k, j] = pool[rng.integers(0, len(pool))]
+                ar["density"][k, j] = rng.uniform(0.5, 15.0)
+                ar["state"][k, j, 0] = rng.uniform(0.0, 0.6)
+                ar["state"][k, j, 7] = rng.uniform(-0.1, 0.4)
+        y["pre_autoreceptor_count"] = na
+        some = rng.random(ny) < 0.5
+        for nt in (5, 6, 8, 9):
+            y["cleft_neurotransmitter"][some, nt] = rng.uniform(0.0, 1.5, int(some.sum())).astype(np.float32)
+    if ecb:
+        segs = net.segments
+        local = np.arange(len(segs)) % seg_per_neuron
+        carriers = np.nonzero((local == 2) | (local == 5))[0].tolist() + [0]
+        mods = segs["modifiers"]
+        for i in carriers:
+            k = int(segs["modifier_count"][i])
+            mods["type"][i, k] = ECB_SYNTHESIS
+            mods["density"][i, k] = rng.uniform(0.5, 2.0)
+            mods["state"][i, k, 0] = rng.uniform(0.0, 3.0)
+            if rng.random() < 0.5:
+                mods["state"][i, k, 1] = rng.uniform(0.005, 0.05)
+                mods["state"][i, k, 2] = rng.uniform(0.001, 0.02)
+            segs["modifier_count"][i] = k + 1
     return net
 
 

@@ -148,6 +148,8 @@ struct Sim {
     std::vector<int32_t> h_src;
     std::vector<float> h_rmax;
     std::vector<uint8_t> h_yflags, h_sflags;
+    std::vector<uint32_t> h_ar;   // per slot: first autoreceptor << 4 | count
+    size_t nar = 0;               // autoreceptors on the device
     std::vector<uint32_t> h_glia, h_hist_base, h_cand, h_gaphaz;
     float ord_dt = NAN;   // dt the ordered pass was planned for (NaN: not planned)
     uint32_t ord_events = 0, ord_segments = 0;
@@ -566,6 +568,9 @@ static int build_device(Sim& S) {
     std::vector<uint8_t> y_rtype((size_t)nrec * ns, 0), y_rav((size_t)nrec * ns, 0);
     std::vector<double> y_rdens((size_t)nrec * ns, 0.0), y_rstate((size_t)nrec * 7 * ns, 0.0);
     std::vector<float> y_cleft((size_t)15 * ns, 0.0f);
+    std::vector<uint32_t> y_ar(ns, 0);   // autoreceptors: first << 4 | count
+    std::vector<uint8_t> ar_type;
+    std::vector<double> ar_dens, ar_st0, ar_st7;
     S.unsupported_syn = 0;
     S.hazard_syn = 0;
     S.h_cand.clear();
@@ -583,7 +588,16 @@ static int build_device(Sim& S) {
         uint8_t fl = 0;
         if (y.type == 1 || y.type == 2) { fl |= YF_SKIP | YF_GAP; ngap++; }
         if (y_glia[k] != UINT32_MAX) fl |= YF_GLIA;
-        if (y.pre_autoreceptor_count > 0) S.unsupported_syn++;
+        if (y.pre_autoreceptor_count > 0) {
+            y_ar[k] = (uint32_t)(ar_type.size() << 4) | y.pre_autoreceptor_count;
+            for (uint32_t j = 0; j < y.pre_autoreceptor_count; j++) {
+                const Modifier& m = y.pre_autoreceptors[j];
+                ar_type.push_back(m.type);
+                ar_dens.push_back(m.density);
+                ar_st0.push_back(m.state[0]);
+                ar_st7.push_back(m.state[7]);
+            }
+        }
         for (uint32_t j = 0; j < y.post_receptor_count; j++) {
             uint8_t rt = y.post_receptors[j].type;
             if (rt == 11 || rt == 17 || rt == 12) fl |= YF_INHIBITORY;
@@ -592,6 +606,7 @@ static int build_device(Sim& S) {
             y_rdens[(size_t)j * ns + k] = y.post_receptors[j].density;
             for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
             if (!modifier_supported(rt)) S.unsupported_mod++;
+            if (rt == 86) S.unsupported_mod++;   // eCB synthesis as a receptor would read the post's DAG
             if (gprotein_type(rt) && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
             if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }
@@ -618,7 +633,7 @@ static int build_device(Sim& S) {
         mask |= 1u << (y.post_receptor_count > 0 ? host_nt_of_receptor(y.post_receptors[0].type) : 0);
         const Segment& post = sg[y.post_segment];
         for (uint32_t j = 0; j < post.modifier_count; j++)
-            if (post.modifiers[j].type == 86) { mask |= 1u << 9; S.unsupported_syn++; break; }
+            if (post.modifiers[j].type == 86) { mask |= 1u << 9; break; }   // eCB deposits
         y_mask[k] = mask;
         if (mask & ~3u) sflags[S.host2dev[y.post_segment]] |= SF_NTX;
         // An immediate release whose vesicle recovery reads a presynaptic Ca that an earlier
@@ -820,6 +835,51 @@ static int build_device(Sim& S) {
         d.gap_sid = S.dupload(gsid);
         d.y_igap = S.dalloc<float>(ns);
     }
+    // ---- presynaptic autoreceptors, and the eCB deposit chains in synapse-index order ----
+    d.y_ar = nullptr; d.ar_type = nullptr; d.ar_dens = nullptr; d.ar_st0 = nullptr; d.ar_st7 = nullptr;
+    if (!ar_type.empty()) {
+        if (ar_type.size() >= (1u << 28)) return BION_ERR_ALLOC;
+        d.y_ar = S.dupload(y_ar);
+        d.ar_type = S.dupload(ar_type);
+        d.ar_dens = S.dupload(ar_dens);
+        d.ar_st0 = S.dupload(ar_st0);
+        d.ar_st7 = S.dupload(ar_st7);
+    }
+    S.nar = ar_type.size();
+    S.h_ar = std::move(y_ar);
+    d.y_ecb = nullptr; d.seg_ecb = nullptr;
+    {
+        std::vector<uint8_t> seg_ecb(n, 0xFF);   // the first eCB synthesis slot
+        bool any = false;
+        for (uint32_t h = 0; h < n; h++)
+            for (uint32_t j = 0; j < sg[h].modifier_count; j++)
+                if (sg[h].modifiers[j].type == 86) {
+                    seg_ecb[S.host2dev[h]] = (uint8_t)j;
+                    any = true;
+                    break;
+                }
+        if (any && ns) {
+            std::vector<uint32_t> y_ecb(ns, EC_NONE), cnt(n, 0), last(n, UINT32_MAX);
+            for (uint32_t i = 0; i < ns; i++) {
+                const Synapse& y = sy[i];
+                if (y.type == 1 || y.type == 2 || y.pre_segment >= n || y.post_segment >= n) continue;
+                const uint32_t p = S.host2dev[y.post_segment];
+                if (seg_ecb[p] == 0xFF) continue;
+                const uint32_t k = syn2slot[i];
+                y_ecb[k] = cnt[p]++;
+                last[p] = k;
+            }
+            for (uint32_t p = 0; p < n; p++)
+                if (last[p] != UINT32_MAX) y_ecb[last[p]] |= EC_LAST;
+            d.y_ecb = S.dupload(y_ecb);
+            d.seg_ecb = S.dupload(seg_ecb);
+            if (!d.y_post) {
+                std::vector<uint32_t> y_post(ns);
+                for (uint32_t k = 0; k < ns; k++) y_post[k] = S.host2dev[sy[S.slot2syn[k]].post_segment];
+                d.y_post = S.dupload(y_post);
+            }
+        }
+    }
     // ---- glial captures: per receiving segment, its synapses in synapse-index order ----
     d.y_cap = nullptr; d.y_capm = nullptr;
     d.glia_off = nullptr; d.glia_slot = nullptr; d.glia_sid = nullptr;
@@ -976,6 +1036,7 @@ static void xfer(Sim& S, bool to_device) {
     std::vector<float> yw(ns), yf(ns), yd(ns), yr(ns), yb(ns), ys(ns), yk(ns), yc(ns), ycl((size_t)15 * ns);
     std::vector<uint8_t> yrf(ns);
     std::vector<double> yrs((size_t)nrec * 7 * ns);
+    std::vector<double> a0(S.nar), a7(S.nar);   // autoreceptor state[0] and state[7]
     if (to_device) {
         for (uint32_t di = 0; di < n; di++) {
             const Segment& s = B.segments[S.dev2host[di]];
@@ -993,6 +1054,10 @@ static void xfer(Sim& S, bool to_device) {
             for (int q = 0; q < 15; q++) ycl[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
             for (uint32_t j = 0; j < y.post_receptor_count; j++)
                 for (int q = 0; q < 7; q++) yrs[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
+            for (uint32_t j = 0, a = S.nar ? S.h_ar[k] : 0; j < (a & 15u); j++) {
+                a0[(a >> 4) + j] = y.pre_autoreceptors[j].state[0];
+                a7[(a >> 4) + j] = y.pre_autoreceptors[j].state[7];
+            }
         }
     }
     cp(d.V, V.data(), (size_t)npad * 8);
@@ -1024,6 +1089,8 @@ static void xfer(Sim& S, bool to_device) {
         cp(d.y_refr, yrf.data(), ns);
         cp(d.y_cleft, ycl.data(), ycl.size() * 4);
         cp(d.y_rstate, yrs.data(), yrs.size() * 8);
+        cp(d.ar_st0, a0.data(), a0.size() * 8);
+        cp(d.ar_st7, a7.data(), a7.size() * 8);
     }
     HIPCHK(hipStreamSynchronize(S.stream));
     if (to_device) return;
@@ -1060,6 +1127,10 @@ static void xfer(Sim& S, bool to_device) {
             uint8_t t = m.type;
             bool blocked = my && ionotropic_type(t);
             if (ran && !(y.type == 1 || y.type == 2) && !blocked) m.state[7] = y.post_area;
+        }
+        for (uint32_t j = 0, a = S.nar ? S.h_ar[k] : 0; j < (a & 15u); j++) {
+            y.pre_autoreceptors[j].state[0] = a0[(a >> 4) + j];
+            y.pre_autoreceptors[j].state[7] = a7[(a >> 4) + j];
         }
     }
 }
@@ -1319,8 +1390,8 @@ static int load_into(Sim& S, BionData* src_owned) {
     if ((S.unsupported_mod || S.unsupported_syn) && !S.allow_unsupported) {
         fprintf(stderr,
                 "biocore_hip: load refused: %llu modifier/receptor instance(s) outside the device set and "
-                "%llu synapse(s) with autoreceptors or eCB synthesis (bchip_allow_unsupported loads them "
-                "as zero current / skipped)\n",
+                "%llu unsupported synapse(s) (bchip_allow_unsupported loads them as zero current / "
+                "skipped)\n",
                 (unsigned long long)S.unsupported_mod, (unsigned long long)S.unsupported_syn);
         HIPCHK(hipStreamSynchronize(S.stream));
         S.free_device();

@@ -95,7 +95,7 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
         case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
-        case 145: case 146: case 152: case 186: case 190:
+        case 145: case 146: case 152: case 186: case 190: case 86:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -116,6 +116,7 @@ struct SegRegs {
     float camkii;           // state[0] of the last CaMKII modifier, read by the connexins
     float camkii01;         // its state[0] + state[1], read by Kv4.2
     float cabp4;            // state[0] of the first CaBP4 modifier, set before each Cav1.4 runs
+    float dag;              // state[1] of the first PLC/IP3/DAG modifier, set before eCB synthesis runs
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -1848,6 +1849,26 @@ BC_DEV double ch_cav14(const ModSlot& m, SegRegs& s, float dt) {
     return -ica;
 }
 
+// eCB synthesis (86, hh_biological.cpp:1746-1764): 2-AG from Ca (Hill 2, Kd 0.5 uM) scaled by
+// 1 + DAG (s.dag), degraded at rate state[2] (0.005 if not positive), synthesis rate state[1]
+// (0.01 if not positive), dt scaled by the density, state[0] clamped to [0, 10]; no current.
+// Step 3 deposits the level into the clefts of the segment's incoming synapses.
+BC_DEV double ch_ecb(const ModSlot& m, const SegRegs& s, float dt) {
+    float ca = s.in[2] * kMmToUm;
+    double synth_rate = (m.get(1) > 0.0f) ? m.get(1) : 0.01f;
+    double degrad = (m.get(2) > 0.0f) ? m.get(2) : 0.005f;
+    float dag = s.dag;
+    dt *= m.dens;
+    float can = ca * ca;
+    float synth = synth_rate * can / (can + 0.25f + 1e-9f) * (1.0f + dag);
+    double e = m.get(0);
+    e += (synth - e * degrad) * dt;
+    if (e < 0.0f) e = 0.0f;
+    if (e > 10.0f) e = 10.0f;
+    m.set(0, e);
+    return 0.0;
+}
+
 // CaBP4 (145, hh_biological.cpp:684-693): no current; state[0] binds towards 1 with tau 1 ms.
 BC_DEV double ch_cabp4(const ModSlot& m, float dt) {
     double g0 = m.get(0);
@@ -2351,6 +2372,7 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 84: return ch_ca_l(m, s, dt);
         case 144: return ch_cav14(m, s, dt);
         case 145: return ch_cabp4(m, dt);
+        case 86: return ch_ecb(m, s, dt);
         case 146: return ch_cacc(m, s, dt);
         case 152: return ch_can(m, s, dt);
         case 186: return ch_aqp4(m, s, dt);

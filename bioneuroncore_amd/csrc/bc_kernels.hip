@@ -91,6 +91,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     break;
                 }
         }
+        if (kExt && t == 86) {   // eCB synthesis reads the first PLC/IP3/DAG modifier's state[1]
+            s.dag = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 136) {
+                    s.dag = d.mstate[((size_t)q * 7 + 1) * P + i];
+                    break;
+                }
+        }
         itot += dispatch<kNt, kExt>(t, m, s, dt);
         if (d.cable) {   // concentrations held: every channel sees the stored ones
 #pragma unroll
@@ -1028,6 +1036,50 @@ __device__ static void release_now(const Net& d, uint32_t y, float p, float ca, 
     d.y_relp[y] = -1.0f;   // finished here
 }
 
+// Presynaptic autoreceptors (simulation_host.cpp:430-436): each, in record order, takes the
+// synapse's pre_area as state[7] and multiplies p by its release modulator's factor
+// (receptors_biological.cpp:427-491; the CPU step's dispatch with no plugin registered,
+// kernel_dispatch.h:575-604). Gi/o types: state[0] relaxes to the cleft activation
+// nt / (nt + 0.5) with tau 50 ms, factor 1 - 0.7 * state[0] * density * area; Gs types the same
+// with tau 30 ms and 1 + 0.5 * ...; nicotinic: 1 + 0.3 * activation * density * area, no state.
+// area: state[7] as float, 1 if not positive. Other IDs: factor 1 (p unchanged).
+__device__ static float release_modulators(const Net& d, uint32_t y, uint32_t a, float p, float dt) {
+    const size_t NS = d.ns;
+    const float pa = d.y_pre_area[y];
+    for (uint32_t k = a >> 4, e = (a >> 4) + (a & 15u); k < e; k++) {
+        d.ar_st7[k] = pa;
+        int nt, kind;   // kind 0: Gi/o, 1: Gs, 2: ionotropic
+        switch (d.ar_type[k]) {
+            case 18: case 19: nt = 3; kind = 0; break;   // 5-HT1A/1B: serotonin
+            case 22: nt = 2; kind = 0; break;            // D2: dopamine
+            case 27: nt = 5; kind = 0; break;            // A1: adenosine
+            case 46: nt = 5; kind = 1; break;            // A2A
+            case 44: nt = 2; kind = 1; break;            // D1
+            case 29: case 30: nt = 0; kind = 0; break;   // mGluR2/3: glutamate
+            case 36: nt = 9; kind = 0; break;            // CB1: endocannabinoid
+            case 38: nt = 8; kind = 0; break;            // H3: histamine
+            case 13: nt = 4; kind = 2; break;            // nicotinic: acetylcholine
+            case 35: nt = 6; kind = 0; break;            // alpha2: noradrenaline
+            default: continue;
+        }
+        const float c = d.y_cleft[nt * NS + y];
+        const float act = c / (c + 0.5f);
+        const float area = (pa > 0.0f) ? pa : 1.0f;
+        const double scale = d.ar_dens[k] * area;
+        float f;
+        if (kind == 2) {
+            f = 1.0f + 0.3f * act * scale;
+        } else {
+            double s0 = d.ar_st0[k];
+            s0 += (act - s0) * dt / (kind == 0 ? 50.0f : 30.0f);
+            d.ar_st0[k] = s0;
+            f = kind == 0 ? 1.0f - 0.7f * s0 * scale : 1.0f + 0.5f * s0 * scale;
+        }
+        p *= f;
+    }
+    return p;
+}
+
 __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     exp_table_stage();
     const uint64_t tick = *d.tick;
@@ -1056,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
             }
             d.d_mask[y] = mask;
         }
-        // bouton Ca microdomain (modifier_helpers.h:764-804), no presynaptic autoreceptors
+        // bouton Ca microdomain (modifier_helpers.h:764-804)
         const PreSum ps = d.presum[d.y_prew[y]];
         float vb = ps.V;
         float bca = d.y_bca[y];
@@ -1091,6 +1143,7 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
             float cu = bca * kMmToUm;
             float act = pow4(cu) / (pow4(cu) + 10000.0);
             float p = d.y_prel[y] * act;
+            if (d.y_ar && d.y_ar[y]) p = release_modulators(d, y, d.y_ar[y], p, dt);
             uint32_t seed = d.y_id[y] ^ ((uint32_t)tick) ^ (uint32_t)(int64_t)(ps.V * 1000.0f);
             if (xorshift_draw(seed) < p) {
                 d.y_relp[y] = p;
@@ -1492,6 +1545,27 @@ __global__ __launch_bounds__(64) void k_syn_ordered(Net d, float dt) {
 }
 
 // ------------------------------------------------------------------ step 3, synapse plasticity
+// eCB deposits (simulation_host.cpp:578-589): each chemical synapse onto a segment with eCB
+// synthesis, in synapse-index order, reads the first such modifier's state[0] as float e and,
+// if e > 0, adds e * 0.1 * dt to its cleft's endocannabinoid and multiplies state[0] by 0.9f.
+// The level synapse y sees is state[0] after the multiplications of its `rank` predecessors;
+// the last synapse writes the level after its own back in k_syn_rrp, when every deposit of the
+// tick has read the stored value.
+__device__ static double ecb_after(double s, uint32_t takes) {
+    for (; takes > 0; takes--) {
+        if (!((float)s > 0.0f)) break;
+        s *= 0.9f;
+    }
+    return s;
+}
+__device__ static size_t ecb_at(const Net& d, uint32_t y) {
+    const uint32_t post = d.y_post[y];
+    return (size_t)d.seg_ecb[post] * 7 * d.npad + post;
+}
+__device__ static float ecb_level(const Net& d, uint32_t y, uint32_t rank) {
+    return (float)ecb_after(d.mstate[ecb_at(d, y)], rank);
+}
+
 // The part of the post-side step that depends only on the synapse's own state and the receptor
 // current the ordered pass stored: spine Ca, cleft clearance, BCM threshold, spine
 // CaMKII/calcineurin and the weight (simulation_host.cpp:505-576). One lane per synapse.
@@ -1572,6 +1646,10 @@ __global__ __launch_bounds__(256) void k_syn_plast(Net d, float dt) {
     d.y_camkii[y] = ck;
     d.y_calcn[y] = cn;
     d.y_weight[y] = w;
+    if (d.y_ecb && d.y_ecb[y] != EC_NONE) {   // eCB retrograde deposit (simulation_host.cpp:578-589)
+        const float e = ecb_level(d, y, d.y_ecb[y] & ~EC_LAST);
+        if (e > 0.0f) d.y_cleft[9 * NS + y] += e * 0.1f * dt;
+    }
 }
 
 // ------------------------------------------------------------------ step 3, vesicle pool
@@ -1580,6 +1658,10 @@ __global__ __launch_bounds__(256) void k_syn_rrp(Net d, float dt) {
     const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= d.ns) return;
     if (d.y_flags[y] & YF_SKIP) return;
+    if (d.y_ecb && (d.y_ecb[y] & EC_LAST) && d.y_ecb[y] != EC_NONE) {   // the chain's final eCB level
+        const size_t at = ecb_at(d, y);
+        d.mstate[at] = ecb_after(d.mstate[at], (d.y_ecb[y] & ~EC_LAST) + 1);
+    }
     const float p = d.y_relp[y];
     if (p < 0.0f) return;   // immediate release, finished in k_syn_pre
     const size_t NS = d.ns;

@@ -200,7 +200,20 @@ struct Net {
     uint64_t* d_arr;             // [4][ns]
     float* d_amt;                // [4][ns]
     uint8_t* d_nt;               // [4][ns]
-    // gap junctions (null when the network has none)
+    // presynaptic autoreceptors (null when no synapse has one): the release modulators of slot
+    // y are entries y_ar[y] >> 4 .. + (y_ar[y] & 15) of the ar_* arrays, in record order
+    const uint32_t* y_ar;
+    const uint8_t* ar_type;
+    const double* ar_dens;
+    double* ar_st0;              // state[0]
+    double* ar_st7;              // state[7]: the synapse's pre_area once a release check ran
+    // eCB retrograde deposits (null when no post segment carries eCB synthesis): per slot, the
+    // number of earlier chemical synapses onto the same post segment (EC_LAST: it is the last);
+    // EC_NONE: the post segment has no eCB synthesis. seg_ecb: per segment, the slot index of
+    // its first eCB synthesis modifier.
+    const uint32_t* y_ecb;
+    const uint8_t* seg_ecb;
+    // gap junctions (null when the network has none; y_post also set for eCB deposits)
     const uint32_t* y_post;      // device post segment of each slot
     float* y_igap;               // junction current this tick (pA, into the post, out of the pre)
     const uint32_t* y_sid;       // synapse index of each slot
@@ -236,6 +249,7 @@ struct Net {
 };
 
 enum : uint8_t { HZ_RELEASE = 1, HZ_GAP = 2 };
+enum : uint32_t { EC_LAST = 1u << 31, EC_NONE = ~0u };
 enum : uint8_t {
     EV_POST = 1,     // the post segment is ordered: cleft, captures, receptors, I_buffer, Ca history
     EV_PREGAP = 2,   // gap junction whose presynaptic end is ordered: its current leaves that end

@@ -59,10 +59,9 @@ void register_modulator_release(uint8_t type, ReleaseModulatorFunc func);   // p
 #define BCHIP_ERR_ISLAND_TOO_LARGE -20   // no longer returned: islands of any size load
 // A load (gpu_backend_load_bion, bchip_load_bion_data, bchip_load_batch) returns this when the
 // network needs work the device does not do - modifier or receptor types outside the device set
-// (BchipStats.unsupported_modifiers), presynaptic autoreceptors or eCB synthesis
-// (unsupported_synapses) - instead of running those parts as zero current; a warning names the
-// counts on stderr and bchip_stats reports them. bchip_allow_unsupported(sim, 1) loads such
-// networks anyway (those parts then do nothing on the device).
+// (BchipStats.unsupported_modifiers) - instead of running those parts as zero current; a warning
+// names the counts on stderr and bchip_stats reports them. bchip_allow_unsupported(sim, 1) loads
+// such networks anyway (those parts then do nothing on the device).
 #define BCHIP_ERR_UNSUPPORTED -21
 #define BCHIP_ERR_SCRATCH -22   // bchip_adjoint_noalloc: the call needs more than its fixed array
 int bchip_allow_unsupported(void* sim, int on);
@@ -78,7 +77,7 @@ typedef struct BchipStats {
     uint32_t num_segments, num_connections, num_synapses, num_tiles;
     uint32_t modifier_slots, receptor_slots, lds_bytes_per_tile, _pad;
     uint64_t unsupported_modifiers;   // modifier/receptor instances outside the device set (run as 0 current)
-    uint64_t unsupported_synapses;    // electrical synapses, autoreceptors, eCB (not run on device)
+    uint64_t unsupported_synapses;    // 0: every synapse kind runs on the device (field kept for layout)
     uint64_t order_hazard_synapses;   // synapses whose step-3 effect depends on another post chain's
                                       // progress (immediate releases behind a synapse onto their
                                       // presynaptic segment, gap junctions behind a Ca-changing synapse),

@@ -10,11 +10,12 @@
 // identity is checked by tests/test_oracle_vs_reference.py against oracle/_ref (the reference
 // compiled from its own sources) and against the committed golden fixtures.
 //
-// Scope: modifier IDs {7,8,9,11,103,104,107,112,114,123,129,133,140,141,142,147,148,149,150,
-// 156,183,184,185,198}; chemical synapses without presynaptic autoreceptors, electrical synapses
-// (gap junctions) and glial transmitter capture into GLAST-carrying segments; no consolidation,
-// mechano stimuli or plugins. Anything outside that set is reported by
-// oracle_unsupported_count() and computed as the reference's "unknown ID" default (0 current).
+// Scope: the modifier and receptor IDs dispatch() names; chemical synapses with the reference's
+// twelve built-in presynaptic release modulators, eCB retrograde deposits, electrical synapses
+// (gap junctions) and glial transmitter capture into GLAST-carrying segments; the protein ->
+// release mapping of consolidation; no mechano stimuli or plugins. Anything outside that set is
+// reported by oracle_unsupported_count() and computed as the reference's "unknown ID" default
+// (0 current, state untouched).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -1823,6 +1824,59 @@ static double chan_tmc1(Modifier* m, Segment* s, float dt) {
     return -(ica + ina);
 }
 
+// eCB synthesis (86, hh_biological.cpp:1746-1764): 2-AG from Ca (Hill 2, Kd 0.5 uM) scaled by
+// DAG (state[1] of the segment's first PLC/IP3/DAG modifier, 136; modifier_helpers.h:1496-1500),
+// degraded at state[2]; dt scaled by the density; no current. Step 3 deposits it into clefts.
+static float seg_dag(const Segment* s) {
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == 136) return s->modifiers[j].state[1];
+    return 0.0f;
+}
+static double mod_ecb(Modifier* m, Segment* s, float dt) {
+    float ca = s->intra_ions[2] * kMmToUm;
+    double synth_rate = (m->state[1] > 0.0f) ? m->state[1] : 0.01f;
+    double degrad = (m->state[2] > 0.0f) ? m->state[2] : 0.005f;
+    float dag = seg_dag(s);
+    dt *= m->density;
+    float can = ca * ca;
+    float synth = synth_rate * can / (can + 0.25f + 1e-9f) * (1.0f + dag);
+    m->state[0] += (synth - m->state[0] * degrad) * dt;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 10.0f) m->state[0] = 10.0f;
+    return 0.0;
+}
+
+// Presynaptic release modulators (receptors_biological.cpp:427-491), the CPU step's dispatch
+// with no plugin registered (kernel_dispatch.h:575-604): Gi/o types lower release by
+// 0.7 * state[0] * density * area (state[0] relaxing to the cleft activation nt / (nt + 0.5)
+// with tau 50 ms), Gs types raise it by 0.5 * ... (tau 30 ms), the nicotinic one by
+// 0.3 * activation * density * area with no state. area is state[7] (the caller stores the
+// synapse's pre_area there) read as float, 1 if not positive. Other IDs: factor 1.
+static float release_modulator(Modifier* m, const abi_float* cleft, float dt) {
+    int nt, kind;   // kind 0: Gi/o, 1: Gs, 2: ionotropic
+    switch (m->type) {
+        case 18: case 19: nt = 3; kind = 0; break;   // 5-HT1A, 5-HT1B: serotonin
+        case 22: nt = 2; kind = 0; break;            // D2: dopamine
+        case 27: nt = 5; kind = 0; break;            // A1: adenosine
+        case 46: nt = 5; kind = 1; break;            // A2A
+        case 44: nt = 2; kind = 1; break;            // D1
+        case 29: case 30: nt = 0; kind = 0; break;   // mGluR2/3: glutamate
+        case 36: nt = 9; kind = 0; break;            // CB1: endocannabinoid
+        case 38: nt = 8; kind = 0; break;            // H3: histamine
+        case 13: nt = 4; kind = 2; break;            // nicotinic: acetylcholine
+        case 35: nt = 6; kind = 0; break;            // alpha2: noradrenaline
+        default: return 1.0f;
+    }
+    float c = cleft[nt];
+    float act = c / (c + 0.5f);
+    float area = (m->state[7] > 0.0f) ? (float)m->state[7] : 1.0f;
+    double scale = m->density * area;
+    if (kind == 2) return 1.0f + 0.3f * act * scale;
+    m->state[0] += (act - m->state[0]) * dt / (kind == 0 ? 50.0f : 30.0f);
+    if (kind == 0) return 1.0f - 0.7f * m->state[0] * scale;
+    return 1.0f + 0.5f * m->state[0] * scale;
+}
+
 // AQP4 (186, hh_biological.cpp:4114-4129, modifier_helpers.h:395-396): water follows the
 // osmotic difference of the summed (float) concentrations and dilutes every intracellular ion
 // by a factor clamped to [0.95, 1]; no current, no floor.
@@ -2160,6 +2214,7 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_EAAT3: return chan_transporter(m, s, dt, kEaat3);
         case T_GLYT2: return chan_transporter(m, s, dt, kGlyt2);
         case T_GAT1: return chan_transporter(m, s, dt, kGat1);
+        case T_ECB: return mod_ecb(m, s, dt);
         default:
             if (type >= 17 && type <= 40) return chan_gio(m, s, dt, gio_nt(type));
             if (type >= 41 && type <= 51) return chan_gs(m, s, dt, gs_nt(type));
@@ -2662,7 +2717,6 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
             gap_junction(y, pre, post, dt);
             continue;
         }
-        if (y->pre_autoreceptor_count > 0) g_unsupported++;
         if (!c.nuclei.empty() && pre->neuron_id < c.nuclei.size())
             protein_map(y, c.nuclei[pre->neuron_id].protein_level);
         rrp_recover(y, pre, dt);
@@ -2671,6 +2725,12 @@ static void synapses_step(Ctx& c, Segment* sg, uint32_t n, Synapse* sy, uint32_t
             float cu = y->bouton_Ca * kMmToUm;
             float act = pow(cu, 4.0f) / (pow(cu, 4.0f) + pow(10.0f, 4.0f));
             float p = y->release_probability * act;
+            for (int j = 0; j < y->pre_autoreceptor_count; j++) {   // simulation_host.cpp:430-436
+                Modifier* ar = &y->pre_autoreceptors[j];
+                ar->state[7] = y->pre_area;
+                float f = release_modulator(ar, y->cleft_neurotransmitter, dt);
+                p *= f;
+            }
             uint32_t seed = y->id ^ ((uint32_t)tick) ^ (uint32_t)(int64_t)(pre->V_m * 1000.0f);
             if (xorshift_draw(seed) < p) {
                 uint8_t nt = (y->post_receptor_count > 0) ? nt_of_receptor(y->post_receptors[0].type) : 0;

@@ -279,7 +279,7 @@ def _gpu_two_dt(net, clamp):
 
 def test_unsupported_work_is_refused(tmp_path):
     """A network that needs work the device does not do loads only on request: a modifier type
-    outside the device set (through the drop-in file load) and a presynaptic autoreceptor
+    outside the device set (through the drop-in file load) and a post receptor of such a type
     (in-memory load) return ERR_UNSUPPORTED, with the counts in the stats, unless
     bchip_allow_unsupported was set."""
     lib = backend.load_library()
@@ -298,11 +298,10 @@ def test_unsupported_work_is_refused(tmp_path):
     finally:
         lib.gpu_backend_destroy(h)
     net = bion.config1(seed=9)
-    net.synapses["pre_autoreceptor_count"][0] = 1
-    net.synapses["pre_autoreceptors"]["type"][0, 0] = 17
+    net.synapses["post_receptors"]["type"][0, 0] = bad
     sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
     assert sim.load_network(net.copy()) == backend.ERR_UNSUPPORTED
-    assert sim.stats().unsupported_synapses == 1
+    assert sim.stats().unsupported_modifiers == 1 and sim.stats().unsupported_synapses == 0
     sim.allow_unsupported(True)
     assert sim.load_network(net) == 0
     sim.close()

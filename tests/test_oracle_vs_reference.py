@@ -169,6 +169,24 @@ def test_remaining_gpcr_families():
     assert (rs["neurotransmitter"] > 0).all(axis=0).sum() >= 10
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_release_modulators_and_ecb(seed):
+    """Presynaptic autoreceptors of all twelve built-in release-modulator types (and one ID
+    without a modulator) on synapses that release glutamate, serotonin, acetylcholine and
+    dopamine, with adenosine, noradrenaline, histamine and eCB in the clefts, and eCB synthesis
+    on post segments depositing into their incoming synapses' clefts in synapse order: segment
+    and synapse records (autoreceptor states included) against the reference."""
+    net = bion.modulated_network(seed)
+    clamp = H.soma_clamp(net, 200.0)
+    rs, ry, sp = _both(net, 600, lambda t: H.Stimuli(current=clamp))
+    assert sp.sum() > 0
+    ar0 = net.synapses["pre_autoreceptors"]["state"][:, :, 0]
+    assert (ry["pre_autoreceptors"]["state"][:, :, 0] != ar0).sum() > 100
+    assert (ry["cleft_neurotransmitter"][:, [2, 3, 4, 9]].max(axis=0) > 0).all()
+    e = rs["modifiers"]["type"] == bion.ECB_SYNTHESIS
+    assert (rs["modifiers"]["state"][:, :, 0][e] != net.segments["modifiers"]["state"][:, :, 0][e]).all()
+
+
 @pytest.mark.parametrize("synapses", [True, False])
 def test_islands_beyond_a_tile(synapses):
     """One island of ~300 segments and ~1700 connections, glia coupled extracellularly to the
