@@ -8,7 +8,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <numeric>
 #include <string>
@@ -148,6 +151,8 @@ struct Sim {
     std::vector<int32_t> h_src;
     std::vector<float> h_rmax;
     std::vector<uint8_t> h_yflags, h_sflags;
+    char* stage = nullptr;        // pinned staging of the COMPAT transfers (hipHostMalloc)
+    size_t stage_bytes = 0;
     std::vector<uint32_t> h_ar;   // per slot: first autoreceptor << 4 | count
     size_t nar = 0;               // autoreceptors on the device
     std::vector<uint32_t> h_glia, h_hist_base, h_cand, h_gaphaz;
@@ -212,11 +217,17 @@ struct Sim {
         st_segs[0].clear();
         st_segs[1].clear();
     }
+    void free_stage() {
+        if (stage) (void)hipHostFree(stage);
+        stage = nullptr;
+        stage_bytes = 0;
+    }
     ~Sim() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         free_device();
+        free_stage();
         free_stim_buffers();
         delete stim_host;
         for (auto& e : ev)
@@ -1020,119 +1031,201 @@ static void plan_ordered(Sim& S, float dt) {
 }
 
 // ------------------------------------------------------------------ coherence
+// Host staging of the transferred state: every array below in one block, 256-byte aligned.
+struct XferBufs {
+    double *V, *in, *ex, *ms, *yrs, *a0, *a7;
+    float *atp, *temp, *txn, *nt, *nti, *yw, *yf, *yd, *yr, *yb, *ys, *yk, *yc, *ycl;
+    uint8_t* yrf;
+};
+// Lays the arrays out from base (nullptr: only sizes them); returns the block size.
+static size_t xfer_layout(const Sim& S, char* base, XferBufs& b) {
+    const Net& d = S.d;
+    const size_t npad = d.npad, ns = d.ns;
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t count) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += (count * sizeof(T) + 255) & ~(size_t)255;
+    };
+    take(b.V, npad); take(b.in, 7 * npad); take(b.ex, 7 * npad); take(b.ms, (size_t)d.nslot * 7 * npad);
+    take(b.atp, npad); take(b.temp, npad); take(b.txn, npad); take(b.nt, 15 * npad); take(b.nti, 15 * npad);
+    take(b.yw, ns); take(b.yf, ns); take(b.yd, ns); take(b.yr, ns); take(b.yb, ns); take(b.ys, ns);
+    take(b.yk, ns); take(b.yc, ns); take(b.ycl, 15 * ns); take(b.yrf, ns); take(b.yrs, (size_t)d.nrec * 7 * ns);
+    take(b.a0, S.nar); take(b.a7, S.nar);
+    return off;
+}
+
+// Runs f(lo, hi) over [0, n) in up to 16 host threads, one contiguous range each.
+template <class F>
+static void par_for(size_t n, F f) {
+    unsigned T = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    if (n < 32768 || T == 1) {
+        f((size_t)0, n);
+        return;
+    }
+    const size_t chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < T; k++) {
+        const size_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi) th.emplace_back(f, lo, hi);
+    }
+    f((size_t)0, std::min(n, chunk));
+    for (auto& x : th) x.join();
+}
+
 // Transfers the dynamic state between the host AoS and the device SoA (both directions use
 // the same field list, so what the device computes is exactly what callers can read/write).
+// COMPAT coherence transfers every tick through one pinned staging block kept for the loaded
+// network; other transfers stage in pageable memory. Packing and unpacking run per record
+// range in host threads.
 static void xfer(Sim& S, bool to_device) {
     Net& d = S.d;
     BionData& B = S.host;
     const uint32_t n = d.n, npad = d.npad, ns = d.ns, nslot = d.nslot, nrec = d.nrec;
+    XferBufs b{};
+    const size_t need = xfer_layout(S, nullptr, b);
+    std::unique_ptr<char[]> tmp;
+    char* base;
+    if (S.coherence == BCHIP_COHERENCE_COMPAT) {
+        if (S.stage_bytes < need) {
+            S.free_stage();
+            HIPCHK(hipHostMalloc((void**)&S.stage, need, hipHostMallocDefault));
+            S.stage_bytes = need;
+            par_for(need, [&](size_t lo, size_t hi) { memset(S.stage + lo, 0, hi - lo); });
+        }
+        base = S.stage;
+    } else {
+        tmp.reset(new char[need]);
+        base = tmp.get();
+        if (to_device) par_for(need, [&](size_t lo, size_t hi) { memset(base + lo, 0, hi - lo); });
+    }
+    xfer_layout(S, base, b);
     auto cp = [&](void* dev, void* hostbuf, size_t bytes) {
         if (!bytes) return;
         if (to_device) HIPCHK(hipMemcpyAsync(dev, hostbuf, bytes, hipMemcpyHostToDevice, S.stream));
         else HIPCHK(hipMemcpyAsync(hostbuf, dev, bytes, hipMemcpyDeviceToHost, S.stream));
     };
-    std::vector<double> V(npad), in((size_t)7 * npad), ex((size_t)7 * npad), ms((size_t)nslot * 7 * npad);
-    std::vector<float> atp(npad), temp(npad), txn(npad), nt((size_t)15 * npad), nti((size_t)15 * npad);
-    std::vector<float> yw(ns), yf(ns), yd(ns), yr(ns), yb(ns), ys(ns), yk(ns), yc(ns), ycl((size_t)15 * ns);
-    std::vector<uint8_t> yrf(ns);
-    std::vector<double> yrs((size_t)nrec * 7 * ns);
-    std::vector<double> a0(S.nar), a7(S.nar);   // autoreceptor state[0] and state[7]
     if (to_device) {
-        for (uint32_t di = 0; di < n; di++) {
-            const Segment& s = B.segments[S.dev2host[di]];
-            V[di] = s.V_m;
-            for (int k = 0; k < 7; k++) { in[(size_t)k * npad + di] = s.intra_ions[k]; ex[(size_t)k * npad + di] = s.extra_ions[k]; }
-            atp[di] = s.ATP_level; temp[di] = s.temperature; txn[di] = s.transcription_signal;
-            for (int k = 0; k < 15; k++) { nt[(size_t)k * npad + di] = s.neurotransmitter[k]; nti[(size_t)k * npad + di] = s.intra_nt[k]; }
-            for (uint32_t j = 0; j < s.modifier_count; j++)
-                for (int q = 0; q < 7; q++) ms[((size_t)j * 7 + q) * npad + di] = s.modifiers[j].state[q];
-        }
-        for (uint32_t k = 0; k < ns; k++) {
-            const Synapse& y = B.synapses[S.slot2syn[k]];
-            yw[k] = y.weight; yf[k] = y.facilitation; yd[k] = y.depression; yr[k] = y.RRP; yb[k] = y.bouton_Ca;
-            ys[k] = y.spine_Ca; yk[k] = y.spine_camkii; yc[k] = y.spine_calcineurin; yrf[k] = y.refractory;
-            for (int q = 0; q < 15; q++) ycl[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
-            for (uint32_t j = 0; j < y.post_receptor_count; j++)
-                for (int q = 0; q < 7; q++) yrs[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
-            for (uint32_t j = 0, a = S.nar ? S.h_ar[k] : 0; j < (a & 15u); j++) {
-                a0[(a >> 4) + j] = y.pre_autoreceptors[j].state[0];
-                a7[(a >> 4) + j] = y.pre_autoreceptors[j].state[7];
+        par_for(n, [&](size_t lo, size_t hi) {
+            for (size_t di = lo; di < hi; di++) {
+                const Segment& s = B.segments[S.dev2host[di]];
+                b.V[di] = s.V_m;
+                for (int k = 0; k < 7; k++) {
+                    b.in[(size_t)k * npad + di] = s.intra_ions[k];
+                    b.ex[(size_t)k * npad + di] = s.extra_ions[k];
+                }
+                b.atp[di] = s.ATP_level; b.temp[di] = s.temperature; b.txn[di] = s.transcription_signal;
+                for (int k = 0; k < 15; k++) {
+                    b.nt[(size_t)k * npad + di] = s.neurotransmitter[k];
+                    b.nti[(size_t)k * npad + di] = s.intra_nt[k];
+                }
+                for (uint32_t j = 0; j < s.modifier_count; j++)
+                    for (int q = 0; q < 7; q++) b.ms[((size_t)j * 7 + q) * npad + di] = s.modifiers[j].state[q];
             }
-        }
+        });
+        par_for(ns, [&](size_t lo, size_t hi) {
+            for (size_t k = lo; k < hi; k++) {
+                const Synapse& y = B.synapses[S.slot2syn[k]];
+                b.yw[k] = y.weight; b.yf[k] = y.facilitation; b.yd[k] = y.depression; b.yr[k] = y.RRP;
+                b.yb[k] = y.bouton_Ca; b.ys[k] = y.spine_Ca; b.yk[k] = y.spine_camkii;
+                b.yc[k] = y.spine_calcineurin; b.yrf[k] = y.refractory;
+                for (int q = 0; q < 15; q++) b.ycl[(size_t)q * ns + k] = y.cleft_neurotransmitter[q];
+                for (uint32_t j = 0; j < y.post_receptor_count; j++)
+                    for (int q = 0; q < 7; q++) b.yrs[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
+                for (uint32_t j = 0, a = S.nar ? S.h_ar[k] : 0; j < (a & 15u); j++) {
+                    b.a0[(a >> 4) + j] = y.pre_autoreceptors[j].state[0];
+                    b.a7[(a >> 4) + j] = y.pre_autoreceptors[j].state[7];
+                }
+            }
+        });
     }
-    cp(d.V, V.data(), (size_t)npad * 8);
-    cp(d.in, in.data(), in.size() * 8);
-    cp(d.ex, ex.data(), ex.size() * 8);
-    cp(d.mstate, ms.data(), ms.size() * 8);
-    cp(d.atp, atp.data(), npad * 4);
-    cp(d.temp, temp.data(), npad * 4);
-    cp(d.txn, txn.data(), npad * 4);
-    cp(d.nt, nt.data(), nt.size() * 4);
-    cp(d.nt_in, nti.data(), nti.size() * 4);
-    if (to_device) {   // intra_nt is live once any value is not +0.0
-        d.nt_in_live = 0;
-        for (size_t q = 0; q < nti.size() && !d.nt_in_live; q++) {
-            uint32_t bits;
-            memcpy(&bits, &nti[q], 4);
-            d.nt_in_live = bits != 0;
-        }
-    }
+    cp(d.V, b.V, (size_t)npad * 8);
+    cp(d.in, b.in, (size_t)7 * npad * 8);
+    cp(d.ex, b.ex, (size_t)7 * npad * 8);
+    cp(d.mstate, b.ms, (size_t)nslot * 7 * npad * 8);
+    cp(d.atp, b.atp, (size_t)npad * 4);
+    cp(d.temp, b.temp, (size_t)npad * 4);
+    cp(d.txn, b.txn, (size_t)npad * 4);
+    cp(d.nt, b.nt, (size_t)15 * npad * 4);
+    cp(d.nt_in, b.nti, (size_t)15 * npad * 4);
     if (ns) {
-        cp(d.y_weight, yw.data(), ns * 4);
-        cp(d.y_fac, yf.data(), ns * 4);
-        cp(d.y_dep, yd.data(), ns * 4);
-        cp(d.y_rrp, yr.data(), ns * 4);
-        cp(d.y_bca, yb.data(), ns * 4);
-        cp(d.y_sca, ys.data(), ns * 4);
-        cp(d.y_camkii, yk.data(), ns * 4);
-        cp(d.y_calcn, yc.data(), ns * 4);
-        cp(d.y_refr, yrf.data(), ns);
-        cp(d.y_cleft, ycl.data(), ycl.size() * 4);
-        cp(d.y_rstate, yrs.data(), yrs.size() * 8);
-        cp(d.ar_st0, a0.data(), a0.size() * 8);
-        cp(d.ar_st7, a7.data(), a7.size() * 8);
+        cp(d.y_weight, b.yw, (size_t)ns * 4);
+        cp(d.y_fac, b.yf, (size_t)ns * 4);
+        cp(d.y_dep, b.yd, (size_t)ns * 4);
+        cp(d.y_rrp, b.yr, (size_t)ns * 4);
+        cp(d.y_bca, b.yb, (size_t)ns * 4);
+        cp(d.y_sca, b.ys, (size_t)ns * 4);
+        cp(d.y_camkii, b.yk, (size_t)ns * 4);
+        cp(d.y_calcn, b.yc, (size_t)ns * 4);
+        cp(d.y_refr, b.yrf, ns);
+        cp(d.y_cleft, b.ycl, (size_t)15 * ns * 4);
+        cp(d.y_rstate, b.yrs, (size_t)nrec * 7 * ns * 8);
+        cp(d.ar_st0, b.a0, S.nar * 8);
+        cp(d.ar_st7, b.a7, S.nar * 8);
+    }
+    if (to_device) {   // intra_nt is live once any value is not +0.0 (read while the copy runs)
+        std::atomic<bool> live{false};
+        par_for((size_t)15 * npad, [&](size_t lo, size_t hi) {
+            for (size_t q = lo; q < hi && !live.load(std::memory_order_relaxed); q++) {
+                uint32_t bits;
+                memcpy(&bits, &b.nti[q], 4);
+                if (bits) live = true;
+            }
+        });
+        d.nt_in_live = live ? 1 : 0;
     }
     HIPCHK(hipStreamSynchronize(S.stream));
     if (to_device) return;
     const bool ran = S.ticks_run > 0;
-    for (uint32_t di = 0; di < n; di++) {
-        Segment& s = B.segments[S.dev2host[di]];
-        s.V_m = V[di];
-        for (int k = 0; k < 7; k++) { s.intra_ions[k] = in[(size_t)k * npad + di]; s.extra_ions[k] = ex[(size_t)k * npad + di]; }
-        s.ATP_level = atp[di]; s.temperature = temp[di]; s.transcription_signal = txn[di];
-        for (int k = 0; k < 15; k++) { s.neurotransmitter[k] = nt[(size_t)k * npad + di]; s.intra_nt[k] = nti[(size_t)k * npad + di]; }
-        if (ran) s.I_buffer = 0.0;
-        bool my = false;
-        for (uint32_t j = 0; j < s.modifier_count; j++) my |= s.modifiers[j].type == 4;
-        for (uint32_t j = 0; j < s.modifier_count; j++) {
-            Modifier& m = s.modifiers[j];
-            for (int q = 0; q < 7; q++) m.state[q] = ms[((size_t)j * 7 + q) * npad + di];
-            uint8_t t = m.type;
-            bool blocked = my && t != 4 && myelin_blocks(t);
-            if (ran && !blocked) m.state[7] = seg_area(s);
+    par_for(n, [&](size_t lo, size_t hi) {
+        for (size_t di = lo; di < hi; di++) {
+            Segment& s = B.segments[S.dev2host[di]];
+            s.V_m = b.V[di];
+            for (int k = 0; k < 7; k++) {
+                s.intra_ions[k] = b.in[(size_t)k * npad + di];
+                s.extra_ions[k] = b.ex[(size_t)k * npad + di];
+            }
+            s.ATP_level = b.atp[di]; s.temperature = b.temp[di]; s.transcription_signal = b.txn[di];
+            for (int k = 0; k < 15; k++) {
+                s.neurotransmitter[k] = b.nt[(size_t)k * npad + di];
+                s.intra_nt[k] = b.nti[(size_t)k * npad + di];
+            }
+            if (ran) s.I_buffer = 0.0;
+            bool my = false;
+            for (uint32_t j = 0; j < s.modifier_count; j++) my |= s.modifiers[j].type == 4;
+            for (uint32_t j = 0; j < s.modifier_count; j++) {
+                Modifier& m = s.modifiers[j];
+                for (int q = 0; q < 7; q++) m.state[q] = b.ms[((size_t)j * 7 + q) * npad + di];
+                uint8_t t = m.type;
+                bool blocked = my && t != 4 && myelin_blocks(t);
+                if (ran && !blocked) m.state[7] = seg_area(s);
+            }
         }
-    }
-    for (uint32_t k = 0; k < ns; k++) {
-        Synapse& y = B.synapses[S.slot2syn[k]];
-        if (ran && S.protein_map) protein_mapped(y, B, y.RRP_max, y.release_probability, y.RRP_recovery);
-        y.weight = yw[k]; y.facilitation = yf[k]; y.depression = yd[k]; y.RRP = yr[k]; y.bouton_Ca = yb[k];
-        y.spine_Ca = ys[k]; y.spine_camkii = yk[k]; y.spine_calcineurin = yc[k]; y.refractory = yrf[k];
-        for (int q = 0; q < 15; q++) y.cleft_neurotransmitter[q] = ycl[(size_t)q * ns + k];
-        const Segment& post = B.segments[y.post_segment];
-        bool my = false;
-        for (uint32_t j = 0; j < post.modifier_count; j++) my |= post.modifiers[j].type == 4;
-        for (uint32_t j = 0; j < y.post_receptor_count; j++) {
-            Modifier& m = y.post_receptors[j];
-            for (int q = 0; q < 7; q++) m.state[q] = yrs[((size_t)j * 7 + q) * ns + k];
-            uint8_t t = m.type;
-            bool blocked = my && ionotropic_type(t);
-            if (ran && !(y.type == 1 || y.type == 2) && !blocked) m.state[7] = y.post_area;
+    });
+    par_for(ns, [&](size_t lo, size_t hi) {
+        for (size_t k = lo; k < hi; k++) {
+            Synapse& y = B.synapses[S.slot2syn[k]];
+            if (ran && S.protein_map) protein_mapped(y, B, y.RRP_max, y.release_probability, y.RRP_recovery);
+            y.weight = b.yw[k]; y.facilitation = b.yf[k]; y.depression = b.yd[k]; y.RRP = b.yr[k];
+            y.bouton_Ca = b.yb[k]; y.spine_Ca = b.ys[k]; y.spine_camkii = b.yk[k];
+            y.spine_calcineurin = b.yc[k]; y.refractory = b.yrf[k];
+            for (int q = 0; q < 15; q++) y.cleft_neurotransmitter[q] = b.ycl[(size_t)q * ns + k];
+            const Segment& post = B.segments[y.post_segment];
+            bool my = false;
+            for (uint32_t j = 0; j < post.modifier_count; j++) my |= post.modifiers[j].type == 4;
+            for (uint32_t j = 0; j < y.post_receptor_count; j++) {
+                Modifier& m = y.post_receptors[j];
+                for (int q = 0; q < 7; q++) m.state[q] = b.yrs[((size_t)j * 7 + q) * ns + k];
+                uint8_t t = m.type;
+                bool blocked = my && ionotropic_type(t);
+                if (ran && !(y.type == 1 || y.type == 2) && !blocked) m.state[7] = y.post_area;
+            }
+            for (uint32_t j = 0, a = S.nar ? S.h_ar[k] : 0; j < (a & 15u); j++) {
+                y.pre_autoreceptors[j].state[0] = b.a0[(a >> 4) + j];
+                y.pre_autoreceptors[j].state[7] = b.a7[(a >> 4) + j];
+            }
         }
-        for (uint32_t j = 0, a = S.nar ? S.h_ar[k] : 0; j < (a & 15u); j++) {
-            y.pre_autoreceptors[j].state[0] = a0[(a >> 4) + j];
-            y.pre_autoreceptors[j].state[7] = a7[(a >> 4) + j];
-        }
-    }
+    });
 }
 
 static void sync_host(Sim& S) {
@@ -1379,6 +1472,7 @@ static int load_into(Sim& S, BionData* src_owned) {
     HIPCHK(hipStreamSynchronize(S.stream));
     S.drop_graph();
     S.free_device();
+    S.free_stage();
     free_bion(&S.host);
     S.host = *src_owned;
     memset(src_owned, 0, sizeof(BionData));

@@ -756,10 +756,10 @@ BC_DEV double rec_nmda(const ModSlot& m, SegRegs& s, float dt) {   // receptors_
 // ions 4-6 (untouched here) once after its walk, which floor7 per call amounts to. One body with the three parameter sets
 // selected per lane (the same operations as iono<true> with the constants of
 // dispatch(): AMPA and NMDA lanes of a wavefront run it together.
-// Receptor slot whose states 0-3 were loaded ahead (the post-side walk issues every load of a
-// chain step at once); a state beyond 3 (a gamma override index) is read from memory.
+// Receptor slot whose states 0-2 were loaded ahead (the post-side walk issues every load of a
+// chain step at once); a state beyond 2 (a gamma override index) is read from memory.
 struct RegSlot {
-    double st[4];
+    double st[3];        // states 0-2, loaded with the chain step; the rest are read on use
     const double* mem;   // state 0 of this receptor, stride `stride`
     size_t stride;
     double dens;
@@ -769,8 +769,7 @@ struct RegSlot {
         if (k == 0) return st[0];
         if (k == 1) return st[1];
         if (k == 2) return st[2];
-        if (k == 3) return st[3];
-        return mem[(size_t)k * stride];
+        return mem[(size_t)k * stride];   // an active_vars parameter state
     }
     __device__ void set(int k, double v) {
         if (k == 0) st[0] = v;

@@ -298,9 +298,34 @@ __device__ __attribute__((always_inline)) inline T sweep_final(int settled, T nx
     return (settled == 2 && ((50 - (it + 1)) & 1)) ? p1 : nx;
 }
 
+// Residual-terminated sweeps. An island is done when every lane's update is <= tol (NaN is
+// not), so each sweep needs per island only whether some lane exceeded: one ballot per
+// wavefront, masked with the lanes of this lane's island (islands are contiguous lane ranges of
+// the tile; il is the island's first lane), and one LDS store per island and wavefront, by the
+// island's first lane in it.
+// island_lanes: this lane's island's lanes within its wavefront, as a ballot mask.
+__device__ __attribute__((always_inline)) inline uint64_t island_lanes(uint32_t t, uint32_t il) {
+    const int lane = (int)(t & 63u);
+    const uint64_t starts = __ballot(t == il) | 1ull;   // lane 0 opens its wavefront's first island
+    const int s0 = max((int)il - (int)(t - (uint32_t)lane), 0);
+    const uint64_t above = starts & ~((2ull << s0) - 1);
+    const int e = above ? __ffsll((unsigned long long)above) - 1 : 64;
+    const uint64_t upto = e == 64 ? ~0ull : ((1ull << e) - 1);
+    return upto & ~((1ull << s0) - 1);
+}
+// Marks the island's slot when some lane of it (in this wavefront) exceeded tol. over: this
+// lane's update is not <= tol (0 for lanes that contribute nothing). Every lane of the
+// wavefront must call it.
+__device__ __attribute__((always_inline)) inline void island_over(unsigned long long* slots, uint32_t t, uint32_t il,
+                                                                  uint64_t imask, bool over) {
+    const uint64_t b = __ballot(over);
+    if ((b & imask) && (uint32_t)__ffsll((unsigned long long)imask) - 1 == (t & 63u)) slots[il] = 1;
+}
+
 // kRes: residual-terminated sweeps (an extension; the reference sweeps 50 times). Each island
 // (cell) stops after the first sweep whose largest update is <= d.jac_tol, or after d.jac_max
-// sweeps; the islands' residuals are max-reduced in LDS beyond the tile's DiffLds.
+// sweeps; per sweep, the islands with an update above tol are marked in LDS beyond the tile's
+// DiffLds (island_over).
 // kRec: cable mode for the adjoint, which records every Jacobi iterate in d.xrec.
 template <int TS, bool kRes, bool kRec = false>
 __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffusion(Net d, TileSet ts, float dt) {
@@ -421,12 +446,14 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
         const double ydiag = rcp_prep(diag);   // diag is fixed across the sweeps
         if constexpr (kRes) {
             const uint32_t il = act ? d.isl_lane0[g] : t;
+            const uint64_t imask = island_lanes(t, il);
             ires[t] = 0;
             ires[TS + t] = 0;
             tile_sync<TS>();
             bool done = !act;
             for (int it = 0; it < d.jac_max; it++) {
                 const int rb = it & 1;
+                bool over = false;
                 if (!done) {
                     double pn = 0.0;
 #pragma unroll
@@ -441,12 +468,13 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     if (ph > 10.0) ph = 10.0;
                     if (ph < -10.0) ph = -10.0;
                     L.u.p.phi[cur ^ 1][t] = ph;
-                    atomicMax(&ires[rb * TS + il], (unsigned long long)__double_as_longlong(fabs(ph - L.u.p.phi[cur][t])));
+                    over = !(fabs(ph - L.u.p.phi[cur][t]) <= d.jac_tol);
                 } else if (act) {
                     L.u.p.phi[cur ^ 1][t] = L.u.p.phi[cur][t];
                 }
+                island_over(ires + rb * TS, t, il, imask, over);
                 tile_sync<TS>();
-                if (!done) done = __longlong_as_double((long long)ires[rb * TS + il]) <= d.jac_tol;
+                if (!done) done = ires[rb * TS + il] == 0;
                 ires[(rb ^ 1) * TS + t] = 0;
                 cur ^= 1;
                 if (tile_all<TS>(done)) break;
@@ -480,7 +508,9 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                     if (ph > 10.0) ph = 10.0;
                     if (ph < -10.0) ph = -10.0;
                     const double nx = act ? ph : pi;
-                    const int pfx = sweep_period(nx, pi, pp, it);
+                    // checked after every second sweep: a fixed point or two-cycle, once
+                    // reached, persists, and sweep_final takes the sweep it is found at
+                    const int pfx = (it & 1) ? sweep_period(nx, pi, pp, it) : 0;
                     if (pfx) {
                         pi = sweep_final(pfx, nx, pi, it);
                         break;
@@ -636,6 +666,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 }
             if constexpr (kRes) {   // residual-terminated, as the phi sweeps
                 const uint32_t il = act ? d.isl_lane0[g] : t;
+                const uint64_t imask = island_lanes(t, il);
                 ires[t] = 0;
                 ires[TS + t] = 0;
                 tile_sync<TS>();
@@ -643,6 +674,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                 int vc = 0;
                 for (int it = 0; it < d.jac_max; it++) {
                     const int rb = it & 1;
+                    bool over = false;
                     if (!done) {
                         float off = 0.0f;
 #pragma unroll
@@ -655,13 +687,14 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                         }
                         const float v = div_wide(base + off, den, rden);
                         L.u.i.vf[vc ^ 1][t] = v;
-                        atomicMax(&ires[rb * TS + il], (unsigned long long)__double_as_longlong((double)fabsf(v - vnew)));
+                        over = !((double)fabsf(v - vnew) <= d.jac_tol);
                         vnew = v;
                     } else if (act) {
                         L.u.i.vf[vc ^ 1][t] = vnew;
                     }
+                    island_over(ires + rb * TS, t, il, imask, over);
                     tile_sync<TS>();
-                    if (!done) done = __longlong_as_double((long long)ires[rb * TS + il]) <= d.jac_tol;
+                    if (!done) done = ires[rb * TS + il] == 0;
                     ires[(rb ^ 1) * TS + t] = 0;
                     vc ^= 1;
                     if (tile_all<TS>(done)) break;
@@ -693,7 +726,7 @@ __global__ __launch_bounds__(TS == 64 ? 64 * kDiffMaxTiles64 : TS) void k_diffus
                         if (kRec) {   // the adjoint's recording keeps every sweep
                             if (act) d.xrec[((size_t)sub * 51 + it + 1) * P + g] = nv;
                         } else {
-                            const int settled = sweep_period(nx, vi, vp, it);
+                            const int settled = (it & 1) ? sweep_period(nx, vi, vp, it) : 0;
                             if (settled) {
                                 vi = sweep_final(settled, nx, vi, it);
                                 break;
@@ -1293,7 +1326,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
                         rs[j].av = d.y_rav[j * NS + y];
                         rs[j].dens = d.y_rdens[j * NS + y];
 #pragma unroll
-                        for (int k = 0; k < 4; k++) rs[j].st[k] = d.y_rstate[b + (size_t)k * NS];
+                        for (int k = 0; k < 3; k++) rs[j].st[k] = d.y_rstate[b + (size_t)k * NS];
                         rs[j].mem = d.y_rstate + b;
                         rs[j].stride = NS;
                         rs[j].a7 = parea;

@@ -134,3 +134,29 @@ def test_schedules_do_not_change_results():
             np.testing.assert_array_equal(ref[k][2], spk)
     for s in sims:
         s.close()
+
+
+def test_config4_batch_against_oracle():
+    """A batch of the batched sweep's networks (chain neurons, each network its own seed and
+    per-type g_max scaling, uneven sizes from 60 to 1400 segments) after autotuning and graph
+    replay, each network compared with the oracle stepping it alone."""
+    sizes = [9, 175, 48, 120, 30]
+    nets = [bion.config4_network(k, n) for k, n in enumerate(sizes)]
+    clamps = [bion.soma_current(n, 200.0) for n in nets]
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_batch([n.copy() for n in nets]) == 0
+    for k, c in enumerate(clamps):
+        sim.set_stimuli(k, current=c)
+    ran = sim.autotune(DT, 2)
+    sim.run_ticks(DT, 300 - ran)
+    for k, net in enumerate(nets):
+        o = H.OracleSim(net.copy())
+        for _ in range(300):
+            o.step(DT, H.Stimuli(current=clamps[k]))
+        seg, syn, spk = sim.batch_read(k)
+        print(f"network {k}: {len(net.segments)} segments, spikes {int(spk.sum())}", flush=True)
+        assert H.first_mismatch(o.net.segments, seg) is None, (k, H.first_mismatch(o.net.segments, seg))
+        assert H.first_mismatch(o.net.synapses, syn) is None, (k, H.first_mismatch(o.net.synapses, syn))
+        np.testing.assert_array_equal(o.spikes(), spk)
+        assert spk.sum() > 0
+    sim.close()
