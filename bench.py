@@ -131,14 +131,12 @@ def build_workload(neurons, ticks, seed=0xB10C0002):
     return net, stims
 
 
-def cpu_baseline(budget_s=12.0, neurons=250):
-    """Reference CPU tick on a bounded sample of the same workload (250 neurons = 8000 segments,
-    25k synapses), one thread, ticks until the time budget is used."""
-    net, stims = build_workload(neurons, 4000, seed=0xB10C0102)
+def _cpu_stepper(net, stim_fn, kind):
+    """step(t): one CPU tick of `net` in place with stim_fn(t) (harness Stimuli or None) -
+    the reference (oracle/_ref, compiled from the reference's sources) or the oracle."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import harness as H
     P = C.c_void_p
-    kind = "reference" if H.ref_available() else "port"
     if kind == "reference":
         lib = C.CDLL(H.REF_SO)
         lib.core_run_tick.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, C.c_uint8, P, C.c_float] + [P] * 12
@@ -147,15 +145,34 @@ def cpu_baseline(budget_s=12.0, neurons=250):
         mc = np.zeros(1, np.uint32)
 
         def step(t):
-            st = H.Stimuli(current=stims[t]["current_stim"][0][:stims[t]["current_stim_count"][0]])
+            st = stim_fn(t) if stim_fn else None
+            args = st.args() if st is not None else [None] * 10
             lib.core_run_tick(net.segments.ctypes.data, len(net.segments), net.connections.ctypes.data,
                               len(net.connections), net.synapses.ctypes.data, len(net.synapses), 0,
-                              tick.ctypes.data, 0.025, *st.args(), mech.ctypes.data, mc.ctypes.data)
-    else:
-        sim = H.OracleSim(net)
+                              tick.ctypes.data, 0.025, *args, mech.ctypes.data, mc.ctypes.data)
+        return step
+    sim = H.OracleSim(net)
+    return lambda t: sim.step(0.025, stim_fn(t) if stim_fn else None)
 
-        def step(t):
-            sim.step(0.025, H.Stimuli(current=stims[t]["current_stim"][0][:stims[t]["current_stim_count"][0]]))
+
+def _cpu_kind():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import harness as H
+    return "reference" if H.ref_available() else "port"
+
+
+def _gpu_stims_as_cpu(stims):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import harness as H
+    return lambda t: H.Stimuli(current=stims[t]["current_stim"][0][:stims[t]["current_stim_count"][0]])
+
+
+def cpu_baseline(budget_s=12.0, neurons=250):
+    """Reference CPU tick on a bounded sample of the same workload (250 neurons = 8000 segments,
+    25k synapses), one thread, ticks until the time budget is used."""
+    net, stims = build_workload(neurons, 4000, seed=0xB10C0102)
+    kind = _cpu_kind()
+    step = _cpu_stepper(net, _gpu_stims_as_cpu(stims), kind)
     step(0)
     t0 = time.perf_counter()
     k = 0
@@ -166,6 +183,56 @@ def cpu_baseline(budget_s=12.0, neurons=250):
     return {"value": len(net.segments) * k / el, "unit": "segment-ticks/s", "cores": 1, "kind": kind,
             "sample": f"{neurons} neurons x 32 segments ({len(net.segments)} seg, {len(net.synapses)} syn), "
                       f"{k} ticks in {el:.1f}s, same builder and stimuli as the GPU workload"}
+
+
+def cpu_full(neurons, ticks, warmup=2):
+    """The CPU tick on the full workload (default 100k neurons = 3.2M segments, 10M synapses),
+    one thread: `warmup` untimed ticks, then `ticks` timed ones, each printed as it ends."""
+    t0 = time.time()
+    net, stims = build_workload(neurons, warmup + ticks)
+    kind = _cpu_kind()
+    print(f"# cpu full size: built {len(net.segments)} segments in {time.time() - t0:.1f}s ({kind})",
+          file=sys.stderr, flush=True)
+    step = _cpu_stepper(net, _gpu_stims_as_cpu(stims), kind)
+    times = []
+    for t in range(warmup + ticks):
+        t1 = time.perf_counter()
+        step(t)
+        dt_ = time.perf_counter() - t1
+        if t >= warmup:
+            times.append(dt_)
+        print(f"# cpu tick {t}: {dt_:.2f}s", file=sys.stderr, flush=True)
+    return {"kind": kind, "cores": 1, "segments": len(net.segments), "synapses": len(net.synapses),
+            "warmup": warmup, "ticks": ticks, "s_per_tick": float(np.mean(times)),
+            "segment_ticks_per_s": len(net.segments) / float(np.mean(times))}
+
+
+def cpu_calibration():
+    """t(oracle) / t(reference) for the same ticks of the same networks, one thread each: the
+    64-neuron chain network (4000 ticks, 200 pA soma clamp) and the 12-channel tree network at
+    1/100 scale (1000 neurons, 32k segments, 40 ticks of its Poisson pulses)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import harness as H
+    if not H.ref_available():
+        return {"error": "oracle/_ref not built"}
+    out = {}
+    c1 = bion.config1()
+    clamp = H.soma_clamp(c1, 200.0)
+    n2, st2 = build_workload(1000, 40)
+    for name, net, fn, ticks in (("config1_4000_ticks", c1, lambda t: H.Stimuli(current=clamp), 4000),
+                                 ("config2_1000_neurons_40_ticks", n2, _gpu_stims_as_cpu(st2), 40)):
+        el = {}
+        for kind in ("reference", "port"):
+            x = net.copy()
+            step = _cpu_stepper(x, fn, kind)
+            t0 = time.perf_counter()
+            for t in range(ticks):
+                step(t)
+            el[kind] = time.perf_counter() - t0
+            print(f"# calibration {name} {kind}: {el[kind]:.2f}s", file=sys.stderr, flush=True)
+        out[name] = {"reference_s": el["reference"], "oracle_s": el["port"],
+                     "oracle_over_reference": el["port"] / el["reference"]}
+    return out
 
 
 def _peak_rss_gb():
@@ -327,6 +394,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--neurons", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compat-ticks", type=int, default=3,
+                    help="ticks timed in COMPAT coherence after the FAST ticks (0: skip)")
     ap.add_argument("--batch-networks", type=int, default=512,
                     help="networks in the batched-vs-plain comparison (512: the batched sweep's ~20M segments)")
     ap.add_argument("--batch-ticks", type=int, default=1000, help="ticks of the batched run")
@@ -343,6 +412,10 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="tick by tick instead of captured hipGraphs in the batched, stress and adjoint "
                          "legs (for kernel tracing)")
+    ap.add_argument("--cpu-full-ticks", type=int, default=0,
+                    help="only time the CPU tick on the full workload: 2 warm-up + this many ticks")
+    ap.add_argument("--cpu-calibration", action="store_true",
+                    help="only time the oracle against the compiled reference on the same ticks")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 1 GiB device-to-device copies (the reference kernel for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -355,6 +428,12 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return
+    if args.cpu_full_ticks > 0:
+        print(json.dumps(cpu_full(args.neurons, args.cpu_full_ticks)), flush=True)
+        return
+    if args.cpu_calibration:
+        print(json.dumps(cpu_calibration()), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; runs on more than one are not launched",
@@ -399,6 +478,19 @@ def main():
         phase += np.array(sim.stats().last_phase_ms[:])
     el = time.perf_counter() - t0
     sim.set_phase_timing(False)
+    compat = None
+    if args.compat_ticks > 0:
+        # COMPAT coherence, the drop-in default: the host records are uploaded before and
+        # refreshed after every tick (the PCIe-inclusive rate of a caller that hands over host
+        # buffers each tick)
+        sim.set_coherence(backend.COHERENCE_COMPAT)
+        sim.run_tick(0.025)
+        t0 = time.perf_counter()
+        for _ in range(args.compat_ticks):
+            sim.run_tick(0.025)
+        c_el = time.perf_counter() - t0
+        compat = {"ticks": args.compat_ticks, "ms_per_tick": c_el / args.compat_ticks * 1e3,
+                  "segment_ticks_per_s": n_seg * args.compat_ticks / c_el}
     sim.close()   # its host records and device state are not needed by the other legs
     del sim, stims
     ms_step = el / args.steps * 1e3
@@ -430,6 +522,8 @@ def main():
                           "unit": "GB/s", "frac": tick_gbs / HBM_PEAK_GBS},
         "phase_ms": {PHASES[k]: float(mean[k]) for k in range(len(PHASES))},
     }
+    if compat:
+        out["compat"] = compat
     if args.batch_networks > 0:
         out["batched"] = batched_vs_plain(args.batch_networks, args.batch_ticks, plain_ticks=args.plain_ticks)
     if args.stress_cables > 0:
