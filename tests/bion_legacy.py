@@ -7,6 +7,8 @@ but valid: finite floats, counts within their limits, indices within the network
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from bioneuroncore_amd import bion
@@ -129,3 +131,207 @@ def header(version: int, n_neurons, n_seg, n_conn, n_syn, n_nuc=0, sizes=None) -
         h["file_sz_" + k] = v
     raw = h.tobytes()
     return raw[:68] if version < 5 else raw[:84] if version < 10 else raw
+
+
+# ---------------------------------------------------------------- seeded files of every version
+def _synapses_v14(n, n_seg, rng):
+    y = fill(bion.SYNAPSE_DT, n, rng)
+    y["pre_segment"] = rng.integers(0, n_seg, n)
+    y["post_segment"] = rng.integers(0, n_seg, n)
+    y["type"] = rng.integers(0, 3, n)
+    y["pre_autoreceptor_count"] = rng.integers(0, 9, n)
+    y["post_receptor_count"] = rng.integers(0, 9, n)
+    return y
+
+
+def _legacy_syn(dt, n, n_seg, rng, counts=8):
+    y = fill(dt, n, rng)
+    y["pre_segment"] = rng.integers(0, n_seg, n)
+    y["post_segment"] = rng.integers(0, n_seg, n)
+    y["type"] = rng.integers(0, 3, n)
+    y["pre_autoreceptor_count"] = rng.integers(0, counts + 1, n)
+    y["post_receptor_count"] = rng.integers(0, counts + 1, n)
+    return y
+
+
+def build(tmp, name, version, seg="auto", syn="auto", nuc="auto", conn="auto", n_neurons=3, seg_per=7,
+          n_syn=40, seed=0, mutate=None):
+    """Writes one file; returns its path. seg/syn/nuc: record size on disk (v10+) or 'auto'."""
+    rng = np.random.default_rng(seed)
+    n_seg = max(n_neurons, 1) * seg_per
+    neu, pairs = network_parts(max(n_neurons, 1), seg_per, rng)
+    neu = neu[:n_neurons]
+    n_conn = len(pairs)
+    sizes = {}
+    # segments
+    if version == 3:
+        s = fill(SEG_V3, n_seg, rng)
+        valid_modifier_counts(s, 16, rng)
+    elif version <= 6:
+        s = fill(SEG_V4, n_seg, rng)
+        valid_modifier_counts(s, 16, rng)
+    elif version <= 9:
+        s = fill(SEG_V7, n_seg, rng)
+        valid_modifier_counts(s, 32, rng)
+    else:
+        dts = {"auto": bion.SEGMENT_DT, 0: bion.SEGMENT_DT, 3216: bion.SEGMENT_DT, 1692: SEG_V7,
+               1696: SEG_V10ER, 1704: SEG_V10, 3208: SEG_V13}
+        if seg in dts:
+            s = fill(dts[seg], n_seg, rng)
+            valid_modifier_counts(s, 32, rng)
+            sizes["segment"] = 3216 if seg == "auto" else seg
+        else:   # an unknown size: raw v14 prefix or v14 record plus a tail
+            full = fill(bion.SEGMENT_DT, n_seg, rng)
+            valid_modifier_counts(full, 32, rng)
+            raw = full.view(np.uint8).reshape(n_seg, -1)
+            if seg < 3216:
+                s = raw[:, :seg].copy()
+            else:
+                s = np.concatenate([raw, rng.integers(0, 256, (n_seg, seg - 3216), dtype=np.uint8)], axis=1)
+            sizes["segment"] = seg
+    # connections
+    if version <= 8:
+        c = fill(CONN_V8, n_conn, rng)
+        c["coupling_resistance"][::3] = 0.0
+    else:
+        c = fill(bion.CONN_DT, n_conn, rng)
+    c["segment_a"], c["segment_b"] = pairs[:, 0], pairs[:, 1]
+    # synapses
+    if version <= 6:
+        y = _synapses_v14(n_syn, n_seg, rng).view(np.uint8).reshape(n_syn, 1568)[:, :824].copy()
+    elif version <= 9:
+        y = _legacy_syn(SYN_V7, n_syn, n_seg, rng)
+    else:
+        if syn in ("auto", 0, 1568):
+            y = _synapses_v14(n_syn, n_seg, rng)
+        elif syn == 836:
+            y = _legacy_syn(SYN_V7, n_syn, n_seg, rng)
+        elif syn == 1560:
+            y = _legacy_syn(SYN_V13, n_syn, n_seg, rng)
+        else:
+            y = _synapses_v14(n_syn, n_seg, rng).view(np.uint8).reshape(n_syn, 1568)[:, :syn].copy()
+        sizes["synapse"] = 1568 if syn == "auto" else syn
+    # nuclei
+    n_nuc = 0
+    nucs = b""
+    if version == 5:
+        n_nuc = n_neurons
+        nucs = fill(NUC_V5, n_nuc, rng).tobytes()
+    elif version == 6:
+        n_nuc = n_neurons
+        nucs = rng.integers(0, 255, (n_nuc, 332), dtype=np.uint8).tobytes()
+    elif version >= 7 and nuc != "none":
+        n_nuc = n_neurons
+        if nuc == 412:
+            nucs = fill(NUC_V46, n_nuc, rng).tobytes()
+            sizes["nucleus"] = 412
+        else:
+            nucs = fill(NUC, n_nuc, rng).tobytes()
+            if version >= 10:
+                sizes["nucleus"] = 500
+    if version >= 10:
+        sizes.update(intraconnection=16, neuron=24, modifier=88)
+    parts = [header(version, n_neurons, n_seg, n_conn, n_syn, n_nuc, sizes), neu.tobytes(), s.tobytes(),
+             c.tobytes(), y.tobytes(), nucs]
+    if mutate:
+        parts = mutate(parts)
+    p = os.path.join(tmp, name + ".bion")
+    with open(p, "wb") as f:
+        f.write(b"".join(parts))
+    return p
+
+
+def cases(tmp):
+    c = {}
+    for v in (3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 14):
+        c[f"v{v}"] = (build(tmp, f"v{v}", v, seed=v), v)
+    c["v4_nosyn"] = (build(tmp, "v4_nosyn", 4, n_syn=0, seed=40), 4)
+    for sz in (0, 1692, 1696, 1704, 3208, 3000, 3400):
+        c[f"v10_seg{sz}"] = (build(tmp, f"v10_seg{sz}", 10, seg=sz, seed=100 + sz), 10)
+    for sz in (0, 836, 1560, 840):
+        c[f"v10_syn{sz}"] = (build(tmp, f"v10_syn{sz}", 10, syn=sz, seed=200 + sz), 10)
+    c["v10_nuc412"] = (build(tmp, "v10_nuc412", 10, nuc=412, seed=300), 10)
+    c["v14_nonuc"] = (build(tmp, "v14_nonuc", 14, nuc="none", seed=301), 14)
+    c["v14_noneuron"] = (build(tmp, "v14_noneuron", 14, n_neurons=0, seg_per=5, n_syn=0, seed=302), 14)
+    return c
+
+
+def defined(version, arrays):
+    """Byte masks of the fields the reference loader writes (neurons, segments, connections,
+    synapses, nuclei as uint8 arrays): reserved bytes of v3-v8 connections, v3-v6 synapses past
+    824 bytes (except the vesicle pool) and v6 nuclei past 332 bytes stay unwritten."""
+    neu, seg, con, syn, nuc = arrays
+    mc = np.ones_like(con, bool)
+    if version <= 8 and len(con):
+        m = mc.reshape(-1, 16)
+        m[:, 9:12] = False   # reserved[3]
+    my = np.ones_like(syn, bool)
+    if version <= 6 and len(syn):
+        m = my.reshape(-1, 1568)
+        m[:, 824:] = False
+        for f in ("RRP", "RRP_max", "RRP_recovery"):
+            o = bion.SYNAPSE_DT.fields[f][1]
+            m[:, o:o + 4] = True
+    mn = np.ones_like(nuc, bool)
+    if version == 6 and len(nuc):
+        m = mn.reshape(-1, bion.NUCLEUS_SIZE)
+        m[:, 332:] = False
+        o = 16 + 57 * 4 + 36 * 4   # protein_level[36..46), zeroed by the loader
+        m[:, o:o + 40] = True
+    return np.ones_like(neu, bool), np.ones_like(seg, bool), mc, my, mn
+
+
+def error_cases(tmp):
+    """Files each of which the loader must refuse, by the case's name."""
+    def bad_magic(p):
+        p[0] = b"XXXX" + p[0][4:]
+        return p
+
+    def version(v):
+        def f(p):
+            h = bytearray(p[0])
+            h[4:8] = int(v).to_bytes(4, "little")
+            p[0] = bytes(h)
+            return p
+        return f
+
+    def zero_segs(p):
+        h = bytearray(p[0])
+        h[16:20] = (0).to_bytes(4, "little")
+        p[0] = bytes(h)
+        return p
+
+    def truncate(k):
+        def f(p):
+            p[k] = p[k][:-5]
+            return p[:k + 1]
+        return f
+
+    def poke(k, off, val):
+        def f(p):
+            b = bytearray(p[k])
+            b[off:off + len(val)] = val
+            p[k] = bytes(b)
+            return p
+        return f
+
+    seg_sz, syn_sz = 3216, 1568
+    return {
+        "bad_magic": build(tmp, "e_magic", 14, mutate=bad_magic),
+        "version2": build(tmp, "e_v2", 14, mutate=version(2)),
+        "version15": build(tmp, "e_v15", 14, mutate=version(15)),
+        "zero_segments": build(tmp, "e_zero", 14, mutate=zero_segs),
+        "short_header": build(tmp, "e_hdr", 14, mutate=lambda p: [p[0][:50]]),
+        "trunc_neurons": build(tmp, "e_tn", 14, mutate=truncate(1)),
+        "trunc_segments": build(tmp, "e_ts", 14, mutate=truncate(2)),
+        "trunc_segments_v7": build(tmp, "e_ts7", 7, mutate=truncate(2)),
+        "trunc_connections": build(tmp, "e_tc", 14, mutate=truncate(3)),
+        "trunc_synapses": build(tmp, "e_ty", 14, mutate=truncate(4)),
+        "trunc_nuclei_v5": build(tmp, "e_tu", 5, mutate=truncate(5)),
+        "conn_range": build(tmp, "e_cr", 14, mutate=poke(3, 0, (99999).to_bytes(4, "little"))),
+        "syn_range": build(tmp, "e_yr", 14, mutate=poke(4, 8, (99999).to_bytes(4, "little"))),
+        "syn_type": build(tmp, "e_yt", 14, mutate=poke(4, 12, bytes([3]))),
+        "syn_receptors": build(tmp, "e_yc", 14, mutate=poke(4, bion.SYNAPSE_DT.fields["post_receptor_count"][1], bytes([9]))),
+        "neuron_range": build(tmp, "e_nr", 14, mutate=poke(1, 8, (999).to_bytes(4, "little"))),
+        "modcount": build(tmp, "e_mc", 14, mutate=poke(2, bion.SEGMENT_DT.fields["modifier_count"][1], bytes([33]))),
+    }
