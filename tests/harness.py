@@ -273,6 +273,8 @@ def run_reference_inproc(net: bion.Network, ticks: int, dt: float, stim_fn=None)
         lib.core_run_tick(_ptr(n.segments), len(n.segments), _ptr(n.connections), len(n.connections),
                           _ptr(n.synapses), len(n.synapses), n.mode, tick.ctypes.data_as(P), dt, *st,
                           mech.ctypes.data_as(P), mcount.ctypes.data_as(P))
+        if len(n.segments) >= 1_000_000:   # a minute per tick at this size: say when each ends
+            print(f"reference tick {t} done", flush=True)
     lib.core_enable_spike_count(None, 0)
     return spikes
 
