@@ -409,6 +409,8 @@ def main():
     ap.add_argument("--adjoint-ticks", type=int, default=20000)
     ap.add_argument("--adjoint-every", type=int, default=500)
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward cost")
+    ap.add_argument("--step1-chunks", type=int, default=0,
+                    help="override the schedule's step1_chunks (1: steps 1 and 2.5 on one stream)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="tick by tick instead of captured hipGraphs in the batched, stress and adjoint "
                          "legs (for kernel tracing)")
@@ -460,6 +462,8 @@ def main():
     rc = sim.load_network(net)
     if rc != 0:
         raise SystemExit(f"load failed: {rc}")
+    if args.step1_chunks > 0:
+        sim.set_schedule(step1_chunks=args.step1_chunks)
     print(f"# loaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     nt_in_live = bool(np.any(net.segments["intra_nt"].view(np.uint32)))
     algo = algorithmic_bytes(net, len(net.synapses) == 0, nt_in_live)

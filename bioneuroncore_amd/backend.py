@@ -69,7 +69,7 @@ class BchipStats(C.Structure):
 
 class BchipSchedule(C.Structure):
     _fields_ = [("modifiers_block", C.c_uint32), ("diffusion_tiles", C.c_uint32), ("pre_grid", C.c_uint32),
-                ("plast_block", C.c_uint32), ("rrp_block", C.c_uint32)]
+                ("plast_block", C.c_uint32), ("rrp_block", C.c_uint32), ("step1_chunks", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -18,8 +18,8 @@
 #include <vector>
 #include "../../include/biocore_abi.h"
 #include "../../include/biocore_hip_backend.h"
-#include "bc_kernels.hip"
-#include "bc_bigisland.hip"
+#include "bc_tick.h"
+#include "bc_bigisland.h"
 #include "bc_adjoint.hip"
 
 namespace bc {
@@ -82,6 +82,12 @@ struct Sim {
     std::vector<uint32_t> dev2host, host2dev, slot2syn, conn2dev;
     uint32_t ntiles = 0;
     uint32_t class_tile0[kTileClasses] = {}, class_ntiles[kTileClasses] = {};
+    std::vector<uint32_t> h_tile64_seg0;   // first device segment of each 64-segment tile, then the end
+    // Second stream of the tick: step 2.5 of the 64-segment tiles, chunk by chunk, while step 1
+    // of the next chunk runs on `stream` (sched.step1_chunks > 1). Default priority, as `stream`.
+    hipStream_t stream2 = nullptr;
+    static constexpr uint32_t kMaxChunks = 16;
+    hipEvent_t fork_ev[kMaxChunks + 1] = {};
     BigSet big{};   // islands larger than a tile (k_diffusion_big); big.count = 0 without
     TileSet tiles[kTileClasses] = {};
     size_t lds_bytes = 0;
@@ -119,7 +125,7 @@ struct Sim {
     std::vector<NetStim> net_stim{NetStim{}};
     bool inj_committed = false;   // injections are in the device lists (cleared after their tick)
     // Launch schedule (bchip_autotune picks it; results do not depend on it).
-    BchipSchedule sched{256, 1, 8192, 256, 256};
+    BchipSchedule sched{256, 1, 8192, 256, 256, 1};
     // Captured tick sequence (bchip_run_ticks): replayed while the launch inputs are unchanged.
     hipGraphExec_t graph = nullptr;
     uint32_t graph_ticks = 0;
@@ -230,9 +236,13 @@ struct Sim {
         free_stage();
         free_stim_buffers();
         delete stim_host;
+        if (stream2) (void)hipStreamSynchronize(stream2);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : fork_ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+        if (stream2) (void)hipStreamDestroy(stream2);
         free_bion(&host);
     }
 };
@@ -356,6 +366,9 @@ static int build_device(Sim& S) {
         flush();
     }
     S.ntiles = (uint32_t)tile_seg0.size();
+    // islands are placed by size class, the 64-segment class first, from device segment 0
+    S.h_tile64_seg0.assign(tile_seg0.begin(), tile_seg0.begin() + S.class_ntiles[0]);
+    S.h_tile64_seg0.push_back(S.class_ntiles[0] ? tile_seg0[S.class_ntiles[0] - 1] + tile_nseg[S.class_ntiles[0] - 1] : 0);
     // each big island is a pseudo-tile past the real ones (placement and incidence lists below
     // treat it as a tile; only k_diffusion_big runs on it)
     uint32_t nbig = 0;
@@ -1331,45 +1344,92 @@ static void stim_after_tick(Sim& S) {
 }
 
 // ------------------------------------------------------------------ tick
-template <bool kRes, bool kRec = false>
-static void launch_diffusion(Sim& S, float dt) {
+static void launch_modifiers(Sim& S, float dt, uint32_t first, uint32_t last) {
+    if (first >= last) return;
     const Net& d = S.d;
-    auto lds = [](size_t tile, uint32_t ts, uint32_t w) { return w * (tile + (kRes ? 2 * ts * 8 : 0)); };
-    if (S.class_ntiles[0]) {
-        const uint32_t W = S.sched.diffusion_tiles;
-        k_diffusion<64, kRes, kRec><<<(S.class_ntiles[0] + W - 1) / W, 64 * W, lds(sizeof(DiffLds<64>), 64, W),
-                                      S.stream>>>(d, S.tiles[0], dt);
+    const uint32_t B = S.sched.modifiers_block;
+    const uint32_t gm = (last - first + B - 1) / B;
+    if (S.ext_mods) {
+        if (S.nt_read) k_modifiers<true, true><<<gm, B, 0, S.stream>>>(d, dt, first, last);
+        else k_modifiers<false, true><<<gm, B, 0, S.stream>>>(d, dt, first, last);
+    } else {
+        if (S.nt_read) k_modifiers<true, false><<<gm, B, 0, S.stream>>>(d, dt, first, last);
+        else k_modifiers<false, false><<<gm, B, 0, S.stream>>>(d, dt, first, last);
     }
+}
+
+// Step 2.5 of the 64-segment tiles [t0, t1) on stream st.
+template <bool kRes, bool kRec>
+static void launch_diffusion64(Sim& S, float dt, uint32_t t0, uint32_t t1, hipStream_t st) {
+    if (t0 >= t1) return;
+    TileSet ts = S.tiles[0];
+    ts.seg0 += t0; ts.nseg += t0; ts.conn0 += t0; ts.nconn += t0; ts.maxinc += t0;
+    ts.count = t1 - t0;
+    const uint32_t W = S.sched.diffusion_tiles;
+    const size_t lds = W * (sizeof(DiffLds<64>) + (kRes ? 2 * 64 * 8 : 0));
+    k_diffusion<64, kRes, kRec><<<(ts.count + W - 1) / W, 64 * W, lds, st>>>(S.d, ts, dt);
+}
+
+// Step 2.5 of the other size classes and the big islands.
+template <bool kRes, bool kRec>
+static void launch_diffusion_rest(Sim& S, float dt) {
+    const Net& d = S.d;
+    auto lds = [](size_t tile, uint32_t ts) { return tile + (kRes ? 2 * ts * 8 : 0); };
     if (S.class_ntiles[1])
-        k_diffusion<128, kRes, kRec><<<S.class_ntiles[1], 128, lds(sizeof(DiffLds<128>), 128, 1), S.stream>>>(
+        k_diffusion<128, kRes, kRec><<<S.class_ntiles[1], 128, lds(sizeof(DiffLds<128>), 128), S.stream>>>(
             d, S.tiles[1], dt);
     if (S.class_ntiles[2])
-        k_diffusion<256, kRes, kRec><<<S.class_ntiles[2], 256, lds(sizeof(DiffLds<256>), 256, 1), S.stream>>>(
+        k_diffusion<256, kRes, kRec><<<S.class_ntiles[2], 256, lds(sizeof(DiffLds<256>), 256), S.stream>>>(
             d, S.tiles[2], dt);
     if (!kRec && S.big.count) k_diffusion_big<kRes><<<S.big.count, kBigThreads, 0, S.stream>>>(d, S.big, dt);
 }
 
-// The kernel sequence of one tick on S.stream (no host synchronisation or copies, so it can be
-// captured). With `tm`, events bracket the phases.
+// Steps 1 and 2.5. With sched.step1_chunks = C > 1 the 64-segment tiles are cut into C chunks of
+// consecutive tiles: step 1 of chunk c runs on `stream`, and step 2.5 of chunk c on `stream2`
+// once step 1 of chunk c is done, so the VALU-bound sweeps of chunk c overlap the memory-bound
+// modifier streams of chunk c + 1. Step 1 of a segment reads and writes only that segment, and
+// step 2.5 of a tile only the tile's segments, so results do not depend on C. `stream2` joins
+// `stream` again before step 3.
+template <bool kRes, bool kRec>
+static void launch_step12(Sim& S, float dt, bool tm) {
+    const Net& d = S.d;
+    const uint32_t nt0 = S.class_ntiles[0];
+    const uint32_t C = d.has_conn ? std::min(std::min(S.sched.step1_chunks, Sim::kMaxChunks), nt0) : 0;
+    if (C <= 1) {
+        launch_modifiers(S, dt, 0, d.n);
+        if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
+        if (d.has_conn) {
+            launch_diffusion64<kRes, kRec>(S, dt, 0, nt0, S.stream);
+            launch_diffusion_rest<kRes, kRec>(S, dt);
+        }
+        return;
+    }
+    for (uint32_t c = 0; c < C; c++) {
+        const uint32_t t0 = (uint32_t)((uint64_t)nt0 * c / C), t1 = (uint32_t)((uint64_t)nt0 * (c + 1) / C);
+        launch_modifiers(S, dt, S.h_tile64_seg0[t0], S.h_tile64_seg0[t1]);
+        HIPCHK(hipEventRecord(S.fork_ev[c], S.stream));
+        HIPCHK(hipStreamWaitEvent(S.stream2, S.fork_ev[c], 0));
+        launch_diffusion64<kRes, kRec>(S, dt, t0, t1, S.stream2);
+    }
+    launch_modifiers(S, dt, S.h_tile64_seg0[nt0], d.n);
+    if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
+    launch_diffusion_rest<kRes, kRec>(S, dt);
+    HIPCHK(hipEventRecord(S.fork_ev[C], S.stream2));
+    HIPCHK(hipStreamWaitEvent(S.stream, S.fork_ev[C], 0));
+}
+
+// The kernel sequence of one tick on S.stream (and S.stream2, joined back; no host
+// synchronisation or copies, so it can be captured). With `tm`, events on S.stream bracket the
+// phases (with step 1 chunked, the first phase ends when step 1 is done, the overlapped step 2.5
+// of its chunks included in either phase).
 static void launch_tick(Sim& S, float dt, bool tm) {
     const Net& d = S.d;
     const BchipSchedule& sc = S.sched;
     const uint32_t gy = (d.ns + 255) / 256;
     if (tm) HIPCHK(hipEventRecord(S.ev[0], S.stream));
-    const uint32_t gm = (d.n + sc.modifiers_block - 1) / sc.modifiers_block;
-    if (S.ext_mods) {
-        if (S.nt_read) k_modifiers<true, true><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
-        else k_modifiers<false, true><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
-    } else {
-        if (S.nt_read) k_modifiers<true, false><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
-        else k_modifiers<false, false><<<gm, sc.modifiers_block, 0, S.stream>>>(d, dt);
-    }
-    if (tm) HIPCHK(hipEventRecord(S.ev[1], S.stream));
-    if (d.has_conn) {
-        if (S.jac_res) launch_diffusion<true>(S, dt);
-        else if (d.xrec) launch_diffusion<false, true>(S, dt);
-        else launch_diffusion<false>(S, dt);
-    }
+    if (S.jac_res) launch_step12<true, false>(S, dt, tm);
+    else if (d.xrec) launch_step12<false, true>(S, dt, tm);
+    else launch_step12<false, false>(S, dt, tm);
     if (tm) HIPCHK(hipEventRecord(S.ev[2], S.stream));
     if (d.ns) {
         k_pre_summary<<<(d.nwatched + 255) / 256, 256, 0, S.stream>>>(d);
@@ -1793,7 +1853,9 @@ void* gpu_backend_create_on_device(int device_index) {
         HIPCHK(hipSetDevice(device_index));
         (void)bc::scratch_setup(device_index);   // bchip_adjoint_noalloc refuses (-1) if this failed
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        for (auto& e : s->fork_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         s->stim_host = new GpuStimulusData();
         memset(s->stim_host, 0, sizeof(GpuStimulusData));
         return s;
@@ -2141,6 +2203,7 @@ int bchip_set_schedule(void* sim, const BchipSchedule* in) {
     if (!blk(in->modifiers_block) || !blk(in->plast_block) || !blk(in->rrp_block)) return -1;
     if (in->diffusion_tiles != 1 && in->diffusion_tiles != 2 && in->diffusion_tiles != 4) return -1;
     if (in->pre_grid < 1 || in->pre_grid > (1u << 20)) return -1;
+    if (in->step1_chunks < 1 || in->step1_chunks > 16) return -1;
     static_cast<Sim*>(sim)->sched = *in;
     return 0;
 }
@@ -2157,6 +2220,7 @@ int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate) {
         {&BchipSchedule::pre_grid, {1024, 4096, 16384, 65536}, 2},
         {&BchipSchedule::plast_block, {64, 128, 256}, 4},
         {&BchipSchedule::rrp_block, {64, 128, 256}, 5},
+        {&BchipSchedule::step1_chunks, {1, 2, 4, 8}, -1},   // timed as steps 1 and 2.5 together
     };
     int ran = 0;
     const int timing = S.timing;
@@ -2164,7 +2228,7 @@ int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate) {
         (void)hipSetDevice(S.device);
         S.timing = 1;
         for (const Field& fd : fields) {
-            if (fd.phase == 1 && (!S.d.has_conn || !S.class_ntiles[0])) continue;
+            if ((fd.phase == 1 || fd.phase == -1) && (!S.d.has_conn || !S.class_ntiles[0])) continue;
             if (fd.phase >= 2 && !S.d.ns) continue;
             uint32_t best = S.sched.*(fd.f);
             float best_ms = 1e30f;
@@ -2173,7 +2237,7 @@ int bchip_autotune(void* sim, float dt, uint32_t ticks_per_candidate) {
                 float acc = 0.0f;
                 for (uint32_t t = 0; t < ticks_per_candidate; t++) {
                     bc::run_tick(S, dt);
-                    acc += S.last_ms[fd.phase];
+                    acc += fd.phase >= 0 ? S.last_ms[fd.phase] : S.last_ms[0] + S.last_ms[1];
                     ran++;
                 }
                 if (acc < best_ms) { best_ms = acc; best = c; }

@@ -1,0 +1,147 @@
+// bc_step1.hip — step 1: k_modifiers, one lane per segment over a range of the device order
+// (modifier slots in order, then the segment's stimuli).
+#include "bc_tick.h"
+
+namespace bc {
+
+// ------------------------------------------------------------------ step 1
+// kNt: some segment carries a modifier that reads or takes up transmitters (SF_NTREAD); kExt:
+// some segment carries a modifier outside the core set. The variants without keep those
+// registers free.
+template <bool kNt, bool kExt>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_modifiers(Net d, float dt,
+                                                                                          uint32_t first, uint32_t last) {
+    exp_table_stage();
+    const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) return;
+    const uint64_t tick = *d.tick;
+    const size_t P = d.npad;
+    SegRegs s;
+    s.V = d.V[i];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        s.in[k] = d.in[k * P + i];
+        s.ex[k] = d.ex[k * P + i];
+    }
+    s.atp = d.atp[i];
+    s.area = d.area[i];
+    s.vol = d.vol[i];
+    s.pkc = d.pkc[i];
+    s.pka = d.pka[i];
+    s.camp = d.camp[i];
+    s.camkii = d.camkii[i];
+    s.camkii01 = d.camkii01[i];
+    seg_consts_init(s);
+    ghk_cache_init(s);
+    const uint8_t sfl = d.sflags[i];
+    if (kNt && (sfl & SF_NTREAD)) {   // receptors read transmitters 0-11, transporters 0, 1 and 7
+#pragma unroll
+        for (int k = 0; k < 12; k++) s.nt[k] = d.nt[k * P + i];
+    }
+    const bool my = sfl & SF_MYELIN;
+    const uint32_t cnt = d.mcount[i];
+    double itot = 0.0;
+    for (uint32_t j = 0; j < cnt; j++) {
+        const uint8_t t = d.mtype[j * P + i];
+        if (my && t != 4 && myelin_blocked(t)) continue;
+        ModSlot m;
+        m.st = d.mstate + (size_t)j * 7 * P + i;
+        m.stride = P;
+        m.dens = d.mdens[j * P + i];
+        m.a7 = s.area;
+        m.av = d.mav[j * P + i];
+        if (kExt && t == 144) {   // Cav1.4 reads the first CaBP4's state[0] as it stands now
+            s.cabp4 = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 145) {
+                    s.cabp4 = d.mstate[(size_t)q * 7 * P + i];
+                    break;
+                }
+        }
+        if (kExt && t == 86) {   // eCB synthesis reads the first PLC/IP3/DAG modifier's state[1]
+            s.dag = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 136) {
+                    s.dag = d.mstate[((size_t)q * 7 + 1) * P + i];
+                    break;
+                }
+        }
+        itot += dispatch<kNt, kExt>(t, m, s, dt);
+        if (d.cable) {   // concentrations held: every channel sees the stored ones
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                s.in[k] = d.in[k * P + i];
+                s.ex[k] = d.ex[k * P + i];
+            }
+            s.atp = d.atp[i];
+            if (kNt && (sfl & SF_NTREAD)) {
+                s.nt[0] = d.nt[0 * P + i];
+                s.nt[1] = d.nt[1 * P + i];
+                s.nt[7] = d.nt[7 * P + i];
+            }
+        }
+    }
+    if (kNt && (sfl & SF_NTREAD) && !d.cable) {   // the transporters' uptake (before injections land)
+        d.nt[0 * P + i] = s.nt[0];
+        d.nt[1 * P + i] = s.nt[1];
+        d.nt[7 * P + i] = s.nt[7];
+    }
+    // stimuli (simulation_host.cpp:247-300): this segment's list, in the reference's order
+    const uint32_t sh = d.stim.any ? d.stim.head[i] : ~0u;
+    const uint64_t wtick = tick + d.stim.tick_shift;   // time windows; injections use the tick
+    if (sh != ~0u) {
+        const uint32_t cnt = d.stim.ent[sh];
+        for (uint32_t e = 1; e <= cnt; e++) {
+            const uint32_t en = d.stim.ent[sh + e];
+            const uint32_t kind = en >> 28, x = en & 0x0FFFFFFFu;
+            if (kind == SK_CUR) {
+                const CurrentStimulus q = d.stim.cur[x];
+                uint64_t dur = (uint64_t)(q.duration_ms / dt);
+                if (wtick >= q.start_tick && wtick < q.start_tick + dur) itot += q.amplitude_pA;
+            } else if (kind == SK_RAMP) {
+                const RampStimulus q = d.stim.ramp[x];
+                if (wtick >= q.start_tick && wtick < q.end_tick) {
+                    float frac = (q.end_tick > q.start_tick)
+                                     ? (float)(wtick - q.start_tick) / (float)(q.end_tick - q.start_tick)
+                                     : 1.0f;
+                    float amp = q.start_pA + (q.end_pA - q.start_pA) * frac;
+                    itot += amp;
+                }
+            } else if (kind == SK_ION) {
+                const IonInjection q = d.stim.ion[x];
+                if (tick == d.stim.inj_tick && q.ion_idx >= 0 && q.ion_idx < 7) {
+                    // register arrays: index with compile-time constants only
+#pragma unroll
+                    for (int k = 0; k < 7; k++)
+                        if (k == q.ion_idx) {
+                            if (q.extracellular) s.ex[k] += q.amount_mM;
+                            else s.in[k] += q.amount_mM;
+                        }
+                }
+            } else if (kind == SK_LIG) {
+                const LigandInjection q = d.stim.lig[x];
+                if (tick == d.stim.inj_tick && q.nt_idx >= 0 && q.nt_idx < 15) {
+                    if (q.intracellular) d.nt_in[q.nt_idx * P + i] += q.amount;
+                    else d.nt[q.nt_idx * P + i] += q.amount;
+                }
+            }
+        }
+    }
+    d.V[i] = s.V;
+    if (!d.cable) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            d.in[k * P + i] = s.in[k];
+            d.ex[k * P + i] = s.ex[k];
+        }
+        d.atp[i] = s.atp;
+    }
+    d.ibuf[i] = itot;
+}
+
+template __global__ void k_modifiers<false, false>(Net, float, uint32_t, uint32_t);
+template __global__ void k_modifiers<true, false>(Net, float, uint32_t, uint32_t);
+template __global__ void k_modifiers<false, true>(Net, float, uint32_t, uint32_t);
+template __global__ void k_modifiers<true, true>(Net, float, uint32_t, uint32_t);
+
+}  // namespace bc

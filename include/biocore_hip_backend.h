@@ -135,6 +135,8 @@ typedef struct BchipSchedule {
     uint32_t pre_grid;          // workgroups of the grid-stride pre-side pass
     uint32_t plast_block;       // plasticity-pass workgroup size: 64, 128 or 256
     uint32_t rrp_block;         // vesicle-pool-pass workgroup size: 64, 128 or 256
+    uint32_t step1_chunks;      // 1-16: the 64-segment tiles in this many chunks, step 2.5 of a
+                                // chunk on a second stream overlapping step 1 of the next (1: none)
 } BchipSchedule;
 int bchip_get_schedule(void* sim, BchipSchedule* out);
 int bchip_set_schedule(void* sim, const BchipSchedule* in);   // -1 if a field is out of range

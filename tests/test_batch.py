@@ -120,7 +120,8 @@ def test_schedules_do_not_change_results():
             s.set_stimuli(k, current=H.soma_clamp(n, 160.0))
     ran = sims[1].autotune(DT, 2)
     assert ran > 0
-    sims[2].set_schedule(modifiers_block=64, diffusion_tiles=4, pre_grid=1024, plast_block=64, rrp_block=128)
+    sims[2].set_schedule(modifiers_block=64, diffusion_tiles=4, pre_grid=1024, plast_block=64, rrp_block=128,
+                         step1_chunks=4)
     sims[0].run_ticks(DT, ran + 20)
     sims[1].run_ticks(DT, 20)
     sims[2].run_ticks(DT, ran + 20)
