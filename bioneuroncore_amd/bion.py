@@ -108,8 +108,10 @@ MISC_MORE = (159, 101, 96)   # Kv3.4, Ca2+ leak, post-inhibitory-rebound T-type 
 CA_ACT = (84, 144, 145, 146, 152, 186, 190)   # L-type Cav1.2, Cav1.4, CaBP4, CaCC, CAN, AQP4, TMC1
 MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channel types
 ECB_SYNTHESIS = 86   # 2-AG synthesis; step 3 deposits it into the clefts of incoming synapses
+SERCA, IP3R, RYR, PLC = 126, 127, 128, 136   # the ER Ca2+ store and the Gq -> PLC -> IP3/DAG pool
+ER_STORE = (SERCA, IP3R, RYR, PLC)
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT
+                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -130,6 +132,7 @@ CANON_DENSITY.update({t: 2.0 for t in CAV_MORE})
 CANON_DENSITY.update({t: 0.05 for t in CATION_MORE})
 CANON_DENSITY.update({159: 1.0, 101: 0.005, 96: 2.0})
 CANON_DENSITY.update({84: 2.0, 144: 5.0, 145: 1.0, 146: 1.0, 152: 0.05, 186: 1e11, 190: 0.05})
+CANON_DENSITY.update({SERCA: 1e5, IP3R: 100.0, RYR: 1.0, PLC: 100.0})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -654,6 +657,36 @@ def receptor_network(seed: int, n_neurons: int = 14, seg_per_neuron: int = 5) ->
 
 GPCR_REST = (19, 20, 21, 23, 24, 26, 27, 28, 30, 31, 32, 33, 34, 35, 37, 40, 41, 42, 45, 47, 49, 51, 53, 54,
              56, 58, 59, 61)   # the G-protein-coupled IDs receptor_network does not place
+
+
+def er_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons whose somas carry Cav2.1 (Ca2+ in), PMCA, the ER store (SERCA, IP3R, RyR) and
+    PLC driven by mGluR5 (a Gq receptor of glutamate), with AMPA+NMDA / GABA-A synapses that put
+    glutamate into the clefts; every other dendrite segment carries SERCA and RyR. ER Ca2+ starts
+    in 0.05-1 mM, cytosolic Ca2+ at 1-20x rest, IP3 in 0-0.5 uM; RyR and IP3R gates start at 0
+    (steady-state initialisation) on every other soma."""
+    soma = [CAV21, PMCA, SERCA, IP3R, RYR, PLC, 61]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], soma,
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    mods = segs["modifiers"]
+    rng = np.random.default_rng(seed + 5)
+    n = len(segs)
+    segs["ER_Ca2"] = rng.uniform(0.05, 1.0, n).astype(np.float32)
+    segs["intra_ions"][:, 2] *= rng.uniform(1.0, 20.0, n)
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    base = 5   # the dendrite set's five slots come first
+    mods["state"][soma_idx, base + 5, 0] = rng.uniform(0.0, 0.0005, len(soma_idx))   # PLC IP3 (mM)
+    mods["state"][soma_idx[1::2], base + 3, 1] = rng.uniform(0.2, 0.9, len(soma_idx[1::2]))   # IP3R h
+    mods["state"][soma_idx[1::2], base + 4, 0] = rng.uniform(0.2, 0.9, len(soma_idx[1::2]))   # RyR w
+    mods["state"][soma_idx[::3], base + 2, 1] = 0.0005   # a SERCA K_m override
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        for j, t in enumerate((SERCA, RYR)):
+            mods["type"][i, k + j] = t
+            mods["density"][i, k + j] = CANON_DENSITY[t] * rng.uniform(0.5, 2.0)
+        segs["modifier_count"][i] = k + 2
+    return net
 
 
 def gpcr_network(seed: int, n_neurons: int = 8, seg_per_neuron: int = 5) -> Network:

@@ -474,6 +474,7 @@ static int build_device(Sim& S) {
             if (m.type == 83) calcn[di] = m.state[0];
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
+            if (m.type == 126 || m.type == 127 || m.type == 128) fl |= SF_ER;
             if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
             if (!modifier_supported(m.type)) S.unsupported_mod++;
             // G-protein receptors read state[active_vars + 1]: past the record's state array
@@ -631,6 +632,8 @@ static int build_device(Sim& S) {
             for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
             if (!modifier_supported(rt)) S.unsupported_mod++;
             if (rt == 86) S.unsupported_mod++;   // eCB synthesis as a receptor would read the post's DAG
+            // the ER store and PLC as receptors would read the post's ER, IP3 and Gq pools
+            if (rt == 126 || rt == 127 || rt == 128 || rt == 136) S.unsupported_mod++;
             if (gprotein_type(rt) && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
             if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }
@@ -698,6 +701,7 @@ static int build_device(Sim& S) {
     d.atp = S.dalloc<float>(npad);
     d.temp = S.dalloc<float>(npad);
     d.txn = S.dalloc<float>(npad);
+    d.er = S.dalloc<float>(npad);
     d.nt = S.dalloc<float>((size_t)15 * npad);
     d.nt_in = S.dalloc<float>((size_t)15 * npad);
     d.ibuf = S.dalloc<double>(npad);
@@ -1047,7 +1051,7 @@ static void plan_ordered(Sim& S, float dt) {
 // Host staging of the transferred state: every array below in one block, 256-byte aligned.
 struct XferBufs {
     double *V, *in, *ex, *ms, *yrs, *a0, *a7;
-    float *atp, *temp, *txn, *nt, *nti, *yw, *yf, *yd, *yr, *yb, *ys, *yk, *yc, *ycl;
+    float *atp, *temp, *txn, *er, *nt, *nti, *yw, *yf, *yd, *yr, *yb, *ys, *yk, *yc, *ycl;
     uint8_t* yrf;
 };
 // Lays the arrays out from base (nullptr: only sizes them); returns the block size.
@@ -1061,7 +1065,8 @@ static size_t xfer_layout(const Sim& S, char* base, XferBufs& b) {
         off += (count * sizeof(T) + 255) & ~(size_t)255;
     };
     take(b.V, npad); take(b.in, 7 * npad); take(b.ex, 7 * npad); take(b.ms, (size_t)d.nslot * 7 * npad);
-    take(b.atp, npad); take(b.temp, npad); take(b.txn, npad); take(b.nt, 15 * npad); take(b.nti, 15 * npad);
+    take(b.atp, npad); take(b.temp, npad); take(b.txn, npad); take(b.er, npad); take(b.nt, 15 * npad);
+    take(b.nti, 15 * npad);
     take(b.yw, ns); take(b.yf, ns); take(b.yd, ns); take(b.yr, ns); take(b.yb, ns); take(b.ys, ns);
     take(b.yk, ns); take(b.yc, ns); take(b.ycl, 15 * ns); take(b.yrf, ns); take(b.yrs, (size_t)d.nrec * 7 * ns);
     take(b.a0, S.nar); take(b.a7, S.nar);
@@ -1128,6 +1133,7 @@ static void xfer(Sim& S, bool to_device) {
                     b.ex[(size_t)k * npad + di] = s.extra_ions[k];
                 }
                 b.atp[di] = s.ATP_level; b.temp[di] = s.temperature; b.txn[di] = s.transcription_signal;
+                b.er[di] = s.ER_Ca2;
                 for (int k = 0; k < 15; k++) {
                     b.nt[(size_t)k * npad + di] = s.neurotransmitter[k];
                     b.nti[(size_t)k * npad + di] = s.intra_nt[k];
@@ -1159,6 +1165,7 @@ static void xfer(Sim& S, bool to_device) {
     cp(d.atp, b.atp, (size_t)npad * 4);
     cp(d.temp, b.temp, (size_t)npad * 4);
     cp(d.txn, b.txn, (size_t)npad * 4);
+    cp(d.er, b.er, (size_t)npad * 4);
     cp(d.nt, b.nt, (size_t)15 * npad * 4);
     cp(d.nt_in, b.nti, (size_t)15 * npad * 4);
     if (ns) {
@@ -1199,6 +1206,7 @@ static void xfer(Sim& S, bool to_device) {
                 s.extra_ions[k] = b.ex[(size_t)k * npad + di];
             }
             s.ATP_level = b.atp[di]; s.temperature = b.temp[di]; s.transcription_signal = b.txn[di];
+            s.ER_Ca2 = b.er[di];
             for (int k = 0; k < 15; k++) {
                 s.neurotransmitter[k] = b.nt[(size_t)k * npad + di];
                 s.intra_nt[k] = b.nti[(size_t)k * npad + di];

@@ -95,7 +95,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 161: case 166: case 167: case 208: case 178: case 176: case 168: case 200: case 201:
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
         case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
-        case 145: case 146: case 152: case 186: case 190: case 86:
+        case 145: case 146: case 152: case 186: case 190: case 86: case 126: case 127: case 128:
+        case 136:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -117,6 +118,9 @@ struct SegRegs {
     float camkii01;         // its state[0] + state[1], read by Kv4.2
     float cabp4;            // state[0] of the first CaBP4 modifier, set before each Cav1.4 runs
     float dag;              // state[1] of the first PLC/IP3/DAG modifier, set before eCB synthesis runs
+    float ip3;              // its state[0], set before IP3R runs
+    float gq;               // the Gq receptors' state[0] summed, set before PLC runs
+    float er;               // ER_Ca2 (segments with SERCA, IP3R or RyR)
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -1848,6 +1852,120 @@ BC_DEV double ch_cav14(const ModSlot& m, SegRegs& s, float dt) {
     return -ica;
 }
 
+// ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
+// segment's ER pool s.er (Segment.ER_Ca2), IP3R (127: Li-Rinzel gating, IP3 = s.ip3) and RyR
+// (128: Keizer-Levine gating with adaptation w) release it; no membrane current. Constants
+// units.h:130-156.
+BC_DEV double ch_serca(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    double km = (m.get(1) > 0.0f) ? m.get(1) : 0.00027f;
+    float ca = s.in[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float fca = (ca * ca) / (ca * ca + km * km);
+    float atp = s.atp;
+    if (atp < 0.0f) atp = 0.0f;
+    float fatp = atp / (atp + 0.1f);
+    double N = m.dens * area;
+    float flux = N * 100.0f * fca * fatp * 2.0f * dt * 1e-3f / 6.02214076e23f;
+    float dconc = flux / (s.vol * 1e-18f);
+    float avail = s.in[2] - 1e-9f;
+    if (avail < 0.0f) avail = 0.0f;
+    float moved = (dconc < avail) ? dconc : avail;
+    s.in[2] -= moved;
+    s.er += moved;
+    floor7(s.in);
+    if (s.er < 0.0f) s.er = 0.0f;
+    s.atp -= flux * 1e6f;
+    if (s.atp < 0.0f) s.atp = 0.0f;
+    return 0.0f;
+}
+BC_DEV double ch_ip3r(const ModSlot& m, SegRegs& s, float dt) {
+    float ip3 = s.ip3;
+    double h = m.get(1);
+    float ca = s.in[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float er = s.er;
+    if (er < 0.0f) er = 0.0f;
+    float minf = ip3 / (ip3 + 0.00013f);
+    float ninf = ca / (ca + 0.0002f);
+    float hinf = 0.001f / (0.001f + ca);
+    if (h <= 0.0f) h = hinf;
+    h += (hinf - h) * dt / 100.0f;
+    if (h < 0.0f) h = 0.0f;
+    if (h > 1.0f) h = 1.0f;
+    float m3 = minf * minf * minf;
+    float n3 = ninf * ninf * ninf;
+    float h3 = h * h * h;
+    float popen = m3 * n3 * h3;
+    float drive = er - ca;
+    if (drive < 0.0f) drive = 0.0f;
+    double dconc = 5.0f * m.dens * popen * drive * dt * 1e-3f;
+    float avail = er - 1e-9f;
+    if (avail < 0.0f) avail = 0.0f;
+    float moved = (dconc < avail) ? dconc : avail;
+    s.er -= moved;
+    s.in[2] += moved;
+    floor7(s.in);
+    if (s.er < 0.0f) s.er = 0.0f;
+    m.set(1, h);
+    return 0.0f;
+}
+BC_DEV double ch_ryr(const ModSlot& m, SegRegs& s, float dt) {
+    double w = m.get(0);
+    float ca = s.in[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float er = s.er;
+    if (er < 0.0f) er = 0.0f;
+    float ckb = ca / 0.000636f;
+    float ckb2 = ckb * ckb;
+    float ckb3 = ckb2 * ckb;
+    float kac = 0.000372f / ca;
+    float kac4 = kac * kac * kac * kac;
+    float winf = (1.0f + kac4 + ckb3) / (1.0f + 1.0f / 0.057f + kac4 + ckb2);
+    if (w <= 0.0f) w = winf;
+    float popen = w * (1.0f + ckb3) / (1.0f + kac4 + ckb3);
+    float tauw = 10000.0f * winf;
+    if (tauw < 1.0f) tauw = 1.0f;
+    w += (winf - w) * dt / tauw;
+    if (w < 0.0f) w = 0.0f;
+    if (w > 1.0f) w = 1.0f;
+    float drive = er - ca;
+    if (drive < 0.0f) drive = 0.0f;
+    double dconc = 20.0f * m.dens * popen * drive * dt * 1e-3f;
+    float avail = er - 1e-9f;
+    if (avail < 0.0f) avail = 0.0f;
+    float moved = (dconc < avail) ? dconc : avail;
+    s.er -= moved;
+    s.in[2] += moved;
+    floor7(s.in);
+    if (s.er < 0.0f) s.er = 0.0f;
+    m.set(0, w);
+    return 0.0f;
+}
+// PLC (136, modifier_helpers.h:1553-1580): the Gq receptors' activation s.gq drives IP3
+// (state[0]), DAG (state[1]) and a desensitisation (state[2]); constants units.h:342-349.
+BC_DEV double ch_plc(const ModSlot& m, const SegRegs& s, float dt) {
+    float gq = s.gq;
+    if (gq > 1.0f) gq = 1.0f;
+    float vip3 = (m.get(3) > 0.0f) ? m.get(3) : 0.0005f;
+    float vdag = (m.get(4) > 0.0f) ? m.get(4) : 0.5f;
+    float drive = 0.02f + gq;
+    float ip3 = m.get(0);
+    ip3 += (vip3 * m.dens * drive) * dt * 0.001f - ip3 * dt / 1000.0f;
+    if (ip3 < 0.0f) ip3 = 0.0f;
+    m.set(0, ip3);
+    float dag = m.get(1);
+    dag += (vdag * m.dens * drive) * dt * 0.001f - dag * dt / 2000.0f;
+    if (dag < 0.0f) dag = 0.0f;
+    m.set(1, dag);
+    float dz = m.get(2);
+    dz += (gq - dz) * dt / 70000.0f;
+    if (dz < 0.0f) dz = 0.0f;
+    if (dz > 1.0f) dz = 1.0f;
+    m.set(2, dz);
+    return 0.0f;
+}
+
 // eCB synthesis (86, hh_biological.cpp:1746-1764): 2-AG from Ca (Hill 2, Kd 0.5 uM) scaled by
 // 1 + DAG (s.dag), degraded at rate state[2] (0.005 if not positive), synthesis rate state[1]
 // (0.01 if not positive), dt scaled by the density, state[0] clamped to [0, 10]; no current.
@@ -2372,6 +2490,10 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 144: return ch_cav14(m, s, dt);
         case 145: return ch_cabp4(m, dt);
         case 86: return ch_ecb(m, s, dt);
+        case 126: return ch_serca(m, s, dt);
+        case 127: return ch_ip3r(m, s, dt);
+        case 128: return ch_ryr(m, s, dt);
+        case 136: return ch_plc(m, s, dt);
         case 146: return ch_cacc(m, s, dt);
         case 152: return ch_can(m, s, dt);
         case 186: return ch_aqp4(m, s, dt);

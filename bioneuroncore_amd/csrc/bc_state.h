@@ -29,6 +29,7 @@ enum : uint8_t {
     SF_WATCHED = 8,       // is the presynaptic segment of some synapse
     SF_NTREAD = 16,       // a modifier reads (receptors) or takes up (transporters) transmitters
     SF_NTX = 32,          // an incoming synapse carries a cleft transmitter other than glutamate/GABA
+    SF_ER = 64,           // carries SERCA, IP3R or RyR: step 1 reads and writes its ER Ca2+ store
 };
 
 // Synapse flag bits (static).
@@ -115,6 +116,7 @@ struct Net {
     float* atp;
     float* temp;
     float* txn;
+    float* er;           // ER_Ca2, the ER Ca2+ store SERCA fills and IP3R/RyR release
     float* nt;           // neurotransmitter [15][npad]
     float* nt_in;        // intra_nt [15][npad]
     double* ibuf;

@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
         for (int k = 0; k < 12; k++) s.nt[k] = d.nt[k * P + i];
     }
+    if (kExt && (sfl & SF_ER)) s.er = d.er[i];
     const bool my = sfl & SF_MYELIN;
     const uint32_t cnt = d.mcount[i];
     double itot = 0.0;
@@ -57,6 +58,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     s.cabp4 = d.mstate[(size_t)q * 7 * P + i];
                     break;
                 }
+        }
+        if (kExt && t == 127) {   // IP3R reads the first PLC/IP3/DAG modifier's state[0] as it stands now
+            s.ip3 = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 136) {
+                    s.ip3 = d.mstate[(size_t)q * 7 * P + i];
+                    break;
+                }
+        }
+        if (kExt && t == 136) {   // PLC sums the Gq receptors' state[0] (52-62, 188) as they stand now
+            float gq = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++) {
+                const uint8_t tq = d.mtype[q * P + i];
+                if ((tq >= 52 && tq <= 62) || tq == 188) gq += d.mstate[(size_t)q * 7 * P + i];
+            }
+            s.gq = gq;
         }
         if (kExt && t == 86) {   // eCB synthesis reads the first PLC/IP3/DAG modifier's state[1]
             s.dag = 0.0f;
@@ -135,6 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             d.ex[k * P + i] = s.ex[k];
         }
         d.atp[i] = s.atp;
+        if (kExt && (sfl & SF_ER)) d.er[i] = s.er;
     }
     d.ibuf[i] = itot;
 }

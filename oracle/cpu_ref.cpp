@@ -1846,6 +1846,129 @@ static double mod_ecb(Modifier* m, Segment* s, float dt) {
     return 0.0;
 }
 
+// ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
+// segment's ER pool Segment.ER_Ca2, IP3R (127) and RyR (128) release it back; none of them
+// carries a membrane current. Constants from units.h:130-156.
+static float seg_ip3(const Segment* s) {   // modifier_helpers.h:1491-1495: the first PLC's state[0]
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == 136) return s->modifiers[j].state[0];
+    return 0.0f;
+}
+static double mod_serca(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:3473-3506
+    float area = eff_area(m, s);
+    double km = (m->state[1] > 0.0f) ? m->state[1] : 0.00027f;
+    float ca = s->intra_ions[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float fca = (ca * ca) / (ca * ca + km * km);   // Hill 2
+    float atp = s->ATP_level;
+    if (atp < 0.0f) atp = 0.0f;
+    float fatp = atp / (atp + 0.1f);
+    double N = m->density * area;
+    float flux = N * 100.0f * fca * fatp * 2.0f * dt * 1e-3f / 6.02214076e23f;   // mol of Ca2+
+    float dconc = flux / (volume(s) * 1e-18f);
+    float avail = s->intra_ions[2] - 1e-9f;
+    if (avail < 0.0f) avail = 0.0f;
+    float moved = (dconc < avail) ? dconc : avail;
+    s->intra_ions[2] -= moved;
+    s->ER_Ca2 += moved;
+    floor_ions(s->intra_ions);
+    if (s->ER_Ca2 < 0.0f) s->ER_Ca2 = 0.0f;
+    s->ATP_level -= flux * 1e6f;   // one ATP per two Ca2+, in the pool's units
+    if (s->ATP_level < 0.0f) s->ATP_level = 0.0f;
+    return 0.0f;
+}
+static double mod_ip3r(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:3508-3548
+    float ip3 = seg_ip3(s);
+    double h = m->state[1];
+    float ca = s->intra_ions[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float er = s->ER_Ca2;
+    if (er < 0.0f) er = 0.0f;
+    float minf = ip3 / (ip3 + 0.00013f);   // Li-Rinzel: m_inf^3 n_inf^3 h^3
+    float ninf = ca / (ca + 0.0002f);
+    float hinf = 0.001f / (0.001f + ca);
+    if (h <= 0.0f) h = hinf;
+    h += (hinf - h) * dt / 100.0f;
+    if (h < 0.0f) h = 0.0f;
+    if (h > 1.0f) h = 1.0f;
+    float m3 = minf * minf * minf;
+    float n3 = ninf * ninf * ninf;
+    float h3 = h * h * h;
+    float popen = m3 * n3 * h3;
+    float drive = er - ca;
+    if (drive < 0.0f) drive = 0.0f;
+    double dconc = 5.0f * m->density * popen * drive * dt * 1e-3f;
+    float avail = er - 1e-9f;
+    if (avail < 0.0f) avail = 0.0f;
+    float moved = (dconc < avail) ? dconc : avail;
+    s->ER_Ca2 -= moved;
+    s->intra_ions[2] += moved;
+    floor_ions(s->intra_ions);
+    if (s->ER_Ca2 < 0.0f) s->ER_Ca2 = 0.0f;
+    m->state[1] = h;
+    return 0.0f;
+}
+static double mod_ryr(Modifier* m, Segment* s, float dt) {   // hh_biological.cpp:3550-3591
+    double w = m->state[0];
+    float ca = s->intra_ions[2];
+    if (ca < 1e-9f) ca = 1e-9f;
+    float er = s->ER_Ca2;
+    if (er < 0.0f) er = 0.0f;
+    float ckb = ca / 0.000636f;   // Keizer-Levine: (c/K_b)^3, (K_a/c)^4
+    float ckb2 = ckb * ckb;
+    float ckb3 = ckb2 * ckb;
+    float kac = 0.000372f / ca;
+    float kac4 = kac * kac * kac * kac;
+    float winf = (1.0f + kac4 + ckb3) / (1.0f + 1.0f / 0.057f + kac4 + ckb2);
+    if (w <= 0.0f) w = winf;
+    float popen = w * (1.0f + ckb3) / (1.0f + kac4 + ckb3);
+    float tauw = 10000.0f * winf;
+    if (tauw < 1.0f) tauw = 1.0f;
+    w += (winf - w) * dt / tauw;
+    if (w < 0.0f) w = 0.0f;
+    if (w > 1.0f) w = 1.0f;
+    float drive = er - ca;
+    if (drive < 0.0f) drive = 0.0f;
+    double dconc = 20.0f * m->density * popen * drive * dt * 1e-3f;
+    float avail = er - 1e-9f;
+    if (avail < 0.0f) avail = 0.0f;
+    float moved = (dconc < avail) ? dconc : avail;
+    s->ER_Ca2 -= moved;
+    s->intra_ions[2] += moved;
+    floor_ions(s->intra_ions);
+    if (s->ER_Ca2 < 0.0f) s->ER_Ca2 = 0.0f;
+    m->state[0] = w;
+    return 0.0f;
+}
+// PLC (136, modifier_helpers.h:1553-1580): the segment's Gq-receptor activations (states[0] of
+// the types is_gq_receptor_type lists, modifier_helpers.h:1426-1432: 52-62 and 188) drive IP3
+// (state[0]), DAG (state[1]) and a desensitisation (state[2]); constants units.h:342-349.
+static double mod_plc(Modifier* m, Segment* s, float dt) {
+    float gq = 0.0f;
+    for (uint8_t j = 0; j < s->modifier_count; j++) {
+        const uint8_t t = s->modifiers[j].type;
+        if ((t >= 52 && t <= 62) || t == 188) gq += s->modifiers[j].state[0];
+    }
+    if (gq > 1.0f) gq = 1.0f;
+    float vip3 = (m->state[3] > 0.0f) ? m->state[3] : 0.0005f;
+    float vdag = (m->state[4] > 0.0f) ? m->state[4] : 0.5f;
+    float drive = 0.02f + gq;
+    float ip3 = m->state[0];
+    ip3 += (vip3 * m->density * drive) * dt * 0.001f - ip3 * dt / 1000.0f;
+    if (ip3 < 0.0f) ip3 = 0.0f;
+    m->state[0] = ip3;
+    float dag = m->state[1];
+    dag += (vdag * m->density * drive) * dt * 0.001f - dag * dt / 2000.0f;
+    if (dag < 0.0f) dag = 0.0f;
+    m->state[1] = dag;
+    float dz = m->state[2];
+    dz += (gq - dz) * dt / 70000.0f;
+    if (dz < 0.0f) dz = 0.0f;
+    if (dz > 1.0f) dz = 1.0f;
+    m->state[2] = dz;
+    return 0.0f;
+}
+
 // Presynaptic release modulators (receptors_biological.cpp:427-491), the CPU step's dispatch
 // with no plugin registered (kernel_dispatch.h:575-604): Gi/o types lower release by
 // 0.7 * state[0] * density * area (state[0] relaxing to the cleft activation nt / (nt + 0.5)
@@ -2215,6 +2338,10 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_GLYT2: return chan_transporter(m, s, dt, kGlyt2);
         case T_GAT1: return chan_transporter(m, s, dt, kGat1);
         case T_ECB: return mod_ecb(m, s, dt);
+        case 126: return mod_serca(m, s, dt);
+        case 127: return mod_ip3r(m, s, dt);
+        case 128: return mod_ryr(m, s, dt);
+        case 136: return mod_plc(m, s, dt);
         default:
             if (type >= 17 && type <= 40) return chan_gio(m, s, dt, gio_nt(type));
             if (type >= 41 && type <= 51) return chan_gs(m, s, dt, gs_nt(type));

@@ -5,6 +5,8 @@ compiled reference on the same networks, tests/test_oracle_vs_reference.py):
   HCN2 (src/hh_biological.cpp:99-141, 2574-2865, 3414-3445, 3599);
 - the channel set of the spinal cell types: developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, the two
   persistent Na+ channels, Kir2.1 and the ClC-2 leak;
+- the ER Ca2+ store (SERCA, IP3R, RyR) and PLC (src/hh_biological.cpp:3473-3591,
+  include/modifier_helpers.h:1553-1580);
 - a network that mixes them with the core channel set runs the extended step-1 kernel, and the
   core-only networks keep the lean one (results identical either way).
 """
@@ -58,6 +60,18 @@ def test_pmca_hill_other_than_two_is_counted():
     assert sim.load_network(net) == 0
     assert sim.stats().unsupported_modifiers == 1
     sim.close()
+
+
+@pytest.mark.parametrize("seed", [61, 62])
+def test_er_store_and_plc(seed):
+    """SERCA, IP3R and RyR with the segment's ER Ca2+ store, PLC fed by mGluR5 (src/hh_biological.cpp:
+    3473-3591, include/modifier_helpers.h:1553-1580), 400 ticks: every byte, ER_Ca2 included."""
+    net = bion.er_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    seg, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0
+    er0 = net.segments["ER_Ca2"]
+    assert np.any(seg["ER_Ca2"] > er0) and np.any(seg["ER_Ca2"] < er0)
 
 
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])

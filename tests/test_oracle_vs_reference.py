@@ -124,6 +124,21 @@ def test_ion_homeostasis_modifiers(seed):
     assert sp.sum() > 0
 
 
+@pytest.mark.parametrize("seed", [61, 62])
+def test_er_store_and_plc(seed):
+    """SERCA, IP3R and RyR moving Ca2+ between the cytosol and ER_Ca2, and PLC turning mGluR5
+    activation into the IP3 IP3R reads (src/hh_biological.cpp:3473-3591,
+    include/modifier_helpers.h:1553-1580): every byte of the records, ER_Ca2 included."""
+    net = bion.er_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    rs, _, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
+    er0 = net.segments["ER_Ca2"]
+    assert np.any(rs["ER_Ca2"] > er0) and np.any(rs["ER_Ca2"] < er0)   # pumped in and released
+    plc = rs["modifiers"]["type"] == bion.PLC
+    assert np.any(rs["modifiers"]["state"][plc][:, 1] > 0.0)   # DAG produced
+
+
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
 def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2;
