@@ -409,6 +409,10 @@ def main():
     ap.add_argument("--adjoint-ticks", type=int, default=20000)
     ap.add_argument("--adjoint-every", type=int, default=500)
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint/forward cost")
+    ap.add_argument("--autotune-ticks", type=int, default=0,
+                    help="live ticks per schedule candidate of the autotuner before the tick leg (0: defaults)")
+    ap.add_argument("--schedule", default="",
+                    help="launch schedule as field=value[,field=value] (replaces the autotuner's search)")
     ap.add_argument("--step1-chunks", type=int, default=0,
                     help="override the schedule's step1_chunks (1: steps 1 and 2.5 on one stream)")
     ap.add_argument("--no-graphs", action="store_true",
@@ -462,9 +466,15 @@ def main():
     rc = sim.load_network(net)
     if rc != 0:
         raise SystemExit(f"load failed: {rc}")
+    print(f"# loaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    tuned = 0
+    if args.schedule:
+        sim.set_schedule(**{k: int(v) for k, v in (f.split("=") for f in args.schedule.split(","))})
+    elif args.autotune_ticks > 0:   # the schedule search runs live ticks of this network
+        tuned = sim.autotune(0.025, args.autotune_ticks)
+        print(f"# autotuned in {tuned} ticks: {sim.schedule().as_dict()}", file=sys.stderr, flush=True)
     if args.step1_chunks > 0:
         sim.set_schedule(step1_chunks=args.step1_chunks)
-    print(f"# loaded in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     nt_in_live = bool(np.any(net.segments["intra_nt"].view(np.uint32)))
     algo = algorithmic_bytes(net, len(net.synapses) == 0, nt_in_live)
     tick_bytes = tick_bytes_model(net)
@@ -495,6 +505,7 @@ def main():
         c_el = time.perf_counter() - t0
         compat = {"ticks": args.compat_ticks, "ms_per_tick": c_el / args.compat_ticks * 1e3,
                   "segment_ticks_per_s": n_seg * args.compat_ticks / c_el}
+    sched = sim.schedule().as_dict()
     sim.close()   # its host records and device state are not needed by the other legs
     del sim, stims
     ms_step = el / args.steps * 1e3
@@ -525,6 +536,7 @@ def main():
         "tick_roofline": {"model_bytes_per_tick": tick_bytes, "achieved": tick_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": tick_gbs / HBM_PEAK_GBS},
         "phase_ms": {PHASES[k]: float(mean[k]) for k in range(len(PHASES))},
+        "schedule": sched, "autotune_ticks": tuned,
     }
     if compat:
         out["compat"] = compat
