@@ -22,6 +22,27 @@
 #include "bc_bigisland.h"
 #include "bc_adjoint.hip"
 
+// Registered plugin functions (src/plugin_manager.cpp:37-75). The reference's CPU tick calls them
+// in place of the built-in modifier current or release modulator of their type
+// (include/kernel_dispatch.h:337-343); its device code never does. A network with an instance
+// of such a type is counted as unsupported work at load (refused unless bchip_allow_unsupported),
+// so a registered plugin cannot make the device silently depart from the CPU tick.
+static std::mutex g_plugin_mu;
+static ModifierFunc g_mod_table[128];
+static ReleaseModulatorFunc g_rel_table[128];
+struct PluginTypes {
+    bool mod[256] = {}, rel[256] = {};
+};
+static PluginTypes plugin_types() {
+    PluginTypes p;
+    std::lock_guard<std::mutex> lk(g_plugin_mu);
+    for (int t = 0; t < 128; t++) {
+        p.mod[t] = g_mod_table[t] != nullptr;
+        p.rel[t] = g_rel_table[t] != nullptr;
+    }
+    return p;
+}
+
 namespace bc {
 
 #define HIPCHK(x)                                                                        \
@@ -447,6 +468,7 @@ static int build_device(Sim& S) {
     std::vector<uint8_t> mtype((size_t)nslot * npad, 0), mav((size_t)nslot * npad, 0);
     std::vector<double> mdens((size_t)nslot * npad, 0.0);
     S.unsupported_mod = 0;
+    const PluginTypes plug = plugin_types();
     S.nt_read = false;
     S.ext_mods = false;
     for (uint32_t di = 0; di < n; di++) {
@@ -476,7 +498,7 @@ static int build_device(Sim& S) {
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
             if (m.type == 126 || m.type == 127 || m.type == 128) fl |= SF_ER;
             if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
-            if (!modifier_supported(m.type)) S.unsupported_mod++;
+            if (!modifier_supported(m.type) || plug.mod[m.type]) S.unsupported_mod++;
             // G-protein receptors read state[active_vars + 1]: past the record's state array
             if (gprotein_type(m.type) && m.active_vars >= 7) S.unsupported_mod++;
             // PMCA's Hill powers are exact squares only for n = 2 and a float-valued K_m
@@ -617,6 +639,7 @@ static int build_device(Sim& S) {
             y_ar[k] = (uint32_t)(ar_type.size() << 4) | y.pre_autoreceptor_count;
             for (uint32_t j = 0; j < y.pre_autoreceptor_count; j++) {
                 const Modifier& m = y.pre_autoreceptors[j];
+                if (plug.rel[m.type]) S.unsupported_mod++;   // a plugin modulator replaces the built-in one
                 ar_type.push_back(m.type);
                 ar_dens.push_back(m.density);
                 ar_st0.push_back(m.state[0]);
@@ -630,7 +653,7 @@ static int build_device(Sim& S) {
             y_rav[(size_t)j * ns + k] = y.post_receptors[j].active_vars;
             y_rdens[(size_t)j * ns + k] = y.post_receptors[j].density;
             for (int q = 0; q < 7; q++) y_rstate[((size_t)j * 7 + q) * ns + k] = y.post_receptors[j].state[q];
-            if (!modifier_supported(rt)) S.unsupported_mod++;
+            if (!modifier_supported(rt) || plug.mod[rt]) S.unsupported_mod++;
             if (rt == 86) S.unsupported_mod++;   // eCB synthesis as a receptor would read the post's DAG
             // the ER store and PLC as receptors would read the post's ER, IP3 and Gq pools
             if (rt == 126 || rt == 127 || rt == 128 || rt == 136) S.unsupported_mod++;
@@ -1839,10 +1862,6 @@ static int adjoint_run(Sim& S, Mem& mem, float dt, uint32_t ticks, uint32_t chec
 using bc::Sim;
 
 // ================================================================== C ABI
-static std::mutex g_plugin_mu;
-static ModifierFunc g_mod_table[128];
-static ReleaseModulatorFunc g_rel_table[128];
-
 extern "C" {
 
 int gpu_backend_device_count(void) {

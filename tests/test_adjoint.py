@@ -267,3 +267,63 @@ def test_adjoint_rejects_unsupported_runs():
     with pytest.raises(ValueError):
         sim.set_cable_mode(True)
     sim.close()
+
+
+def test_long_horizon_gradients_with_config5_checkpoints():
+    """2000 ticks (50 ms) with checkpoints every 500 ticks, the adjoint workload's interval: the
+    loss equals the FP64 restatement's and the gradient's directional derivatives (each channel
+    type over both cells, every coupling scale, a random direction per cell) equal FP64 central
+    differences (h = 1e-3 relative) within 1e-3 relative."""
+    T, K = 2000, 500
+    cells, scales = _cells(), _scales(_cells())
+    t = np.arange(T, dtype=np.float32)
+    tgt = np.stack([-60.0 + 4.0 * np.sin(t / 300.0), -55.0 - 3.0 * np.cos(t / 250.0)], axis=1).astype(np.float32)
+    sim = _gpu(cells, scales)
+    L, gd, gs = sim.adjoint(DT, T, K, _soma(cells), tgt)
+    sim.close()
+
+    def loss64(cs, ss):
+        tot = 0.0
+        for k, c in enumerate(cs):
+            soma = int(np.nonzero(c.segments["segment_type"] == 1)[0][0])
+            tot += H.Cable64(c.segments, c.connections, scale=ss[k], current=_stim(c, k)).loss(
+                DT, T, [soma], tgt[:, k:k + 1])
+        return tot / len(cs)
+
+    L64 = loss64(cells, scales)
+    print(f"loss GPU {L:.9g} FP64 {L64:.9g}", flush=True)
+    assert abs(L - L64) < 1e-3 * L64
+    per = _split(cells, gd, gs)
+    rng = np.random.default_rng(11)
+    checks = []
+
+    def fd(dens, scale):
+        lp = loss64(*_perturbed(cells, scales, {q: 1 + H_REL * r for q, r in dens.items()},
+                                {q: 1 + H_REL * r for q, r in scale.items()}))
+        lm = loss64(*_perturbed(cells, scales, {q: 1 - H_REL * r for q, r in dens.items()},
+                                {q: 1 - H_REL * r for q, r in scale.items()}))
+        return (lp - lm) / (2 * H_REL)
+
+    for j in range(5):
+        q = {(k, i, j): 1.0 for k, c in enumerate(cells) for i in range(len(c.segments))}
+        a = sum(float((per[k][0][:, j] * c.segments["modifiers"]["density"][:, j]).sum()) for k, c in enumerate(cells))
+        checks.append((f"slot {j} over both cells", a, fd(q, {})))
+    q = {(k, e): 1.0 for k, c in enumerate(cells) for e in range(len(c.connections))}
+    a = sum(float((per[k][1] * scales[k]).sum()) for k in range(len(cells)))
+    checks.append(("every coupling scale", a, fd({}, q)))
+    for k, c in enumerate(cells):
+        dens = c.segments["modifiers"]["density"]
+        rd = rng.choice([-1.0, 1.0], size=(len(c.segments), 5))
+        rs = rng.choice([-1.0, 1.0], size=len(c.connections))
+        a = float((per[k][0][:, :5] * dens[:, :5] * rd).sum() + (per[k][1] * scales[k] * rs).sum())
+        f = fd({(k, i, j): rd[i, j] for i in range(len(c.segments)) for j in range(5)},
+               {(k, e): rs[e] for e in range(len(c.connections))})
+        checks.append((f"cell {k} random direction", a, f))
+    big = max(abs(f) for _, _, f in checks)
+    bad = []
+    for name, a, f in checks:
+        err = abs(a - f)
+        print(f"{name:28s} adjoint {a: .6e} fd {f: .6e} err {err:.2e}", flush=True)
+        if err > 1e-3 * abs(f) + 1e-6 * big:
+            bad.append(name)
+    assert not bad, bad

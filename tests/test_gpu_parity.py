@@ -1,6 +1,8 @@
 """HIP tick vs the oracle on identical inputs: bit-exact on every byte of the segment and
 synapse records, on spike counts and on the delay ring (integer, index and floating-point
 state alike - the device reproduces the reference's arithmetic operation by operation)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -275,6 +277,44 @@ def _gpu_two_dt(net, clamp):
     res = (out.segments.copy(), out.synapses.copy())
     sim.close()
     return res
+
+
+def test_registered_plugins_are_refused():
+    """A modifier type or release modulator with a registered plugin function runs that function
+    on the reference's CPU tick (include/kernel_dispatch.h:337-343, 575-582) and the built-in one
+    on the device: a network with such an instance is refused unless allowed, and loads again
+    once the registration is cleared."""
+    lib = backend.load_library()
+    mod_fn = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_void_p, C.c_float)(lambda m, s, dt: 0.0)
+    rel_fn = C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_void_p, C.c_float)(lambda m, c, dt: 1.0)
+    lib.register_modifier_current.argtypes = [C.c_uint8, C.c_void_p]
+    lib.register_modulator_release.argtypes = [C.c_uint8, C.c_void_p]
+    net = bion.config1(seed=9)
+    n_k2p = int((net.segments["modifiers"]["type"] == bion.K2P).sum())
+    try:
+        lib.register_modifier_current(bion.K2P, C.cast(mod_fn, C.c_void_p))
+        assert lib.bchip_registered_modifier(bion.K2P) == 1
+        sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+        assert sim.load_network(net.copy()) == backend.ERR_UNSUPPORTED
+        assert sim.stats().unsupported_modifiers == n_k2p
+        sim.allow_unsupported(True)
+        assert sim.load_network(net.copy()) == 0
+        sim.close()
+    finally:
+        lib.register_modifier_current(bion.K2P, None)
+    mnet = bion.modulated_network(1)
+    assert (mnet.synapses["pre_autoreceptor_count"] > 0).any()
+    t = int(mnet.synapses["pre_autoreceptors"]["type"][mnet.synapses["pre_autoreceptor_count"] > 0][0, 0])
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    try:
+        lib.register_modulator_release(t, C.cast(rel_fn, C.c_void_p))
+        assert sim.load_network(mnet.copy()) == backend.ERR_UNSUPPORTED
+        assert sim.stats().unsupported_modifiers > 0
+    finally:
+        lib.register_modulator_release(t, None)
+    assert sim.load_network(mnet.copy()) == 0
+    assert sim.stats().unsupported_modifiers == 0
+    sim.close()
 
 
 def test_unsupported_work_is_refused(tmp_path):
