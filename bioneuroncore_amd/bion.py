@@ -110,8 +110,10 @@ MYELIN = 4   # no current; blocks the ionotropic receptors and the listed channe
 ECB_SYNTHESIS = 86   # 2-AG synthesis; step 3 deposits it into the clefts of incoming synapses
 SERCA, IP3R, RYR, PLC = 126, 127, 128, 136   # the ER Ca2+ store and the Gq -> PLC -> IP3/DAG pool
 ER_STORE = (SERCA, IP3R, RYR, PLC)
+METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR = 3, 5, 77, 78, 79, 188
+SIGNALLING = (METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR)
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE
+                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -133,6 +135,7 @@ CANON_DENSITY.update({t: 0.05 for t in CATION_MORE})
 CANON_DENSITY.update({159: 1.0, 101: 0.005, 96: 2.0})
 CANON_DENSITY.update({84: 2.0, 144: 5.0, 145: 1.0, 146: 1.0, 152: 0.05, 186: 1e11, 190: 0.05})
 CANON_DENSITY.update({SERCA: 1e5, IP3R: 100.0, RYR: 1.0, PLC: 100.0})
+CANON_DENSITY.update({METABOTROPIC: 2.0, RELEASE_SITE: 1.0, SOMA_ATP: 1.0, SOMA_THERMAL: 1.0, CREB: 1.0, MRGPR: 1.0})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -683,6 +686,39 @@ def er_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Netwo
     for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
         k = int(segs["modifier_count"][i])
         for j, t in enumerate((SERCA, RYR)):
+            mods["type"][i, k + j] = t
+            mods["density"][i, k + j] = CANON_DENSITY[t] * rng.uniform(0.5, 2.0)
+        segs["modifier_count"][i] = k + 2
+    return net
+
+
+def signalling_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons whose somas carry Cav2.1, the metabotropic G protein (3) on the summed
+    transmitters, the SK-like release site (5), somatic ATP synthesis (77) and thermoregulation
+    (78), CREB transcription (79), and MRGPR (188) with PLC, with AMPA+NMDA / GABA-A synapses;
+    every other dendrite segment carries the metabotropic G protein and the release site.
+    Temperatures start in 35-41 C, ATP in 0.5-4, cytosolic Ca2+ at 1-20x rest, MRGPR ligand in
+    0-60 uM on every other soma."""
+    soma = [CAV21, METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR, PLC]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], soma,
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    mods = segs["modifiers"]
+    rng = np.random.default_rng(seed + 7)
+    n = len(segs)
+    segs["temperature"] = rng.uniform(35.0, 41.0, n).astype(np.float32)
+    segs["ATP_level"] = rng.uniform(0.5, 4.0, n).astype(np.float32)
+    segs["intra_ions"][:, 2] *= rng.uniform(1.0, 20.0, n)
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    base = 5
+    mods["state"][soma_idx[::2], base + 6, 4] = rng.uniform(0.0, 60.0, len(soma_idx[::2]))   # MRGPR ligand
+    mods["state"][soma_idx[1::3], base + 1, 2] = 0.5    # metabotropic coupling override
+    mods["state"][soma_idx[1::3], base + 2, 4] = 40.0   # release-site facilitation tau
+    mods["state"][soma_idx[::3], base + 4, 1] = 800.0   # faster thermoregulation
+    mods["state"][soma_idx[::4], base + 5, 3] = 0.0002  # CREB Ca2+ K_d override
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        for j, t in enumerate((METABOTROPIC, RELEASE_SITE)):
             mods["type"][i, k + j] = t
             mods["density"][i, k + j] = CANON_DENSITY[t] * rng.uniform(0.5, 2.0)
         segs["modifier_count"][i] = k + 2

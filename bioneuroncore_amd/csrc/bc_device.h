@@ -76,7 +76,7 @@ __host__ __device__ static inline bool gprotein_type(uint8_t t) {
 __host__ __device__ static inline bool modifier_reads_nt(uint8_t t) {
     return t == M_AMPA || t == M_NMDA || t == M_GABAA || (t >= M_GLAST && t <= M_GAT3) ||
            (t >= M_EAAT3 && t <= M_GAT1) || (t >= 10 && t <= 64) || (t >= 67 && t <= 68) ||
-           (t >= 70 && t <= 74) || t == 96 || t == 97 || t == 134 || t == 194;
+           (t >= 70 && t <= 74) || t == 96 || t == 97 || t == 134 || t == 194 || t == 3;
 }
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
@@ -96,7 +96,7 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
         case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
         case 145: case 146: case 152: case 186: case 190: case 86: case 126: case 127: case 128:
-        case 136:
+        case 136: case 3: case 5: case 77: case 78: case 79: case 188:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -121,6 +121,7 @@ struct SegRegs {
     float ip3;              // its state[0], set before IP3R runs
     float gq;               // the Gq receptors' state[0] summed, set before PLC runs
     float er;               // ER_Ca2 (segments with SERCA, IP3R or RyR)
+    float temp, txn;        // temperature, transcription signal (segments with 77, 78 or 79)
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -1852,6 +1853,128 @@ BC_DEV double ch_cav14(const ModSlot& m, SegRegs& s, float dt) {
     return -ica;
 }
 
+// Metabotropic G protein (3, hh_biological.cpp metabotropic_current): driven by the sum of all
+// 15 transmitters, effector state[1] = coupling x state[0], a 35 pS GIRK-like K+ GHK current.
+BC_DEV double ch_metabotropic(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    float total = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 15; n++) total += s.nt[n];
+    double coupling = (m.get(2) > 0.0f) ? m.get(2) : 1.0f;
+    double tau = (m.get(3) > 0.0f) ? m.get(3) : 100.0f;
+    float act = total / (total + 1.0f);
+    double s0 = m.get(0);
+    s0 += (act - s0) * dt / tau;
+    m.set(0, s0);
+    double s1 = coupling * s0;
+    m.set(1, s1);
+    double P = p_single(35.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, s.V, s.in[1], s.ex[1]);
+    double N = m.dens * area;
+    double ik = P * flux * N * s1 * kAtoPa;
+    s.in[1] -= (ik * dt) / s.cpm;
+    floor7(s.in);
+    return -ik;
+}
+// Synaptic release site (5, hh_biological.cpp synaptic_release_current): Ca2+ sensor (Hill 2,
+// K_d 0.5 uM) -> facilitation state[1] and depression state[2]; a 10 pS SK-like K+ GHK current
+// opens with facilitation / (1 + depression).
+BC_DEV double ch_release_site(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    float ca = s.in[2];
+    float sensor = (ca * ca) / (ca * ca + 0.0005f * 0.0005f);
+    double fac_tau = (m.get(4) > 0.0f) ? m.get(4) : 100.0f;
+    double dep_tau = (m.get(5) > 0.0f) ? m.get(5) : 500.0f;
+    double s0 = m.get(0), s1 = m.get(1), s2 = m.get(2);
+    s0 += (sensor - s0) * dt / 1.0f;
+    s1 += (s0 - s1) * dt / fac_tau;
+    s2 += (s0 - s2) * dt / dep_tau;
+    m.set(0, s0);
+    m.set(1, s1);
+    m.set(2, s2);
+    double open = s1 / (1.0f + s2);
+    double P = p_single(10.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, s.V, s.in[1], s.ex[1]);
+    double N = m.dens * area;
+    double ik = P * flux * N * open * kAtoPa;
+    s.in[1] -= (ik * dt) / s.cpm;
+    floor7(s.in);
+    return -ik;
+}
+// MRGPR (188, modifier_helpers.h mrgpr_signal): Gq activation state[0] by the ligand in state[4]
+// (EC50 28 uM; tau 70 ms up, 1500 ms down), desensitisation state[1] (tau 70 s); no current.
+BC_DEV double ch_mrgpr(const ModSlot& m, const SegRegs&, float dt) {
+    float lig = (m.get(4) > 0.0f) ? m.get(4) : 0.0f;
+    float occ = lig / (lig + 28.0f);
+    float act = m.get(0);
+    float tau = (occ > act) ? 70.0f : 1500.0f;
+    act += (occ - act) * dt / tau;
+    if (act < 0.0f) act = 0.0f;
+    if (act > 1.0f) act = 1.0f;
+    m.set(0, act);
+    float dz = m.get(1);
+    dz += (occ - dz) * dt / 70000.0f;
+    if (dz < 0.0f) dz = 0.0f;
+    if (dz > 1.0f) dz = 1.0f;
+    m.set(1, dz);
+    return 0.0f;
+}
+// Somatic ATP synthesis (77), thermoregulation (78) and CREB transcription (79)
+// (hh_biological.cpp nb_soma_atp_current, nb_soma_thermal_current, nb_creb_transcription_current):
+// ATP, temperature (s.temp) and the transcription signal (s.txn); no current.
+BC_DEV double ch_soma_atp(const ModSlot& m, SegRegs& s, float dt) {
+    double nadh = m.get(0);
+    float atp = s.atp;
+    float inh = 1.0f - atp / 5.0f;
+    if (inh < 0.0f) inh = 0.0f;
+    nadh += (0.001f - nadh * 0.01f) * dt;
+    if (nadh < 0.0f) nadh = 0.0f;
+    if (nadh > 1.0f) nadh = 1.0f;
+    double rate = (m.get(1) > 0.0f) ? m.get(1) : 0.01f;
+    s.atp += rate * nadh * inh * m.dens * dt;
+    if (s.atp > 5.0f) s.atp = 5.0f;
+    s.temp += 0.01f * nadh * inh * 0.5f * dt * 0.001f;
+    if (s.temp > 42.0f) s.temp = 42.0f;
+    m.set(0, nadh);
+    return 0.0f;
+}
+BC_DEV double ch_soma_thermal(const ModSlot& m, SegRegs& s, float dt) {
+    double stress = m.get(0);
+    float T = s.temp;
+    double tau = (m.get(1) > 0.0f) ? m.get(1) : 5000.0f;
+    T += (37.0f - T) * dt / tau;
+    if (T < 30.0f) T = 30.0f;
+    if (T > 42.0f) T = 42.0f;
+    if (T > 39.0f) stress += (T - 39.0f) * dt * 0.0001f;
+    else stress -= stress * dt * 0.001f;
+    if (stress < 0.0f) stress = 0.0f;
+    if (stress > 1.0f) stress = 1.0f;
+    s.temp = T;
+    m.set(0, stress);
+    return 0.0f;
+}
+BC_DEV double ch_creb(const ModSlot& m, SegRegs& s, float dt) {
+    float ca = s.in[2];
+    double p = m.get(0), late = m.get(1), st = m.get(2);
+    double kd = (m.get(3) > 0.0f) ? m.get(3) : 0.5f;
+    float can = ca * ca, kn = kd * kd;
+    float camk = can / (can + kn);
+    p += (camk * (1.0f - p) / 5000.0f - 0.0001f * p) * dt;
+    late += (p * (1.0f - late) / 30000.0f - late * 0.00001f) * dt;
+    st += (late * (1.0f - st) / 300000.0f - st * 0.000001f) * dt;
+    if (p < 0.0f) p = 0.0f;
+    if (p > 1.0f) p = 1.0f;
+    if (late < 0.0f) late = 0.0f;
+    if (late > 1.0f) late = 1.0f;
+    if (st < 0.0f) st = 0.0f;
+    if (st > 1.0f) st = 1.0f;
+    m.set(0, p);
+    m.set(1, late);
+    m.set(2, st);
+    s.txn = 0.5f * p + 0.3f * late + 0.2f * st;
+    return 0.0f;
+}
+
 // ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
 // segment's ER pool s.er (Segment.ER_Ca2), IP3R (127: Li-Rinzel gating, IP3 = s.ip3) and RyR
 // (128: Keizer-Levine gating with adaptation w) release it; no membrane current. Constants
@@ -2494,6 +2617,12 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 127: return ch_ip3r(m, s, dt);
         case 128: return ch_ryr(m, s, dt);
         case 136: return ch_plc(m, s, dt);
+        case 3: return ch_metabotropic(m, s, dt);
+        case 5: return ch_release_site(m, s, dt);
+        case 188: return ch_mrgpr(m, s, dt);
+        case 77: return ch_soma_atp(m, s, dt);
+        case 78: return ch_soma_thermal(m, s, dt);
+        case 79: return ch_creb(m, s, dt);
         case 146: return ch_cacc(m, s, dt);
         case 152: return ch_can(m, s, dt);
         case 186: return ch_aqp4(m, s, dt);

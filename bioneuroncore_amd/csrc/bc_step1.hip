@@ -36,9 +36,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const uint8_t sfl = d.sflags[i];
     if (kNt && (sfl & SF_NTREAD)) {   // receptors read transmitters 0-11, transporters 0, 1 and 7
 #pragma unroll
-        for (int k = 0; k < 12; k++) s.nt[k] = d.nt[k * P + i];
+        for (int k = 0; k < 15; k++) s.nt[k] = d.nt[k * P + i];
     }
     if (kExt && (sfl & SF_ER)) s.er = d.er[i];
+    if (kExt && (sfl & SF_TT)) {
+        s.temp = d.temp[i];
+        s.txn = d.txn[i];
+    }
     const bool my = sfl & SF_MYELIN;
     const uint32_t cnt = d.mcount[i];
     double itot = 0.0;
@@ -153,6 +157,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
         d.atp[i] = s.atp;
         if (kExt && (sfl & SF_ER)) d.er[i] = s.er;
+        if (kExt && (sfl & SF_TT)) {
+            d.temp[i] = s.temp;
+            d.txn[i] = s.txn;
+        }
     }
     d.ibuf[i] = itot;
 }

@@ -1846,6 +1846,131 @@ static double mod_ecb(Modifier* m, Segment* s, float dt) {
     return 0.0;
 }
 
+// Metabotropic (3, hh_biological.cpp metabotropic_current): a G protein driven by the sum of
+// all 15 transmitters (state[0]), an effector state[1] = coupling * state[0], and a GIRK-like
+// K+ GHK current (35 pS) through the effector.
+static double mod_metabotropic(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    float total = 0.0f;
+    for (int n = 0; n < 15; n++) total += s->neurotransmitter[n];
+    double coupling = (m->state[2] > 0.0f) ? m->state[2] : 1.0f;
+    double tau = (m->state[3] > 0.0f) ? m->state[3] : 100.0f;
+    float act = total / (total + 1.0f);
+    m->state[0] += (act - m->state[0]) * dt / tau;
+    m->state[1] = coupling * m->state[0];
+    double cpm = kFaraday * volume(s) / 1000.0;
+    double P = p_single(35.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s->V_m, s->intra_ions[1], s->extra_ions[1]);
+    double N = m->density * area;
+    double ik = P * flux * N * m->state[1] * kAtoPa;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    floor_ions(s->intra_ions);
+    return -ik;
+}
+// Synaptic release site (5, hh_biological.cpp synaptic_release_current): a Ca2+ sensor (Hill 2,
+// K_d 0.5 uM) feeding facilitation (state[1], tau state[4] or 100 ms) and depression (state[2],
+// tau state[5] or 500 ms); an SK-like K+ GHK current (10 pS) opens with facilitation / (1 +
+// depression).
+static double mod_release_site(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    float ca = s->intra_ions[2];
+    float sensor = (ca * ca) / (ca * ca + 0.0005f * 0.0005f);
+    double fac_tau = (m->state[4] > 0.0f) ? m->state[4] : 100.0f;
+    double dep_tau = (m->state[5] > 0.0f) ? m->state[5] : 500.0f;
+    m->state[0] += (sensor - m->state[0]) * dt / 1.0f;
+    m->state[1] += (m->state[0] - m->state[1]) * dt / fac_tau;
+    m->state[2] += (m->state[0] - m->state[2]) * dt / dep_tau;
+    double open = m->state[1] / (1.0f + m->state[2]);
+    double cpm = kFaraday * volume(s) / 1000.0;
+    double P = p_single(10.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s->V_m, s->intra_ions[1], s->extra_ions[1]);
+    double N = m->density * area;
+    double ik = P * flux * N * open * kAtoPa;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    floor_ions(s->intra_ions);
+    return -ik;
+}
+// MRGPR (188, modifier_helpers.h mrgpr_signal): a Gq receptor of the ligand level in state[4]
+// (EC50 28 uM): activation state[0] (tau 70 ms up, 1500 ms down), desensitisation state[1]
+// (tau 70 s); no current. PLC reads its state[0] with the other Gq receptors.
+static double mod_mrgpr(Modifier* m, Segment*, float dt) {
+    float lig = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    float occ = lig / (lig + 28.0f);
+    float act = m->state[0];
+    float tau = (occ > act) ? 70.0f : 1500.0f;
+    act += (occ - act) * dt / tau;
+    if (act < 0.0f) act = 0.0f;
+    if (act > 1.0f) act = 1.0f;
+    m->state[0] = act;
+    float dz = m->state[1];
+    dz += (occ - dz) * dt / 70000.0f;
+    if (dz < 0.0f) dz = 0.0f;
+    if (dz > 1.0f) dz = 1.0f;
+    m->state[1] = dz;
+    return 0.0f;
+}
+// Somatic ATP synthesis (77, hh_biological.cpp nb_soma_atp_current): NADH (state[0]) relaxing
+// to 0.1, ATP made at state[1] (0.01 if not positive) x NADH x (1 - ATP/5) x density up to 5,
+// warming the segment by a little (capped at 42 C); no current.
+static double mod_soma_atp(Modifier* m, Segment* s, float dt) {
+    double nadh = m->state[0];
+    float atp = s->ATP_level;
+    float inh = 1.0f - atp / 5.0f;
+    if (inh < 0.0f) inh = 0.0f;
+    nadh += (0.001f - nadh * 0.01f) * dt;
+    if (nadh < 0.0f) nadh = 0.0f;
+    if (nadh > 1.0f) nadh = 1.0f;
+    double rate = (m->state[1] > 0.0f) ? m->state[1] : 0.01f;
+    s->ATP_level += rate * nadh * inh * m->density * dt;
+    if (s->ATP_level > 5.0f) s->ATP_level = 5.0f;
+    s->temperature += 0.01f * nadh * inh * 0.5f * dt * 0.001f;
+    if (s->temperature > 42.0f) s->temperature = 42.0f;
+    m->state[0] = nadh;
+    return 0.0f;
+}
+// Somatic thermoregulation (78, hh_biological.cpp nb_soma_thermal_current): temperature relaxes
+// to 37 C with tau state[1] (5000 ms if not positive), clamped to [30, 42]; heat stress
+// (state[0]) builds above 39 C and decays below; no current.
+static double mod_soma_thermal(Modifier* m, Segment* s, float dt) {
+    double stress = m->state[0];
+    float T = s->temperature;
+    double tau = (m->state[1] > 0.0f) ? m->state[1] : 5000.0f;
+    T += (37.0f - T) * dt / tau;
+    if (T < 30.0f) T = 30.0f;
+    if (T > 42.0f) T = 42.0f;
+    if (T > 39.0f) stress += (T - 39.0f) * dt * 0.0001f;
+    else stress -= stress * dt * 0.001f;
+    if (stress < 0.0f) stress = 0.0f;
+    if (stress > 1.0f) stress = 1.0f;
+    s->temperature = T;
+    m->state[0] = stress;
+    return 0.0f;
+}
+// CREB transcription (79, hh_biological.cpp nb_creb_transcription_current): Ca2+ (Hill 2, K_d
+// state[3] or 0.5) drives pCREB (state[0]) -> late genes (state[1]) -> structural plasticity
+// (state[2]); the segment's transcription signal is their weighted sum; no current.
+static double mod_creb(Modifier* m, Segment* s, float dt) {
+    float ca = s->intra_ions[2];
+    double p = m->state[0], late = m->state[1], st = m->state[2];
+    double kd = (m->state[3] > 0.0f) ? m->state[3] : 0.5f;
+    float can = ca * ca, kn = kd * kd;
+    float camk = can / (can + kn);
+    p += (camk * (1.0f - p) / 5000.0f - 0.0001f * p) * dt;
+    late += (p * (1.0f - late) / 30000.0f - late * 0.00001f) * dt;
+    st += (late * (1.0f - st) / 300000.0f - st * 0.000001f) * dt;
+    if (p < 0.0f) p = 0.0f;
+    if (p > 1.0f) p = 1.0f;
+    if (late < 0.0f) late = 0.0f;
+    if (late > 1.0f) late = 1.0f;
+    if (st < 0.0f) st = 0.0f;
+    if (st > 1.0f) st = 1.0f;
+    m->state[0] = p;
+    m->state[1] = late;
+    m->state[2] = st;
+    s->transcription_signal = 0.5f * p + 0.3f * late + 0.2f * st;
+    return 0.0f;
+}
+
 // ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
 // segment's ER pool Segment.ER_Ca2, IP3R (127) and RyR (128) release it back; none of them
 // carries a membrane current. Constants from units.h:130-156.
@@ -2338,6 +2463,12 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_GLYT2: return chan_transporter(m, s, dt, kGlyt2);
         case T_GAT1: return chan_transporter(m, s, dt, kGat1);
         case T_ECB: return mod_ecb(m, s, dt);
+        case 3: return mod_metabotropic(m, s, dt);
+        case 5: return mod_release_site(m, s, dt);
+        case 188: return mod_mrgpr(m, s, dt);
+        case 77: return mod_soma_atp(m, s, dt);
+        case 78: return mod_soma_thermal(m, s, dt);
+        case 79: return mod_creb(m, s, dt);
         case 126: return mod_serca(m, s, dt);
         case 127: return mod_ip3r(m, s, dt);
         case 128: return mod_ryr(m, s, dt);

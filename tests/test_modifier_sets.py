@@ -7,6 +7,8 @@ compiled reference on the same networks, tests/test_oracle_vs_reference.py):
   persistent Na+ channels, Kir2.1 and the ClC-2 leak;
 - the ER Ca2+ store (SERCA, IP3R, RyR) and PLC (src/hh_biological.cpp:3473-3591,
   include/modifier_helpers.h:1553-1580);
+- the metabotropic G protein, the release site, somatic ATP synthesis and thermoregulation,
+  CREB and MRGPR (src/hh_biological.cpp, include/modifier_helpers.h mrgpr_signal);
 - a network that mixes them with the core channel set runs the extended step-1 kernel, and the
   core-only networks keep the lean one (results identical either way).
 """
@@ -72,6 +74,18 @@ def test_er_store_and_plc(seed):
     assert spk.sum() > 0
     er0 = net.segments["ER_Ca2"]
     assert np.any(seg["ER_Ca2"] > er0) and np.any(seg["ER_Ca2"] < er0)
+
+
+@pytest.mark.parametrize("seed", [71, 72])
+def test_signalling_modifiers(seed):
+    """The metabotropic G protein (3), the release site (5), somatic ATP synthesis (77) and
+    thermoregulation (78), CREB (79) and MRGPR (188) feeding PLC, 400 ticks: every byte of the
+    records, temperature and transcription signal included."""
+    net = bion.signalling_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    seg, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0
+    assert np.any(seg["transcription_signal"] > 0.0)
 
 
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])

@@ -139,6 +139,19 @@ def test_er_store_and_plc(seed):
     assert np.any(rs["modifiers"]["state"][plc][:, 1] > 0.0)   # DAG produced
 
 
+@pytest.mark.parametrize("seed", [71, 72])
+def test_signalling_modifiers(seed):
+    """The metabotropic G protein (3), the release site (5), somatic ATP synthesis (77) and
+    thermoregulation (78), CREB (79) and MRGPR (188) feeding PLC: every byte of the records,
+    temperature and transcription signal included."""
+    net = bion.signalling_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    rs, _, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
+    assert np.any(rs["transcription_signal"] > 0.0)
+    assert np.any(rs["temperature"] != net.segments["temperature"])
+
+
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
 def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2;
