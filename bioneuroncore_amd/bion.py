@@ -112,8 +112,10 @@ SERCA, IP3R, RYR, PLC = 126, 127, 128, 136   # the ER Ca2+ store and the Gq -> P
 ER_STORE = (SERCA, IP3R, RYR, PLC)
 METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR = 3, 5, 77, 78, 79, 188
 SIGNALLING = (METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR)
+TRPV1_R, MAPK, NFKB, MTORC1, PROTEIN_SYNTH, COFILIN, MMP9, ROD = 16, 87, 88, 89, 90, 91, 92, 192
+CASCADES = (TRPV1_R, MAPK, NFKB, MTORC1, PROTEIN_SYNTH, COFILIN, MMP9, ROD)
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING
+                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING + CASCADES
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -136,6 +138,8 @@ CANON_DENSITY.update({159: 1.0, 101: 0.005, 96: 2.0})
 CANON_DENSITY.update({84: 2.0, 144: 5.0, 145: 1.0, 146: 1.0, 152: 0.05, 186: 1e11, 190: 0.05})
 CANON_DENSITY.update({SERCA: 1e5, IP3R: 100.0, RYR: 1.0, PLC: 100.0})
 CANON_DENSITY.update({METABOTROPIC: 2.0, RELEASE_SITE: 1.0, SOMA_ATP: 1.0, SOMA_THERMAL: 1.0, CREB: 1.0, MRGPR: 1.0})
+CANON_DENSITY.update({TRPV1_R: 0.05, MAPK: 1.0, NFKB: 1.0, MTORC1: 1.0, PROTEIN_SYNTH: 1.0, COFILIN: 1.0, MMP9: 1.0,
+                      ROD: 0.05})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -721,6 +725,39 @@ def signalling_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) 
         for j, t in enumerate((METABOTROPIC, RELEASE_SITE)):
             mods["type"][i, k + j] = t
             mods["density"][i, k + j] = CANON_DENSITY[t] * rng.uniform(0.5, 2.0)
+        segs["modifier_count"][i] = k + 2
+    return net
+
+
+def cascade_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons whose somas carry Cav2.1, the MAPK/ERK, NF-kB, mTORC1, protein-synthesis,
+    cofilin and MMP-9 cascades with CREB (MAPK raises every CREB's pCREB; protein synthesis,
+    cofilin and MMP-9 read the last mTORC1, MAPK and NF-kB), thermoregulation, the TRPV1
+    transmitter receptor (heat and capsaicin, transmitter 14) and rod phototransduction, with
+    AMPA+NMDA / GABA-A synapses. Temperatures in 36-46 C, amino-acid inputs and light levels
+    vary per soma; every other dendrite segment carries MAPK and CREB."""
+    soma = [CAV21, MAPK, NFKB, MTORC1, PROTEIN_SYNTH, COFILIN, MMP9, CREB, SOMA_THERMAL, TRPV1_R, ROD]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], soma,
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    mods = segs["modifiers"]
+    rng = np.random.default_rng(seed + 9)
+    n = len(segs)
+    segs["temperature"] = rng.uniform(36.0, 46.0, n).astype(np.float32)
+    segs["intra_ions"][:, 2] *= rng.uniform(1.0, 50.0, n)
+    segs["neurotransmitter"][:, 14] = rng.uniform(0.0, 2.0, n).astype(np.float32)   # capsaicin
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    base = 5
+    mods["state"][soma_idx, base + 3, 3] = rng.uniform(0.0, 1.0, len(soma_idx))   # mTORC1 amino acids
+    mods["state"][soma_idx[::2], base + 10, 4] = rng.uniform(0.0, 1.0, len(soma_idx[::2]))   # rod light
+    mods["state"][soma_idx, base + 10, 3] = 0.8   # rod cGMP in the dark
+    mods["density"][soma_idx, base + 1:base + 7] *= rng.uniform(20.0, 200.0, (len(soma_idx), 6))   # faster cascades
+    mods["state"][soma_idx[1::3], base + 1, 4] = 0.2   # a MAPK K_d override
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        for j, t in enumerate((MAPK, CREB)):
+            mods["type"][i, k + j] = t
+            mods["density"][i, k + j] = 50.0 * rng.uniform(0.5, 2.0)
         segs["modifier_count"][i] = k + 2
     return net
 

@@ -497,7 +497,7 @@ static int build_device(Sim& S) {
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
             if (m.type == 126 || m.type == 127 || m.type == 128) fl |= SF_ER;
-            if (m.type == 77 || m.type == 78 || m.type == 79) fl |= SF_TT;
+            if (m.type == 77 || m.type == 78 || m.type == 79 || m.type == 16 || m.type == 88) fl |= SF_TT;
             if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
             if (!modifier_supported(m.type) || plug.mod[m.type]) S.unsupported_mod++;
             // G-protein receptors read state[active_vars + 1]: past the record's state array
@@ -659,7 +659,9 @@ static int build_device(Sim& S) {
             // the ER store and PLC as receptors would read the post's ER, IP3 and Gq pools
             if (rt == 126 || rt == 127 || rt == 128 || rt == 136) S.unsupported_mod++;
             // and somatic ATP, thermoregulation and CREB the post's temperature and transcription
-            if (rt == 77 || rt == 78 || rt == 79) S.unsupported_mod++;
+            if (rt == 77 || rt == 78 || rt == 79 || rt == 16 || rt == 88) S.unsupported_mod++;
+            // and the cascades that read or write sibling modifiers of the post segment
+            if (rt == 87 || rt == 90 || rt == 91 || rt == 92) S.unsupported_mod++;
             if (gprotein_type(rt) && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
             if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }

@@ -96,7 +96,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 191: case 210: case 162: case 211: case 175: case 182: case 172: case 173: case 174:
         case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
         case 145: case 146: case 152: case 186: case 190: case 86: case 126: case 127: case 128:
-        case 136: case 3: case 5: case 77: case 78: case 79: case 188:
+        case 136: case 3: case 5: case 77: case 78: case 79: case 188: case 16: case 87: case 88:
+        case 89: case 90: case 91: case 92: case 192:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -121,7 +122,9 @@ struct SegRegs {
     float ip3;              // its state[0], set before IP3R runs
     float gq;               // the Gq receptors' state[0] summed, set before PLC runs
     float er;               // ER_Ca2 (segments with SERCA, IP3R or RyR)
-    float temp, txn;        // temperature, transcription signal (segments with 77, 78 or 79)
+    float temp, txn;        // temperature, transcription signal (segments with 16, 77-79 or 88)
+    float xa, xb;           // the last mTORC1 / MAPK / NF-kB states protein synthesis, cofilin
+                            // and MMP-9 read, set before they run
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -1975,6 +1978,218 @@ BC_DEV double ch_creb(const ModSlot& m, SegRegs& s, float dt) {
     return 0.0f;
 }
 
+// TRPV1 as a transmitter receptor (16, hh_biological.cpp trpv1_current): heat (43 C, slope 0.5 C)
+// or capsaicin (transmitter 14) opens it (state[0], 200 ms; desensitisation state[1], 5 s); a
+// 90 pS Na+/K+/Ca2+ GHK current.
+BC_DEV double ch_trpv1r(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    float V = s.V;
+    float ft = 1.0f / (1.0f + bc::exp((43.0f - s.temp) / 0.5f));
+    float cap = s.nt[14];
+    if (cap < 0.0f) cap = 0.0f;
+    float fc = cap / (cap + 0.5f);
+    double r = m.get(0);
+    float pinf = ft + fc - ft * fc;
+    r += (pinf - r) * dt / 200.0f;
+    if (r < 0.0f) r = 0.0f;
+    if (r > 1.0f) r = 1.0f;
+    m.set(0, r);
+    double d = m.get(1);
+    d += (pinf - d) * dt / 5000.0f;
+    if (d < 0.0f) d = 0.0f;
+    if (d > 1.0f) d = 1.0f;
+    m.set(1, d);
+    float open = r * (1.0f - d);
+    const P3 pp = p_partial3(90.0f, 0.125f, 0.125f, 0.75f);
+    double N = m.dens * area;
+    double ina = pp.na * ghk_mono(s, V, s.in[0], s.ex[0]) * N * open * kAtoPa;
+    float ik = (float)(pp.k * ghk_mono(s, V, s.in[1], s.ex[1]) * N * open * kAtoPa);
+    double ica = pp.ca * ghk_di(s, V, s.in[2], s.ex[2]) * N * open * kAtoPa;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.in[1] -= (ik * dt) / s.cpm;
+    s.in[2] -= (ica * dt) / (2.0f * s.cpm);
+    floor7(s.in);
+    return -(ina + ik + ica);
+}
+// Rod phototransduction (192, hh_biological.cpp rod_phototransduction_current): light state[4]
+// -> R* -> transducin -> PDE -> cGMP (states 0-3); CNG current (Hill 3, K 0.3, 25 pS).
+BC_DEV double ch_rod(const ModSlot& m, SegRegs& s, float dt) {
+    float V = s.V;
+    double light = (m.get(4) > 0.0f) ? m.get(4) : 0.0f;
+    double s0 = m.get(0), s1 = m.get(1), s2 = m.get(2), s3 = m.get(3);
+    s0 += (light - s0) * dt / 1500.0f;
+    s1 += (s0 - s1) * dt / 10.0f;
+    s2 += (s1 - s2) * dt / 200.0f;
+    s3 += (1.0f - s3) * dt / 500.0f - s2 * s3 * dt / 50.0f;
+    double st[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (st[k] < 0) st[k] = 0;
+        if (st[k] > 1) st[k] = 1;
+    }
+    m.set(0, st[0]);
+    m.set(1, st[1]);
+    m.set(2, st[2]);
+    m.set(3, st[3]);
+    double cg = st[3];
+    float cg3 = cg * cg * cg;
+    float kh = 0.3f, kh3 = kh * kh * kh;
+    float po = cg3 / (kh3 + cg3);
+    float area = eff_area(m, s);
+    const P3 pp = p_partial3(25.0f, 0.5f, 0.3f, 0.2f);
+    double N = m.dens * area * po;
+    double ina = pp.na * ghk_mono(s, V, s.in[0], s.ex[0]) * N * kAtoPa;
+    float ik = (float)(pp.k * ghk_mono(s, V, s.in[1], s.ex[1]) * N * kAtoPa);
+    double ica = pp.ca * ghk_di(s, V, s.in[2], s.ex[2]) * N * kAtoPa;
+    s.in[0] -= (ina * dt) / s.cpm;
+    s.in[1] -= (ik * dt) / s.cpm;
+    s.in[2] -= (ica * dt) / (2.0f * s.cpm);
+    floor7(s.in);
+    return -(ina + ik + ica);
+}
+BC_DEV void clamp01_4(double* st) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (st[k] < 0.0f) st[k] = 0.0f;
+        if (st[k] > 1.0f) st[k] = 1.0f;
+    }
+}
+// The intracellular cascades (hh_biological.cpp mapk_erk_current, ikk_nfkb_current,
+// mtorc1_current, protein_synth_current, cofilin_current, mmp9_current): MAPK/ERK 87 (its raise
+// of the segment's CREB states is done by the step-1 kernel after it runs), NF-kB 88 (adds to
+// the transcription signal s.txn), mTORC1 89 (scaled by ATP), protein synthesis 90 (reads the
+// last mTORC1, s.xa; consumes ATP), cofilin 91 (reads the last MAPK's ERK, s.xa), MMP-9 92 (reads
+// the last MAPK's and NF-kB's nuclear states, s.xa and s.xb). Time is scaled by the density; no
+// current.
+BC_DEV double ch_mapk(const ModSlot& m, SegRegs& s, float dt) {
+    float ca = s.in[2] * 1e3f;
+    double kd = (m.get(4) > 0.0f) ? m.get(4) : 0.5f;
+    double tau_erk = (m.get(5) > 0.0f) ? m.get(5) : 5000.0f;
+    double tau_nuc = (m.get(6) > 0.0f) ? m.get(6) : 30000.0f;
+    dt *= m.dens;
+    float ca3 = ca * ca * ca;
+    float kd3 = kd * kd * kd;
+    float ras = ca3 / (ca3 + kd3 + 1e-9f);
+    double st[4] = {m.get(0), m.get(1), m.get(2), m.get(3)};
+    st[2] += (ras - st[2]) * dt / 500.0f;
+    st[1] += (st[2] - st[1]) * dt / 1000.0f;
+    double erk = st[1] * st[1];
+    st[0] += (erk - st[0]) * dt / tau_erk;
+    st[3] += (st[0] * 0.8f - st[3]) * dt / tau_nuc;
+    clamp01_4(st);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m.set(k, st[k]);
+    return 0.0f;
+}
+BC_DEV double ch_nfkb(const ModSlot& m, SegRegs& s, float dt) {
+    float ca = s.in[2] * 1e3f;
+    double kd = (m.get(4) > 0.0f) ? m.get(4) : 0.3f;
+    double tau_nfkb = (m.get(5) > 0.0f) ? m.get(5) : 10000.0f;
+    double tau_ikb = (m.get(6) > 0.0f) ? m.get(6) : 5000.0f;
+    dt *= m.dens;
+    float ikk = ca / (ca + kd + 1e-9f);
+    double st[4] = {m.get(0), m.get(1), m.get(2), m.get(3)};
+    st[1] += (ikk - st[1]) * dt / 2000.0f;
+    st[2] += (1.0f - st[1] * 0.9f - st[2]) * dt / tau_ikb;
+    if (st[2] < 0.1f) st[2] = 0.1f;
+    double fr = 1.0f - st[2];
+    st[0] += (fr - st[0]) * dt / tau_nfkb;
+    st[3] += (st[0] * 0.7f - st[3]) * dt / (tau_nfkb * 2.0f);
+    clamp01_4(st);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m.set(k, st[k]);
+    s.txn += st[3] * 0.1f;
+    if (s.txn > 1.0f) s.txn = 1.0f;
+    return 0.0f;
+}
+BC_DEV double ch_mtorc1(const ModSlot& m, SegRegs& s, float dt) {
+    double tau_mtor = (m.get(5) > 0.0f) ? m.get(5) : 15000.0f;
+    double tau_akt = (m.get(6) > 0.0f) ? m.get(6) : 8000.0f;
+    double kd = (m.get(4) > 0.0f) ? m.get(4) : 0.3f;
+    dt *= m.dens;
+    double aa = m.get(3);
+    float rheb = aa / (aa + kd + 1e-9f);
+    double s1 = m.get(1), s2 = m.get(2), s0 = m.get(0);
+    s1 += (rheb - s1) * dt / 5000.0f;
+    float akt = aa * 0.8f;
+    s2 += (akt - s2) * dt / tau_akt;
+    double inf = s1 * s2;
+    float an = s.atp / 5.0f;
+    if (an > 1.0f) an = 1.0f;
+    if (an < 0.0f) an = 0.0f;
+    inf *= an;
+    s0 += (inf - s0) * dt / tau_mtor;
+    if (s0 < 0.0f) s0 = 0.0f;
+    if (s0 > 1.0f) s0 = 1.0f;
+    if (s1 < 0.0f) s1 = 0.0f;
+    if (s2 < 0.0f) s2 = 0.0f;
+    m.set(0, s0);
+    m.set(1, s1);
+    m.set(2, s2);
+    return 0.0f;
+}
+BC_DEV double ch_protein_synth(const ModSlot& m, SegRegs& s, float dt) {
+    double tau_s6k = (m.get(4) > 0.0f) ? m.get(4) : 10000.0f;
+    double tau_4ebp = (m.get(5) > 0.0f) ? m.get(5) : 8000.0f;
+    double tau_tr = (m.get(6) > 0.0f) ? m.get(6) : 60000.0f;
+    dt *= m.dens;
+    float mt = s.xa;
+    double st[4] = {m.get(0), m.get(1), m.get(2), m.get(3)};
+    st[1] += (mt - st[1]) * dt / tau_s6k;
+    st[2] += (mt - st[2]) * dt / tau_4ebp;
+    double tr = st[1] * st[2];
+    st[0] += (tr - st[0]) * dt / tau_tr;
+    st[3] = st[0] * 0.9f;
+    clamp01_4(st);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m.set(k, st[k]);
+    s.atp -= st[0] * 0.0001f * dt;
+    if (s.atp < 0.0f) s.atp = 0.0f;
+    return 0.0f;
+}
+BC_DEV double ch_cofilin(const ModSlot& m, const SegRegs& s, float dt) {
+    double tau_cof = (m.get(4) > 0.0f) ? m.get(4) : 5000.0f;
+    double tau_limk = (m.get(5) > 0.0f) ? m.get(5) : 8000.0f;
+    double tau_actin = (m.get(6) > 0.0f) ? m.get(6) : 30000.0f;
+    dt *= m.dens;
+    float erk = s.xa;
+    double st[4] = {m.get(0), m.get(1), m.get(2), m.get(3)};
+    st[1] += (erk * 0.8f - st[1]) * dt / tau_limk;
+    double pc = st[1] * 0.9f;
+    st[2] += (pc - st[2]) * dt / tau_cof;
+    double ct = 1.0f - st[2];
+    st[0] += (ct - st[0]) * dt / tau_cof;
+    float poly = 0.3f;
+    double sev = st[0] * 0.1f;
+    st[3] += (poly - sev * st[3]) * dt / tau_actin;
+    clamp01_4(st);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m.set(k, st[k]);
+    return 0.0f;
+}
+BC_DEV double ch_mmp9(const ModSlot& m, const SegRegs& s, float dt) {
+    double tau_mmp = (m.get(4) > 0.0f) ? m.get(4) : 20000.0f;
+    double tau_timp = (m.get(5) > 0.0f) ? m.get(5) : 15000.0f;
+    double tau_ecm = (m.get(6) > 0.0f) ? m.get(6) : 100000.0f;
+    dt *= m.dens;
+    float tr = (s.xa + s.xb) * 0.5f;
+    double st[4] = {m.get(0), m.get(1), m.get(2), m.get(3)};
+    st[0] += (tr - st[0]) * dt / tau_mmp;
+    double timp = 1.0f - st[0] * 0.5f;
+    st[1] += (timp - st[1]) * dt / tau_timp;
+    double deg = st[0] * (1.0f - st[1]) * 0.01f;
+    float syn = 0.005f;
+    st[2] += (syn - deg * st[2]) * dt / tau_ecm;
+    if (st[2] < 0.1f) st[2] = 0.1f;
+    if (st[2] > 1.0f) st[2] = 1.0f;
+    double integ = 1.0f - st[2];
+    st[3] += (integ - st[3]) * dt / 10000.0f;
+    clamp01_4(st);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m.set(k, st[k]);
+    return 0.0f;
+}
+
 // ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
 // segment's ER pool s.er (Segment.ER_Ca2), IP3R (127: Li-Rinzel gating, IP3 = s.ip3) and RyR
 // (128: Keizer-Levine gating with adaptation w) release it; no membrane current. Constants
@@ -2623,6 +2838,14 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 77: return ch_soma_atp(m, s, dt);
         case 78: return ch_soma_thermal(m, s, dt);
         case 79: return ch_creb(m, s, dt);
+        case 16: return ch_trpv1r(m, s, dt);
+        case 192: return ch_rod(m, s, dt);
+        case 87: return ch_mapk(m, s, dt);
+        case 88: return ch_nfkb(m, s, dt);
+        case 89: return ch_mtorc1(m, s, dt);
+        case 90: return ch_protein_synth(m, s, dt);
+        case 91: return ch_cofilin(m, s, dt);
+        case 92: return ch_mmp9(m, s, dt);
         case 146: return ch_cacc(m, s, dt);
         case 152: return ch_can(m, s, dt);
         case 186: return ch_aqp4(m, s, dt);

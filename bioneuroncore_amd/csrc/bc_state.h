@@ -30,8 +30,9 @@ enum : uint8_t {
     SF_NTREAD = 16,       // a modifier reads (receptors) or takes up (transporters) transmitters
     SF_NTX = 32,          // an incoming synapse carries a cleft transmitter other than glutamate/GABA
     SF_ER = 64,           // carries SERCA, IP3R or RyR: step 1 reads and writes its ER Ca2+ store
-    SF_TT = 128,          // carries somatic ATP synthesis, thermoregulation or CREB: step 1 reads
-                          // and writes its temperature and transcription signal
+    SF_TT = 128,          // carries the TRPV1 receptor, somatic ATP synthesis, thermoregulation,
+                          // CREB or NF-kB: step 1 reads and writes its temperature and
+                          // transcription signal
 };
 
 // Synapse flag bits (static).

@@ -79,6 +79,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
             s.gq = gq;
         }
+        if (kExt && (t == 90 || t == 91 || t == 92)) {   // the last mTORC1 / MAPK / NF-kB state as it stands now
+            float xa = 0.0f, xb = 0.0f;
+            for (uint32_t q = 0; q < cnt; q++) {
+                const uint8_t tq = d.mtype[q * P + i];
+                if (t == 90 && tq == 89) xa = d.mstate[(size_t)q * 7 * P + i];
+                if (t == 91 && tq == 87) xa = d.mstate[(size_t)q * 7 * P + i];
+                if (t == 92 && tq == 87) xa = d.mstate[((size_t)q * 7 + 3) * P + i];
+                if (t == 92 && tq == 88) xb = d.mstate[((size_t)q * 7 + 3) * P + i];
+            }
+            s.xa = xa;
+            s.xb = xb;
+        }
         if (kExt && t == 86) {   // eCB synthesis reads the first PLC/IP3/DAG modifier's state[1]
             s.dag = 0.0f;
             for (uint32_t q = 0; q < cnt; q++)
@@ -88,6 +100,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
         }
         itot += dispatch<kNt, kExt>(t, m, s, dt);
+        if (kExt && t == 87) {   // nuclear ERK raises every CREB's pCREB on the segment (hh_biological.cpp mapk_erk_current)
+            float dts = dt;
+            dts *= m.dens;
+            const double nuc = m.get(3);
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 79) {
+                    double* c = d.mstate + (size_t)q * 7 * P + i;
+                    double v = *c + nuc * 0.01f * dts / 1000.0f;
+                    if (v > 1.0f) v = 1.0f;
+                    *c = v;
+                }
+        }
         if (d.cable) {   // concentrations held: every channel sees the stored ones
 #pragma unroll
             for (int k = 0; k < 7; k++) {

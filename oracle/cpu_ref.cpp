@@ -1971,6 +1971,219 @@ static double mod_creb(Modifier* m, Segment* s, float dt) {
     return 0.0f;
 }
 
+// The last instance's state[k] of a type on the segment, as the reference's loops without a
+// break read it (0 if none).
+static float last_state_of(const Segment* s, uint8_t type, int k) {
+    float v = 0.0f;
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == type) v = s->modifiers[j].state[k];
+    return v;
+}
+// TRPV1 as a transmitter receptor (16, hh_biological.cpp trpv1_current): heat (Boltzmann around
+// 43 C, slope 0.5 C) or capsaicin (transmitter 14, EC50 0.5) opens it with slow activation
+// (state[0], 200 ms) and desensitisation (state[1], 5 s); a 90 pS Na+/K+/Ca2+ GHK current.
+static double mod_trpv1_receptor(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    float V = s->V_m;
+    double cpm = kFaraday * volume(s) / 1000.0;
+    float ft = 1.0f / (1.0f + exp((43.0f - s->temperature) / 0.5f));
+    float cap = s->neurotransmitter[14];
+    if (cap < 0.0f) cap = 0.0f;
+    float fc = cap / (cap + 0.5f);
+    double r = m->state[0];
+    float pinf = ft + fc - ft * fc;
+    r += (pinf - r) * dt / 200.0f;
+    if (r < 0.0f) r = 0.0f;
+    if (r > 1.0f) r = 1.0f;
+    m->state[0] = r;
+    double d = m->state[1];
+    d += (pinf - d) * dt / 5000.0f;
+    if (d < 0.0f) d = 0.0f;
+    if (d > 1.0f) d = 1.0f;
+    m->state[1] = d;
+    float open = r * (1.0f - d);
+    double pna, pk, pca;
+    p_partial(90.0f, 0.125f, 0.125f, 0.75f, &pna, &pk, &pca);
+    double N = m->density * area;
+    double ina = pna * ghk_mono(V, s->intra_ions[0], s->extra_ions[0]) * N * open * kAtoPa;
+    float ik = (float)(pk * ghk_mono(V, s->intra_ions[1], s->extra_ions[1]) * N * open * kAtoPa);
+    double ica = pca * ghk_di(V, s->intra_ions[2], s->extra_ions[2]) * N * open * kAtoPa;
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    floor_ions(s->intra_ions);
+    return -(ina + ik + ica);
+}
+// Rod phototransduction (192, hh_biological.cpp rod_phototransduction_current): light (state[4])
+// -> R* (state[0]) -> transducin (state[1]) -> PDE (state[2]) hydrolysing cGMP (state[3]); CNG
+// channels (Hill 3 in cGMP, K 0.3; 25 pS Na+/K+/Ca2+) carry the dark current.
+static double mod_rod(Modifier* m, Segment* s, float dt) {
+    float V = s->V_m;
+    double light = (m->state[4] > 0.0f) ? m->state[4] : 0.0f;
+    m->state[0] += (light - m->state[0]) * dt / 1500.0f;
+    m->state[1] += (m->state[0] - m->state[1]) * dt / 10.0f;
+    m->state[2] += (m->state[1] - m->state[2]) * dt / 200.0f;
+    m->state[3] += (1.0f - m->state[3]) * dt / 500.0f - m->state[2] * m->state[3] * dt / 50.0f;
+    for (int k = 0; k < 4; k++) {
+        if (m->state[k] < 0) m->state[k] = 0;
+        if (m->state[k] > 1) m->state[k] = 1;
+    }
+    double cg = m->state[3];
+    float cg3 = cg * cg * cg;
+    float kh = 0.3f, kh3 = kh * kh * kh;
+    float po = cg3 / (kh3 + cg3);
+    float area = eff_area(m, s);
+    double cpm = kFaraday * volume(s) / 1000.0;
+    double pna, pk, pca;
+    p_partial(25.0f, 0.5f, 0.3f, 0.2f, &pna, &pk, &pca);
+    double N = m->density * area * po;
+    double ina = pna * ghk_mono(V, s->intra_ions[0], s->extra_ions[0]) * N * kAtoPa;
+    float ik = (float)(pk * ghk_mono(V, s->intra_ions[1], s->extra_ions[1]) * N * kAtoPa);
+    double ica = pca * ghk_di(V, s->intra_ions[2], s->extra_ions[2]) * N * kAtoPa;
+    s->intra_ions[0] -= (ina * dt) / cpm;
+    s->intra_ions[1] -= (ik * dt) / cpm;
+    s->intra_ions[2] -= (ica * dt) / (2.0f * cpm);
+    floor_ions(s->intra_ions);
+    return -(ina + ik + ica);
+}
+static inline void clamp01_4(double* st) {
+    for (int k = 0; k < 4; k++) {
+        if (st[k] < 0.0f) st[k] = 0.0f;
+        if (st[k] > 1.0f) st[k] = 1.0f;
+    }
+}
+// MAPK/ERK (87, hh_biological.cpp mapk_erk_current): Ca2+ (uM, Hill 3, K_d state[4] or 0.5) ->
+// Ras (state[2]) -> MEK (state[1]) -> ERK (state[0], squared MEK) -> nuclear ERK (state[3]);
+// time scaled by the density; nuclear ERK raises every CREB's pCREB (state[0]) on the segment.
+static double mod_mapk(Modifier* m, Segment* s, float dt) {
+    float ca = s->intra_ions[2] * 1e3f;
+    double kd = (m->state[4] > 0.0f) ? m->state[4] : 0.5f;
+    double tau_erk = (m->state[5] > 0.0f) ? m->state[5] : 5000.0f;
+    double tau_nuc = (m->state[6] > 0.0f) ? m->state[6] : 30000.0f;
+    dt *= m->density;
+    float ca3 = ca * ca * ca;
+    float kd3 = kd * kd * kd;
+    float ras = ca3 / (ca3 + kd3 + 1e-9f);
+    m->state[2] += (ras - m->state[2]) * dt / 500.0f;
+    m->state[1] += (m->state[2] - m->state[1]) * dt / 1000.0f;
+    double erk = m->state[1] * m->state[1];
+    m->state[0] += (erk - m->state[0]) * dt / tau_erk;
+    m->state[3] += (m->state[0] * 0.8f - m->state[3]) * dt / tau_nuc;
+    clamp01_4(m->state);
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == 79) {
+            s->modifiers[j].state[0] += m->state[3] * 0.01f * dt / 1000.0f;
+            if (s->modifiers[j].state[0] > 1.0f) s->modifiers[j].state[0] = 1.0f;
+        }
+    return 0.0f;
+}
+// IKK/NF-kB (88, hh_biological.cpp ikk_nfkb_current): Ca2+ -> IKK (state[1]) -> IkB (state[2],
+// floor 0.1) -> free NF-kB (state[0]) -> nuclear NF-kB (state[3]), which adds to the segment's
+// transcription signal (capped at 1); time scaled by the density.
+static double mod_nfkb(Modifier* m, Segment* s, float dt) {
+    float ca = s->intra_ions[2] * 1e3f;
+    double kd = (m->state[4] > 0.0f) ? m->state[4] : 0.3f;
+    double tau_nfkb = (m->state[5] > 0.0f) ? m->state[5] : 10000.0f;
+    double tau_ikb = (m->state[6] > 0.0f) ? m->state[6] : 5000.0f;
+    dt *= m->density;
+    float ikk = ca / (ca + kd + 1e-9f);
+    m->state[1] += (ikk - m->state[1]) * dt / 2000.0f;
+    m->state[2] += (1.0f - m->state[1] * 0.9f - m->state[2]) * dt / tau_ikb;
+    if (m->state[2] < 0.1f) m->state[2] = 0.1f;
+    double free_ = 1.0f - m->state[2];
+    m->state[0] += (free_ - m->state[0]) * dt / tau_nfkb;
+    m->state[3] += (m->state[0] * 0.7f - m->state[3]) * dt / (tau_nfkb * 2.0f);
+    clamp01_4(m->state);
+    s->transcription_signal += m->state[3] * 0.1f;
+    if (s->transcription_signal > 1.0f) s->transcription_signal = 1.0f;
+    return 0.0f;
+}
+// mTORC1 (89, hh_biological.cpp mtorc1_current): amino acids (state[3]) -> Rheb (state[1]) and
+// AKT (state[2]); mTORC1 (state[0]) follows their product scaled by ATP/5; time scaled by the
+// density.
+static double mod_mtorc1(Modifier* m, Segment* s, float dt) {
+    double tau_mtor = (m->state[5] > 0.0f) ? m->state[5] : 15000.0f;
+    double tau_akt = (m->state[6] > 0.0f) ? m->state[6] : 8000.0f;
+    double kd = (m->state[4] > 0.0f) ? m->state[4] : 0.3f;
+    dt *= m->density;
+    double aa = m->state[3];
+    float rheb = aa / (aa + kd + 1e-9f);
+    m->state[1] += (rheb - m->state[1]) * dt / 5000.0f;
+    float akt = aa * 0.8f;
+    m->state[2] += (akt - m->state[2]) * dt / tau_akt;
+    double inf = m->state[1] * m->state[2];
+    float an = s->ATP_level / 5.0f;
+    if (an > 1.0f) an = 1.0f;
+    if (an < 0.0f) an = 0.0f;
+    inf *= an;
+    m->state[0] += (inf - m->state[0]) * dt / tau_mtor;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[2] < 0.0f) m->state[2] = 0.0f;
+    return 0.0f;
+}
+// Protein synthesis (90, hh_biological.cpp protein_synth_current): the last mTORC1's state[0]
+// drives S6K (state[1]) and 4E-BP (state[2]); translation (state[0]) consumes ATP.
+static double mod_protein_synth(Modifier* m, Segment* s, float dt) {
+    double tau_s6k = (m->state[4] > 0.0f) ? m->state[4] : 10000.0f;
+    double tau_4ebp = (m->state[5] > 0.0f) ? m->state[5] : 8000.0f;
+    double tau_tr = (m->state[6] > 0.0f) ? m->state[6] : 60000.0f;
+    dt *= m->density;
+    float mt = last_state_of(s, 89, 0);
+    m->state[1] += (mt - m->state[1]) * dt / tau_s6k;
+    m->state[2] += (mt - m->state[2]) * dt / tau_4ebp;
+    double tr = m->state[1] * m->state[2];
+    m->state[0] += (tr - m->state[0]) * dt / tau_tr;
+    m->state[3] = m->state[0] * 0.9f;
+    clamp01_4(m->state);
+    s->ATP_level -= m->state[0] * 0.0001f * dt;
+    if (s->ATP_level < 0.0f) s->ATP_level = 0.0f;
+    return 0.0f;
+}
+// Cofilin (91, hh_biological.cpp cofilin_current): the last MAPK's ERK (state[0]) -> LIMK
+// (state[1]) -> phospho-cofilin (state[2]); active cofilin (state[0]) severs F-actin (state[3]).
+static double mod_cofilin(Modifier* m, Segment* s, float dt) {
+    double tau_cof = (m->state[4] > 0.0f) ? m->state[4] : 5000.0f;
+    double tau_limk = (m->state[5] > 0.0f) ? m->state[5] : 8000.0f;
+    double tau_actin = (m->state[6] > 0.0f) ? m->state[6] : 30000.0f;
+    dt *= m->density;
+    float erk = last_state_of(s, 87, 0);
+    m->state[1] += (erk * 0.8f - m->state[1]) * dt / tau_limk;
+    double pc = m->state[1] * 0.9f;
+    m->state[2] += (pc - m->state[2]) * dt / tau_cof;
+    double ct = 1.0f - m->state[2];
+    m->state[0] += (ct - m->state[0]) * dt / tau_cof;
+    float poly = 0.3f;
+    double sev = m->state[0] * 0.1f;
+    m->state[3] += (poly - sev * m->state[3]) * dt / tau_actin;
+    clamp01_4(m->state);
+    return 0.0f;
+}
+// MMP-9 (92, hh_biological.cpp mmp9_current): nuclear ERK and NF-kB (the last MAPK's and IKK's
+// state[3]) drive MMP-9 (state[0]) against TIMP (state[1]); extracellular matrix (state[2],
+// floor 0.1) and integrin activation (state[3]) follow.
+static double mod_mmp9(Modifier* m, Segment* s, float dt) {
+    double tau_mmp = (m->state[4] > 0.0f) ? m->state[4] : 20000.0f;
+    double tau_timp = (m->state[5] > 0.0f) ? m->state[5] : 15000.0f;
+    double tau_ecm = (m->state[6] > 0.0f) ? m->state[6] : 100000.0f;
+    dt *= m->density;
+    float erk = last_state_of(s, 87, 3), nfkb = last_state_of(s, 88, 3);
+    float tr = (erk + nfkb) * 0.5f;
+    m->state[0] += (tr - m->state[0]) * dt / tau_mmp;
+    double timp = 1.0f - m->state[0] * 0.5f;
+    m->state[1] += (timp - m->state[1]) * dt / tau_timp;
+    double deg = m->state[0] * (1.0f - m->state[1]) * 0.01f;
+    float syn = 0.005f;
+    m->state[2] += (syn - deg * m->state[2]) * dt / tau_ecm;
+    if (m->state[2] < 0.1f) m->state[2] = 0.1f;
+    if (m->state[2] > 1.0f) m->state[2] = 1.0f;
+    double integ = 1.0f - m->state[2];
+    m->state[3] += (integ - m->state[3]) * dt / 10000.0f;
+    clamp01_4(m->state);
+    return 0.0f;
+}
+
 // ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
 // segment's ER pool Segment.ER_Ca2, IP3R (127) and RyR (128) release it back; none of them
 // carries a membrane current. Constants from units.h:130-156.
@@ -2463,6 +2676,14 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case T_GLYT2: return chan_transporter(m, s, dt, kGlyt2);
         case T_GAT1: return chan_transporter(m, s, dt, kGat1);
         case T_ECB: return mod_ecb(m, s, dt);
+        case 16: return mod_trpv1_receptor(m, s, dt);
+        case 192: return mod_rod(m, s, dt);
+        case 87: return mod_mapk(m, s, dt);
+        case 88: return mod_nfkb(m, s, dt);
+        case 89: return mod_mtorc1(m, s, dt);
+        case 90: return mod_protein_synth(m, s, dt);
+        case 91: return mod_cofilin(m, s, dt);
+        case 92: return mod_mmp9(m, s, dt);
         case 3: return mod_metabotropic(m, s, dt);
         case 5: return mod_release_site(m, s, dt);
         case 188: return mod_mrgpr(m, s, dt);

@@ -9,6 +9,8 @@ compiled reference on the same networks, tests/test_oracle_vs_reference.py):
   include/modifier_helpers.h:1553-1580);
 - the metabotropic G protein, the release site, somatic ATP synthesis and thermoregulation,
   CREB and MRGPR (src/hh_biological.cpp, include/modifier_helpers.h mrgpr_signal);
+- the MAPK/ERK, NF-kB, mTORC1, protein-synthesis, cofilin and MMP-9 cascades, the TRPV1
+  transmitter receptor and rod phototransduction (src/hh_biological.cpp);
 - a network that mixes them with the core channel set runs the extended step-1 kernel, and the
   core-only networks keep the lean one (results identical either way).
 """
@@ -86,6 +88,16 @@ def test_signalling_modifiers(seed):
     seg, spk = _compare(net, 400, cur)
     assert spk.sum() > 0
     assert np.any(seg["transcription_signal"] > 0.0)
+
+
+@pytest.mark.parametrize("seed", [81, 82])
+def test_cascade_modifiers(seed):
+    """MAPK/ERK (raising CREB), NF-kB, mTORC1, protein synthesis, cofilin and MMP-9, the TRPV1
+    transmitter receptor and rod phototransduction, 400 ticks: every byte of the records."""
+    net = bion.cascade_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    seg, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0
 
 
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])

@@ -152,6 +152,21 @@ def test_signalling_modifiers(seed):
     assert np.any(rs["temperature"] != net.segments["temperature"])
 
 
+@pytest.mark.parametrize("seed", [81, 82])
+def test_cascade_modifiers(seed):
+    """MAPK/ERK (raising CREB), NF-kB (adding to the transcription signal), mTORC1, protein
+    synthesis, cofilin and MMP-9 (reading the last mTORC1, MAPK and NF-kB), the TRPV1 transmitter
+    receptor and rod phototransduction: every byte of the records."""
+    net = bion.cascade_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    rs, _, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
+    st = rs["modifiers"]["state"]
+    ty = rs["modifiers"]["type"]
+    for t in (bion.MAPK, bion.NFKB, bion.MTORC1, bion.PROTEIN_SYNTH, bion.COFILIN, bion.MMP9):
+        assert np.any(st[ty == t][:, 0] != 0.0), t
+
+
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
 def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2;
