@@ -114,8 +114,11 @@ METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR = 3, 5, 77, 78, 
 SIGNALLING = (METABOTROPIC, RELEASE_SITE, SOMA_ATP, SOMA_THERMAL, CREB, MRGPR)
 TRPV1_R, MAPK, NFKB, MTORC1, PROTEIN_SYNTH, COFILIN, MMP9, ROD = 16, 87, 88, 89, 90, 91, 92, 192
 CASCADES = (TRPV1_R, MAPK, NFKB, MTORC1, PROTEIN_SYNTH, COFILIN, MMP9, ROD)
+PHYS_SENSOR, HUNGER, LIGHT, OLFACTORY, THERMAL_NOISE = 6, 94, 95, 108, 111
+VIBRATION, OTOLITH, CUPULA, CRYPTOCHROME = 115, 116, 117, 120
+SENSORS = (PHYS_SENSOR, HUNGER, LIGHT, OLFACTORY, THERMAL_NOISE, VIBRATION, OTOLITH, CUPULA, CRYPTOCHROME)
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING + CASCADES
+                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING + CASCADES + SENSORS
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -140,6 +143,8 @@ CANON_DENSITY.update({SERCA: 1e5, IP3R: 100.0, RYR: 1.0, PLC: 100.0})
 CANON_DENSITY.update({METABOTROPIC: 2.0, RELEASE_SITE: 1.0, SOMA_ATP: 1.0, SOMA_THERMAL: 1.0, CREB: 1.0, MRGPR: 1.0})
 CANON_DENSITY.update({TRPV1_R: 0.05, MAPK: 1.0, NFKB: 1.0, MTORC1: 1.0, PROTEIN_SYNTH: 1.0, COFILIN: 1.0, MMP9: 1.0,
                       ROD: 0.05})
+CANON_DENSITY.update({t: 1.0 for t in SENSORS})
+CANON_DENSITY[PHYS_SENSOR] = 0.8   # cpg_config.h:85
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -758,6 +763,56 @@ def cascade_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> 
         for j, t in enumerate((MAPK, CREB)):
             mods["type"][i, k + j] = t
             mods["density"][i, k + j] = 50.0 * rng.uniform(0.5, 2.0)
+        segs["modifier_count"][i] = k + 2
+    return net
+
+
+def sensor_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons whose somas carry Cav2.1 and every sensor: physical (temperature and stretch),
+    hunger (ATP), light, olfactory, thermal noise (seeded by Segment.id and V), vibration, otolith
+    hair cell, semicircular cupula and cryptochrome (which boosts the first light sensor, here
+    before it in slot order), with AMPA+NMDA / GABA-A synapses; every other dendrite segment
+    carries a cryptochrome ahead of a light sensor. Temperatures in 30-41 C, lengths in 20-120 um,
+    ATP in 0.2-4; stimulus states (light, ligand, vibration, shear and its direction, angular
+    acceleration, sin^2 of the field angle) vary per segment."""
+    soma = [CAV21, PHYS_SENSOR, HUNGER, LIGHT, OLFACTORY, THERMAL_NOISE, VIBRATION, OTOLITH, CUPULA,
+            CRYPTOCHROME]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], soma,
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    segs = net.segments
+    mods = segs["modifiers"]
+    rng = np.random.default_rng(seed + 11)
+    n = len(segs)
+    segs["temperature"] = rng.uniform(30.0, 41.0, n).astype(np.float32)
+    segs["length"] = rng.uniform(20.0, 120.0, n).astype(np.float32)
+    segs["ATP_level"] = rng.uniform(0.2, 4.0, n).astype(np.float32)
+    segs["id"] = rng.permutation(n).astype(np.uint32) * 3 + 17
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    ns = len(soma_idx)
+    base = 5
+    st = mods["state"]
+    st[soma_idx[::2], base + 1, 1] = rng.uniform(0.5, 2.0, len(soma_idx[::2]))   # physical sensitivity
+    st[soma_idx[1::2], base + 1, 6] = 500.0                                       # faster habituation
+    st[soma_idx, base + 3, 1] = rng.uniform(0.0, 0.5, ns)                         # light intensity
+    st[soma_idx, base + 4, 1] = rng.uniform(0.0, 0.03, ns)                        # odorant
+    st[soma_idx[::2], base + 5, 0] = rng.uniform(0.02, 0.2, len(soma_idx[::2]))   # noise sigma
+    st[soma_idx, base + 5, 1] = rng.uniform(0.0, 50.0, ns)                        # noise seed offset
+    st[soma_idx, base + 6, 1] = rng.uniform(0.0, 1.0, ns)                         # vibration
+    st[soma_idx, base + 7, 1] = rng.uniform(0.0, 1.0, ns)                         # shear
+    st[soma_idx, base + 7, 5] = rng.uniform(-1.0, 1.0, ns)                        # shear direction
+    st[soma_idx[::3], base + 7, 5] = -100.0                                       # unset: along it
+    st[soma_idx, base + 8, 1] = rng.uniform(-1.0, 1.0, ns)                        # angular acceleration
+    st[soma_idx[1::3], base + 8, 4] = 2.0                                         # stiffer cupula
+    st[soma_idx, base + 9, 1] = rng.uniform(0.0, 1.0, ns)                         # light at the cryptochrome
+    st[soma_idx, base + 9, 2] = rng.uniform(0.0, 1.0, ns)                         # sin^2 of the field angle
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        for j, t in enumerate((CRYPTOCHROME, LIGHT)):
+            mods["type"][i, k + j] = t
+            mods["density"][i, k + j] = CANON_DENSITY[t] * rng.uniform(0.5, 2.0)
+        st[i, k, 1:3] = rng.uniform(0.2, 1.0, 2)
+        st[i, k + 1, 0] = rng.uniform(0.3, 0.9)
+        st[i, k + 1, 1] = rng.uniform(0.0, 0.3)
         segs["modifier_count"][i] = k + 2
     return net
 

@@ -464,6 +464,7 @@ static int build_device(Sim& S) {
         camkii01(npad, 0);
     std::vector<double> vol(npad, 0);
     std::vector<uint8_t> stype(npad, 0), sflags(npad, 0), mcount(npad, 0);
+    std::vector<uint32_t> segid(npad, 0);
     std::vector<int8_t> cav21(npad, -1);
     std::vector<uint8_t> mtype((size_t)nslot * npad, 0), mav((size_t)nslot * npad, 0);
     std::vector<double> mdens((size_t)nslot * npad, 0.0);
@@ -473,7 +474,7 @@ static int build_device(Sim& S) {
     S.ext_mods = false;
     for (uint32_t di = 0; di < n; di++) {
         const Segment& s = sg[S.dev2host[di]];
-        radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type;
+        radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type; segid[di] = s.id;
         area[di] = seg_area(s); vol[di] = seg_vol(s); cap[di] = seg_cap(s);
         mcount[di] = s.modifier_count;
         bool seen_pkc = false, seen_pka = false;
@@ -497,7 +498,8 @@ static int build_device(Sim& S) {
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
             if (m.type == 126 || m.type == 127 || m.type == 128) fl |= SF_ER;
-            if (m.type == 77 || m.type == 78 || m.type == 79 || m.type == 16 || m.type == 88) fl |= SF_TT;
+            if (m.type == 77 || m.type == 78 || m.type == 79 || m.type == 16 || m.type == 88 || m.type == 6)
+                fl |= SF_TT;
             if (!modifier_core(m.type) && modifier_supported(m.type)) S.ext_mods = true;
             if (!modifier_supported(m.type) || plug.mod[m.type]) S.unsupported_mod++;
             // G-protein receptors read state[active_vars + 1]: past the record's state array
@@ -662,6 +664,9 @@ static int build_device(Sim& S) {
             if (rt == 77 || rt == 78 || rt == 79 || rt == 16 || rt == 88) S.unsupported_mod++;
             // and the cascades that read or write sibling modifiers of the post segment
             if (rt == 87 || rt == 90 || rt == 91 || rt == 92) S.unsupported_mod++;
+            // and the sensors, which read the post's ATP, temperature, length and id or its light sensor
+            if (rt == 6 || rt == 94 || rt == 95 || rt == 108 || rt == 111 || (rt >= 115 && rt <= 117) || rt == 120)
+                S.unsupported_mod++;
             if (gprotein_type(rt) && y.post_receptors[j].active_vars >= 7) S.unsupported_mod++;
             if (rt != M_AMPA && rt != M_NMDA && rt != M_GABAA) S.receptors_only = false;
         }
@@ -736,6 +741,7 @@ static int build_device(Sim& S) {
     d.spikes = S.dalloc<uint32_t>(npad);
     d.radius = S.dupload(radius);
     d.length = S.dupload(length);
+    d.segid = S.dupload(segid);
     d.stype = S.dupload(stype);
     d.area = S.dupload(area);
     d.vol = S.dupload(vol);

@@ -97,7 +97,8 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 179: case 163: case 164: case 180: case 159: case 101: case 96: case 84: case 144:
         case 145: case 146: case 152: case 186: case 190: case 86: case 126: case 127: case 128:
         case 136: case 3: case 5: case 77: case 78: case 79: case 188: case 16: case 87: case 88:
-        case 89: case 90: case 91: case 92: case 192:
+        case 89: case 90: case 91: case 92: case 192: case 6: case 94: case 95: case 108: case 111:
+        case 115: case 116: case 117: case 120:
             return true;
         default:
             return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
@@ -125,6 +126,8 @@ struct SegRegs {
     float temp, txn;        // temperature, transcription signal (segments with 16, 77-79 or 88)
     float xa, xb;           // the last mTORC1 / MAPK / NF-kB states protein synthesis, cofilin
                             // and MMP-9 read, set before they run
+    float len;              // segment length, set before a physical sensor runs
+    uint32_t sid;           // Segment.id of the record, set before thermal noise runs
     float nt[15];  // neurotransmitter (used by receptors)
     // exp(-zFV/RT) of the GHK helpers for the two V values the channels pass (V rounded to
     // float, and V as double): V is fixed while one segment's modifiers or receptors run, so
@@ -2190,6 +2193,251 @@ BC_DEV double ch_mmp9(const ModSlot& m, const SegRegs& s, float dt) {
     return 0.0f;
 }
 
+// Sensors (hh_biological.cpp:40-52, 483-540, 2181-2440, 3122-3141). Adaptation states relax
+// toward a float activation with a float time constant (no update below 1 ms), clamped to
+// [0, 1]; the adaptation gain (1 + k_s S)(1 - k_h H) is a float expression.
+BC_DEV double clamp01d(double v) {
+    if (v < 0.0f) v = 0.0f;
+    if (v > 1.0f) v = 1.0f;
+    return v;
+}
+BC_DEV void sensor_relax(const ModSlot& m, int k, float act, float dt, float tau) {
+    if (tau < 1.0f) return;
+    double v = m.get(k);
+    v += (act - v) * dt / tau;
+    m.set(k, clamp01d(v));
+}
+BC_DEV float sensor_gain(float H, float S, float kh, float ks) { return (1.0f + ks * S) * (1.0f - kh * H); }
+// GHK Na+ (double), K+ (float) and, with kCa, Ca2+ (double) currents at the segment's double V
+// for N channels of partial permeabilities pp; updates the intracellular pools.
+struct SensorI { double na; float k; double ca; };
+template <bool kCa>
+BC_DEV SensorI sensor_ghk(SegRegs& s, const P3& pp, double N, float dt) {
+    SensorI r;
+    r.na = pp.na * ghk_mono(s, s.V, s.in[0], s.ex[0]) * N * kAtoPa;
+    r.k = (float)(pp.k * ghk_mono(s, s.V, s.in[1], s.ex[1]) * N * kAtoPa);
+    r.ca = kCa ? pp.ca * ghk_di(s, s.V, s.in[2], s.ex[2]) * N * kAtoPa : 0.0;
+    s.in[0] -= (r.na * dt) / s.cpm;
+    s.in[1] -= (r.k * dt) / s.cpm;
+    if (kCa) s.in[2] -= (r.ca * dt) / (2.0f * s.cpm);
+    floor7(s.in);
+    return r;
+}
+// Physical sensor (6): heat above 37 C, cold below 33 C and stretch of segments longer than
+// 50 um (s.len), scaled by state[1], drive state[0] (10 ms); habituation state[2] (tau state[6]
+// or 2 s), sensitisation state[3] (30 s); 50 pS Na+/K+/Ca2+ 0.4/0.3/0.3.
+BC_DEV double ch_physical_sensor(const ModSlot& m, SegRegs& s, float dt) {
+    float area = eff_area(m, s);
+    double sens = (m.get(1) > 0.0f) ? m.get(1) : 1.0f;
+    float T = s.temp;
+    float hot = (T > 37.0f) ? (T - 37.0f) / 6.0f : 0.0f;
+    float cold = (T < 33.0f) ? (33.0f - T) / 8.0f : 0.0f;
+    float stretch = (s.len - 50.0f) / 50.0f;
+    if (stretch < 0.0f) stretch = 0.0f;
+    float stim = sens * ((hot + cold) + stretch * 0.5f);
+    double a = m.get(0);
+    a = clamp01d(a + (stim - a) * dt / 10.0f);
+    m.set(0, a);
+    double tau_h = (m.get(6) > 0.0f) ? m.get(6) : 2000.0f;
+    sensor_relax(m, 2, a, dt, tau_h);
+    sensor_relax(m, 3, a, dt, 30000.0f);
+    double kh = (m.get(4) > 0.0f) ? m.get(4) : 0.6f;
+    double ks = (m.get(5) > 0.0f) ? m.get(5) : 0.2f;
+    double g = sensor_gain(m.get(2), m.get(3), kh, ks);
+    constexpr P3 pp = p_partial3(50.0f, 0.4f, 0.3f, 0.3f);
+    const SensorI I = sensor_ghk<true>(s, pp, m.dens * area * a * g, dt);
+    return -(I.na - I.k + I.ca);
+}
+// Hunger sensor (94): ATP / 5 mM below 0.3 (sigmoid slope 10) drives state[0] (500 ms);
+// habituation state[1] (300 s), sensitisation state[2] (600 s); 50 pS Na+/K+ 0.6/0.3.
+BC_DEV double ch_hunger_sensor(const ModSlot& m, SegRegs& s, float dt) {
+    float an = s.atp / 5.0f;
+    if (an > 1.0f) an = 1.0f;
+    if (an < 0.0f) an = 0.0f;
+    float h = 1.0f / (1.0f + bc::exp(10.0f * (an - 0.3f)));
+    double a = m.get(0);
+    a = clamp01d(a + (h - a) * dt / 500.0f);
+    m.set(0, a);
+    sensor_relax(m, 1, a, dt, 300000.0f);
+    sensor_relax(m, 2, a, dt, 600000.0f);
+    double kh = (m.get(3) > 0.0f) ? m.get(3) : 0.5f;
+    double ks = (m.get(4) > 0.0f) ? m.get(4) : 0.4f;
+    double g = sensor_gain(m.get(1), m.get(2), kh, ks);
+    float area = eff_area(m, s);
+    constexpr P3 pp = p_partial3(50.0f, 0.6f, 0.3f, 0.0f);
+    const SensorI I = sensor_ghk<false>(s, pp, m.dens * area * a * g, dt);
+    return -(I.na - I.k);
+}
+// Light sensor (95): intensity state[1]; habituation state[2] rises (2 s) above 0.1 and decays
+// (5 s) below, sensitisation state[3] (60 s); the adapted intensity sets state[0] (50 ms); the
+// CNG channel (25 pS, Na+/K+/Ca2+ 0.5/0.3/0.2) is open in the dark, 1 - state[0].
+BC_DEV double ch_light_sensor(const ModSlot& m, SegRegs& s, float dt) {
+    double I0 = m.get(1);
+    if (I0 > 0.1f) sensor_relax(m, 2, 1.0f, dt, 2000.0f);
+    else m.set(2, m.get(2) - dt * m.get(2) / 5000.0f);
+    m.set(2, clamp01d(m.get(2)));
+    sensor_relax(m, 3, I0, dt, 60000.0f);
+    float kh = 0.7f;
+    double ks = (m.get(4) > 0.0f) ? m.get(4) : 0.5f;
+    double ad = I0 * (1.0f - kh * m.get(2)) * (1.0f + ks * m.get(3));
+    if (ad < 0.0f) ad = 0.0f;
+    double a = m.get(0);
+    a = clamp01d(a + (ad - a) * dt / 50.0f);
+    m.set(0, a);
+    float area = eff_area(m, s);
+    double open = 1.0f - a;
+    constexpr P3 pp = p_partial3(25.0f, 0.5f, 0.3f, 0.2f);
+    const SensorI I = sensor_ghk<true>(s, pp, m.dens * area * open, dt);
+    return -(I.na + I.k + I.ca);
+}
+// Olfactory receptor (108): ligand state[1] (sigmoid at 0.01, slope 200) -> state[0] (50 ms);
+// habituation state[2] (2 s), sensitisation state[3] (10 s); the Gs path closes up to 80% of a
+// 35 pS K+ conductance and the current is the closed part.
+BC_DEV double ch_olfactory(const ModSlot& m, SegRegs& s, float dt) {
+    double L = m.get(1);
+    if (L < 0.0f) L = 0.0f;
+    if (L > 1.0f) L = 1.0f;
+    float act = 1.0f / (1.0f + bc::exp(-2.0f * (L - 0.01f) * 100.0f));
+    double a = m.get(0);
+    a = clamp01d(a + (act - a) * dt / 50.0f);
+    m.set(0, a);
+    sensor_relax(m, 2, a, dt, 2000.0f);
+    sensor_relax(m, 3, a, dt, 10000.0f);
+    float kh = 0.7f;
+    double ks = (m.get(4) > 0.0f) ? m.get(4) : 0.3f;
+    double g = sensor_gain(m.get(2), m.get(3), kh, ks);
+    float area = eff_area(m, s);
+    double e = a * g;
+    constexpr double P = p_single(35.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s, s.V, s.in[1], s.ex[1]);
+    double N = m.dens * area;
+    float basal = (float)(P * flux * N * kAtoPa);
+    float modulated = (float)(P * (1.0f - e * 0.8f) * flux * N * kAtoPa);
+    float eff = basal - modulated;
+    s.in[1] -= (eff * dt) / s.cpm;
+    floor7(s.in);
+    return -eff;
+}
+// A double converted to uint32_t the way the reference's x86-64 build does it: truncation to a
+// 64-bit integer (cvttsd2si) keeping the low 32 bits; outside the int64 range, or NaN, gives 0.
+BC_DEV uint32_t u32_trunc(double x) {
+    if (!(x > -9223372036854775808.0 && x < 9223372036854775808.0)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)x;
+}
+// Thermal noise (111): xorshift32 of (segment id s.sid, state[1], V) -> uniform r; sigma state[0]
+// (default 0.05 mV) x 1.732; the equivalent current through the segment's capacitance.
+BC_DEV double ch_thermal_noise(const ModSlot& m, const SegRegs& s, float dt) {
+    double sigma = (m.get(0) > 0.0f) ? m.get(0) : 0.05f;
+    uint32_t x = s.sid * 7919u + u32_trunc(m.get(1) * 131.0f);
+    x ^= u32_trunc(s.V * 1000.0f);
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    float r = (float)(x & 0x7FFFFFFFu) / (float)0x7FFFFFFFu;
+    float noise = (r * 2.0f - 1.0f) * sigma * 1.732f;
+    float cm = eff_area(m, s) * 0.01f;
+    if (cm < 1e-4f) cm = 1e-4f;
+    return noise * cm / dt;
+}
+// Vibration sensor (115): intensity state[1] (sigmoid at 0.3, slope 30) -> state[0] (30 ms);
+// habituation state[2] (3 s), sensitisation state[3] (10 s); 150 pS MET, Na+/K+ 0.6/0.4.
+BC_DEV double ch_vibration(const ModSlot& m, SegRegs& s, float dt) {
+    double I0 = m.get(1);
+    if (I0 < 0.0f) I0 = 0.0f;
+    if (I0 > 1.0f) I0 = 1.0f;
+    float act = 1.0f / (1.0f + bc::exp(-3.0f * (I0 - 0.3f) * 10.0f));
+    double a = m.get(0);
+    a = clamp01d(a + (act - a) * dt / 30.0f);
+    m.set(0, a);
+    sensor_relax(m, 2, a, dt, 3000.0f);
+    sensor_relax(m, 3, a, dt, 10000.0f);
+    double ks = (m.get(4) > 0.0f) ? m.get(4) : 0.25f;
+    double g = sensor_gain(m.get(2), m.get(3), 0.6f, ks);
+    double e = a * g;
+    float area = eff_area(m, s);
+    constexpr P3 pp = p_partial3(150.0f, 0.6f, 0.4f, 0.0f);
+    const SensorI I = sensor_ghk<false>(s, pp, m.dens * area * e, dt);
+    return -(I.na + I.k);
+}
+// Otolith hair cell (116): shear state[1] along the preferred direction (cosine state[5]; -99
+// or below means 1), reduced by Ca2+ adaptation state[2] (30 ms), sets the MET open probability
+// state[0] (tonic 0.15, 1 ms); 150 pS Na+/K+/Ca2+ 0.6/0.3/0.1.
+BC_DEV double ch_otolith(const ModSlot& m, SegRegs& s, float dt) {
+    double sh = m.get(1);
+    if (sh < 0.0f) sh = 0.0f;
+    if (sh > 1.0f) sh = 1.0f;
+    double c = (m.get(5) > -99.0f) ? m.get(5) : 1.0f;
+    if (c < -1.0f) c = -1.0f;
+    if (c > 1.0f) c = 1.0f;
+    float es = sh * c;
+    if (es < 0.0f) es = 0.0f;
+    double adapt = m.get(2);
+    float ad = es * (1.0f - 0.7f * adapt);
+    double k = (m.get(4) > 0.0f) ? m.get(4) : 5.0f;
+    float target = 0.15f + 0.85f / (1.0f + bc::exp(-k * (ad - 0.15f) * 10.0f));
+    double a = m.get(0);
+    a = clamp01d(a + (target - a) * dt / 1.0f);
+    m.set(0, a);
+    m.set(2, clamp01d(adapt + (a - adapt) * dt / 30.0f));
+    float area = eff_area(m, s);
+    constexpr P3 pp = p_partial3(150.0f, 0.60f, 0.30f, 0.10f);
+    const SensorI I = sensor_ghk<true>(s, pp, m.dens * area * a, dt);
+    float tot = I.na - I.k + I.ca;
+    return -tot;
+}
+// Semicircular-canal cupula (117): a damped torsion pendulum driven by angular acceleration
+// state[1] (theta state[0], omega state[3]; omega0^2 state[4] or 1, 2 zeta omega0 state[5] or
+// 0.9) with an 8 s elastic return; |theta| opens the hair cell, adapted by state[2] (10 ms);
+// 150 pS Na+/K+/Ca2+ 0.55/0.35/0.10.
+BC_DEV double ch_cupula(const ModSlot& m, SegRegs& s, float dt) {
+    double al = m.get(1);
+    if (al < -1.0f) al = -1.0f;
+    if (al > 1.0f) al = 1.0f;
+    double w2 = (m.get(4) > 0.0f) ? m.get(4) : 1.0f;
+    double zw = (m.get(5) > 0.0f) ? m.get(5) : 0.9f;
+    float gam = 1.0f;
+    double om = m.get(3), th = m.get(0);
+    float dom = al * gam - zw * om - w2 * th;
+    om += dom * dt / 1000.0f;
+    th += om * dt / 1000.0f;
+    th *= bc::exp(-dt / 8000.0f);
+    om *= bc::exp(-dt / 8000.0f);
+    if (th < -1.0f) th = -1.0f;
+    if (th > 1.0f) th = 1.0f;
+    m.set(0, th);
+    m.set(3, om);
+    float defl = fabsf((float)th);
+    double hs = (m.get(4) > 0.0f) ? m.get(4) * 0.4f : 2.5f;
+    float target = 1.0f / (1.0f + bc::exp(-hs * (defl - 0.1f) * 15.0f));
+    double ca = m.get(2);
+    ca = clamp01d(ca + (target - ca) * dt / 10.0f);
+    m.set(2, ca);
+    double open = target * (1.0f - 0.6f * ca);
+    float area = eff_area(m, s);
+    constexpr P3 pp = p_partial3(150.0f, 0.55f, 0.35f, 0.10f);
+    const SensorI I = sensor_ghk<true>(s, pp, m.dens * area * open, dt);
+    float tot = I.na - I.k + I.ca;
+    return -tot;
+}
+// Cryptochrome magnetic sensor (120): radical-pair yield light x sin^2(theta) x wavelength x
+// sensitivity (states 1, 2, 5, 4) -> state[0] (50 ms). Its boost of the segment's first light
+// sensor (state[0] x (1 + 0.3 state[0]), capped at 1) is applied by the step-1 kernel after it
+// runs. No current.
+BC_DEV double ch_cryptochrome(const ModSlot& m, const SegRegs& s, float dt) {
+    double L = m.get(1);
+    if (L < 0.0f) L = 0.0f;
+    if (L > 1.0f) L = 1.0f;
+    double s2 = m.get(2);
+    if (s2 < 0.0f) s2 = 0.0f;
+    if (s2 > 1.0f) s2 = 1.0f;
+    double wf = (m.get(5) > 0.0f) ? m.get(5) : 1.0f;
+    double sens = (m.get(4) > 0.0f) ? m.get(4) : 1.0f;
+    float rp = L * s2 * wf * sens;
+    double a = m.get(0);
+    m.set(0, clamp01d(a + (rp - a) * dt / 50.0f));
+    return 0.0f;
+}
+
 // ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
 // segment's ER pool s.er (Segment.ER_Ca2), IP3R (127: Li-Rinzel gating, IP3 = s.ip3) and RyR
 // (128: Keizer-Levine gating with adaptation w) release it; no membrane current. Constants
@@ -2846,6 +3094,15 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 90: return ch_protein_synth(m, s, dt);
         case 91: return ch_cofilin(m, s, dt);
         case 92: return ch_mmp9(m, s, dt);
+        case 6: return ch_physical_sensor(m, s, dt);
+        case 94: return ch_hunger_sensor(m, s, dt);
+        case 95: return ch_light_sensor(m, s, dt);
+        case 108: return ch_olfactory(m, s, dt);
+        case 111: return ch_thermal_noise(m, s, dt);
+        case 115: return ch_vibration(m, s, dt);
+        case 116: return ch_otolith(m, s, dt);
+        case 117: return ch_cupula(m, s, dt);
+        case 120: return ch_cryptochrome(m, s, dt);
         case 146: return ch_cacc(m, s, dt);
         case 152: return ch_can(m, s, dt);
         case 186: return ch_aqp4(m, s, dt);

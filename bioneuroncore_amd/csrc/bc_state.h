@@ -31,8 +31,8 @@ enum : uint8_t {
     SF_NTX = 32,          // an incoming synapse carries a cleft transmitter other than glutamate/GABA
     SF_ER = 64,           // carries SERCA, IP3R or RyR: step 1 reads and writes its ER Ca2+ store
     SF_TT = 128,          // carries the TRPV1 receptor, somatic ATP synthesis, thermoregulation,
-                          // CREB or NF-kB: step 1 reads and writes its temperature and
-                          // transcription signal
+                          // CREB, NF-kB or a physical sensor: step 1 reads and writes its
+                          // temperature and transcription signal
 };
 
 // Synapse flag bits (static).
@@ -131,6 +131,7 @@ struct Net {
     // segment static data
     const float* radius;
     const float* length;
+    const uint32_t* segid;   // Segment.id of each record (thermal noise seeds its draw with it)
     const uint8_t* stype;
     const float* area;
     const double* vol;

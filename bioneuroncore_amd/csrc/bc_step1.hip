@@ -99,7 +99,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     break;
                 }
         }
+        if (kExt && t == 6) s.len = d.length[i];
+        if (kExt && t == 111) s.sid = d.segid[i];
         itot += dispatch<kNt, kExt>(t, m, s, dt);
+        if (kExt && t == 120) {   // cryptochrome boosts the first light sensor's state[0] (hh_biological.cpp cryptochrome_magnetic_current)
+            const double rp = m.get(0);
+            for (uint32_t q = 0; q < cnt; q++)
+                if (d.mtype[q * P + i] == 95) {
+                    double* c = d.mstate + (size_t)q * 7 * P + i;
+                    double v = *c * (1.0f + 0.3f * rp);
+                    if (v > 1.0f) v = 1.0f;
+                    *c = v;
+                    break;
+                }
+        }
         if (kExt && t == 87) {   // nuclear ERK raises every CREB's pCREB on the segment (hh_biological.cpp mapk_erk_current)
             float dts = dt;
             dts *= m.dens;

@@ -2184,6 +2184,251 @@ static double mod_mmp9(Modifier* m, Segment* s, float dt) {
     return 0.0f;
 }
 
+// Sensors (hh_biological.cpp:40-52, 483-540, 2181-2440, 3122-3141). Adaptation: state[k] relaxes
+// toward a float activation with a float time constant (skipped below 1 ms), clamped to [0, 1]
+// (sensor_habituation_step and sensor_sensitization_step are the same update); the gain
+// (1 + k_s S)(1 - k_h H) is evaluated in float.
+static inline void sensor_relax(Modifier* m, int k, float act, float dt, float tau) {
+    if (tau < 1.0f) return;
+    m->state[k] += (act - m->state[k]) * dt / tau;
+    if (m->state[k] < 0.0f) m->state[k] = 0.0f;
+    if (m->state[k] > 1.0f) m->state[k] = 1.0f;
+}
+static inline float sensor_gain(float H, float S, float kh, float ks) { return (1.0f + ks * S) * (1.0f - kh * H); }
+static inline void clamp_state01(Modifier* m, int k) {
+    if (m->state[k] < 0.0f) m->state[k] = 0.0f;
+    if (m->state[k] > 1.0f) m->state[k] = 1.0f;
+}
+// GHK Na+/K+/Ca2+ currents of a channel with partial permeabilities; fca == 0 drops the Ca2+ term.
+struct SensorI { double na; float k; double ca; };
+static SensorI sensor_ghk(Segment* s, float gamma, float fna, float fk, float fca, double N, float dt) {
+    double cpm = kFaraday * volume(s) / 1000.0;
+    double pna, pk, pca;
+    p_partial(gamma, fna, fk, fca, &pna, &pk, &pca);
+    double V = s->V_m;   // the segment's potential in double, as the sensors pass it
+    SensorI r;
+    r.na = pna * ghk_mono(V, s->intra_ions[0], s->extra_ions[0]) * N * kAtoPa;
+    r.k = (float)(pk * ghk_mono(V, s->intra_ions[1], s->extra_ions[1]) * N * kAtoPa);
+    r.ca = (fca != 0.0f) ? pca * ghk_di(V, s->intra_ions[2], s->extra_ions[2]) * N * kAtoPa : 0.0;
+    s->intra_ions[0] -= (r.na * dt) / cpm;
+    s->intra_ions[1] -= (r.k * dt) / cpm;
+    if (fca != 0.0f) s->intra_ions[2] -= (r.ca * dt) / (2.0f * cpm);
+    floor_ions(s->intra_ions);
+    return r;
+}
+// Physical sensor (6): heat above 37 C and cold below 33 C, plus stretch of segments longer than
+// 50 um, scaled by state[1] (default 1), drive state[0] (10 ms); habituation state[2] (tau
+// state[6] or 2 s), sensitisation state[3] (30 s); a 50 pS Na+/K+/Ca2+ (0.4/0.3/0.3) current.
+static double mod_physical_sensor(Modifier* m, Segment* s, float dt) {
+    float area = eff_area(m, s);
+    double sens = (m->state[1] > 0.0f) ? m->state[1] : 1.0f;
+    float T = s->temperature;
+    float hot = (T > 37.0f) ? (T - 37.0f) / 6.0f : 0.0f;
+    float cold = (T < 33.0f) ? (33.0f - T) / 8.0f : 0.0f;
+    float stretch = (s->length - 50.0f) / 50.0f;
+    if (stretch < 0.0f) stretch = 0.0f;
+    float stim = sens * ((hot + cold) + stretch * 0.5f);
+    m->state[0] += (stim - m->state[0]) * dt / 10.0f;
+    clamp_state01(m, 0);
+    double tau_h = (m->state[6] > 0.0f) ? m->state[6] : 2000.0f;
+    sensor_relax(m, 2, m->state[0], dt, tau_h);
+    sensor_relax(m, 3, m->state[0], dt, 30000.0f);
+    double kh = (m->state[4] > 0.0f) ? m->state[4] : 0.6f;
+    double ks = (m->state[5] > 0.0f) ? m->state[5] : 0.2f;
+    double g = sensor_gain(m->state[2], m->state[3], kh, ks);
+    SensorI I = sensor_ghk(s, 50.0f, 0.4f, 0.3f, 0.3f, m->density * area * m->state[0] * g, dt);
+    return -(I.na - I.k + I.ca);
+}
+// Hunger sensor (94): low ATP (normalised to 5 mM, sigmoid at 0.3) drives state[0] (500 ms);
+// habituation state[1] (300 s), sensitisation state[2] (600 s); 50 pS Na+/K+ (0.6/0.3).
+static double mod_hunger_sensor(Modifier* m, Segment* s, float dt) {
+    float a = s->ATP_level / 5.0f;
+    if (a > 1.0f) a = 1.0f;
+    if (a < 0.0f) a = 0.0f;
+    float h = 1.0f / (1.0f + exp(10.0f * (a - 0.3f)));
+    m->state[0] += (h - m->state[0]) * dt / 500.0f;
+    clamp_state01(m, 0);
+    sensor_relax(m, 1, m->state[0], dt, 300000.0f);
+    sensor_relax(m, 2, m->state[0], dt, 600000.0f);
+    double kh = (m->state[3] > 0.0f) ? m->state[3] : 0.5f;
+    double ks = (m->state[4] > 0.0f) ? m->state[4] : 0.4f;
+    double g = sensor_gain(m->state[1], m->state[2], kh, ks);
+    float area = eff_area(m, s);
+    SensorI I = sensor_ghk(s, 50.0f, 0.6f, 0.3f, 0.0f, m->density * area * m->state[0] * g, dt);
+    return -(I.na - I.k);
+}
+// Light sensor (95): intensity state[1]; habituation state[2] rises (2 s) above 0.1 and decays
+// (5 s) below; sensitisation state[3] (60 s); adapted intensity -> state[0] (50 ms); the CNG
+// channel (25 pS, Na+/K+/Ca2+ 0.5/0.3/0.2) is open in the dark (1 - state[0]).
+static double mod_light_sensor(Modifier* m, Segment* s, float dt) {
+    double I0 = m->state[1];
+    if (I0 > 0.1f) sensor_relax(m, 2, 1.0f, dt, 2000.0f);
+    else m->state[2] -= dt * m->state[2] / 5000.0f;
+    clamp_state01(m, 2);
+    sensor_relax(m, 3, I0, dt, 60000.0f);
+    float kh = 0.7f;
+    double ks = (m->state[4] > 0.0f) ? m->state[4] : 0.5f;
+    double ad = I0 * (1.0f - kh * m->state[2]) * (1.0f + ks * m->state[3]);
+    if (ad < 0.0f) ad = 0.0f;
+    m->state[0] += (ad - m->state[0]) * dt / 50.0f;
+    clamp_state01(m, 0);
+    float area = eff_area(m, s);
+    double open = 1.0f - m->state[0];
+    SensorI I = sensor_ghk(s, 25.0f, 0.5f, 0.3f, 0.2f, m->density * area * open, dt);
+    return -(I.na + I.k + I.ca);
+}
+// Olfactory receptor (108): ligand state[1] (steep sigmoid at 0.01) -> state[0] (50 ms);
+// habituation state[2] (2 s), sensitisation state[3] (10 s); the Gs path closes up to 80% of a
+// 35 pS K+ conductance: the current is the closed part.
+static double mod_olfactory(Modifier* m, Segment* s, float dt) {
+    double L = m->state[1];
+    if (L < 0.0f) L = 0.0f;
+    if (L > 1.0f) L = 1.0f;
+    float act = 1.0f / (1.0f + exp(-2.0f * (L - 0.01f) * 100.0f));
+    m->state[0] += (act - m->state[0]) * dt / 50.0f;
+    clamp_state01(m, 0);
+    sensor_relax(m, 2, m->state[0], dt, 2000.0f);
+    sensor_relax(m, 3, m->state[0], dt, 10000.0f);
+    float kh = 0.7f;
+    double ks = (m->state[4] > 0.0f) ? m->state[4] : 0.3f;
+    double g = sensor_gain(m->state[2], m->state[3], kh, ks);
+    float area = eff_area(m, s);
+    double cpm = kFaraday * volume(s) / 1000.0;
+    double e = m->state[0] * g;
+    double P = p_single(35.0f, 1.0f, kCbarK);
+    double flux = ghk_mono(s->V_m, s->intra_ions[1], s->extra_ions[1]);
+    double N = m->density * area;
+    float basal = (float)(P * flux * N * kAtoPa);
+    float modulated = (float)(P * (1.0f - e * 0.8f) * flux * N * kAtoPa);
+    float eff = basal - modulated;
+    s->intra_ions[1] -= (eff * dt) / cpm;
+    floor_ions(s->intra_ions);
+    return -eff;
+}
+// The reference converts doubles to uint32_t by a plain cast; its x86-64 build truncates to a
+// 64-bit integer (cvttsd2si) and keeps the low 32 bits, so negative values wrap and values
+// outside the int64 range (or NaN) give 0. This states that conversion.
+static inline uint32_t u32_trunc(double x) {
+    if (!(x > -9223372036854775808.0 && x < 9223372036854775808.0)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)x;
+}
+// Thermal noise (111): a uniform draw from xorshift32 seeded with the segment id, state[1] and
+// the membrane potential, scaled to sigma state[0] (default 0.05 mV) times sqrt(3), returned as
+// the equivalent current through the segment's capacitance (1 uF/cm^2, floor 1e-4).
+static double mod_thermal_noise(Modifier* m, Segment* s, float dt) {
+    double sigma = (m->state[0] > 0.0f) ? m->state[0] : 0.05f;
+    uint32_t x = s->id * 7919u + u32_trunc(m->state[1] * 131.0f);
+    x ^= u32_trunc(s->V_m * 1000.0f);
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    float r = (float)(x & 0x7FFFFFFFu) / (float)0x7FFFFFFFu;
+    float noise = (r * 2.0f - 1.0f) * sigma * 1.732f;
+    float cm = eff_area(m, s) * 0.01f;
+    if (cm < 1e-4f) cm = 1e-4f;
+    return noise * cm / dt;
+}
+// Vibration sensor (115): intensity state[1] (sigmoid at 0.3) -> state[0] (30 ms); habituation
+// state[2] (3 s), sensitisation state[3] (10 s); a 150 pS MET current (Na+/K+ 0.6/0.4).
+static double mod_vibration(Modifier* m, Segment* s, float dt) {
+    double I0 = m->state[1];
+    if (I0 < 0.0f) I0 = 0.0f;
+    if (I0 > 1.0f) I0 = 1.0f;
+    float act = 1.0f / (1.0f + exp(-3.0f * (I0 - 0.3f) * 10.0f));
+    m->state[0] += (act - m->state[0]) * dt / 30.0f;
+    clamp_state01(m, 0);
+    sensor_relax(m, 2, m->state[0], dt, 3000.0f);
+    sensor_relax(m, 3, m->state[0], dt, 10000.0f);
+    double ks = (m->state[4] > 0.0f) ? m->state[4] : 0.25f;
+    double g = sensor_gain(m->state[2], m->state[3], 0.6f, ks);
+    double e = m->state[0] * g;
+    float area = eff_area(m, s);
+    SensorI I = sensor_ghk(s, 150.0f, 0.6f, 0.4f, 0.0f, m->density * area * e, dt);
+    return -(I.na + I.k);
+}
+// Otolith hair cell (116): shear state[1] along the preferred direction (cosine state[5], unset
+// below -99), reduced by Ca2+ adaptation state[2] (30 ms), sets the MET open probability
+// state[0] (tonic 0.15, 1 ms); a 150 pS Na+/K+/Ca2+ (0.6/0.3/0.1) current.
+static double mod_otolith(Modifier* m, Segment* s, float dt) {
+    double sh = m->state[1];
+    if (sh < 0.0f) sh = 0.0f;
+    if (sh > 1.0f) sh = 1.0f;
+    double c = (m->state[5] > -99.0f) ? m->state[5] : 1.0f;
+    if (c < -1.0f) c = -1.0f;
+    if (c > 1.0f) c = 1.0f;
+    float es = sh * c;
+    if (es < 0.0f) es = 0.0f;
+    double adapt = m->state[2];
+    float ad = es * (1.0f - 0.7f * adapt);
+    double k = (m->state[4] > 0.0f) ? m->state[4] : 5.0f;
+    float target = 0.15f + 0.85f / (1.0f + exp(-k * (ad - 0.15f) * 10.0f));
+    m->state[0] += (target - m->state[0]) * dt / 1.0f;
+    clamp_state01(m, 0);
+    m->state[2] += (m->state[0] - m->state[2]) * dt / 30.0f;
+    clamp_state01(m, 2);
+    float area = eff_area(m, s);
+    SensorI I = sensor_ghk(s, 150.0f, 0.60f, 0.30f, 0.10f, m->density * area * m->state[0], dt);
+    float tot = I.na - I.k + I.ca;
+    return -tot;
+}
+// Semicircular-canal cupula (117): a damped torsion pendulum driven by angular acceleration
+// state[1] (theta state[0], omega state[3]; omega0^2 state[4] or 1, 2 zeta omega0 state[5] or
+// 0.9) with an 8 s elastic return; |theta| opens the hair cell (state[2] is its 10 ms Ca2+
+// adaptation); a 150 pS Na+/K+/Ca2+ (0.55/0.35/0.10) current.
+static double mod_cupula(Modifier* m, Segment* s, float dt) {
+    double a = m->state[1];
+    if (a < -1.0f) a = -1.0f;
+    if (a > 1.0f) a = 1.0f;
+    double w2 = (m->state[4] > 0.0f) ? m->state[4] : 1.0f;
+    double zw = (m->state[5] > 0.0f) ? m->state[5] : 0.9f;
+    float gam = 1.0f;
+    double om = m->state[3], th = m->state[0];
+    float dom = a * gam - zw * om - w2 * th;
+    om += dom * dt / 1000.0f;
+    th += om * dt / 1000.0f;
+    th *= exp(-dt / 8000.0f);
+    om *= exp(-dt / 8000.0f);
+    if (th < -1.0f) th = -1.0f;
+    if (th > 1.0f) th = 1.0f;
+    m->state[0] = th;
+    m->state[3] = om;
+    float defl = fabsf(th);
+    double hs = (m->state[4] > 0.0f) ? m->state[4] * 0.4f : 2.5f;
+    float target = 1.0f / (1.0f + exp(-hs * (defl - 0.1f) * 15.0f));
+    m->state[2] += (target - m->state[2]) * dt / 10.0f;
+    clamp_state01(m, 2);
+    double open = target * (1.0f - 0.6f * m->state[2]);
+    float area = eff_area(m, s);
+    SensorI I = sensor_ghk(s, 150.0f, 0.55f, 0.35f, 0.10f, m->density * area * open, dt);
+    float tot = I.na - I.k + I.ca;
+    return -tot;
+}
+// Cryptochrome magnetic sensor (120): radical-pair yield light x sin^2(theta) x wavelength x
+// sensitivity (states 1, 2, 5, 4) -> state[0] (50 ms); it multiplies the first light sensor's
+// state[0] on the segment by 1 + 0.3 state[0] (capped at 1). No current.
+static double mod_cryptochrome(Modifier* m, Segment* s, float dt) {
+    double L = m->state[1];
+    if (L < 0.0f) L = 0.0f;
+    if (L > 1.0f) L = 1.0f;
+    double s2 = m->state[2];
+    if (s2 < 0.0f) s2 = 0.0f;
+    if (s2 > 1.0f) s2 = 1.0f;
+    double wf = (m->state[5] > 0.0f) ? m->state[5] : 1.0f;
+    double sens = (m->state[4] > 0.0f) ? m->state[4] : 1.0f;
+    float rp = L * s2 * wf * sens;
+    m->state[0] += (rp - m->state[0]) * dt / 50.0f;
+    clamp_state01(m, 0);
+    for (int i = 0; i < s->modifier_count; i++)
+        if (s->modifiers[i].type == 95) {
+            float kc = 0.3f;
+            double boost = 1.0f + kc * m->state[0];
+            s->modifiers[i].state[0] *= boost;
+            if (s->modifiers[i].state[0] > 1.0f) s->modifiers[i].state[0] = 1.0f;
+            break;
+        }
+    return 0.0f;
+}
+
 // ER Ca2+ store (hh_biological.cpp:3473-3591): SERCA (126) pumps cytosolic Ca2+ into the
 // segment's ER pool Segment.ER_Ca2, IP3R (127) and RyR (128) release it back; none of them
 // carries a membrane current. Constants from units.h:130-156.
@@ -2684,6 +2929,15 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 90: return mod_protein_synth(m, s, dt);
         case 91: return mod_cofilin(m, s, dt);
         case 92: return mod_mmp9(m, s, dt);
+        case 6: return mod_physical_sensor(m, s, dt);
+        case 94: return mod_hunger_sensor(m, s, dt);
+        case 95: return mod_light_sensor(m, s, dt);
+        case 108: return mod_olfactory(m, s, dt);
+        case 111: return mod_thermal_noise(m, s, dt);
+        case 115: return mod_vibration(m, s, dt);
+        case 116: return mod_otolith(m, s, dt);
+        case 117: return mod_cupula(m, s, dt);
+        case 120: return mod_cryptochrome(m, s, dt);
         case 3: return mod_metabotropic(m, s, dt);
         case 5: return mod_release_site(m, s, dt);
         case 188: return mod_mrgpr(m, s, dt);

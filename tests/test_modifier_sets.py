@@ -11,6 +11,8 @@ compiled reference on the same networks, tests/test_oracle_vs_reference.py):
   CREB and MRGPR (src/hh_biological.cpp, include/modifier_helpers.h mrgpr_signal);
 - the MAPK/ERK, NF-kB, mTORC1, protein-synthesis, cofilin and MMP-9 cascades, the TRPV1
   transmitter receptor and rod phototransduction (src/hh_biological.cpp);
+- the physical, hunger, light, olfactory, vibration, otolith, cupula and cryptochrome sensors and
+  thermal noise (src/hh_biological.cpp:483-540, 2181-2440, 3122-3141);
 - a network that mixes them with the core channel set runs the extended step-1 kernel, and the
   core-only networks keep the lean one (results identical either way).
 """
@@ -95,6 +97,17 @@ def test_cascade_modifiers(seed):
     """MAPK/ERK (raising CREB), NF-kB, mTORC1, protein synthesis, cofilin and MMP-9, the TRPV1
     transmitter receptor and rod phototransduction, 400 ticks: every byte of the records."""
     net = bion.cascade_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    seg, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0
+
+
+@pytest.mark.parametrize("seed", [91, 92])
+def test_sensor_modifiers(seed):
+    """The sensors (6, 94, 95, 108, 115-117), thermal noise seeded by Segment.id and V (111) and
+    the cryptochrome's boost of the first light sensor (120, before and after it in slot order),
+    400 ticks: every byte of the records."""
+    net = bion.sensor_network(seed)
     cur = H.soma_clamp(net, 160.0)
     seg, spk = _compare(net, 400, cur)
     assert spk.sum() > 0

@@ -167,6 +167,22 @@ def test_cascade_modifiers(seed):
         assert np.any(st[ty == t][:, 0] != 0.0), t
 
 
+@pytest.mark.parametrize("seed", [91, 92])
+def test_sensor_modifiers(seed):
+    """Physical (temperature and stretch), hunger, light, olfactory, thermal-noise (seeded by
+    Segment.id and V), vibration, otolith, cupula and cryptochrome sensors, the cryptochrome
+    before and after the light sensor it boosts: every byte of the records."""
+    net = bion.sensor_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    rs, _, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
+    st = rs["modifiers"]["state"]
+    ty = rs["modifiers"]["type"]
+    for t in (bion.PHYS_SENSOR, bion.HUNGER, bion.LIGHT, bion.OLFACTORY, bion.VIBRATION, bion.OTOLITH,
+              bion.CUPULA, bion.CRYPTOCHROME):
+        assert np.any(st[ty == t][:, 0] != 0.0), t
+
+
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
 def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2;
