@@ -370,8 +370,9 @@ BC_DEV double ch_nav(const ModSlot& m, SegRegs& s, float dt, const NavK& g, floa
     float mtau = 1.0f / (am + bm);
     float hinf = 1.0f / (1.0f + bc::exp((V + g.h_off) / 6.2f));
     float htau = 1.0f / (nav_ah(g, V) + nav_bh(g, V));
-    double mv = minf + (m.get(0) - minf) * bc::exp(-dt / mtau);
-    double hv = hinf + (m.get(1) - hinf) * bc::exp(-dt / htau);
+    const double s0 = m.get(0), s1 = m.get(1), s5 = m.get(5);   // all loads before the stores
+    double mv = minf + (s0 - minf) * bc::exp(-dt / mtau);
+    double hv = hinf + (s1 - hinf) * bc::exp(-dt / htau);
     if (mv < 0.0f) mv = 0.0f;
     if (mv > 1.0f) mv = 1.0f;
     if (hv < 0.0f) hv = 0.0f;
@@ -379,7 +380,6 @@ BC_DEV double ch_nav(const ModSlot& m, SegRegs& s, float dt, const NavK& g, floa
     m.set(0, mv);
     m.set(1, hv);
     float op = mv * mv * mv * hv;
-    double s5 = m.get(5);
     double ttx = (s5 > 0.0f) ? s5 : 0.0f;
     op /= (1.0f + ttx / ttx_ic50);
     op *= (1.0f - pkx * (s.pkc + s.pka) * 0.5f);
@@ -1179,8 +1179,9 @@ BC_DEV double ch_nav11(const ModSlot& m, SegRegs& s, float dt) {
     float mtau = 1.0f / (am + bm);
     float hinf = 1.0f / (1.0f + bc::exp((V + 65.0f) / 5.0f));
     float htau = 1.0f / (nav_ah(g, V) + nav_bh(g, V));
-    double mv = minf + (m.get(0) - minf) * bc::exp(-dt / mtau);
-    double hv = hinf + (m.get(1) - hinf) * bc::exp(-dt / htau);
+    const double s0 = m.get(0), s1 = m.get(1), s5 = m.get(5);   // all loads before the stores
+    double mv = minf + (s0 - minf) * bc::exp(-dt / mtau);
+    double hv = hinf + (s1 - hinf) * bc::exp(-dt / htau);
     if (mv < 0.0f) mv = 0.0f;
     if (mv > 1.0f) mv = 1.0f;
     if (hv < 0.0f) hv = 0.0f;
@@ -1188,7 +1189,6 @@ BC_DEV double ch_nav11(const ModSlot& m, SegRegs& s, float dt) {
     m.set(0, mv);
     m.set(1, hv);
     float op = mv * mv * mv * hv;
-    double s5 = m.get(5);
     double ttx = (s5 > 0.0f) ? s5 : 0.0f;
     op /= (1.0f + ttx / 10.0f);
     double P = p_single(17.0f, 1.0f, kCbarNa);

@@ -46,15 +46,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const bool my = sfl & SF_MYELIN;
     const uint32_t cnt = d.mcount[i];
     double itot = 0.0;
+    // Slot j + 1's type, active-vars index and density load while slot j runs (none of them
+    // changes during step 1), so each slot waits on one round trip to memory, for its states.
+    uint8_t tn = 0, avn = 0;
+    double dn = 0.0;
+    if (cnt > 0) {
+        tn = d.mtype[i];
+        avn = d.mav[i];
+        dn = d.mdens[i];
+    }
     for (uint32_t j = 0; j < cnt; j++) {
-        const uint8_t t = d.mtype[j * P + i];
-        if (my && t != 4 && myelin_blocked(t)) continue;
+        const uint8_t t = tn;
         ModSlot m;
+        m.dens = dn;
+        m.av = avn;
+        if (j + 1 < cnt) {
+            tn = d.mtype[(j + 1) * P + i];
+            avn = d.mav[(j + 1) * P + i];
+            dn = d.mdens[(j + 1) * P + i];
+        }
+        if (my && t != 4 && myelin_blocked(t)) continue;
         m.st = d.mstate + (size_t)j * 7 * P + i;
         m.stride = P;
-        m.dens = d.mdens[j * P + i];
         m.a7 = s.area;
-        m.av = d.mav[j * P + i];
         if (kExt && t == 144) {   // Cav1.4 reads the first CaBP4's state[0] as it stands now
             s.cabp4 = 0.0f;
             for (uint32_t q = 0; q < cnt; q++)
