@@ -117,8 +117,15 @@ CASCADES = (TRPV1_R, MAPK, NFKB, MTORC1, PROTEIN_SYNTH, COFILIN, MMP9, ROD)
 PHYS_SENSOR, HUNGER, LIGHT, OLFACTORY, THERMAL_NOISE = 6, 94, 95, 108, 111
 VIBRATION, OTOLITH, CUPULA, CRYPTOCHROME = 115, 116, 117, 120
 SENSORS = (PHYS_SENSOR, HUNGER, LIGHT, OLFACTORY, THERMAL_NOISE, VIBRATION, OTOLITH, CUPULA, CRYPTOCHROME)
+CAMKII, CALCINEURIN, PP1, PKC = 82, 83, 113, 137   # the kinase/phosphatase activities others read
+KINASES = (CAMKII, CALCINEURIN, PP1, PKC)
+MSN_NMDA, MSN_AMPA = 75, 76   # striatal NMDA/AMPA scaled by the segment's D1/D2 modulator
+# IDs without code in the reference: reserved, the plugin stubs 124/125 (no plugin loaded), and
+# 212 (the type count) and above
+RESERVED = (0, 80, 81, 93, 124, 125, 138, 139, 143)
+NO_CODE = RESERVED + tuple(range(212, 256))
 SUPPORTED_MODIFIERS = (MYELIN, PUMP, AMPA, NMDA, GABAA, SK, BK, K2P, KV7, KVDR, HCN1, NALCN, CAV21,
-                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING + CASCADES + SENSORS
+                       NAV16, NAV12, KV31, KV11, CX36, CX43, ECB_SYNTHESIS) + TRANSPORTERS + HOMEOSTASIS + CHANNELS2 + RECEPTORS2 + RECEPTORS3 + NAV_VARIANTS + KV_VARIANTS + K_MORE + CAV_MORE + CATION_MORE + MISC_MORE + CA_ACT + ER_STORE + SIGNALLING + CASCADES + SENSORS + KINASES + (MSN_NMDA, MSN_AMPA) + NO_CODE
 
 # Physiological ion means (reference include/cpg_config.h:400-412).
 INTRA_IONS = np.array([15.0, 140.0, 0.0001, 5.0, 0.5, 151.0, 6.3e-5, 0.0], dtype=np.float32)
@@ -145,6 +152,8 @@ CANON_DENSITY.update({TRPV1_R: 0.05, MAPK: 1.0, NFKB: 1.0, MTORC1: 1.0, PROTEIN_
                       ROD: 0.05})
 CANON_DENSITY.update({t: 1.0 for t in SENSORS})
 CANON_DENSITY[PHYS_SENSOR] = 0.8   # cpg_config.h:85
+CANON_DENSITY.update({CAMKII: 40.0, CALCINEURIN: 4.0, PP1: 40.0, PKC: 1.0, MSN_NMDA: 40.0, MSN_AMPA: 40.0})
+CANON_DENSITY.update({t: 1.0 for t in NO_CODE})
 # Gate initial values near rest (-65 mV).
 GATE_INIT = {NAV12: {0: 0.02, 1: 0.75}, NAV16: {0: 0.01, 1: 0.6}, KVDR: {0: 0.05},
              CAV21: {0: 0.0, 1: 0.5}, HCN1: {0: 0.3}, KV11: {0: 0.03, 1: 1.0},
@@ -814,6 +823,52 @@ def sensor_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> N
         st[i, k + 1, 0] = rng.uniform(0.3, 0.9)
         st[i, k + 1, 1] = rng.uniform(0.0, 0.3)
         segs["modifier_count"][i] = k + 2
+    return net
+
+
+def kinase_network(seed: int, n_neurons: int = 12, seg_per_neuron: int = 6) -> Network:
+    """Chain neurons whose somas carry Cav2.1, CaMKII, calcineurin, PP1 / DARPP-32, PLC/IP3/DAG
+    (DAG from its initial state), PKC, Cx36, Kv4.2 and the MSN NMDA/AMPA receptors (75/76, scale
+    1 without D1/D2 modulators) beside Nav1.2 (a PKC reader), with AMPA+NMDA / GABA-A synapses
+    (every third excitatory one with the MSN pair as its receptors): vesicle recovery reads the presynaptic CaMKII and calcineurin,
+    plasticity the post segment's PP1. Every other dendrite segment appends PLC, PKC, CaMKII, Cx36,
+    PP1, Kv4.2, a second CaMKII and PKC (the readers take the last CaMKII and the first PKC),
+    calcineurin and the reserved IDs 80, 124 and 143 with non-zero states. Intracellular Ca
+    varies 1-50x per segment."""
+    soma = [CAV21, CAMKII, CALCINEURIN, PP1, PLC, PKC, CX36, KV42, MSN_NMDA, MSN_AMPA]
+    net = build_network(n_neurons, seg_per_neuron, "chain", [NAV12, KVDR, K2P, NALCN, PUMP], soma,
+                        3.0, "ampa_nmda_gaba", 4, seed)
+    pr = net.synapses["post_receptors"]
+    msn = np.nonzero(net.synapses["post_receptor_count"] == 2)[0][::3]   # MSN receptors on some
+    pr["type"][msn, 0] = MSN_AMPA
+    pr["type"][msn, 1] = MSN_NMDA
+    segs = net.segments
+    mods = segs["modifiers"]
+    rng = np.random.default_rng(seed + 11)
+    n = len(segs)
+    segs["intra_ions"][:, 2] *= rng.uniform(1.0, 50.0, n)
+    base = 5
+    soma_idx = np.nonzero(segs["segment_type"] == 1)[0]
+    mods["state"][soma_idx, base + 4, 1] = rng.uniform(0.05, 1.0, len(soma_idx))   # DAG
+    mods["state"][soma_idx, base + 1, 0] = rng.uniform(0.0, 0.5, len(soma_idx))    # CaMKII
+    mods["state"][soma_idx, base + 3, 0] = rng.uniform(0.0, 1.0, len(soma_idx))    # p-DARPP-32
+    mods["state"][soma_idx, base + 3, 1] = rng.uniform(0.0, 1.0, len(soma_idx))    # p-inhibitor-1
+    mods["state"][soma_idx[::3], base + 1, 2] = 0.6   # a CaMKII K_d override
+    for i in np.nonzero(segs["segment_type"] == 2)[0][::2]:
+        k = int(segs["modifier_count"][i])
+        extra = (PLC, PKC, CAMKII, CX36, PP1, KV42, CAMKII, PKC, CALCINEURIN, 80, 124, 143)
+        for j, t in enumerate(extra):
+            mods["type"][i, k + j] = t
+            mods["density"][i, k + j] = CANON_DENSITY[t] * rng.uniform(0.5, 2.0)
+            for q, v in GATE_INIT.get(t, {}).items():
+                mods["state"][i, k + j, q] = v
+        mods["state"][i, k, 1] = rng.uniform(0.05, 1.0)        # DAG
+        mods["state"][i, k + 2, 0] = rng.uniform(0.0, 0.5)     # the first CaMKII
+        mods["state"][i, k + 4, 0] = rng.uniform(0.0, 1.0)     # p-DARPP-32
+        for j in (9, 10, 11):
+            mods["state"][i, k + j, :7] = rng.uniform(0.1, 1.0, 7)
+            mods["active_vars"][i, k + j] = 2
+        segs["modifier_count"][i] = k + len(extra)
     return net
 
 

@@ -297,10 +297,10 @@ static void protein_mapped(const Synapse& y, const BionData& B, float& rmax, flo
     if (rrec > 0.2f) rrec = 0.2f;
 }
 static void release_params(const Sim& S, std::vector<float>& rmax, std::vector<float>& prel,
-                           std::vector<float>& prim0) {
+                           std::vector<float>& prim0, std::vector<float>& rrecs) {
     const BionData& B = S.host;
     const uint32_t ns = B.num_synapses;
-    rmax.resize(ns); prel.resize(ns); prim0.resize(ns);
+    rmax.resize(ns); prel.resize(ns); prim0.resize(ns); rrecs.resize(ns);
     for (uint32_t k = 0; k < ns; k++) {
         const Synapse& y = B.synapses[S.slot2syn[k]];
         float rrec;
@@ -314,6 +314,7 @@ static void release_params(const Sim& S, std::vector<float>& rmax, std::vector<f
         }
         float a = rrec + ck * 0.02f;
         prim0[k] = a + cn * 0.01f;
+        rrecs[k] = rrec;
     }
 }
 
@@ -472,6 +473,7 @@ static int build_device(Sim& S) {
     const PluginTypes plug = plugin_types();
     S.nt_read = false;
     S.ext_mods = false;
+    bool any_ck = false, any_pp1 = false;   // CaMKII / calcineurin, PP1 on some segment
     for (uint32_t di = 0; di < n; di++) {
         const Segment& s = sg[S.dev2host[di]];
         radius[di] = s.radius; length[di] = s.length; stype[di] = s.segment_type; segid[di] = s.id;
@@ -495,6 +497,8 @@ static int build_device(Sim& S) {
                 camkii01[di] = m.state[0] + m.state[1];
             }
             if (m.type == 83) calcn[di] = m.state[0];
+            if (m.type == 82 || m.type == 83) any_ck = true;
+            if (m.type == 113) any_pp1 = true;
             if (m.type == M_CAV21) cav21[di] = (int8_t)j;
             if (modifier_reads_nt(m.type)) fl |= SF_NTREAD;
             if (m.type == 126 || m.type == 127 || m.type == 128) fl |= SF_ER;
@@ -664,6 +668,8 @@ static int build_device(Sim& S) {
             if (rt == 77 || rt == 78 || rt == 79 || rt == 16 || rt == 88) S.unsupported_mod++;
             // and the cascades that read or write sibling modifiers of the post segment
             if (rt == 87 || rt == 90 || rt == 91 || rt == 92) S.unsupported_mod++;
+            // and the cascades whose activity the post segment's readers take from its own slots
+            if (rt == 82 || rt == 83 || rt == 113 || rt == 137) S.unsupported_mod++;
             // and the sensors, which read the post's ATP, temperature, length and id or its light sensor
             if (rt == 6 || rt == 94 || rt == 95 || rt == 108 || rt == 111 || (rt >= 115 && rt <= 117) || rt == 120)
                 S.unsupported_mod++;
@@ -752,6 +758,7 @@ static int build_device(Sim& S) {
     d.pp1 = S.dupload(pp1);
     d.camkii = S.dupload(camkii);
     d.camkii01 = S.dupload(camkii01);
+    d.calcn = S.dupload(calcn);
     d.sflags = S.dupload(sflags);
     d.cav21_slot = S.dupload(cav21);
     d.mcount = S.dupload(mcount);
@@ -834,10 +841,12 @@ static int build_device(Sim& S) {
     d.y_post_area = S.dupload(y_qa);
     d.y_amount = S.dupload(y_amt);
     d.y_tau = S.dupload(y_tau);
-    release_params(S, y_rmax, y_prel, y_prim0);
+    std::vector<float> y_rrecs;
+    release_params(S, y_rmax, y_prel, y_prim0, y_rrecs);
     d.y_prel = S.dupload(y_prel);
     d.y_rrp_max = S.dupload(y_rmax);
     d.y_prim0 = S.dupload(y_prim0);
+    d.y_rrec = any_ck ? S.dupload(y_rrecs) : nullptr;
     d.y_cleft_mask = S.dupload(y_mask);
     d.y_src = S.dupload(src);
     d.post_len = S.dupload(post_len);
@@ -970,6 +979,12 @@ static int build_device(Sim& S) {
     }
     d.y_isyn = S.dalloc<float>(ns);
     d.y_pp1 = S.dupload(y_pp1);
+    d.y_postd = nullptr;
+    if (any_pp1) {
+        std::vector<uint32_t> yp(ns);
+        for (uint32_t k = 0; k < ns; k++) yp[k] = S.host2dev[sy[S.slot2syn[k]].post_segment];
+        d.y_postd = S.dupload(yp);
+    }
     d.hist_base = S.dupload(hist_base);
     d.ca_hist = S.dalloc<double>(std::max<uint32_t>(nhist, 1));
     d.d_mask = S.dalloc<uint8_t>(ns);
@@ -2366,9 +2381,11 @@ int bchip_set_protein_mapping(void* sim, int on) {
     S.protein_map = on != 0;
     if (!S.loaded || !S.d.ns) return 0;
     HIPCHK(hipSetDevice(S.device));
-    std::vector<float> rmax, prel, prim0;
-    release_params(S, rmax, prel, prim0);
+    std::vector<float> rmax, prel, prim0, rrecs;
+    release_params(S, rmax, prel, prim0, rrecs);
     const size_t b = sizeof(float) * rmax.size();
+    if (S.d.y_rrec)
+        HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_rrec), rrecs.data(), b, hipMemcpyHostToDevice, S.stream));
     HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_rrp_max), rmax.data(), b, hipMemcpyHostToDevice, S.stream));
     HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_prel), prel.data(), b, hipMemcpyHostToDevice, S.stream));
     HIPCHK(hipMemcpyAsync(const_cast<float*>(S.d.y_prim0), prim0.data(), b, hipMemcpyHostToDevice, S.stream));

@@ -76,7 +76,7 @@ __host__ __device__ static inline bool gprotein_type(uint8_t t) {
 __host__ __device__ static inline bool modifier_reads_nt(uint8_t t) {
     return t == M_AMPA || t == M_NMDA || t == M_GABAA || (t >= M_GLAST && t <= M_GAT3) ||
            (t >= M_EAAT3 && t <= M_GAT1) || (t >= 10 && t <= 64) || (t >= 67 && t <= 68) ||
-           (t >= 70 && t <= 74) || t == 96 || t == 97 || t == 134 || t == 194 || t == 3;
+           (t >= 70 && t <= 76) || t == 96 || t == 97 || t == 134 || t == 194 || t == 3;
 }
 
 __host__ __device__ static inline bool modifier_supported(uint8_t t) {
@@ -98,10 +98,14 @@ __host__ __device__ static inline bool modifier_supported(uint8_t t) {
         case 145: case 146: case 152: case 186: case 190: case 86: case 126: case 127: case 128:
         case 136: case 3: case 5: case 77: case 78: case 79: case 188: case 16: case 87: case 88:
         case 89: case 90: case 91: case 92: case 192: case 6: case 94: case 95: case 108: case 111:
-        case 115: case 116: case 117: case 120:
+        case 115: case 116: case 117: case 120: case 82: case 83: case 113: case 137: case 75: case 76:
+        // IDs without code in the reference (reserved, or plugin stubs with no plugin loaded):
+        // no current and no state change there (kernel_dispatch.h:565, dispatch_generated.cpp:
+        // 26-31), which is the device switch's default
+        case 0: case 80: case 81: case 93: case 124: case 125: case 138: case 139: case 143:
             return true;
         default:
-            return t >= 17 && t <= 62;   // the Gi/o-, Gs- and Gq-coupled receptor families
+            return (t >= 17 && t <= 62) || t >= 212;   // the G-protein-coupled receptor families
     }
 }
 
@@ -115,7 +119,7 @@ struct SegRegs {
     float area;    // membrane area: sphere 4*pi*r^2 or cylinder 2*pi*r*L (float)
     double vol;    // volume (double, run_tick_common.h:260-264)
     double cpm;    // F * vol / 1000, the charge-per-mole factor every current divides by
-    float pkc, pka, camp;   // cascade activities read by Nav/Kv3.1/Kv7/HCN (constant on device)
+    float pkc, pka, camp;   // first PKC state[0] (live), first cAMP/PKA state[1] and state[0] (constants)
     float camkii;           // state[0] of the last CaMKII modifier, read by the connexins
     float camkii01;         // its state[0] + state[1], read by Kv4.2
     float cabp4;            // state[0] of the first CaBP4 modifier, set before each Cav1.4 runs
@@ -684,8 +688,10 @@ BC_DEV float nt_pick(const SegRegs& s, int k) {
 // here); larger k, past the record's state array, are refused at load.
 BC_DEV double state_any(const ModSlot& m, uint32_t k) { return k < 7 ? m.get(k) : (double)m.a7; }
 
+// scale: the reference's mod_scale factor of the current (modifier_helpers.h:1216), 1 except
+// for the MSN receptors 75/76.
 template <bool kSegV = false>
-BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
+BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p, float scale = 1.0f) {
     float area = eff_area(m, s);
     double V = s.V;
     double cpm = s.cpm;
@@ -715,7 +721,6 @@ BC_DEV double iono(const ModSlot& m, SegRegs& s, float dt, const IonoK& p) {
         op *= (1.0f - d);
     }
     double N = m.dens * area;
-    const float scale = 1.0f;
     if (p.anion) {
         double pcl = (gamma == p.gamma) ? p_single(p.gamma, 1.0f, kCbarCl) : p_single(gamma, 1.0f, kCbarCl);
         double fcl = ghk_an<kSegV>(s, V, s.in[3], s.ex[3]);
@@ -2572,6 +2577,84 @@ BC_DEV double ch_ecb(const ModSlot& m, const SegRegs& s, float dt) {
     return 0.0;
 }
 
+// CaMKII (82, hh_biological.cpp:545-570): Ca2+ (uM, Hill 4, K_d state[2] or 1) drives state[0]
+// with time constant state[3] (2000 ms); autophosphorylation state[1] at rate state[5] (0.0005)
+// decays at 1e-4 per ms; dt scaled by the density; no current.
+BC_DEV double ch_camkii(const ModSlot& m, const SegRegs& s, float dt) {
+    float ca = s.in[2];
+    float cu = ca * kMmToUm;
+    const double s0 = m.get(0), s1 = m.get(1), s2 = m.get(2), s3 = m.get(3), s5 = m.get(5);
+    double kd = (s2 > 0.0f) ? s2 : 1.0f;
+    double tau = (s3 > 0.0f) ? s3 : 2000.0f;
+    double ar = (s5 > 0.0f) ? s5 : 0.0005f;
+    dt *= m.dens;
+    float c4 = cu * cu * cu * cu;
+    float inf = c4 / (c4 + kd * kd * kd * kd + 1e-9f);
+    double a = s0 + (inf - s0) * dt / tau;
+    double b = s1 + (a * ar - s1 * 0.0001f) * dt;
+    if (a < 0.0f) a = 0.0f;
+    if (b < 0.0f) b = 0.0f;
+    if (a > 1.0f) a = 1.0f;
+    if (b > 1.0f) b = 1.0f;
+    m.set(0, a);
+    m.set(1, b);
+    return 0.0;
+}
+
+// Calcineurin (83, hh_biological.cpp:575-591): Ca2+ (uM, Hill 2, K_d state[1] or 0.05) drives
+// state[0] with time constant state[2] (300 ms); dt scaled by the density; no current.
+BC_DEV double ch_calcineurin(const ModSlot& m, const SegRegs& s, float dt) {
+    float ca = s.in[2];
+    float cu = ca * kMmToUm;
+    const double s0 = m.get(0), s1 = m.get(1), s2 = m.get(2);
+    double kd = (s1 > 0.0f) ? s1 : 0.05f;
+    double tau = (s2 > 0.0f) ? s2 : 300.0f;
+    dt *= m.dens;
+    float c2 = cu * cu;
+    float inf = c2 / (c2 + kd * kd + 1e-9f);
+    double a = s0 + (inf - s0) * dt / tau;
+    if (a < 0.0f) a = 0.0f;
+    if (a > 1.0f) a = 1.0f;
+    m.set(0, a);
+    return 0.0;
+}
+
+// PP1 / DARPP-32 (113, hh_biological.cpp:3222-3253): the PKA activity (s.pka: state[1] of the
+// segment's first cAMP/PKA modifier) drives p-DARPP-32 (state[0]) and p-inhibitor-1 (state[1])
+// with time constant state[4] (5000 ms), dt scaled by the density; PP1 activity state[3] =
+// basal (state[5] or 0.5) / (1 + 10 state[0] + 5 state[1] + 1e-9); state[6] = PKA; no current.
+BC_DEV double ch_pp1(const ModSlot& m, const SegRegs& s, float dt) {
+    const double s0 = m.get(0), s1 = m.get(1), s4 = m.get(4), s5 = m.get(5);
+    double tau = (s4 > 0.0f) ? s4 : 5000.0f;
+    double basal = (s5 > 0.0f) ? s5 : 0.5f;
+    dt *= m.dens;
+    float pka = s.pka;
+    double a = s0 + (pka - s0) * dt / tau;
+    double b = s1 + (pka - s1) * dt / tau;
+    if (a < 0.0f) a = 0.0f;
+    if (b < 0.0f) b = 0.0f;
+    if (a > 1.0f) a = 1.0f;
+    if (b > 1.0f) b = 1.0f;
+    m.set(0, a);
+    m.set(1, b);
+    double inh = 10.0f * a + 5.0f * b;
+    m.set(3, basal / (1.0f + inh + 1e-9f));
+    m.set(6, pka);
+    return 0.0;
+}
+
+// PKC (137, modifier_helpers.h:1584-1593): state[0] = DAG / (DAG + 0.1) * Ca / (Ca + 0.0005 mM),
+// DAG = s.dag (state[1] of the segment's first PLC/IP3/DAG modifier), Ca floored at 0; no current.
+BC_DEV double ch_pkc(const ModSlot& m, const SegRegs& s) {
+    float dag = s.dag;
+    float ca = s.in[2];
+    if (ca < 0.0f) ca = 0.0f;
+    float df = dag / (dag + 0.1f);
+    float cf = ca / (ca + 0.0005f);
+    m.set(0, df * cf);
+    return 0.0;
+}
+
 // CaBP4 (145, hh_biological.cpp:684-693): no current; state[0] binds towards 1 with tau 1 ms.
 BC_DEV double ch_cabp4(const ModSlot& m, float dt) {
     double g0 = m.get(0);
@@ -2928,6 +3011,20 @@ BC_DEV double rec_nmda_mg(const ModSlot& m, SegRegs& s, float dt) {
     return I * block;
 }
 
+// The MSN receptors' mod_scale (receptors_biological.cpp:338-345, 358-365): state[1] of the
+// segment's first D1 (65, for 75) or D2 (66, for 76) modulator as it stands, floored at 0, else 1.
+BC_DEV float msn_scale(const uint8_t* mtype, const double* mstate, size_t P, uint32_t i, uint32_t cnt,
+                       uint8_t t) {
+    const uint8_t want = t == 75 ? 65 : 66;
+    for (uint32_t q = 0; q < cnt; q++)
+        if (mtype[q * P + i] == want) {
+            float v = mstate[((size_t)q * 7 + 1) * P + i];
+            if (v < 0.0f) v = 0.0f;
+            return v;
+        }
+    return 1.0f;
+}
+
 // Gi/o-coupled receptors (17-40, modifier_helpers.h:1332-1363): occupancy of transmitter NT
 // drives a GIRK gate (state[0]) with desensitisation (state[1]); EC50 and gamma from
 // state[active_vars] and state[active_vars + 1] when positive.
@@ -2999,6 +3096,16 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 68: return iono(m, s, dt, IonoK{30.0f, 0.5f, 1.0f, 10.0f, 4, 0.48f, 0.48f, 0.04f, false, 2000.0f});
         case 74: return ch_gio<1>(m, s, dt);
         case 63: case 72: return rec_nmda_mg(m, s, dt);
+        case 75: {   // MSN NMDA (receptors_biological.cpp:337-352): scaled by s.xa, the D1 term
+            const IonoK k = {50.0f, 1.0f, 5.0f, 50.0f, 0, 0.08f, 0.08f, 0.84f, false, 0.0f};
+            double I = iono(m, s, dt, k, s.xa);
+            float mg = s.ex[4];
+            if (mg < 0.01f) mg = 1.0f;
+            float block = 1.0f / (1.0f + (mg / 3.57f) * bc::exp(-0.062f * s.V));
+            return I * block;
+        }
+        case 76:   // MSN AMPA (receptors_biological.cpp:357-369): scaled by s.xa, the D2 term
+            return iono(m, s, dt, IonoK{8.0f, 0.5f, 0.5f, 3.0f, 0, 0.5f, 0.5f, 0.0f, false, 3.0f}, s.xa);
         case 52: case 53: case 54: return ch_gs<3>(m, s, dt);   // Gq: the Gs signal's form
         case 55: case 56: return ch_gs<6>(m, s, dt);
         case 57: case 58: case 59: case 70: return ch_gs<4>(m, s, dt);
@@ -3080,6 +3187,10 @@ BC_DEV double dispatch_ext(uint8_t type, const ModSlot& m, SegRegs& s, float dt)
         case 127: return ch_ip3r(m, s, dt);
         case 128: return ch_ryr(m, s, dt);
         case 136: return ch_plc(m, s, dt);
+        case 82: return ch_camkii(m, s, dt);
+        case 83: return ch_calcineurin(m, s, dt);
+        case 113: return ch_pp1(m, s, dt);
+        case 137: return ch_pkc(m, s);
         case 3: return ch_metabotropic(m, s, dt);
         case 5: return ch_release_site(m, s, dt);
         case 188: return ch_mrgpr(m, s, dt);

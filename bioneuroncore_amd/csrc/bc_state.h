@@ -136,12 +136,13 @@ struct Net {
     const float* area;
     const double* vol;
     const float* cap;
-    const float* pkc;
+    float* pkc;                 // state[0] of the first PKC modifier (0 if none); step 1 keeps it current
     const float* pka;
     const float* camp;
-    const float* pp1;
-    const float* camkii;        // state[0] of the last CaMKII modifier (0 if none)
-    const float* camkii01;      // its state[0] + state[1] (0 if none)
+    float* pp1;                 // state[3] of the last PP1 modifier (0.5 if none); step 1 keeps it current
+    float* camkii;              // state[0] of the last CaMKII modifier (0 if none); step 1 keeps it current
+    float* camkii01;            // its state[0] + state[1] (0 if none)
+    float* calcn;               // state[0] of the last calcineurin modifier (0 if none)
     const uint8_t* sflags;
     const int8_t* cav21_slot;   // last Cav2.1 modifier slot, -1 if none
 
@@ -174,6 +175,8 @@ struct Net {
     const float* y_tau;
     const float* y_rrp_max;
     const float* y_prim0;        // RRP_recovery + CaMKII(pre) * 0.02 + calcineurin(pre) * 0.01
+    const float* y_rrec;         // RRP_recovery alone when some segment carries CaMKII or
+                                 // calcineurin (then the sum is formed each tick), else null
     const uint16_t* y_cleft_mask;
     const int32_t* y_src;        // Ca-history entry of the last earlier synapse whose post is my
                                  // pre (see ca_hist), -1 if none
@@ -201,6 +204,8 @@ struct Net {
     float* y_calcn;
     float* y_isyn;               // receptor current of the synapse this tick (post pass -> plasticity)
     const float* y_pp1;          // PP1 activity of the post segment
+    const uint32_t* y_postd;     // post segment of each slot when some segment carries PP1 (its
+                                 // activity then read from pp1 each tick), else null
     float* y_relp;               // release probability of a deferred release this tick, 0 if none
     uint8_t* d_mask;             // occupied delay slots (bit s)
     uint64_t* d_arr;             // [4][ns]

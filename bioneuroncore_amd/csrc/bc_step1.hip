@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             s.xa = xa;
             s.xb = xb;
         }
-        if (kExt && t == 86) {   // eCB synthesis reads the first PLC/IP3/DAG modifier's state[1]
+        if (kExt && (t == 86 || t == 137)) {   // eCB synthesis and PKC read the first PLC/IP3/DAG modifier's state[1]
             s.dag = 0.0f;
             for (uint32_t q = 0; q < cnt; q++)
                 if (d.mtype[q * P + i] == 136) {
@@ -113,6 +113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     break;
                 }
         }
+        if (kExt && (t == 75 || t == 76)) s.xa = msn_scale(d.mtype, d.mstate, P, i, cnt, t);
         if (kExt && t == 6) s.len = d.length[i];
         if (kExt && t == 111) s.sid = d.segid[i];
         itot += dispatch<kNt, kExt>(t, m, s, dt);
@@ -126,6 +127,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     *c = v;
                     break;
                 }
+        }
+        // The cascades other modifiers and step 3 read: CaMKII (the last one's state[0], and
+        // state[0] + state[1]), calcineurin (the last one's state[0]), PP1 (the last one's
+        // state[3]) and PKC (the first one's state[0]). The per-segment planes hold those values
+        // as they stand, for the readers later in this loop and in step 3 and for the next tick.
+        if (kExt && (t == 82 || t == 83 || t == 113)) {
+            bool last = true;
+            for (uint32_t q = j + 1; q < cnt; q++)
+                if (d.mtype[q * P + i] == t) last = false;
+            if (last) {
+                if (t == 82) {
+                    const double a = m.get(0), b = m.get(1);
+                    s.camkii = a;
+                    s.camkii01 = a + b;
+                    d.camkii[i] = s.camkii;
+                    d.camkii01[i] = s.camkii01;
+                } else if (t == 83) {
+                    d.calcn[i] = m.get(0);
+                } else {
+                    d.pp1[i] = m.get(3);
+                }
+            }
+        }
+        if (kExt && t == 137) {
+            bool first = true;
+            for (uint32_t q = 0; q < j; q++)
+                if (d.mtype[q * P + i] == 137) first = false;
+            if (first) {
+                s.pkc = m.get(0);
+                d.pkc[i] = s.pkc;
+            }
         }
         if (kExt && t == 87) {   // nuclear ERK raises every CREB's pCREB on the segment (hh_biological.cpp mapk_erk_current)
             float dts = dt;

@@ -126,6 +126,13 @@ __device__ static void glial_capture(const Net& d, uint32_t y, float dt) {
     d.y_capm[y] = got;
 }
 
+// The vesicle-recovery rate of slot y with its presynaptic segment's CaMKII and calcineurin as
+// they stand (kernel_dispatch.h:614-627): RRP_recovery + CaMKII * 0.02 + calcineurin * 0.01.
+__device__ static inline float prim_live(const Net& d, uint32_t y, uint32_t pre) {
+    const float a = d.y_rrec[y] + d.camkii[pre] * 0.02f;
+    return a + d.calcn[pre] * 0.01f;
+}
+
 // Immediate release of probability p (simulation_host.cpp:412-456): vesicle recovery with the
 // presynaptic Ca the reference would read at this synapse (rrp_do_recovery, kernel_dispatch.h:
 // 612-631), depletion (rrp_do_release, :636-645) and the cleft deposit; y_relp = -1 marks it done.
@@ -134,7 +141,7 @@ __device__ static void release_now(const Net& d, uint32_t y, float p, float ca, 
     float rrp = d.y_rrp[y], rmax = d.y_rrp_max[y];
     if (!(rmax <= 0.0f || rrp >= rmax)) {
         float caf = ca / (0.0005f + ca);
-        float prim = d.y_prim0[y];
+        float prim = d.y_rrec ? prim_live(d, y, d.y_pre[y]) : d.y_prim0[y];
         prim *= caf;
         float rec = (rmax - rrp) * prim * dt;
         rrp += rec;
@@ -473,6 +480,7 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
                 m.dens = d.y_rdens[j * NS + y];
                 m.a7 = parea;
                 m.av = d.y_rav[j * NS + y];
+                if (!kReceptorsOnly && (rt == 75 || rt == 76)) s.xa = msn_scale(d.mtype, d.mstate, P, i, d.mcount[i], rt);
                 isyn += kReceptorsOnly ? dispatch_receptor(rt, m, s, dt) : dispatch(rt, m, s, dt);
                 ran = true;
             }
@@ -753,7 +761,7 @@ __global__ __launch_bounds__(256) void k_syn_plast(Net d, float dt) {
     }
     float dw = 0.0f;
     if (ck > 0.0f) dw += 0.0003f * ck * dt;
-    if (cn > 0.0f) dw -= 0.0001f * cn * d.y_pp1[y] * dt;
+    if (cn > 0.0f) dw -= 0.0001f * cn * (d.y_postd ? d.pp1[d.y_postd[y]] : d.y_pp1[y]) * dt;
     if (yf & YF_INHIBITORY) dw = -dw;
     float w = d.y_weight[y] + dw;
     if (w != w) w = 0.5f;
@@ -792,7 +800,7 @@ __global__ __launch_bounds__(256) void k_syn_rrp(Net d, float dt) {
         int32_t src = d.y_src[y];
         float ca = src >= 0 ? (float)d.ca_hist[src] : (float)d.presum[d.y_prew[y]].ca_snap;
         float caf = ca / (0.0005f + ca);
-        float prim = d.y_prim0[y];   // RRP_recovery + CaMKII(pre) * 0.02 + calcineurin(pre) * 0.01
+        float prim = d.y_rrec ? prim_live(d, y, pre) : d.y_prim0[y];   // RRP_recovery + CaMKII(pre) * 0.02 + calcineurin(pre) * 0.01
         prim *= caf;
         float rec = (rmax - rrp) * prim * dt;
         rrp += rec;

@@ -578,7 +578,7 @@ static constexpr Iono kNmda2b = {50.0f, 1.0f, 10.0f, 150.0f, 0, 0.08f, 0.08f, 0.
 static constexpr Iono kP2x = {15.0f, 0.5f, 1.0f, 20.0f, 10, 0.22f, 0.22f, 0.56f, false, 20.0f};
 static constexpr Iono kP2x2 = {15.0f, 0.5f, 1.0f, 20.0f, 10, 0.22f, 0.22f, 0.56f, false, 5000.0f};
 
-static double iono_current(Modifier* m, Segment* s, float dt, const Iono& p) {
+static double iono_current(Modifier* m, Segment* s, float dt, const Iono& p, float scale = 1.0f) {
     float area = eff_area(m, s);
     double V = s->V_m;
     double vol = volume(s);
@@ -603,7 +603,6 @@ static double iono_current(Modifier* m, Segment* s, float dt, const Iono& p) {
         op *= (1.0f - d);
     }
     double N = m->density * area;
-    const float scale = 1.0f;
     if (p.anion) {
         double pcl = p_single(gamma, 1.0f, kCbarCl);
         double fcl = ghk_an(V, s->intra_ions[3], s->extra_ions[3]);
@@ -647,8 +646,20 @@ static double rec_nmda(Modifier* m, Segment* s, float dt, const Iono& k = kNmda)
 // The network-builder NMDA variants (63, 72; receptors_biological.cpp:222-229, 312-319): NMDA
 // gating without the Ca-dependent inactivation, Mg2+ block by the extracellular Mg2+ (1 mM when
 // below 0.01).
-static double rec_nmda_mg(Modifier* m, Segment* s, float dt) {
-    double I = iono_current(m, s, dt, kNmda);
+// MSN NMDA (75, receptors_biological.cpp:337-352) and MSN AMPA (76, :357-369): NMDA with the Mg
+// block and AMPA, their current scaled by state[1] of the segment's first D1 (65) or D2 (66)
+// modulator, floored at 0 (1 without one).
+static float msn_scale(const Segment* s, uint8_t want) {
+    for (uint8_t j = 0; j < s->modifier_count; j++)
+        if (s->modifiers[j].type == want) {
+            float v = s->modifiers[j].state[1];
+            if (v < 0.0f) v = 0.0f;
+            return v;
+        }
+    return 1.0f;
+}
+static double rec_nmda_mg(Modifier* m, Segment* s, float dt, float scale = 1.0f) {
+    double I = iono_current(m, s, dt, kNmda, scale);
     float mg = s->extra_ions[4];
     if (mg < 0.01f) mg = 1.0f;
     float block = 1.0f / (1.0f + (mg / 3.57f) * exp(-0.062f * s->V_m));
@@ -2052,6 +2063,79 @@ static inline void clamp01_4(double* st) {
         if (st[k] > 1.0f) st[k] = 1.0f;
     }
 }
+// CaMKII (82, hh_biological.cpp:545-570): activation by Ca2+ (uM, Hill 4, K_d state[2] or 1
+// uM) with time constant state[3] (2000 ms), autophosphorylation state[1] at rate state[5]
+// (0.0005) decaying at 1e-4 per ms; time scaled by the density; no current. Connexins, Kv4.2 and
+// vesicle recovery read the last CaMKII's state[0] (and state[1]).
+static double mod_camkii(Modifier* m, Segment* s, float dt) {
+    float ca = s->intra_ions[2];
+    float cu = ca * kMmToUm;
+    double kd = (m->state[2] > 0.0f) ? m->state[2] : 1.0f;
+    double tau = (m->state[3] > 0.0f) ? m->state[3] : 2000.0f;
+    double ar = (m->state[5] > 0.0f) ? m->state[5] : 0.0005f;
+    dt *= m->density;
+    float c4 = cu * cu * cu * cu;
+    float inf = c4 / (c4 + kd * kd * kd * kd + 1e-9f);
+    m->state[0] += (inf - m->state[0]) * dt / tau;
+    m->state[1] += (m->state[0] * ar - m->state[1] * 0.0001f) * dt;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    return 0.0f;
+}
+// Calcineurin (83, hh_biological.cpp:575-591): Ca2+ (uM, Hill 2, K_d state[1] or 0.05 uM), time
+// constant state[2] (300 ms), time scaled by the density; no current. Vesicle recovery reads the
+// last calcineurin's state[0].
+static double mod_calcineurin(Modifier* m, Segment* s, float dt) {
+    float ca = s->intra_ions[2];
+    float cu = ca * kMmToUm;
+    double kd = (m->state[1] > 0.0f) ? m->state[1] : 0.05f;
+    double tau = (m->state[2] > 0.0f) ? m->state[2] : 300.0f;
+    dt *= m->density;
+    float c2 = cu * cu;
+    float inf = c2 / (c2 + kd * kd + 1e-9f);
+    m->state[0] += (inf - m->state[0]) * dt / tau;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    return 0.0f;
+}
+// PP1 / DARPP-32 (113, hh_biological.cpp:3222-3253): PKA activity (state[1] of the segment's
+// first cAMP/PKA modifier 135, modifier_helpers.h:1486-1490) phosphorylates DARPP-32 (state[0])
+// and inhibitor-1 (state[1]) with time constant state[4] (5000 ms), time scaled by the density;
+// PP1 activity state[3] = basal (state[5] or 0.5) / (1 + 10 p-DARPP + 5 p-I1 + 1e-9); state[6]
+// holds the PKA activity; no current. The plasticity step reads the last PP1's state[3].
+static double mod_pp1(Modifier* m, Segment* s, float dt) {
+    double tau = (m->state[4] > 0.0f) ? m->state[4] : 5000.0f;
+    double basal = (m->state[5] > 0.0f) ? m->state[5] : 0.5f;
+    dt *= m->density;
+    float pka = first_state_of(s, T_CAMP_PKA, 1, 0.0f);
+    float pd = pka, pi = pka;
+    m->state[0] += (pd - m->state[0]) * dt / tau;
+    m->state[1] += (pi - m->state[1]) * dt / tau;
+    if (m->state[0] < 0.0f) m->state[0] = 0.0f;
+    if (m->state[1] < 0.0f) m->state[1] = 0.0f;
+    if (m->state[0] > 1.0f) m->state[0] = 1.0f;
+    if (m->state[1] > 1.0f) m->state[1] = 1.0f;
+    double inh = 10.0f * m->state[0] + 5.0f * m->state[1];
+    m->state[3] = basal / (1.0f + inh + 1e-9f);
+    m->state[6] = pka;
+    return 0.0f;
+}
+// PKC (137, hh_biological.cpp:3365-3367 -> modifier_helpers.h:1584-1593): state[0] = DAG / (DAG
+// + 0.1) * Ca / (Ca + 0.0005 mM), DAG the state[1] of the segment's first PLC/IP3/DAG modifier
+// and Ca the intracellular Ca2+ floored at 0; no current. Channels read the first PKC's state[0].
+static float seg_dag(const Segment* s);
+static double mod_pkc(Modifier* m, Segment* s, float dt) {
+    (void)dt;
+    float dag = seg_dag(s);
+    float ca = s->intra_ions[2];
+    if (ca < 0.0f) ca = 0.0f;
+    float df = dag / (dag + 0.1f);
+    float cf = ca / (ca + 0.0005f);
+    m->state[0] = df * cf;
+    return 0.0f;
+}
 // MAPK/ERK (87, hh_biological.cpp mapk_erk_current): Ca2+ (uM, Hill 3, K_d state[4] or 0.5) ->
 // Ras (state[2]) -> MEK (state[1]) -> ERK (state[0], squared MEK) -> nuclear ERK (state[3]);
 // time scaled by the density; nuclear ERK raises every CREB's pCREB (state[0]) on the segment.
@@ -2857,6 +2941,8 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 68: return iono_current(m, s, dt, kNicotinic);
         case 74: return chan_gio(m, s, dt, 1);
         case 63: case 72: return rec_nmda_mg(m, s, dt);
+        case 75: return rec_nmda_mg(m, s, dt, msn_scale(s, 65));
+        case 76: return iono_current(m, s, dt, kAmpa, msn_scale(s, 66));
         case 131: return chan_hcn1(m, s, dt, -77.0f, 1408.0f, 0.54f);
         case 132: return chan_hcn1(m, s, dt, -100.0f, 3586.0f, 0.51f);
         case T_CAV13: return chan_cav13(m, s, dt);
@@ -2948,7 +3034,16 @@ static double dispatch(uint8_t type, Modifier* m, Segment* s, float dt) {
         case 127: return mod_ip3r(m, s, dt);
         case 128: return mod_ryr(m, s, dt);
         case 136: return mod_plc(m, s, dt);
+        case 82: return mod_camkii(m, s, dt);
+        case 83: return mod_calcineurin(m, s, dt);
+        case 113: return mod_pp1(m, s, dt);
+        case 137: return mod_pkc(m, s, dt);
+        // IDs without code in the reference (the reserved 80, 81, 93, 138, 139, 143; the plugin
+        // stubs 124 and 125; 212 and above): no current, no state change
+        // (kernel_dispatch.h:565 -> dispatch_generated.cpp:26-31 with no plugin loaded)
+        case 0: case 80: case 81: case 93: case 124: case 125: case 138: case 139: case 143: return 0.0;
         default:
+            if (type >= 212) return 0.0;
             if (type >= 17 && type <= 40) return chan_gio(m, s, dt, gio_nt(type));
             if (type >= 41 && type <= 51) return chan_gs(m, s, dt, gs_nt(type));
             // Gq-coupled (52-62 and the muscarinic 70; modifier_helpers.h:1600-1621): the Gs

@@ -11,6 +11,9 @@ compiled reference on the same networks, tests/test_oracle_vs_reference.py):
   CREB and MRGPR (src/hh_biological.cpp, include/modifier_helpers.h mrgpr_signal);
 - the MAPK/ERK, NF-kB, mTORC1, protein-synthesis, cofilin and MMP-9 cascades, the TRPV1
   transmitter receptor and rod phototransduction (src/hh_biological.cpp);
+- CaMKII, calcineurin, PP1 / DARPP-32 and PKC, whose activities step 1 and step 3 read as they
+  stand (src/hh_biological.cpp:545-591,3222-3253, include/modifier_helpers.h:1584-1593,
+  include/kernel_dispatch.h:612-631), and the IDs without code in the reference;
 - the physical, hunger, light, olfactory, vibration, otolith, cupula and cryptochrome sensors and
   thermal noise (src/hh_biological.cpp:483-540, 2181-2440, 3122-3141);
 - a network that mixes them with the core channel set runs the extended step-1 kernel, and the
@@ -108,6 +111,17 @@ def test_sensor_modifiers(seed):
     the cryptochrome's boost of the first light sensor (120, before and after it in slot order),
     400 ticks: every byte of the records."""
     net = bion.sensor_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    seg, spk = _compare(net, 400, cur)
+    assert spk.sum() > 0
+
+
+@pytest.mark.parametrize("seed", [91, 92])
+def test_kinase_modifiers(seed):
+    """CaMKII, calcineurin, PP1 / DARPP-32 and PKC read live by Nav1.2, Cx36, Kv4.2, vesicle
+    recovery and plasticity, with the IDs that have no code in the reference, 400 ticks: every
+    byte of the records."""
+    net = bion.kinase_network(seed)
     cur = H.soma_clamp(net, 160.0)
     seg, spk = _compare(net, 400, cur)
     assert spk.sum() > 0

@@ -183,6 +183,25 @@ def test_sensor_modifiers(seed):
         assert np.any(st[ty == t][:, 0] != 0.0), t
 
 
+@pytest.mark.parametrize("seed", [91, 92])
+def test_kinase_modifiers(seed):
+    """CaMKII, calcineurin, PP1 / DARPP-32 and PKC with their readers (Nav1.2 and the first PKC,
+    Cx36 / Kv4.2 and the last CaMKII, vesicle recovery and the presynaptic CaMKII / calcineurin,
+    plasticity and the post segment's PP1), and the IDs without code in the reference: every
+    byte of the records."""
+    net = bion.kinase_network(seed)
+    cur = H.soma_clamp(net, 160.0)
+    rs, ry, sp = _both(net, 400, lambda t: H.Stimuli(current=cur))
+    assert sp.sum() > 0
+    st = rs["modifiers"]["state"]
+    ty = rs["modifiers"]["type"]
+    st0 = net.segments["modifiers"]["state"]
+    for t in bion.KINASES:
+        assert np.any(st[ty == t][:, 0] != st0[ty == t][:, 0]), t
+    for t in (80, 124, 143):   # no code: states 0-6 untouched (state[7] is the tick's area)
+        assert np.array_equal(st[ty == t][:, :7], st0[ty == t][:, :7]), t
+
+
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
 def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2;
