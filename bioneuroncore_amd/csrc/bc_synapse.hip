@@ -286,6 +286,43 @@ __global__ __launch_bounds__(256) void k_syn_pre(Net d, float dt) {
     }
 }
 
+// One chain step of the receptors-only post pass: every load of slot y at once. The slot arrays
+// cover every slot and the receptor arrays d.nrec <= 2 receptors, so the loads a lane does not
+// use are still in bounds.
+struct PostStep {
+    double st[2][3];
+    double dens[2];
+    float cl0, cl1, parea;
+    uint32_t rc;
+    uint16_t cm;
+    uint8_t yf, rt[2], av[2];
+};
+__device__ __attribute__((always_inline)) inline void load_step(const Net& d, uint32_t y, PostStep& p) {
+    const size_t NS = d.ns;
+    p.yf = d.y_flags[y];
+    p.cm = d.y_cleft_mask[y];
+    p.cl0 = d.y_cleft[y];
+    p.cl1 = d.y_cleft[NS + y];
+    p.rc = d.y_rcount[y];
+    p.parea = d.y_post_area[y];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        p.rt[j] = 0;
+        p.av[j] = 0;
+        p.dens[j] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) p.st[j][k] = 0.0;
+        if (j < (int)d.nrec) {
+            const size_t b = (size_t)j * 7 * NS + y;
+            p.rt[j] = d.y_rtype[j * NS + y];
+            p.av[j] = d.y_rav[j * NS + y];
+            p.dens[j] = d.y_rdens[j * NS + y];
+#pragma unroll
+            for (int k = 0; k < 3; k++) p.st[j][k] = d.y_rstate[b + (size_t)k * NS];
+        }
+    }
+}
+
 // Step 4 of segment i (simulation_host.cpp:596-649): I_buffer clamp, dV = I / C_m * dt clamped
 // to +-50 mV, spike count, voltage clamps in index order (the last active one wins), V clamp.
 __device__ static double step4(const Net& d, uint32_t i, double ibuf, double V, float dt) {
@@ -391,41 +428,43 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
             if (m & 2) s.nt[7] += d.y_cap[1 * NS + sl];
             if (m & 4) s.nt[1] += d.y_cap[2 * NS + sl];
         };
+        // kPre: the loads of chain step q + 1 are issued before step q is computed (the slot
+        // of step q + 1 is known from the lanes still walking then), so each step's memory
+        // round trip overlaps the previous step's receptor arithmetic.
+        PostStep nx;
+        if (kPre && len > 0 && !ord) load_step(d, off + wl, nx);
         for (uint32_t q = 0; q < len; q++) {
             // the lanes still walking are lanes 0..cnt-1 of this wavefront (sorted by length)
             const uint32_t cnt = (uint32_t)__popcll(__ballot(1));
             const uint32_t y = off + wl;
             off += cnt;
+            PostStep c;
+            if constexpr (kPre) {
+                c = nx;
+                if (q + 1 < len && !ord) load_step(d, off + wl, nx);
+            }
             if (ord) continue;   // the ordered pass walks this chain
             if (go < ge) {   // this segment's junctions as the presynaptic end, up to slot y
                 const uint32_t sid = d.y_sid[y];
                 for (; go < ge && d.gap_sid[go] <= sid; go++) ibuf -= d.y_igap[d.gap_slot[go]];
             }
             if constexpr (kPre) {
-                // every load of the step at once (one memory round trip per step): the slot
-                // arrays cover every slot and the receptor arrays d.nrec <= 2 receptors, so the
-                // loads a lane does not use are still in bounds
-                const uint8_t yf = d.y_flags[y];
-                const uint16_t cm = d.y_cleft_mask[y];
-                const float cl0 = d.y_cleft[y], cl1 = d.y_cleft[NS + y];
-                const uint32_t rc = d.y_rcount[y];
-                const float parea = d.y_post_area[y];
+                const uint8_t yf = c.yf;
+                const uint16_t cm = c.cm;
+                const float cl0 = c.cl0, cl1 = c.cl1;
+                const uint32_t rc = c.rc;
                 RegSlot rs[2];
                 uint8_t rt[2];
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
-                    rt[j] = 0;
-                    if (j < (int)d.nrec) {
-                        const size_t b = (size_t)j * 7 * NS + y;
-                        rt[j] = d.y_rtype[j * NS + y];
-                        rs[j].av = d.y_rav[j * NS + y];
-                        rs[j].dens = d.y_rdens[j * NS + y];
+                    rt[j] = c.rt[j];
+                    rs[j].av = c.av[j];
+                    rs[j].dens = c.dens[j];
 #pragma unroll
-                        for (int k = 0; k < 3; k++) rs[j].st[k] = d.y_rstate[b + (size_t)k * NS];
-                        rs[j].mem = d.y_rstate + b;
-                        rs[j].stride = NS;
-                        rs[j].a7 = parea;
-                    }
+                    for (int k = 0; k < 3; k++) rs[j].st[k] = c.st[j][k];
+                    rs[j].mem = d.y_rstate + (size_t)j * 7 * NS + y;
+                    rs[j].stride = NS;
+                    rs[j].a7 = c.parea;
                 }
                 if (yf & YF_SKIP) {
                     if (yf & YF_GAP) ibuf += d.y_igap[y];
@@ -549,10 +588,11 @@ __device__ __attribute__((always_inline)) inline void syn_post(const Net& d, flo
     }
 }
 
-// AMPA/NMDA/GABA-A receptors only: four wavefronts per SIMD fit in the register budget.
-// kPre: at most two receptors per synapse, each chain step's loads issued together.
+// AMPA/NMDA/GABA-A receptors only. kPre: at most two receptors per synapse, each chain step's
+// loads issued together and one step ahead; three wavefronts per SIMD hold the look-ahead
+// registers without spilling (four spill, and measure slower).
 template <bool kPre>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_syn_post_rec(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_syn_post_rec(
     Net d, float dt) {
     exp_table_stage();
     syn_post<true, kPre>(d, dt);
