@@ -127,6 +127,42 @@ def test_kinase_modifiers(seed):
     assert spk.sum() > 0
 
 
+def test_msn_receptors_read_the_d1_d2_modulators():
+    """The MSN NMDA/AMPA receptors (75/76) scale their current by state[1] of the segment's first
+    D1 (65) / D2 (66) modulator, floored at 0 (receptors_biological.cpp:337-369). The modulators
+    themselves need glibc pow and are outside the device set, so the network loads only with
+    bchip_allow_unsupported and both the device and the oracle hold their states; the device's
+    scaling must then equal the oracle's byte for byte, 200 ticks."""
+    net = bion.kinase_network(93)
+    segs = net.segments
+    mods = segs["modifiers"]
+    rng = np.random.default_rng(93)
+    somas = np.nonzero(segs["segment_type"] == 1)[0]
+    for q, i in enumerate(somas):
+        k = int(segs["modifier_count"][i])
+        for j, t in enumerate((65, 66)):
+            mods["type"][i, k + j] = t
+            mods["density"][i, k + j] = 1.0
+            mods["state"][i, k + j, 1] = -0.3 if q % 4 == 0 else rng.uniform(0.2, 1.8)
+        segs["modifier_count"][i] = k + 2
+    cur = H.soma_clamp(net, 160.0)
+    o = H.OracleSim(net.copy())
+    for _ in range(200):
+        o.step(DT, H.Stimuli(current=cur))
+    sim = backend.Simulation(coherence=backend.COHERENCE_FAST)
+    assert sim.load_network(net.copy()) == backend.ERR_UNSUPPORTED
+    sim.allow_unsupported(True)
+    assert sim.load_network(net.copy()) == 0
+    assert sim.stats().unsupported_modifiers == 2 * len(somas)
+    sim.upload_stimuli(backend.make_stimuli(current=cur))
+    sim.run_ticks(DT, 200)
+    out = sim.bion_data()
+    seg, syn = out.segments.copy(), out.synapses.copy()
+    sim.close()
+    assert H.first_mismatch(o.net.segments, seg) is None, H.first_mismatch(o.net.segments, seg)
+    assert H.first_mismatch(o.net.synapses, syn) is None, H.first_mismatch(o.net.synapses, syn)
+
+
 @pytest.mark.parametrize("seed,myelin", [(41, False), (42, False), (43, True)])
 def test_channel_set_of_the_spinal_cell_types(seed, myelin):
     """Developmental Nav1.2, Kv2.1, Kv4.2, Cav1.3, both persistent Na+ channels, Kir2.1, ClC-2
