@@ -834,6 +834,7 @@ static int build_device(Sim& S) {
     d.watched = S.dupload(watched);
     d.nwatched = (uint32_t)watched.size();
     d.presum = S.dalloc<PreSum>(watched.size());
+    d.ca_snap = S.dalloc<double>(std::max<size_t>(watched.size(), 1));
     d.y_id = S.dupload(y_id);
     d.y_flags = S.dupload(y_flags);
     d.y_delay = S.dupload(y_delay);

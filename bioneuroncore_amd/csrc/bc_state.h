@@ -165,6 +165,7 @@ struct Net {
     const uint32_t* watched;     // device segment of each presynaptic summary
     uint32_t nwatched;
     PreSum* presum;
+    double* ca_snap;     // each summary's ca_snap again, compact (the vesicle pass gathers only it)
     const uint32_t* y_id;
     const uint8_t* y_flags;
     const float* y_delay;

@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) void k_pre_summary(Net d) {
         ps.h0 = 1.0f / (1.0f + bc::exp((vb - -50.0f) / 8.0f));
     }
     ps.ca_snap = d.in[2 * P + pre];
+    d.ca_snap[w] = ps.ca_snap;
     d.presum[w] = ps;
 }
 
@@ -838,7 +839,7 @@ __global__ __launch_bounds__(256) void k_syn_rrp(Net d, float dt) {
     const uint32_t pre = d.y_pre[y];
     if (!(rmax <= 0.0f || rrp >= rmax)) {   // rrp_do_recovery (kernel_dispatch.h:612-631)
         int32_t src = d.y_src[y];
-        float ca = src >= 0 ? (float)d.ca_hist[src] : (float)d.presum[d.y_prew[y]].ca_snap;
+        float ca = src >= 0 ? (float)d.ca_hist[src] : (float)d.ca_snap[d.y_prew[y]];
         float caf = ca / (0.0005f + ca);
         float prim = d.y_rrec ? prim_live(d, y, pre) : d.y_prim0[y];   // RRP_recovery + CaMKII(pre) * 0.02 + calcineurin(pre) * 0.01
         prim *= caf;
